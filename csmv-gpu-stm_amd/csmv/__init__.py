@@ -1,0 +1,229 @@
+"""ctypes harness over the gfx950 CSMV engine's C-ABI (include/csmv.h).
+
+The engine is the native library ``libcsmv.so`` built in-tree from
+``csmv-gpu-stm_amd/csrc``.  This module only marshals arguments; every
+transaction runs in the HIP kernels.  There is no CPU fallback: if the library
+is missing or no gfx950 device is visible, the calls raise.
+
+Reference interfaces mirrored here (file:line in DMRNunes/csmv-gpu-stm):
+  Engine(...)               TXInit + create_gbc   bank/csmv_sh/API.cu:293-333, sync_lib/one_phase_server.h:13-42
+  Engine.fill / load        TXInit data upload    bank/csmv_sh/API.cu:310-322
+  Engine.read_state         TXEnd read-back       bank/csmv_sh/API.cu:337-348
+  Engine.bank_run           test_fine_grain_offloading   bank/csmv_sh/exe.cu:340-529
+  Engine.close              TXEnd frees + free_gbc       bank/csmv_sh/API.cu:354-358
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+import os
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libcsmv.so")
+
+# variant bits (csmv.h)
+VAL_SCAN, VAL_COLLAB, VAL_STAMP = 0x0, 0x1, 0x2
+INSERT_LANE, INSERT_BATCH = 0x0, 0x4
+WB_CLIENT, WB_SERVER = 0x0, 0x8
+VARIANTS = {
+    "vanilla": VAL_SCAN | INSERT_LANE | WB_SERVER,
+    "batchedAtrInsert": VAL_SCAN | INSERT_BATCH | WB_CLIENT,
+    "clientWriteback": VAL_SCAN | INSERT_BATCH | WB_SERVER,
+    "collabValidation": VAL_COLLAB | INSERT_BATCH | WB_CLIENT,
+    "stamp": VAL_STAMP | INSERT_BATCH | WB_CLIENT,
+}
+
+STATUS = {0: "ok", 1: "HIP runtime error", 2: "invalid argument", 3: "capacity exceeded",
+          4: "device wait timed out", 5: "no gfx950 device", 6: "call out of order",
+          7: "transaction attempt limit exceeded"}
+
+
+class CsmvError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"{STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+class Config(C.Structure):
+    _fields_ = [("n_items", C.c_uint32), ("depth", C.c_uint32), ("atr_entries", C.c_uint32),
+                ("client_waves", C.c_uint32), ("wg_waves", C.c_uint32), ("variant", C.c_uint32),
+                ("timeout_ms", C.c_uint32), ("max_attempts", C.c_uint32)]
+
+
+class BankSpec(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("n_txns", C.c_uint64), ("n_reads", C.c_uint32),
+                ("n_reads_ro", C.c_uint32), ("pct_ro", C.c_uint32), ("dist", C.c_uint32),
+                ("amount_max", C.c_uint32), ("reserved", C.c_uint32), ("theta", C.c_double)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("commits_update", C.c_uint64), ("commits_ro", C.c_uint64),
+                ("aborts_dataage", C.c_uint64), ("aborts_readwrite", C.c_uint64),
+                ("aborts_prevalid", C.c_uint64), ("aborts_recordage", C.c_uint64),
+                ("batches", C.c_uint64), ("final_ts", C.c_uint64), ("kernel_ms", C.c_double),
+                ("error", C.c_uint32), ("client_lanes", C.c_uint32)]
+
+    def as_dict(self) -> dict:
+        d = {f: getattr(self, f) for f, _ in self._fields_}
+        commits = d["commits_update"] + d["commits_ro"]
+        aborts = (d["aborts_dataage"] + d["aborts_readwrite"] + d["aborts_prevalid"]
+                  + d["aborts_recordage"])
+        d["commits"] = commits
+        d["aborts"] = aborts
+        d["abort_rate"] = aborts / (aborts + commits) if aborts + commits else 0.0
+        d["txn_per_s"] = commits / (d["kernel_ms"] * 1e-3) if d["kernel_ms"] > 0 else 0.0
+        return d
+
+
+_lib: Optional[C.CDLL] = None
+
+
+def lib() -> C.CDLL:
+    """Loads libcsmv.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+    L = C.CDLL(LIB_PATH)
+    P, U32, U64, I32, SZ = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int32, C.c_size_t
+    sigs = {
+        "csmv_status_str": ([C.c_int], C.c_char_p),
+        "csmv_last_error": ([P], C.c_char_p),
+        "csmv_create": ([C.POINTER(Config), C.POINTER(P)], C.c_int),
+        "csmv_destroy": ([P], C.c_int),
+        "csmv_load_i32": ([P, P, SZ], C.c_int),
+        "csmv_fill_i32": ([P, I32], C.c_int),
+        "csmv_read_state_i32": ([P, P, SZ], C.c_int),
+        "csmv_zipf_cdf": ([U32, C.c_double, P], C.c_int),
+        "csmv_bank_generate": ([P, C.POINTER(BankSpec)], C.c_int),
+        "csmv_bank_read_stream": ([P, P, P], C.c_int),
+        "csmv_bank_run": ([P, C.POINTER(Stats)], C.c_int),
+        "csmv_bank_read_log": ([P, P, P, P], C.c_int),
+        "csmv_atomic_rate": ([P, U32, U32, C.POINTER(C.c_double)], C.c_int),
+    }
+    for name, (args, res) in sigs.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L
+
+
+def _ptr(a: np.ndarray) -> C.c_void_p:
+    assert a.flags["C_CONTIGUOUS"]
+    return C.c_void_p(a.ctypes.data)
+
+
+def zipf_cdf(n: int, theta: float) -> np.ndarray:
+    """Host Zipf table (needs no device): cdf[i] = floor(2^31 H(i+1)/H(n)), cdf[n-1] = 2^31."""
+    out = np.empty(n, dtype=np.uint32)
+    st = lib().csmv_zipf_cdf(n, theta, _ptr(out))
+    if st:
+        raise CsmvError(st, "csmv_zipf_cdf")
+    return out
+
+
+@dataclasses.dataclass
+class Bank:
+    """The bank workload (bank/csmv_sh/exe.cu:147-207): transfers and audits."""
+    n_txns: int
+    seed: int = 1
+    n_reads: int = 2
+    n_reads_ro: int = 32
+    pct_ro: int = 10
+    zipf: bool = False
+    theta: float = 0.99
+    amount_max: int = 10
+
+    def spec(self) -> BankSpec:
+        return BankSpec(self.seed, self.n_txns, self.n_reads, self.n_reads_ro, self.pct_ro,
+                        1 if self.zipf else 0, self.amount_max, 0, self.theta)
+
+    @property
+    def rmax(self) -> int:
+        return max(self.n_reads, self.n_reads_ro)
+
+
+class Engine:
+    def __init__(self, n_items: int, depth: int = 4, atr_entries: int = 4096,
+                 client_waves: int = 64, wg_waves: int = 4, variant="stamp",
+                 timeout_ms: int = 4000, max_attempts: int = 0):
+        if isinstance(variant, str):
+            variant = VARIANTS[variant]
+        self.cfg = Config(n_items, depth, atr_entries, client_waves, wg_waves, variant,
+                          timeout_ms, max_attempts)
+        self.n_items = n_items
+        self._h = C.c_void_p()
+        st = lib().csmv_create(C.byref(self.cfg), C.byref(self._h))
+        if st:
+            raise CsmvError(st, "csmv_create")
+        self.bank: Optional[Bank] = None
+
+    def _chk(self, st: int, what: str):
+        if st:
+            msg = lib().csmv_last_error(self._h) or b""
+            raise CsmvError(st, f"{what}: {msg.decode(errors='replace')}")
+
+    def close(self):
+        if self._h:
+            lib().csmv_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- pool ---------------------------------------------------------------------------
+    def fill(self, value: int):
+        self._chk(lib().csmv_fill_i32(self._h, int(value)), "fill")
+
+    def load(self, values: np.ndarray):
+        v = np.ascontiguousarray(values, dtype=np.int32)
+        self._chk(lib().csmv_load_i32(self._h, _ptr(v), v.size), "load")
+
+    def read_state(self) -> np.ndarray:
+        out = np.empty(self.n_items, dtype=np.int32)
+        self._chk(lib().csmv_read_state_i32(self._h, _ptr(out), out.size), "read_state")
+        return out
+
+    # ---- bank ---------------------------------------------------------------------------
+    def bank_generate(self, bank: Bank):
+        self._chk(lib().csmv_bank_generate(self._h, C.byref(bank.spec())), "bank_generate")
+        self.bank = bank
+
+    def bank_read_stream(self):
+        b = self.bank
+        op = np.empty(b.n_txns, dtype=np.uint32)
+        addr = np.empty(b.rmax * b.n_txns, dtype=np.uint32)
+        self._chk(lib().csmv_bank_read_stream(self._h, _ptr(op), _ptr(addr)), "read_stream")
+        return op, addr
+
+    def bank_run(self) -> dict:
+        st = Stats()
+        self._chk(lib().csmv_bank_run(self._h, C.byref(st)), "bank_run")
+        return st.as_dict()
+
+    def bank_read_log(self):
+        b = self.bank
+        ts = np.empty(b.n_txns, dtype=np.uint32)
+        v = np.empty(b.n_reads * b.n_txns, dtype=np.int32)
+        ab = np.empty(b.n_txns, dtype=np.uint32)
+        self._chk(lib().csmv_bank_read_log(self._h, _ptr(ts), _ptr(v), _ptr(ab)), "read_log")
+        return ts, v, ab
+
+    def atomic_rate(self, waves: int = 4096, iters: int = 256) -> float:
+        r = C.c_double()
+        self._chk(lib().csmv_atomic_rate(self._h, waves, iters, C.byref(r)), "atomic_rate")
+        return r.value

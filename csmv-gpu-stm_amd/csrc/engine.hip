@@ -1,0 +1,689 @@
+// engine.hip -- gfx950 CSMV engine: bank driver, persistent kernel, host engine, C-ABI.
+//
+// One persistent kernel replaces the reference's device-side launch of server and client
+// grids (parent_kernel, bank/csmv_sh/exe.cu:319-338; HIP has no dynamic parallelism):
+// workgroup 0 is the commit server (server.hpp), every other workgroup holds client
+// wavefronts.  The grid never exceeds what the device keeps resident, so spinning
+// clients cannot starve the server, and every wait is bounded (Spin).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "csmv.h"
+#include "csmv_device.hpp"
+#include "server.hpp"
+
+namespace csmv {
+
+// Reads of one update transaction on the wire, writes of one transaction.
+constexpr int kBankR = 4;
+constexpr int kBankW = 2;
+constexpr uint32_t kKindTransfer = 0, kKindAudit = 1;
+
+// ---- stream generation ----------------------------------------------------------------------
+__host__ __device__ __forceinline__ uint64_t splitmix64(uint64_t x)
+{
+    uint64_t z = x + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// RAND_R_FNC (bank/csmv_sh/API.cuh:37-53): three LCG steps (a = 1103515245, c = 12345,
+// mod 2^64) combining 11 + 10 + 10 bits taken from bit 16 of each state.
+__host__ __device__ __forceinline__ uint32_t rand_r_fnc(uint64_t &state)
+{
+    uint64_t next = state * 1103515245ull + 12345ull;
+    uint64_t result = (next >> 16) & 2047u;
+    next = next * 1103515245ull + 12345ull;
+    result = (result << 10) ^ ((next >> 16) & 1023u);
+    next = next * 1103515245ull + 12345ull;
+    result = (result << 10) ^ ((next >> 16) & 1023u);
+    state = next;
+    return (uint32_t)result;
+}
+
+__device__ __forceinline__ uint32_t zipf_rank(const uint32_t *cdf, uint32_t n, uint32_t r)
+{
+    uint32_t lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const uint32_t mid = lo + ((hi - lo) >> 1);
+        if (r < cdf[mid]) hi = mid; else lo = mid + 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ uint32_t scramble(uint32_t rank, uint32_t n)
+{
+    return (n & (n - 1)) == 0 ? (rank * 2654435761u) & (n - 1) : rank;
+}
+
+struct GenArgs {
+    uint64_t seed, n_txns;
+    uint32_t n_items, n_reads, n_reads_ro, ro_thr, dist, amount_max, rmax;
+    const uint32_t *cdf;
+    uint32_t *op, *addr;
+};
+
+// One lane per transaction k; state = splitmix64(seed + k), then the driver's draws:
+// read-only test rnd < prRead*0xFFFF (bank/csmv_sh/exe.cu:134-139), amount, operands.
+__global__ void __launch_bounds__(256) bank_generate_kernel(GenArgs g)
+{
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= g.n_txns) return;
+    uint64_t st = splitmix64(g.seed + k);
+    const uint32_t rnd = rand_r_fnc(st) & 0xFFFFu;
+    const uint32_t kind = rnd < g.ro_thr ? kKindAudit : kKindTransfer;
+    uint32_t amt = 0, nr = g.n_reads_ro;
+    if (kind == kKindTransfer) {
+        amt = 1 + rand_r_fnc(st) % g.amount_max;
+        nr = g.n_reads;
+    }
+    g.op[k] = kind | (amt << 8) | (nr << 16);
+    for (uint32_t j = 0; j < g.rmax; ++j) {
+        uint32_t it = kNone;
+        if (j < nr) {
+            const uint32_t r = rand_r_fnc(st);
+            it = g.dist == 1 ? scramble(zipf_rank(g.cdf, g.n_items, r), g.n_items)
+                             : r % g.n_items;
+        }
+        g.addr[(uint64_t)j * g.n_txns + k] = it;
+    }
+}
+
+__global__ void __launch_bounds__(256) pool_fill_kernel(uint32_t *ht, uint64_t *slot,
+                                                        uint32_t n, uint32_t depth,
+                                                        const int32_t *values, int32_t fill)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    ht[i] = 0;
+    slot[i] = (uint32_t)(values ? values[i] : fill);
+    for (uint32_t d = 1; d < depth; ++d) slot[(size_t)d * n + i] = 0;
+}
+
+__global__ void __launch_bounds__(256) pool_head_kernel(const uint32_t *ht, const uint64_t *slot,
+                                                        uint32_t n, int32_t *out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = (int32_t)(uint32_t)slot[(size_t)(ht[i] & 0xFFFFu) * n + i];
+}
+
+// ---- bank driver (client wavefronts) -----------------------------------------------------------
+// client_kernel (bank/csmv_sh/exe.cu:89-284) on a resident stream: a lane holds one
+// transaction until it commits; lanes that finish are refilled with one fetch_add per
+// wave (ballot + prefix count).  Audits commit locally at their snapshot
+// (exe.cu:213-223); transfers go through tx_commit.
+template <int R, int W> __device__ __forceinline__ void bank_client(const Ctx &c, uint32_t cw)
+{
+    const uint32_t lane = lane_id();
+    const uint64_t T = c.n_txns;
+    bool has = false, exhausted = false;
+    uint64_t k = 0;
+    uint32_t op = 0, ab = 0;
+    uint64_t n_upd = 0, n_ro = 0, n_age = 0;
+    for (;;) {
+        if (ld_rlx(c.err)) break;
+        if (!exhausted) {
+            const uint64_t need = ballot(!has);
+            if (need) {
+                const uint32_t cnt = (uint32_t)__builtin_popcountll(need);
+                uint64_t base = 0;
+                if (lane == 0) base = fetch_add(c.next_txn, (uint64_t)cnt);
+                base = bcast64(base);
+                if (base + cnt >= T) exhausted = true;
+                if (!has) {
+                    const uint64_t mine = base + prefix_count(need);
+                    if (mine < T) {
+                        has = true;
+                        k = mine;
+                        op = c.op[k];
+                        ab = 0;
+                    }
+                }
+            }
+        }
+        if (ballot(has) == 0) break;
+
+        Tx<R, W> tx;
+        tx_begin(c, tx);
+        const uint32_t kind = op & 0xFFu, amt = (op >> 8) & 0xFFu, nr = (op >> 16) & 0xFFu;
+        const bool ro = has && kind == kKindAudit;
+        const bool upd = has && kind == kKindTransfer;
+        int32_t reads[R];
+#pragma unroll
+        for (int j = 0; j < R; ++j) reads[j] = 0;
+        uint32_t sum = 0;
+        if (ro) {
+            for (uint32_t j = 0; j < nr && !tx.aborted; ++j)
+                sum += (uint32_t)tx_read_only(c, tx, c.addr[(uint64_t)j * T + k]);
+        } else if (upd) {
+            // transfer body (exe.cu:184-201): read a, write a-amt, read b, write b+amt,
+            // then the extra reads of the wider transaction
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                if ((uint32_t)j < nr && !tx.aborted) {
+                    const uint32_t it = c.addr[(uint64_t)j * T + k];
+                    const int32_t v = tx_read(c, tx, it);
+                    reads[j] = v;
+                    if (j == 0) tx_write(tx, it, (int32_t)((uint32_t)v - amt));
+                    else if (j == 1) tx_write(tx, it, (int32_t)((uint32_t)v + amt));
+                }
+            }
+        }
+        if (tx.overflow) raise_error(c, kErrReadSetOverflow);
+        if (ro) {
+            if (!tx.aborted) {
+                c.log_ts[k] = tx.snap;
+                c.log_v[k] = (int32_t)sum;
+                c.log_ab[k] = ab;
+                has = false;
+                ++n_ro;
+            } else {
+                ++ab;
+                ++n_age;
+            }
+        }
+        if (upd && tx.aborted) { ++ab; ++n_age; }
+        const bool req = upd && !tx.aborted;
+        if (ballot(req)) {
+            bool fail = false;
+            const uint32_t ts = tx_commit<R, W>(c, cw, tx, req, &fail);
+            if (fail) break;
+            if (req) {
+                if (ts) {
+                    c.log_ts[k] = ts;
+#pragma unroll
+                    for (int j = 0; j < R; ++j)
+                        if ((uint32_t)j < nr) c.log_v[(uint64_t)j * T + k] = reads[j];
+                    c.log_ab[k] = ab;
+                    has = false;
+                    ++n_upd;
+                } else {
+                    ++ab;
+                }
+            }
+        }
+        if (has && ab > c.max_attempts) raise_error(c, kErrLivelock);
+    }
+    if (n_upd) fetch_add(&c.stats[kStatCommitUpd], n_upd);
+    if (n_ro) fetch_add(&c.stats[kStatCommitRo], n_ro);
+    if (n_age) fetch_add(&c.stats[kStatAbortAge], n_age);
+    fence_rel();
+    if (lane == 0) __hip_atomic_fetch_add(c.clients_done, 1u, __ATOMIC_RELEASE,
+                                          __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int R, int W> __global__ void __launch_bounds__(512) bank_kernel(Ctx c)
+{
+    extern __shared__ uint32_t lds[];
+    if (blockIdx.x == 0) {
+        server_workgroup<R, W>(c, lds);
+        return;
+    }
+    const uint32_t cw = (blockIdx.x - 1) * c.wg_waves + threadIdx.x / kWave;
+    if (cw < c.n_client_waves) bank_client<R, W>(c, cw);
+}
+
+// Same-address agent-scope atomic rate: lane 0 of every wave adds to one word.
+__global__ void __launch_bounds__(256) atomic_rate_kernel(uint64_t *p, uint32_t iters)
+{
+    if (lane_id() != 0) return;
+    for (uint32_t i = 0; i < iters; ++i)
+        __hip_atomic_fetch_add(p, (uint64_t)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+}  // namespace csmv
+
+// =============================================================================================
+// host engine
+// =============================================================================================
+using namespace csmv;
+
+struct csmv_engine {
+    csmv_config cfg{};
+    int device = 0;
+    uint32_t n_cu = 0;
+    uint32_t lds_max = 0;
+    uint32_t wallclock_khz = 0;
+    uint32_t Q = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // pool
+    uint32_t *ht = nullptr;
+    uint64_t *slot = nullptr;
+    bool pool_loaded = false;
+    // control + messaging
+    uint8_t *ctl = nullptr;
+    uint64_t *ring = nullptr, *reply = nullptr, *done = nullptr;
+    uint32_t *mbox = nullptr, *stamps = nullptr;
+    // bank stream + log
+    csmv_bank_spec spec{};
+    bool has_stream = false;
+    uint32_t rmax = 0, log_rows = 0;
+    uint64_t stream_cap = 0;
+    uint32_t *op = nullptr, *addr = nullptr, *cdf = nullptr;
+    uint32_t *log_ts = nullptr, *log_ab = nullptr;
+    int32_t *log_v = nullptr;
+    int32_t *tmp = nullptr;
+    std::string err;
+};
+
+namespace {
+
+constexpr size_t kCtlBytes = 4096;
+constexpr size_t kOffGc = 0, kOffErr = 128, kOffDone = 256, kOffNext = 384, kOffTail = 512,
+                 kOffStats = 640;
+
+csmv_status fail(csmv_engine *e, csmv_status s, const std::string &msg)
+{
+    if (e) e->err = msg;
+    return s;
+}
+
+#define HIPCHK(e, call)                                                                     \
+    do {                                                                                    \
+        hipError_t _r = (call);                                                             \
+        if (_r != hipSuccess)                                                               \
+            return fail((e), CSMV_E_HIP,                                                    \
+                        std::string(#call) + ": " + hipGetErrorString(_r));                 \
+    } while (0)
+
+template <typename T> csmv_status dalloc(csmv_engine *e, T **p, size_t count)
+{
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t r = hipMalloc((void **)p, sizeof(T) * count);
+    if (r != hipSuccess)
+        return fail(e, r == hipErrorOutOfMemory ? CSMV_E_CAPACITY : CSMV_E_HIP,
+                    std::string("hipMalloc: ") + hipGetErrorString(r));
+    return CSMV_OK;
+}
+
+template <typename T> void dfree(T *&p)
+{
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+uint32_t pow2_at_least(uint64_t v)
+{
+    uint32_t q = 1;
+    while (q < v) q <<= 1;
+    return q;
+}
+
+struct Layout {
+    uint32_t nmb, mb_off, atr_off, stamp_off, stamp_in_lds, words;
+};
+
+Layout server_layout(const csmv_engine *e, uint32_t rows, uint32_t W)
+{
+    Layout L{};
+    const uint32_t M = e->cfg.wg_waves - 2;
+    L.nmb = std::min<uint32_t>(2 * M, 32);
+    const uint32_t hdr = kLdsHdr + 2 * L.nmb;
+    L.mb_off = (hdr + 63) & ~63u;
+    uint32_t w = L.mb_off + L.nmb * rows * kWave;
+    const uint32_t vmode = e->cfg.variant & 3u;
+    L.atr_off = w;
+    if (vmode != kValStamp) w += W * e->cfg.atr_entries;
+    L.stamp_off = w;
+    L.stamp_in_lds = 0;
+    if (vmode == kValStamp && (uint64_t)(w + e->cfg.n_items) * 4 <= e->lds_max) {
+        L.stamp_in_lds = 1;
+        w += e->cfg.n_items;
+    }
+    L.words = w;
+    return L;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *csmv_status_str(csmv_status s)
+{
+    switch (s) {
+    case CSMV_OK: return "ok";
+    case CSMV_E_HIP: return "HIP runtime error";
+    case CSMV_E_ARG: return "invalid argument";
+    case CSMV_E_CAPACITY: return "capacity exceeded";
+    case CSMV_E_TIMEOUT: return "device wait timed out";
+    case CSMV_E_NODEVICE: return "no gfx950 device";
+    case CSMV_E_STATE: return "call out of order";
+    case CSMV_E_LIVELOCK: return "transaction attempt limit exceeded";
+    }
+    return "unknown status";
+}
+
+const char *csmv_last_error(const csmv_engine *e) { return e ? e->err.c_str() : ""; }
+
+csmv_status csmv_zipf_cdf(uint32_t n, double theta, uint32_t *cdf)
+{
+    if (n == 0 || !cdf || !(theta >= 0.0)) return CSMV_E_ARG;
+    double *h = new (std::nothrow) double[n];
+    if (!h) return CSMV_E_CAPACITY;
+    double acc = 0.0;
+    for (uint32_t i = 0; i < n; ++i) {
+        acc += std::pow((double)(i + 1), -theta);
+        h[i] = acc;
+    }
+    const double total = h[n - 1];
+    for (uint32_t i = 0; i + 1 < n; ++i) {
+        double v = std::floor(2147483648.0 * (h[i] / total));
+        if (v > 2147483648.0) v = 2147483648.0;
+        cdf[i] = (uint32_t)v;
+    }
+    cdf[n - 1] = 0x80000000u;
+    delete[] h;
+    return CSMV_OK;
+}
+
+csmv_status csmv_create(const csmv_config *cfg_in, csmv_engine **out)
+{
+    if (!cfg_in || !out) return CSMV_E_ARG;
+    *out = nullptr;
+    csmv_config cfg = *cfg_in;
+    if (cfg.timeout_ms == 0) cfg.timeout_ms = 4000;
+    if (cfg.max_attempts == 0) cfg.max_attempts = 1u << 20;
+    if (cfg.wg_waves == 0) cfg.wg_waves = 4;
+    if (cfg.n_items == 0 || cfg.depth == 0 || cfg.depth > 64 || cfg.client_waves == 0 ||
+        cfg.wg_waves < 3 || cfg.wg_waves > 8 || cfg.atr_entries < 64 ||
+        (cfg.atr_entries & (cfg.atr_entries - 1)) || cfg.atr_entries > (1u << 16) ||
+        (cfg.variant & ~0xFu) || (cfg.variant & 3u) == 3u)
+        return CSMV_E_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return CSMV_E_NODEVICE;
+    csmv_engine *e = new (std::nothrow) csmv_engine();
+    if (!e) return CSMV_E_CAPACITY;
+    e->cfg = cfg;
+    csmv_status s;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&e->device) != hipSuccess ||
+        hipGetDeviceProperties(&prop, e->device) != hipSuccess) {
+        delete e;
+        return CSMV_E_HIP;
+    }
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        delete e;
+        return CSMV_E_NODEVICE;
+    }
+    e->n_cu = (uint32_t)prop.multiProcessorCount;
+    e->lds_max = 64 * 1024;
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, e->device) != hipSuccess ||
+        khz <= 0)
+        khz = 100000;
+    e->wallclock_khz = (uint32_t)khz;
+    e->Q = pow2_at_least(std::max<uint64_t>(4096, 2ull * cfg.client_waves));
+    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess) {
+        csmv_destroy(e);
+        return CSMV_E_HIP;
+    }
+    const uint32_t rows = mbox_rows<kBankR, kBankW>();
+    if ((s = dalloc(e, &e->ht, cfg.n_items)) ||
+        (s = dalloc(e, &e->slot, (size_t)cfg.n_items * cfg.depth)) ||
+        (s = dalloc(e, &e->ctl, kCtlBytes)) || (s = dalloc(e, &e->ring, e->Q)) ||
+        (s = dalloc(e, &e->done, e->Q)) ||
+        (s = dalloc(e, &e->reply, (size_t)cfg.client_waves * kWave)) ||
+        (s = dalloc(e, &e->mbox, (size_t)cfg.client_waves * rows * kWave)) ||
+        (s = dalloc(e, &e->stamps, cfg.n_items)) || (s = dalloc(e, &e->tmp, cfg.n_items))) {
+        csmv_destroy(e);
+        return s;
+    }
+    *out = e;
+    return CSMV_OK;
+}
+
+csmv_status csmv_destroy(csmv_engine *e)
+{
+    if (!e) return CSMV_E_ARG;
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    dfree(e->ht); dfree(e->slot); dfree(e->ctl); dfree(e->ring); dfree(e->done);
+    dfree(e->reply); dfree(e->mbox); dfree(e->stamps); dfree(e->tmp);
+    dfree(e->op); dfree(e->addr); dfree(e->cdf); dfree(e->log_ts); dfree(e->log_v);
+    dfree(e->log_ab);
+    if (e->ev0) (void)hipEventDestroy(e->ev0);
+    if (e->ev1) (void)hipEventDestroy(e->ev1);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+    return CSMV_OK;
+}
+
+static csmv_status fill_pool(csmv_engine *e, const int32_t *dvals, int32_t v)
+{
+    const uint32_t n = e->cfg.n_items;
+    hipLaunchKernelGGL(pool_fill_kernel, dim3((n + 255) / 256), dim3(256), 0, e->stream, e->ht,
+                       e->slot, n, e->cfg.depth, dvals, v);
+    HIPCHK(e, hipGetLastError());
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    e->pool_loaded = true;
+    return CSMV_OK;
+}
+
+csmv_status csmv_load_i32(csmv_engine *e, const int32_t *values, size_t n)
+{
+    if (!e || !values || n != e->cfg.n_items) return fail(e, CSMV_E_ARG, "load: size mismatch");
+    HIPCHK(e, hipMemcpyAsync(e->tmp, values, n * 4, hipMemcpyHostToDevice, e->stream));
+    return fill_pool(e, e->tmp, 0);
+}
+
+csmv_status csmv_fill_i32(csmv_engine *e, int32_t value)
+{
+    if (!e) return CSMV_E_ARG;
+    return fill_pool(e, nullptr, value);
+}
+
+csmv_status csmv_read_state_i32(csmv_engine *e, int32_t *out, size_t n)
+{
+    if (!e || !out || n != e->cfg.n_items) return fail(e, CSMV_E_ARG, "read_state: size mismatch");
+    hipLaunchKernelGGL(pool_head_kernel, dim3((n + 255) / 256), dim3(256), 0, e->stream, e->ht,
+                       e->slot, (uint32_t)n, e->tmp);
+    HIPCHK(e, hipGetLastError());
+    HIPCHK(e, hipMemcpyAsync(out, e->tmp, n * 4, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    return CSMV_OK;
+}
+
+csmv_status csmv_bank_generate(csmv_engine *e, const csmv_bank_spec *s)
+{
+    if (!e || !s) return CSMV_E_ARG;
+    if (s->n_txns == 0 || s->n_txns >= (1ull << 31) || s->n_reads < 2 ||
+        s->n_reads > (uint32_t)kBankR || s->n_reads_ro < 1 || s->n_reads_ro > 64 ||
+        s->pct_ro > 100 || s->dist > 1 || s->amount_max == 0 || s->amount_max > 255 ||
+        (s->dist == 1 && !(s->theta >= 0.0)))
+        return fail(e, CSMV_E_ARG, "bank spec out of range");
+    const uint32_t rmax = std::max(s->n_reads, s->n_reads_ro);
+    if (s->n_txns > e->stream_cap || rmax > e->rmax || s->n_reads > e->log_rows) {
+        dfree(e->op); dfree(e->addr); dfree(e->log_ts); dfree(e->log_v); dfree(e->log_ab);
+        e->stream_cap = 0;
+        csmv_status st;
+        if ((st = dalloc(e, &e->op, s->n_txns)) || (st = dalloc(e, &e->addr, rmax * s->n_txns)) ||
+            (st = dalloc(e, &e->log_ts, s->n_txns)) ||
+            (st = dalloc(e, &e->log_v, (size_t)s->n_reads * s->n_txns)) ||
+            (st = dalloc(e, &e->log_ab, s->n_txns)))
+            return st;
+        e->stream_cap = s->n_txns;
+        e->rmax = rmax;
+        e->log_rows = s->n_reads;
+    }
+    dfree(e->cdf);
+    if (s->dist == 1) {
+        uint32_t *h = new (std::nothrow) uint32_t[e->cfg.n_items];
+        if (!h) return fail(e, CSMV_E_CAPACITY, "cdf host buffer");
+        csmv_zipf_cdf(e->cfg.n_items, s->theta, h);
+        csmv_status st = dalloc(e, &e->cdf, e->cfg.n_items);
+        if (st) { delete[] h; return st; }
+        hipError_t r = hipMemcpy(e->cdf, h, e->cfg.n_items * 4, hipMemcpyHostToDevice);
+        delete[] h;
+        HIPCHK(e, r);
+    }
+    GenArgs g{};
+    g.seed = s->seed;
+    g.n_txns = s->n_txns;
+    g.n_items = e->cfg.n_items;
+    g.n_reads = s->n_reads;
+    g.n_reads_ro = s->n_reads_ro;
+    g.ro_thr = (uint32_t)((uint64_t)s->pct_ro * 0xFFFFu / 100u);
+    g.dist = s->dist;
+    g.amount_max = s->amount_max;
+    g.rmax = rmax;
+    g.cdf = e->cdf;
+    g.op = e->op;
+    g.addr = e->addr;
+    hipLaunchKernelGGL(bank_generate_kernel, dim3((unsigned)((s->n_txns + 255) / 256)), dim3(256),
+                       0, e->stream, g);
+    HIPCHK(e, hipGetLastError());
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    e->spec = *s;
+    e->has_stream = true;
+    return CSMV_OK;
+}
+
+csmv_status csmv_bank_read_stream(csmv_engine *e, uint32_t *op, uint32_t *addr)
+{
+    if (!e || !op || !addr) return CSMV_E_ARG;
+    if (!e->has_stream) return fail(e, CSMV_E_STATE, "no stream generated");
+    const uint64_t T = e->spec.n_txns;
+    const uint32_t rmax = std::max(e->spec.n_reads, e->spec.n_reads_ro);
+    HIPCHK(e, hipMemcpy(op, e->op, T * 4, hipMemcpyDeviceToHost));
+    HIPCHK(e, hipMemcpy(addr, e->addr, (size_t)rmax * T * 4, hipMemcpyDeviceToHost));
+    return CSMV_OK;
+}
+
+csmv_status csmv_bank_run(csmv_engine *e, csmv_stats *out)
+{
+    if (!e) return CSMV_E_ARG;
+    if (!e->has_stream) return fail(e, CSMV_E_STATE, "run before generate");
+    if (!e->pool_loaded) return fail(e, CSMV_E_STATE, "run needs a freshly loaded pool");
+    const uint32_t rows = mbox_rows<kBankR, kBankW>();
+    const Layout L = server_layout(e, rows, kBankW);
+    const size_t lds_bytes = (size_t)L.words * 4;
+    if (lds_bytes > e->lds_max)
+        return fail(e, CSMV_E_CAPACITY, "server LDS layout exceeds 64 KiB (reduce atr_entries)");
+    const uint32_t block = e->cfg.wg_waves * kWave;
+    const uint32_t client_wgs = (e->cfg.client_waves + e->cfg.wg_waves - 1) / e->cfg.wg_waves;
+    const uint32_t grid = 1 + client_wgs;
+    auto kern = bank_kernel<kBankR, kBankW>;
+    int per_cu = 0;
+    HIPCHK(e, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, (int)block, lds_bytes));
+    if ((uint64_t)per_cu * e->n_cu < grid)
+        return fail(e, CSMV_E_CAPACITY,
+                    "grid of " + std::to_string(grid) + " workgroups exceeds the " +
+                        std::to_string((uint64_t)per_cu * e->n_cu) + " the device keeps resident");
+
+    Ctx c{};
+    c.ht = e->ht;
+    c.slot = e->slot;
+    c.n_items = e->cfg.n_items;
+    c.depth = e->cfg.depth;
+    c.gc = (uint32_t *)(e->ctl + kOffGc);
+    c.err = (uint32_t *)(e->ctl + kOffErr);
+    c.clients_done = (uint32_t *)(e->ctl + kOffDone);
+    c.next_txn = (uint64_t *)(e->ctl + kOffNext);
+    c.req_tail = (uint64_t *)(e->ctl + kOffTail);
+    c.stats = (uint64_t *)(e->ctl + kOffStats);
+    c.ring = e->ring;
+    c.reply = e->reply;
+    c.done = e->done;
+    c.mbox = e->mbox;
+    c.stamps = e->stamps;
+    c.qmask = e->Q - 1;
+    c.n_client_waves = e->cfg.client_waves;
+    c.wg_waves = e->cfg.wg_waves;
+    c.variant = e->cfg.variant;
+    c.atr_entries = e->cfg.atr_entries;
+    c.nmb = L.nmb;
+    c.stamp_in_lds = L.stamp_in_lds;
+    c.lds_mb_off = L.mb_off;
+    c.lds_atr_off = L.atr_off;
+    c.lds_stamp_off = L.stamp_off;
+    c.timeout_ticks = (uint64_t)e->cfg.timeout_ms * e->wallclock_khz;
+    c.max_attempts = e->cfg.max_attempts;
+    c.op = e->op;
+    c.addr = e->addr;
+    c.n_txns = e->spec.n_txns;
+    c.log_ts = e->log_ts;
+    c.log_v = e->log_v;
+    c.log_ab = e->log_ab;
+
+    HIPCHK(e, hipMemsetAsync(e->ctl, 0, kCtlBytes, e->stream));
+    HIPCHK(e, hipMemsetAsync(e->ring, 0, e->Q * 8, e->stream));
+    HIPCHK(e, hipMemsetAsync(e->done, 0, e->Q * 8, e->stream));
+    HIPCHK(e, hipMemsetAsync(e->reply, 0, (size_t)e->cfg.client_waves * kWave * 8, e->stream));
+    if ((e->cfg.variant & 3u) == kValStamp && !L.stamp_in_lds)
+        HIPCHK(e, hipMemsetAsync(e->stamps, 0, (size_t)e->cfg.n_items * 4, e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+
+    HIPCHK(e, hipEventRecord(e->ev0, e->stream));
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(block), lds_bytes, e->stream, c);
+    HIPCHK(e, hipGetLastError());
+    HIPCHK(e, hipEventRecord(e->ev1, e->stream));
+    HIPCHK(e, hipEventSynchronize(e->ev1));
+    float ms = 0.f;
+    HIPCHK(e, hipEventElapsedTime(&ms, e->ev0, e->ev1));
+    e->pool_loaded = false;
+
+    uint8_t host_ctl[kCtlBytes];
+    HIPCHK(e, hipMemcpy(host_ctl, e->ctl, kCtlBytes, hipMemcpyDeviceToHost));
+    const uint64_t *st = (const uint64_t *)(host_ctl + kOffStats);
+    uint32_t derr;
+    std::memcpy(&derr, host_ctl + kOffErr, 4);
+    if (out) {
+        std::memset(out, 0, sizeof(*out));
+        out->commits_update = st[kStatCommitUpd];
+        out->commits_ro = st[kStatCommitRo];
+        out->aborts_dataage = st[kStatAbortAge];
+        out->aborts_readwrite = st[kStatAbortRw];
+        out->aborts_prevalid = st[kStatAbortPrevalid];
+        out->aborts_recordage = st[kStatAbortRecord];
+        out->batches = st[kStatBatches];
+        out->final_ts = st[kStatFinalTs];
+        out->kernel_ms = ms;
+        out->error = derr;
+        out->client_lanes = e->cfg.client_waves * kWave;
+    }
+    if (derr == kErrLivelock) return fail(e, CSMV_E_LIVELOCK, "transaction attempt limit");
+    if (derr == kErrReadSetOverflow) return fail(e, CSMV_E_CAPACITY, "read/write set overflow");
+    if (derr) return fail(e, CSMV_E_TIMEOUT, "device wait timed out, code " + std::to_string(derr));
+    return CSMV_OK;
+}
+
+csmv_status csmv_bank_read_log(csmv_engine *e, uint32_t *ts, int32_t *v, uint32_t *aborts)
+{
+    if (!e) return CSMV_E_ARG;
+    if (!e->has_stream) return fail(e, CSMV_E_STATE, "no stream");
+    const uint64_t T = e->spec.n_txns;
+    if (ts) HIPCHK(e, hipMemcpy(ts, e->log_ts, T * 4, hipMemcpyDeviceToHost));
+    if (v) HIPCHK(e, hipMemcpy(v, e->log_v, (size_t)e->spec.n_reads * T * 4, hipMemcpyDeviceToHost));
+    if (aborts) HIPCHK(e, hipMemcpy(aborts, e->log_ab, T * 4, hipMemcpyDeviceToHost));
+    return CSMV_OK;
+}
+
+csmv_status csmv_atomic_rate(csmv_engine *e, uint32_t waves, uint32_t iters, double *rate)
+{
+    if (!e || !rate || waves == 0 || iters == 0 || waves > 65536) return CSMV_E_ARG;
+    uint64_t *p = (uint64_t *)(e->ctl + kOffStats);
+    HIPCHK(e, hipMemsetAsync(e->ctl, 0, kCtlBytes, e->stream));
+    const uint32_t blocks = (waves + 3) / 4;
+    hipLaunchKernelGGL(atomic_rate_kernel, dim3(blocks), dim3(256), 0, e->stream, p, iters);
+    HIPCHK(e, hipGetLastError());
+    HIPCHK(e, hipEventRecord(e->ev0, e->stream));
+    hipLaunchKernelGGL(atomic_rate_kernel, dim3(blocks), dim3(256), 0, e->stream, p, iters);
+    HIPCHK(e, hipEventRecord(e->ev1, e->stream));
+    HIPCHK(e, hipEventSynchronize(e->ev1));
+    float ms = 0.f;
+    HIPCHK(e, hipEventElapsedTime(&ms, e->ev0, e->ev1));
+    *rate = (double)blocks * 4.0 * iters / (ms * 1e-3);
+    return CSMV_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,138 @@
+/*
+ * csmv.h -- C-ABI of the MI355X (gfx950) CSMV engine.
+ *
+ * The reference has no FFI; its boundary is a host init/teardown pair, the
+ * message-ring constructor and the device transaction API.  Each entry point
+ * below names the reference interface it replaces.  All functions return a
+ * csmv_status; none exits the process or throws.  The engine owns device
+ * memory; the caller owns every host buffer it passes.
+ */
+#ifndef CSMV_H
+#define CSMV_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    CSMV_OK = 0,
+    CSMV_E_HIP = 1,        /* a HIP runtime call failed (see csmv_last_error) */
+    CSMV_E_ARG = 2,        /* invalid argument */
+    CSMV_E_CAPACITY = 3,   /* size beyond a configured or device limit */
+    CSMV_E_TIMEOUT = 4,    /* a device wait exceeded the spin timeout */
+    CSMV_E_NODEVICE = 5,   /* no gfx950 device visible */
+    CSMV_E_STATE = 6,      /* call out of order (e.g. run before generate) */
+    CSMV_E_LIVELOCK = 7    /* a transaction exceeded the attempt limit */
+} csmv_status;
+
+/* Commit-path variants.  They are feature flags of one code path; the named
+ * presets reproduce the reference's optimisation directories:
+ *   vanilla           csmv_optimizations/vanilla/csmv_sh           SCAN | LANE  | SERVER_WB
+ *   batchedAtrInsert  csmv_optimizations/batchedAtrInsert/csmv_sh  SCAN | BATCH | CLIENT_WB
+ *   clientWriteback   csmv_optimizations/clientWriteback/csmv_sh   SCAN | BATCH | SERVER_WB
+ *   collabValidation  csmv_optimizations/collabValidation/csmv_sh  COLLAB | BATCH | CLIENT_WB
+ * (the directory named clientWriteback is the one whose server writes back).
+ * STAMP validation is this engine's own mode: an item-indexed commit-stamp
+ * table replaces the linear record scan. */
+enum {
+    CSMV_VAL_SCAN = 0x0,      /* per-lane serial record-ring scan (TXValidate, bank/csmv_sh/API.cu:88-144) */
+    CSMV_VAL_COLLAB = 0x1,    /* 64 lanes scan one client's window (collabValidation/csmv_sh/API.cu:193-266) */
+    CSMV_VAL_STAMP = 0x2,     /* per-item last-commit stamp */
+    CSMV_VAL_MASK = 0x3,
+    CSMV_INSERT_LANE = 0x0,   /* per-lane record insertion (bank/csmv_sh/API.cu:193-272) */
+    CSMV_INSERT_BATCH = 0x4,  /* one batched insertion per wave (batchedAtrInsert/csmv_sh/API.cu:266-389) */
+    CSMV_WB_CLIENT = 0x0,     /* clients install versions (batchedAtrInsert/csmv_sh/API.cu:231-263) */
+    CSMV_WB_SERVER = 0x8      /* the server installs versions (vanilla/csmv_sh/API.cu:269-272) */
+};
+
+#define CSMV_VARIANT_VANILLA          (CSMV_VAL_SCAN | CSMV_INSERT_LANE | CSMV_WB_SERVER)
+#define CSMV_VARIANT_BATCHED_ATR      (CSMV_VAL_SCAN | CSMV_INSERT_BATCH | CSMV_WB_CLIENT)
+#define CSMV_VARIANT_CLIENT_WRITEBACK (CSMV_VAL_SCAN | CSMV_INSERT_BATCH | CSMV_WB_SERVER)
+#define CSMV_VARIANT_COLLAB           (CSMV_VAL_COLLAB | CSMV_INSERT_BATCH | CSMV_WB_CLIENT)
+#define CSMV_VARIANT_STAMP            (CSMV_VAL_STAMP | CSMV_INSERT_BATCH | CSMV_WB_CLIENT)
+
+typedef struct {
+    uint32_t n_items;          /* data items (accounts); VertionedDataItem count */
+    uint32_t depth;            /* versions per item (MaxVersions, API.cuh:18), 1..64 */
+    uint32_t atr_entries;      /* record ring capacity (TXRecordSize, API.cuh:17); power of two */
+    uint32_t client_waves;     /* persistent client wavefronts (64 lanes each) */
+    uint32_t wg_waves;         /* waves per workgroup, 4..16; the server workgroup uses all */
+    uint32_t variant;          /* CSMV_VAL_* | CSMV_INSERT_* | CSMV_WB_* */
+    uint32_t timeout_ms;       /* bound of every device-side wait (0 = 4000) */
+    uint32_t max_attempts;     /* per-transaction attempt bound (0 = 1<<20) */
+} csmv_config;
+
+typedef struct {
+    uint64_t seed;
+    uint64_t n_txns;
+    uint32_t n_reads;      /* reads per transfer (2..4); the first two are the transfer pair */
+    uint32_t n_reads_ro;   /* reads per audit (1..64) */
+    uint32_t pct_ro;       /* percent audits 0..100 */
+    uint32_t dist;         /* 0 uniform, 1 Zipf */
+    uint32_t amount_max;   /* amount uniform in 1..amount_max (1..255) */
+    uint32_t reserved;
+    double theta;          /* Zipf exponent */
+} csmv_bank_spec;
+
+typedef struct {
+    uint64_t commits_update;
+    uint64_t commits_ro;
+    uint64_t aborts_dataage;     /* no version <= snapshot left in the ring */
+    uint64_t aborts_readwrite;   /* read item written by an earlier batch after the snapshot */
+    uint64_t aborts_prevalid;    /* read item written by a lower lane of the same batch */
+    uint64_t aborts_recordage;   /* snapshot older than the record ring covers */
+    uint64_t batches;            /* commit requests served */
+    uint64_t final_ts;           /* last commit timestamp */
+    double kernel_ms;            /* persistent kernel time (HIP events) */
+    uint32_t error;              /* device error word (0 = none) */
+    uint32_t client_lanes;
+} csmv_stats;
+
+const char *csmv_status_str(csmv_status s);
+
+typedef struct csmv_engine csmv_engine;
+
+/* Replaces TXInit's allocations (bank/csmv_sh/API.cu:293-333) and create_gbc
+ * (bank/sync_lib/one_phase_server.h:13-42). */
+csmv_status csmv_create(const csmv_config *cfg, csmv_engine **out);
+/* Replaces TXEnd's frees (API.cu:354-358) and free_gbc (msg_aux.h:94-102). */
+csmv_status csmv_destroy(csmv_engine *e);
+/* Text of the last failure on this engine ("" if none). */
+const char *csmv_last_error(const csmv_engine *e);
+
+/* TXInit's data upload (API.cu:310-322): item i gets one version (ts 0, values[i]). */
+csmv_status csmv_load_i32(csmv_engine *e, const int32_t *values, size_t n);
+/* Same with every item set to one value. */
+csmv_status csmv_fill_i32(csmv_engine *e, int32_t value);
+/* TXEnd's read-back (API.cu:337-348): the head value of every item. */
+csmv_status csmv_read_state_i32(csmv_engine *e, int32_t *out, size_t n);
+
+/* Host: Zipf CDF scaled to 2^31 (cdf[n-1] = 2^31), the table the device
+ * generator searches.  Needs no device. */
+csmv_status csmv_zipf_cdf(uint32_t n, double theta, uint32_t *cdf);
+
+/* Generates the bank stream on the device (the driver's operand draws,
+ * bank/csmv_sh/exe.cu:134-201, as a resident stream). */
+csmv_status csmv_bank_generate(csmv_engine *e, const csmv_bank_spec *spec);
+/* Copies the stream back: op[n_txns], addr[rmax][n_txns]. */
+csmv_status csmv_bank_read_stream(csmv_engine *e, uint32_t *op, uint32_t *addr);
+/* Runs the whole resident stream through the persistent kernel (replaces
+ * parent_kernel/server_kernel/client_kernel, bank/csmv_sh/exe.cu:89-338). */
+csmv_status csmv_bank_run(csmv_engine *e, csmv_stats *stats);
+/* Commit log: ts[n_txns] (commit ts; audit snapshot), v[n_reads][n_txns]
+ * (transfer reads; v[k] = audit sum), aborts[n_txns] (failed attempts). */
+csmv_status csmv_bank_read_log(csmv_engine *e, uint32_t *ts, int32_t *v,
+                               uint32_t *aborts);
+
+/* Same-address device-scope atomic rate (atomics per second), measured by a
+ * microbenchmark kernel: the ceiling unit for the per-lane commit protocol. */
+csmv_status csmv_atomic_rate(csmv_engine *e, uint32_t waves, uint32_t iters,
+                             double *atomics_per_s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,365 @@
+// csmv_device.hpp -- device-side transaction API of the gfx950 CSMV engine.
+//
+// Shape kept from the reference's device API (collabValidation/csmv_sh/API.cuh:135-151,
+// memcachedGPU/memcdSH.cu:154-231):
+//   tx_begin      TXBegin      snapshot timestamp from the global clock
+//   tx_read       TXRead       newest version <= snapshot, read set logged, read-own-write
+//   tx_read_only  TXReadOnly   same, nothing logged
+//   tx_write      TXWrite      write-set upsert (buffered in registers)
+//   tx_commit     TXCommit     wave-synchronous commit; returns the commit timestamp
+//                              (>= 1) or 0 on abort, like the server's reply
+// One transaction per lane.  tx_begin and tx_commit must be called by all 64 lanes of
+// the wavefront (uniform control flow); a lane with nothing to commit passes req=false.
+//
+// Memory layout (structure of arrays in HBM):
+//   ht[i]                  head | tail << 16 of item i's version ring
+//   slot[d * n_items + i]  version << 32 | value (one 64-bit word, read and written
+//                          atomically, so a reader never sees a torn pair)
+// Cross-workgroup traffic uses agent-scope atomics: gfx950's eight XCDs have private
+// L2 caches, and agent scope is what keeps them coherent.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace csmv {
+
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+constexpr uint32_t kWave = 64;
+
+// ---- wave64 primitives --------------------------------------------------------------
+__device__ __forceinline__ uint32_t lane_id()
+{
+    return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+}
+__device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+// Number of set bits of m below this lane (the reference's popc(ballot & lanemask_lt)).
+__device__ __forceinline__ uint32_t prefix_count(uint64_t m)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+__device__ __forceinline__ uint32_t bcast(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ uint64_t bcast64(uint64_t v)
+{
+    uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ uint32_t readlane(uint32_t v, uint32_t l)
+{
+    return __builtin_amdgcn_readlane(v, l);
+}
+__device__ __forceinline__ uint64_t readlane64(uint64_t v, uint32_t l)
+{
+    uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v, l);
+    uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), l);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// ---- agent-scope atomics --------------------------------------------------------------
+template <typename T> __device__ __forceinline__ T ld_rlx(const T *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T> __device__ __forceinline__ T ld_acq(const T *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T> __device__ __forceinline__ void st_rlx(T *p, T v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T> __device__ __forceinline__ void st_rel(T *p, T v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T> __device__ __forceinline__ T fetch_add(T *p, T v)
+{
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void fence_acq() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
+__device__ __forceinline__ void fence_rel() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
+
+// ---- commit-path variant bits (csmv.h CSMV_VAL_* / CSMV_INSERT_* / CSMV_WB_*) -----------
+constexpr uint32_t kValScan = 0x0, kValCollab = 0x1, kValStamp = 0x2;
+constexpr uint32_t kInsertBatch = 0x4, kServerWriteback = 0x8;
+
+// ---- device error codes (ctx.err) ------------------------------------------------------
+enum : uint32_t {
+    kErrReplyTimeout = 1,
+    kErrRingTimeout = 2,
+    kErrMailboxTimeout = 3,
+    kErrAdvanceTimeout = 4,
+    kErrSlotTimeout = 5,
+    kErrDoneRingTimeout = 6,
+    kErrLivelock = 7,
+    kErrReadSetOverflow = 8,
+};
+
+// Engine view passed by value to every kernel.
+struct Ctx {
+    // version pool
+    uint32_t *ht;
+    uint64_t *slot;
+    uint32_t n_items, depth;
+    // control words (each on its own 128-byte line)
+    uint32_t *gc;            // global clock: every commit <= gc is installed
+    uint32_t *err;           // first device error code
+    uint32_t *clients_done;  // client waves that have finished
+    uint64_t *next_txn;      // next stream index to hand out
+    uint64_t *req_tail;      // commit requests issued
+    uint64_t *stats;         // kStat* counters
+    // messaging
+    uint64_t *ring;          // [Q] (n+1) << 32 | client wave
+    uint64_t *reply;         // [client_waves * 64] (n+1) << 32 | commit ts
+    uint64_t *done;          // [Q] (n+1) << 32 | last ts of batch n (write-back finished)
+    uint32_t *mbox;          // [client_waves][rows][64] request mailboxes
+    uint32_t *stamps;        // [n_items] commit stamps (server-private, STAMP mode)
+    uint32_t qmask;
+    uint32_t n_client_waves, wg_waves;
+    uint32_t variant, atr_entries, nmb, stamp_in_lds;
+    uint32_t lds_mb_off, lds_atr_off, lds_stamp_off;  // word offsets in the server LDS
+    uint64_t timeout_ticks;
+    uint32_t max_attempts;
+    // resident bank stream and its commit log
+    const uint32_t *op;
+    const uint32_t *addr;
+    uint64_t n_txns;
+    uint32_t *log_ts;
+    int32_t *log_v;
+    uint32_t *log_ab;
+};
+
+enum : uint32_t {
+    kStatCommitUpd = 0,
+    kStatCommitRo,
+    kStatAbortAge,
+    kStatAbortRw,
+    kStatAbortPrevalid,
+    kStatAbortRecord,
+    kStatBatches,
+    kStatFinalTs,
+    kStatCount
+};
+
+// Bounded wait: every spin loop in the engine ends, either by its condition or by the
+// device timeout, which records an error code that makes all other roles leave too.
+struct Spin {
+    uint64_t t0;
+    uint32_t it;
+    __device__ Spin() : t0(wall_clock64()), it(0) {}
+    __device__ bool tick(const Ctx &c, uint32_t code)
+    {
+        __builtin_amdgcn_s_sleep(1);
+        if ((++it & 31u) == 0) {
+            if (ld_rlx(c.err)) return false;
+            if (wall_clock64() - t0 > c.timeout_ticks) {
+                if (lane_id() == 0) {
+                    uint32_t expected = 0;
+                    __hip_atomic_compare_exchange_strong(c.err, &expected, code,
+                                                         __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_AGENT);
+                }
+                return false;
+            }
+        }
+        return true;
+    }
+};
+
+__device__ __forceinline__ void raise_error(const Ctx &c, uint32_t code)
+{
+    uint32_t expected = 0;
+    __hip_atomic_compare_exchange_strong(c.err, &expected, code, __ATOMIC_RELAXED,
+                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---- per-lane transaction state ----------------------------------------------------------
+template <int R, int W> struct Tx {
+    uint32_t snap;
+    uint32_t nr, nw;
+    bool aborted;
+    bool overflow;
+    uint32_t rs[R];
+    uint32_t wa[W];
+    int32_t wv[W];
+};
+
+// Newest version <= snap of item; false when the ring holds none ("DataAge",
+// bank/csmv_sh/API.cu:20-24 and 26-40).  The head word is read once; a slot rewritten
+// during the walk carries a version newer than the snapshot and is skipped.
+__device__ __forceinline__ bool pool_read(const Ctx &c, uint32_t item, uint32_t snap,
+                                          int32_t &out)
+{
+    const uint32_t ht = ld_rlx(&c.ht[item]);
+    const uint32_t h = ht & 0xFFFFu, t = ht >> 16;
+    uint32_t j = h;
+    for (uint32_t d = 0; d < c.depth; ++d) {
+        const uint64_t x = ld_rlx(&c.slot[(size_t)j * c.n_items + item]);
+        if ((uint32_t)(x >> 32) <= snap) {
+            out = (int32_t)(uint32_t)x;
+            return true;
+        }
+        if (j == t) break;
+        j = (j == 0) ? c.depth - 1 : j - 1;
+    }
+    return false;
+}
+
+// TXBegin (collabValidation/csmv_sh/API.cu:581-586): wave-uniform snapshot.
+template <int R, int W> __device__ __forceinline__ void tx_begin(const Ctx &c, Tx<R, W> &tx)
+{
+    tx.snap = bcast(ld_acq(c.gc));
+    tx.nr = 0;
+    tx.nw = 0;
+    tx.aborted = false;
+    tx.overflow = false;
+#pragma unroll
+    for (int r = 0; r < R; ++r) tx.rs[r] = kNone;
+#pragma unroll
+    for (int w = 0; w < W; ++w) { tx.wa[w] = kNone; tx.wv[w] = 0; }
+}
+
+// TXWrite (bank/csmv_sh/API.cu:69-85): upsert into the register write set.
+template <int R, int W>
+__device__ __forceinline__ void tx_write(Tx<R, W> &tx, uint32_t item, int32_t v)
+{
+    // Register arrays are only updated with constant indices and selects, so they stay
+    // in VGPRs (a predicated store a[n] = x would be folded into scratch memory).
+    bool hit = false;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        const bool m = (uint32_t)w < tx.nw && tx.wa[w] == item;
+        tx.wv[w] = m ? v : tx.wv[w];
+        hit |= m;
+    }
+    if (!hit) {
+        if (tx.nw >= (uint32_t)W) { tx.overflow = true; tx.aborted = true; return; }
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            const bool m = (uint32_t)w == tx.nw;
+            tx.wa[w] = m ? item : tx.wa[w];
+            tx.wv[w] = m ? v : tx.wv[w];
+        }
+        tx.nw++;
+    }
+}
+
+template <int R, int W>
+__device__ __forceinline__ void log_read(Tx<R, W> &tx, uint32_t item)
+{
+    if (tx.nr >= (uint32_t)R) { tx.overflow = true; tx.aborted = true; return; }
+#pragma unroll
+    for (int r = 0; r < R; ++r) tx.rs[r] = (uint32_t)r == tx.nr ? item : tx.rs[r];
+    tx.nr++;
+}
+
+// TXRead (bank/csmv_sh/API.cu:5-41).
+template <int R, int W>
+__device__ __forceinline__ int32_t tx_read(const Ctx &c, Tx<R, W> &tx, uint32_t item)
+{
+    if (tx.aborted) return 0;
+    bool own = false;
+    int32_t v = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        const bool m = (uint32_t)w < tx.nw && tx.wa[w] == item;
+        v = m ? tx.wv[w] : v;
+        own |= m;
+    }
+    if (!own && !pool_read(c, item, tx.snap, v)) { tx.aborted = true; return 0; }
+    log_read(tx, item);
+    return v;
+}
+
+// TXReadOnly (bank/csmv_sh/API.cu:43-67).
+template <int R, int W>
+__device__ __forceinline__ int32_t tx_read_only(const Ctx &c, Tx<R, W> &tx, uint32_t item)
+{
+    if (tx.aborted) return 0;
+    int32_t v = 0;
+    if (!pool_read(c, item, tx.snap, v)) { tx.aborted = true; return 0; }
+    return v;
+}
+
+// Client write-back (TXWriteBack, bank/csmv_sh/API.cu:147-190): evict the oldest version
+// when the ring is full, write (ts, value) at head+1, then publish the new head.  The
+// global clock is advanced by the server workgroup once every lower batch is installed.
+template <int R, int W>
+__device__ __forceinline__ void writeback(const Ctx &c, const uint32_t *wa, const int32_t *wv,
+                                          uint32_t nw, uint32_t ts)
+{
+    uint32_t nh[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        nh[w] = 0;
+        if (ts && (uint32_t)w < nw && wa[w] != kNone) {
+            const uint32_t item = wa[w];
+            const uint32_t ht = ld_rlx(&c.ht[item]);
+            uint32_t h = ht & 0xFFFFu, t = ht >> 16;
+            h = (h + 1 == c.depth) ? 0 : h + 1;
+            if (h == t) t = (t + 1 == c.depth) ? 0 : t + 1;
+            st_rlx(&c.slot[(size_t)h * c.n_items + item],
+                   ((uint64_t)ts << 32) | (uint32_t)wv[w]);
+            nh[w] = h | (t << 16);
+        }
+    }
+    fence_rel();
+#pragma unroll
+    for (int w = 0; w < W; ++w)
+        if (ts && (uint32_t)w < nw && wa[w] != kNone) st_rlx(&c.ht[wa[w]], nh[w]);
+}
+
+template <int R, int W> constexpr uint32_t mbox_rows() { return 1 + R + 2 * W; }
+
+// TXCommit (memcachedGPU/memcdSH.cu:154-231) for one wavefront.  Lanes with req=true
+// send (snapshot, read set, write set) through the wave's mailbox; one agent-scope
+// fetch_add reserves the request slot for all 64 lanes, one release store publishes it.
+// Returns the commit timestamp (0 = aborted).  On a device error *fail is set and the
+// caller must leave its loop.
+template <int R, int W>
+__device__ __forceinline__ uint32_t tx_commit(const Ctx &c, uint32_t cw, Tx<R, W> &tx, bool req, bool *fail)
+{
+    const uint32_t lane = lane_id();
+    constexpr uint32_t rows = mbox_rows<R, W>();
+    uint32_t *mb = c.mbox + (size_t)cw * rows * kWave;
+    st_rlx(&mb[lane], req ? tx.snap : kNone);
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+        st_rlx(&mb[(1 + r) * kWave + lane], (req && (uint32_t)r < tx.nr) ? tx.rs[r] : kNone);
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        st_rlx(&mb[(1 + R + w) * kWave + lane], (req && (uint32_t)w < tx.nw) ? tx.wa[w] : kNone);
+        st_rlx(&mb[(1 + R + W + w) * kWave + lane], (uint32_t)tx.wv[w]);
+    }
+    fence_rel();
+    uint64_t n = 0;
+    if (lane == 0) {
+        n = fetch_add(c.req_tail, (uint64_t)1);
+        st_rel(&c.ring[n & c.qmask], ((n + 1) << 32) | cw);
+    }
+    n = bcast64(n);
+
+    uint64_t v;
+    Spin sp;
+    for (;;) {
+        v = ld_rlx(&c.reply[(size_t)cw * kWave + lane]);
+        if (ballot((v >> 32) != n + 1) == 0) break;
+        if (!sp.tick(c, kErrReplyTimeout)) { *fail = true; return 0; }
+    }
+    const uint32_t ts = req ? (uint32_t)v : 0u;
+    if ((c.variant & kServerWriteback) == 0) {
+        const uint64_t cm = ballot(ts != 0);
+        if (cm) {
+            writeback<R, W>(c, tx.wa, tx.wv, tx.nw, ts);
+            const uint32_t last_lane = 63u - (uint32_t)__builtin_clzll(cm);
+            const uint32_t end_ts = readlane(ts, last_lane);
+            fence_rel();
+            if (lane == 0) st_rlx(&c.done[n & c.qmask], ((n + 1) << 32) | end_ts);
+        }
+    }
+    return ts;
+}
+
+}  // namespace csmv

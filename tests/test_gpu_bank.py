@@ -1,0 +1,86 @@
+"""GPU parity tests of the bank path: every run's commit log replays bit-exactly.
+
+Each case prints what it runs before touching the device.  The checker is the
+CPU oracle (oracle/csmv_oracle.c): serial replay in commit-timestamp order of
+every committed transfer, audit snapshots, final balances, conservation.
+"""
+import numpy as np
+import pytest
+
+import csmv
+import orc
+
+pytestmark = pytest.mark.gpu
+
+INIT = 1000
+
+
+def run_bank(n_items, bank, variant="stamp", depth=4, atr=4096, waves=8, wg_waves=4,
+             check_stream=True, timeout_ms=4000):
+    print(f"\n[case] n_items={n_items} bank={bank} variant={variant} depth={depth} atr={atr} "
+          f"waves={waves}", flush=True)
+    with csmv.Engine(n_items, depth=depth, atr_entries=atr, client_waves=waves,
+                     wg_waves=wg_waves, variant=variant, timeout_ms=timeout_ms) as e:
+        e.fill(INIT)
+        e.bank_generate(bank)
+        stats = e.bank_run()
+        op, addr = e.bank_read_stream()
+        ts, v, ab = e.bank_read_log()
+        final = e.read_state()
+    spec = orc.bank_spec(n_items, bank.n_txns, seed=bank.seed, n_reads=bank.n_reads,
+                         n_reads_ro=bank.n_reads_ro, pct_ro=bank.pct_ro, zipf=bank.zipf,
+                         theta=bank.theta, amount_max=bank.amount_max)
+    if check_stream:
+        oop, oaddr = orc.bank_gen(spec)
+        assert np.array_equal(oop, op), "op stream differs from the oracle's"
+        assert np.array_equal(oaddr, addr), "operand stream differs from the oracle's"
+    rep, _ = orc.bank_replay(spec, op, addr, INIT, ts, v, final)
+    print("[stats]", {k: stats[k] for k in ("commits", "aborts", "abort_rate", "kernel_ms",
+                                             "txn_per_s", "batches")}, flush=True)
+    assert rep["fails"] == 0, rep
+    assert rep["total"] == n_items * INIT
+    assert stats["commits"] == bank.n_txns
+    assert stats["commits_update"] == rep["n_updates"]
+    assert stats["final_ts"] == rep["n_updates"]
+    assert stats["error"] == 0
+    assert ab.sum() == stats["aborts"]
+    return stats
+
+
+@pytest.mark.parametrize("variant", list(csmv.VARIANTS))
+def test_bank_plain_small(variant):
+    """Config 'bank plain' shape, reduced stream: 4096 accounts, 2-account transfers."""
+    run_bank(4096, csmv.Bank(n_txns=1 << 15, seed=11), variant=variant, waves=8)
+
+
+@pytest.mark.parametrize("variant", list(csmv.VARIANTS))
+def test_bank_zipf_small(variant):
+    """Config 'bank at scale' shape at 2^16 accounts: Zipf 0.99, 4 reads / 2 writes."""
+    run_bank(1 << 16, csmv.Bank(n_txns=1 << 15, seed=5, n_reads=4, n_reads_ro=4, zipf=True),
+             variant=variant, depth=8, waves=16)
+
+
+def test_record_age_aborts():
+    """A record ring much smaller than the in-flight window forces RecordAge aborts
+    (bank/csmv_sh/API.cu:93-100); the log must still replay."""
+    st = run_bank(4096, csmv.Bank(n_txns=1 << 14, seed=3), variant="batchedAtrInsert",
+                  atr=64, waves=32)
+    assert st["aborts_recordage"] > 0
+
+
+def test_data_age_aborts():
+    """Depth 1 keeps only the newest version, so stale snapshots abort (DataAge)."""
+    st = run_bank(256, csmv.Bank(n_txns=1 << 14, seed=4, n_reads_ro=16, pct_ro=50),
+                  variant="stamp", depth=1, waves=16)
+    assert st["aborts_dataage"] > 0
+
+
+def test_single_wave_all_commit_order():
+    """One client wave: batches arrive one at a time; every transfer commits exactly once."""
+    run_bank(1 << 16, csmv.Bank(n_txns=1 << 12, seed=9), variant="stamp", waves=1)
+
+
+def test_bank_plain_full():
+    """Config 'bank plain' at full size: 2^20 transactions, 64 client waves, depth 4."""
+    run_bank(4096, csmv.Bank(n_txns=1 << 20, seed=1), variant="stamp", waves=64,
+             check_stream=False)
