@@ -50,7 +50,8 @@ class CsmvError(RuntimeError):
 class Config(C.Structure):
     _fields_ = [("n_items", C.c_uint32), ("depth", C.c_uint32), ("atr_entries", C.c_uint32),
                 ("client_waves", C.c_uint32), ("wg_waves", C.c_uint32), ("variant", C.c_uint32),
-                ("timeout_ms", C.c_uint32), ("max_attempts", C.c_uint32)]
+                ("timeout_ms", C.c_uint32), ("max_attempts", C.c_uint32),
+                ("stage_ahead", C.c_uint32), ("lds_kib", C.c_uint32)]
 
 
 class BankSpec(C.Structure):
@@ -64,10 +65,15 @@ class Stats(C.Structure):
                 ("aborts_dataage", C.c_uint64), ("aborts_readwrite", C.c_uint64),
                 ("aborts_prevalid", C.c_uint64), ("aborts_recordage", C.c_uint64),
                 ("batches", C.c_uint64), ("final_ts", C.c_uint64), ("kernel_ms", C.c_double),
-                ("error", C.c_uint32), ("client_lanes", C.c_uint32)]
+                ("error", C.c_uint32), ("client_lanes", C.c_uint32),
+                ("prof_ns", C.c_double * 8), ("stamp_buckets", C.c_uint32),
+                ("stamp_exact", C.c_uint32)]
+    PROF = ("commit_idle", "commit_busy", "recv_window", "recv_ring", "recv_busy",
+            "client_reply", "client_round", "client_rounds")
 
     def as_dict(self) -> dict:
-        d = {f: getattr(self, f) for f, _ in self._fields_}
+        d = {f: getattr(self, f) for f, _ in self._fields_ if f != "prof_ns"}
+        d["prof"] = {k: self.prof_ns[i] for i, k in enumerate(self.PROF)}
         commits = d["commits_update"] + d["commits_ro"]
         aborts = (d["aborts_dataage"] + d["aborts_readwrite"] + d["aborts_prevalid"]
                   + d["aborts_recordage"])
@@ -150,12 +156,13 @@ class Bank:
 
 class Engine:
     def __init__(self, n_items: int, depth: int = 4, atr_entries: int = 4096,
-                 client_waves: int = 64, wg_waves: int = 4, variant="stamp",
-                 timeout_ms: int = 4000, max_attempts: int = 0):
+                 client_waves: int = 64, wg_waves: int = 8, variant="stamp",
+                 timeout_ms: int = 4000, max_attempts: int = 0, stage_ahead: int = 0,
+                 lds_kib: int = 0):
         if isinstance(variant, str):
             variant = VARIANTS[variant]
         self.cfg = Config(n_items, depth, atr_entries, client_waves, wg_waves, variant,
-                          timeout_ms, max_attempts)
+                          timeout_ms, max_attempts, stage_ahead, lds_kib)
         self.n_items = n_items
         self._h = C.c_void_p()
         st = lib().csmv_create(C.byref(self.cfg), C.byref(self._h))

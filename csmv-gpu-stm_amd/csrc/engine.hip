@@ -127,9 +127,10 @@ template <int R, int W> __device__ __forceinline__ void bank_client(const Ctx &c
     bool has = false, exhausted = false;
     uint64_t k = 0;
     uint32_t op = 0, ab = 0;
-    uint64_t n_upd = 0, n_ro = 0, n_age = 0;
+    uint64_t n_upd = 0, n_ro = 0, n_age = 0, t_reply = 0, t_round = 0, rounds = 0;
     for (;;) {
         if (ld_rlx(c.err)) break;
+        const uint64_t tr0 = wall_clock64();
         if (!exhausted) {
             const uint64_t need = ballot(!has);
             if (need) {
@@ -194,7 +195,7 @@ template <int R, int W> __device__ __forceinline__ void bank_client(const Ctx &c
         const bool req = upd && !tx.aborted;
         if (ballot(req)) {
             bool fail = false;
-            const uint32_t ts = tx_commit<R, W>(c, cw, tx, req, &fail);
+            const uint32_t ts = tx_commit<R, W>(c, cw, tx, req, &fail, &t_reply);
             if (fail) break;
             if (req) {
                 if (ts) {
@@ -211,6 +212,13 @@ template <int R, int W> __device__ __forceinline__ void bank_client(const Ctx &c
             }
         }
         if (has && ab > c.max_attempts) raise_error(c, kErrLivelock);
+        t_round += wall_clock64() - tr0;
+        ++rounds;
+    }
+    if (lane == 0) {
+        fetch_add(&c.prof[kProfClientReply], t_reply);
+        fetch_add(&c.prof[kProfClientRound], t_round);
+        fetch_add(&c.prof[kProfClientRounds], rounds);
     }
     if (n_upd) fetch_add(&c.stats[kStatCommitUpd], n_upd);
     if (n_ro) fetch_add(&c.stats[kStatCommitRo], n_ro);
@@ -220,11 +228,12 @@ template <int R, int W> __device__ __forceinline__ void bank_client(const Ctx &c
                                           __HIP_MEMORY_SCOPE_AGENT);
 }
 
-template <int R, int W> __global__ void __launch_bounds__(512) bank_kernel(Ctx c)
+template <int R, int W>
+__global__ void __launch_bounds__(512) bank_kernel(Ctx c, ServerLayout L)
 {
     extern __shared__ uint32_t lds[];
     if (blockIdx.x == 0) {
-        server_workgroup<R, W>(c, lds);
+        server_workgroup<R, W>(c, L, lds);
         return;
     }
     const uint32_t cw = (blockIdx.x - 1) * c.wg_waves + threadIdx.x / kWave;
@@ -279,7 +288,7 @@ namespace {
 
 constexpr size_t kCtlBytes = 4096;
 constexpr size_t kOffGc = 0, kOffErr = 128, kOffDone = 256, kOffNext = 384, kOffTail = 512,
-                 kOffStats = 640;
+                 kOffStats = 640, kOffProf = 1024;
 
 csmv_status fail(csmv_engine *e, csmv_status s, const std::string &msg)
 {
@@ -319,26 +328,42 @@ uint32_t pow2_at_least(uint64_t v)
     return q;
 }
 
-struct Layout {
-    uint32_t nmb, mb_off, atr_off, stamp_off, stamp_in_lds, words;
-};
-
-Layout server_layout(const csmv_engine *e, uint32_t rows, uint32_t W)
+uint32_t pow2_floor(uint64_t v)
 {
-    Layout L{};
-    const uint32_t M = e->cfg.wg_waves - 2;
-    L.nmb = std::min<uint32_t>(2 * M, 32);
-    const uint32_t hdr = kLdsHdr + 2 * L.nmb;
-    L.mb_off = (hdr + 63) & ~63u;
-    uint32_t w = L.mb_off + L.nmb * rows * kWave;
+    uint32_t q = 1;
+    while ((uint64_t)q * 2 <= v) q <<= 1;
+    return q;
+}
+
+// Server LDS layout for a request wire of R reads and W writes.
+template <int R, int W> ServerLayout server_layout(const csmv_engine *e)
+{
+    ServerLayout L{};
+    const bool swb = (e->cfg.variant & kServerWriteback) != 0;
     const uint32_t vmode = e->cfg.variant & 3u;
+    const uint32_t M = e->cfg.wg_waves - 2;
+    L.nmb = std::min<uint32_t>(std::max<uint32_t>(2 * M, 4), 16);
+    L.mb_rows = swb ? Rows<R, W>::count_server_wb : Rows<R, W>::count_client_wb;
+    L.state_off = kLdsHdr;
+    L.wave_off = L.state_off + L.nmb;
+    L.tg_off = L.wave_off + L.nmb;
+    L.mb_off = (L.tg_off + L.nmb + 63) & ~63u;
+    uint32_t w = L.mb_off + L.nmb * L.mb_rows * kWave;
+    L.hsz = pow2_at_least(2ull * kWave * W);
+    L.hash_off = w;
+    w += M * 3 * L.hsz;
     L.atr_off = w;
     if (vmode != kValStamp) w += W * e->cfg.atr_entries;
-    L.stamp_off = w;
-    L.stamp_in_lds = 0;
-    if (vmode == kValStamp && (uint64_t)(w + e->cfg.n_items) * 4 <= e->lds_max) {
-        L.stamp_in_lds = 1;
-        w += e->cfg.n_items;
+    L.bucket_off = w;
+    L.nbuckets = 0;
+    L.exact = 0;
+    if (vmode == kValStamp) {
+        const uint64_t budget_words = e->lds_max / 4;
+        const uint64_t room = budget_words > w ? budget_words - w : 0;
+        const uint32_t want = pow2_at_least(e->cfg.n_items);
+        L.nbuckets = std::min<uint32_t>(want, pow2_floor(room));
+        L.exact = L.nbuckets >= e->cfg.n_items ? 1u : 0u;
+        w += L.nbuckets;
     }
     L.words = w;
     return L;
@@ -393,9 +418,10 @@ csmv_status csmv_create(const csmv_config *cfg_in, csmv_engine **out)
     csmv_config cfg = *cfg_in;
     if (cfg.timeout_ms == 0) cfg.timeout_ms = 4000;
     if (cfg.max_attempts == 0) cfg.max_attempts = 1u << 20;
-    if (cfg.wg_waves == 0) cfg.wg_waves = 4;
+    if (cfg.wg_waves == 0) cfg.wg_waves = 8;
+    if (cfg.stage_ahead == 0) cfg.stage_ahead = 3;
     if (cfg.n_items == 0 || cfg.depth == 0 || cfg.depth > 64 || cfg.client_waves == 0 ||
-        cfg.wg_waves < 3 || cfg.wg_waves > 8 || cfg.atr_entries < 64 ||
+        cfg.wg_waves < 3 || cfg.wg_waves > 8 || cfg.stage_ahead > 64 || cfg.atr_entries < 64 ||
         (cfg.atr_entries & (cfg.atr_entries - 1)) || cfg.atr_entries > (1u << 16) ||
         (cfg.variant & ~0xFu) || (cfg.variant & 3u) == 3u)
         return CSMV_E_ARG;
@@ -416,7 +442,8 @@ csmv_status csmv_create(const csmv_config *cfg_in, csmv_engine **out)
         return CSMV_E_NODEVICE;
     }
     e->n_cu = (uint32_t)prop.multiProcessorCount;
-    e->lds_max = 64 * 1024;
+    e->lds_max = (uint32_t)prop.sharedMemPerBlock;
+    if (cfg.lds_kib) e->lds_max = std::min<uint32_t>(e->lds_max, cfg.lds_kib * 1024u);
     int khz = 0;
     if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, e->device) != hipSuccess ||
         khz <= 0)
@@ -564,15 +591,17 @@ csmv_status csmv_bank_run(csmv_engine *e, csmv_stats *out)
     if (!e) return CSMV_E_ARG;
     if (!e->has_stream) return fail(e, CSMV_E_STATE, "run before generate");
     if (!e->pool_loaded) return fail(e, CSMV_E_STATE, "run needs a freshly loaded pool");
-    const uint32_t rows = mbox_rows<kBankR, kBankW>();
-    const Layout L = server_layout(e, rows, kBankW);
+    const ServerLayout L = server_layout<kBankR, kBankW>(e);
     const size_t lds_bytes = (size_t)L.words * 4;
     if (lds_bytes > e->lds_max)
-        return fail(e, CSMV_E_CAPACITY, "server LDS layout exceeds 64 KiB (reduce atr_entries)");
+        return fail(e, CSMV_E_CAPACITY, "server LDS layout of " + std::to_string(lds_bytes) +
+                                            " bytes exceeds the budget (reduce atr_entries)");
     const uint32_t block = e->cfg.wg_waves * kWave;
     const uint32_t client_wgs = (e->cfg.client_waves + e->cfg.wg_waves - 1) / e->cfg.wg_waves;
     const uint32_t grid = 1 + client_wgs;
     auto kern = bank_kernel<kBankR, kBankW>;
+    HIPCHK(e, hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds_bytes));
     int per_cu = 0;
     HIPCHK(e, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, (int)block, lds_bytes));
     if ((uint64_t)per_cu * e->n_cu < grid)
@@ -596,16 +625,13 @@ csmv_status csmv_bank_run(csmv_engine *e, csmv_stats *out)
     c.done = e->done;
     c.mbox = e->mbox;
     c.stamps = e->stamps;
+    c.prof = (uint64_t *)(e->ctl + kOffProf);
     c.qmask = e->Q - 1;
     c.n_client_waves = e->cfg.client_waves;
     c.wg_waves = e->cfg.wg_waves;
     c.variant = e->cfg.variant;
     c.atr_entries = e->cfg.atr_entries;
-    c.nmb = L.nmb;
-    c.stamp_in_lds = L.stamp_in_lds;
-    c.lds_mb_off = L.mb_off;
-    c.lds_atr_off = L.atr_off;
-    c.lds_stamp_off = L.stamp_off;
+    c.stage_ahead = e->cfg.stage_ahead;
     c.timeout_ticks = (uint64_t)e->cfg.timeout_ms * e->wallclock_khz;
     c.max_attempts = e->cfg.max_attempts;
     c.op = e->op;
@@ -619,12 +645,12 @@ csmv_status csmv_bank_run(csmv_engine *e, csmv_stats *out)
     HIPCHK(e, hipMemsetAsync(e->ring, 0, e->Q * 8, e->stream));
     HIPCHK(e, hipMemsetAsync(e->done, 0, e->Q * 8, e->stream));
     HIPCHK(e, hipMemsetAsync(e->reply, 0, (size_t)e->cfg.client_waves * kWave * 8, e->stream));
-    if ((e->cfg.variant & 3u) == kValStamp && !L.stamp_in_lds)
+    if ((e->cfg.variant & 3u) == kValStamp && !L.exact)
         HIPCHK(e, hipMemsetAsync(e->stamps, 0, (size_t)e->cfg.n_items * 4, e->stream));
     HIPCHK(e, hipStreamSynchronize(e->stream));
 
     HIPCHK(e, hipEventRecord(e->ev0, e->stream));
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(block), lds_bytes, e->stream, c);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(block), lds_bytes, e->stream, c, L);
     HIPCHK(e, hipGetLastError());
     HIPCHK(e, hipEventRecord(e->ev1, e->stream));
     HIPCHK(e, hipEventSynchronize(e->ev1));
@@ -650,6 +676,12 @@ csmv_status csmv_bank_run(csmv_engine *e, csmv_stats *out)
         out->kernel_ms = ms;
         out->error = derr;
         out->client_lanes = e->cfg.client_waves * kWave;
+        const uint64_t *pf = (const uint64_t *)(host_ctl + kOffProf);
+        const double ns_per_tick = 1e6 / (double)e->wallclock_khz;
+        for (int i = 0; i < 7; ++i) out->prof_ns[i] = (double)pf[i] * ns_per_tick;
+        out->prof_ns[7] = (double)pf[kProfClientRounds];
+        out->stamp_buckets = L.nbuckets;
+        out->stamp_exact = L.exact;
     }
     if (derr == kErrLivelock) return fail(e, CSMV_E_LIVELOCK, "transaction attempt limit");
     if (derr == kErrReadSetOverflow) return fail(e, CSMV_E_CAPACITY, "read/write set overflow");

@@ -8,22 +8,75 @@
 //
 // Here the server is ONE workgroup whose waves have fixed roles:
 //   wave 0       committer: takes requests strictly in arrival order from LDS
-//                mailboxes, validates the 64 lanes of a request at once, resolves
-//                conflicts between lanes of the same request in lane order, assigns
-//                consecutive timestamps, records the write sets and replies;
+//                mailboxes and does only what must be serial: the check against
+//                commits decided since the request was staged, the lane-order
+//                resolution of conflicts inside the request, timestamps, stamping,
+//                the reply.  A single wave issues about one VALU op per 4 cycles, so
+//                everything that can be computed per request in parallel is moved
+//                off this wave.
 //   wave 1       clock advancer (client write-back): scans 64 completion words per
-//                step and moves the global clock over every installed batch;
+//                step and moves the global clock over every installed batch.
 //   waves 2..    receivers: request n goes to receiver n mod M, which polls the
-//                request ring and copies the client's mailbox into LDS mailbox
-//                n mod NMB, so the committer never waits on HBM for request data.
+//                request ring, copies the client's mailbox into LDS mailbox n mod NMB
+//                and precomputes (a) each lane's conflicts with lower lanes of the
+//                same request (an LDS hash of the request's write set; the
+//                reference's TXPreValidation, batchedAtrInsert/csmv_sh/API.cu:94-191)
+//                and (b) in hashed-stamp mode, each lane's conflicts with commits
+//                already stamped when the request was staged.
+//
+// Commit stamps (validation mode STAMP): stamp[x] is the timestamp of the last commit
+// that wrote item x.  A lane with snapshot s and read/write items X may commit iff no
+// x in X has stamp[x] > s -- the reference's record-ring test (TXValidate,
+// bank/csmv_sh/API.cu:104-122) with an unbounded ring, so there are no RecordAge
+// aborts.  When all items fit, the exact stamps live in LDS.  Otherwise the exact
+// table is in HBM and LDS holds B buckets, bucket[h(x)] = max stamp of the items
+// hashing there; the receiver gathers exact stamps as of commit T_g, and the committer
+// flags x when bucket[h(x)] > T_g (written after staging).  That is sound, and wrong
+// only towards aborting when another item shares the bucket.
 #pragma once
 
 #include "csmv_device.hpp"
 
 namespace csmv {
 
-enum : uint32_t { kLdsCnext = 0, kLdsCdone = 1, kLdsCfinal = 2, kLdsAdv = 3, kLdsHdr = 8 };
+enum : uint32_t {
+    kLdsCnext = 0,    // requests whose mailbox the committer has consumed
+    kLdsCdone = 1,    // committer finished
+    kLdsCfinal = 2,   // requests served (valid once kLdsCdone)
+    kLdsAdv = 3,      // batches the advancer has moved the clock over
+    kLdsStampTs = 4,  // every stamp <= this value is visible in the HBM stamp table
+    kLdsHdr = 16
+};
 constexpr uint32_t kExitBit = 0x80000000u;
+
+// Server LDS layout (word offsets), computed on the host.
+struct ServerLayout {
+    uint32_t nmb;          // LDS mailboxes
+    uint32_t mb_rows;      // rows of 64 words per LDS mailbox
+    uint32_t mb_off;       // mailboxes
+    uint32_t state_off;    // mailbox state words [nmb]
+    uint32_t wave_off;     // client wave per mailbox [nmb]
+    uint32_t tg_off;       // T_g per mailbox [nmb]
+    uint32_t bucket_off;   // commit-stamp buckets [nbuckets]
+    uint32_t nbuckets;     // power of two (STAMP mode)
+    uint32_t exact;        // bucket i is item i (n_items <= nbuckets)
+    uint32_t atr_off;      // record ring [W][E] (SCAN / COLLAB)
+    uint32_t hash_off;     // per-receiver hash tables [M][3][hsz]
+    uint32_t hsz;          // power of two >= 2 * 64 * W
+    uint32_t words;        // total
+};
+
+// row indices inside an LDS mailbox
+template <int R, int W> struct Rows {
+    static constexpr uint32_t snap = 0;
+    static constexpr uint32_t items = 1;              // R reads then W writes
+    static constexpr uint32_t conf_lo = 1 + R + W;
+    static constexpr uint32_t conf_hi = 2 + R + W;
+    static constexpr uint32_t pref = 3 + R + W;
+    static constexpr uint32_t wv = 4 + R + W;         // server write-back only
+    static constexpr uint32_t count_client_wb = 4 + R + W;
+    static constexpr uint32_t count_server_wb = 4 + R + 2 * W;
+};
 
 __device__ __forceinline__ uint32_t lds_ld_acq(const uint32_t *p)
 {
@@ -33,6 +86,7 @@ __device__ __forceinline__ void lds_st_rel(uint32_t *p, uint32_t v)
 {
     __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
+__device__ __forceinline__ void wg_fence() { __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup"); }
 
 __device__ __forceinline__ uint32_t wave_min(uint32_t v)
 {
@@ -42,6 +96,21 @@ __device__ __forceinline__ uint32_t wave_min(uint32_t v)
         v = o < v ? o : v;
     }
     return v;
+}
+
+__device__ __forceinline__ uint32_t mix32(uint32_t x)
+{
+    x ^= x >> 16;
+    x *= 0x7FEB352Du;
+    x ^= x >> 15;
+    x *= 0x846CA68Bu;
+    x ^= x >> 16;
+    return x;
+}
+
+__device__ __forceinline__ uint32_t bucket_of(const ServerLayout &L, uint32_t x)
+{
+    return L.exact ? x : (mix32(x) & (L.nbuckets - 1));
 }
 
 template <int R, int W>
@@ -56,27 +125,31 @@ __device__ __forceinline__ bool hits(uint32_t x, const uint32_t *rs, const uint3
 }
 
 // ---- committer --------------------------------------------------------------------------
-template <int R, int W> __device__ __forceinline__ void committer(const Ctx &c, uint32_t *lds)
+template <int R, int W>
+__device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, uint32_t *lds)
 {
-    constexpr uint32_t rows = mbox_rows<R, W>();
+    using RW = Rows<R, W>;
     const uint32_t lane = lane_id();
     const uint32_t E = c.atr_entries, emask = E - 1;
     const uint32_t vmode = c.variant & 0x3u;
     const bool batched = (c.variant & kInsertBatch) != 0;
     const bool server_wb = (c.variant & kServerWriteback) != 0;
-    uint32_t *mbst = lds + kLdsHdr;
-    uint32_t *mbwave = lds + kLdsHdr + c.nmb;
-    uint32_t *atr = lds + c.lds_atr_off;
-    uint32_t *lstamp = lds + c.lds_stamp_off;
+    uint32_t *mbst = lds + L.state_off;
+    uint32_t *mbwave = lds + L.wave_off;
+    uint32_t *mbtg = lds + L.tg_off;
+    uint32_t *bucket = lds + L.bucket_off;
+    uint32_t *atr = lds + L.atr_off;
     const uint32_t Q = c.qmask + 1;
+    const uint64_t lt = (1ull << lane) - 1;
 
     uint32_t last = 0;  // newest assigned commit timestamp
-    uint64_t n_rw = 0, n_pv = 0, n_rec = 0;
+    uint64_t n_rw = 0, n_pv = 0, n_rec = 0, t_idle = 0, t_busy = 0;
     uint32_t n = 0;
     for (;; ++n) {
-        const uint32_t mbi = n % c.nmb;
+        const uint32_t mbi = n % L.nmb;
         uint32_t s = 0;
         bool bad = false;
+        const uint64_t t0 = wall_clock64();
         {
             Spin sp;
             for (;;) {
@@ -92,33 +165,42 @@ template <int R, int W> __device__ __forceinline__ void committer(const Ctx &c, 
                 if (!sp.tick(c, kErrDoneRingTimeout)) { bad = true; break; }
             if (bad) break;
         }
+        const uint64_t t1 = wall_clock64();
         const uint32_t cw = mbwave[mbi];
-        const uint32_t *box = lds + c.lds_mb_off + mbi * rows * kWave;
-        const uint32_t snap = box[lane];
+        const uint32_t tg = mbtg[mbi];
+        const uint32_t *box = lds + L.mb_off + mbi * L.mb_rows * kWave;
+        const uint32_t snap = box[RW::snap * kWave + lane];
         uint32_t rs[R], wa[W];
         int32_t wv[W];
 #pragma unroll
-        for (int r = 0; r < R; ++r) rs[r] = box[(1 + r) * kWave + lane];
+        for (int r = 0; r < R; ++r) rs[r] = box[(RW::items + r) * kWave + lane];
 #pragma unroll
         for (int w = 0; w < W; ++w) {
-            wa[w] = box[(1 + R + w) * kWave + lane];
-            wv[w] = server_wb ? (int32_t)box[(1 + R + W + w) * kWave + lane] : 0;
+            wa[w] = box[(RW::items + R + w) * kWave + lane];
+            wv[w] = server_wb ? (int32_t)box[(RW::wv + w) * kWave + lane] : 0;
         }
-        // the mailbox is in registers now: hand it back to the receivers
-        if (lane == 0) lds_st_rel(&lds[kLdsCnext], n + 1);
+        const uint64_t conf = ((uint64_t)box[RW::conf_hi * kWave + lane] << 32) |
+                              box[RW::conf_lo * kWave + lane];
+        const bool pref = box[RW::pref * kWave + lane] != 0;
+        // mailbox consumed; every stamp <= last is visible in HBM once this release
+        // has waited for the previous batch's stores
+        if (lane == 0) {
+            lds_st_rel(&lds[kLdsStampTs], last);
+            lds_st_rel(&lds[kLdsCnext], n + 1);
+        }
 
         const bool valid = snap != kNone;
         bool rec = false, conflict = false;
         if (vmode == kValStamp) {
             if (valid) {
+                const uint32_t thr = L.exact ? snap : tg;
+                conflict = pref;
 #pragma unroll
                 for (int r = 0; r < R; ++r)
-                    if (rs[r] != kNone)
-                        conflict |= (c.stamp_in_lds ? lstamp[rs[r]] : c.stamps[rs[r]]) > snap;
+                    if (rs[r] != kNone) conflict |= bucket[bucket_of(L, rs[r])] > thr;
 #pragma unroll
                 for (int w = 0; w < W; ++w)
-                    if (wa[w] != kNone)
-                        conflict |= (c.stamp_in_lds ? lstamp[wa[w]] : c.stamps[wa[w]]) > snap;
+                    if (wa[w] != kNone) conflict |= bucket[bucket_of(L, wa[w])] > thr;
             }
         } else {
             // record ring indexed by timestamp: entry t lives at t mod E and the ring
@@ -181,79 +263,55 @@ template <int R, int W> __device__ __forceinline__ void committer(const Ctx &c, 
         n_rec += __builtin_popcountll(ballot(rec));
         n_rw += __builtin_popcountll(ballot(valid && !rec && conflict));
 
-        // Lanes of one request, in lane order (TXPreValidation,
-        // batchedAtrInsert/csmv_sh/API.cu:94-191): lane L conflicts with lower lane i
-        // when i writes an item L reads or writes; L commits only if no conflicting
-        // lower lane commits.
+        // Lane order inside the request: lane L commits only if no lower lane it
+        // conflicts with commits.  Lanes without such conflicts are final at once;
+        // the others are decided in ascending order.
         const uint64_t m = ballot(ok);
-        uint64_t conf = 0;
-        {
-            uint64_t it = m;
-            while (it) {
-                const uint32_t i = (uint32_t)__builtin_ctzll(it);
-                it &= it - 1;
-                bool h = false;
-#pragma unroll
-                for (int w = 0; w < W; ++w) {
-                    const uint32_t x = readlane(wa[w], i);
-                    if (x != kNone) h |= hits<R, W>(x, rs, wa);
-                }
-                if (h && lane > i) conf |= 1ull << i;
-            }
-        }
         uint64_t committed = m;
-        if (ballot(ok && conf != 0)) {
-            committed = 0;
-            uint64_t it = m;
-            while (it) {
-                const uint32_t i = (uint32_t)__builtin_ctzll(it);
-                it &= it - 1;
-                if ((readlane64(conf, i) & committed) == 0) committed |= 1ull << i;
-            }
+        uint64_t it = ballot(ok && (conf & m) != 0);
+        committed &= ~it;
+        while (it) {
+            const uint32_t i = (uint32_t)__builtin_ctzll(it);
+            it &= it - 1;
+            if ((readlane64(conf, i) & committed) == 0) committed |= 1ull << i;
         }
         n_pv += __builtin_popcountll(m & ~committed);
         const bool mine = (committed >> lane) & 1ull;
         const uint32_t nc = (uint32_t)__builtin_popcountll(committed);
         uint32_t myts = 0;
+        auto record = [&](uint32_t ts) {
+            if (vmode == kValStamp) {
+#pragma unroll
+                for (int w = 0; w < W; ++w)
+                    if (wa[w] != kNone) {
+                        __hip_atomic_fetch_max(&bucket[bucket_of(L, wa[w])], ts, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+                        if (!L.exact) c.stamps[wa[w]] = ts;
+                    }
+            } else {
+#pragma unroll
+                for (int w = 0; w < W; ++w) atr[w * E + (ts & emask)] = wa[w];
+            }
+        };
         if (batched) {
             // one batched insertion (batchedAtrInsert/csmv_sh/API.cu:296-377):
             // consecutive timestamps in lane order, offset = popc(committed & lt)
             if (mine) {
-                myts = last + 1 + prefix_count(committed);
-                if (vmode == kValStamp) {
-#pragma unroll
-                    for (int w = 0; w < W; ++w)
-                        if (wa[w] != kNone) {
-                            if (c.stamp_in_lds) lstamp[wa[w]] = myts;
-                            else c.stamps[wa[w]] = myts;
-                        }
-                } else {
-#pragma unroll
-                    for (int w = 0; w < W; ++w) atr[w * E + (myts & emask)] = wa[w];
-                }
+                myts = last + 1 + (uint32_t)__builtin_popcountll(committed & lt);
+                record(myts);
             }
             last += nc;
         } else {
             // per-lane insertion (bank/csmv_sh/API.cu:219-265): one record and one
             // head publication per committing lane, in lane order.
-            uint64_t it = committed;
-            while (it) {
-                const uint32_t i = (uint32_t)__builtin_ctzll(it);
-                it &= it - 1;
+            uint64_t jt = committed;
+            while (jt) {
+                const uint32_t i = (uint32_t)__builtin_ctzll(jt);
+                jt &= jt - 1;
                 ++last;
                 if (lane == i) {
                     myts = last;
-                    if (vmode == kValStamp) {
-#pragma unroll
-                        for (int w = 0; w < W; ++w)
-                            if (wa[w] != kNone) {
-                                if (c.stamp_in_lds) lstamp[wa[w]] = myts;
-                                else c.stamps[wa[w]] = myts;
-                            }
-                    } else {
-#pragma unroll
-                        for (int w = 0; w < W; ++w) atr[w * E + (myts & emask)] = wa[w];
-                    }
+                    record(myts);
                 }
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
             }
@@ -268,6 +326,9 @@ template <int R, int W> __device__ __forceinline__ void committer(const Ctx &c, 
         st_rlx(&c.reply[(size_t)cw * kWave + lane], ((uint64_t)(n + 1) << 32) | myts);
         if (!server_wb && nc == 0 && lane == 0)
             st_rlx(&c.done[n & c.qmask], ((uint64_t)(n + 1) << 32) | last);
+        const uint64_t t2 = wall_clock64();
+        t_idle += t1 - t0;
+        t_busy += t2 - t1;
     }
     if (lane == 0) {
         lds[kLdsCfinal] = n;
@@ -277,6 +338,8 @@ template <int R, int W> __device__ __forceinline__ void committer(const Ctx &c, 
         fetch_add(&c.stats[kStatAbortRecord], n_rec);
         fetch_add(&c.stats[kStatBatches], (uint64_t)n);
         st_rlx(&c.stats[kStatFinalTs], (uint64_t)last);
+        fetch_add(&c.prof[kProfCommitIdle], t_idle);
+        fetch_add(&c.prof[kProfCommitBusy], t_busy);
     }
 }
 
@@ -311,61 +374,195 @@ __device__ __forceinline__ void advancer(const Ctx &c, uint32_t *lds)
 
 // ---- receivers ------------------------------------------------------------------------------
 template <int R, int W>
-__device__ __forceinline__ void receiver(const Ctx &c, uint32_t *lds, uint32_t rid, uint32_t M)
+__device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, uint32_t *lds,
+                                         uint32_t rid, uint32_t M)
 {
-    constexpr uint32_t rows = mbox_rows<R, W>();
-    const uint32_t rows_copy = (c.variant & kServerWriteback) ? rows : 1 + R + W;
+    using RW = Rows<R, W>;
+    const bool server_wb = (c.variant & kServerWriteback) != 0;
+    const bool stamp_mode = (c.variant & 0x3u) == kValStamp;
     const uint32_t lane = lane_id();
-    uint32_t *mbst = lds + kLdsHdr;
-    uint32_t *mbwave = lds + kLdsHdr + c.nmb;
+    const uint64_t lt = (1ull << lane) - 1;
+    uint32_t *mbst = lds + L.state_off;
+    uint32_t *mbwave = lds + L.wave_off;
+    uint32_t *mbtg = lds + L.tg_off;
+    uint32_t *hkey = lds + L.hash_off + rid * 3 * L.hsz;
+    uint32_t *hlo = hkey + L.hsz;
+    uint32_t *hhi = hlo + L.hsz;
+    const uint32_t hmask = L.hsz - 1;
+    constexpr uint32_t in_rows = mbox_rows<R, W>();
+    uint64_t t_window = 0, t_ring = 0, t_busy = 0;
     for (uint32_t n = rid;; n += M) {
-        const uint32_t mbi = n % c.nmb;
-        if (n >= c.nmb) {
+        const uint32_t mbi = n % L.nmb;
+        const uint64_t t0 = wall_clock64();
+        // stage request n once its mailbox is free and the committer is at most
+        // stage_ahead requests behind it (a short window keeps T_g fresh)
+        const uint32_t need = n + 1 > L.nmb ? n + 1 - L.nmb : 0;
+        const uint32_t need2 = n > c.stage_ahead ? n - c.stage_ahead : 0;
+        const uint32_t want = need > need2 ? need : need2;
+        if (want) {
+            // a receiver past the last request waits here until the committer ends
             Spin sp;
             bool bad = false;
-            while (lds_ld_acq(&lds[kLdsCnext]) < n - c.nmb + 1)
-                if (!sp.tick(c, kErrSlotTimeout)) { bad = true; break; }
-            if (bad) return;
+            while (lds_ld_acq(&lds[kLdsCnext]) < want)
+                if (lds_ld_acq(&lds[kLdsCdone]) || !sp.tick(c, kErrSlotTimeout)) {
+                    bad = true;
+                    break;
+                }
+            if (bad) break;
         }
+        const uint64_t t1 = wall_clock64();
         uint64_t e;
+        bool leave = false;
         {
             Spin sp;
-            for (;;) {
+            for (uint32_t polls = 0;; ++polls) {
                 e = ld_rlx(&c.ring[n & c.qmask]);
                 if ((e >> 32) == (uint64_t)n + 1) break;
-                if (ld_acq(c.clients_done) == c.n_client_waves && ld_acq(c.req_tail) <= n) {
+                if ((polls & 7u) == 7u && ld_acq(c.clients_done) == c.n_client_waves &&
+                    ld_acq(c.req_tail) <= n) {
                     // every client has left and request n will never come
                     if (ld_rlx(c.req_tail) == n && lane == 0)
                         lds_st_rel(&mbst[mbi], kExitBit | (n + 1));
-                    return;
+                    leave = true;
+                    break;
                 }
-                if (!sp.tick(c, kErrRingTimeout)) return;
+                if (!sp.tick(c, kErrRingTimeout)) { leave = true; break; }
             }
         }
+        if (leave) break;
+        const uint64_t t2 = wall_clock64();
         fence_acq();
         const uint32_t cw = (uint32_t)e;
-        const uint32_t *src = c.mbox + (size_t)cw * rows * kWave;
-        uint32_t *dst = lds + c.lds_mb_off + mbi * rows * kWave;
-        for (uint32_t row = 0; row < rows_copy; ++row)
-            dst[row * kWave + lane] = ld_rlx(&src[row * kWave + lane]);
-        if (lane == 0) mbwave[mbi] = cw;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        const uint32_t *src = c.mbox + (size_t)cw * in_rows * kWave;
+        uint32_t *dst = lds + L.mb_off + mbi * L.mb_rows * kWave;
+        const uint32_t snap = ld_rlx(&src[lane]);
+        uint32_t it[R + W];
+#pragma unroll
+        for (int r = 0; r < R + W; ++r) it[r] = ld_rlx(&src[(1 + r) * kWave + lane]);
+        int32_t wv[W];
+        if (server_wb) {
+#pragma unroll
+            for (int w = 0; w < W; ++w) wv[w] = (int32_t)ld_rlx(&src[(1 + R + W + w) * kWave + lane]);
+        }
+        const bool valid = snap != kNone;
+
+        // (b) commits already stamped in HBM (hashed-stamp mode)
+        const uint32_t tg = lds_ld_acq(&lds[kLdsStampTs]);
+        bool pref = false;
+        if (stamp_mode && !L.exact && valid) {
+            uint32_t p = 0;
+#pragma unroll
+            for (int r = 0; r < R + W; ++r)
+                if (it[r] != kNone) {
+                    const uint32_t v = __hip_atomic_load(&c.stamps[it[r]], __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_WORKGROUP);
+                    p = v > p ? v : p;
+                }
+            pref = p > snap;
+        }
+
+        // (a) conflicts with lower lanes of the request: insert the write set into an
+        // LDS hash (item -> mask of writing lanes), then look up reads and writes
+        uint32_t slot[W];
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            slot[w] = kNone;
+            const uint32_t x = it[R + w];
+            if (valid && x != kNone) {
+                uint32_t h = mix32(x) & hmask;
+                for (uint32_t p = 0; p < L.hsz; ++p) {
+                    const uint32_t old = atomicCAS(&hkey[h], kNone, x);
+                    if (old == kNone || old == x) {
+                        if (lane < 32) atomicOr(&hlo[h], 1u << lane);
+                        else atomicOr(&hhi[h], 1u << (lane - 32));
+                        slot[w] = h;
+                        break;
+                    }
+                    h = (h + 1) & hmask;
+                }
+            }
+        }
+        wg_fence();
+        uint64_t conf = 0;
+        if (valid) {
+#pragma unroll
+            for (int r = 0; r < R + W; ++r) {
+                const uint32_t x = it[r];
+                if (x == kNone) continue;
+                uint32_t h = mix32(x) & hmask;
+                for (uint32_t p = 0; p < L.hsz; ++p) {
+                    const uint32_t k = __hip_atomic_load(&hkey[h], __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_WORKGROUP);
+                    if (k == x) {
+                        const uint32_t mh = __hip_atomic_load(&hhi[h], __ATOMIC_RELAXED,
+                                                              __HIP_MEMORY_SCOPE_WORKGROUP);
+                        const uint32_t ml = __hip_atomic_load(&hlo[h], __ATOMIC_RELAXED,
+                                                              __HIP_MEMORY_SCOPE_WORKGROUP);
+                        conf |= (((uint64_t)mh << 32) | ml) & lt;
+                        break;
+                    }
+                    if (k == kNone) break;
+                    h = (h + 1) & hmask;
+                }
+            }
+        }
+        wg_fence();
+#pragma unroll
+        for (int w = 0; w < W; ++w)
+            if (slot[w] != kNone) {
+                __hip_atomic_store(&hkey[slot[w]], kNone, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                __hip_atomic_store(&hlo[slot[w]], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                __hip_atomic_store(&hhi[slot[w]], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+
+        dst[RW::snap * kWave + lane] = snap;
+#pragma unroll
+        for (int r = 0; r < R + W; ++r) dst[(RW::items + r) * kWave + lane] = it[r];
+        dst[RW::conf_lo * kWave + lane] = (uint32_t)conf;
+        dst[RW::conf_hi * kWave + lane] = (uint32_t)(conf >> 32);
+        dst[RW::pref * kWave + lane] = pref ? 1u : 0u;
+        if (server_wb) {
+#pragma unroll
+            for (int w = 0; w < W; ++w) dst[(RW::wv + w) * kWave + lane] = (uint32_t)wv[w];
+        }
+        if (lane == 0) {
+            mbwave[mbi] = cw;
+            mbtg[mbi] = tg;
+        }
+        wg_fence();
         if (lane == 0) lds_st_rel(&mbst[mbi], n + 1);
+        const uint64_t t3 = wall_clock64();
+        t_window += t1 - t0;
+        t_ring += t2 - t1;
+        t_busy += t3 - t2;
+    }
+    if (lane == 0) {
+        fetch_add(&c.prof[kProfRecvWindow], t_window);
+        fetch_add(&c.prof[kProfRecvRing], t_ring);
+        fetch_add(&c.prof[kProfRecvBusy], t_busy);
     }
 }
 
-template <int R, int W> __device__ __forceinline__ void server_workgroup(const Ctx &c, uint32_t *lds)
+template <int R, int W>
+__device__ __forceinline__ void server_workgroup(const Ctx &c, const ServerLayout &L, uint32_t *lds)
 {
     const uint32_t wave = threadIdx.x / kWave;
-    const uint32_t hdr = c.lds_mb_off;
-    for (uint32_t i = threadIdx.x; i < hdr; i += blockDim.x) lds[i] = 0;
-    if (c.stamp_in_lds)
-        for (uint32_t i = threadIdx.x; i < c.n_items; i += blockDim.x) lds[c.lds_stamp_off + i] = 0;
+    for (uint32_t i = threadIdx.x; i < L.mb_off; i += blockDim.x) lds[i] = 0;
+    if ((c.variant & 0x3u) == kValStamp)
+        for (uint32_t i = threadIdx.x; i < L.nbuckets; i += blockDim.x) lds[L.bucket_off + i] = 0;
+    const uint32_t M = c.wg_waves - 2;
+    for (uint32_t r = 0; r < M; ++r)
+        for (uint32_t i = threadIdx.x; i < L.hsz; i += blockDim.x) {
+            uint32_t *h = lds + L.hash_off + r * 3 * L.hsz;
+            h[i] = kNone;
+            h[L.hsz + i] = 0;
+            h[2 * L.hsz + i] = 0;
+        }
     __syncthreads();
-    if (wave == 0) committer<R, W>(c, lds);
+    if (wave == 0) committer<R, W>(c, L, lds);
     else if (wave == 1) {
         if ((c.variant & kServerWriteback) == 0) advancer(c, lds);
-    } else receiver<R, W>(c, lds, wave - 2, c.wg_waves - 2);
+    } else receiver<R, W>(c, L, lds, wave - 2, M);
 }
 
 }  // namespace csmv

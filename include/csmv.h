@@ -63,6 +63,8 @@ typedef struct {
     uint32_t variant;          /* CSMV_VAL_* | CSMV_INSERT_* | CSMV_WB_* */
     uint32_t timeout_ms;       /* bound of every device-side wait (0 = 4000) */
     uint32_t max_attempts;     /* per-transaction attempt bound (0 = 1<<20) */
+    uint32_t stage_ahead;      /* requests staged ahead of the committer (0 = 3) */
+    uint32_t lds_kib;          /* server LDS budget in KiB (0 = device maximum) */
 } csmv_config;
 
 typedef struct {
@@ -89,6 +91,13 @@ typedef struct {
     double kernel_ms;            /* persistent kernel time (HIP events) */
     uint32_t error;              /* device error word (0 = none) */
     uint32_t client_lanes;
+    /* timing counters in nanoseconds (the reference's time_rate, API.cuh:121-135):
+     * [0] committer idle, [1] committer busy, [2] receivers waiting for the
+     * committer, [3] receivers waiting for requests, [4] receivers staging,
+     * [5] client waves waiting for replies, [6] client wave rounds, [7] rounds (count) */
+    double prof_ns[8];
+    uint32_t stamp_buckets;      /* LDS commit-stamp buckets (STAMP mode) */
+    uint32_t stamp_exact;        /* 1 if every item has its own bucket */
 } csmv_stats;
 
 const char *csmv_status_str(csmv_status s);

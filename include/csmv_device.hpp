@@ -116,10 +116,11 @@ struct Ctx {
     uint64_t *done;          // [Q] (n+1) << 32 | last ts of batch n (write-back finished)
     uint32_t *mbox;          // [client_waves][rows][64] request mailboxes
     uint32_t *stamps;        // [n_items] commit stamps (server-private, STAMP mode)
+    uint64_t *prof;          // kProf* timing counters (wall_clock64 ticks)
     uint32_t qmask;
     uint32_t n_client_waves, wg_waves;
-    uint32_t variant, atr_entries, nmb, stamp_in_lds;
-    uint32_t lds_mb_off, lds_atr_off, lds_stamp_off;  // word offsets in the server LDS
+    uint32_t variant, atr_entries;
+    uint32_t stage_ahead;    // receivers stage request n once request n - stage_ahead is taken
     uint64_t timeout_ticks;
     uint32_t max_attempts;
     // resident bank stream and its commit log
@@ -141,6 +142,20 @@ enum : uint32_t {
     kStatBatches,
     kStatFinalTs,
     kStatCount
+};
+
+// Timing counters (wall_clock64 ticks), the counterpart of the reference's per-thread
+// time_rate fields (bank/csmv_sh/API.cuh:121-135).
+enum : uint32_t {
+    kProfCommitIdle = 0,   // committer waiting for a staged request
+    kProfCommitBusy,       // committer deciding
+    kProfRecvWindow,       // receivers waiting for the committer to come close
+    kProfRecvRing,         // receivers waiting for requests
+    kProfRecvBusy,         // receivers staging
+    kProfClientReply,      // client waves waiting for replies (sum over waves)
+    kProfClientRound,      // client wave rounds (sum over waves)
+    kProfClientRounds,     // number of client rounds
+    kProfCount
 };
 
 // Bounded wait: every spin loop in the engine ends, either by its condition or by the
@@ -319,7 +334,8 @@ template <int R, int W> constexpr uint32_t mbox_rows() { return 1 + R + 2 * W; }
 // Returns the commit timestamp (0 = aborted).  On a device error *fail is set and the
 // caller must leave its loop.
 template <int R, int W>
-__device__ __forceinline__ uint32_t tx_commit(const Ctx &c, uint32_t cw, Tx<R, W> &tx, bool req, bool *fail)
+__device__ __forceinline__ uint32_t tx_commit(const Ctx &c, uint32_t cw, Tx<R, W> &tx, bool req, bool *fail,
+                                               uint64_t *wait_ticks)
 {
     const uint32_t lane = lane_id();
     constexpr uint32_t rows = mbox_rows<R, W>();
@@ -348,6 +364,7 @@ __device__ __forceinline__ uint32_t tx_commit(const Ctx &c, uint32_t cw, Tx<R, W
         if (ballot((v >> 32) != n + 1) == 0) break;
         if (!sp.tick(c, kErrReplyTimeout)) { *fail = true; return 0; }
     }
+    *wait_ticks += wall_clock64() - sp.t0;
     const uint32_t ts = req ? (uint32_t)v : 0u;
     if ((c.variant & kServerWriteback) == 0) {
         const uint64_t cm = ballot(ts != 0);

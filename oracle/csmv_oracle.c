@@ -602,3 +602,201 @@ int orc_bank_cpu_csmv(const orc_bank_spec *s, const uint32_t *op,
     free((void *)c.reply); free(c.mb); free(th); free(args);
     return 0;
 }
+
+/* ---- memcached-style cache ------------------------------------------------ */
+
+uint64_t orc_memcd_key(uint32_t rank)
+{
+    return orc_splitmix64(0x6D656D6364ull ^ ((uint64_t)rank << 1));
+}
+
+uint32_t orc_memcd_set_of(uint64_t key, uint32_t n_sets)
+{
+    return (uint32_t)(orc_splitmix64(key) & (uint64_t)(n_sets - 1));
+}
+
+uint32_t orc_memcd_value_word(uint64_t key, uint64_t k, uint32_t w)
+{
+    return (uint32_t)orc_splitmix64(key ^ (k * 8 + w));
+}
+
+/* GET/SET choice as the driver's rnd > prRead test (memcachedGPU/memcdSH.cu:335-343);
+ * key rank from the Zipf table over the keyspace. */
+int orc_memcd_gen(const orc_memcd_spec *s, const uint32_t *cdf, uint64_t *req_key,
+                  uint32_t *req_op)
+{
+    if (!s || !req_key || !req_op || s->n_keys == 0 || s->n_keys >= (1ull << 31) ||
+        s->pct_get > 100 || (s->dist == ORC_DIST_ZIPF && !cdf))
+        return -1;
+    const uint32_t get_thr = (uint32_t)((uint64_t)s->pct_get * 0xFFFFu / 100u);
+    for (uint64_t k = 0; k < s->n_reqs; ++k) {
+        uint64_t st = orc_txn_state(s->seed, k);
+        uint32_t rnd = orc_rand_r_fnc(&st) & 0xFFFFu;
+        req_op[k] = rnd < get_thr ? ORC_MC_GET : ORC_MC_SET;
+        uint32_t r = orc_rand_r_fnc(&st);
+        uint32_t rank = s->dist == ORC_DIST_ZIPF ? orc_zipf_rank(cdf, (uint32_t)s->n_keys, r)
+                                                 : (uint32_t)(r % s->n_keys);
+        req_key[k] = orc_memcd_key(rank);
+    }
+    return 0;
+}
+
+typedef struct {
+    uint32_t res;      /* bit0 hit, bit1 wrote, bits 4..7 way */
+    uint32_t vsum;
+    uint32_t nw;
+    uint32_t wi[12];   /* written items */
+    int32_t wv[12];
+} mc_out;
+
+static inline uint32_t mc_vsum_acc(uint32_t acc, uint32_t j, uint32_t v)
+{
+    return acc + v * (2u * j + 1u);
+}
+
+/* One request against a plain item array (field-major, see the header). */
+static void memcd_body(const orc_memcd_spec *s, const int32_t *it, uint64_t key, uint32_t op,
+                       uint64_t k, mc_out *o)
+{
+    const uint32_t ns = s->n_sets * s->n_ways;
+    const uint32_t set = orc_memcd_set_of(key, s->n_sets);
+    const uint32_t lo = (uint32_t)key, hi = (uint32_t)(key >> 32);
+    const uint32_t clock = (uint32_t)(k / s->batch + 1);
+#define F(f, slot) ((uint32_t)(f) * ns + (slot))
+    o->res = 0; o->vsum = 0; o->nw = 0;
+    if (op == ORC_MC_GET) {
+        for (uint32_t w = 0; w < s->n_ways; ++w) {
+            uint32_t slot = set * s->n_ways + w;
+            if (it[F(ORC_MC_STATE, slot)] != 1) continue;
+            if ((uint32_t)it[F(ORC_MC_KLO, slot)] != lo || (uint32_t)it[F(ORC_MC_KHI, slot)] != hi)
+                continue;
+            o->res = 1u | (w << 4);
+            uint32_t acc = 0;
+            for (uint32_t j = 0; j < 8; ++j)
+                acc = mc_vsum_acc(acc, j, (uint32_t)it[F(ORC_MC_V0 + j, slot)]);
+            o->vsum = acc;
+            if ((uint32_t)it[F(ORC_MC_TS, slot)] < clock) {
+                o->res |= 2u;
+                o->wi[0] = F(ORC_MC_TS, slot);
+                o->wv[0] = (int32_t)clock;
+                o->nw = 1;
+            }
+            return;
+        }
+        return;
+    }
+    /* SET: key present -> that way; else first invalid way; else least recent */
+    int32_t found = -1, invalid = -1, lru = 0;
+    uint32_t lru_ts = 0xFFFFFFFFu;
+    for (uint32_t w = 0; w < s->n_ways; ++w) {
+        uint32_t slot = set * s->n_ways + w;
+        int32_t stv = it[F(ORC_MC_STATE, slot)];
+        uint32_t klo = (uint32_t)it[F(ORC_MC_KLO, slot)], khi = (uint32_t)it[F(ORC_MC_KHI, slot)];
+        uint32_t ts = (uint32_t)it[F(ORC_MC_TS, slot)];
+        if (stv == 1 && klo == lo && khi == hi && found < 0) found = (int32_t)w;
+        if (stv != 1 && invalid < 0) invalid = (int32_t)w;
+        if (ts < lru_ts) { lru_ts = ts; lru = (int32_t)w; }
+    }
+    uint32_t w = found >= 0 ? (uint32_t)found : invalid >= 0 ? (uint32_t)invalid : (uint32_t)lru;
+    uint32_t slot = set * s->n_ways + w;
+    o->res = (found >= 0 ? 1u : 0u) | 2u | (w << 4);
+    o->wi[0] = F(ORC_MC_KLO, slot); o->wv[0] = (int32_t)lo;
+    o->wi[1] = F(ORC_MC_KHI, slot); o->wv[1] = (int32_t)hi;
+    for (uint32_t j = 0; j < 8; ++j) {
+        o->wi[2 + j] = F(ORC_MC_V0 + j, slot);
+        o->wv[2 + j] = (int32_t)orc_memcd_value_word(key, k, j);
+    }
+    o->wi[10] = F(ORC_MC_TS, slot); o->wv[10] = (int32_t)clock;
+    o->wi[11] = F(ORC_MC_STATE, slot); o->wv[11] = 1;
+    o->nw = 12;
+#undef F
+}
+
+int orc_memcd_serial(const orc_memcd_spec *s, const uint64_t *req_key,
+                     const uint32_t *req_op, const uint64_t *order, int32_t *items,
+                     uint32_t *log_ts, uint32_t *log_res, uint32_t *log_vsum)
+{
+    const uint64_t n_items = (uint64_t)ORC_MC_FIELDS * s->n_sets * s->n_ways;
+    memset(items, 0, sizeof(int32_t) * n_items);
+    uint32_t ts = 0;
+    for (uint64_t q = 0; q < s->n_reqs; ++q) {
+        uint64_t k = order ? order[q] : q;
+        mc_out o;
+        memcd_body(s, items, req_key[k], req_op[k], k, &o);
+        for (uint32_t i = 0; i < o.nw; ++i) items[o.wi[i]] = o.wv[i];
+        if (o.nw) ++ts;
+        if (log_ts) log_ts[k] = ts;
+        if (log_res) log_res[k] = o.res;
+        if (log_vsum) log_vsum[k] = o.vsum;
+    }
+    return 0;
+}
+
+int64_t orc_memcd_replay(const orc_memcd_spec *s, const uint64_t *req_key,
+                         const uint32_t *req_op, const uint32_t *log_ts,
+                         const uint32_t *log_res, const uint32_t *log_vsum,
+                         const int32_t *final, orc_memcd_report *rep)
+{
+    const uint64_t R = s->n_reqs;
+    const uint64_t n_items = (uint64_t)ORC_MC_FIELDS * s->n_sets * s->n_ways;
+    memset(rep, 0, sizeof(*rep));
+    rep->first_bad_req = -1;
+#define BAD(field, k) do { rep->field++; if (rep->first_bad_req < 0) rep->first_bad_req = (int64_t)(k); } while (0)
+    uint64_t U = 0;
+    for (uint64_t k = 0; k < R; ++k) if (log_res[k] & 2u) ++U;
+    rep->n_updates = U;
+    rep->n_readonly = R - U;
+    uint64_t *order = (uint64_t *)malloc(sizeof(uint64_t) * (U + 1));
+    uint64_t *cnt = (uint64_t *)calloc(U + 2, sizeof(uint64_t));
+    uint64_t *lst = (uint64_t *)malloc(sizeof(uint64_t) * (R - U + 1));
+    int32_t *it = (int32_t *)calloc(n_items, sizeof(int32_t));
+    if (!order || !cnt || !lst || !it) {
+        free(order); free(cnt); free(lst); free(it);
+        return -1;
+    }
+    for (uint64_t t = 0; t <= U; ++t) order[t] = UINT64_MAX;
+    for (uint64_t k = 0; k < R; ++k) {
+        uint32_t ts = log_ts[k];
+        if (log_res[k] & 2u) {
+            if (ts == 0 || ts > U || order[ts] != UINT64_MAX) BAD(bad_ts, k);
+            else order[ts] = k;
+        } else {
+            if (ts > U) BAD(bad_snapshot, k); else cnt[ts + 1]++;
+        }
+    }
+    for (uint64_t t = 1; t <= U + 1; ++t) cnt[t] += cnt[t - 1];
+    {
+        uint64_t *fill = (uint64_t *)malloc(sizeof(uint64_t) * (U + 2));
+        memcpy(fill, cnt, sizeof(uint64_t) * (U + 2));
+        for (uint64_t k = 0; k < R; ++k)
+            if (!(log_res[k] & 2u) && log_ts[k] <= U) lst[fill[log_ts[k]]++] = k;
+        free(fill);
+    }
+    for (uint64_t t = 0; t <= U; ++t) {
+        for (uint64_t q = cnt[t]; q < cnt[t + 1]; ++q) {
+            uint64_t k = lst[q];
+            mc_out o;
+            memcd_body(s, it, req_key[k], req_op[k], k, &o);
+            if (o.nw) BAD(bad_kind, k);
+            else if (o.res != log_res[k] || o.vsum != log_vsum[k]) BAD(bad_result, k);
+            if (req_op[k] == ORC_MC_GET) { if (o.res & 1u) rep->hits++; else rep->misses++; }
+        }
+        if (t == U) break;
+        uint64_t k = order[t + 1];
+        if (k == UINT64_MAX) continue;
+        mc_out o;
+        memcd_body(s, it, req_key[k], req_op[k], k, &o);
+        if (!o.nw) BAD(bad_kind, k);
+        else if (o.res != log_res[k] || o.vsum != log_vsum[k]) BAD(bad_result, k);
+        if (req_op[k] == ORC_MC_GET) { if (o.res & 1u) rep->hits++; else rep->misses++; }
+        else rep->sets++;
+        for (uint32_t i = 0; i < o.nw; ++i) it[o.wi[i]] = o.wv[i];
+    }
+    if (final)
+        for (uint64_t i = 0; i < n_items; ++i)
+            if (final[i] != it[i]) BAD(bad_final, i);
+#undef BAD
+    free(order); free(cnt); free(lst); free(it);
+    return (int64_t)(rep->bad_ts + rep->bad_snapshot + rep->bad_result + rep->bad_kind +
+                     rep->bad_final);
+}

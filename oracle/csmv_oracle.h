@@ -121,25 +121,60 @@ int orc_bank_cpu_csmv(const orc_bank_spec *s, const uint32_t *op,
                       const orc_cpu_cfg *cfg, int32_t *state,
                       uint32_t *log_ts, int32_t *log_v, orc_cpu_stats *st);
 
-/* ---- memcached-style cache stream (config "memcached KV") -------------- */
+/* ---- memcached-style cache (config "memcached KV") -------------------------
+ * Restates memcachedGPU/memcdSH.cu:299-448 with 64-bit keys and 32-byte values.
+ * The cache is n_sets x n_ways slots; slot s has 12 int32 items, stored field-major
+ * (item = field * n_slots + s): state, key low, key high, LRU timestamp, 8 value
+ * words.  Request k carries the LRU clock k / batch + 1.
+ *   GET: probe the ways for a valid matching key; on a hit read the value and, if
+ *        the slot's timestamp is older than the clock, write the clock (the only
+ *        write of a GET, so most GETs are read-only).
+ *   SET: read state/key/timestamp of every way, pick the slot holding the key,
+ *        else the first invalid way, else the least recently used (lowest way on a
+ *        tie), and write key, value, clock and VALID.
+ * Result word: bit 0 hit (GET) or key present (SET), bit 1 the request wrote,
+ * bits 4..7 the way. */
+enum { ORC_MC_GET = 0, ORC_MC_SET = 1 };
+enum { ORC_MC_STATE = 0, ORC_MC_KLO = 1, ORC_MC_KHI = 2, ORC_MC_TS = 3, ORC_MC_V0 = 4,
+       ORC_MC_FIELDS = 12 };
+
 typedef struct {
     uint64_t seed;
     uint64_t n_reqs;
-    uint64_t n_keys;       /* keyspace size */
+    uint64_t n_keys;       /* keyspace size (< 2^31) */
     uint32_t n_sets;       /* power of two */
-    uint32_t n_ways;       /* <= 8 */
+    uint32_t n_ways;       /* 1..8 */
     uint32_t pct_get;      /* percent GET, rest SET */
-    uint32_t batch;        /* requests per batch; the LRU clock is the batch index + 1 */
+    uint32_t batch;        /* requests per batch */
     uint32_t dist;         /* ORC_DIST_* over key ranks */
+    uint32_t reserved;
     double   theta;
 } orc_memcd_spec;
 
-/* req_key[k] (uint64 key), req_op[k] (0 GET, 1 SET). */
-int orc_memcd_gen(const orc_memcd_spec *s, const uint32_t *cdf,
-                  uint64_t *req_key, uint32_t *req_op);
-/* Value word w (0..7) a SET of request k stores. */
-uint32_t orc_memcd_value_word(uint64_t key, uint64_t k, uint32_t w);
+uint64_t orc_memcd_key(uint32_t rank);
 uint32_t orc_memcd_set_of(uint64_t key, uint32_t n_sets);
+uint32_t orc_memcd_value_word(uint64_t key, uint64_t k, uint32_t w);
+/* req_key[k], req_op[k]; cdf (over n_keys ranks) may be NULL for uniform keys. */
+int orc_memcd_gen(const orc_memcd_spec *s, const uint32_t *cdf, uint64_t *req_key,
+                  uint32_t *req_op);
+
+typedef struct {
+    uint64_t n_updates, n_readonly, hits, misses, sets;
+    uint64_t bad_ts, bad_snapshot, bad_result, bad_kind, bad_final;
+    int64_t first_bad_req;
+} orc_memcd_report;
+
+/* Serial execution in the given order (NULL = stream order): final items
+ * [12 * n_slots] and a log in the engine's format. */
+int orc_memcd_serial(const orc_memcd_spec *s, const uint64_t *req_key,
+                     const uint32_t *req_op, const uint64_t *order, int32_t *items,
+                     uint32_t *log_ts, uint32_t *log_res, uint32_t *log_vsum);
+/* Replay of an engine log (ts: commit ts of writers, snapshot of read-only
+ * requests); compares results and the final items (may be NULL). */
+int64_t orc_memcd_replay(const orc_memcd_spec *s, const uint64_t *req_key,
+                         const uint32_t *req_op, const uint32_t *log_ts,
+                         const uint32_t *log_res, const uint32_t *log_vsum,
+                         const int32_t *final, orc_memcd_report *rep);
 
 #ifdef __cplusplus
 }

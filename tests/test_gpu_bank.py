@@ -15,7 +15,7 @@ pytestmark = pytest.mark.gpu
 INIT = 1000
 
 
-def run_bank(n_items, bank, variant="stamp", depth=4, atr=4096, waves=8, wg_waves=4,
+def run_bank(n_items, bank, variant="stamp", depth=4, atr=4096, waves=8, wg_waves=8,
              check_stream=True, timeout_ms=4000):
     print(f"\n[case] n_items={n_items} bank={bank} variant={variant} depth={depth} atr={atr} "
           f"waves={waves}", flush=True)
@@ -84,3 +84,14 @@ def test_bank_plain_full():
     """Config 'bank plain' at full size: 2^20 transactions, 64 client waves, depth 4."""
     run_bank(4096, csmv.Bank(n_txns=1 << 20, seed=1), variant="stamp", waves=64,
              check_stream=False)
+
+
+def test_four_wave_server_workgroup():
+    """Two receivers instead of six: the staging pipeline depth changes, results may not."""
+    run_bank(4096, csmv.Bank(n_txns=1 << 15, seed=21), variant="stamp", waves=16, wg_waves=4)
+
+
+def test_hashed_stamps_zipf():
+    """More items than LDS stamp buckets: hashed stamps with exact HBM stamps behind."""
+    run_bank(1 << 20, csmv.Bank(n_txns=1 << 16, seed=8, n_reads=4, n_reads_ro=4, zipf=True),
+             variant="stamp", depth=8, waves=64)
