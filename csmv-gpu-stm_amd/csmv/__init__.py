@@ -51,7 +51,8 @@ class Config(C.Structure):
     _fields_ = [("n_items", C.c_uint32), ("depth", C.c_uint32), ("atr_entries", C.c_uint32),
                 ("client_waves", C.c_uint32), ("wg_waves", C.c_uint32), ("variant", C.c_uint32),
                 ("timeout_ms", C.c_uint32), ("max_attempts", C.c_uint32),
-                ("stage_ahead", C.c_uint32), ("lds_kib", C.c_uint32)]
+                ("stage_ahead", C.c_uint32), ("lds_kib", C.c_uint32),
+                ("n_serial_waves", C.c_uint32), ("begin_wait", C.c_uint32)]
 
 
 class BankSpec(C.Structure):
@@ -67,7 +68,8 @@ class Stats(C.Structure):
                 ("batches", C.c_uint64), ("final_ts", C.c_uint64), ("kernel_ms", C.c_double),
                 ("error", C.c_uint32), ("client_lanes", C.c_uint32),
                 ("prof_ns", C.c_double * 8), ("stamp_buckets", C.c_uint32),
-                ("stamp_exact", C.c_uint32)]
+                ("stamp_exact", C.c_uint32), ("aborts_unknown", C.c_uint64),
+                ("lds_bytes", C.c_uint32), ("reserved", C.c_uint32)]
     PROF = ("commit_idle", "commit_busy", "recv_window", "recv_ring", "recv_busy",
             "client_reply", "client_round", "client_rounds")
 
@@ -76,7 +78,7 @@ class Stats(C.Structure):
         d["prof"] = {k: self.prof_ns[i] for i, k in enumerate(self.PROF)}
         commits = d["commits_update"] + d["commits_ro"]
         aborts = (d["aborts_dataage"] + d["aborts_readwrite"] + d["aborts_prevalid"]
-                  + d["aborts_recordage"])
+                  + d["aborts_recordage"] + d["aborts_unknown"])
         d["commits"] = commits
         d["aborts"] = aborts
         d["abort_rate"] = aborts / (aborts + commits) if aborts + commits else 0.0
@@ -108,6 +110,7 @@ def lib() -> C.CDLL:
         "csmv_bank_generate": ([P, C.POINTER(BankSpec)], C.c_int),
         "csmv_bank_read_stream": ([P, P, P], C.c_int),
         "csmv_bank_run": ([P, C.POINTER(Stats)], C.c_int),
+        "csmv_bank_route": ([P, P, SZ], C.c_int),
         "csmv_bank_read_log": ([P, P, P, P], C.c_int),
         "csmv_atomic_rate": ([P, U32, U32, C.POINTER(C.c_double)], C.c_int),
     }
@@ -158,11 +161,12 @@ class Engine:
     def __init__(self, n_items: int, depth: int = 4, atr_entries: int = 4096,
                  client_waves: int = 64, wg_waves: int = 8, variant="stamp",
                  timeout_ms: int = 4000, max_attempts: int = 0, stage_ahead: int = 0,
-                 lds_kib: int = 0):
+                 lds_kib: int = 0, n_serial_waves: int = 0, begin_wait: bool = True):
         if isinstance(variant, str):
             variant = VARIANTS[variant]
         self.cfg = Config(n_items, depth, atr_entries, client_waves, wg_waves, variant,
-                          timeout_ms, max_attempts, stage_ahead, lds_kib)
+                          timeout_ms, max_attempts, stage_ahead, lds_kib, n_serial_waves,
+                          1 if begin_wait else 0)
         self.n_items = n_items
         self._h = C.c_void_p()
         st = lib().csmv_create(C.byref(self.cfg), C.byref(self._h))
@@ -216,6 +220,14 @@ class Engine:
         addr = np.empty(b.rmax * b.n_txns, dtype=np.uint32)
         self._chk(lib().csmv_bank_read_stream(self._h, _ptr(op), _ptr(addr)), "read_stream")
         return op, addr
+
+    def bank_route(self, hot: Optional[np.ndarray]):
+        """Sends transfers with hot[k] set to the serialized lane (None: no routing)."""
+        if hot is None:
+            self._chk(lib().csmv_bank_route(self._h, None, 0), "bank_route")
+            return
+        h = np.ascontiguousarray(hot, dtype=np.uint8)
+        self._chk(lib().csmv_bank_route(self._h, _ptr(h), h.size), "bank_route")
 
     def bank_run(self) -> dict:
         st = Stats()

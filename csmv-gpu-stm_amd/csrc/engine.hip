@@ -96,14 +96,16 @@ __global__ void __launch_bounds__(256) bank_generate_kernel(GenArgs g)
     }
 }
 
-__global__ void __launch_bounds__(256) pool_fill_kernel(uint32_t *ht, uint64_t *slot,
+__global__ void __launch_bounds__(256) pool_fill_kernel(uint32_t *ht, uint64_t *slot, int32_t *vals,
                                                         uint32_t n, uint32_t depth,
                                                         const int32_t *values, int32_t fill)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     ht[i] = 0;
-    slot[i] = (uint32_t)(values ? values[i] : fill);
+    const int32_t v = values ? values[i] : fill;
+    slot[i] = (uint32_t)v;
+    vals[i] = v;
     for (uint32_t d = 1; d < depth; ++d) slot[(size_t)d * n + i] = 0;
 }
 
@@ -124,9 +126,10 @@ template <int R, int W> __device__ __forceinline__ void bank_client(const Ctx &c
 {
     const uint32_t lane = lane_id();
     const uint64_t T = c.n_txns;
+    const uint64_t NT = c.n_norm;
     bool has = false, exhausted = false;
     uint64_t k = 0;
-    uint32_t op = 0, ab = 0;
+    uint32_t op = 0, ab = 0, my_last = 0;
     uint64_t n_upd = 0, n_ro = 0, n_age = 0, t_reply = 0, t_round = 0, rounds = 0;
     for (;;) {
         if (ld_rlx(c.err)) break;
@@ -138,12 +141,12 @@ template <int R, int W> __device__ __forceinline__ void bank_client(const Ctx &c
                 uint64_t base = 0;
                 if (lane == 0) base = fetch_add(c.next_txn, (uint64_t)cnt);
                 base = bcast64(base);
-                if (base + cnt >= T) exhausted = true;
+                if (base + cnt >= NT) exhausted = true;
                 if (!has) {
                     const uint64_t mine = base + prefix_count(need);
-                    if (mine < T) {
+                    if (mine < NT) {
                         has = true;
-                        k = mine;
+                        k = c.norm_list ? c.norm_list[mine] : mine;
                         op = c.op[k];
                         ab = 0;
                     }
@@ -153,7 +156,7 @@ template <int R, int W> __device__ __forceinline__ void bank_client(const Ctx &c
         if (ballot(has) == 0) break;
 
         Tx<R, W> tx;
-        tx_begin(c, tx);
+        if (!tx_begin(c, tx, c.begin_wait ? my_last : 0u)) break;
         const uint32_t kind = op & 0xFFu, amt = (op >> 8) & 0xFFu, nr = (op >> 16) & 0xFFu;
         const bool ro = has && kind == kKindAudit;
         const bool upd = has && kind == kKindTransfer;
@@ -197,6 +200,8 @@ template <int R, int W> __device__ __forceinline__ void bank_client(const Ctx &c
             bool fail = false;
             const uint32_t ts = tx_commit<R, W>(c, cw, tx, req, &fail, &t_reply);
             if (fail) break;
+            const uint64_t cm = ballot(ts != 0);
+            if (cm) my_last = readlane(ts, 63u - (uint32_t)__builtin_clzll(cm));
             if (req) {
                 if (ts) {
                     c.log_ts[k] = ts;
@@ -228,6 +233,90 @@ template <int R, int W> __device__ __forceinline__ void bank_client(const Ctx &c
                                           __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Serialized lane: the wave hands routed transfers to the commit server, which executes
+// them in commit order on the newest committed values (no snapshot, so no conflict
+// window); the wave installs the versions the server returns.
+template <int R, int W> __device__ __forceinline__ void bank_serial_client(const Ctx &c, uint32_t cw)
+{
+    const uint32_t lane = lane_id();
+    const uint64_t NH = c.n_hot;
+    bool has = false, exhausted = false;
+    uint64_t k = 0;
+    uint32_t op = 0, ab = 0;
+    uint64_t n_upd = 0, t_reply = 0, t_round = 0, rounds = 0;
+    for (;;) {
+        if (ld_rlx(c.err)) break;
+        const uint64_t tr0 = wall_clock64();
+        if (!exhausted) {
+            const uint64_t need = ballot(!has);
+            if (need) {
+                const uint32_t cnt = (uint32_t)__builtin_popcountll(need);
+                uint64_t base = 0;
+                if (lane == 0) base = fetch_add(c.next_hot, (uint64_t)cnt);
+                base = bcast64(base);
+                if (base + cnt >= NH) exhausted = true;
+                if (!has) {
+                    const uint64_t mine = base + prefix_count(need);
+                    if (mine < NH) {
+                        has = true;
+                        k = c.hot_list[mine];
+                        op = c.op[k];
+                        ab = 0;
+                    }
+                }
+            }
+        }
+        if (ballot(has) == 0) break;
+        Tx<R, W> tx;
+        tx.snap = has ? 0u : kNone;
+        tx.nr = 0;
+        tx.nw = 0;
+        tx.aborted = false;
+        tx.overflow = false;
+#pragma unroll
+        for (int r = 0; r < R; ++r) tx.rs[r] = kNone;
+#pragma unroll
+        for (int w = 0; w < W; ++w) { tx.wa[w] = kNone; tx.wv[w] = 0; }
+        const uint32_t amt = (op >> 8) & 0xFFu, nr = (op >> 16) & 0xFFu;
+        if (has) {
+#pragma unroll
+            for (int j = 0; j < R; ++j)
+                if ((uint32_t)j < nr) {
+                    const uint32_t it = c.addr[(uint64_t)j * c.n_txns + k];
+                    tx.rs[j] = it;
+                    if (j < 2) tx_write(tx, it, 0);
+                }
+            tx.nr = nr;
+            tx.wv[0] = (int32_t)(uint32_t)k;
+            tx.wv[1] = (int32_t)amt;
+        }
+        bool fail = false;
+        const uint32_t ts = tx_commit<R, W>(c, cw, tx, has, &fail, &t_reply, true);
+        if (fail) break;
+        if (has) {
+            if (ts) {
+                c.log_ab[k] = ab;
+                has = false;
+                ++n_upd;
+            } else {
+                ++ab;
+            }
+        }
+        if (has && ab > c.max_attempts) raise_error(c, kErrLivelock);
+        t_round += wall_clock64() - tr0;
+        ++rounds;
+    }
+    if (lane == 0) {
+        fetch_add(&c.prof[kProfClientReply], t_reply);
+        fetch_add(&c.prof[kProfClientRound], t_round);
+        fetch_add(&c.prof[kProfClientRounds], rounds);
+    }
+    if (n_upd) fetch_add(&c.stats[kStatCommitUpd], n_upd);
+    fence_rel();
+    if (lane == 0) __hip_atomic_fetch_add(c.clients_done, 1u, __ATOMIC_RELEASE,
+                                          __HIP_MEMORY_SCOPE_AGENT);
+}
+
 template <int R, int W>
 __global__ void __launch_bounds__(512) bank_kernel(Ctx c, ServerLayout L)
 {
@@ -237,7 +326,9 @@ __global__ void __launch_bounds__(512) bank_kernel(Ctx c, ServerLayout L)
         return;
     }
     const uint32_t cw = (blockIdx.x - 1) * c.wg_waves + threadIdx.x / kWave;
-    if (cw < c.n_client_waves) bank_client<R, W>(c, cw);
+    if (cw >= c.n_client_waves) return;
+    if (cw >= c.n_client_waves - c.n_serial_waves) bank_serial_client<R, W>(c, cw);
+    else bank_client<R, W>(c, cw);
 }
 
 // Same-address agent-scope atomic rate: lane 0 of every wave adds to one word.
@@ -272,6 +363,12 @@ struct csmv_engine {
     uint8_t *ctl = nullptr;
     uint64_t *ring = nullptr, *reply = nullptr, *done = nullptr;
     uint32_t *mbox = nullptr, *stamps = nullptr;
+    int32_t *vals = nullptr;
+    uint64_t *reply_v = nullptr;
+    // routing of the resident stream (empty = everything on the normal lane)
+    uint32_t *norm_list = nullptr, *hot_list = nullptr;
+    uint64_t n_norm = 0, n_hot = 0;
+    bool routed = false;
     // bank stream + log
     csmv_bank_spec spec{};
     bool has_stream = false;
@@ -288,7 +385,7 @@ namespace {
 
 constexpr size_t kCtlBytes = 4096;
 constexpr size_t kOffGc = 0, kOffErr = 128, kOffDone = 256, kOffNext = 384, kOffTail = 512,
-                 kOffStats = 640, kOffProf = 1024;
+                 kOffStats = 640, kOffProf = 1024, kOffNextHot = 1536;
 
 csmv_status fail(csmv_engine *e, csmv_status s, const std::string &msg)
 {
@@ -339,31 +436,44 @@ uint32_t pow2_floor(uint64_t v)
 template <int R, int W> ServerLayout server_layout(const csmv_engine *e)
 {
     ServerLayout L{};
-    const bool swb = (e->cfg.variant & kServerWriteback) != 0;
     const uint32_t vmode = e->cfg.variant & 3u;
     const uint32_t M = e->cfg.wg_waves - 2;
-    L.nmb = std::min<uint32_t>(std::max<uint32_t>(2 * M, 4), 16);
-    L.mb_rows = swb ? Rows<R, W>::count_server_wb : Rows<R, W>::count_client_wb;
+    L.nmb = std::min<uint32_t>(std::max<uint32_t>(M + 2, 4), 16);
+    L.mb_rows = Rows<R, W>::count;
     L.state_off = kLdsHdr;
     L.wave_off = L.state_off + L.nmb;
     L.tg_off = L.wave_off + L.nmb;
     L.mb_off = (L.tg_off + L.nmb + 63) & ~63u;
     uint32_t w = L.mb_off + L.nmb * L.mb_rows * kWave;
-    L.hsz = pow2_at_least(2ull * kWave * W);
+    // every item a request inserts (W writes, plus R reads when delegated) fits
+    L.hsz = pow2_at_least((uint64_t)kWave * (R + W));
     L.hash_off = w;
-    w += M * 3 * L.hsz;
+    w += M * 5 * L.hsz;
+    L.sup_off = w;
+    w += kWave * W;
     L.atr_off = w;
     if (vmode != kValStamp) w += W * e->cfg.atr_entries;
     L.bucket_off = w;
+    L.val_off = w;
     L.nbuckets = 0;
+    L.nrc = 0;
     L.exact = 0;
     if (vmode == kValStamp) {
-        const uint64_t budget_words = e->lds_max / 4;
-        const uint64_t room = budget_words > w ? budget_words - w : 0;
-        const uint32_t want = pow2_at_least(e->cfg.n_items);
-        L.nbuckets = std::min<uint32_t>(want, pow2_floor(room));
-        L.exact = L.nbuckets >= e->cfg.n_items ? 1u : 0u;
-        w += L.nbuckets;
+        const uint64_t budget = e->lds_max / 4;
+        const uint64_t room = budget > w ? budget - w : 0;
+        if (2ull * pow2_at_least(e->cfg.n_items) <= room) {
+            // exact: stamp and value of every item
+            L.exact = 1;
+            L.nbuckets = pow2_at_least(e->cfg.n_items);
+            L.val_off = L.bucket_off + L.nbuckets;
+            w += L.nbuckets + e->cfg.n_items;
+        } else {
+            // hashed stamps + recent-value cache (key, value, stamp)
+            L.nrc = std::min<uint32_t>(pow2_floor(room / 12), 4096);
+            L.nbuckets = pow2_floor(room - 3ull * L.nrc);
+            L.val_off = L.bucket_off + L.nbuckets;
+            w += L.nbuckets + 3 * L.nrc;
+        }
     }
     L.words = w;
     return L;
@@ -420,6 +530,7 @@ csmv_status csmv_create(const csmv_config *cfg_in, csmv_engine **out)
     if (cfg.max_attempts == 0) cfg.max_attempts = 1u << 20;
     if (cfg.wg_waves == 0) cfg.wg_waves = 8;
     if (cfg.stage_ahead == 0) cfg.stage_ahead = 3;
+    if (cfg.n_serial_waves >= cfg.client_waves && cfg.n_serial_waves) return CSMV_E_ARG;
     if (cfg.n_items == 0 || cfg.depth == 0 || cfg.depth > 64 || cfg.client_waves == 0 ||
         cfg.wg_waves < 3 || cfg.wg_waves > 8 || cfg.stage_ahead > 64 || cfg.atr_entries < 64 ||
         (cfg.atr_entries & (cfg.atr_entries - 1)) || cfg.atr_entries > (1u << 16) ||
@@ -462,7 +573,9 @@ csmv_status csmv_create(const csmv_config *cfg_in, csmv_engine **out)
         (s = dalloc(e, &e->done, e->Q)) ||
         (s = dalloc(e, &e->reply, (size_t)cfg.client_waves * kWave)) ||
         (s = dalloc(e, &e->mbox, (size_t)cfg.client_waves * rows * kWave)) ||
-        (s = dalloc(e, &e->stamps, cfg.n_items)) || (s = dalloc(e, &e->tmp, cfg.n_items))) {
+        (s = dalloc(e, &e->stamps, cfg.n_items)) || (s = dalloc(e, &e->tmp, cfg.n_items)) ||
+        (s = dalloc(e, &e->vals, cfg.n_items)) ||
+        (s = dalloc(e, &e->reply_v, (size_t)cfg.client_waves * kWave * 2 * kBankW))) {
         csmv_destroy(e);
         return s;
     }
@@ -475,7 +588,8 @@ csmv_status csmv_destroy(csmv_engine *e)
     if (!e) return CSMV_E_ARG;
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     dfree(e->ht); dfree(e->slot); dfree(e->ctl); dfree(e->ring); dfree(e->done);
-    dfree(e->reply); dfree(e->mbox); dfree(e->stamps); dfree(e->tmp);
+    dfree(e->reply); dfree(e->mbox); dfree(e->stamps); dfree(e->tmp); dfree(e->vals);
+    dfree(e->reply_v); dfree(e->norm_list); dfree(e->hot_list);
     dfree(e->op); dfree(e->addr); dfree(e->cdf); dfree(e->log_ts); dfree(e->log_v);
     dfree(e->log_ab);
     if (e->ev0) (void)hipEventDestroy(e->ev0);
@@ -489,7 +603,7 @@ static csmv_status fill_pool(csmv_engine *e, const int32_t *dvals, int32_t v)
 {
     const uint32_t n = e->cfg.n_items;
     hipLaunchKernelGGL(pool_fill_kernel, dim3((n + 255) / 256), dim3(256), 0, e->stream, e->ht,
-                       e->slot, n, e->cfg.depth, dvals, v);
+                       e->slot, e->vals, n, e->cfg.depth, dvals, v);
     HIPCHK(e, hipGetLastError());
     HIPCHK(e, hipStreamSynchronize(e->stream));
     e->pool_loaded = true;
@@ -572,6 +686,48 @@ csmv_status csmv_bank_generate(csmv_engine *e, const csmv_bank_spec *s)
     HIPCHK(e, hipStreamSynchronize(e->stream));
     e->spec = *s;
     e->has_stream = true;
+    e->routed = false;
+    return CSMV_OK;
+}
+
+csmv_status csmv_bank_route(csmv_engine *e, const uint8_t *hot, size_t n)
+{
+    if (!e) return CSMV_E_ARG;
+    if (!e->has_stream) return fail(e, CSMV_E_STATE, "route before generate");
+    if (!hot) {
+        e->routed = false;
+        return CSMV_OK;
+    }
+    const uint64_t T = e->spec.n_txns;
+    if (n != T) return fail(e, CSMV_E_ARG, "route: one flag per transaction");
+    uint32_t *op = new (std::nothrow) uint32_t[T];
+    uint32_t *lists = new (std::nothrow) uint32_t[T];
+    if (!op || !lists) { delete[] op; delete[] lists; return fail(e, CSMV_E_CAPACITY, "route buffers"); }
+    hipError_t r = hipMemcpy(op, e->op, T * 4, hipMemcpyDeviceToHost);
+    if (r != hipSuccess) { delete[] op; delete[] lists; HIPCHK(e, r); }
+    // normal lane first (stream order), then the serialized lane (stream order);
+    // audits are read-only and always stay on the normal lane
+    uint64_t nn = 0;
+    for (uint64_t k = 0; k < T; ++k)
+        if (!(hot[k] && (op[k] & 0xFFu) == kKindTransfer)) lists[nn++] = (uint32_t)k;
+    uint64_t nh = 0;
+    for (uint64_t k = 0; k < T; ++k)
+        if (hot[k] && (op[k] & 0xFFu) == kKindTransfer) lists[nn + nh++] = (uint32_t)k;
+    delete[] op;
+    dfree(e->norm_list);
+    dfree(e->hot_list);
+    csmv_status st;
+    if ((st = dalloc(e, &e->norm_list, nn)) || (st = dalloc(e, &e->hot_list, nh))) {
+        delete[] lists;
+        return st;
+    }
+    if (nn) r = hipMemcpy(e->norm_list, lists, nn * 4, hipMemcpyHostToDevice);
+    if (r == hipSuccess && nh) r = hipMemcpy(e->hot_list, lists + nn, nh * 4, hipMemcpyHostToDevice);
+    delete[] lists;
+    HIPCHK(e, r);
+    e->n_norm = nn;
+    e->n_hot = nh;
+    e->routed = true;
     return CSMV_OK;
 }
 
@@ -625,6 +781,8 @@ csmv_status csmv_bank_run(csmv_engine *e, csmv_stats *out)
     c.done = e->done;
     c.mbox = e->mbox;
     c.stamps = e->stamps;
+    c.vals = e->vals;
+    c.reply_v = e->reply_v;
     c.prof = (uint64_t *)(e->ctl + kOffProf);
     c.qmask = e->Q - 1;
     c.n_client_waves = e->cfg.client_waves;
@@ -632,6 +790,8 @@ csmv_status csmv_bank_run(csmv_engine *e, csmv_stats *out)
     c.variant = e->cfg.variant;
     c.atr_entries = e->cfg.atr_entries;
     c.stage_ahead = e->cfg.stage_ahead;
+    c.n_serial_waves = e->cfg.n_serial_waves;
+    c.begin_wait = e->cfg.begin_wait;
     c.timeout_ticks = (uint64_t)e->cfg.timeout_ms * e->wallclock_khz;
     c.max_attempts = e->cfg.max_attempts;
     c.op = e->op;
@@ -640,11 +800,22 @@ csmv_status csmv_bank_run(csmv_engine *e, csmv_stats *out)
     c.log_ts = e->log_ts;
     c.log_v = e->log_v;
     c.log_ab = e->log_ab;
+    c.norm_list = e->routed ? e->norm_list : nullptr;
+    c.hot_list = e->routed ? e->hot_list : nullptr;
+    c.n_norm = e->routed ? e->n_norm : e->spec.n_txns;
+    c.n_hot = e->routed ? e->n_hot : 0;
+    c.next_hot = (uint64_t *)(e->ctl + kOffNextHot);
+    if (c.n_hot && (c.n_serial_waves == 0 || (e->cfg.variant & 3u) != kValStamp ||
+                    (e->cfg.variant & kServerWriteback)))
+        return fail(e, CSMV_E_ARG, "the serialized lane needs n_serial_waves > 0 and the "
+                                   "STAMP variant with client write-back");
 
     HIPCHK(e, hipMemsetAsync(e->ctl, 0, kCtlBytes, e->stream));
     HIPCHK(e, hipMemsetAsync(e->ring, 0, e->Q * 8, e->stream));
     HIPCHK(e, hipMemsetAsync(e->done, 0, e->Q * 8, e->stream));
     HIPCHK(e, hipMemsetAsync(e->reply, 0, (size_t)e->cfg.client_waves * kWave * 8, e->stream));
+    HIPCHK(e, hipMemsetAsync(e->reply_v, 0, (size_t)e->cfg.client_waves * kWave * 2 * kBankW * 8,
+                             e->stream));
     if ((e->cfg.variant & 3u) == kValStamp && !L.exact)
         HIPCHK(e, hipMemsetAsync(e->stamps, 0, (size_t)e->cfg.n_items * 4, e->stream));
     HIPCHK(e, hipStreamSynchronize(e->stream));
@@ -682,6 +853,8 @@ csmv_status csmv_bank_run(csmv_engine *e, csmv_stats *out)
         out->prof_ns[7] = (double)pf[kProfClientRounds];
         out->stamp_buckets = L.nbuckets;
         out->stamp_exact = L.exact;
+        out->aborts_unknown = st[kStatAbortUnknown];
+        out->lds_bytes = (uint32_t)lds_bytes;
     }
     if (derr == kErrLivelock) return fail(e, CSMV_E_LIVELOCK, "transaction attempt limit");
     if (derr == kErrReadSetOverflow) return fail(e, CSMV_E_CAPACITY, "read/write set overflow");

@@ -55,27 +55,36 @@ struct ServerLayout {
     uint32_t mb_rows;      // rows of 64 words per LDS mailbox
     uint32_t mb_off;       // mailboxes
     uint32_t state_off;    // mailbox state words [nmb]
-    uint32_t wave_off;     // client wave per mailbox [nmb]
+    uint32_t wave_off;     // client wave (| kDelegateBit) per mailbox [nmb]
     uint32_t tg_off;       // T_g per mailbox [nmb]
     uint32_t bucket_off;   // commit-stamp buckets [nbuckets]
     uint32_t nbuckets;     // power of two (STAMP mode)
     uint32_t exact;        // bucket i is item i (n_items <= nbuckets)
+    uint32_t val_off;      // exact: value of every item [n_items]; hashed: recent-value
+                           // cache keys [nrc], values [nrc], stamps [nrc]
+    uint32_t sup_off;      // serialized lane: superseded-write flags [64 * W]
+    uint32_t nrc;          // recent-value cache entries (power of two, hashed mode)
     uint32_t atr_off;      // record ring [W][E] (SCAN / COLLAB)
-    uint32_t hash_off;     // per-receiver hash tables [M][3][hsz]
-    uint32_t hsz;          // power of two >= 2 * 64 * W
+    uint32_t hash_off;     // per-receiver hash tables [M][5][hsz]
+    uint32_t hsz;          // power of two >= 64 * (R + W)
     uint32_t words;        // total
 };
 
-// row indices inside an LDS mailbox
+// Row indices inside an LDS mailbox.  For a delegated request the wv rows carry the
+// transaction id and amount, prod packs each read's producer (the lower lane whose
+// write it must see: lane | slot << 6, 0xFF = none) and pval holds committed values
+// gathered at staging.
 template <int R, int W> struct Rows {
     static constexpr uint32_t snap = 0;
     static constexpr uint32_t items = 1;              // R reads then W writes
-    static constexpr uint32_t conf_lo = 1 + R + W;
-    static constexpr uint32_t conf_hi = 2 + R + W;
-    static constexpr uint32_t pref = 3 + R + W;
-    static constexpr uint32_t wv = 4 + R + W;         // server write-back only
-    static constexpr uint32_t count_client_wb = 4 + R + W;
-    static constexpr uint32_t count_server_wb = 4 + R + 2 * W;
+    static constexpr uint32_t wv = 1 + R + W;
+    static constexpr uint32_t conf_lo = 1 + R + 2 * W;
+    static constexpr uint32_t conf_hi = 2 + R + 2 * W;
+    static constexpr uint32_t pref = 3 + R + 2 * W;
+    static constexpr uint32_t prod = 4 + R + 2 * W;   // (R + 3) / 4 rows
+    static constexpr uint32_t pval = prod + (R + 3) / 4;
+    static constexpr uint32_t pstamp = pval + R;     // stamps of the writes at staging
+    static constexpr uint32_t count = pstamp + W;
 };
 
 __device__ __forceinline__ uint32_t lds_ld_acq(const uint32_t *p)
@@ -125,6 +134,108 @@ __device__ __forceinline__ bool hits(uint32_t x, const uint32_t *rs, const uint3
 }
 
 // ---- committer --------------------------------------------------------------------------
+// Committed-value tables: the committer knows the newest committed value of an item
+// when it fits the exact LDS table, when the recent-value cache still holds it, or when
+// the value gathered at staging is still current (no write to its bucket since T_g).
+struct ValueTables {
+    uint32_t *bucket, *lval, *rc_key, *rc_val, *rc_ts;
+};
+
+__device__ __forceinline__ uint32_t rc_slot(const ServerLayout &L, uint32_t x)
+{
+    return mix32(x ^ 0x9E3779B9u) & (L.nrc - 1);
+}
+
+// newest committed value and stamp of item x for commits decided before the current
+// request; false when neither table has it (hashed mode only)
+__device__ __forceinline__ bool committed_of(const ServerLayout &L, const ValueTables &V, uint32_t x,
+                                             uint32_t tg, int32_t pv, uint32_t ps, int32_t &val,
+                                             uint32_t &ts)
+{
+    if (L.exact) {
+        val = (int32_t)V.lval[x];
+        ts = V.bucket[x];
+        return true;
+    }
+    const uint32_t h = rc_slot(L, x);
+    if (V.rc_key[h] == x) {
+        val = (int32_t)V.rc_val[h];
+        ts = V.rc_ts[h];
+        return true;
+    }
+    if (V.bucket[bucket_of(L, x)] <= tg) {  // no write to x since the request was staged
+        val = pv;
+        ts = ps;
+        return true;
+    }
+    return false;
+}
+
+template <int R, int W>
+__device__ __forceinline__ void record_values(const Ctx &c, const ServerLayout &L, const ValueTables &V,
+                                              const uint32_t *wa, const int32_t *v, uint32_t ts,
+                                              const bool *on)
+{
+#pragma unroll
+    for (int w = 0; w < W; ++w)
+        if (on[w] && wa[w] != kNone) {
+            if (L.exact) {
+                V.lval[wa[w]] = (uint32_t)v[w];
+            } else {
+                const uint32_t h = rc_slot(L, wa[w]);
+                V.rc_key[h] = wa[w];
+                V.rc_val[h] = (uint32_t)v[w];
+                V.rc_ts[h] = ts;
+                c.vals[wa[w]] = v[w];
+            }
+        }
+}
+
+template <int R, int W>
+__device__ __forceinline__ void record_stamps(const Ctx &c, const ServerLayout &L, const ValueTables &V,
+                                              const uint32_t *wa, uint32_t ts, bool on)
+{
+#pragma unroll
+    for (int w = 0; w < W; ++w)
+        if (on && wa[w] != kNone) {
+            __hip_atomic_fetch_max(&V.bucket[bucket_of(L, wa[w])], ts, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (!L.exact)
+                __hip_atomic_fetch_max(&c.stamps[wa[w]], ts, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+}
+
+// The bank transfer body executed by the serialized lane on the values the items held
+// before this transaction (in[j]); read-own-write as bank/csmv_sh/API.cu:9-18.
+template <int R, int W>
+__device__ __forceinline__ void transfer_exec(const uint32_t *it, uint32_t amt, const int32_t *in,
+                                              int32_t *rd, const uint32_t *wa, int32_t *nv)
+{
+    uint32_t wi[2] = {kNone, kNone};
+    int32_t wvv[2] = {0, 0};
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        if (it[j] == kNone) { rd[j] = 0; continue; }
+        int32_t v = in[j];
+        v = (wi[0] == it[j]) ? wvv[0] : v;
+        v = (wi[1] == it[j]) ? wvv[1] : v;
+        rd[j] = v;
+        if (j == 0) { wi[0] = it[0]; wvv[0] = (int32_t)((uint32_t)v - amt); }
+        if (j == 1) {
+            const int32_t nvb = (int32_t)((uint32_t)v + amt);
+            if (wi[0] == it[1]) wvv[0] = nvb;
+            else { wi[1] = it[1]; wvv[1] = nvb; }
+        }
+    }
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        nv[w] = 0;
+        if (wa[w] == wi[0]) nv[w] = wvv[0];
+        if (w < 2 && wa[w] == wi[1] && wa[w] != kNone) nv[w] = wvv[1];
+    }
+}
+
 template <int R, int W>
 __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, uint32_t *lds)
 {
@@ -137,13 +248,19 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
     uint32_t *mbst = lds + L.state_off;
     uint32_t *mbwave = lds + L.wave_off;
     uint32_t *mbtg = lds + L.tg_off;
-    uint32_t *bucket = lds + L.bucket_off;
     uint32_t *atr = lds + L.atr_off;
+    ValueTables V;
+    V.bucket = lds + L.bucket_off;
+    V.lval = lds + L.val_off;
+    V.rc_key = lds + L.val_off;
+    V.rc_val = lds + L.val_off + L.nrc;
+    V.rc_ts = lds + L.val_off + 2 * L.nrc;
+    uint32_t *sup = lds + L.sup_off;
     const uint32_t Q = c.qmask + 1;
     const uint64_t lt = (1ull << lane) - 1;
 
     uint32_t last = 0;  // newest assigned commit timestamp
-    uint64_t n_rw = 0, n_pv = 0, n_rec = 0, t_idle = 0, t_busy = 0;
+    uint64_t n_rw = 0, n_pv = 0, n_rec = 0, n_unk = 0, t_idle = 0, t_busy = 0;
     uint32_t n = 0;
     for (;; ++n) {
         const uint32_t mbi = n % L.nmb;
@@ -166,162 +283,275 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
             if (bad) break;
         }
         const uint64_t t1 = wall_clock64();
-        const uint32_t cw = mbwave[mbi];
+        const uint32_t cwr = mbwave[mbi];
+        const uint32_t cw = cwr & ~kDelegateBit;
+        const bool delegate = (cwr & kDelegateBit) != 0;
         const uint32_t tg = mbtg[mbi];
         const uint32_t *box = lds + L.mb_off + mbi * L.mb_rows * kWave;
         const uint32_t snap = box[RW::snap * kWave + lane];
-        uint32_t rs[R], wa[W];
-        int32_t wv[W];
+        uint32_t rs[R], wa[W], prod[(R + 3) / 4], pst[W];
+        int32_t wv[W], pval[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) rs[r] = box[(RW::items + r) * kWave + lane];
 #pragma unroll
         for (int w = 0; w < W; ++w) {
             wa[w] = box[(RW::items + R + w) * kWave + lane];
-            wv[w] = server_wb ? (int32_t)box[(RW::wv + w) * kWave + lane] : 0;
+            wv[w] = (int32_t)box[(RW::wv + w) * kWave + lane];
         }
         const uint64_t conf = ((uint64_t)box[RW::conf_hi * kWave + lane] << 32) |
                               box[RW::conf_lo * kWave + lane];
         const bool pref = box[RW::pref * kWave + lane] != 0;
+        if (delegate) {
+#pragma unroll
+            for (int q = 0; q < (R + 3) / 4; ++q) prod[q] = box[(RW::prod + q) * kWave + lane];
+#pragma unroll
+            for (int r = 0; r < R; ++r) pval[r] = (int32_t)box[(RW::pval + r) * kWave + lane];
+#pragma unroll
+            for (int w = 0; w < W; ++w) pst[w] = box[(RW::pstamp + w) * kWave + lane];
+        }
         // mailbox consumed; every stamp <= last is visible in HBM once this release
         // has waited for the previous batch's stores
         if (lane == 0) {
             lds_st_rel(&lds[kLdsStampTs], last);
             lds_st_rel(&lds[kLdsCnext], n + 1);
         }
-
         const bool valid = snap != kNone;
-        bool rec = false, conflict = false;
-        if (vmode == kValStamp) {
-            if (valid) {
-                const uint32_t thr = L.exact ? snap : tg;
-                conflict = pref;
+        uint32_t myts = 0;
+        uint32_t nc = 0;
+
+        if (delegate) {
+            // ---- serialized lane: the server executes the transactions ----------------
+            const uint32_t k = (uint32_t)wv[0], amt = (uint32_t)wv[1];
+            // values the reads see from before this request
+            int32_t ext[R];
+            bool unknown = false;
 #pragma unroll
-                for (int r = 0; r < R; ++r)
-                    if (rs[r] != kNone) conflict |= bucket[bucket_of(L, rs[r])] > thr;
-#pragma unroll
-                for (int w = 0; w < W; ++w)
-                    if (wa[w] != kNone) conflict |= bucket[bucket_of(L, wa[w])] > thr;
+            for (int r = 0; r < R; ++r) {
+                ext[r] = 0;
+                const uint32_t pj = (prod[r / 4] >> (8 * (r % 4))) & 0xFFu;
+                uint32_t dummy;
+                if (valid && rs[r] != kNone && pj == 0xFFu)
+                    unknown |= !committed_of(L, V, rs[r], tg, pval[r], 0u, ext[r], dummy);
             }
-        } else {
-            // record ring indexed by timestamp: entry t lives at t mod E and the ring
-            // covers (last - E, last]; an older snapshot cannot be validated
-            // ("RecordAge", bank/csmv_sh/API.cu:93-100).
-            rec = valid && (last - snap) > E;
-            const bool live = valid && !rec;
-            const uint64_t lm = ballot(live);
-            if (lm && vmode == kValScan) {
-                // TXValidate: every lane walks back from the newest record to its
-                // own snapshot (bank/csmv_sh/API.cu:104-122); lanes read the same
-                // record at the same step, so the LDS read is a broadcast.
-                const uint32_t lo = wave_min(live ? snap : kNone);
-                for (uint32_t t = last; t > lo; --t) {
-                    const bool chk = live && !conflict && t > snap;
-                    if (ballot(chk) == 0) break;
-                    const uint32_t e = t & emask;
-                    if (chk) {
+            // stamp of each written item before this request: the version a client
+            // installing the write must find at the head of the item's ring first
+            uint32_t prev[W];
 #pragma unroll
-                        for (int w = 0; w < W; ++w) {
-                            const uint32_t x = atr[w * E + e];
-                            if (x != kNone) conflict |= hits<R, W>(x, rs, wa);
-                        }
+            for (int w = 0; w < W; ++w) {
+                prev[w] = kNone;
+                int32_t dv;
+                if (valid && wa[w] != kNone)
+                    unknown |= !committed_of(L, V, wa[w], tg, 0, pst[w], dv, prev[w]);
+            }
+            // execute along the request's dependency order: a lane runs once every
+            // lower lane it conflicts with has run; values written inside the request
+            // reach their readers through ds_bpermute
+            uint64_t pending = ballot(valid);
+            bool okx = false, failed = !valid || unknown;
+            int32_t rd[R], nv[W];
+#pragma unroll
+            for (int r = 0; r < R; ++r) rd[r] = 0;
+#pragma unroll
+            for (int w = 0; w < W; ++w) nv[w] = 0;
+            for (uint32_t guard = 0; pending && guard < kWave; ++guard) {
+                const bool ready = ((pending >> lane) & 1ull) && (conf & pending) == 0;
+                const uint64_t rm = ballot(ready);
+                int32_t in[R];
+                bool dep_fail = false;
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const uint32_t pj = (prod[r / 4] >> (8 * (r % 4))) & 0xFFu;
+                    const int src = (int)(pj == 0xFFu ? lane : (pj & 63u));
+                    int32_t from = 0;
+#pragma unroll
+                    for (int w = 0; w < W; ++w) {
+                        const int32_t x = __shfl(nv[w], src, 64);
+                        from = ((pj >> 6) == (uint32_t)w) ? x : from;
+                    }
+                    const int okp = __shfl((int)okx, src, 64);
+                    in[r] = pj == 0xFFu ? ext[r] : from;
+                    dep_fail |= (pj != 0xFFu) && !okp;
+                }
+                if (ready) {
+                    if (!failed && !dep_fail) {
+                        transfer_exec<R, W>(rs, amt, in, rd, wa, nv);
+                        okx = true;
+                    } else {
+                        failed = true;
                     }
                 }
-            } else if (lm) {
-                // collaborative validation (collabValidation/csmv_sh/API.cu:212-264):
-                // the 64 lanes split one client lane's window, lane l taking records
-                // last - l, last - l - 64, ...; client lanes are served in lane order.
-                uint64_t it = lm;
-                while (it) {
-                    const uint32_t cl = (uint32_t)__builtin_ctzll(it);
-                    it &= it - 1;
-                    const uint32_t sc = readlane(snap, cl);
-                    uint32_t rsc[R], wac[W];
+                pending &= ~rm;
+            }
+            const uint64_t committed = ballot(okx);
+            n_unk += __builtin_popcountll(ballot(valid && !okx));
+            nc = (uint32_t)__builtin_popcountll(committed);
+            if (okx) myts = last + 1 + (uint32_t)__builtin_popcountll(committed & lt);
+            last += nc;
+            // A write is superseded when a higher committed lane of the request read the
+            // item from it (bank transfers read every item they write): only the last
+            // write of an item in the request is installed.  No snapshot can fall inside
+            // the request's timestamp range, since the clock moves over whole requests.
 #pragma unroll
-                    for (int r = 0; r < R; ++r) rsc[r] = readlane(rs[r], cl);
+            for (int w = 0; w < W; ++w) sup[lane * W + w] = 0;
+            wg_fence();
 #pragma unroll
-                    for (int w = 0; w < W; ++w) wac[w] = readlane(wa[w], cl);
-                    const uint32_t K = last - sc;
-                    bool cf = false;
-                    for (uint32_t base = 0; base < K; base += kWave) {
-                        const uint32_t d = base + lane;
-                        if (d < K) {
-                            const uint32_t e = (last - d) & emask;
+            for (int r = 0; r < R; ++r) {
+                const uint32_t pj = (prod[r / 4] >> (8 * (r % 4))) & 0xFFu;
+                bool mine_w = false;
+#pragma unroll
+                for (int w = 0; w < W; ++w) mine_w |= wa[w] == rs[r];
+                if (okx && pj != 0xFFu && mine_w) sup[(pj & 63u) * W + (pj >> 6)] = 1u;
+            }
+            wg_fence();
+            bool inst[W];
+#pragma unroll
+            for (int w = 0; w < W; ++w) inst[w] = okx && wa[w] != kNone && sup[lane * W + w] == 0;
+            record_values<R, W>(c, L, V, wa, nv, myts, inst);
+            record_stamps<R, W>(c, L, V, wa, myts, okx);
+            if (okx) {
+                c.log_ts[k] = myts;
+                const uint32_t nr = (c.op[k] >> 16) & 0xFFu;
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                    if ((uint32_t)r < nr) c.log_v[(uint64_t)r * c.n_txns + k] = rd[r];
+            }
+            uint64_t *rv = c.reply_v + ((size_t)cw * kWave + lane) * 2 * W;
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                st_rlx(&rv[w], ((uint64_t)(n + 1) << 32) | (uint32_t)nv[w]);
+                st_rlx(&rv[W + w], ((uint64_t)(n + 1) << 32) | (inst[w] ? prev[w] : kNone));
+            }
+        } else {
+            bool rec = false, conflict = false;
+            if (vmode == kValStamp) {
+                if (valid) {
+                    const uint32_t thr = L.exact ? snap : tg;
+                    conflict = pref;
+#pragma unroll
+                    for (int r = 0; r < R; ++r)
+                        if (rs[r] != kNone) conflict |= V.bucket[bucket_of(L, rs[r])] > thr;
+#pragma unroll
+                    for (int w = 0; w < W; ++w)
+                        if (wa[w] != kNone) conflict |= V.bucket[bucket_of(L, wa[w])] > thr;
+                }
+            } else {
+                // record ring indexed by timestamp: entry t lives at t mod E and the ring
+                // covers (last - E, last]; an older snapshot cannot be validated
+                // ("RecordAge", bank/csmv_sh/API.cu:93-100).
+                rec = valid && (last - snap) > E;
+                const bool live = valid && !rec;
+                const uint64_t lm = ballot(live);
+                if (lm && vmode == kValScan) {
+                    // TXValidate: every lane walks back from the newest record to its
+                    // own snapshot (bank/csmv_sh/API.cu:104-122); lanes read the same
+                    // record at the same step, so the LDS read is a broadcast.
+                    const uint32_t lo = wave_min(live ? snap : kNone);
+                    for (uint32_t t = last; t > lo; --t) {
+                        const bool chk = live && !conflict && t > snap;
+                        if (ballot(chk) == 0) break;
+                        const uint32_t e = t & emask;
+                        if (chk) {
 #pragma unroll
                             for (int w = 0; w < W; ++w) {
                                 const uint32_t x = atr[w * E + e];
-                                if (x != kNone) cf |= hits<R, W>(x, rsc, wac);
+                                if (x != kNone) conflict |= hits<R, W>(x, rs, wa);
                             }
                         }
-                        if (ballot(cf)) break;
                     }
-                    const bool any = ballot(cf) != 0;
-                    if (lane == cl) conflict = any;
+                } else if (lm) {
+                    // collaborative validation (collabValidation/csmv_sh/API.cu:212-264):
+                    // the 64 lanes split one client lane's window, lane l taking records
+                    // last - l, last - l - 64, ...; client lanes are served in lane order.
+                    uint64_t it = lm;
+                    while (it) {
+                        const uint32_t cl = (uint32_t)__builtin_ctzll(it);
+                        it &= it - 1;
+                        const uint32_t sc = readlane(snap, cl);
+                        uint32_t rsc[R], wac[W];
+#pragma unroll
+                        for (int r = 0; r < R; ++r) rsc[r] = readlane(rs[r], cl);
+#pragma unroll
+                        for (int w = 0; w < W; ++w) wac[w] = readlane(wa[w], cl);
+                        const uint32_t K = last - sc;
+                        bool cf = false;
+                        for (uint32_t base = 0; base < K; base += kWave) {
+                            const uint32_t d = base + lane;
+                            if (d < K) {
+                                const uint32_t e = (last - d) & emask;
+#pragma unroll
+                                for (int w = 0; w < W; ++w) {
+                                    const uint32_t x = atr[w * E + e];
+                                    if (x != kNone) cf |= hits<R, W>(x, rsc, wac);
+                                }
+                            }
+                            if (ballot(cf)) break;
+                        }
+                        const bool any = ballot(cf) != 0;
+                        if (lane == cl) conflict = any;
+                    }
                 }
             }
-        }
-        const bool ok = valid && !rec && !conflict;
-        n_rec += __builtin_popcountll(ballot(rec));
-        n_rw += __builtin_popcountll(ballot(valid && !rec && conflict));
+            const bool ok = valid && !rec && !conflict;
+            n_rec += __builtin_popcountll(ballot(rec));
+            n_rw += __builtin_popcountll(ballot(valid && !rec && conflict));
 
-        // Lane order inside the request: lane L commits only if no lower lane it
-        // conflicts with commits.  Lanes without such conflicts are final at once;
-        // the others are decided in ascending order.
-        const uint64_t m = ballot(ok);
-        uint64_t committed = m;
-        uint64_t it = ballot(ok && (conf & m) != 0);
-        committed &= ~it;
-        while (it) {
-            const uint32_t i = (uint32_t)__builtin_ctzll(it);
-            it &= it - 1;
-            if ((readlane64(conf, i) & committed) == 0) committed |= 1ull << i;
-        }
-        n_pv += __builtin_popcountll(m & ~committed);
-        const bool mine = (committed >> lane) & 1ull;
-        const uint32_t nc = (uint32_t)__builtin_popcountll(committed);
-        uint32_t myts = 0;
-        auto record = [&](uint32_t ts) {
-            if (vmode == kValStamp) {
-#pragma unroll
-                for (int w = 0; w < W; ++w)
-                    if (wa[w] != kNone) {
-                        __hip_atomic_fetch_max(&bucket[bucket_of(L, wa[w])], ts, __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_WORKGROUP);
-                        if (!L.exact) c.stamps[wa[w]] = ts;
-                    }
-            } else {
-#pragma unroll
-                for (int w = 0; w < W; ++w) atr[w * E + (ts & emask)] = wa[w];
+            // Lane order inside the request: lane L commits only if no lower lane it
+            // conflicts with commits.  Lanes without such conflicts are final at once;
+            // the others are decided in ascending order.
+            const uint64_t m = ballot(ok);
+            uint64_t committed = m;
+            uint64_t it = ballot(ok && (conf & m) != 0);
+            committed &= ~it;
+            while (it) {
+                const uint32_t i = (uint32_t)__builtin_ctzll(it);
+                it &= it - 1;
+                if ((readlane64(conf, i) & committed) == 0) committed |= 1ull << i;
             }
-        };
-        if (batched) {
-            // one batched insertion (batchedAtrInsert/csmv_sh/API.cu:296-377):
-            // consecutive timestamps in lane order, offset = popc(committed & lt)
-            if (mine) {
-                myts = last + 1 + (uint32_t)__builtin_popcountll(committed & lt);
-                record(myts);
-            }
-            last += nc;
-        } else {
-            // per-lane insertion (bank/csmv_sh/API.cu:219-265): one record and one
-            // head publication per committing lane, in lane order.
-            uint64_t jt = committed;
-            while (jt) {
-                const uint32_t i = (uint32_t)__builtin_ctzll(jt);
-                jt &= jt - 1;
-                ++last;
-                if (lane == i) {
-                    myts = last;
+            n_pv += __builtin_popcountll(m & ~committed);
+            const bool mine = (committed >> lane) & 1ull;
+            nc = (uint32_t)__builtin_popcountll(committed);
+            auto record = [&](uint32_t ts) {
+                if (vmode == kValStamp) {
+                    bool on[W];
+#pragma unroll
+                    for (int w = 0; w < W; ++w) on[w] = true;
+                    record_stamps<R, W>(c, L, V, wa, ts, true);
+                    record_values<R, W>(c, L, V, wa, wv, ts, on);
+                } else {
+#pragma unroll
+                    for (int w = 0; w < W; ++w) atr[w * E + (ts & emask)] = wa[w];
+                }
+            };
+            if (batched) {
+                // one batched insertion (batchedAtrInsert/csmv_sh/API.cu:296-377):
+                // consecutive timestamps in lane order, offset = popc(committed & lt)
+                if (mine) {
+                    myts = last + 1 + (uint32_t)__builtin_popcountll(committed & lt);
                     record(myts);
                 }
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                last += nc;
+            } else {
+                // per-lane insertion (bank/csmv_sh/API.cu:219-265): one record and one
+                // head publication per committing lane, in lane order.
+                uint64_t jt = committed;
+                while (jt) {
+                    const uint32_t i = (uint32_t)__builtin_ctzll(jt);
+                    jt &= jt - 1;
+                    ++last;
+                    if (lane == i) {
+                        myts = last;
+                        record(myts);
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                }
             }
-        }
-        if (server_wb && nc) {
-            // the server installs the versions and moves the clock itself
-            // (vanilla/csmv_sh/API.cu:269-272, clientWriteback/csmv_sh/API.cu:386-393)
-            writeback<R, W>(c, wa, wv, (uint32_t)W, myts);
-            fence_rel();
-            if (lane == 0) st_rlx(c.gc, last);
+            if (server_wb && nc) {
+                // the server installs the versions and moves the clock itself
+                // (vanilla/csmv_sh/API.cu:269-272, clientWriteback/csmv_sh/API.cu:386-393)
+                writeback<R, W>(c, wa, wv, (uint32_t)W, myts);
+                fence_rel();
+                if (lane == 0) st_rlx(c.gc, last);
+            }
         }
         st_rlx(&c.reply[(size_t)cw * kWave + lane], ((uint64_t)(n + 1) << 32) | myts);
         if (!server_wb && nc == 0 && lane == 0)
@@ -336,6 +566,7 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
         fetch_add(&c.stats[kStatAbortRw], n_rw);
         fetch_add(&c.stats[kStatAbortPrevalid], n_pv);
         fetch_add(&c.stats[kStatAbortRecord], n_rec);
+        fetch_add(&c.stats[kStatAbortUnknown], n_unk);
         fetch_add(&c.stats[kStatBatches], (uint64_t)n);
         st_rlx(&c.stats[kStatFinalTs], (uint64_t)last);
         fetch_add(&c.prof[kProfCommitIdle], t_idle);
@@ -373,22 +604,54 @@ __device__ __forceinline__ void advancer(const Ctx &c, uint32_t *lds)
 }
 
 // ---- receivers ------------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t hash_claim(uint32_t *hkey, uint32_t hmask, uint32_t x)
+{
+    uint32_t h = mix32(x) & hmask;
+    for (uint32_t p = 0; p <= hmask; ++p) {
+        const uint32_t old = atomicCAS(&hkey[h], kNone, x);
+        if (old == kNone || old == x) return h;
+        h = (h + 1) & hmask;
+    }
+    return kNone;
+}
+
+__device__ __forceinline__ uint32_t hash_find(const uint32_t *hkey, uint32_t hmask, uint32_t x)
+{
+    uint32_t h = mix32(x) & hmask;
+    for (uint32_t p = 0; p <= hmask; ++p) {
+        const uint32_t k = __hip_atomic_load(&hkey[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (k == x) return h;
+        if (k == kNone) return kNone;
+        h = (h + 1) & hmask;
+    }
+    return kNone;
+}
+
 template <int R, int W>
 __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, uint32_t *lds,
                                          uint32_t rid, uint32_t M)
 {
     using RW = Rows<R, W>;
-    const bool server_wb = (c.variant & kServerWriteback) != 0;
     const bool stamp_mode = (c.variant & 0x3u) == kValStamp;
     const uint32_t lane = lane_id();
     const uint64_t lt = (1ull << lane) - 1;
     uint32_t *mbst = lds + L.state_off;
     uint32_t *mbwave = lds + L.wave_off;
     uint32_t *mbtg = lds + L.tg_off;
-    uint32_t *hkey = lds + L.hash_off + rid * 3 * L.hsz;
-    uint32_t *hlo = hkey + L.hsz;
-    uint32_t *hhi = hlo + L.hsz;
+    // per-receiver hash: key, write-lane mask (lo, hi), read-lane mask (lo, hi)
+    uint32_t *hkey = lds + L.hash_off + rid * 5 * L.hsz;
+    uint32_t *hwlo = hkey + L.hsz, *hwhi = hkey + 2 * L.hsz;
+    uint32_t *hrlo = hkey + 3 * L.hsz, *hrhi = hkey + 4 * L.hsz;
     const uint32_t hmask = L.hsz - 1;
+    const uint32_t mybit = 1u << (lane & 31);
+    auto wmask = [&](uint32_t h) -> uint64_t {
+        return ((uint64_t)__hip_atomic_load(&hwhi[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) << 32) |
+               __hip_atomic_load(&hwlo[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    };
+    auto rmask = [&](uint32_t h) -> uint64_t {
+        return ((uint64_t)__hip_atomic_load(&hrhi[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) << 32) |
+               __hip_atomic_load(&hrlo[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    };
     constexpr uint32_t in_rows = mbox_rows<R, W>();
     uint64_t t_window = 0, t_ring = 0, t_busy = 0;
     for (uint32_t n = rid;; n += M) {
@@ -432,101 +695,147 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
         if (leave) break;
         const uint64_t t2 = wall_clock64();
         fence_acq();
-        const uint32_t cw = (uint32_t)e;
+        const uint32_t cwr = (uint32_t)e;
+        const uint32_t cw = cwr & ~kDelegateBit;
+        const bool delegate = (cwr & kDelegateBit) != 0;
         const uint32_t *src = c.mbox + (size_t)cw * in_rows * kWave;
         uint32_t *dst = lds + L.mb_off + mbi * L.mb_rows * kWave;
         const uint32_t snap = ld_rlx(&src[lane]);
-        uint32_t it[R + W];
+        uint32_t it[R + W], wv[W];
 #pragma unroll
         for (int r = 0; r < R + W; ++r) it[r] = ld_rlx(&src[(1 + r) * kWave + lane]);
-        int32_t wv[W];
-        if (server_wb) {
 #pragma unroll
-            for (int w = 0; w < W; ++w) wv[w] = (int32_t)ld_rlx(&src[(1 + R + W + w) * kWave + lane]);
-        }
+        for (int w = 0; w < W; ++w) wv[w] = ld_rlx(&src[(1 + R + W + w) * kWave + lane]);
         const bool valid = snap != kNone;
 
-        // (b) commits already stamped in HBM (hashed-stamp mode)
+        // commits already visible in the HBM tables when the request is staged
         const uint32_t tg = lds_ld_acq(&lds[kLdsStampTs]);
         bool pref = false;
-        if (stamp_mode && !L.exact && valid) {
-            uint32_t p = 0;
+        int32_t pval[R];
+        uint32_t pst[W];
 #pragma unroll
-            for (int r = 0; r < R + W; ++r)
-                if (it[r] != kNone) {
-                    const uint32_t v = __hip_atomic_load(&c.stamps[it[r]], __ATOMIC_RELAXED,
-                                                         __HIP_MEMORY_SCOPE_WORKGROUP);
-                    p = v > p ? v : p;
-                }
-            pref = p > snap;
+        for (int r = 0; r < R; ++r) pval[r] = 0;
+#pragma unroll
+        for (int w = 0; w < W; ++w) pst[w] = 0;
+        if (!L.exact && valid) {
+            if (delegate) {
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                    if (it[r] != kNone)
+                        pval[r] = __hip_atomic_load(&c.vals[it[r]], __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_WORKGROUP);
+#pragma unroll
+                for (int w = 0; w < W; ++w)
+                    if (it[R + w] != kNone)
+                        pst[w] = __hip_atomic_load(&c.stamps[it[R + w]], __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+            } else if (stamp_mode) {
+                uint32_t p = 0;
+#pragma unroll
+                for (int r = 0; r < R + W; ++r)
+                    if (it[r] != kNone) {
+                        const uint32_t v = __hip_atomic_load(&c.stamps[it[r]], __ATOMIC_RELAXED,
+                                                             __HIP_MEMORY_SCOPE_WORKGROUP);
+                        p = v > p ? v : p;
+                    }
+                pref = p > snap;
+            }
         }
 
-        // (a) conflicts with lower lanes of the request: insert the write set into an
-        // LDS hash (item -> mask of writing lanes), then look up reads and writes
-        uint32_t slot[W];
+        // Conflicts with lower lanes of the request.  Insert the write set (and for a
+        // delegated request the read set) into an LDS hash of item -> lane masks, then
+        // look up: conf = lower lanes that write an item this lane reads or writes
+        // (and, delegated, lower lanes that read an item this lane writes).
+        uint32_t slot_w[W], slot_r[R];
 #pragma unroll
         for (int w = 0; w < W; ++w) {
-            slot[w] = kNone;
+            slot_w[w] = kNone;
             const uint32_t x = it[R + w];
             if (valid && x != kNone) {
-                uint32_t h = mix32(x) & hmask;
-                for (uint32_t p = 0; p < L.hsz; ++p) {
-                    const uint32_t old = atomicCAS(&hkey[h], kNone, x);
-                    if (old == kNone || old == x) {
-                        if (lane < 32) atomicOr(&hlo[h], 1u << lane);
-                        else atomicOr(&hhi[h], 1u << (lane - 32));
-                        slot[w] = h;
-                        break;
-                    }
-                    h = (h + 1) & hmask;
-                }
+                slot_w[w] = hash_claim(hkey, hmask, x);
+                if (slot_w[w] != kNone) atomicOr(lane < 32 ? &hwlo[slot_w[w]] : &hwhi[slot_w[w]], mybit);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            slot_r[r] = kNone;
+            const uint32_t x = it[r];
+            if (delegate && valid && x != kNone) {
+                slot_r[r] = hash_claim(hkey, hmask, x);
+                if (slot_r[r] != kNone) atomicOr(lane < 32 ? &hrlo[slot_r[r]] : &hrhi[slot_r[r]], mybit);
             }
         }
         wg_fence();
         uint64_t conf = 0;
-        if (valid) {
+        uint32_t prod[(R + 3) / 4];
 #pragma unroll
-            for (int r = 0; r < R + W; ++r) {
-                const uint32_t x = it[r];
-                if (x == kNone) continue;
-                uint32_t h = mix32(x) & hmask;
-                for (uint32_t p = 0; p < L.hsz; ++p) {
-                    const uint32_t k = __hip_atomic_load(&hkey[h], __ATOMIC_RELAXED,
-                                                         __HIP_MEMORY_SCOPE_WORKGROUP);
-                    if (k == x) {
-                        const uint32_t mh = __hip_atomic_load(&hhi[h], __ATOMIC_RELAXED,
-                                                              __HIP_MEMORY_SCOPE_WORKGROUP);
-                        const uint32_t ml = __hip_atomic_load(&hlo[h], __ATOMIC_RELAXED,
-                                                              __HIP_MEMORY_SCOPE_WORKGROUP);
-                        conf |= (((uint64_t)mh << 32) | ml) & lt;
-                        break;
-                    }
-                    if (k == kNone) break;
-                    h = (h + 1) & hmask;
+        for (int q = 0; q < (R + 3) / 4; ++q) prod[q] = 0xFFFFFFFFu;
+        // producer of read r: the highest lower lane writing the item (0xFF: none)
+        uint32_t plane[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) plane[r] = 0xFFu;
+#pragma unroll
+        for (int r = 0; r < R + W; ++r) {
+            const uint32_t x = it[r];
+            if (valid && x != kNone) {
+                const uint32_t h = hash_find(hkey, hmask, x);
+                if (h != kNone) {
+                    const uint64_t wm = wmask(h) & lt;
+                    conf |= wm;
+                    if (delegate && r >= R) conf |= rmask(h) & lt;
+                    if (delegate && r < R && wm) plane[r < R ? r : 0] = 63u - (uint32_t)__builtin_clzll(wm);
                 }
+            }
+        }
+        if (delegate) {
+            // which write slot of the producer holds the item; the shuffles run on
+            // every lane (a ds_bpermute from an inactive lane is undefined)
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const uint32_t p = plane[r];
+                const int srcl = (int)(p == 0xFFu ? lane : p);
+                uint32_t code = p;
+#pragma unroll
+                for (int w = 0; w < W; ++w) {
+                    const uint32_t pw = (uint32_t)__shfl((int)it[R + w], srcl, 64);
+                    code = (p != 0xFFu && pw == it[r]) ? (p | ((uint32_t)w << 6)) : code;
+                }
+                const uint32_t sh = 8u * (r % 4);
+                prod[r / 4] = (prod[r / 4] & ~(0xFFu << sh)) | (code << sh);
             }
         }
         wg_fence();
 #pragma unroll
         for (int w = 0; w < W; ++w)
-            if (slot[w] != kNone) {
-                __hip_atomic_store(&hkey[slot[w]], kNone, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                __hip_atomic_store(&hlo[slot[w]], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                __hip_atomic_store(&hhi[slot[w]], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (slot_w[w] != kNone) {
+                hkey[slot_w[w]] = kNone;
+                hwlo[slot_w[w]] = 0; hwhi[slot_w[w]] = 0; hrlo[slot_w[w]] = 0; hrhi[slot_w[w]] = 0;
+            }
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (slot_r[r] != kNone) {
+                hkey[slot_r[r]] = kNone;
+                hwlo[slot_r[r]] = 0; hwhi[slot_r[r]] = 0; hrlo[slot_r[r]] = 0; hrhi[slot_r[r]] = 0;
             }
 
         dst[RW::snap * kWave + lane] = snap;
 #pragma unroll
         for (int r = 0; r < R + W; ++r) dst[(RW::items + r) * kWave + lane] = it[r];
+#pragma unroll
+        for (int w = 0; w < W; ++w) dst[(RW::wv + w) * kWave + lane] = wv[w];
         dst[RW::conf_lo * kWave + lane] = (uint32_t)conf;
         dst[RW::conf_hi * kWave + lane] = (uint32_t)(conf >> 32);
         dst[RW::pref * kWave + lane] = pref ? 1u : 0u;
-        if (server_wb) {
+        if (delegate) {
 #pragma unroll
-            for (int w = 0; w < W; ++w) dst[(RW::wv + w) * kWave + lane] = (uint32_t)wv[w];
+            for (int q = 0; q < (R + 3) / 4; ++q) dst[(RW::prod + q) * kWave + lane] = prod[q];
+#pragma unroll
+            for (int r = 0; r < R; ++r) dst[(RW::pval + r) * kWave + lane] = (uint32_t)pval[r];
+#pragma unroll
+            for (int w = 0; w < W; ++w) dst[(RW::pstamp + w) * kWave + lane] = pst[w];
         }
         if (lane == 0) {
-            mbwave[mbi] = cw;
+            mbwave[mbi] = cwr;
             mbtg[mbi] = tg;
         }
         wg_fence();
@@ -548,15 +857,21 @@ __device__ __forceinline__ void server_workgroup(const Ctx &c, const ServerLayou
 {
     const uint32_t wave = threadIdx.x / kWave;
     for (uint32_t i = threadIdx.x; i < L.mb_off; i += blockDim.x) lds[i] = 0;
-    if ((c.variant & 0x3u) == kValStamp)
+    if ((c.variant & 0x3u) == kValStamp) {
         for (uint32_t i = threadIdx.x; i < L.nbuckets; i += blockDim.x) lds[L.bucket_off + i] = 0;
+        if (L.exact)
+            for (uint32_t i = threadIdx.x; i < c.n_items; i += blockDim.x)
+                lds[L.val_off + i] = (uint32_t)c.vals[i];
+        else
+            for (uint32_t i = threadIdx.x; i < L.nrc; i += blockDim.x) lds[L.val_off + i] = kNone;
+        for (uint32_t i = threadIdx.x; i < kWave * W; i += blockDim.x) lds[L.sup_off + i] = 0;
+    }
     const uint32_t M = c.wg_waves - 2;
     for (uint32_t r = 0; r < M; ++r)
         for (uint32_t i = threadIdx.x; i < L.hsz; i += blockDim.x) {
-            uint32_t *h = lds + L.hash_off + r * 3 * L.hsz;
+            uint32_t *h = lds + L.hash_off + r * 5 * L.hsz;
             h[i] = kNone;
-            h[L.hsz + i] = 0;
-            h[2 * L.hsz + i] = 0;
+            for (uint32_t q = 1; q < 5; ++q) h[q * L.hsz + i] = 0;
         }
     __syncthreads();
     if (wave == 0) committer<R, W>(c, L, lds);

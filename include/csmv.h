@@ -65,6 +65,9 @@ typedef struct {
     uint32_t max_attempts;     /* per-transaction attempt bound (0 = 1<<20) */
     uint32_t stage_ahead;      /* requests staged ahead of the committer (0 = 3) */
     uint32_t lds_kib;          /* server LDS budget in KiB (0 = device maximum) */
+    uint32_t n_serial_waves;   /* client waves of the serialized lane (routed transfers) */
+    uint32_t begin_wait;       /* 1: a wave's snapshot covers its own last commit
+                                  (TXBegin's prevTs wait, bank/csmv_sh/API.cu:283) */
 } csmv_config;
 
 typedef struct {
@@ -98,6 +101,9 @@ typedef struct {
     double prof_ns[8];
     uint32_t stamp_buckets;      /* LDS commit-stamp buckets (STAMP mode) */
     uint32_t stamp_exact;        /* 1 if every item has its own bucket */
+    uint64_t aborts_unknown;     /* serialized lane: an operand's committed value was not at hand */
+    uint32_t lds_bytes;          /* server workgroup LDS */
+    uint32_t reserved;
 } csmv_stats;
 
 const char *csmv_status_str(csmv_status s);
@@ -128,6 +134,12 @@ csmv_status csmv_zipf_cdf(uint32_t n, double theta, uint32_t *cdf);
 csmv_status csmv_bank_generate(csmv_engine *e, const csmv_bank_spec *spec);
 /* Copies the stream back: op[n_txns], addr[rmax][n_txns]. */
 csmv_status csmv_bank_read_stream(csmv_engine *e, uint32_t *op, uint32_t *addr);
+/* Routes the resident stream: transfers with hot[k] != 0 go to the serialized lane
+ * (executed by the commit server in commit order), everything else to the normal
+ * lane.  hot = NULL removes the routing.  The reference has no counterpart: its
+ * drivers run every transaction through the same client path
+ * (bank/csmv_sh/exe.cu:142-269). */
+csmv_status csmv_bank_route(csmv_engine *e, const uint8_t *hot, size_t n);
 /* Runs the whole resident stream through the persistent kernel (replaces
  * parent_kernel/server_kernel/client_kernel, bank/csmv_sh/exe.cu:89-338). */
 csmv_status csmv_bank_run(csmv_engine *e, csmv_stats *stats);

@@ -25,6 +25,8 @@
 namespace csmv {
 
 constexpr uint32_t kNone = 0xFFFFFFFFu;
+// request-ring flag: the request's transactions are executed by the commit server
+constexpr uint32_t kDelegateBit = 0x80000000u;
 constexpr uint32_t kWave = 64;
 
 // ---- wave64 primitives --------------------------------------------------------------
@@ -116,17 +118,27 @@ struct Ctx {
     uint64_t *done;          // [Q] (n+1) << 32 | last ts of batch n (write-back finished)
     uint32_t *mbox;          // [client_waves][rows][64] request mailboxes
     uint32_t *stamps;        // [n_items] commit stamps (server-private, STAMP mode)
+    int32_t *vals;           // [n_items] newest committed values (server-private)
+    uint64_t *reply_v;       // [client_waves * 64 * W] (n+1) << 32 | new value (delegated)
     uint64_t *prof;          // kProf* timing counters (wall_clock64 ticks)
     uint32_t qmask;
     uint32_t n_client_waves, wg_waves;
     uint32_t variant, atr_entries;
     uint32_t stage_ahead;    // receivers stage request n once request n - stage_ahead is taken
+    uint32_t n_serial_waves; // client waves of the serialized lane (the last ones)
+    uint32_t begin_wait;     // a wave's next snapshot covers its own last commit
     uint64_t timeout_ticks;
     uint32_t max_attempts;
     // resident bank stream and its commit log
     const uint32_t *op;
     const uint32_t *addr;
     uint64_t n_txns;
+    // routing: the normal lane takes norm_list[i] (i itself when norm_list is null),
+    // the serialized lane takes hot_list[i]
+    const uint32_t *norm_list;
+    const uint32_t *hot_list;
+    uint64_t n_norm, n_hot;
+    uint64_t *next_hot;
     uint32_t *log_ts;
     int32_t *log_v;
     uint32_t *log_ab;
@@ -141,6 +153,7 @@ enum : uint32_t {
     kStatAbortRecord,
     kStatBatches,
     kStatFinalTs,
+    kStatAbortUnknown,   // serialized lane: an operand's committed value was not at hand
     kStatCount
 };
 
@@ -223,9 +236,19 @@ __device__ __forceinline__ bool pool_read(const Ctx &c, uint32_t item, uint32_t 
 }
 
 // TXBegin (collabValidation/csmv_sh/API.cu:581-586): wave-uniform snapshot.
-template <int R, int W> __device__ __forceinline__ void tx_begin(const Ctx &c, Tx<R, W> &tx)
+// With wait_ts > 0 the snapshot is taken once the clock covers the wave's own last
+// commit (TXBegin's prevTs wait, bank/csmv_sh/API.cu:283); otherwise a wave re-reading
+// items it just wrote would validate against its own commits.
+template <int R, int W>
+__device__ __forceinline__ bool tx_begin(const Ctx &c, Tx<R, W> &tx, uint32_t wait_ts = 0)
 {
-    tx.snap = bcast(ld_acq(c.gc));
+    uint32_t g = bcast(ld_acq(c.gc));
+    if (g < wait_ts) {
+        Spin sp;
+        while ((g = bcast(ld_acq(c.gc))) < wait_ts)
+            if (!sp.tick(c, kErrReplyTimeout)) return false;
+    }
+    tx.snap = g;
     tx.nr = 0;
     tx.nw = 0;
     tx.aborted = false;
@@ -234,6 +257,7 @@ template <int R, int W> __device__ __forceinline__ void tx_begin(const Ctx &c, T
     for (int r = 0; r < R; ++r) tx.rs[r] = kNone;
 #pragma unroll
     for (int w = 0; w < W; ++w) { tx.wa[w] = kNone; tx.wv[w] = 0; }
+    return true;
 }
 
 // TXWrite (bank/csmv_sh/API.cu:69-85): upsert into the register write set.
@@ -333,9 +357,13 @@ template <int R, int W> constexpr uint32_t mbox_rows() { return 1 + R + 2 * W; }
 // fetch_add reserves the request slot for all 64 lanes, one release store publishes it.
 // Returns the commit timestamp (0 = aborted).  On a device error *fail is set and the
 // caller must leave its loop.
+//
+// With delegate=true the request carries transactions for the serialized lane: the
+// server executes them (rs = operands, wv = transaction id and amount) and returns the
+// written values, which replace tx.wv before the client installs them.
 template <int R, int W>
 __device__ __forceinline__ uint32_t tx_commit(const Ctx &c, uint32_t cw, Tx<R, W> &tx, bool req, bool *fail,
-                                               uint64_t *wait_ticks)
+                                               uint64_t *wait_ticks, bool delegate = false)
 {
     const uint32_t lane = lane_id();
     constexpr uint32_t rows = mbox_rows<R, W>();
@@ -353,7 +381,7 @@ __device__ __forceinline__ uint32_t tx_commit(const Ctx &c, uint32_t cw, Tx<R, W
     uint64_t n = 0;
     if (lane == 0) {
         n = fetch_add(c.req_tail, (uint64_t)1);
-        st_rel(&c.ring[n & c.qmask], ((n + 1) << 32) | cw);
+        st_rel(&c.ring[n & c.qmask], ((n + 1) << 32) | cw | (delegate ? kDelegateBit : 0u));
     }
     n = bcast64(n);
 
@@ -364,8 +392,39 @@ __device__ __forceinline__ uint32_t tx_commit(const Ctx &c, uint32_t cw, Tx<R, W
         if (ballot((v >> 32) != n + 1) == 0) break;
         if (!sp.tick(c, kErrReplyTimeout)) { *fail = true; return 0; }
     }
-    *wait_ticks += wall_clock64() - sp.t0;
     const uint32_t ts = req ? (uint32_t)v : 0u;
+    if (delegate && ballot(ts != 0)) {
+        // new values, and for each write the stamp its install must follow (kNone: the
+        // write was superseded inside the request and is not installed)
+        const uint64_t *rv = c.reply_v + ((size_t)cw * kWave + lane) * 2 * W;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            uint64_t x, y;
+            for (;;) {
+                x = ld_rlx(&rv[w]);
+                y = ld_rlx(&rv[W + w]);
+                if (ballot(ts != 0 && (uint32_t)w < tx.nw &&
+                           ((x >> 32) != n + 1 || (y >> 32) != n + 1)) == 0)
+                    break;
+                if (!sp.tick(c, kErrReplyTimeout)) { *fail = true; return 0; }
+            }
+            tx.wv[w] = (int32_t)(uint32_t)x;
+            const uint32_t prev = (uint32_t)y;
+            if (ts == 0 || (uint32_t)w >= tx.nw || prev == kNone) tx.wa[w] = kNone;
+            // installs of one item happen in commit order: wait for the predecessor
+            for (;;) {
+                bool wait = false;
+                if (tx.wa[w] != kNone) {
+                    const uint32_t ht = ld_rlx(&c.ht[tx.wa[w]]);
+                    const uint64_t head = ld_rlx(&c.slot[(size_t)(ht & 0xFFFFu) * c.n_items + tx.wa[w]]);
+                    wait = (uint32_t)(head >> 32) != prev;
+                }
+                if (ballot(wait) == 0) break;
+                if (!sp.tick(c, kErrReplyTimeout)) { *fail = true; return 0; }
+            }
+        }
+    }
+    *wait_ticks += wall_clock64() - sp.t0;
     if ((c.variant & kServerWriteback) == 0) {
         const uint64_t cm = ballot(ts != 0);
         if (cm) {

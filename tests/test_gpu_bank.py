@@ -16,13 +16,17 @@ INIT = 1000
 
 
 def run_bank(n_items, bank, variant="stamp", depth=4, atr=4096, waves=8, wg_waves=8,
-             check_stream=True, timeout_ms=4000):
+             check_stream=True, timeout_ms=4000, route=None, serial_waves=0, begin_wait=True):
     print(f"\n[case] n_items={n_items} bank={bank} variant={variant} depth={depth} atr={atr} "
-          f"waves={waves}", flush=True)
+          f"waves={waves} serial_waves={serial_waves} routed={route is not None}", flush=True)
     with csmv.Engine(n_items, depth=depth, atr_entries=atr, client_waves=waves,
-                     wg_waves=wg_waves, variant=variant, timeout_ms=timeout_ms) as e:
+                     wg_waves=wg_waves, variant=variant, timeout_ms=timeout_ms,
+                     n_serial_waves=serial_waves, begin_wait=begin_wait) as e:
         e.fill(INIT)
         e.bank_generate(bank)
+        if route is not None:
+            op, _ = e.bank_read_stream()
+            e.bank_route(route(op))
         stats = e.bank_run()
         op, addr = e.bank_read_stream()
         ts, v, ab = e.bank_read_log()
@@ -95,3 +99,66 @@ def test_hashed_stamps_zipf():
     """More items than LDS stamp buckets: hashed stamps with exact HBM stamps behind."""
     run_bank(1 << 20, csmv.Bank(n_txns=1 << 16, seed=8, n_reads=4, n_reads_ro=4, zipf=True),
              variant="stamp", depth=8, waves=64)
+
+
+def test_no_begin_wait():
+    """Snapshots that may predate the wave's own last commit (the collabValidation
+    TXBegin, which does not wait) still replay."""
+    run_bank(4096, csmv.Bank(n_txns=1 << 15, seed=31), variant="stamp", waves=16,
+             begin_wait=False)
+
+
+def _random_route(frac, seed):
+    def f(op):
+        return (np.random.default_rng(seed).random(op.size) < frac).astype(np.uint8)
+    return f
+
+
+@pytest.mark.parametrize("frac", [0.5, 1.0])
+def test_serialized_lane_exact(frac):
+    """Transfers routed to the serialized lane run on the commit server, interleaved
+    with normal-lane commits in one timestamp order (exact LDS value table)."""
+    st = run_bank(4096, csmv.Bank(n_txns=1 << 16, seed=41), variant="stamp", waves=16,
+                  serial_waves=4, route=_random_route(frac, 1))
+    assert st["commits_update"] > 0
+
+
+def test_serialized_lane_hot_zipf_hashed():
+    """1M accounts (hashed stamps, recent-value cache): transfers touching the 256
+    hottest accounts go to the serialized lane."""
+    def route(op):
+        return None
+    n = 1 << 20
+    bank = csmv.Bank(n_txns=1 << 17, seed=43, n_reads=4, n_reads_ro=4, zipf=True)
+    hot_items = np.array([orc.scramble(r, n) for r in range(256)], dtype=np.uint32)
+
+    def hot_route(op):
+        T = op.size
+        spec = orc.bank_spec(n, T, seed=bank.seed, n_reads=4, n_reads_ro=4, zipf=True)
+        _, addr = orc.bank_gen(spec)
+        A = addr.reshape(-1, T)
+        return np.isin(A, hot_items).any(axis=0).astype(np.uint8)
+    run_bank(n, bank, variant="stamp", depth=8, waves=32, serial_waves=8, route=hot_route,
+             wg_waves=6, check_stream=False)
+
+
+def test_serialized_lane_small_cache_unknowns():
+    """A tiny LDS budget forces a small recent-value cache: operands whose value is not at
+    hand make the server refuse (the client resends); results must not change."""
+    n = 1 << 16
+    bank = csmv.Bank(n_txns=1 << 15, seed=47, n_reads=4, n_reads_ro=4, zipf=True)
+    print("\n[case] lds_kib=64 serialized lane", flush=True)
+    with csmv.Engine(n, depth=8, client_waves=16, wg_waves=4, variant="stamp",
+                     n_serial_waves=4, lds_kib=64) as e:
+        e.fill(INIT)
+        e.bank_generate(bank)
+        op, addr = e.bank_read_stream()
+        e.bank_route(((op & 0xFF) == 0).astype(np.uint8))
+        st = e.bank_run()
+        ts, v, ab = e.bank_read_log()
+        final = e.read_state()
+    spec = orc.bank_spec(n, bank.n_txns, seed=bank.seed, n_reads=4, n_reads_ro=4, zipf=True)
+    rep, _ = orc.bank_replay(spec, op, addr, INIT, ts, v, final)
+    print("[stats]", {k: st[k] for k in ("commits", "aborts_unknown", "stamp_exact")}, flush=True)
+    assert rep["fails"] == 0, rep
+    assert st["stamp_exact"] == 0
