@@ -79,8 +79,15 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--variant", default="stamp")
-    ap.add_argument("--waves", type=int, default=int(os.environ.get("CSMV_BENCH_WAVES", "256")))
+    ap.add_argument("--waves", type=int, default=int(os.environ.get("CSMV_BENCH_WAVES", "64")))
+    ap.add_argument("--serial-waves", type=int, default=16)
+    ap.add_argument("--wg-waves", type=int, default=6)
+    ap.add_argument("--router", default="transfers", choices=("none", "transfers"),
+                    help="none: every transaction on the normal (client) lane; transfers: "
+                         "every transfer on the serialized lane")
     ap.add_argument("--txns", type=int, default=1 << 24)
+    ap.add_argument("--variant-txns", type=int, default=1 << 18,
+                    help="stream length for the side-by-side reference variants")
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=1 << 22)
@@ -95,13 +102,19 @@ def main():
     bank = csmv.Bank(n_txns=args.txns, seed=1, n_reads=4, n_reads_ro=4, pct_ro=10, zipf=True,
                      theta=0.99)
 
-    def make(variant):
+    import numpy as np
+
+    def make(variant, b, router, serial):
         e = csmv.Engine(N_ACCOUNTS, depth=8, atr_entries=4096, client_waves=args.waves,
-                        variant=variant, timeout_ms=8000)
-        e.bank_generate(bank)
+                        wg_waves=args.wg_waves, variant=variant, timeout_ms=8000,
+                        n_serial_waves=serial if router != "none" else 0)
+        e.bank_generate(b)
+        if router == "transfers":
+            op, _ = e.bank_read_stream()
+            e.bank_route(((op & 0xFF) == 0).astype(np.uint8))
         return e
 
-    e = make(args.variant)
+    e = make(args.variant, bank, args.router, args.serial_waves)
     for _ in range(args.warmup):
         e.fill(INIT)
         e.bank_run()
@@ -132,13 +145,16 @@ def main():
     plain_ceiling = atomic_rate / 2.0
     variants = {}
     if not args.no_variants:
-        for v in REF_VARIANTS:
-            ev = make(v)
+        vbank = csmv.Bank(n_txns=args.variant_txns, seed=1, n_reads=4, n_reads_ro=4, pct_ro=10,
+                          zipf=True, theta=0.99)
+        for v in REF_VARIANTS + ("stamp",):
+            ev = make(v, vbank, "none", 0)
             ev.fill(INIT)
             r = ev.bank_run()
             ev.close()
             variants[v] = {"txn_per_s": r["txn_per_s"], "abort_rate": r["abort_rate"],
-                           "kernel_ms": r["kernel_ms"]}
+                           "kernel_ms": r["kernel_ms"], "txns": args.variant_txns,
+                           "lane": "normal (client-executed CSMV)"}
     out = {
         "metric": METRIC,
         "value": value,
@@ -157,10 +173,17 @@ def main():
                         "/ write 2, audits read 4, 90/10, 2^24 txns, version depth 8",
             "txns_per_step": args.txns,
             "variant": args.variant,
+            "router": args.router,
             "client_waves": args.waves,
+            "serial_lane_waves": args.serial_waves if args.router != "none" else 0,
+            "server_wg_waves": args.wg_waves,
             "client_lanes": args.waves * 64,
             "record_ring": 4096,
         },
+        "aborts_by_class": {k: sum(r[k] for r in runs) for k in
+                            ("aborts_dataage", "aborts_readwrite", "aborts_prevalid",
+                             "aborts_recordage", "aborts_unknown")},
+        "committer_busy_frac": sum(r["prof"]["commit_busy"] for r in runs) / (kernel_s * 1e9),
         "abort_rate": aborts / (aborts + commits),
         "wall_s_timed_steps": wall,
         "roofline": roofline,
