@@ -38,13 +38,16 @@ def bytes_per_commit(stats, n_reads=4, n_writes=2, n_reads_ro=4):
     return stats["commits_update"] * upd + stats["commits_ro"] * ro
 
 
-def load_traffic():
+def load_traffic(commits):
+    """HBM bytes of one launch: the per-commit traffic of the committed counter profile
+    (FETCH_SIZE and WRITE_SIZE passes, scaled by a copy kernel of known size) times the
+    launch's commits."""
     p = os.path.join(ROOT, "profiles", "bank_zipf_1M_traffic.json")
     if not os.path.exists(p):
         return None
     try:
         with open(p) as f:
-            return json.load(f).get("hbm_bytes_per_launch")
+            return json.load(f)["hbm_bytes_per_commit"] * commits
     except Exception:
         return None
 
@@ -137,7 +140,7 @@ def main():
 
     alg_bytes = bytes_per_commit(runs[-1])
     achieved = alg_bytes / (runs[-1]["kernel_ms"] * 1e-3) / 1e9
-    traffic = load_traffic()
+    traffic = load_traffic(runs[-1]["commits"])
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic}
     # the per-lane protocol spends at least two same-address atomics per committed

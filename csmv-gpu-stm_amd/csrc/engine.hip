@@ -451,12 +451,13 @@ template <int R, int W> ServerLayout server_layout(const csmv_engine *e)
     w += M * 5 * L.hsz;
     L.sup_off = w;
     w += kWave * W;
+    L.jmp_off = w;
+    w += 3 * kWave * R + kWave * W;
     L.atr_off = w;
     if (vmode != kValStamp) w += W * e->cfg.atr_entries;
     L.bucket_off = w;
     L.val_off = w;
     L.nbuckets = 0;
-    L.nrc = 0;
     L.exact = 0;
     if (vmode == kValStamp) {
         const uint64_t budget = e->lds_max / 4;
@@ -468,11 +469,9 @@ template <int R, int W> ServerLayout server_layout(const csmv_engine *e)
             L.val_off = L.bucket_off + L.nbuckets;
             w += L.nbuckets + e->cfg.n_items;
         } else {
-            // hashed stamps + recent-value cache (key, value, stamp)
-            L.nrc = std::min<uint32_t>(pow2_floor(room / 12), 4096);
-            L.nbuckets = pow2_floor(room - 3ull * L.nrc);
-            L.val_off = L.bucket_off + L.nbuckets;
-            w += L.nbuckets + 3 * L.nrc;
+            // hashed stamps; exact stamps and values stay in HBM
+            L.nbuckets = pow2_floor(room);
+            w += L.nbuckets;
         }
     }
     L.words = w;

@@ -60,10 +60,10 @@ struct ServerLayout {
     uint32_t bucket_off;   // commit-stamp buckets [nbuckets]
     uint32_t nbuckets;     // power of two (STAMP mode)
     uint32_t exact;        // bucket i is item i (n_items <= nbuckets)
-    uint32_t val_off;      // exact: value of every item [n_items]; hashed: recent-value
-                           // cache keys [nrc], values [nrc], stamps [nrc]
+    uint32_t val_off;      // exact mode: value of every item [n_items]
     uint32_t sup_off;      // serialized lane: superseded-write flags [64 * W]
-    uint32_t nrc;          // recent-value cache entries (power of two, hashed mode)
+    uint32_t jmp_off;      // serialized lane: next [64 * R], weight [64 * R], base [64 * R],
+                           // write deltas [64 * W]
     uint32_t atr_off;      // record ring [W][E] (SCAN / COLLAB)
     uint32_t hash_off;     // per-receiver hash tables [M][5][hsz]
     uint32_t hsz;          // power of two >= 64 * (R + W)
@@ -138,56 +138,38 @@ __device__ __forceinline__ bool hits(uint32_t x, const uint32_t *rs, const uint3
 // when it fits the exact LDS table, when the recent-value cache still holds it, or when
 // the value gathered at staging is still current (no write to its bucket since T_g).
 struct ValueTables {
-    uint32_t *bucket, *lval, *rc_key, *rc_val, *rc_ts;
+    uint32_t *bucket, *lval;
 };
 
-__device__ __forceinline__ uint32_t rc_slot(const ServerLayout &L, uint32_t x)
-{
-    return mix32(x ^ 0x9E3779B9u) & (L.nrc - 1);
-}
-
-// newest committed value and stamp of item x for commits decided before the current
-// request; false when neither table has it (hashed mode only)
-__device__ __forceinline__ bool committed_of(const ServerLayout &L, const ValueTables &V, uint32_t x,
-                                             uint32_t tg, int32_t pv, uint32_t ps, int32_t &val,
-                                             uint32_t &ts)
+// Newest committed value and stamp of item x as of the commits decided so far.  Exact
+// mode reads the LDS tables.  Hashed mode uses the values gathered at staging (pv, ps,
+// as of commit tg) when no commit since then wrote x's bucket, and otherwise the HBM
+// tables, which hold this wave's own recent stores.
+__device__ __forceinline__ void committed_of(const Ctx &c, const ServerLayout &L, const ValueTables &V,
+                                             uint32_t x, uint32_t tg, int32_t pv, uint32_t ps,
+                                             int32_t &val, uint32_t &ts)
 {
     if (L.exact) {
         val = (int32_t)V.lval[x];
         ts = V.bucket[x];
-        return true;
-    }
-    const uint32_t h = rc_slot(L, x);
-    if (V.rc_key[h] == x) {
-        val = (int32_t)V.rc_val[h];
-        ts = V.rc_ts[h];
-        return true;
-    }
-    if (V.bucket[bucket_of(L, x)] <= tg) {  // no write to x since the request was staged
+    } else if (V.bucket[bucket_of(L, x)] <= tg) {
         val = pv;
         ts = ps;
-        return true;
+    } else {
+        val = __hip_atomic_load(&c.vals[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        ts = __hip_atomic_load(&c.stamps[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
-    return false;
 }
 
 template <int R, int W>
 __device__ __forceinline__ void record_values(const Ctx &c, const ServerLayout &L, const ValueTables &V,
-                                              const uint32_t *wa, const int32_t *v, uint32_t ts,
-                                              const bool *on)
+                                              const uint32_t *wa, const int32_t *v, const bool *on)
 {
 #pragma unroll
     for (int w = 0; w < W; ++w)
         if (on[w] && wa[w] != kNone) {
-            if (L.exact) {
-                V.lval[wa[w]] = (uint32_t)v[w];
-            } else {
-                const uint32_t h = rc_slot(L, wa[w]);
-                V.rc_key[h] = wa[w];
-                V.rc_val[h] = (uint32_t)v[w];
-                V.rc_ts[h] = ts;
-                c.vals[wa[w]] = v[w];
-            }
+            if (L.exact) V.lval[wa[w]] = (uint32_t)v[w];
+            else c.vals[wa[w]] = v[w];
         }
 }
 
@@ -252,10 +234,11 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
     ValueTables V;
     V.bucket = lds + L.bucket_off;
     V.lval = lds + L.val_off;
-    V.rc_key = lds + L.val_off;
-    V.rc_val = lds + L.val_off + L.nrc;
-    V.rc_ts = lds + L.val_off + 2 * L.nrc;
     uint32_t *sup = lds + L.sup_off;
+    uint32_t *jnext = lds + L.jmp_off;
+    int32_t *jw = (int32_t *)(jnext + kWave * R);
+    int32_t *jbase = jw + kWave * R;
+    int32_t *jdelta = jbase + kWave * R;
     const uint32_t Q = c.qmask + 1;
     const uint64_t lt = (1ull << lane) - 1;
 
@@ -322,68 +305,72 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
         if (delegate) {
             // ---- serialized lane: the server executes the transactions ----------------
             const uint32_t k = (uint32_t)wv[0], amt = (uint32_t)wv[1];
-            // values the reads see from before this request
-            int32_t ext[R];
-            bool unknown = false;
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                ext[r] = 0;
-                const uint32_t pj = (prod[r / 4] >> (8 * (r % 4))) & 0xFFu;
-                uint32_t dummy;
-                if (valid && rs[r] != kNone && pj == 0xFFu)
-                    unknown |= !committed_of(L, V, rs[r], tg, pval[r], 0u, ext[r], dummy);
-            }
-            // stamp of each written item before this request: the version a client
-            // installing the write must find at the head of the item's ring first
+            // A transfer changes account a by -amt and b by +amt (0 when a == b), so the
+            // value a read sees inside the request is the committed value before the
+            // request plus the deltas of the lower lanes on its producer chain (the
+            // highest lower writer of the item, then that writer's producer, ...).
+            // Chains are summed by pointer jumping: ceil(log2(chain)) rounds instead of one
+            // step per writer of a hot account.
             uint32_t prev[W];
 #pragma unroll
             for (int w = 0; w < W; ++w) {
                 prev[w] = kNone;
                 int32_t dv;
-                if (valid && wa[w] != kNone)
-                    unknown |= !committed_of(L, V, wa[w], tg, 0, pst[w], dv, prev[w]);
+                if (valid && wa[w] != kNone) committed_of(c, L, V, wa[w], tg, 0, pst[w], dv, prev[w]);
             }
-            // execute along the request's dependency order: a lane runs once every
-            // lower lane it conflicts with has run; values written inside the request
-            // reach their readers through ds_bpermute
-            uint64_t pending = ballot(valid);
-            bool okx = false, failed = !valid || unknown;
-            int32_t rd[R], nv[W];
+            const int32_t d0 = (wa[1] == kNone) ? 0 : -(int32_t)amt;
+            jdelta[lane * W + 0] = d0;
+            jdelta[lane * W + 1] = (int32_t)amt;
 #pragma unroll
-            for (int r = 0; r < R; ++r) rd[r] = 0;
+            for (int r = 0; r < R; ++r) {
+                const uint32_t node = lane * R + r;
+                const uint32_t pj = (prod[r / 4] >> (8 * (r % 4))) & 0xFFu;
+                int32_t base = 0;
+                uint32_t dts;
+                if (valid && rs[r] != kNone && pj == 0xFFu)
+                    committed_of(c, L, V, rs[r], tg, pval[r], 0u, base, dts);
+                jbase[node] = base;
+                jnext[node] = (pj == 0xFFu) ? node : ((pj & 63u) * R + (pj >> 6));
+            }
+            wg_fence();
 #pragma unroll
-            for (int w = 0; w < W; ++w) nv[w] = 0;
-            for (uint32_t guard = 0; pending && guard < kWave; ++guard) {
-                const bool ready = ((pending >> lane) & 1ull) && (conf & pending) == 0;
-                const uint64_t rm = ballot(ready);
-                int32_t in[R];
-                bool dep_fail = false;
+            for (int r = 0; r < R; ++r) {
+                const uint32_t node = lane * R + r;
+                const uint32_t pj = (prod[r / 4] >> (8 * (r % 4))) & 0xFFu;
+                jw[node] = (pj == 0xFFu) ? 0 : jdelta[(pj & 63u) * W + (pj >> 6)];
+            }
+            wg_fence();
+            for (uint32_t round = 0; round < 6; ++round) {
+                uint32_t nn[R];
+                int32_t ww[R];
+                bool more = false;
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
-                    const uint32_t pj = (prod[r / 4] >> (8 * (r % 4))) & 0xFFu;
-                    const int src = (int)(pj == 0xFFu ? lane : (pj & 63u));
-                    int32_t from = 0;
+                    const uint32_t node = lane * R + r;
+                    const uint32_t t = jnext[node];
+                    const uint32_t tt = jnext[t];
+                    ww[r] = jw[node] + (t != node ? jw[t] : 0);
+                    nn[r] = tt;
+                    more |= tt != t;
+                }
+                wg_fence();
 #pragma unroll
-                    for (int w = 0; w < W; ++w) {
-                        const int32_t x = __shfl(nv[w], src, 64);
-                        from = ((pj >> 6) == (uint32_t)w) ? x : from;
-                    }
-                    const int okp = __shfl((int)okx, src, 64);
-                    in[r] = pj == 0xFFu ? ext[r] : from;
-                    dep_fail |= (pj != 0xFFu) && !okp;
+                for (int r = 0; r < R; ++r) {
+                    jw[lane * R + r] = ww[r];
+                    jnext[lane * R + r] = nn[r];
                 }
-                if (ready) {
-                    if (!failed && !dep_fail) {
-                        transfer_exec<R, W>(rs, amt, in, rd, wa, nv);
-                        okx = true;
-                    } else {
-                        failed = true;
-                    }
-                }
-                pending &= ~rm;
+                wg_fence();
+                if (ballot(more) == 0) break;
             }
+            int32_t in[R], rd[R], nv[W];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const uint32_t node = lane * R + r;
+                in[r] = jbase[jnext[node]] + jw[node];
+            }
+            const bool okx = valid;
+            transfer_exec<R, W>(rs, amt, in, rd, wa, nv);
             const uint64_t committed = ballot(okx);
-            n_unk += __builtin_popcountll(ballot(valid && !okx));
             nc = (uint32_t)__builtin_popcountll(committed);
             if (okx) myts = last + 1 + (uint32_t)__builtin_popcountll(committed & lt);
             last += nc;
@@ -406,7 +393,7 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
             bool inst[W];
 #pragma unroll
             for (int w = 0; w < W; ++w) inst[w] = okx && wa[w] != kNone && sup[lane * W + w] == 0;
-            record_values<R, W>(c, L, V, wa, nv, myts, inst);
+            record_values<R, W>(c, L, V, wa, nv, inst);
             record_stamps<R, W>(c, L, V, wa, myts, okx);
             if (okx) {
                 c.log_ts[k] = myts;
@@ -516,7 +503,7 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
 #pragma unroll
                     for (int w = 0; w < W; ++w) on[w] = true;
                     record_stamps<R, W>(c, L, V, wa, ts, true);
-                    record_values<R, W>(c, L, V, wa, wv, ts, on);
+                    record_values<R, W>(c, L, V, wa, wv, on);
                 } else {
 #pragma unroll
                     for (int w = 0; w < W; ++w) atr[w * E + (ts & emask)] = wa[w];
@@ -862,8 +849,6 @@ __device__ __forceinline__ void server_workgroup(const Ctx &c, const ServerLayou
         if (L.exact)
             for (uint32_t i = threadIdx.x; i < c.n_items; i += blockDim.x)
                 lds[L.val_off + i] = (uint32_t)c.vals[i];
-        else
-            for (uint32_t i = threadIdx.x; i < L.nrc; i += blockDim.x) lds[L.val_off + i] = kNone;
         for (uint32_t i = threadIdx.x; i < kWave * W; i += blockDim.x) lds[L.sup_off + i] = 0;
     }
     const uint32_t M = c.wg_waves - 2;
