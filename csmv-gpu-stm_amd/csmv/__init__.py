@@ -67,11 +67,12 @@ class Stats(C.Structure):
                 ("aborts_prevalid", C.c_uint64), ("aborts_recordage", C.c_uint64),
                 ("batches", C.c_uint64), ("final_ts", C.c_uint64), ("kernel_ms", C.c_double),
                 ("error", C.c_uint32), ("client_lanes", C.c_uint32),
-                ("prof_ns", C.c_double * 8), ("stamp_buckets", C.c_uint32),
+                ("prof_ns", C.c_double * 16), ("stamp_buckets", C.c_uint32),
                 ("stamp_exact", C.c_uint32), ("aborts_unknown", C.c_uint64),
                 ("lds_bytes", C.c_uint32), ("reserved", C.c_uint32)]
     PROF = ("commit_idle", "commit_busy", "recv_window", "recv_ring", "recv_busy",
-            "client_reply", "client_round", "client_rounds")
+            "client_reply", "client_round", "client_rounds", "commit_read", "commit_resolve",
+            "commit_chain", "commit_finish")
 
     def as_dict(self) -> dict:
         d = {f: getattr(self, f) for f, _ in self._fields_ if f != "prof_ns"}

@@ -848,8 +848,8 @@ csmv_status csmv_bank_run(csmv_engine *e, csmv_stats *out)
         out->client_lanes = e->cfg.client_waves * kWave;
         const uint64_t *pf = (const uint64_t *)(host_ctl + kOffProf);
         const double ns_per_tick = 1e6 / (double)e->wallclock_khz;
-        for (int i = 0; i < 7; ++i) out->prof_ns[i] = (double)pf[i] * ns_per_tick;
-        out->prof_ns[7] = (double)pf[kProfClientRounds];
+        for (int i = 0; i < kProfCount; ++i) out->prof_ns[i] = (double)pf[i] * ns_per_tick;
+        out->prof_ns[kProfClientRounds] = (double)pf[kProfClientRounds];
         out->stamp_buckets = L.nbuckets;
         out->stamp_exact = L.exact;
         out->aborts_unknown = st[kStatAbortUnknown];

@@ -96,6 +96,9 @@ __device__ __forceinline__ void lds_st_rel(uint32_t *p, uint32_t v)
     __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 __device__ __forceinline__ void wg_fence() { __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup"); }
+// Orders LDS accesses between lanes of one wave (DS instructions of a wave execute in
+// order, so this only constrains the compiler).
+__device__ __forceinline__ void wave_fence() { __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront"); }
 
 __device__ __forceinline__ uint32_t wave_min(uint32_t v)
 {
@@ -244,6 +247,7 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
 
     uint32_t last = 0;  // newest assigned commit timestamp
     uint64_t n_rw = 0, n_pv = 0, n_rec = 0, n_unk = 0, t_idle = 0, t_busy = 0;
+    uint64_t t_read = 0, t_res = 0, t_chain = 0, t_fin = 0;
     uint32_t n = 0;
     for (;; ++n) {
         const uint32_t mbi = n % L.nmb;
@@ -298,6 +302,8 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
             lds_st_rel(&lds[kLdsStampTs], last);
             lds_st_rel(&lds[kLdsCnext], n + 1);
         }
+        const uint64_t tr = wall_clock64();
+        uint64_t tres = tr, tchain = tr;
         const bool valid = snap != kNone;
         uint32_t myts = 0;
         uint32_t nc = 0;
@@ -310,65 +316,98 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
             // request plus the deltas of the lower lanes on its producer chain (the
             // highest lower writer of the item, then that writer's producer, ...).
             // Chains are summed by pointer jumping: ceil(log2(chain)) rounds instead of one
-            // step per writer of a hot account.
+            // step per writer of a hot account.  The lanes exchange through LDS inside
+            // this one wave, so wave-scope fences order the accesses; the HBM gathers of
+            // committed values are issued first and consumed after the chain sums.
+            bool need_hbm_r[R], need_hbm_w[W];
+            int32_t base[R];
             uint32_t prev[W];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const uint32_t pj = (prod[r / 4] >> (8 * (r % 4))) & 0xFFu;
+                const bool root_item = valid && rs[r] != kNone && pj == 0xFFu;
+                base[r] = 0;
+                need_hbm_r[r] = false;
+                if (root_item) {
+                    if (L.exact) base[r] = (int32_t)V.lval[rs[r]];
+                    else if (V.bucket[bucket_of(L, rs[r])] <= tg) base[r] = pval[r];
+                    else need_hbm_r[r] = true;
+                }
+            }
 #pragma unroll
             for (int w = 0; w < W; ++w) {
                 prev[w] = kNone;
-                int32_t dv;
-                if (valid && wa[w] != kNone) committed_of(c, L, V, wa[w], tg, 0, pst[w], dv, prev[w]);
+                need_hbm_w[w] = false;
+                if (valid && wa[w] != kNone) {
+                    if (L.exact) prev[w] = V.bucket[wa[w]];
+                    else if (V.bucket[bucket_of(L, wa[w])] <= tg) prev[w] = pst[w];
+                    else need_hbm_w[w] = true;
+                }
             }
-            const int32_t d0 = (wa[1] == kNone) ? 0 : -(int32_t)amt;
-            jdelta[lane * W + 0] = d0;
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                if (need_hbm_r[r])
+                    base[r] = __hip_atomic_load(&c.vals[rs[r]], __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_WORKGROUP);
+#pragma unroll
+            for (int w = 0; w < W; ++w)
+                if (need_hbm_w[w])
+                    prev[w] = __hip_atomic_load(&c.stamps[wa[w]], __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_WORKGROUP);
+            tres = wall_clock64();
+            jdelta[lane * W + 0] = (wa[1] == kNone) ? 0 : -(int32_t)amt;
             jdelta[lane * W + 1] = (int32_t)amt;
+            uint32_t nxt[R];
+            int32_t wgt[R];
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const uint32_t node = lane * R + r;
                 const uint32_t pj = (prod[r / 4] >> (8 * (r % 4))) & 0xFFu;
-                int32_t base = 0;
-                uint32_t dts;
-                if (valid && rs[r] != kNone && pj == 0xFFu)
-                    committed_of(c, L, V, rs[r], tg, pval[r], 0u, base, dts);
-                jbase[node] = base;
-                jnext[node] = (pj == 0xFFu) ? node : ((pj & 63u) * R + (pj >> 6));
+                nxt[r] = (pj == 0xFFu) ? node : ((pj & 63u) * R + (pj >> 6));
+                jnext[node] = nxt[r];
             }
-            wg_fence();
+            wave_fence();
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                const uint32_t node = lane * R + r;
                 const uint32_t pj = (prod[r / 4] >> (8 * (r % 4))) & 0xFFu;
-                jw[node] = (pj == 0xFFu) ? 0 : jdelta[(pj & 63u) * W + (pj >> 6)];
+                wgt[r] = (pj == 0xFFu) ? 0 : jdelta[(pj & 63u) * W + (pj >> 6)];
+                jw[lane * R + r] = wgt[r];
             }
-            wg_fence();
+            wave_fence();
             for (uint32_t round = 0; round < 6; ++round) {
+                bool more = false;
                 uint32_t nn[R];
                 int32_t ww[R];
-                bool more = false;
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     const uint32_t node = lane * R + r;
-                    const uint32_t t = jnext[node];
+                    const uint32_t t = nxt[r];
                     const uint32_t tt = jnext[t];
-                    ww[r] = jw[node] + (t != node ? jw[t] : 0);
+                    ww[r] = wgt[r] + (t != node ? jw[t] : 0);
                     nn[r] = tt;
                     more |= tt != t;
                 }
-                wg_fence();
+                wave_fence();
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
-                    jw[lane * R + r] = ww[r];
+                    nxt[r] = nn[r];
+                    wgt[r] = ww[r];
                     jnext[lane * R + r] = nn[r];
+                    jw[lane * R + r] = ww[r];
                 }
-                wg_fence();
+                wave_fence();
                 if (ballot(more) == 0) break;
             }
+            // every node now points at its chain's root; the root's base is the value
+            // the chain starts from
+#pragma unroll
+            for (int r = 0; r < R; ++r) jbase[lane * R + r] = base[r];
+            wave_fence();
             int32_t in[R], rd[R], nv[W];
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const uint32_t node = lane * R + r;
-                in[r] = jbase[jnext[node]] + jw[node];
-            }
+            for (int r = 0; r < R; ++r) in[r] = jbase[nxt[r]] + wgt[r];
             const bool okx = valid;
+            tchain = wall_clock64();
             transfer_exec<R, W>(rs, amt, in, rd, wa, nv);
             const uint64_t committed = ballot(okx);
             nc = (uint32_t)__builtin_popcountll(committed);
@@ -380,7 +419,7 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
             // the request's timestamp range, since the clock moves over whole requests.
 #pragma unroll
             for (int w = 0; w < W; ++w) sup[lane * W + w] = 0;
-            wg_fence();
+            wave_fence();
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const uint32_t pj = (prod[r / 4] >> (8 * (r % 4))) & 0xFFu;
@@ -389,7 +428,7 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
                 for (int w = 0; w < W; ++w) mine_w |= wa[w] == rs[r];
                 if (okx && pj != 0xFFu && mine_w) sup[(pj & 63u) * W + (pj >> 6)] = 1u;
             }
-            wg_fence();
+            wave_fence();
             bool inst[W];
 #pragma unroll
             for (int w = 0; w < W; ++w) inst[w] = okx && wa[w] != kNone && sup[lane * W + w] == 0;
@@ -397,10 +436,9 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
             record_stamps<R, W>(c, L, V, wa, myts, okx);
             if (okx) {
                 c.log_ts[k] = myts;
-                const uint32_t nr = (c.op[k] >> 16) & 0xFFu;
 #pragma unroll
                 for (int r = 0; r < R; ++r)
-                    if ((uint32_t)r < nr) c.log_v[(uint64_t)r * c.n_txns + k] = rd[r];
+                    if (rs[r] != kNone) c.log_v[(uint64_t)r * c.n_txns + k] = rd[r];
             }
             uint64_t *rv = c.reply_v + ((size_t)cw * kWave + lane) * 2 * W;
 #pragma unroll
@@ -546,6 +584,14 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
         const uint64_t t2 = wall_clock64();
         t_idle += t1 - t0;
         t_busy += t2 - t1;
+        t_read += tr - t1;
+        if (delegate) {
+            t_res += tres - tr;
+            t_chain += tchain - tres;
+            t_fin += t2 - tchain;
+        } else {
+            t_res += t2 - tr;
+        }
     }
     if (lane == 0) {
         lds[kLdsCfinal] = n;
@@ -558,6 +604,10 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
         st_rlx(&c.stats[kStatFinalTs], (uint64_t)last);
         fetch_add(&c.prof[kProfCommitIdle], t_idle);
         fetch_add(&c.prof[kProfCommitBusy], t_busy);
+        fetch_add(&c.prof[kProfCommitRead], t_read);
+        fetch_add(&c.prof[kProfCommitResolve], t_res);
+        fetch_add(&c.prof[kProfCommitChain], t_chain);
+        fetch_add(&c.prof[kProfCommitFinish], t_fin);
     }
 }
 

@@ -168,6 +168,10 @@ enum : uint32_t {
     kProfClientReply,      // client waves waiting for replies (sum over waves)
     kProfClientRound,      // client wave rounds (sum over waves)
     kProfClientRounds,     // number of client rounds
+    kProfCommitRead,       // committer: mailbox to registers + publication of progress
+    kProfCommitResolve,    // committer: committed values / validation
+    kProfCommitChain,      // committer: serialized-lane chain sums
+    kProfCommitFinish,     // committer: timestamps, tables, replies, log
     kProfCount
 };
 
