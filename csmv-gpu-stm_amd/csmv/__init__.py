@@ -61,6 +61,13 @@ class BankSpec(C.Structure):
                 ("amount_max", C.c_uint32), ("reserved", C.c_uint32), ("theta", C.c_double)]
 
 
+class MemcdSpec(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("n_reqs", C.c_uint64), ("n_keys", C.c_uint64),
+                ("n_sets", C.c_uint32), ("n_ways", C.c_uint32), ("pct_get", C.c_uint32),
+                ("batch", C.c_uint32), ("dist", C.c_uint32), ("reserved", C.c_uint32),
+                ("theta", C.c_double)]
+
+
 class Stats(C.Structure):
     _fields_ = [("commits_update", C.c_uint64), ("commits_ro", C.c_uint64),
                 ("aborts_dataage", C.c_uint64), ("aborts_readwrite", C.c_uint64),
@@ -114,6 +121,11 @@ def lib() -> C.CDLL:
         "csmv_bank_route": ([P, P, SZ], C.c_int),
         "csmv_bank_read_log": ([P, P, P, P], C.c_int),
         "csmv_atomic_rate": ([P, U32, U32, C.POINTER(C.c_double)], C.c_int),
+        "csmv_memcd_generate": ([P, C.POINTER(MemcdSpec)], C.c_int),
+        "csmv_memcd_read_stream": ([P, P, P], C.c_int),
+        "csmv_memcd_run": ([P, C.POINTER(Stats)], C.c_int),
+        "csmv_memcd_read_log": ([P, P, P, P, P, P], C.c_int),
+        "csmv_wallclock_khz": ([P, C.POINTER(U32)], C.c_int),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(L, name)
@@ -156,6 +168,28 @@ class Bank:
     @property
     def rmax(self) -> int:
         return max(self.n_reads, self.n_reads_ro)
+
+
+@dataclasses.dataclass
+class Memcd:
+    """The memcached-style cache workload (memcachedGPU/memcdSH.cu:299-448)."""
+    n_reqs: int
+    seed: int = 1
+    n_keys: int = 10_000_000
+    n_sets: int = 1 << 17
+    n_ways: int = 8
+    pct_get: int = 95
+    batch: int = 65536
+    zipf: bool = True
+    theta: float = 0.99
+
+    def spec(self) -> MemcdSpec:
+        return MemcdSpec(self.seed, self.n_reqs, self.n_keys, self.n_sets, self.n_ways,
+                         self.pct_get, self.batch, 1 if self.zipf else 0, 0, self.theta)
+
+    @property
+    def n_items(self) -> int:
+        return 12 * self.n_sets * self.n_ways
 
 
 class Engine:
@@ -242,6 +276,34 @@ class Engine:
         ab = np.empty(b.n_txns, dtype=np.uint32)
         self._chk(lib().csmv_bank_read_log(self._h, _ptr(ts), _ptr(v), _ptr(ab)), "read_log")
         return ts, v, ab
+
+    # ---- memcached-style cache ----------------------------------------------------------
+    def memcd_generate(self, m: Memcd):
+        self._chk(lib().csmv_memcd_generate(self._h, C.byref(m.spec())), "memcd_generate")
+        self.memcd = m
+
+    def memcd_read_stream(self):
+        m = self.memcd
+        key = np.empty(m.n_reqs, dtype=np.uint64)
+        op = np.empty(m.n_reqs, dtype=np.uint32)
+        self._chk(lib().csmv_memcd_read_stream(self._h, _ptr(key), _ptr(op)), "memcd_read_stream")
+        return key, op
+
+    def memcd_run(self) -> dict:
+        st = Stats()
+        self._chk(lib().csmv_memcd_run(self._h, C.byref(st)), "memcd_run")
+        return st.as_dict()
+
+    def memcd_read_log(self):
+        n = self.memcd.n_reqs
+        out = [np.empty(n, dtype=np.uint32) for _ in range(5)]
+        self._chk(lib().csmv_memcd_read_log(self._h, *[_ptr(a) for a in out]), "memcd_read_log")
+        return dict(zip(("ts", "res", "vsum", "aborts", "lat_ticks"), out))
+
+    def wallclock_khz(self) -> int:
+        k = C.c_uint32()
+        self._chk(lib().csmv_wallclock_khz(self._h, C.byref(k)), "wallclock_khz")
+        return k.value
 
     def atomic_rate(self, waves: int = 4096, iters: int = 256) -> float:
         r = C.c_double()

@@ -17,6 +17,7 @@
 #include "csmv.h"
 #include "csmv_device.hpp"
 #include "server.hpp"
+#include "memcd.hpp"
 
 namespace csmv {
 
@@ -331,6 +332,41 @@ __global__ void __launch_bounds__(512) bank_kernel(Ctx c, ServerLayout L)
     else bank_client<R, W>(c, cw);
 }
 
+struct McGenArgs {
+    uint64_t seed, n_reqs, n_keys;
+    uint32_t get_thr, dist;
+    const uint32_t *cdf;
+    uint64_t *key;
+    uint32_t *op;
+};
+
+// Request k: GET/SET as the driver's rnd > prRead test (memcachedGPU/memcdSH.cu:335-343),
+// then a key rank from the Zipf table over the keyspace and a 64-bit key of that rank.
+__global__ void __launch_bounds__(256) memcd_generate_kernel(McGenArgs g)
+{
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= g.n_reqs) return;
+    uint64_t st = splitmix64(g.seed + k);
+    const uint32_t rnd = rand_r_fnc(st) & 0xFFFFu;
+    g.op[k] = rnd < g.get_thr ? kMcGet : kMcSet;
+    const uint32_t r = rand_r_fnc(st);
+    const uint32_t rank = g.dist == 1 ? zipf_rank(g.cdf, (uint32_t)g.n_keys, r)
+                                      : (uint32_t)(r % g.n_keys);
+    g.key[k] = splitmix64(0x6D656D6364ull ^ ((uint64_t)rank << 1));
+}
+
+template <int R, int W>
+__global__ void __launch_bounds__(512) memcd_kernel(Ctx c, ServerLayout L)
+{
+    extern __shared__ uint32_t lds[];
+    if (blockIdx.x == 0) {
+        server_workgroup<R, W>(c, L, lds);
+        return;
+    }
+    const uint32_t cw = (blockIdx.x - 1) * c.wg_waves + threadIdx.x / kWave;
+    if (cw < c.n_client_waves) memcd_client<R, W>(c, cw);
+}
+
 // Same-address agent-scope atomic rate: lane 0 of every wave adds to one word.
 __global__ void __launch_bounds__(256) atomic_rate_kernel(uint64_t *p, uint32_t iters)
 {
@@ -369,6 +405,14 @@ struct csmv_engine {
     uint32_t *norm_list = nullptr, *hot_list = nullptr;
     uint64_t n_norm = 0, n_hot = 0;
     bool routed = false;
+    // memcached-style cache stream and log
+    csmv_memcd_spec mspec{};
+    bool has_mc = false;
+    uint64_t mc_cap = 0;
+    uint64_t *mc_key = nullptr;
+    uint32_t *mc_op = nullptr, *mc_cdf = nullptr;
+    uint32_t *mc_ts = nullptr, *mc_res = nullptr, *mc_vsum = nullptr, *mc_ab = nullptr,
+             *mc_lat = nullptr;
     // bank stream + log
     csmv_bank_spec spec{};
     bool has_stream = false;
@@ -433,26 +477,28 @@ uint32_t pow2_floor(uint64_t v)
 }
 
 // Server LDS layout for a request wire of R reads and W writes.
-template <int R, int W> ServerLayout server_layout(const csmv_engine *e)
+template <int R, int W> ServerLayout server_layout(const csmv_engine *e, bool delegation)
 {
     ServerLayout L{};
     const uint32_t vmode = e->cfg.variant & 3u;
     const uint32_t M = e->cfg.wg_waves - 2;
     L.nmb = std::min<uint32_t>(std::max<uint32_t>(M + 2, 4), 16);
-    L.mb_rows = Rows<R, W>::count;
+    L.mb_rows = delegation ? Rows<R, W>::count : Rows<R, W>::pref + 1;
     L.state_off = kLdsHdr;
     L.wave_off = L.state_off + L.nmb;
     L.tg_off = L.wave_off + L.nmb;
     L.mb_off = (L.tg_off + L.nmb + 63) & ~63u;
     uint32_t w = L.mb_off + L.nmb * L.mb_rows * kWave;
-    // every item a request inserts (W writes, plus R reads when delegated) fits
-    L.hsz = pow2_at_least((uint64_t)kWave * (R + W));
+    // every item a request inserts (W writes, plus R reads when delegated) fits at a
+    // load factor of at most 3/4
+    L.hsz = delegation ? pow2_at_least((uint64_t)kWave * (R + W))
+                       : pow2_at_least(2ull * kWave * W);
     L.hash_off = w;
     w += M * 5 * L.hsz;
     L.sup_off = w;
     w += kWave * W;
     L.jmp_off = w;
-    w += 3 * kWave * R + kWave * W;
+    if (delegation) w += 3 * kWave * R + kWave * W;
     L.atr_off = w;
     if (vmode != kValStamp) w += W * e->cfg.atr_entries;
     L.bucket_off = w;
@@ -565,7 +611,9 @@ csmv_status csmv_create(const csmv_config *cfg_in, csmv_engine **out)
         csmv_destroy(e);
         return CSMV_E_HIP;
     }
-    const uint32_t rows = mbox_rows<kBankR, kBankW>();
+    // request mailboxes sized for the widest wire of the drivers
+    const uint32_t rows = std::max(mbox_rows<kBankR, kBankW>(), mbox_rows<kMcR, kMcW>());
+    static_assert(kMcW <= kBankW, "reply values are sized for the bank wire");
     if ((s = dalloc(e, &e->ht, cfg.n_items)) ||
         (s = dalloc(e, &e->slot, (size_t)cfg.n_items * cfg.depth)) ||
         (s = dalloc(e, &e->ctl, kCtlBytes)) || (s = dalloc(e, &e->ring, e->Q)) ||
@@ -589,6 +637,8 @@ csmv_status csmv_destroy(csmv_engine *e)
     dfree(e->ht); dfree(e->slot); dfree(e->ctl); dfree(e->ring); dfree(e->done);
     dfree(e->reply); dfree(e->mbox); dfree(e->stamps); dfree(e->tmp); dfree(e->vals);
     dfree(e->reply_v); dfree(e->norm_list); dfree(e->hot_list);
+    dfree(e->mc_key); dfree(e->mc_op); dfree(e->mc_cdf); dfree(e->mc_ts); dfree(e->mc_res);
+    dfree(e->mc_vsum); dfree(e->mc_ab); dfree(e->mc_lat);
     dfree(e->op); dfree(e->addr); dfree(e->cdf); dfree(e->log_ts); dfree(e->log_v);
     dfree(e->log_ab);
     if (e->ev0) (void)hipEventDestroy(e->ev0);
@@ -741,30 +791,10 @@ csmv_status csmv_bank_read_stream(csmv_engine *e, uint32_t *op, uint32_t *addr)
     return CSMV_OK;
 }
 
-csmv_status csmv_bank_run(csmv_engine *e, csmv_stats *out)
-{
-    if (!e) return CSMV_E_ARG;
-    if (!e->has_stream) return fail(e, CSMV_E_STATE, "run before generate");
-    if (!e->pool_loaded) return fail(e, CSMV_E_STATE, "run needs a freshly loaded pool");
-    const ServerLayout L = server_layout<kBankR, kBankW>(e);
-    const size_t lds_bytes = (size_t)L.words * 4;
-    if (lds_bytes > e->lds_max)
-        return fail(e, CSMV_E_CAPACITY, "server LDS layout of " + std::to_string(lds_bytes) +
-                                            " bytes exceeds the budget (reduce atr_entries)");
-    const uint32_t block = e->cfg.wg_waves * kWave;
-    const uint32_t client_wgs = (e->cfg.client_waves + e->cfg.wg_waves - 1) / e->cfg.wg_waves;
-    const uint32_t grid = 1 + client_wgs;
-    auto kern = bank_kernel<kBankR, kBankW>;
-    HIPCHK(e, hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)lds_bytes));
-    int per_cu = 0;
-    HIPCHK(e, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, (int)block, lds_bytes));
-    if ((uint64_t)per_cu * e->n_cu < grid)
-        return fail(e, CSMV_E_CAPACITY,
-                    "grid of " + std::to_string(grid) + " workgroups exceeds the " +
-                        std::to_string((uint64_t)per_cu * e->n_cu) + " the device keeps resident");
+}  // extern "C"
 
-    Ctx c{};
+static void base_ctx(csmv_engine *e, Ctx &c)
+{
     c.ht = e->ht;
     c.slot = e->slot;
     c.n_items = e->cfg.n_items;
@@ -793,22 +823,30 @@ csmv_status csmv_bank_run(csmv_engine *e, csmv_stats *out)
     c.begin_wait = e->cfg.begin_wait;
     c.timeout_ticks = (uint64_t)e->cfg.timeout_ms * e->wallclock_khz;
     c.max_attempts = e->cfg.max_attempts;
-    c.op = e->op;
-    c.addr = e->addr;
-    c.n_txns = e->spec.n_txns;
-    c.log_ts = e->log_ts;
-    c.log_v = e->log_v;
-    c.log_ab = e->log_ab;
-    c.norm_list = e->routed ? e->norm_list : nullptr;
-    c.hot_list = e->routed ? e->hot_list : nullptr;
-    c.n_norm = e->routed ? e->n_norm : e->spec.n_txns;
-    c.n_hot = e->routed ? e->n_hot : 0;
     c.next_hot = (uint64_t *)(e->ctl + kOffNextHot);
-    if (c.n_hot && (c.n_serial_waves == 0 || (e->cfg.variant & 3u) != kValStamp ||
-                    (e->cfg.variant & kServerWriteback)))
-        return fail(e, CSMV_E_ARG, "the serialized lane needs n_serial_waves > 0 and the "
-                                   "STAMP variant with client write-back");
+}
 
+// Launches one persistent kernel (server workgroup 0 + client workgroups) over the
+// prepared context and collects its statistics.
+template <typename K>
+static csmv_status launch_persistent(csmv_engine *e, K kern, const Ctx &c, const ServerLayout &L,
+                                     csmv_stats *out)
+{
+    const size_t lds_bytes = (size_t)L.words * 4;
+    if (lds_bytes > e->lds_max)
+        return fail(e, CSMV_E_CAPACITY, "server LDS layout of " + std::to_string(lds_bytes) +
+                                            " bytes exceeds the budget (reduce atr_entries)");
+    const uint32_t block = e->cfg.wg_waves * kWave;
+    const uint32_t client_wgs = (e->cfg.client_waves + e->cfg.wg_waves - 1) / e->cfg.wg_waves;
+    const uint32_t grid = 1 + client_wgs;
+    HIPCHK(e, hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds_bytes));
+    int per_cu = 0;
+    HIPCHK(e, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, (int)block, lds_bytes));
+    if ((uint64_t)per_cu * e->n_cu < grid)
+        return fail(e, CSMV_E_CAPACITY,
+                    "grid of " + std::to_string(grid) + " workgroups exceeds the " +
+                        std::to_string((uint64_t)per_cu * e->n_cu) + " the device keeps resident");
     HIPCHK(e, hipMemsetAsync(e->ctl, 0, kCtlBytes, e->stream));
     HIPCHK(e, hipMemsetAsync(e->ring, 0, e->Q * 8, e->stream));
     HIPCHK(e, hipMemsetAsync(e->done, 0, e->Q * 8, e->stream));
@@ -858,6 +896,143 @@ csmv_status csmv_bank_run(csmv_engine *e, csmv_stats *out)
     if (derr == kErrLivelock) return fail(e, CSMV_E_LIVELOCK, "transaction attempt limit");
     if (derr == kErrReadSetOverflow) return fail(e, CSMV_E_CAPACITY, "read/write set overflow");
     if (derr) return fail(e, CSMV_E_TIMEOUT, "device wait timed out, code " + std::to_string(derr));
+    return CSMV_OK;
+}
+
+extern "C" {
+
+csmv_status csmv_bank_run(csmv_engine *e, csmv_stats *out)
+{
+    if (!e) return CSMV_E_ARG;
+    if (!e->has_stream) return fail(e, CSMV_E_STATE, "run before generate");
+    if (!e->pool_loaded) return fail(e, CSMV_E_STATE, "run needs a freshly loaded pool");
+    const ServerLayout L = server_layout<kBankR, kBankW>(e, true);
+    Ctx c{};
+    base_ctx(e, c);
+    c.op = e->op;
+    c.addr = e->addr;
+    c.n_txns = e->spec.n_txns;
+    c.log_ts = e->log_ts;
+    c.log_v = e->log_v;
+    c.log_ab = e->log_ab;
+    c.norm_list = e->routed ? e->norm_list : nullptr;
+    c.hot_list = e->routed ? e->hot_list : nullptr;
+    c.n_norm = e->routed ? e->n_norm : e->spec.n_txns;
+    c.n_hot = e->routed ? e->n_hot : 0;
+    if (c.n_hot && (c.n_serial_waves == 0 || (e->cfg.variant & 3u) != kValStamp ||
+                    (e->cfg.variant & kServerWriteback)))
+        return fail(e, CSMV_E_ARG, "the serialized lane needs n_serial_waves > 0 and the "
+                                   "STAMP variant with client write-back");
+    return launch_persistent(e, bank_kernel<kBankR, kBankW>, c, L, out);
+}
+
+// ---- memcached-style cache ---------------------------------------------------------------
+csmv_status csmv_memcd_generate(csmv_engine *e, const csmv_memcd_spec *s)
+{
+    if (!e || !s) return CSMV_E_ARG;
+    if (s->n_reqs == 0 || s->n_reqs >= (1ull << 31) || s->n_keys == 0 ||
+        s->n_keys >= (1ull << 31) || s->n_sets == 0 || (s->n_sets & (s->n_sets - 1)) ||
+        s->n_ways == 0 || s->n_ways > 8 || s->pct_get > 100 || s->batch == 0 || s->dist > 1 ||
+        (s->dist == 1 && !(s->theta >= 0.0)))
+        return fail(e, CSMV_E_ARG, "memcd spec out of range");
+    if ((uint64_t)kMcFields * s->n_sets * s->n_ways != e->cfg.n_items)
+        return fail(e, CSMV_E_ARG, "memcd: the engine needs n_items = 12 * n_sets * n_ways");
+    const uint64_t T = s->n_reqs;
+    if (T > e->mc_cap) {
+        dfree(e->mc_key); dfree(e->mc_op); dfree(e->mc_ts); dfree(e->mc_res); dfree(e->mc_vsum);
+        dfree(e->mc_ab); dfree(e->mc_lat);
+        e->mc_cap = 0;
+        csmv_status st;
+        if ((st = dalloc(e, &e->mc_key, T)) || (st = dalloc(e, &e->mc_op, T)) ||
+            (st = dalloc(e, &e->mc_ts, T)) || (st = dalloc(e, &e->mc_res, T)) ||
+            (st = dalloc(e, &e->mc_vsum, T)) || (st = dalloc(e, &e->mc_ab, T)) ||
+            (st = dalloc(e, &e->mc_lat, T)))
+            return st;
+        e->mc_cap = T;
+    }
+    dfree(e->mc_cdf);
+    if (s->dist == 1) {
+        uint32_t *h = new (std::nothrow) uint32_t[s->n_keys];
+        if (!h) return fail(e, CSMV_E_CAPACITY, "cdf host buffer");
+        csmv_zipf_cdf((uint32_t)s->n_keys, s->theta, h);
+        csmv_status st = dalloc(e, &e->mc_cdf, s->n_keys);
+        if (st) { delete[] h; return st; }
+        hipError_t r = hipMemcpy(e->mc_cdf, h, s->n_keys * 4, hipMemcpyHostToDevice);
+        delete[] h;
+        HIPCHK(e, r);
+    }
+    McGenArgs g{};
+    g.seed = s->seed;
+    g.n_reqs = T;
+    g.n_keys = s->n_keys;
+    g.get_thr = (uint32_t)((uint64_t)s->pct_get * 0xFFFFu / 100u);
+    g.dist = s->dist;
+    g.cdf = e->mc_cdf;
+    g.key = e->mc_key;
+    g.op = e->mc_op;
+    hipLaunchKernelGGL(memcd_generate_kernel, dim3((unsigned)((T + 255) / 256)), dim3(256), 0,
+                       e->stream, g);
+    HIPCHK(e, hipGetLastError());
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    e->mspec = *s;
+    e->has_mc = true;
+    return CSMV_OK;
+}
+
+csmv_status csmv_memcd_read_stream(csmv_engine *e, uint64_t *key, uint32_t *op)
+{
+    if (!e || !key || !op) return CSMV_E_ARG;
+    if (!e->has_mc) return fail(e, CSMV_E_STATE, "no memcd stream");
+    HIPCHK(e, hipMemcpy(key, e->mc_key, e->mspec.n_reqs * 8, hipMemcpyDeviceToHost));
+    HIPCHK(e, hipMemcpy(op, e->mc_op, e->mspec.n_reqs * 4, hipMemcpyDeviceToHost));
+    return CSMV_OK;
+}
+
+csmv_status csmv_memcd_run(csmv_engine *e, csmv_stats *out)
+{
+    if (!e) return CSMV_E_ARG;
+    if (!e->has_mc) return fail(e, CSMV_E_STATE, "run before generate");
+    if (!e->pool_loaded) return fail(e, CSMV_E_STATE, "run needs a freshly loaded pool");
+    if ((e->cfg.variant & 3u) != kValStamp || (e->cfg.variant & kServerWriteback))
+        return fail(e, CSMV_E_ARG, "the cache driver runs the STAMP variant with client write-back");
+    const ServerLayout L = server_layout<kMcR, kMcW>(e, false);
+    Ctx c{};
+    base_ctx(e, c);
+    c.n_serial_waves = 0;
+    c.n_txns = e->mspec.n_reqs;
+    c.req_key = e->mc_key;
+    c.req_op = e->mc_op;
+    c.n_sets = e->mspec.n_sets;
+    c.n_ways = e->mspec.n_ways;
+    c.n_slots = e->mspec.n_sets * e->mspec.n_ways;
+    c.mc_batch = e->mspec.batch;
+    c.log_ts = e->mc_ts;
+    c.log_res = e->mc_res;
+    c.log_vsum = e->mc_vsum;
+    c.log_ab = e->mc_ab;
+    c.log_lat = e->mc_lat;
+    c.n_norm = c.n_txns;
+    return launch_persistent(e, memcd_kernel<kMcR, kMcW>, c, L, out);
+}
+
+csmv_status csmv_memcd_read_log(csmv_engine *e, uint32_t *ts, uint32_t *res, uint32_t *vsum,
+                                uint32_t *aborts, uint32_t *lat_ticks)
+{
+    if (!e) return CSMV_E_ARG;
+    if (!e->has_mc) return fail(e, CSMV_E_STATE, "no memcd stream");
+    const uint64_t T = e->mspec.n_reqs;
+    if (ts) HIPCHK(e, hipMemcpy(ts, e->mc_ts, T * 4, hipMemcpyDeviceToHost));
+    if (res) HIPCHK(e, hipMemcpy(res, e->mc_res, T * 4, hipMemcpyDeviceToHost));
+    if (vsum) HIPCHK(e, hipMemcpy(vsum, e->mc_vsum, T * 4, hipMemcpyDeviceToHost));
+    if (aborts) HIPCHK(e, hipMemcpy(aborts, e->mc_ab, T * 4, hipMemcpyDeviceToHost));
+    if (lat_ticks) HIPCHK(e, hipMemcpy(lat_ticks, e->mc_lat, T * 4, hipMemcpyDeviceToHost));
+    return CSMV_OK;
+}
+
+csmv_status csmv_wallclock_khz(csmv_engine *e, uint32_t *khz)
+{
+    if (!e || !khz) return CSMV_E_ARG;
+    *khz = e->wallclock_khz;
     return CSMV_OK;
 }
 

@@ -449,15 +449,28 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
         } else {
             bool rec = false, conflict = false;
             if (vmode == kValStamp) {
-                if (valid) {
-                    const uint32_t thr = L.exact ? snap : tg;
-                    conflict = pref;
+                if (valid && L.exact) {
 #pragma unroll
                     for (int r = 0; r < R; ++r)
-                        if (rs[r] != kNone) conflict |= V.bucket[bucket_of(L, rs[r])] > thr;
+                        if (rs[r] != kNone) conflict |= V.bucket[rs[r]] > snap;
 #pragma unroll
                     for (int w = 0; w < W; ++w)
-                        if (wa[w] != kNone) conflict |= V.bucket[bucket_of(L, wa[w])] > thr;
+                        if (wa[w] != kNone) conflict |= V.bucket[wa[w]] > snap;
+                } else if (valid) {
+                    // stamps <= T_g were checked at staging (pref); an item whose bucket
+                    // was written since is checked against its exact stamp in HBM
+                    conflict = pref;
+                    uint32_t ex[R + W];
+#pragma unroll
+                    for (int r = 0; r < R + W; ++r) {
+                        const uint32_t x = r < R ? rs[r] : wa[r - R];
+                        ex[r] = 0;
+                        if (x != kNone && V.bucket[bucket_of(L, x)] > tg)
+                            ex[r] = __hip_atomic_load(&c.stamps[x], __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_WORKGROUP);
+                    }
+#pragma unroll
+                    for (int r = 0; r < R + W; ++r) conflict |= ex[r] > snap;
                 }
             } else {
                 // record ring indexed by timestamp: entry t lives at t mod E and the ring

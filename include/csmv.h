@@ -151,6 +151,37 @@ csmv_status csmv_bank_run(csmv_engine *e, csmv_stats *stats);
 csmv_status csmv_bank_read_log(csmv_engine *e, uint32_t *ts, int32_t *v,
                                uint32_t *aborts);
 
+/* Memcached-style cache (memcachedGPU/memcdSH.cu:486-768): n_sets x n_ways slots of 12
+ * int32 items (field-major: state, key low, key high, LRU timestamp, 8 value words), so
+ * the engine must be created with n_items = 12 * n_sets * n_ways; the final cache
+ * contents are csmv_read_state_i32. */
+typedef struct {
+    uint64_t seed;
+    uint64_t n_reqs;
+    uint64_t n_keys;       /* keyspace (< 2^31) */
+    uint32_t n_sets;       /* power of two */
+    uint32_t n_ways;       /* 1..8 */
+    uint32_t pct_get;      /* percent GET, rest SET */
+    uint32_t batch;        /* requests per LRU clock tick */
+    uint32_t dist;         /* 0 uniform, 1 Zipf over key ranks */
+    uint32_t reserved;
+    double theta;
+} csmv_memcd_spec;
+
+/* Request stream on the device (the host key generation loop, memcdSH.cu:586-595). */
+csmv_status csmv_memcd_generate(csmv_engine *e, const csmv_memcd_spec *spec);
+csmv_status csmv_memcd_read_stream(csmv_engine *e, uint64_t *key, uint32_t *op);
+/* Serves the whole stream with one persistent kernel (replaces the per-batch memcdTx
+ * launches with a persistent server_kernel, memcdSH.cu:603-651). */
+csmv_status csmv_memcd_run(csmv_engine *e, csmv_stats *stats);
+/* Per request: commit ts (writers) or snapshot (read-only), result word (bit 0 hit,
+ * bit 1 wrote, bits 4..7 way), GET value checksum, failed attempts, latency in
+ * wall-clock ticks from pick-up to commit. */
+csmv_status csmv_memcd_read_log(csmv_engine *e, uint32_t *ts, uint32_t *res, uint32_t *vsum,
+                                uint32_t *aborts, uint32_t *lat_ticks);
+/* Rate of the device wall clock used for latencies (kHz). */
+csmv_status csmv_wallclock_khz(csmv_engine *e, uint32_t *khz);
+
 /* Same-address device-scope atomic rate (atomics per second), measured by a
  * microbenchmark kernel: the ceiling unit for the per-lane commit protocol. */
 csmv_status csmv_atomic_rate(csmv_engine *e, uint32_t waves, uint32_t iters,

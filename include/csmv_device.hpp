@@ -139,6 +139,13 @@ struct Ctx {
     const uint32_t *hot_list;
     uint64_t n_norm, n_hot;
     uint64_t *next_hot;
+    // memcached-style cache: request stream and results
+    const uint64_t *req_key;
+    const uint32_t *req_op;
+    uint32_t n_sets, n_ways, n_slots, mc_batch;
+    uint32_t *log_res;     // bit 0 hit / key present, bit 1 wrote, bits 4..7 way
+    uint32_t *log_vsum;    // GET hit: value checksum
+    uint32_t *log_lat;     // pick-up to commit, wall-clock ticks
     uint32_t *log_ts;
     int32_t *log_v;
     uint32_t *log_ab;
@@ -365,9 +372,16 @@ template <int R, int W> constexpr uint32_t mbox_rows() { return 1 + R + 2 * W; }
 // With delegate=true the request carries transactions for the serialized lane: the
 // server executes them (rs = operands, wv = transaction id and amount) and returns the
 // written values, which replace tx.wv before the client installs them.
-template <int R, int W>
+//
+// A driver whose conflicts are covered by a subset of its writes may send only that
+// subset and install the full write set: install_n >= 0 selects install_wa / install_wv
+// (install_n entries) instead of the transaction's own write set.
+template <int R, int W, int WI = W>
 __device__ __forceinline__ uint32_t tx_commit(const Ctx &c, uint32_t cw, Tx<R, W> &tx, bool req, bool *fail,
-                                               uint64_t *wait_ticks, bool delegate = false)
+                                               uint64_t *wait_ticks, bool delegate = false,
+                                               const uint32_t *install_wa = nullptr,
+                                               const int32_t *install_wv = nullptr,
+                                               int install_n = -1)
 {
     const uint32_t lane = lane_id();
     constexpr uint32_t rows = mbox_rows<R, W>();
@@ -432,7 +446,8 @@ __device__ __forceinline__ uint32_t tx_commit(const Ctx &c, uint32_t cw, Tx<R, W
     if ((c.variant & kServerWriteback) == 0) {
         const uint64_t cm = ballot(ts != 0);
         if (cm) {
-            writeback<R, W>(c, tx.wa, tx.wv, tx.nw, ts);
+            if (install_n >= 0) writeback<R, WI>(c, install_wa, install_wv, (uint32_t)install_n, ts);
+            else writeback<R, W>(c, tx.wa, tx.wv, tx.nw, ts);
             const uint32_t last_lane = 63u - (uint32_t)__builtin_clzll(cm);
             const uint32_t end_ts = readlane(ts, last_lane);
             fence_rel();

@@ -163,3 +163,75 @@ def bank_cpu_csmv(spec, op, addr, init, n_run=None, depth=4, atr_entries=4096, n
                                  C.byref(cfg), _p(state), _p(log_ts), _p(log_v), C.byref(st))
     assert rc == 0, rc
     return state, log_ts, log_v, st.as_dict()
+
+
+class MemcdSpec(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("n_reqs", C.c_uint64), ("n_keys", C.c_uint64),
+                ("n_sets", C.c_uint32), ("n_ways", C.c_uint32), ("pct_get", C.c_uint32),
+                ("batch", C.c_uint32), ("dist", C.c_uint32), ("reserved", C.c_uint32),
+                ("theta", C.c_double)]
+
+
+class MemcdReport(C.Structure):
+    _fields_ = [(f, C.c_uint64) for f in ("n_updates", "n_readonly", "hits", "misses", "sets",
+                                          "bad_ts", "bad_snapshot", "bad_result", "bad_kind",
+                                          "bad_final")] + [("first_bad_req", C.c_int64)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+def _memcd_lib():
+    L = lib()
+    if not getattr(L, "_memcd_ready", False):
+        P = C.c_void_p
+        L.orc_memcd_key.argtypes = [C.c_uint32]
+        L.orc_memcd_key.restype = C.c_uint64
+        L.orc_memcd_set_of.argtypes = [C.c_uint64, C.c_uint32]
+        L.orc_memcd_set_of.restype = C.c_uint32
+        L.orc_memcd_value_word.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32]
+        L.orc_memcd_value_word.restype = C.c_uint32
+        L.orc_memcd_gen.argtypes = [C.POINTER(MemcdSpec), P, P, P]
+        L.orc_memcd_serial.argtypes = [C.POINTER(MemcdSpec), P, P, P, P, P, P, P]
+        L.orc_memcd_replay.argtypes = [C.POINTER(MemcdSpec), P, P, P, P, P, P,
+                                       C.POINTER(MemcdReport)]
+        L.orc_memcd_replay.restype = C.c_int64
+        L._memcd_ready = True
+    return L
+
+
+def memcd_spec(n_reqs, seed=1, n_keys=10_000_000, n_sets=1 << 17, n_ways=8, pct_get=95,
+               batch=65536, zipf=True, theta=0.99) -> MemcdSpec:
+    return MemcdSpec(seed, n_reqs, n_keys, n_sets, n_ways, pct_get, batch, 1 if zipf else 0, 0,
+                     theta)
+
+
+def memcd_gen(spec: MemcdSpec):
+    L = _memcd_lib()
+    cdf = zipf_cdf(spec.n_keys, spec.theta) if spec.dist == 1 else None
+    key = np.empty(spec.n_reqs, dtype=np.uint64)
+    op = np.empty(spec.n_reqs, dtype=np.uint32)
+    assert L.orc_memcd_gen(C.byref(spec), _p(cdf), _p(key), _p(op)) == 0
+    return key, op
+
+
+def memcd_serial(spec, key, op, order=None):
+    L = _memcd_lib()
+    items = np.empty(12 * spec.n_sets * spec.n_ways, dtype=np.int32)
+    ts = np.zeros(spec.n_reqs, dtype=np.uint32)
+    res = np.zeros(spec.n_reqs, dtype=np.uint32)
+    vsum = np.zeros(spec.n_reqs, dtype=np.uint32)
+    o = None if order is None else np.ascontiguousarray(order, dtype=np.uint64)
+    assert L.orc_memcd_serial(C.byref(spec), _p(key), _p(op), _p(o), _p(items), _p(ts), _p(res),
+                              _p(vsum)) == 0
+    return items, ts, res, vsum
+
+
+def memcd_replay(spec, key, op, ts, res, vsum, final=None):
+    L = _memcd_lib()
+    rep = MemcdReport()
+    fails = L.orc_memcd_replay(C.byref(spec), _p(key), _p(op), _p(ts), _p(res), _p(vsum),
+                               _p(final), C.byref(rep))
+    d = rep.as_dict()
+    d["fails"] = fails
+    return d
