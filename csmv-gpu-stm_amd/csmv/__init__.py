@@ -11,6 +11,9 @@ Reference interfaces mirrored here (file:line in DMRNunes/csmv-gpu-stm):
   Engine.read_state         TXEnd read-back       bank/csmv_sh/API.cu:337-348
   Engine.bank_run           test_fine_grain_offloading   bank/csmv_sh/exe.cu:340-529
   Engine.close              TXEnd frees + free_gbc       bank/csmv_sh/API.cu:354-358
+  Engine.abort_fold ...     per-bucket abort log  (the reference keeps per-class totals only,
+                            Statistics, bank/csmv_sh/API.cuh:112-119)
+  Engine.pred_* / bank_route_learned   learned contention predictor: no reference counterpart
 """
 from __future__ import annotations
 
@@ -39,6 +42,11 @@ VARIANTS = {
 STATUS = {0: "ok", 1: "HIP runtime error", 2: "invalid argument", 3: "capacity exceeded",
           4: "device wait timed out", 5: "no gfx950 device", 6: "call out of order",
           7: "transaction attempt limit exceeded"}
+
+
+PRED_PARAMS = 4225      # W1[64][64] f-major, b1[64], w2[64], b2
+PRED_FEATURES = 64
+ABORT_BUCKETS = 4096
 
 
 class CsmvError(RuntimeError):
@@ -126,6 +134,21 @@ def lib() -> C.CDLL:
         "csmv_memcd_run": ([P, C.POINTER(Stats)], C.c_int),
         "csmv_memcd_read_log": ([P, P, P, P, P, P], C.c_int),
         "csmv_wallclock_khz": ([P, C.POINTER(U32)], C.c_int),
+        "csmv_pred_init": ([P, U64], C.c_int),
+        "csmv_pred_set_params": ([P, P, SZ], C.c_int),
+        "csmv_pred_get_params": ([P, P, SZ], C.c_int),
+        "csmv_abort_fold": ([P, U32], C.c_int),
+        "csmv_abort_read_counters": ([P, P, SZ], C.c_int),
+        "csmv_abort_load_counters": ([P, P, SZ], C.c_int),
+        "csmv_pred_collect": ([P, U64, C.POINTER(U64)], C.c_int),
+        "csmv_pred_read_dataset": ([P, P, P, SZ], C.c_int),
+        "csmv_pred_clear_dataset": ([P], C.c_int),
+        "csmv_pred_train": ([P, U32, C.c_float, P], C.c_int),
+        "csmv_pred_forward": ([P, P, SZ, P], C.c_int),
+        "csmv_pred_grad": ([P, P, P, SZ, P, P], C.c_int),
+        "csmv_pred_infer": ([P, C.c_float, P, SZ], C.c_int),
+        "csmv_bank_route_learned": ([P, C.c_float, C.POINTER(U64)], C.c_int),
+        "csmv_bank_read_route": ([P, P, P, C.POINTER(U64), C.POINTER(U64)], C.c_int),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(L, name)
@@ -309,3 +332,86 @@ class Engine:
         r = C.c_double()
         self._chk(lib().csmv_atomic_rate(self._h, waves, iters, C.byref(r)), "atomic_rate")
         return r.value
+
+    # ---- learned contention predictor ---------------------------------------------------
+    def pred_init(self, seed: int = 1):
+        self._chk(lib().csmv_pred_init(self._h, seed), "pred_init")
+
+    def pred_set_params(self, params: np.ndarray):
+        a = np.ascontiguousarray(params, dtype=np.float32)
+        self._chk(lib().csmv_pred_set_params(self._h, _ptr(a), a.size), "pred_set_params")
+
+    def pred_get_params(self) -> np.ndarray:
+        out = np.empty(PRED_PARAMS, dtype=np.float32)
+        self._chk(lib().csmv_pred_get_params(self._h, _ptr(out), out.size), "pred_get_params")
+        return out
+
+    def abort_fold(self, decay_shift: int = 1):
+        """counters = (counters >> decay_shift) + the last run's abort log."""
+        self._chk(lib().csmv_abort_fold(self._h, decay_shift), "abort_fold")
+
+    def abort_counters(self) -> np.ndarray:
+        out = np.empty(ABORT_BUCKETS, dtype=np.uint32)
+        self._chk(lib().csmv_abort_read_counters(self._h, _ptr(out), out.size), "abort_counters")
+        return out
+
+    def abort_load_counters(self, counters: np.ndarray):
+        a = np.ascontiguousarray(counters, dtype=np.uint32)
+        self._chk(lib().csmv_abort_load_counters(self._h, _ptr(a), a.size), "abort_load")
+
+    def pred_collect(self, max_samples: int = 1 << 62) -> int:
+        n = C.c_uint64()
+        self._chk(lib().csmv_pred_collect(self._h, max_samples, C.byref(n)), "pred_collect")
+        return n.value
+
+    def pred_dataset(self, n: int):
+        X = np.empty((n, PRED_FEATURES), dtype=np.uint16)
+        y = np.empty(n, dtype=np.float32)
+        self._chk(lib().csmv_pred_read_dataset(self._h, _ptr(X), _ptr(y), n), "pred_dataset")
+        return X, y
+
+    def pred_clear_dataset(self):
+        self._chk(lib().csmv_pred_clear_dataset(self._h), "pred_clear_dataset")
+
+    def pred_train(self, steps: int, lr: float = 1e-2) -> np.ndarray:
+        loss = np.empty(steps, dtype=np.float32)
+        self._chk(lib().csmv_pred_train(self._h, steps, lr, _ptr(loss)), "pred_train")
+        return loss
+
+    def pred_forward(self, X: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(X, dtype=np.uint16)
+        n = x.shape[0]
+        p = np.empty(n, dtype=np.float32)
+        self._chk(lib().csmv_pred_forward(self._h, _ptr(x), n, _ptr(p)), "pred_forward")
+        return p
+
+    def pred_grad(self, X: np.ndarray, y: np.ndarray):
+        x = np.ascontiguousarray(X, dtype=np.uint16)
+        yy = np.ascontiguousarray(y, dtype=np.float32)
+        g = np.empty(PRED_PARAMS, dtype=np.float32)
+        loss = C.c_float()
+        self._chk(lib().csmv_pred_grad(self._h, _ptr(x), _ptr(yy), x.shape[0], _ptr(g),
+                                       C.byref(loss)), "pred_grad")
+        return g, loss.value
+
+    def pred_infer(self, tau: float = 0.5) -> np.ndarray:
+        p = np.empty(self.bank.n_txns, dtype=np.float32)
+        self._chk(lib().csmv_pred_infer(self._h, tau, _ptr(p), p.size), "pred_infer")
+        return p
+
+    def bank_route_learned(self, tau: float = 0.5) -> int:
+        """Routes transfers with predicted abort probability above tau to the serialized
+        lane; returns how many."""
+        n = C.c_uint64()
+        self._chk(lib().csmv_bank_route_learned(self._h, tau, C.byref(n)), "route_learned")
+        return n.value
+
+    def bank_read_route(self):
+        nn, nh = C.c_uint64(), C.c_uint64()
+        self._chk(lib().csmv_bank_read_route(self._h, None, None, C.byref(nn), C.byref(nh)),
+                  "read_route")
+        norm = np.empty(nn.value, dtype=np.uint32)
+        hot = np.empty(nh.value, dtype=np.uint32)
+        self._chk(lib().csmv_bank_read_route(self._h, _ptr(norm), _ptr(hot), None, None),
+                  "read_route")
+        return norm, hot

@@ -18,13 +18,13 @@
 #include "csmv_device.hpp"
 #include "server.hpp"
 #include "memcd.hpp"
+#include "mlp.hpp"
 
 namespace csmv {
 
 // Reads of one update transaction on the wire, writes of one transaction.
 constexpr int kBankR = 4;
 constexpr int kBankW = 2;
-constexpr uint32_t kKindTransfer = 0, kKindAudit = 1;
 
 // ---- stream generation ----------------------------------------------------------------------
 __host__ __device__ __forceinline__ uint64_t splitmix64(uint64_t x)
@@ -421,7 +421,26 @@ struct csmv_engine {
     uint32_t *op = nullptr, *addr = nullptr, *cdf = nullptr;
     uint32_t *log_ts = nullptr, *log_ab = nullptr;
     int32_t *log_v = nullptr;
+    bool bank_logged = false;   // log_* hold a completed run of the resident stream
     int32_t *tmp = nullptr;
+    // learned contention predictor (allocated on first use)
+    float *mlp_params = nullptr, *mlp_m = nullptr, *mlp_v = nullptr, *mlp_grad = nullptr,
+          *mlp_part = nullptr, *mlp_loss = nullptr;
+    uint16_t *mlp_w1t = nullptr;
+    uint32_t mlp_step = 0, mlp_loss_cap = 0;
+    uint32_t *abort_ctr = nullptr;
+    uint16_t *ds_x = nullptr;
+    float *ds_y = nullptr;
+    uint64_t ds_n = 0, ds_cap = 0;
+    uint16_t *io_x = nullptr;
+    float *io_y = nullptr, *io_p = nullptr;
+    uint64_t io_cap = 0;
+    // per-transaction scheduler buffers and the routing partition
+    float *pred_p = nullptr;
+    uint8_t *pred_flag = nullptr;
+    uint32_t *part_cnt = nullptr, *scr_a = nullptr, *scr_b = nullptr;
+    uint64_t *part_total = nullptr;
+    uint64_t pred_cap = 0, route_cap = 0;
     std::string err;
 };
 
@@ -467,6 +486,19 @@ uint32_t pow2_at_least(uint64_t v)
     uint32_t q = 1;
     while (q < v) q <<= 1;
     return q;
+}
+
+// Both routing lists hold up to T transactions.
+csmv_status route_reserve(csmv_engine *e, uint64_t T)
+{
+    if (T <= e->route_cap) return CSMV_OK;
+    dfree(e->norm_list);
+    dfree(e->hot_list);
+    e->route_cap = 0;
+    csmv_status st;
+    if ((st = dalloc(e, &e->norm_list, T)) || (st = dalloc(e, &e->hot_list, T))) return st;
+    e->route_cap = T;
+    return CSMV_OK;
 }
 
 uint32_t pow2_floor(uint64_t v)
@@ -641,6 +673,11 @@ csmv_status csmv_destroy(csmv_engine *e)
     dfree(e->mc_vsum); dfree(e->mc_ab); dfree(e->mc_lat);
     dfree(e->op); dfree(e->addr); dfree(e->cdf); dfree(e->log_ts); dfree(e->log_v);
     dfree(e->log_ab);
+    dfree(e->mlp_params); dfree(e->mlp_m); dfree(e->mlp_v); dfree(e->mlp_grad);
+    dfree(e->mlp_part); dfree(e->mlp_loss); dfree(e->mlp_w1t); dfree(e->abort_ctr);
+    dfree(e->ds_x); dfree(e->ds_y); dfree(e->io_x); dfree(e->io_y); dfree(e->io_p);
+    dfree(e->pred_p); dfree(e->pred_flag); dfree(e->part_cnt); dfree(e->scr_a);
+    dfree(e->scr_b); dfree(e->part_total);
     if (e->ev0) (void)hipEventDestroy(e->ev0);
     if (e->ev1) (void)hipEventDestroy(e->ev1);
     if (e->stream) (void)hipStreamDestroy(e->stream);
@@ -735,6 +772,7 @@ csmv_status csmv_bank_generate(csmv_engine *e, const csmv_bank_spec *s)
     HIPCHK(e, hipStreamSynchronize(e->stream));
     e->spec = *s;
     e->has_stream = true;
+    e->bank_logged = false;
     e->routed = false;
     return CSMV_OK;
 }
@@ -763,10 +801,9 @@ csmv_status csmv_bank_route(csmv_engine *e, const uint8_t *hot, size_t n)
     for (uint64_t k = 0; k < T; ++k)
         if (hot[k] && (op[k] & 0xFFu) == kKindTransfer) lists[nn + nh++] = (uint32_t)k;
     delete[] op;
-    dfree(e->norm_list);
-    dfree(e->hot_list);
-    csmv_status st;
-    if ((st = dalloc(e, &e->norm_list, nn)) || (st = dalloc(e, &e->hot_list, nh))) {
+    e->routed = false;
+    const csmv_status st = route_reserve(e, T);
+    if (st) {
         delete[] lists;
         return st;
     }
@@ -923,7 +960,9 @@ csmv_status csmv_bank_run(csmv_engine *e, csmv_stats *out)
                     (e->cfg.variant & kServerWriteback)))
         return fail(e, CSMV_E_ARG, "the serialized lane needs n_serial_waves > 0 and the "
                                    "STAMP variant with client write-back");
-    return launch_persistent(e, bank_kernel<kBankR, kBankW>, c, L, out);
+    const csmv_status st = launch_persistent(e, bank_kernel<kBankR, kBankW>, c, L, out);
+    e->bank_logged = st == CSMV_OK;
+    return st;
 }
 
 // ---- memcached-style cache ---------------------------------------------------------------
@@ -1062,6 +1101,360 @@ csmv_status csmv_atomic_rate(csmv_engine *e, uint32_t waves, uint32_t iters, dou
     float ms = 0.f;
     HIPCHK(e, hipEventElapsedTime(&ms, e->ev0, e->ev1));
     *rate = (double)blocks * 4.0 * iters / (ms * 1e-3);
+    return CSMV_OK;
+}
+
+}  // extern "C"
+
+// =============================================================================================
+// learned contention predictor (mlp.hpp): abort-log counters, training set, training,
+// inference and the device routing partition
+// =============================================================================================
+namespace {
+
+constexpr float kAdamBeta1 = 0.9f, kAdamBeta2 = 0.999f, kAdamEps = 1e-8f;
+
+csmv_status pred_ensure(csmv_engine *e)
+{
+    if (e->mlp_params) return CSMV_OK;
+    csmv_status st;
+    if ((st = dalloc(e, &e->mlp_params, kParamCount)) || (st = dalloc(e, &e->mlp_m, kParamCount)) ||
+        (st = dalloc(e, &e->mlp_v, kParamCount)) || (st = dalloc(e, &e->mlp_grad, kPartStride)) ||
+        (st = dalloc(e, &e->mlp_part, (size_t)kMlpGradBlocks * kMlpWaves * kPartStride)) ||
+        (st = dalloc(e, &e->mlp_w1t, (size_t)kFeat * kHidden)) ||
+        (st = dalloc(e, &e->abort_ctr, kAbortBuckets)) || (st = dalloc(e, &e->part_total, 1))) {
+        dfree(e->mlp_params);
+        return st;
+    }
+    HIPCHK(e, hipMemsetAsync(e->mlp_params, 0, kParamCount * 4, e->stream));
+    HIPCHK(e, hipMemsetAsync(e->mlp_m, 0, kParamCount * 4, e->stream));
+    HIPCHK(e, hipMemsetAsync(e->mlp_v, 0, kParamCount * 4, e->stream));
+    HIPCHK(e, hipMemsetAsync(e->mlp_w1t, 0, kFeat * kHidden * 2, e->stream));
+    HIPCHK(e, hipMemsetAsync(e->abort_ctr, 0, kAbortBuckets * 4, e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    return CSMV_OK;
+}
+
+csmv_status pred_upload(csmv_engine *e, const float *params)
+{
+    HIPCHK(e, hipMemcpyAsync(e->mlp_params, params, kParamCount * 4, hipMemcpyHostToDevice,
+                             e->stream));
+    HIPCHK(e, hipMemsetAsync(e->mlp_m, 0, kParamCount * 4, e->stream));
+    HIPCHK(e, hipMemsetAsync(e->mlp_v, 0, kParamCount * 4, e->stream));
+    hipLaunchKernelGGL(mlp_sync_kernel, dim3((kFeat * kHidden + 255) / 256), dim3(256), 0,
+                       e->stream, e->mlp_params, e->mlp_w1t);
+    HIPCHK(e, hipGetLastError());
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    e->mlp_step = 0;
+    return CSMV_OK;
+}
+
+// per-transaction buffers of the resident stream
+csmv_status pred_reserve(csmv_engine *e, uint64_t T)
+{
+    if (T <= e->pred_cap) return CSMV_OK;
+    dfree(e->pred_p); dfree(e->pred_flag); dfree(e->part_cnt); dfree(e->scr_a); dfree(e->scr_b);
+    e->pred_cap = 0;
+    csmv_status st;
+    if ((st = dalloc(e, &e->pred_p, T)) || (st = dalloc(e, &e->pred_flag, T)) ||
+        (st = dalloc(e, &e->part_cnt, (T + kPartItems - 1) / kPartItems)) ||
+        (st = dalloc(e, &e->scr_a, T)) || (st = dalloc(e, &e->scr_b, T)))
+        return st;
+    e->pred_cap = T;
+    return CSMV_OK;
+}
+
+csmv_status io_reserve(csmv_engine *e, uint64_t n)
+{
+    if (n <= e->io_cap) return CSMV_OK;
+    dfree(e->io_x); dfree(e->io_y); dfree(e->io_p);
+    e->io_cap = 0;
+    csmv_status st;
+    if ((st = dalloc(e, &e->io_x, n * kFeat)) || (st = dalloc(e, &e->io_y, n)) ||
+        (st = dalloc(e, &e->io_p, n)))
+        return st;
+    e->io_cap = n;
+    return CSMV_OK;
+}
+
+// Stable partition of [0, T) by pred_flag into (hot, rest); returns the hot count.
+csmv_status partition(csmv_engine *e, uint64_t T, uint32_t *hot, uint32_t *rest, uint64_t *n_hot)
+{
+    const uint32_t nb = (uint32_t)((T + kPartItems - 1) / kPartItems);
+    hipLaunchKernelGGL(part_count_kernel, dim3(nb), dim3(256), 0, e->stream, e->pred_flag, T,
+                       e->part_cnt);
+    hipLaunchKernelGGL(part_scan_kernel, dim3(1), dim3(1024), 0, e->stream, e->part_cnt, nb,
+                       e->part_total);
+    hipLaunchKernelGGL(part_scatter_kernel, dim3(nb), dim3(256), 0, e->stream, e->pred_flag, T,
+                       e->part_cnt, hot, rest);
+    HIPCHK(e, hipGetLastError());
+    HIPCHK(e, hipMemcpyAsync(n_hot, e->part_total, 8, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    return CSMV_OK;
+}
+
+__global__ void __launch_bounds__(256) transfer_flag_kernel(const uint32_t *op, uint64_t T,
+                                                            uint8_t *flag)
+{
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < T) flag[k] = (op[k] & 0xFFu) == kKindTransfer ? 1 : 0;
+}
+
+// one gradient pass over n rows of (X, y): partials -> e->mlp_grad (+ loss at the end)
+csmv_status grad_pass(csmv_engine *e, const uint16_t *X, const float *y, uint64_t n)
+{
+    const uint64_t ntiles = (n + 63) / 64;
+    const uint32_t blocks =
+        (uint32_t)std::min<uint64_t>(kMlpGradBlocks, (ntiles + kMlpWaves - 1) / kMlpWaves);
+    hipLaunchKernelGGL(mlp_grad_kernel, dim3(blocks), dim3(64 * kMlpWaves), 0, e->stream, X, y,
+                       n, e->mlp_w1t, e->mlp_params, e->mlp_part);
+    hipLaunchKernelGGL(mlp_reduce_kernel, dim3((kPartStride + 255) / 256), dim3(256), 0,
+                       e->stream, e->mlp_part, blocks * kMlpWaves, e->mlp_grad);
+    HIPCHK(e, hipGetLastError());
+    return CSMV_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+csmv_status csmv_pred_init(csmv_engine *e, uint64_t seed)
+{
+    if (!e) return CSMV_E_ARG;
+    csmv_status st = pred_ensure(e);
+    if (st) return st;
+    // Glorot-uniform weights from a counter-based stream, zero biases
+    float h[kParamCount];
+    const double a1 = std::sqrt(6.0 / (kFeat + kHidden)), a2 = std::sqrt(6.0 / (kHidden + 1));
+    for (uint32_t i = 0; i < kParamCount; ++i) {
+        const double u = (double)(splitmix64(seed * 0x10001ull + i) >> 11) * 0x1.0p-53;
+        if (i < kParamB1) h[i] = (float)((2.0 * u - 1.0) * a1);
+        else if (i >= kParamW2 && i < kParamB2) h[i] = (float)((2.0 * u - 1.0) * a2);
+        else h[i] = 0.f;
+    }
+    return pred_upload(e, h);
+}
+
+csmv_status csmv_pred_set_params(csmv_engine *e, const float *params, size_t n)
+{
+    if (!e || !params || n != kParamCount) return fail(e, CSMV_E_ARG, "params: size mismatch");
+    csmv_status st = pred_ensure(e);
+    if (st) return st;
+    return pred_upload(e, params);
+}
+
+csmv_status csmv_pred_get_params(csmv_engine *e, float *params, size_t n)
+{
+    if (!e || !params || n != kParamCount) return fail(e, CSMV_E_ARG, "params: size mismatch");
+    csmv_status st = pred_ensure(e);
+    if (st) return st;
+    HIPCHK(e, hipMemcpy(params, e->mlp_params, kParamCount * 4, hipMemcpyDeviceToHost));
+    return CSMV_OK;
+}
+
+csmv_status csmv_abort_fold(csmv_engine *e, uint32_t decay_shift)
+{
+    if (!e) return CSMV_E_ARG;
+    csmv_status st = pred_ensure(e);
+    if (st) return st;
+    if (!e->bank_logged) return fail(e, CSMV_E_STATE, "fold needs a completed bank run");
+    const uint64_t T = e->spec.n_txns;
+    hipLaunchKernelGGL(abort_decay_kernel, dim3(kAbortBuckets / 256), dim3(256), 0, e->stream,
+                       e->abort_ctr, decay_shift);
+    hipLaunchKernelGGL(abort_fold_kernel, dim3((unsigned)((T + 255) / 256)), dim3(256), 0,
+                       e->stream, e->op, e->addr, e->log_ab, T, e->abort_ctr);
+    HIPCHK(e, hipGetLastError());
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    return CSMV_OK;
+}
+
+csmv_status csmv_abort_read_counters(csmv_engine *e, uint32_t *out, size_t n)
+{
+    if (!e || !out || n != kAbortBuckets) return fail(e, CSMV_E_ARG, "counters: size mismatch");
+    csmv_status st = pred_ensure(e);
+    if (st) return st;
+    HIPCHK(e, hipMemcpy(out, e->abort_ctr, kAbortBuckets * 4, hipMemcpyDeviceToHost));
+    return CSMV_OK;
+}
+
+csmv_status csmv_abort_load_counters(csmv_engine *e, const uint32_t *in, size_t n)
+{
+    if (!e || !in || n != kAbortBuckets) return fail(e, CSMV_E_ARG, "counters: size mismatch");
+    csmv_status st = pred_ensure(e);
+    if (st) return st;
+    HIPCHK(e, hipMemcpy(e->abort_ctr, in, kAbortBuckets * 4, hipMemcpyHostToDevice));
+    return CSMV_OK;
+}
+
+csmv_status csmv_pred_collect(csmv_engine *e, uint64_t max_samples, uint64_t *n_total)
+{
+    if (!e) return CSMV_E_ARG;
+    csmv_status st = pred_ensure(e);
+    if (st) return st;
+    if (!e->bank_logged) return fail(e, CSMV_E_STATE, "collect needs a completed bank run");
+    const uint64_t T = e->spec.n_txns;
+    if ((st = pred_reserve(e, T))) return st;
+    hipLaunchKernelGGL(transfer_flag_kernel, dim3((unsigned)((T + 255) / 256)), dim3(256), 0,
+                       e->stream, e->op, T, e->pred_flag);
+    HIPCHK(e, hipGetLastError());
+    uint64_t n_tr = 0;
+    if ((st = partition(e, T, e->scr_a, e->scr_b, &n_tr))) return st;
+    const uint64_t n = std::min(n_tr, max_samples);
+    if (e->ds_n + n > e->ds_cap) {
+        const uint64_t cap = std::max<uint64_t>(e->ds_n + n, 2 * e->ds_cap);
+        uint16_t *nx = nullptr;
+        float *ny = nullptr;
+        if ((st = dalloc(e, &nx, cap * kFeat)) || (st = dalloc(e, &ny, cap))) {
+            dfree(nx);
+            return st;
+        }
+        if (e->ds_n) {
+            HIPCHK(e, hipMemcpyAsync(nx, e->ds_x, e->ds_n * kFeat * 2, hipMemcpyDeviceToDevice,
+                                     e->stream));
+            HIPCHK(e, hipMemcpyAsync(ny, e->ds_y, e->ds_n * 4, hipMemcpyDeviceToDevice, e->stream));
+            HIPCHK(e, hipStreamSynchronize(e->stream));
+        }
+        dfree(e->ds_x);
+        dfree(e->ds_y);
+        e->ds_x = nx;
+        e->ds_y = ny;
+        e->ds_cap = cap;
+    }
+    if (n) {
+        const uint64_t tiles = (n + 63) / 64;
+        hipLaunchKernelGGL(features_kernel, dim3((unsigned)((tiles + kMlpWaves - 1) / kMlpWaves)),
+                           dim3(64 * kMlpWaves), 0, e->stream, e->op, e->addr, T, e->log_ab,
+                           e->scr_a, n, e->abort_ctr, e->ds_x + e->ds_n * kFeat,
+                           e->ds_y + e->ds_n);
+        HIPCHK(e, hipGetLastError());
+        HIPCHK(e, hipStreamSynchronize(e->stream));
+    }
+    e->ds_n += n;
+    if (n_total) *n_total = e->ds_n;
+    return CSMV_OK;
+}
+
+csmv_status csmv_pred_read_dataset(csmv_engine *e, uint16_t *X, float *y, size_t n)
+{
+    if (!e || n > e->ds_n) return fail(e, CSMV_E_ARG, "dataset: n exceeds the collected rows");
+    if (X && n) HIPCHK(e, hipMemcpy(X, e->ds_x, n * kFeat * 2, hipMemcpyDeviceToHost));
+    if (y && n) HIPCHK(e, hipMemcpy(y, e->ds_y, n * 4, hipMemcpyDeviceToHost));
+    return CSMV_OK;
+}
+
+csmv_status csmv_pred_clear_dataset(csmv_engine *e)
+{
+    if (!e) return CSMV_E_ARG;
+    e->ds_n = 0;
+    return CSMV_OK;
+}
+
+csmv_status csmv_pred_train(csmv_engine *e, uint32_t steps, float lr, float *loss)
+{
+    if (!e || steps == 0 || !(lr > 0.f)) return CSMV_E_ARG;
+    csmv_status st = pred_ensure(e);
+    if (st) return st;
+    if (e->ds_n == 0) return fail(e, CSMV_E_STATE, "train needs a collected dataset");
+    if (steps > e->mlp_loss_cap) {
+        dfree(e->mlp_loss);
+        e->mlp_loss_cap = 0;
+        if ((st = dalloc(e, &e->mlp_loss, steps))) return st;
+        e->mlp_loss_cap = steps;
+    }
+    for (uint32_t s = 0; s < steps; ++s) {
+        if ((st = grad_pass(e, e->ds_x, e->ds_y, e->ds_n))) return st;
+        const uint32_t t = ++e->mlp_step;
+        const float c1 = (float)(1.0 - std::pow((double)kAdamBeta1, (double)t));
+        const float c2 = (float)(1.0 - std::pow((double)kAdamBeta2, (double)t));
+        hipLaunchKernelGGL(mlp_adam_kernel, dim3((kParamCount + 255) / 256), dim3(256), 0,
+                           e->stream, e->mlp_params, e->mlp_m, e->mlp_v, e->mlp_grad, e->mlp_w1t,
+                           lr, kAdamBeta1, kAdamBeta2, kAdamEps, c1, c2);
+        // the loss of this pass (before the update)
+        HIPCHK(e, hipMemcpyAsync(e->mlp_loss + s, e->mlp_grad + kParamCount, 4,
+                                 hipMemcpyDeviceToDevice, e->stream));
+        HIPCHK(e, hipGetLastError());
+    }
+    if (loss) HIPCHK(e, hipMemcpyAsync(loss, e->mlp_loss, steps * 4, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    return CSMV_OK;
+}
+
+csmv_status csmv_pred_forward(csmv_engine *e, const uint16_t *X, size_t n, float *p)
+{
+    if (!e || !X || !p || n == 0) return CSMV_E_ARG;
+    csmv_status st = pred_ensure(e);
+    if (st || (st = io_reserve(e, n))) return st;
+    HIPCHK(e, hipMemcpyAsync(e->io_x, X, n * kFeat * 2, hipMemcpyHostToDevice, e->stream));
+    const uint64_t tiles = (n + 63) / 64;
+    hipLaunchKernelGGL(mlp_forward_kernel, dim3((unsigned)((tiles + kMlpWaves - 1) / kMlpWaves)),
+                       dim3(64 * kMlpWaves), 0, e->stream, e->io_x, (uint64_t)n, e->mlp_w1t,
+                       e->mlp_params, e->io_p);
+    HIPCHK(e, hipGetLastError());
+    HIPCHK(e, hipMemcpyAsync(p, e->io_p, n * 4, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    return CSMV_OK;
+}
+
+csmv_status csmv_pred_grad(csmv_engine *e, const uint16_t *X, const float *y, size_t n,
+                           float *grad, float *loss)
+{
+    if (!e || !X || !y || !grad || n == 0) return CSMV_E_ARG;
+    csmv_status st = pred_ensure(e);
+    if (st || (st = io_reserve(e, n))) return st;
+    HIPCHK(e, hipMemcpyAsync(e->io_x, X, n * kFeat * 2, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(e, hipMemcpyAsync(e->io_y, y, n * 4, hipMemcpyHostToDevice, e->stream));
+    if ((st = grad_pass(e, e->io_x, e->io_y, n))) return st;
+    float h[kPartStride];
+    HIPCHK(e, hipMemcpyAsync(h, e->mlp_grad, kPartStride * 4, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    std::memcpy(grad, h, kParamCount * 4);
+    if (loss) *loss = h[kParamCount];
+    return CSMV_OK;
+}
+
+csmv_status csmv_pred_infer(csmv_engine *e, float tau, float *p, size_t n)
+{
+    if (!e) return CSMV_E_ARG;
+    if (!e->has_stream) return fail(e, CSMV_E_STATE, "infer before generate");
+    const uint64_t T = e->spec.n_txns;
+    if (p && n != T) return fail(e, CSMV_E_ARG, "infer: one probability per transaction");
+    csmv_status st = pred_ensure(e);
+    if (st || (st = pred_reserve(e, T))) return st;
+    const uint64_t tiles = (T + 63) / 64;
+    hipLaunchKernelGGL(mlp_infer_kernel, dim3((unsigned)((tiles + kMlpWaves - 1) / kMlpWaves)),
+                       dim3(64 * kMlpWaves), 0, e->stream, e->op, e->addr, T, e->abort_ctr,
+                       e->mlp_w1t, e->mlp_params, tau, e->pred_p, e->pred_flag);
+    HIPCHK(e, hipGetLastError());
+    if (p) HIPCHK(e, hipMemcpyAsync(p, e->pred_p, T * 4, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    return CSMV_OK;
+}
+
+csmv_status csmv_bank_route_learned(csmv_engine *e, float tau, uint64_t *n_hot)
+{
+    csmv_status st = csmv_pred_infer(e, tau, nullptr, 0);
+    if (st) return st;
+    const uint64_t T = e->spec.n_txns;
+    e->routed = false;
+    if ((st = route_reserve(e, T))) return st;
+    uint64_t nh = 0;
+    if ((st = partition(e, T, e->hot_list, e->norm_list, &nh))) return st;
+    e->n_hot = nh;
+    e->n_norm = T - nh;
+    e->routed = true;
+    if (n_hot) *n_hot = nh;
+    return CSMV_OK;
+}
+
+csmv_status csmv_bank_read_route(csmv_engine *e, uint32_t *norm, uint32_t *hot, uint64_t *n_norm,
+                                 uint64_t *n_hot)
+{
+    if (!e) return CSMV_E_ARG;
+    if (!e->routed) return fail(e, CSMV_E_STATE, "stream is not routed");
+    if (n_norm) *n_norm = e->n_norm;
+    if (n_hot) *n_hot = e->n_hot;
+    if (norm && e->n_norm) HIPCHK(e, hipMemcpy(norm, e->norm_list, e->n_norm * 4, hipMemcpyDeviceToHost));
+    if (hot && e->n_hot) HIPCHK(e, hipMemcpy(hot, e->hot_list, e->n_hot * 4, hipMemcpyDeviceToHost));
     return CSMV_OK;
 }
 

@@ -187,6 +187,54 @@ csmv_status csmv_wallclock_khz(csmv_engine *e, uint32_t *khz);
 csmv_status csmv_atomic_rate(csmv_engine *e, uint32_t waves, uint32_t iters,
                              double *atomics_per_s);
 
+/* ---- learned contention predictor ----------------------------------------------------
+ * No counterpart in the reference, whose only abort record is the per-class totals of
+ * Statistics (bank/csmv_sh/API.cuh:112-119) and whose drivers dispatch in arrival order
+ * (bank/csmv_sh/exe.cu:142-269).  The predictor only chooses the lane of a transfer.
+ *
+ * Abort counters: CSMV_ABORT_BUCKETS uint32 buckets; an item x maps to bucket
+ * h0(x) mod 4096.  Features of a transfer: CSMV_PRED_FEATURES bf16 values (32 hashed
+ * read/write-set buckets, 32 hashed floor(log2(1 + counter)) / 8 terms).  Network:
+ * 64 -> 64 (ReLU) -> 1 (sigmoid), bf16 weights, fp32 accumulation; parameter vector of
+ * CSMV_PRED_PARAMS floats: W1[f][h] (f-major), b1[64], w2[64], b2. */
+#define CSMV_ABORT_BUCKETS 4096
+#define CSMV_PRED_FEATURES 64
+#define CSMV_PRED_PARAMS 4225
+
+/* Glorot-uniform weights and zero biases from a seed; resets the optimizer. */
+csmv_status csmv_pred_init(csmv_engine *e, uint64_t seed);
+csmv_status csmv_pred_set_params(csmv_engine *e, const float *params, size_t n);
+csmv_status csmv_pred_get_params(csmv_engine *e, float *params, size_t n);
+/* counters = (counters >> decay_shift) + the abort log of the last completed bank run
+ * (failed attempts of every transfer, added to the bucket of each item it touches);
+ * decay_shift >= 32 clears first. */
+csmv_status csmv_abort_fold(csmv_engine *e, uint32_t decay_shift);
+csmv_status csmv_abort_read_counters(csmv_engine *e, uint32_t *out, size_t n);
+csmv_status csmv_abort_load_counters(csmv_engine *e, const uint32_t *in, size_t n);
+/* Appends the features (under the current counters) and labels (aborted at least once
+ * in the last completed run) of up to max_samples transfers of the resident stream, in
+ * stream order, to the device training set; *n_total = rows held. */
+csmv_status csmv_pred_collect(csmv_engine *e, uint64_t max_samples, uint64_t *n_total);
+csmv_status csmv_pred_read_dataset(csmv_engine *e, uint16_t *X, float *y, size_t n);
+csmv_status csmv_pred_clear_dataset(csmv_engine *e);
+/* steps full-batch Adam steps (beta 0.9 / 0.999, eps 1e-8) on the mean binary
+ * cross-entropy of the training set; loss[steps] (nullable) = loss before each step. */
+csmv_status csmv_pred_train(csmv_engine *e, uint32_t steps, float lr, float *loss);
+/* Host-buffer forms of the forward and backward kernels (X: n x 64 bf16 bits):
+ * p[n]; grad[CSMV_PRED_PARAMS] of the mean loss and the loss itself. */
+csmv_status csmv_pred_forward(csmv_engine *e, const uint16_t *X, size_t n, float *p);
+csmv_status csmv_pred_grad(csmv_engine *e, const uint16_t *X, const float *y, size_t n,
+                           float *grad, float *loss);
+/* Predicted abort probability of every transaction of the resident stream (0 for
+ * audits) under the current counters; p (nullable) gets n_txns floats. */
+csmv_status csmv_pred_infer(csmv_engine *e, float tau, float *p, size_t n);
+/* Learned routing of the resident stream: transfers with p > tau go to the serialized
+ * lane, the rest stay on the normal lane, both in stream order. */
+csmv_status csmv_bank_route_learned(csmv_engine *e, float tau, uint64_t *n_hot);
+/* The current routing lists (either pointer may be NULL). */
+csmv_status csmv_bank_read_route(csmv_engine *e, uint32_t *norm, uint32_t *hot,
+                                 uint64_t *n_norm, uint64_t *n_hot);
+
 #ifdef __cplusplus
 }
 #endif

@@ -28,6 +28,8 @@ constexpr uint32_t kNone = 0xFFFFFFFFu;
 // request-ring flag: the request's transactions are executed by the commit server
 constexpr uint32_t kDelegateBit = 0x80000000u;
 constexpr uint32_t kWave = 64;
+// bank stream: op[k] = kind | amount << 8 | reads << 16
+constexpr uint32_t kKindTransfer = 0, kKindAudit = 1;
 
 // ---- wave64 primitives --------------------------------------------------------------
 __device__ __forceinline__ uint32_t lane_id()
