@@ -14,6 +14,8 @@ Reference interfaces mirrored here (file:line in DMRNunes/csmv-gpu-stm):
   Engine.abort_fold ...     per-bucket abort log  (the reference keeps per-class totals only,
                             Statistics, bank/csmv_sh/API.cuh:112-119)
   Engine.pred_* / bank_route_learned   learned contention predictor: no reference counterpart
+  serve(cache, bank, ...)   persistent serving loop; the reference serves cache batches from a
+                            host loop against a persistent server, memcachedGPU/memcdSH.cu:603-651
 """
 from __future__ import annotations
 
@@ -66,7 +68,7 @@ class Config(C.Structure):
 class BankSpec(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("n_txns", C.c_uint64), ("n_reads", C.c_uint32),
                 ("n_reads_ro", C.c_uint32), ("pct_ro", C.c_uint32), ("dist", C.c_uint32),
-                ("amount_max", C.c_uint32), ("reserved", C.c_uint32), ("theta", C.c_double)]
+                ("amount_max", C.c_uint32), ("n_reads_min", C.c_uint32), ("theta", C.c_double)]
 
 
 class MemcdSpec(C.Structure):
@@ -102,6 +104,18 @@ class Stats(C.Structure):
         return d
 
 
+class ServeSpec(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("offered_rps", C.c_double), ("pct_cache", C.c_uint32),
+                ("reserved", C.c_uint32), ("pool_budget", C.c_uint64)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("cache", Stats), ("bank", Stats), ("kernel_ms", C.c_double),
+                ("admit_stalls", C.c_uint64), ("peak_inflight", C.c_uint64),
+                ("final_inflight", C.c_uint64), ("wallclock_khz", C.c_uint32),
+                ("reserved", C.c_uint32)]
+
+
 _lib: Optional[C.CDLL] = None
 
 
@@ -134,6 +148,9 @@ def lib() -> C.CDLL:
         "csmv_memcd_run": ([P, C.POINTER(Stats)], C.c_int),
         "csmv_memcd_read_log": ([P, P, P, P, P, P], C.c_int),
         "csmv_wallclock_khz": ([P, C.POINTER(U32)], C.c_int),
+        "csmv_serve_run": ([P, P, C.POINTER(ServeSpec), C.POINTER(ServeStats)], C.c_int),
+        "csmv_serve_read_arrivals": ([P, P, SZ], C.c_int),
+        "csmv_serve_read_latency": ([P, P, SZ], C.c_int),
         "csmv_pred_init": ([P, U64], C.c_int),
         "csmv_pred_set_params": ([P, P, SZ], C.c_int),
         "csmv_pred_get_params": ([P, P, SZ], C.c_int),
@@ -183,10 +200,11 @@ class Bank:
     zipf: bool = False
     theta: float = 0.99
     amount_max: int = 10
+    n_reads_min: int = 0     # > 0: transfers read uniform n_reads_min..n_reads accounts
 
     def spec(self) -> BankSpec:
         return BankSpec(self.seed, self.n_txns, self.n_reads, self.n_reads_ro, self.pct_ro,
-                        1 if self.zipf else 0, self.amount_max, 0, self.theta)
+                        1 if self.zipf else 0, self.amount_max, self.n_reads_min, self.theta)
 
     @property
     def rmax(self) -> int:
@@ -415,3 +433,48 @@ class Engine:
         self._chk(lib().csmv_bank_read_route(self._h, _ptr(norm), _ptr(hot), None, None),
                   "read_route")
         return norm, hot
+
+    # ---- serving loop -------------------------------------------------------------------
+    def _served_len(self) -> int:
+        served = getattr(self, "_served", None)
+        if served == "cache":
+            return self.memcd.n_reqs
+        if served == "bank":
+            return self.bank.n_txns
+        raise RuntimeError("engine has not served a stream")
+
+    def serve_arrivals(self) -> np.ndarray:
+        n = self._served_len()
+        out = np.empty(n, dtype=np.uint32)
+        self._chk(lib().csmv_serve_read_arrivals(self._h, _ptr(out), n), "serve_arrivals")
+        return out
+
+    def serve_latency(self) -> np.ndarray:
+        """Latency of each request from arrival to commit, in wall-clock ticks."""
+        n = self._served_len()
+        out = np.empty(n, dtype=np.uint32)
+        self._chk(lib().csmv_serve_read_latency(self._h, _ptr(out), n), "serve_latency")
+        return out
+
+
+def serve(cache: Optional[Engine], bank: Optional[Engine], offered_rps: float,
+          pct_cache: int = 80, seed: int = 1, pool_budget: int = 0) -> dict:
+    """Runs the persistent serving loop over a cache partition and a bank partition
+    (either may be None: that class alone, at the arrival times it has in the mix)."""
+    sp = ServeSpec(seed, float(offered_rps), pct_cache, 0, pool_budget)
+    st = ServeStats()
+    h_c = cache._h if cache is not None else None
+    h_b = bank._h if bank is not None else None
+    rc = lib().csmv_serve_run(h_c, h_b, C.byref(sp), C.byref(st))
+    for e, name in ((cache, "cache"), (bank, "bank")):
+        if e is not None:
+            e._served = name
+    if rc:
+        e = cache if cache is not None else bank
+        msg = lib().csmv_last_error(e._h) or b""
+        raise CsmvError(rc, f"serve: {msg.decode(errors='replace')}")
+    return {"cache": st.cache.as_dict() if cache is not None else None,
+            "bank": st.bank.as_dict() if bank is not None else None,
+            "kernel_ms": st.kernel_ms, "admit_stalls": st.admit_stalls,
+            "peak_inflight": st.peak_inflight, "final_inflight": st.final_inflight,
+            "wallclock_khz": st.wallclock_khz}

@@ -88,3 +88,31 @@ def transfer_state(src: Engine, dst: Engine):
 def route(e: Engine, tau: float = 0.5) -> int:
     """Learned routing of e's resident stream; returns the serialized-lane count."""
     return e.bank_route_learned(tau)
+
+
+TAUS = (0.02, 0.05, 0.1, 0.2, 0.3, 0.5)
+
+
+def calibrate(plain: Engine, routed: Engine, thetas: Iterable[float] = (0.6, 0.9, 1.2),
+              taus: Iterable[float] = TAUS, n_txns: int = 1 << 17, seed: int = 2000,
+              init: int = INIT, make_bank=c1_bank) -> dict:
+    """Chooses the routing threshold on the training skews: for each skew, a history
+    stream refreshes the counters (plain engine, arrival order), then a validation stream
+    is routed at every threshold on the routed engine; the threshold with the highest
+    geometric-mean committed txns/s wins.  Returns {"tau", "table"}."""
+    thetas, taus = tuple(thetas), tuple(taus)
+    logs = {t: [] for t in taus}
+    table = []
+    for i, th in enumerate(thetas):
+        history(plain, make_bank(n_txns, seed + 2 * i, th), init)
+        transfer_state(plain, routed)
+        routed.bank_generate(make_bank(n_txns, seed + 2 * i + 1, th))
+        for t in taus:
+            nh = routed.bank_route_learned(t)
+            routed.fill(init)
+            st = routed.bank_run()
+            logs[t].append(np.log(max(st["txn_per_s"], 1.0)))
+            table.append({"theta": th, "tau": t, "routed": nh, "txn_per_s": st["txn_per_s"],
+                          "abort_rate": st["abort_rate"]})
+    best = max(taus, key=lambda t: float(np.mean(logs[t])))
+    return {"tau": best, "table": table}

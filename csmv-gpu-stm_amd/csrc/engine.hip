@@ -19,12 +19,15 @@
 #include "server.hpp"
 #include "memcd.hpp"
 #include "mlp.hpp"
+#include "serve.hpp"
 
 namespace csmv {
 
 // Reads of one update transaction on the wire, writes of one transaction.
 constexpr int kBankR = 4;
 constexpr int kBankW = 2;
+// serving loop: bank transactions read 2..64 accounts
+constexpr int kServeBankR = 64;
 
 // ---- stream generation ----------------------------------------------------------------------
 __host__ __device__ __forceinline__ uint64_t splitmix64(uint64_t x)
@@ -66,13 +69,14 @@ __device__ __forceinline__ uint32_t scramble(uint32_t rank, uint32_t n)
 
 struct GenArgs {
     uint64_t seed, n_txns;
-    uint32_t n_items, n_reads, n_reads_ro, ro_thr, dist, amount_max, rmax;
+    uint32_t n_items, n_reads, n_reads_ro, ro_thr, dist, amount_max, rmax, n_reads_min;
     const uint32_t *cdf;
     uint32_t *op, *addr;
 };
 
 // One lane per transaction k; state = splitmix64(seed + k), then the driver's draws:
-// read-only test rnd < prRead*0xFFFF (bank/csmv_sh/exe.cu:134-139), amount, operands.
+// read-only test rnd < prRead*0xFFFF (bank/csmv_sh/exe.cu:134-139), amount, the
+// read-set size when it varies (serving mix), operands.
 __global__ void __launch_bounds__(256) bank_generate_kernel(GenArgs g)
 {
     const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -84,6 +88,7 @@ __global__ void __launch_bounds__(256) bank_generate_kernel(GenArgs g)
     if (kind == kKindTransfer) {
         amt = 1 + rand_r_fnc(st) % g.amount_max;
         nr = g.n_reads;
+        if (g.n_reads_min) nr = g.n_reads_min + rand_r_fnc(st) % (g.n_reads - g.n_reads_min + 1);
     }
     g.op[k] = kind | (amt << 8) | (nr << 16);
     for (uint32_t j = 0; j < g.rmax; ++j) {
@@ -367,6 +372,35 @@ __global__ void __launch_bounds__(512) memcd_kernel(Ctx c, ServerLayout L)
     if (cw < c.n_client_waves) memcd_client<R, W>(c, cw);
 }
 
+// Serving loop: one persistent kernel with a commit server per partition (the cache
+// items and the bank accounts are disjoint, so each partition keeps its own clock and
+// commit order) and client workgroups of each class.
+struct ServeRoles {
+    uint32_t has_cache, has_bank, cache_wgs;
+};
+
+template <int RM, int WM, int RB, int WB>
+__global__ void __launch_bounds__(512) serve_kernel(Ctx cm, ServerLayout Lm, Ctx cb,
+                                                    ServerLayout Lb, ServeRoles roles)
+{
+    extern __shared__ uint32_t lds[];
+    const uint32_t n_srv = roles.has_cache + roles.has_bank;
+    if (blockIdx.x < n_srv) {
+        if (roles.has_cache && blockIdx.x == 0) server_workgroup<RM, WM>(cm, Lm, lds);
+        else server_workgroup<RB, WB>(cb, Lb, lds);
+        return;
+    }
+    const uint32_t b = blockIdx.x - n_srv;
+    const uint32_t wave = threadIdx.x / kWave;
+    if (b < roles.cache_wgs) {
+        const uint32_t cw = b * cm.wg_waves + wave;
+        if (cw < cm.n_client_waves) memcd_client<RM, WM>(cm, cw);
+    } else {
+        const uint32_t cw = (b - roles.cache_wgs) * cb.wg_waves + wave;
+        if (cw < cb.n_client_waves) bank_serve_client<RB, WB>(cb, cw);
+    }
+}
+
 // Same-address agent-scope atomic rate: lane 0 of every wave adds to one word.
 __global__ void __launch_bounds__(256) atomic_rate_kernel(uint64_t *p, uint32_t iters)
 {
@@ -419,8 +453,12 @@ struct csmv_engine {
     uint32_t rmax = 0, log_rows = 0;
     uint64_t stream_cap = 0;
     uint32_t *op = nullptr, *addr = nullptr, *cdf = nullptr;
-    uint32_t *log_ts = nullptr, *log_ab = nullptr;
+    uint32_t *log_ts = nullptr, *log_ab = nullptr, *log_lat = nullptr;
     int32_t *log_v = nullptr;
+    // serving loop: arrival ticks of the stream this engine serves, and its class
+    uint32_t *arr = nullptr;
+    uint64_t arr_cap = 0;
+    uint32_t served = 0;   // 0 none, 1 cache requests, 2 bank transactions
     bool bank_logged = false;   // log_* hold a completed run of the resident stream
     int32_t *tmp = nullptr;
     // learned contention predictor (allocated on first use)
@@ -448,7 +486,9 @@ namespace {
 
 constexpr size_t kCtlBytes = 4096;
 constexpr size_t kOffGc = 0, kOffErr = 128, kOffDone = 256, kOffNext = 384, kOffTail = 512,
-                 kOffStats = 640, kOffProf = 1024, kOffNextHot = 1536;
+                 kOffStats = 640, kOffProf = 1024, kOffNextHot = 1536,
+                 // serving loop: start tick, in-flight version bytes, [stalls, peak]
+                 kOffServeStart = 2048, kOffInflight = 2176, kOffServeCtr = 2304;
 
 csmv_status fail(csmv_engine *e, csmv_status s, const std::string &msg)
 {
@@ -509,12 +549,13 @@ uint32_t pow2_floor(uint64_t v)
 }
 
 // Server LDS layout for a request wire of R reads and W writes.
-template <int R, int W> ServerLayout server_layout(const csmv_engine *e, bool delegation)
+template <int R, int W>
+ServerLayout server_layout(const csmv_engine *e, bool delegation, uint32_t nmb = 0)
 {
     ServerLayout L{};
     const uint32_t vmode = e->cfg.variant & 3u;
     const uint32_t M = e->cfg.wg_waves - 2;
-    L.nmb = std::min<uint32_t>(std::max<uint32_t>(M + 2, 4), 16);
+    L.nmb = nmb ? nmb : std::min<uint32_t>(std::max<uint32_t>(M + 2, 4), 16);
     L.mb_rows = delegation ? Rows<R, W>::count : Rows<R, W>::pref + 1;
     L.state_off = kLdsHdr;
     L.wave_off = L.state_off + L.nmb;
@@ -644,7 +685,8 @@ csmv_status csmv_create(const csmv_config *cfg_in, csmv_engine **out)
         return CSMV_E_HIP;
     }
     // request mailboxes sized for the widest wire of the drivers
-    const uint32_t rows = std::max(mbox_rows<kBankR, kBankW>(), mbox_rows<kMcR, kMcW>());
+    const uint32_t rows = std::max({mbox_rows<kBankR, kBankW>(), mbox_rows<kMcR, kMcW>(),
+                                    mbox_rows<kServeBankR, kBankW>()});
     static_assert(kMcW <= kBankW, "reply values are sized for the bank wire");
     if ((s = dalloc(e, &e->ht, cfg.n_items)) ||
         (s = dalloc(e, &e->slot, (size_t)cfg.n_items * cfg.depth)) ||
@@ -672,7 +714,7 @@ csmv_status csmv_destroy(csmv_engine *e)
     dfree(e->mc_key); dfree(e->mc_op); dfree(e->mc_cdf); dfree(e->mc_ts); dfree(e->mc_res);
     dfree(e->mc_vsum); dfree(e->mc_ab); dfree(e->mc_lat);
     dfree(e->op); dfree(e->addr); dfree(e->cdf); dfree(e->log_ts); dfree(e->log_v);
-    dfree(e->log_ab);
+    dfree(e->log_ab); dfree(e->log_lat); dfree(e->arr);
     dfree(e->mlp_params); dfree(e->mlp_m); dfree(e->mlp_v); dfree(e->mlp_grad);
     dfree(e->mlp_part); dfree(e->mlp_loss); dfree(e->mlp_w1t); dfree(e->abort_ctr);
     dfree(e->ds_x); dfree(e->ds_y); dfree(e->io_x); dfree(e->io_y); dfree(e->io_p);
@@ -724,19 +766,21 @@ csmv_status csmv_bank_generate(csmv_engine *e, const csmv_bank_spec *s)
 {
     if (!e || !s) return CSMV_E_ARG;
     if (s->n_txns == 0 || s->n_txns >= (1ull << 31) || s->n_reads < 2 ||
-        s->n_reads > (uint32_t)kBankR || s->n_reads_ro < 1 || s->n_reads_ro > 64 ||
+        s->n_reads > (uint32_t)kServeBankR || s->n_reads_ro < 1 || s->n_reads_ro > 64 ||
         s->pct_ro > 100 || s->dist > 1 || s->amount_max == 0 || s->amount_max > 255 ||
+        (s->n_reads_min && (s->n_reads_min < 2 || s->n_reads_min > s->n_reads)) ||
         (s->dist == 1 && !(s->theta >= 0.0)))
         return fail(e, CSMV_E_ARG, "bank spec out of range");
     const uint32_t rmax = std::max(s->n_reads, s->n_reads_ro);
     if (s->n_txns > e->stream_cap || rmax > e->rmax || s->n_reads > e->log_rows) {
         dfree(e->op); dfree(e->addr); dfree(e->log_ts); dfree(e->log_v); dfree(e->log_ab);
+        dfree(e->log_lat);
         e->stream_cap = 0;
         csmv_status st;
         if ((st = dalloc(e, &e->op, s->n_txns)) || (st = dalloc(e, &e->addr, rmax * s->n_txns)) ||
             (st = dalloc(e, &e->log_ts, s->n_txns)) ||
             (st = dalloc(e, &e->log_v, (size_t)s->n_reads * s->n_txns)) ||
-            (st = dalloc(e, &e->log_ab, s->n_txns)))
+            (st = dalloc(e, &e->log_ab, s->n_txns)) || (st = dalloc(e, &e->log_lat, s->n_txns)))
             return st;
         e->stream_cap = s->n_txns;
         e->rmax = rmax;
@@ -763,6 +807,7 @@ csmv_status csmv_bank_generate(csmv_engine *e, const csmv_bank_spec *s)
     g.dist = s->dist;
     g.amount_max = s->amount_max;
     g.rmax = rmax;
+    g.n_reads_min = s->n_reads_min;
     g.cdf = e->cdf;
     g.op = e->op;
     g.addr = e->addr;
@@ -863,6 +908,61 @@ static void base_ctx(csmv_engine *e, Ctx &c)
     c.next_hot = (uint64_t *)(e->ctl + kOffNextHot);
 }
 
+// Clears the control words and message rings of one engine before a launch.
+static csmv_status reset_run_state(csmv_engine *e, const ServerLayout &L)
+{
+    HIPCHK(e, hipMemsetAsync(e->ctl, 0, kCtlBytes, e->stream));
+    HIPCHK(e, hipMemsetAsync(e->ring, 0, e->Q * 8, e->stream));
+    HIPCHK(e, hipMemsetAsync(e->done, 0, e->Q * 8, e->stream));
+    HIPCHK(e, hipMemsetAsync(e->reply, 0, (size_t)e->cfg.client_waves * kWave * 8, e->stream));
+    HIPCHK(e, hipMemsetAsync(e->reply_v, 0, (size_t)e->cfg.client_waves * kWave * 2 * kBankW * 8,
+                             e->stream));
+    if ((e->cfg.variant & 3u) == kValStamp && !L.exact)
+        HIPCHK(e, hipMemsetAsync(e->stamps, 0, (size_t)e->cfg.n_items * 4, e->stream));
+    return CSMV_OK;
+}
+
+// Reads one engine's counters after a launch; returns the device error word.
+static csmv_status collect_stats(csmv_engine *e, const ServerLayout &L, float ms, csmv_stats *out,
+                                 uint32_t *derr)
+{
+    uint8_t host_ctl[kCtlBytes];
+    HIPCHK(e, hipMemcpy(host_ctl, e->ctl, kCtlBytes, hipMemcpyDeviceToHost));
+    const uint64_t *st = (const uint64_t *)(host_ctl + kOffStats);
+    std::memcpy(derr, host_ctl + kOffErr, 4);
+    if (out) {
+        std::memset(out, 0, sizeof(*out));
+        out->commits_update = st[kStatCommitUpd];
+        out->commits_ro = st[kStatCommitRo];
+        out->aborts_dataage = st[kStatAbortAge];
+        out->aborts_readwrite = st[kStatAbortRw];
+        out->aborts_prevalid = st[kStatAbortPrevalid];
+        out->aborts_recordage = st[kStatAbortRecord];
+        out->batches = st[kStatBatches];
+        out->final_ts = st[kStatFinalTs];
+        out->kernel_ms = ms;
+        out->error = *derr;
+        out->client_lanes = e->cfg.client_waves * kWave;
+        const uint64_t *pf = (const uint64_t *)(host_ctl + kOffProf);
+        const double ns_per_tick = 1e6 / (double)e->wallclock_khz;
+        for (int i = 0; i < kProfCount; ++i) out->prof_ns[i] = (double)pf[i] * ns_per_tick;
+        out->prof_ns[kProfClientRounds] = (double)pf[kProfClientRounds];
+        out->stamp_buckets = L.nbuckets;
+        out->stamp_exact = L.exact;
+        out->aborts_unknown = st[kStatAbortUnknown];
+        out->lds_bytes = (uint32_t)L.words * 4;
+    }
+    return CSMV_OK;
+}
+
+static csmv_status device_error(csmv_engine *e, uint32_t derr)
+{
+    if (derr == kErrLivelock) return fail(e, CSMV_E_LIVELOCK, "transaction attempt limit");
+    if (derr == kErrReadSetOverflow) return fail(e, CSMV_E_CAPACITY, "read/write set overflow");
+    if (derr) return fail(e, CSMV_E_TIMEOUT, "device wait timed out, code " + std::to_string(derr));
+    return CSMV_OK;
+}
+
 // Launches one persistent kernel (server workgroup 0 + client workgroups) over the
 // prepared context and collects its statistics.
 template <typename K>
@@ -884,14 +984,8 @@ static csmv_status launch_persistent(csmv_engine *e, K kern, const Ctx &c, const
         return fail(e, CSMV_E_CAPACITY,
                     "grid of " + std::to_string(grid) + " workgroups exceeds the " +
                         std::to_string((uint64_t)per_cu * e->n_cu) + " the device keeps resident");
-    HIPCHK(e, hipMemsetAsync(e->ctl, 0, kCtlBytes, e->stream));
-    HIPCHK(e, hipMemsetAsync(e->ring, 0, e->Q * 8, e->stream));
-    HIPCHK(e, hipMemsetAsync(e->done, 0, e->Q * 8, e->stream));
-    HIPCHK(e, hipMemsetAsync(e->reply, 0, (size_t)e->cfg.client_waves * kWave * 8, e->stream));
-    HIPCHK(e, hipMemsetAsync(e->reply_v, 0, (size_t)e->cfg.client_waves * kWave * 2 * kBankW * 8,
-                             e->stream));
-    if ((e->cfg.variant & 3u) == kValStamp && !L.exact)
-        HIPCHK(e, hipMemsetAsync(e->stamps, 0, (size_t)e->cfg.n_items * 4, e->stream));
+    csmv_status s = reset_run_state(e, L);
+    if (s) return s;
     HIPCHK(e, hipStreamSynchronize(e->stream));
 
     HIPCHK(e, hipEventRecord(e->ev0, e->stream));
@@ -902,38 +996,9 @@ static csmv_status launch_persistent(csmv_engine *e, K kern, const Ctx &c, const
     float ms = 0.f;
     HIPCHK(e, hipEventElapsedTime(&ms, e->ev0, e->ev1));
     e->pool_loaded = false;
-
-    uint8_t host_ctl[kCtlBytes];
-    HIPCHK(e, hipMemcpy(host_ctl, e->ctl, kCtlBytes, hipMemcpyDeviceToHost));
-    const uint64_t *st = (const uint64_t *)(host_ctl + kOffStats);
-    uint32_t derr;
-    std::memcpy(&derr, host_ctl + kOffErr, 4);
-    if (out) {
-        std::memset(out, 0, sizeof(*out));
-        out->commits_update = st[kStatCommitUpd];
-        out->commits_ro = st[kStatCommitRo];
-        out->aborts_dataage = st[kStatAbortAge];
-        out->aborts_readwrite = st[kStatAbortRw];
-        out->aborts_prevalid = st[kStatAbortPrevalid];
-        out->aborts_recordage = st[kStatAbortRecord];
-        out->batches = st[kStatBatches];
-        out->final_ts = st[kStatFinalTs];
-        out->kernel_ms = ms;
-        out->error = derr;
-        out->client_lanes = e->cfg.client_waves * kWave;
-        const uint64_t *pf = (const uint64_t *)(host_ctl + kOffProf);
-        const double ns_per_tick = 1e6 / (double)e->wallclock_khz;
-        for (int i = 0; i < kProfCount; ++i) out->prof_ns[i] = (double)pf[i] * ns_per_tick;
-        out->prof_ns[kProfClientRounds] = (double)pf[kProfClientRounds];
-        out->stamp_buckets = L.nbuckets;
-        out->stamp_exact = L.exact;
-        out->aborts_unknown = st[kStatAbortUnknown];
-        out->lds_bytes = (uint32_t)lds_bytes;
-    }
-    if (derr == kErrLivelock) return fail(e, CSMV_E_LIVELOCK, "transaction attempt limit");
-    if (derr == kErrReadSetOverflow) return fail(e, CSMV_E_CAPACITY, "read/write set overflow");
-    if (derr) return fail(e, CSMV_E_TIMEOUT, "device wait timed out, code " + std::to_string(derr));
-    return CSMV_OK;
+    uint32_t derr = 0;
+    if ((s = collect_stats(e, L, ms, out, &derr))) return s;
+    return device_error(e, derr);
 }
 
 extern "C" {
@@ -943,6 +1008,8 @@ csmv_status csmv_bank_run(csmv_engine *e, csmv_stats *out)
     if (!e) return CSMV_E_ARG;
     if (!e->has_stream) return fail(e, CSMV_E_STATE, "run before generate");
     if (!e->pool_loaded) return fail(e, CSMV_E_STATE, "run needs a freshly loaded pool");
+    if (e->spec.n_reads > (uint32_t)kBankR)
+        return fail(e, CSMV_E_ARG, "transfers of more than 4 reads run in the serving loop");
     const ServerLayout L = server_layout<kBankR, kBankW>(e, true);
     Ctx c{};
     base_ctx(e, c);
@@ -1455,6 +1522,213 @@ csmv_status csmv_bank_read_route(csmv_engine *e, uint32_t *norm, uint32_t *hot, 
     if (n_hot) *n_hot = e->n_hot;
     if (norm && e->n_norm) HIPCHK(e, hipMemcpy(norm, e->norm_list, e->n_norm * 4, hipMemcpyDeviceToHost));
     if (hot && e->n_hot) HIPCHK(e, hipMemcpy(hot, e->hot_list, e->n_hot * 4, hipMemcpyDeviceToHost));
+    return CSMV_OK;
+}
+
+}  // extern "C"
+
+// =============================================================================================
+// serving loop (serve.hpp): Poisson arrivals, admission by in-flight version bytes,
+// continuous batching over a cache partition and a bank partition
+// =============================================================================================
+namespace {
+
+// Poisson arrivals at `rate` per second as wall-clock ticks from the serving start:
+// exponential gaps -ln(u) / rate with u from splitmix64(seed + k) in (0, 1].
+csmv_status poisson_arrivals(csmv_engine *e, uint64_t seed, double rate, uint64_t n)
+{
+    if (n > e->arr_cap) {
+        dfree(e->arr);
+        e->arr_cap = 0;
+        csmv_status st = dalloc(e, &e->arr, n);
+        if (st) return st;
+        e->arr_cap = n;
+    }
+    uint32_t *h = new (std::nothrow) uint32_t[n];
+    if (!h) return fail(e, CSMV_E_CAPACITY, "arrival host buffer");
+    const double ticks_per_s = (double)e->wallclock_khz * 1e3;
+    double t = 0.0;
+    for (uint64_t k = 0; k < n; ++k) {
+        const double u = (double)((splitmix64(seed + k) >> 11) + 1) * 0x1.0p-53;
+        t += -std::log(u) / rate;
+        const double tick = t * ticks_per_s;
+        if (tick >= 4294967295.0) {
+            delete[] h;
+            return fail(e, CSMV_E_CAPACITY, "arrival span beyond 2^32 clock ticks");
+        }
+        h[k] = (uint32_t)tick;
+    }
+    hipError_t r = hipMemcpy(e->arr, h, n * 4, hipMemcpyHostToDevice);
+    delete[] h;
+    HIPCHK(e, r);
+    return CSMV_OK;
+}
+
+// Server layout of a partition: fewer mailboxes (down to 4) when the wire is wide, so
+// that the stamp buckets keep at least 4096 entries.
+template <int R, int W> csmv_status fit_layout(csmv_engine *e, ServerLayout &L)
+{
+    uint32_t nmb = 0;
+    for (;;) {
+        L = server_layout<R, W>(e, false, nmb);
+        const bool fits = (uint64_t)L.words * 4 <= e->lds_max && (L.exact || L.nbuckets >= 4096);
+        if (fits) return CSMV_OK;
+        if (L.nmb <= 4)
+            return fail(e, CSMV_E_CAPACITY, "serving server layout does not fit the LDS budget");
+        nmb = L.nmb - 1;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+csmv_status csmv_serve_run(csmv_engine *cache, csmv_engine *bank, const csmv_serve_spec *sp,
+                           csmv_serve_stats *out)
+{
+    // each class arrives at its share of the offered load whether or not the other class
+    // is served, so a partition served alone sees the arrivals it sees in the mix
+    const double rate_c = sp ? sp->offered_rps * sp->pct_cache / 100.0 : 0.0;
+    const double rate_b = sp ? sp->offered_rps * (100 - sp->pct_cache) / 100.0 : 0.0;
+    if (!sp || (!cache && !bank) || cache == bank || !(sp->offered_rps > 0.0) ||
+        sp->pct_cache > 100 || (cache && !(rate_c > 0.0)) || (bank && !(rate_b > 0.0)) ||
+        (sp->pool_budget && sp->pool_budget < 1024))
+        return fail(cache ? cache : bank, CSMV_E_ARG, "serve spec out of range");
+    csmv_engine *p = cache ? cache : bank;   // holds the shared serving words
+    for (csmv_engine *e : {cache, bank}) {
+        if (!e) continue;
+        if ((e->cfg.variant & 3u) != kValStamp || (e->cfg.variant & kServerWriteback))
+            return fail(e, CSMV_E_ARG, "serving runs the STAMP variant with client write-back");
+        if (!e->pool_loaded) return fail(e, CSMV_E_STATE, "serve needs a freshly loaded pool");
+        if (e->device != p->device || e->cfg.wg_waves != p->cfg.wg_waves)
+            return fail(e, CSMV_E_ARG, "partitions must share the device and wg_waves");
+    }
+    if (cache && !cache->has_mc) return fail(cache, CSMV_E_STATE, "cache partition has no stream");
+    if (bank && (!bank->has_stream || bank->spec.pct_ro != 0))
+        return fail(bank, CSMV_E_STATE, "bank partition needs a transfer-only stream");
+    // per-class Poisson processes: the thinning of one process of the offered rate
+    csmv_status st;
+    ServerLayout Lm{}, Lb{};
+    Ctx cm{}, cb{};
+    if (cache) {
+        if ((st = poisson_arrivals(cache, splitmix64(sp->seed * 2 + 1), rate_c, cache->mspec.n_reqs)))
+            return st;
+        if ((st = fit_layout<kMcR, kMcW>(cache, Lm))) return st;
+        base_ctx(cache, cm);
+        cm.n_serial_waves = 0;
+        cm.n_txns = cache->mspec.n_reqs;
+        cm.req_key = cache->mc_key;
+        cm.req_op = cache->mc_op;
+        cm.n_sets = cache->mspec.n_sets;
+        cm.n_ways = cache->mspec.n_ways;
+        cm.n_slots = cache->mspec.n_sets * cache->mspec.n_ways;
+        cm.mc_batch = cache->mspec.batch;
+        cm.log_ts = cache->mc_ts;
+        cm.log_res = cache->mc_res;
+        cm.log_vsum = cache->mc_vsum;
+        cm.log_ab = cache->mc_ab;
+        cm.log_lat = cache->mc_lat;
+        cm.n_norm = cm.n_txns;
+        cm.arr = cache->arr;
+    }
+    if (bank) {
+        if ((st = poisson_arrivals(bank, splitmix64(sp->seed * 2 + 2), rate_b, bank->spec.n_txns)))
+            return st;
+        if ((st = fit_layout<kServeBankR, kBankW>(bank, Lb))) return st;
+        base_ctx(bank, cb);
+        cb.n_serial_waves = 0;
+        cb.op = bank->op;
+        cb.addr = bank->addr;
+        cb.n_txns = bank->spec.n_txns;
+        cb.log_ts = bank->log_ts;
+        cb.log_v = bank->log_v;
+        cb.log_ab = bank->log_ab;
+        cb.log_lat = bank->log_lat;
+        cb.norm_list = bank->routed ? bank->norm_list : nullptr;
+        cb.n_norm = bank->routed ? bank->n_norm : bank->spec.n_txns;
+        cb.arr = bank->arr;
+    }
+    for (Ctx *c : {&cm, &cb}) {
+        c->t_start = (uint64_t *)(p->ctl + kOffServeStart);
+        c->inflight = (uint64_t *)(p->ctl + kOffInflight);
+        c->serve_ctr = (uint64_t *)(p->ctl + kOffServeCtr);
+        c->budget = sp->pool_budget;
+    }
+    const uint32_t words = std::max(cache ? Lm.words : 0u, bank ? Lb.words : 0u);
+    const size_t lds_bytes = (size_t)words * 4;
+    if (lds_bytes > p->lds_max) return fail(p, CSMV_E_CAPACITY, "serving LDS layout too large");
+    const uint32_t wg = p->cfg.wg_waves;
+    ServeRoles roles{};
+    roles.has_cache = cache ? 1u : 0u;
+    roles.has_bank = bank ? 1u : 0u;
+    roles.cache_wgs = cache ? (cache->cfg.client_waves + wg - 1) / wg : 0u;
+    const uint32_t bank_wgs = bank ? (bank->cfg.client_waves + wg - 1) / wg : 0u;
+    const uint32_t grid = roles.has_cache + roles.has_bank + roles.cache_wgs + bank_wgs;
+    auto kern = serve_kernel<kMcR, kMcW, kServeBankR, kBankW>;
+    HIPCHK(p, hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds_bytes));
+    int per_cu = 0;
+    HIPCHK(p, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, (int)(wg * kWave),
+                                                           lds_bytes));
+    if ((uint64_t)per_cu * p->n_cu < grid)
+        return fail(p, CSMV_E_CAPACITY, "serving grid exceeds what the device keeps resident");
+    if (cache && (st = reset_run_state(cache, Lm))) return st;
+    if (bank && (st = reset_run_state(bank, Lb))) return st;
+    if (cache) HIPCHK(cache, hipStreamSynchronize(cache->stream));
+    if (bank) HIPCHK(bank, hipStreamSynchronize(bank->stream));
+
+    HIPCHK(p, hipEventRecord(p->ev0, p->stream));
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(wg * kWave), lds_bytes, p->stream, cm, Lm, cb, Lb,
+                       roles);
+    HIPCHK(p, hipGetLastError());
+    HIPCHK(p, hipEventRecord(p->ev1, p->stream));
+    HIPCHK(p, hipEventSynchronize(p->ev1));
+    float ms = 0.f;
+    HIPCHK(p, hipEventElapsedTime(&ms, p->ev0, p->ev1));
+    uint32_t derr_c = 0, derr_b = 0;
+    csmv_serve_stats o{};
+    if (cache) {
+        cache->pool_loaded = false;
+        cache->served = 1;
+        if ((st = collect_stats(cache, Lm, ms, &o.cache, &derr_c))) return st;
+    }
+    if (bank) {
+        bank->pool_loaded = false;
+        bank->served = 2;
+        bank->bank_logged = false;
+        if ((st = collect_stats(bank, Lb, ms, &o.bank, &derr_b))) return st;
+    }
+    uint64_t words3[3] = {0, 0, 0};
+    uint64_t inflight = 0;
+    HIPCHK(p, hipMemcpy(words3, p->ctl + kOffServeCtr, 16, hipMemcpyDeviceToHost));
+    HIPCHK(p, hipMemcpy(&inflight, p->ctl + kOffInflight, 8, hipMemcpyDeviceToHost));
+    o.kernel_ms = ms;
+    o.admit_stalls = words3[0];
+    o.peak_inflight = words3[1];
+    o.final_inflight = inflight;
+    o.wallclock_khz = p->wallclock_khz;
+    if (out) *out = o;
+    if ((st = device_error(cache ? cache : bank, cache ? derr_c : derr_b))) return st;
+    if (cache && bank) return device_error(bank, derr_b);
+    return CSMV_OK;
+}
+
+csmv_status csmv_serve_read_arrivals(csmv_engine *e, uint32_t *arr_ticks, size_t n)
+{
+    if (!e || !arr_ticks) return CSMV_E_ARG;
+    const uint64_t T = e->served == 1 ? e->mspec.n_reqs : e->served == 2 ? e->spec.n_txns : 0;
+    if (!T || n != T) return fail(e, CSMV_E_ARG, "arrivals: one tick per request of the served stream");
+    HIPCHK(e, hipMemcpy(arr_ticks, e->arr, T * 4, hipMemcpyDeviceToHost));
+    return CSMV_OK;
+}
+
+csmv_status csmv_serve_read_latency(csmv_engine *e, uint32_t *lat_ticks, size_t n)
+{
+    if (!e || !lat_ticks) return CSMV_E_ARG;
+    const uint64_t T = e->served == 1 ? e->mspec.n_reqs : e->served == 2 ? e->spec.n_txns : 0;
+    if (!T || n != T) return fail(e, CSMV_E_ARG, "latency: one entry per request of the served stream");
+    HIPCHK(e, hipMemcpy(lat_ticks, e->served == 1 ? e->mc_lat : e->log_lat, T * 4,
+                        hipMemcpyDeviceToHost));
     return CSMV_OK;
 }
 

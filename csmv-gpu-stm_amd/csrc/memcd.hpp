@@ -20,6 +20,7 @@
 #pragma once
 
 #include "csmv_device.hpp"
+#include "serve.hpp"
 
 namespace csmv {
 
@@ -51,10 +52,14 @@ template <int R, int W> __device__ __forceinline__ void memcd_client(const Ctx &
     const uint32_t lane = lane_id();
     const uint64_t NT = c.n_txns;
     const uint32_t ns = c.n_slots, ways = c.n_ways;
-    bool has = false, exhausted = false;
+    // serving loop: requests become eligible at their arrival tick and are admitted by
+    // in-flight version bytes; latency counts from arrival
+    const bool serving = c.arr != nullptr;
+    const uint64_t t0 = serving ? serve_start(c) : 0;
+    bool has = false, adm = false, exhausted = false;
     uint64_t k = 0, t_pick = 0;
-    uint32_t ab = 0, my_last = 0;
-    uint64_t n_upd = 0, n_ro = 0, n_age = 0, t_reply = 0, t_round = 0, rounds = 0;
+    uint32_t ab = 0, my_last = 0, held = 0;
+    uint64_t n_upd = 0, n_ro = 0, n_age = 0, stalls = 0, t_reply = 0, t_round = 0, rounds = 0;
     for (;;) {
         if (ld_rlx(c.err)) break;
         const uint64_t tr0 = wall_clock64();
@@ -70,6 +75,7 @@ template <int R, int W> __device__ __forceinline__ void memcd_client(const Ctx &
                     const uint64_t mine = base + prefix_count(need);
                     if (mine < NT) {
                         has = true;
+                        adm = false;
                         k = mine;
                         ab = 0;
                         t_pick = wall_clock64();
@@ -78,6 +84,16 @@ template <int R, int W> __device__ __forceinline__ void memcd_client(const Ctx &
             }
         }
         if (ballot(has) == 0) break;
+        if (serving) {
+            const uint32_t need =
+                has ? (c.req_op[k] == kMcSet ? kMcInstall : 1u) * kVersionBytes : 0u;
+            serve_gate(c, t0, has, adm, k, need, held, stalls);
+        }
+        const bool act = has && (!serving || adm);
+        if (serving && ballot(act) == 0) {
+            __builtin_amdgcn_s_sleep(2);
+            continue;
+        }
 
         Tx<R, W> tx;
         if (!tx_begin(c, tx, c.begin_wait ? my_last : 0u)) break;
@@ -87,7 +103,7 @@ template <int R, int W> __device__ __forceinline__ void memcd_client(const Ctx &
 #pragma unroll
         for (int i = 0; i < kMcInstall; ++i) { iwa[i] = kNone; iwv[i] = 0; }
         int inw = 0;
-        if (has) {
+        if (act) {
             const uint64_t key = c.req_key[k];
             const uint32_t op = c.req_op[k];
             const uint32_t lo = (uint32_t)key, hi = (uint32_t)(key >> 32);
@@ -152,15 +168,17 @@ template <int R, int W> __device__ __forceinline__ void memcd_client(const Ctx &
             }
         }
         if (tx.overflow) raise_error(c, kErrReadSetOverflow);
-        const bool ok_exec = has && !tx.aborted;
-        if (has && tx.aborted) { ++ab; ++n_age; }
+        const bool ok_exec = act && !tx.aborted;
+        if (act && tx.aborted) { ++ab; ++n_age; }
+        bool done = false;
         if (ok_exec && !(res & 2u)) {
             // read-only: commits at its snapshot without a round trip (memcdSH.cu:176-183)
             c.log_ts[k] = tx.snap;
             c.log_res[k] = res;
             c.log_vsum[k] = vsum;
             c.log_ab[k] = ab;
-            c.log_lat[k] = (uint32_t)(wall_clock64() - t_pick);
+            c.log_lat[k] = (uint32_t)(wall_clock64() - (serving ? t0 + c.arr[k] : t_pick));
+            done = true;
             has = false;
             ++n_ro;
         }
@@ -178,7 +196,8 @@ template <int R, int W> __device__ __forceinline__ void memcd_client(const Ctx &
                     c.log_res[k] = res;
                     c.log_vsum[k] = vsum;
                     c.log_ab[k] = ab;
-                    c.log_lat[k] = (uint32_t)(wall_clock64() - t_pick);
+                    c.log_lat[k] = (uint32_t)(wall_clock64() - (serving ? t0 + c.arr[k] : t_pick));
+                    done = true;
                     has = false;
                     ++n_upd;
                 } else {
@@ -186,6 +205,7 @@ template <int R, int W> __device__ __forceinline__ void memcd_client(const Ctx &
                 }
             }
         }
+        if (serving) serve_release(c, done, held);
         if (has && ab > c.max_attempts) raise_error(c, kErrLivelock);
         t_round += wall_clock64() - tr0;
         ++rounds;
@@ -194,6 +214,7 @@ template <int R, int W> __device__ __forceinline__ void memcd_client(const Ctx &
         fetch_add(&c.prof[kProfClientReply], t_reply);
         fetch_add(&c.prof[kProfClientRound], t_round);
         fetch_add(&c.prof[kProfClientRounds], rounds);
+        if (stalls) fetch_add(&c.serve_ctr[0], stalls);
     }
     if (n_upd) fetch_add(&c.stats[kStatCommitUpd], n_upd);
     if (n_ro) fetch_add(&c.stats[kStatCommitRo], n_ro);

@@ -1,0 +1,254 @@
+// serve.hpp -- the persistent serving loop's client side: arrival-gated pick-up,
+// admission by in-flight version-pool bytes, latency from arrival to commit, and the
+// wide bank transaction (2..64 reads) with one batched snapshot read.
+//
+// The reference serves memcached requests batch by batch from a host loop, one kernel
+// launch per batch against a persistent server (memcachedGPU/memcdSH.cu:603-651).  Here
+// requests wait in device-resident queues (the resident streams in arrival order, each
+// with its arrival tick) and are admitted lane by lane: a lane that commits takes the
+// next arrived request in the same round (continuous batching), so requests of
+// different ages and sizes share a wavefront and overlap on the device.
+#pragma once
+
+#include "csmv_device.hpp"
+
+namespace csmv {
+
+// Version-pool bytes a request may install: (version, value) + head word per item.
+constexpr uint32_t kVersionBytes = 12;
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v)
+{
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) v += (uint32_t)__shfl_xor((int)v, m, 64);
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
+{
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) {
+        const uint32_t o = (uint32_t)__shfl_up((int)v, m, 64);
+        if (lane >= (uint32_t)m) v += o;
+    }
+    return v;
+}
+
+// The serving start: the first client wave to run publishes its clock; arrival ticks
+// count from there.
+__device__ __forceinline__ uint64_t serve_start(const Ctx &c)
+{
+    uint64_t t = 0;
+    if (lane_id() == 0) {
+        const uint64_t now = wall_clock64();
+        uint64_t expected = 0;
+        t = __hip_atomic_compare_exchange_strong(c.t_start, &expected, now, __ATOMIC_RELAXED,
+                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                ? now
+                : expected;
+    }
+    return bcast64(t);
+}
+
+// Admission.  Lanes holding an arrived request ask for `need` bytes of in-flight
+// versions; the longest lane-order prefix of them that fits under the budget is admitted
+// with one compare-and-swap (so *inflight never exceeds the budget).  A refused round
+// counts as a stall; the lanes ask again next round.
+__device__ __forceinline__ void serve_gate(const Ctx &c, uint64_t t0, bool has, bool &adm,
+                                           uint64_t k, uint32_t need, uint32_t &held,
+                                           uint64_t &stalls)
+{
+    const uint64_t now = wall_clock64() - t0;
+    const bool cand = has && !adm && (uint64_t)c.arr[k] <= now;
+    if (ballot(cand) == 0) return;
+    if (c.budget == 0) {
+        if (cand) { adm = true; held = 0; }
+        return;
+    }
+    const uint32_t incl = wave_incl_scan(cand ? need : 0u);
+    bool ok = false;
+    uint64_t take = 0;
+    for (int attempt = 0; attempt < 8 && !ok; ++attempt) {
+        const uint64_t cur = bcast64(ld_rlx(c.inflight));
+        const uint64_t room = c.budget > cur ? c.budget - cur : 0;
+        const uint64_t fit = ballot(cand && incl <= room);
+        if (fit == 0) break;
+        // the admitted prefix ends at the highest fitting candidate
+        const uint32_t last = 63u - (uint32_t)__builtin_clzll(fit);
+        take = (uint64_t)readlane(incl, last);
+        bool won = false;
+        if (lane_id() == 0) {
+            uint64_t expected = cur;
+            won = __hip_atomic_compare_exchange_strong(c.inflight, &expected, cur + take,
+                                                       __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_AGENT);
+            if (won) atomicMax((unsigned long long *)&c.serve_ctr[1],
+                               (unsigned long long)(cur + take));
+        }
+        ok = bcast(won ? 1u : 0u) != 0;
+        if (ok && cand && incl <= take) {
+            adm = true;
+            held = need;
+        }
+    }
+    if (!ok) ++stalls;
+}
+
+// Returns the bytes of lanes that committed this round to the budget.
+__device__ __forceinline__ void serve_release(const Ctx &c, bool done, uint32_t &held)
+{
+    if (c.budget == 0) return;
+    const uint32_t s = wave_sum(done ? held : 0u);
+    if (done) held = 0;
+    if (s && lane_id() == 0) __hip_atomic_fetch_add(c.inflight, (uint64_t)0 - s, __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Snapshot reads of up to N items (kNone entries are skipped): all head words, then all
+// head slots, as two rounds of independent loads; a head newer than the snapshot falls
+// back to the ring walk.
+template <int N>
+__device__ __forceinline__ bool pool_read_chunk(const Ctx &c, const uint32_t (&it)[N], uint32_t snap,
+                                                int32_t (&out)[N])
+{
+    uint32_t ht[N];
+    uint64_t x[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) ht[j] = it[j] != kNone ? ld_rlx(&c.ht[it[j]]) : 0u;
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+        x[j] = it[j] != kNone ? ld_rlx(&c.slot[(size_t)(ht[j] & 0xFFFFu) * c.n_items + it[j]]) : 0ull;
+    bool ok = true;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        int32_t v = (int32_t)(uint32_t)x[j];
+        if (it[j] != kNone && (uint32_t)(x[j] >> 32) > snap) ok = pool_read(c, it[j], snap, v) && ok;
+        out[j] = v;
+    }
+    return ok;
+}
+
+// Serving-loop bank transaction: a transfer over the first two of nr accounts
+// (2 <= nr <= R) that also reads the other nr - 2 (bank/csmv_sh/exe.cu:184-201 widened
+// to the serving mix's read sets).  Same commit path and log as the bank driver, plus
+// the latency from arrival to commit.
+template <int R, int W> __device__ __forceinline__ void bank_serve_client(const Ctx &c, uint32_t cw)
+{
+    const uint32_t lane = lane_id();
+    const uint64_t T = c.n_txns;
+    const uint64_t NT = c.n_norm;
+    const uint64_t t0 = serve_start(c);
+    bool has = false, adm = false, exhausted = false;
+    uint64_t k = 0;
+    uint32_t op = 0, ab = 0, my_last = 0, held = 0;
+    uint64_t n_upd = 0, n_age = 0, stalls = 0, t_reply = 0, t_round = 0, rounds = 0;
+    for (;;) {
+        if (ld_rlx(c.err)) break;
+        const uint64_t tr0 = wall_clock64();
+        if (!exhausted) {
+            const uint64_t need = ballot(!has);
+            if (need) {
+                const uint32_t cnt = (uint32_t)__builtin_popcountll(need);
+                uint64_t base = 0;
+                if (lane == 0) base = fetch_add(c.next_txn, (uint64_t)cnt);
+                base = bcast64(base);
+                if (base + cnt >= NT) exhausted = true;
+                if (!has) {
+                    const uint64_t mine = base + prefix_count(need);
+                    if (mine < NT) {
+                        has = true;
+                        adm = false;
+                        k = c.norm_list ? c.norm_list[mine] : mine;
+                        op = c.op[k];
+                        ab = 0;
+                    }
+                }
+            }
+        }
+        if (ballot(has) == 0) break;
+        serve_gate(c, t0, has, adm, k, W * kVersionBytes, held, stalls);
+        const bool act = has && adm;
+        if (ballot(act) == 0) {
+            __builtin_amdgcn_s_sleep(2);
+            continue;
+        }
+
+        Tx<R, W> tx;
+        if (!tx_begin(c, tx, c.begin_wait ? my_last : 0u)) break;
+        const uint32_t amt = (op >> 8) & 0xFFu;
+        const uint32_t nr = act ? min((op >> 16) & 0xFFu, (uint32_t)R) : 0u;
+        // program order over snapshot values, 16 reads at a time: read a, write a - amt,
+        // read b (own write if b == a), write b + amt, then the other reads (own writes
+        // first).  The read set stays in the stream (the mailbox rows are copied from it
+        // at commit) and each read value goes to the log at once: only the committed
+        // attempt's rows are kept.
+        bool ok = true;
+        constexpr int kChunk = 16;
+#pragma unroll 1
+        for (int c0 = 0; c0 < R; c0 += kChunk) {
+            if ((uint32_t)c0 < nr) {
+                uint32_t it[kChunk];
+                int32_t v[kChunk];
+#pragma unroll
+                for (int j = 0; j < kChunk; ++j)
+                    it[j] = (uint32_t)(c0 + j) < nr ? c.addr[(uint64_t)(c0 + j) * T + k] : kNone;
+                ok = pool_read_chunk<kChunk>(c, it, tx.snap, v) && ok;
+#pragma unroll
+                for (int j = 0; j < kChunk; ++j) {
+                    if (it[j] == kNone) continue;
+                    int32_t x = v[j];
+#pragma unroll
+                    for (int w = 0; w < W; ++w)
+                        x = ((uint32_t)w < tx.nw && tx.wa[w] == it[j]) ? tx.wv[w] : x;
+                    if (c0 + j == 0) tx_write(tx, it[j], (int32_t)((uint32_t)x - amt));
+                    else if (c0 + j == 1) tx_write(tx, it[j], (int32_t)((uint32_t)x + amt));
+                    c.log_v[(uint64_t)(c0 + j) * T + k] = x;
+                }
+            }
+        }
+        tx.nr = nr;
+        tx.aborted = tx.aborted || !ok;
+        if (tx.overflow) raise_error(c, kErrReadSetOverflow);
+        if (act && tx.aborted) { ++ab; ++n_age; }
+        const bool req = act && !tx.aborted;
+        bool done = false;
+        if (ballot(req)) {
+            bool fail = false;
+            const uint32_t ts = tx_commit<R, W, W, true>(c, cw, tx, req, &fail, &t_reply, false,
+                                                         nullptr, nullptr, -1, c.addr + k, T);
+            if (fail) break;
+            const uint64_t cm = ballot(ts != 0);
+            if (cm) my_last = readlane(ts, 63u - (uint32_t)__builtin_clzll(cm));
+            if (req) {
+                if (ts) {
+                    c.log_ts[k] = ts;
+                    c.log_ab[k] = ab;
+                    c.log_lat[k] = (uint32_t)(wall_clock64() - t0 - c.arr[k]);
+                    done = true;
+                    has = false;
+                    ++n_upd;
+                } else {
+                    ++ab;
+                }
+            }
+        }
+        serve_release(c, done, held);
+        if (has && ab > c.max_attempts) raise_error(c, kErrLivelock);
+        t_round += wall_clock64() - tr0;
+        ++rounds;
+    }
+    if (lane == 0) {
+        fetch_add(&c.prof[kProfClientReply], t_reply);
+        fetch_add(&c.prof[kProfClientRound], t_round);
+        fetch_add(&c.prof[kProfClientRounds], rounds);
+        if (stalls) fetch_add(&c.serve_ctr[0], stalls);
+    }
+    if (n_upd) fetch_add(&c.stats[kStatCommitUpd], n_upd);
+    if (n_age) fetch_add(&c.stats[kStatAbortAge], n_age);
+    fence_rel();
+    if (lane == 0) __hip_atomic_fetch_add(c.clients_done, 1u, __ATOMIC_RELEASE,
+                                          __HIP_MEMORY_SCOPE_AGENT);
+}
+
+}  // namespace csmv

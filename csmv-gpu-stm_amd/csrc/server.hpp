@@ -74,6 +74,10 @@ struct ServerLayout {
 // transaction id and amount, prod packs each read's producer (the lower lane whose
 // write it must see: lane | slot << 6, 0xFF = none) and pval holds committed values
 // gathered at staging.
+// The serialized lane (server-executed requests) is compiled for narrow wires only: its
+// per-read staging rows and chain tables grow with R.
+template <int R> constexpr bool kDelegation = R <= 8;
+
 template <int R, int W> struct Rows {
     static constexpr uint32_t snap = 0;
     static constexpr uint32_t items = 1;              // R reads then W writes
@@ -225,9 +229,12 @@ template <int R, int W>
 __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, uint32_t *lds)
 {
     using RW = Rows<R, W>;
+    // a wide wire is validated with stamps only, and its read set stays in the staged
+    // rows (LDS) instead of registers
+    constexpr bool kWide = R > 16;
     const uint32_t lane = lane_id();
     const uint32_t E = c.atr_entries, emask = E - 1;
-    const uint32_t vmode = c.variant & 0x3u;
+    const uint32_t vmode = kWide ? kValStamp : (c.variant & 0x3u);
     const bool batched = (c.variant & kInsertBatch) != 0;
     const bool server_wb = (c.variant & kServerWriteback) != 0;
     uint32_t *mbst = lds + L.state_off;
@@ -272,14 +279,20 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
         const uint64_t t1 = wall_clock64();
         const uint32_t cwr = mbwave[mbi];
         const uint32_t cw = cwr & ~kDelegateBit;
-        const bool delegate = (cwr & kDelegateBit) != 0;
+        const bool delegate = kDelegation<R> && (cwr & kDelegateBit) != 0;
         const uint32_t tg = mbtg[mbi];
         const uint32_t *box = lds + L.mb_off + mbi * L.mb_rows * kWave;
         const uint32_t snap = box[RW::snap * kWave + lane];
-        uint32_t rs[R], wa[W], prod[(R + 3) / 4], pst[W];
+        uint32_t rs[kWide ? 1 : R], wa[W], prod[(R + 3) / 4], pst[W];
         int32_t wv[W], pval[R];
+        if constexpr (!kWide) {
 #pragma unroll
-        for (int r = 0; r < R; ++r) rs[r] = box[(RW::items + r) * kWave + lane];
+            for (int r = 0; r < R; ++r) rs[r] = box[(RW::items + r) * kWave + lane];
+        }
+        auto RS = [&](int r) -> uint32_t {
+            if constexpr (kWide) return box[(RW::items + r) * kWave + lane];
+            else return rs[r];
+        };
 #pragma unroll
         for (int w = 0; w < W; ++w) {
             wa[w] = box[(RW::items + R + w) * kWave + lane];
@@ -325,12 +338,12 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const uint32_t pj = (prod[r / 4] >> (8 * (r % 4))) & 0xFFu;
-                const bool root_item = valid && rs[r] != kNone && pj == 0xFFu;
+                const bool root_item = valid && RS(r) != kNone && pj == 0xFFu;
                 base[r] = 0;
                 need_hbm_r[r] = false;
                 if (root_item) {
-                    if (L.exact) base[r] = (int32_t)V.lval[rs[r]];
-                    else if (V.bucket[bucket_of(L, rs[r])] <= tg) base[r] = pval[r];
+                    if (L.exact) base[r] = (int32_t)V.lval[RS(r)];
+                    else if (V.bucket[bucket_of(L, RS(r))] <= tg) base[r] = pval[r];
                     else need_hbm_r[r] = true;
                 }
             }
@@ -347,7 +360,7 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
 #pragma unroll
             for (int r = 0; r < R; ++r)
                 if (need_hbm_r[r])
-                    base[r] = __hip_atomic_load(&c.vals[rs[r]], __ATOMIC_RELAXED,
+                    base[r] = __hip_atomic_load(&c.vals[RS(r)], __ATOMIC_RELAXED,
                                                 __HIP_MEMORY_SCOPE_WORKGROUP);
 #pragma unroll
             for (int w = 0; w < W; ++w)
@@ -425,7 +438,7 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
                 const uint32_t pj = (prod[r / 4] >> (8 * (r % 4))) & 0xFFu;
                 bool mine_w = false;
 #pragma unroll
-                for (int w = 0; w < W; ++w) mine_w |= wa[w] == rs[r];
+                for (int w = 0; w < W; ++w) mine_w |= wa[w] == RS(r);
                 if (okx && pj != 0xFFu && mine_w) sup[(pj & 63u) * W + (pj >> 6)] = 1u;
             }
             wave_fence();
@@ -438,7 +451,7 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
                 c.log_ts[k] = myts;
 #pragma unroll
                 for (int r = 0; r < R; ++r)
-                    if (rs[r] != kNone) c.log_v[(uint64_t)r * c.n_txns + k] = rd[r];
+                    if (RS(r) != kNone) c.log_v[(uint64_t)r * c.n_txns + k] = rd[r];
             }
             uint64_t *rv = c.reply_v + ((size_t)cw * kWave + lane) * 2 * W;
 #pragma unroll
@@ -452,7 +465,7 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
                 if (valid && L.exact) {
 #pragma unroll
                     for (int r = 0; r < R; ++r)
-                        if (rs[r] != kNone) conflict |= V.bucket[rs[r]] > snap;
+                        if (RS(r) != kNone) conflict |= V.bucket[RS(r)] > snap;
 #pragma unroll
                     for (int w = 0; w < W; ++w)
                         if (wa[w] != kNone) conflict |= V.bucket[wa[w]] > snap;
@@ -463,7 +476,7 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
                     uint32_t ex[R + W];
 #pragma unroll
                     for (int r = 0; r < R + W; ++r) {
-                        const uint32_t x = r < R ? rs[r] : wa[r - R];
+                        const uint32_t x = r < R ? RS(r) : wa[r - R];
                         ex[r] = 0;
                         if (x != kNone && V.bucket[bucket_of(L, x)] > tg)
                             ex[r] = __hip_atomic_load(&c.stamps[x], __ATOMIC_RELAXED,
@@ -492,7 +505,14 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
 #pragma unroll
                             for (int w = 0; w < W; ++w) {
                                 const uint32_t x = atr[w * E + e];
-                                if (x != kNone) conflict |= hits<R, W>(x, rs, wa);
+                                if (x != kNone) {
+                                    bool h = false;
+#pragma unroll
+                                    for (int r = 0; r < R; ++r) h |= RS(r) == x;
+#pragma unroll
+                                    for (int q = 0; q < W; ++q) h |= wa[q] == x;
+                                    conflict |= h;
+                                }
                             }
                         }
                     }
@@ -507,7 +527,7 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
                         const uint32_t sc = readlane(snap, cl);
                         uint32_t rsc[R], wac[W];
 #pragma unroll
-                        for (int r = 0; r < R; ++r) rsc[r] = readlane(rs[r], cl);
+                        for (int r = 0; r < R; ++r) rsc[r] = readlane(RS(r), cl);
 #pragma unroll
                         for (int w = 0; w < W; ++w) wac[w] = readlane(wa[w], cl);
                         const uint32_t K = last - sc;
@@ -682,6 +702,7 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
                                          uint32_t rid, uint32_t M)
 {
     using RW = Rows<R, W>;
+    constexpr bool kWide = R > 16;
     const bool stamp_mode = (c.variant & 0x3u) == kValStamp;
     const uint32_t lane = lane_id();
     const uint64_t lt = (1ull << lane) - 1;
@@ -747,142 +768,206 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
         fence_acq();
         const uint32_t cwr = (uint32_t)e;
         const uint32_t cw = cwr & ~kDelegateBit;
-        const bool delegate = (cwr & kDelegateBit) != 0;
+        const bool delegate = kDelegation<R> && (cwr & kDelegateBit) != 0;
         const uint32_t *src = c.mbox + (size_t)cw * in_rows * kWave;
         uint32_t *dst = lds + L.mb_off + mbi * L.mb_rows * kWave;
-        const uint32_t snap = ld_rlx(&src[lane]);
-        uint32_t it[R + W], wv[W];
-#pragma unroll
-        for (int r = 0; r < R + W; ++r) it[r] = ld_rlx(&src[(1 + r) * kWave + lane]);
-#pragma unroll
-        for (int w = 0; w < W; ++w) wv[w] = ld_rlx(&src[(1 + R + W + w) * kWave + lane]);
-        const bool valid = snap != kNone;
-
         // commits already visible in the HBM tables when the request is staged
         const uint32_t tg = lds_ld_acq(&lds[kLdsStampTs]);
-        bool pref = false;
-        int32_t pval[R];
-        uint32_t pst[W];
+        if constexpr (kWide) {
+            // wide wire (normal lane, STAMP validation): the writes go into the hash
+            // first, then the read rows stream through in chunks: load, prefetch the
+            // exact stamps, look up the lower-lane writers, copy to the staging rows
+            const uint32_t snap = ld_rlx(&src[lane]);
+            const bool valid = snap != kNone;
+            uint32_t wa[W], wv[W], slot_w[W];
 #pragma unroll
-        for (int r = 0; r < R; ++r) pval[r] = 0;
+            for (int w = 0; w < W; ++w) {
+                wa[w] = ld_rlx(&src[(1 + R + w) * kWave + lane]);
+                wv[w] = ld_rlx(&src[(1 + R + W + w) * kWave + lane]);
+                slot_w[w] = kNone;
+                if (valid && wa[w] != kNone) {
+                    slot_w[w] = hash_claim(hkey, hmask, wa[w]);
+                    if (slot_w[w] != kNone)
+                        atomicOr(lane < 32 ? &hwlo[slot_w[w]] : &hwhi[slot_w[w]], mybit);
+                }
+            }
+            wg_fence();
+            uint64_t conf = 0;
+            uint32_t p = 0;
+            constexpr int kChunk = 16;
 #pragma unroll
-        for (int w = 0; w < W; ++w) pst[w] = 0;
-        if (!L.exact && valid) {
-            if (delegate) {
+            for (int c0 = 0; c0 < R + W; c0 += kChunk) {
+                uint32_t x[kChunk], st[kChunk];
 #pragma unroll
-                for (int r = 0; r < R; ++r)
-                    if (it[r] != kNone)
-                        pval[r] = __hip_atomic_load(&c.vals[it[r]], __ATOMIC_RELAXED,
-                                                    __HIP_MEMORY_SCOPE_WORKGROUP);
+                for (int j = 0; j < kChunk; ++j) {
+                    const int r = c0 + j;
+                    x[j] = r < R ? ld_rlx(&src[(1 + r) * kWave + lane]) : r < R + W ? wa[r - R] : kNone;
+                }
 #pragma unroll
-                for (int w = 0; w < W; ++w)
-                    if (it[R + w] != kNone)
-                        pst[w] = __hip_atomic_load(&c.stamps[it[R + w]], __ATOMIC_RELAXED,
-                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
-            } else if (stamp_mode) {
-                uint32_t p = 0;
+                for (int j = 0; j < kChunk; ++j)
+                    st[j] = (!L.exact && valid && x[j] != kNone)
+                                ? __hip_atomic_load(&c.stamps[x[j]], __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_WORKGROUP)
+                                : 0u;
 #pragma unroll
-                for (int r = 0; r < R + W; ++r)
-                    if (it[r] != kNone) {
-                        const uint32_t v = __hip_atomic_load(&c.stamps[it[r]], __ATOMIC_RELAXED,
-                                                             __HIP_MEMORY_SCOPE_WORKGROUP);
-                        p = v > p ? v : p;
+                for (int j = 0; j < kChunk; ++j) {
+                    const int r = c0 + j;
+                    if (r < R + W) {
+                        p = st[j] > p ? st[j] : p;
+                        if (valid && x[j] != kNone) {
+                            const uint32_t h = hash_find(hkey, hmask, x[j]);
+                            if (h != kNone) conf |= wmask(h) & lt;
+                        }
+                        dst[(RW::items + r) * kWave + lane] = x[j];
                     }
-                pref = p > snap;
-            }
-        }
-
-        // Conflicts with lower lanes of the request.  Insert the write set (and for a
-        // delegated request the read set) into an LDS hash of item -> lane masks, then
-        // look up: conf = lower lanes that write an item this lane reads or writes
-        // (and, delegated, lower lanes that read an item this lane writes).
-        uint32_t slot_w[W], slot_r[R];
-#pragma unroll
-        for (int w = 0; w < W; ++w) {
-            slot_w[w] = kNone;
-            const uint32_t x = it[R + w];
-            if (valid && x != kNone) {
-                slot_w[w] = hash_claim(hkey, hmask, x);
-                if (slot_w[w] != kNone) atomicOr(lane < 32 ? &hwlo[slot_w[w]] : &hwhi[slot_w[w]], mybit);
-            }
-        }
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            slot_r[r] = kNone;
-            const uint32_t x = it[r];
-            if (delegate && valid && x != kNone) {
-                slot_r[r] = hash_claim(hkey, hmask, x);
-                if (slot_r[r] != kNone) atomicOr(lane < 32 ? &hrlo[slot_r[r]] : &hrhi[slot_r[r]], mybit);
-            }
-        }
-        wg_fence();
-        uint64_t conf = 0;
-        uint32_t prod[(R + 3) / 4];
-#pragma unroll
-        for (int q = 0; q < (R + 3) / 4; ++q) prod[q] = 0xFFFFFFFFu;
-        // producer of read r: the highest lower lane writing the item (0xFF: none)
-        uint32_t plane[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) plane[r] = 0xFFu;
-#pragma unroll
-        for (int r = 0; r < R + W; ++r) {
-            const uint32_t x = it[r];
-            if (valid && x != kNone) {
-                const uint32_t h = hash_find(hkey, hmask, x);
-                if (h != kNone) {
-                    const uint64_t wm = wmask(h) & lt;
-                    conf |= wm;
-                    if (delegate && r >= R) conf |= rmask(h) & lt;
-                    if (delegate && r < R && wm) plane[r < R ? r : 0] = 63u - (uint32_t)__builtin_clzll(wm);
                 }
             }
-        }
-        if (delegate) {
-            // which write slot of the producer holds the item; the shuffles run on
-            // every lane (a ds_bpermute from an inactive lane is undefined)
+            wg_fence();
 #pragma unroll
+            for (int w = 0; w < W; ++w)
+                if (slot_w[w] != kNone) {
+                    hkey[slot_w[w]] = kNone;
+                    hwlo[slot_w[w]] = 0; hwhi[slot_w[w]] = 0; hrlo[slot_w[w]] = 0; hrhi[slot_w[w]] = 0;
+                }
+            dst[RW::snap * kWave + lane] = snap;
+#pragma unroll
+            for (int w = 0; w < W; ++w) dst[(RW::wv + w) * kWave + lane] = wv[w];
+            dst[RW::conf_lo * kWave + lane] = (uint32_t)conf;
+            dst[RW::conf_hi * kWave + lane] = (uint32_t)(conf >> 32);
+            dst[RW::pref * kWave + lane] = (!L.exact && valid && p > snap) ? 1u : 0u;
+        } else {
+            const uint32_t snap = ld_rlx(&src[lane]);
+            uint32_t it[R + W], wv[W];
+    #pragma unroll
+            for (int r = 0; r < R + W; ++r) it[r] = ld_rlx(&src[(1 + r) * kWave + lane]);
+    #pragma unroll
+            for (int w = 0; w < W; ++w) wv[w] = ld_rlx(&src[(1 + R + W + w) * kWave + lane]);
+            const bool valid = snap != kNone;
+
+            bool pref = false;
+            int32_t pval[R];
+            uint32_t pst[W];
+    #pragma unroll
+            for (int r = 0; r < R; ++r) pval[r] = 0;
+    #pragma unroll
+            for (int w = 0; w < W; ++w) pst[w] = 0;
+            if (!L.exact && valid) {
+                if (delegate) {
+    #pragma unroll
+                    for (int r = 0; r < R; ++r)
+                        if (it[r] != kNone)
+                            pval[r] = __hip_atomic_load(&c.vals[it[r]], __ATOMIC_RELAXED,
+                                                        __HIP_MEMORY_SCOPE_WORKGROUP);
+    #pragma unroll
+                    for (int w = 0; w < W; ++w)
+                        if (it[R + w] != kNone)
+                            pst[w] = __hip_atomic_load(&c.stamps[it[R + w]], __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+                } else if (stamp_mode) {
+                    uint32_t p = 0;
+    #pragma unroll
+                    for (int r = 0; r < R + W; ++r)
+                        if (it[r] != kNone) {
+                            const uint32_t v = __hip_atomic_load(&c.stamps[it[r]], __ATOMIC_RELAXED,
+                                                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+                            p = v > p ? v : p;
+                        }
+                    pref = p > snap;
+                }
+            }
+
+            // Conflicts with lower lanes of the request.  Insert the write set (and for a
+            // delegated request the read set) into an LDS hash of item -> lane masks, then
+            // look up: conf = lower lanes that write an item this lane reads or writes
+            // (and, delegated, lower lanes that read an item this lane writes).
+            uint32_t slot_w[W], slot_r[R];
+    #pragma unroll
+            for (int w = 0; w < W; ++w) {
+                slot_w[w] = kNone;
+                const uint32_t x = it[R + w];
+                if (valid && x != kNone) {
+                    slot_w[w] = hash_claim(hkey, hmask, x);
+                    if (slot_w[w] != kNone) atomicOr(lane < 32 ? &hwlo[slot_w[w]] : &hwhi[slot_w[w]], mybit);
+                }
+            }
+    #pragma unroll
             for (int r = 0; r < R; ++r) {
-                const uint32_t p = plane[r];
-                const int srcl = (int)(p == 0xFFu ? lane : p);
-                uint32_t code = p;
-#pragma unroll
-                for (int w = 0; w < W; ++w) {
-                    const uint32_t pw = (uint32_t)__shfl((int)it[R + w], srcl, 64);
-                    code = (p != 0xFFu && pw == it[r]) ? (p | ((uint32_t)w << 6)) : code;
+                slot_r[r] = kNone;
+                const uint32_t x = it[r];
+                if (delegate && valid && x != kNone) {
+                    slot_r[r] = hash_claim(hkey, hmask, x);
+                    if (slot_r[r] != kNone) atomicOr(lane < 32 ? &hrlo[slot_r[r]] : &hrhi[slot_r[r]], mybit);
                 }
-                const uint32_t sh = 8u * (r % 4);
-                prod[r / 4] = (prod[r / 4] & ~(0xFFu << sh)) | (code << sh);
             }
-        }
-        wg_fence();
-#pragma unroll
-        for (int w = 0; w < W; ++w)
-            if (slot_w[w] != kNone) {
-                hkey[slot_w[w]] = kNone;
-                hwlo[slot_w[w]] = 0; hwhi[slot_w[w]] = 0; hrlo[slot_w[w]] = 0; hrhi[slot_w[w]] = 0;
+            wg_fence();
+            uint64_t conf = 0;
+            uint32_t prod[(R + 3) / 4];
+    #pragma unroll
+            for (int q = 0; q < (R + 3) / 4; ++q) prod[q] = 0xFFFFFFFFu;
+            // producer of read r: the highest lower lane writing the item (0xFF: none)
+            uint32_t plane[R];
+    #pragma unroll
+            for (int r = 0; r < R; ++r) plane[r] = 0xFFu;
+    #pragma unroll
+            for (int r = 0; r < R + W; ++r) {
+                const uint32_t x = it[r];
+                if (valid && x != kNone) {
+                    const uint32_t h = hash_find(hkey, hmask, x);
+                    if (h != kNone) {
+                        const uint64_t wm = wmask(h) & lt;
+                        conf |= wm;
+                        if (delegate && r >= R) conf |= rmask(h) & lt;
+                        if (delegate && r < R && wm) plane[r < R ? r : 0] = 63u - (uint32_t)__builtin_clzll(wm);
+                    }
+                }
             }
-#pragma unroll
-        for (int r = 0; r < R; ++r)
-            if (slot_r[r] != kNone) {
-                hkey[slot_r[r]] = kNone;
-                hwlo[slot_r[r]] = 0; hwhi[slot_r[r]] = 0; hrlo[slot_r[r]] = 0; hrhi[slot_r[r]] = 0;
+            if (delegate) {
+                // which write slot of the producer holds the item; the shuffles run on
+                // every lane (a ds_bpermute from an inactive lane is undefined)
+    #pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const uint32_t p = plane[r];
+                    const int srcl = (int)(p == 0xFFu ? lane : p);
+                    uint32_t code = p;
+    #pragma unroll
+                    for (int w = 0; w < W; ++w) {
+                        const uint32_t pw = (uint32_t)__shfl((int)it[R + w], srcl, 64);
+                        code = (p != 0xFFu && pw == it[r]) ? (p | ((uint32_t)w << 6)) : code;
+                    }
+                    const uint32_t sh = 8u * (r % 4);
+                    prod[r / 4] = (prod[r / 4] & ~(0xFFu << sh)) | (code << sh);
+                }
             }
+            wg_fence();
+    #pragma unroll
+            for (int w = 0; w < W; ++w)
+                if (slot_w[w] != kNone) {
+                    hkey[slot_w[w]] = kNone;
+                    hwlo[slot_w[w]] = 0; hwhi[slot_w[w]] = 0; hrlo[slot_w[w]] = 0; hrhi[slot_w[w]] = 0;
+                }
+    #pragma unroll
+            for (int r = 0; r < R; ++r)
+                if (slot_r[r] != kNone) {
+                    hkey[slot_r[r]] = kNone;
+                    hwlo[slot_r[r]] = 0; hwhi[slot_r[r]] = 0; hrlo[slot_r[r]] = 0; hrhi[slot_r[r]] = 0;
+                }
 
-        dst[RW::snap * kWave + lane] = snap;
-#pragma unroll
-        for (int r = 0; r < R + W; ++r) dst[(RW::items + r) * kWave + lane] = it[r];
-#pragma unroll
-        for (int w = 0; w < W; ++w) dst[(RW::wv + w) * kWave + lane] = wv[w];
-        dst[RW::conf_lo * kWave + lane] = (uint32_t)conf;
-        dst[RW::conf_hi * kWave + lane] = (uint32_t)(conf >> 32);
-        dst[RW::pref * kWave + lane] = pref ? 1u : 0u;
-        if (delegate) {
-#pragma unroll
-            for (int q = 0; q < (R + 3) / 4; ++q) dst[(RW::prod + q) * kWave + lane] = prod[q];
-#pragma unroll
-            for (int r = 0; r < R; ++r) dst[(RW::pval + r) * kWave + lane] = (uint32_t)pval[r];
-#pragma unroll
-            for (int w = 0; w < W; ++w) dst[(RW::pstamp + w) * kWave + lane] = pst[w];
+            dst[RW::snap * kWave + lane] = snap;
+    #pragma unroll
+            for (int r = 0; r < R + W; ++r) dst[(RW::items + r) * kWave + lane] = it[r];
+    #pragma unroll
+            for (int w = 0; w < W; ++w) dst[(RW::wv + w) * kWave + lane] = wv[w];
+            dst[RW::conf_lo * kWave + lane] = (uint32_t)conf;
+            dst[RW::conf_hi * kWave + lane] = (uint32_t)(conf >> 32);
+            dst[RW::pref * kWave + lane] = pref ? 1u : 0u;
+            if (delegate) {
+    #pragma unroll
+                for (int q = 0; q < (R + 3) / 4; ++q) dst[(RW::prod + q) * kWave + lane] = prod[q];
+    #pragma unroll
+                for (int r = 0; r < R; ++r) dst[(RW::pval + r) * kWave + lane] = (uint32_t)pval[r];
+    #pragma unroll
+                for (int w = 0; w < W; ++w) dst[(RW::pstamp + w) * kWave + lane] = pst[w];
+            }
         }
         if (lane == 0) {
             mbwave[mbi] = cwr;

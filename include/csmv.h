@@ -73,12 +73,13 @@ typedef struct {
 typedef struct {
     uint64_t seed;
     uint64_t n_txns;
-    uint32_t n_reads;      /* reads per transfer (2..4); the first two are the transfer pair */
+    uint32_t n_reads;      /* reads per transfer (2..4 for csmv_bank_run, 2..64 for the
+                              serving loop); the first two are the transfer pair */
     uint32_t n_reads_ro;   /* reads per audit (1..64) */
     uint32_t pct_ro;       /* percent audits 0..100 */
     uint32_t dist;         /* 0 uniform, 1 Zipf */
     uint32_t amount_max;   /* amount uniform in 1..amount_max (1..255) */
-    uint32_t reserved;
+    uint32_t n_reads_min;  /* 0: transfers read n_reads; else uniform in n_reads_min..n_reads */
     double theta;          /* Zipf exponent */
 } csmv_bank_spec;
 
@@ -186,6 +187,46 @@ csmv_status csmv_wallclock_khz(csmv_engine *e, uint32_t *khz);
  * microbenchmark kernel: the ceiling unit for the per-lane commit protocol. */
 csmv_status csmv_atomic_rate(csmv_engine *e, uint32_t waves, uint32_t iters,
                              double *atomics_per_s);
+
+/* ---- serving loop ----------------------------------------------------------------------
+ * One persistent kernel serves a cache partition (an engine holding a cache request
+ * stream, csmv_memcd_generate) and a bank partition (an engine holding a transfer-only
+ * bank stream, read sets up to 64) as requests arrive.  The reference serves cache
+ * requests batch by batch from a host loop against a persistent server
+ * (memcachedGPU/memcdSH.cu:603-651); here arrivals are a Poisson process per class
+ * (rate = offered_rps * pct_cache / 100 for cache requests, the rest for bank
+ * transactions), requests wait in device-resident queues, a lane that commits takes the
+ * next arrived request at once (continuous batching), and admission is capped by the
+ * in-flight version-pool bytes: 12 bytes per item a request may install (cache GET 1,
+ * SET 12, bank transfer 2).  The partitions hold disjoint data, so each has its own
+ * commit server and clock.  Either engine may be NULL (that class served alone, at the
+ * same arrival times).  Both need fresh pools, the STAMP variant with client
+ * write-back, and the same wg_waves. */
+typedef struct {
+    uint64_t seed;          /* arrival process */
+    double offered_rps;     /* offered load over both classes, requests per second */
+    uint32_t pct_cache;     /* percent of the offered load that is cache requests */
+    uint32_t reserved;
+    uint64_t pool_budget;   /* cap on admitted in-flight version bytes (0 = none, else >= 1024) */
+} csmv_serve_spec;
+
+typedef struct {
+    csmv_stats cache;       /* cache partition (its commit server's counters) */
+    csmv_stats bank;        /* bank partition */
+    double kernel_ms;       /* the serving kernel, HIP events */
+    uint64_t admit_stalls;  /* wave rounds whose arrived requests the budget refused */
+    uint64_t peak_inflight; /* largest admitted in-flight version bytes */
+    uint64_t final_inflight;/* admitted bytes left at the end (0 when every request committed) */
+    uint32_t wallclock_khz; /* tick rate of arrivals and latencies */
+    uint32_t reserved;
+} csmv_serve_stats;
+
+csmv_status csmv_serve_run(csmv_engine *cache, csmv_engine *bank, const csmv_serve_spec *spec,
+                           csmv_serve_stats *stats);
+/* Per request of the stream an engine served: arrival tick, and latency from arrival to
+ * commit in wall-clock ticks (queueing, admission and every attempt included). */
+csmv_status csmv_serve_read_arrivals(csmv_engine *e, uint32_t *arr_ticks, size_t n);
+csmv_status csmv_serve_read_latency(csmv_engine *e, uint32_t *lat_ticks, size_t n);
 
 /* ---- learned contention predictor ----------------------------------------------------
  * No counterpart in the reference, whose only abort record is the per-class totals of

@@ -151,6 +151,12 @@ struct Ctx {
     uint32_t *log_ts;
     int32_t *log_v;
     uint32_t *log_ab;
+    // serving loop (arr == nullptr: the whole stream is present at launch)
+    const uint32_t *arr;   // arrival tick of each request, counted from the serving start
+    uint64_t *t_start;     // serving start tick (published by the first client wave)
+    uint64_t *inflight;    // admitted in-flight version bytes (shared by the partitions)
+    uint64_t budget;       // admission cap on *inflight (0 = none)
+    uint64_t *serve_ctr;   // [0] admission rounds refused, [1] peak in-flight bytes
 };
 
 enum : uint32_t {
@@ -378,20 +384,30 @@ template <int R, int W> constexpr uint32_t mbox_rows() { return 1 + R + 2 * W; }
 // A driver whose conflicts are covered by a subset of its writes may send only that
 // subset and install the full write set: install_n >= 0 selects install_wa / install_wv
 // (install_n entries) instead of the transaction's own write set.
-template <int R, int W, int WI = W>
+//
+// A driver whose read set is a column of its stream sets kStreamRows and passes rows_src
+// (read r at rows_src[r * rows_stride], tx.nr entries) instead of keeping tx.rs.
+template <int R, int W, int WI = W, bool kStreamRows = false>
 __device__ __forceinline__ uint32_t tx_commit(const Ctx &c, uint32_t cw, Tx<R, W> &tx, bool req, bool *fail,
                                                uint64_t *wait_ticks, bool delegate = false,
                                                const uint32_t *install_wa = nullptr,
                                                const int32_t *install_wv = nullptr,
-                                               int install_n = -1)
+                                               int install_n = -1,
+                                               const uint32_t *rows_src = nullptr,
+                                               uint64_t rows_stride = 0)
 {
     const uint32_t lane = lane_id();
     constexpr uint32_t rows = mbox_rows<R, W>();
     uint32_t *mb = c.mbox + (size_t)cw * rows * kWave;
     st_rlx(&mb[lane], req ? tx.snap : kNone);
-#pragma unroll
-    for (int r = 0; r < R; ++r)
-        st_rlx(&mb[(1 + r) * kWave + lane], (req && (uint32_t)r < tx.nr) ? tx.rs[r] : kNone);
+#pragma unroll 16
+    for (int r = 0; r < R; ++r) {
+        const bool on = req && (uint32_t)r < tx.nr;
+        uint32_t x;
+        if constexpr (kStreamRows) x = on ? rows_src[(uint64_t)r * rows_stride] : kNone;
+        else x = on ? tx.rs[r] : kNone;
+        st_rlx(&mb[(1 + r) * kWave + lane], x);
+    }
 #pragma unroll
     for (int w = 0; w < W; ++w) {
         st_rlx(&mb[(1 + R + w) * kWave + lane], (req && (uint32_t)w < tx.nw) ? tx.wa[w] : kNone);

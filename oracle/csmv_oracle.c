@@ -112,13 +112,15 @@ static inline uint32_t rmax_of(const orc_bank_spec *s)
 /* Transfer/audit choice and operand draws follow the reference driver
  * (bank/csmv_sh/exe.cu:134-139 for the read-only test rnd < prRead*0xFFFF,
  * exe.cu:184-201 for the two-account transfer); the amount is drawn as
- * 1 + RAND % amount_max. */
+ * 1 + RAND % amount_max, then (n_reads_min > 0) the read-set size as
+ * n_reads_min + RAND % (n_reads - n_reads_min + 1). */
 int orc_bank_gen(const orc_bank_spec *s, const uint32_t *cdf, uint32_t *op,
                  uint32_t *addr)
 {
     if (!s || !op || !addr || s->n_accounts == 0 || s->n_reads < 2 ||
         s->n_reads > 255 || s->n_reads_ro > 255 || s->pct_ro > 100 ||
-        s->amount_max == 0 || s->amount_max > 255)
+        s->amount_max == 0 || s->amount_max > 255 ||
+        (s->n_reads_min && (s->n_reads_min < 2 || s->n_reads_min > s->n_reads)))
         return -1;
     if (s->dist == ORC_DIST_ZIPF && !cdf) return -1;
     const uint32_t rmax = rmax_of(s);
@@ -132,6 +134,9 @@ int orc_bank_gen(const orc_bank_spec *s, const uint32_t *cdf, uint32_t *op,
         if (kind == ORC_KIND_TRANSFER) {
             amt = 1 + orc_rand_r_fnc(&st) % s->amount_max;
             nr = s->n_reads;
+            /* read-set size uniform in n_reads_min..n_reads (the serving mix) */
+            if (s->n_reads_min)
+                nr = s->n_reads_min + orc_rand_r_fnc(&st) % (s->n_reads - s->n_reads_min + 1);
         } else {
             nr = s->n_reads_ro;
         }
@@ -716,8 +721,18 @@ int orc_memcd_serial(const orc_memcd_spec *s, const uint64_t *req_key,
                      const uint32_t *req_op, const uint64_t *order, int32_t *items,
                      uint32_t *log_ts, uint32_t *log_res, uint32_t *log_vsum)
 {
+    return orc_memcd_serial_from(s, NULL, req_key, req_op, order, items, log_ts, log_res,
+                                 log_vsum);
+}
+
+int orc_memcd_serial_from(const orc_memcd_spec *s, const int32_t *init,
+                          const uint64_t *req_key, const uint32_t *req_op,
+                          const uint64_t *order, int32_t *items, uint32_t *log_ts,
+                          uint32_t *log_res, uint32_t *log_vsum)
+{
     const uint64_t n_items = (uint64_t)ORC_MC_FIELDS * s->n_sets * s->n_ways;
-    memset(items, 0, sizeof(int32_t) * n_items);
+    if (init) memcpy(items, init, sizeof(int32_t) * n_items);
+    else memset(items, 0, sizeof(int32_t) * n_items);
     uint32_t ts = 0;
     for (uint64_t q = 0; q < s->n_reqs; ++q) {
         uint64_t k = order ? order[q] : q;
@@ -737,6 +752,16 @@ int64_t orc_memcd_replay(const orc_memcd_spec *s, const uint64_t *req_key,
                          const uint32_t *log_res, const uint32_t *log_vsum,
                          const int32_t *final, orc_memcd_report *rep)
 {
+    return orc_memcd_replay_from(s, NULL, req_key, req_op, log_ts, log_res, log_vsum, final,
+                                 rep);
+}
+
+int64_t orc_memcd_replay_from(const orc_memcd_spec *s, const int32_t *init,
+                              const uint64_t *req_key, const uint32_t *req_op,
+                              const uint32_t *log_ts, const uint32_t *log_res,
+                              const uint32_t *log_vsum, const int32_t *final,
+                              orc_memcd_report *rep)
+{
     const uint64_t R = s->n_reqs;
     const uint64_t n_items = (uint64_t)ORC_MC_FIELDS * s->n_sets * s->n_ways;
     memset(rep, 0, sizeof(*rep));
@@ -754,6 +779,7 @@ int64_t orc_memcd_replay(const orc_memcd_spec *s, const uint64_t *req_key,
         free(order); free(cnt); free(lst); free(it);
         return -1;
     }
+    if (init) memcpy(it, init, sizeof(int32_t) * n_items);
     for (uint64_t t = 0; t <= U; ++t) order[t] = UINT64_MAX;
     for (uint64_t k = 0; k < R; ++k) {
         uint32_t ts = log_ts[k];

@@ -59,6 +59,7 @@ typedef struct {
     uint32_t pct_ro;       /* percent of audits, 0..100 */
     uint32_t dist;         /* ORC_DIST_* */
     uint32_t amount_max;   /* transfer amount uniform in 1..amount_max */
+    uint32_t n_reads_min;  /* 0: transfers read n_reads; else uniform n_reads_min..n_reads */
     double   theta;        /* Zipf exponent */
 } orc_bank_spec;
 
@@ -175,6 +176,16 @@ int64_t orc_memcd_replay(const orc_memcd_spec *s, const uint64_t *req_key,
                          const uint32_t *req_op, const uint32_t *log_ts,
                          const uint32_t *log_res, const uint32_t *log_vsum,
                          const int32_t *final, orc_memcd_report *rep);
+/* The same two from a given initial cache (init: 12 * n_slots items; NULL = empty). */
+int orc_memcd_serial_from(const orc_memcd_spec *s, const int32_t *init,
+                          const uint64_t *req_key, const uint32_t *req_op,
+                          const uint64_t *order, int32_t *items, uint32_t *log_ts,
+                          uint32_t *log_res, uint32_t *log_vsum);
+int64_t orc_memcd_replay_from(const orc_memcd_spec *s, const int32_t *init,
+                              const uint64_t *req_key, const uint32_t *req_op,
+                              const uint32_t *log_ts, const uint32_t *log_res,
+                              const uint32_t *log_vsum, const int32_t *final,
+                              orc_memcd_report *rep);
 
 #ifdef __cplusplus
 }

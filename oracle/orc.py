@@ -22,7 +22,8 @@ def build():
 class BankSpec(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("n_txns", C.c_uint64), ("n_accounts", C.c_uint32),
                 ("n_reads", C.c_uint32), ("n_reads_ro", C.c_uint32), ("pct_ro", C.c_uint32),
-                ("dist", C.c_uint32), ("amount_max", C.c_uint32), ("theta", C.c_double)]
+                ("dist", C.c_uint32), ("amount_max", C.c_uint32), ("n_reads_min", C.c_uint32),
+                ("theta", C.c_double)]
 
 
 class ReplayReport(C.Structure):
@@ -114,9 +115,9 @@ def scramble(rank: int, n: int) -> int:
 
 
 def bank_spec(n_accounts, n_txns, seed=1, n_reads=2, n_reads_ro=32, pct_ro=10, zipf=False,
-              theta=0.99, amount_max=10) -> BankSpec:
+              theta=0.99, amount_max=10, n_reads_min=0) -> BankSpec:
     return BankSpec(seed, n_txns, n_accounts, n_reads, n_reads_ro, pct_ro, 1 if zipf else 0,
-                    amount_max, theta)
+                    amount_max, n_reads_min, theta)
 
 
 def rmax(spec: BankSpec) -> int:
@@ -196,6 +197,10 @@ def _memcd_lib():
         L.orc_memcd_replay.argtypes = [C.POINTER(MemcdSpec), P, P, P, P, P, P,
                                        C.POINTER(MemcdReport)]
         L.orc_memcd_replay.restype = C.c_int64
+        L.orc_memcd_serial_from.argtypes = [C.POINTER(MemcdSpec), P, P, P, P, P, P, P, P]
+        L.orc_memcd_replay_from.argtypes = [C.POINTER(MemcdSpec), P, P, P, P, P, P, P,
+                                            C.POINTER(MemcdReport)]
+        L.orc_memcd_replay_from.restype = C.c_int64
         L._memcd_ready = True
     return L
 
@@ -215,23 +220,25 @@ def memcd_gen(spec: MemcdSpec):
     return key, op
 
 
-def memcd_serial(spec, key, op, order=None):
+def memcd_serial(spec, key, op, order=None, init=None):
     L = _memcd_lib()
     items = np.empty(12 * spec.n_sets * spec.n_ways, dtype=np.int32)
     ts = np.zeros(spec.n_reqs, dtype=np.uint32)
     res = np.zeros(spec.n_reqs, dtype=np.uint32)
     vsum = np.zeros(spec.n_reqs, dtype=np.uint32)
     o = None if order is None else np.ascontiguousarray(order, dtype=np.uint64)
-    assert L.orc_memcd_serial(C.byref(spec), _p(key), _p(op), _p(o), _p(items), _p(ts), _p(res),
-                              _p(vsum)) == 0
+    i0 = None if init is None else np.ascontiguousarray(init, dtype=np.int32)
+    assert L.orc_memcd_serial_from(C.byref(spec), _p(i0), _p(key), _p(op), _p(o), _p(items),
+                                   _p(ts), _p(res), _p(vsum)) == 0
     return items, ts, res, vsum
 
 
-def memcd_replay(spec, key, op, ts, res, vsum, final=None):
+def memcd_replay(spec, key, op, ts, res, vsum, final=None, init=None):
     L = _memcd_lib()
     rep = MemcdReport()
-    fails = L.orc_memcd_replay(C.byref(spec), _p(key), _p(op), _p(ts), _p(res), _p(vsum),
-                               _p(final), C.byref(rep))
+    i0 = None if init is None else np.ascontiguousarray(init, dtype=np.int32)
+    fails = L.orc_memcd_replay_from(C.byref(spec), _p(i0), _p(key), _p(op), _p(ts), _p(res),
+                                    _p(vsum), _p(final), C.byref(rep))
     d = rep.as_dict()
     d["fails"] = fails
     return d

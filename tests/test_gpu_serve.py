@@ -1,0 +1,179 @@
+"""GPU tests of the persistent serving loop: Poisson arrivals of cache requests mixed with
+bank transactions of read sets 2..64, continuous batching, admission by in-flight
+version bytes, and latency from arrival to commit.
+
+Checks: every partition's log replays serially in commit order (oracle), every request
+commits once after its arrival, the budget is never exceeded, and each result is the same
+alone and in the mix.  "The same" is exact (tolerance 0) on what does not depend on the
+interleaving of a partition's own requests: GET results (hit, way, value checksum) over a
+warmed cache with no SETs in flight, the final cache, and the final bank balances
+(transfers commute); what does depend on it (the LRU "wrote" bit of a GET, a transfer's
+read values) is checked by the replay of each run.
+"""
+import numpy as np
+import pytest
+
+import csmv
+import orc
+
+pytestmark = pytest.mark.gpu
+
+INIT = 1000
+N_ACC = 1 << 16
+
+
+def cache_engine(m: csmv.Memcd, waves=16):
+    return csmv.Engine(m.n_items, depth=4, client_waves=waves, wg_waves=6, variant="stamp",
+                       timeout_ms=8000)
+
+
+def bank_engine(waves=8):
+    return csmv.Engine(N_ACC, depth=8, client_waves=waves, wg_waves=6, variant="stamp",
+                       timeout_ms=8000)
+
+
+def serve_bank(n, seed=7):
+    return csmv.Bank(n_txns=n, seed=seed, n_reads=64, n_reads_min=2, pct_ro=0, zipf=True,
+                     theta=0.99)
+
+
+def bank_spec(b):
+    return orc.bank_spec(N_ACC, b.n_txns, seed=b.seed, n_reads=b.n_reads,
+                         n_reads_ro=b.n_reads_ro, pct_ro=b.pct_ro, zipf=b.zipf, theta=b.theta,
+                         amount_max=b.amount_max, n_reads_min=b.n_reads_min)
+
+
+def memcd_spec(m):
+    return orc.memcd_spec(m.n_reqs, seed=m.seed, n_keys=m.n_keys, n_sets=m.n_sets,
+                          n_ways=m.n_ways, pct_get=m.pct_get, batch=m.batch, zipf=m.zipf,
+                          theta=m.theta)
+
+
+def check_cache(e, m, init_state=None):
+    key, op = e.memcd_read_stream()
+    log = e.memcd_read_log()
+    final = e.read_state()
+    rep = orc.memcd_replay(memcd_spec(m), key, op, log["ts"], log["res"], log["vsum"], final,
+                           init=init_state)
+    assert rep["fails"] == 0, rep
+    return log, final
+
+
+def check_bank(e, b):
+    op, addr = e.bank_read_stream()
+    ts, v, ab = e.bank_read_log()
+    final = e.read_state()
+    spec = bank_spec(b)
+    oop, oaddr = orc.bank_gen(spec)
+    assert np.array_equal(op, oop) and np.array_equal(addr, oaddr), "stream differs"
+    nr = (op >> 16) & 0xFF
+    assert nr.min() >= 2 and nr.max() <= 64 and nr.max() > 32
+    rep, _ = orc.bank_replay(spec, op, addr, INIT, ts, v, final)
+    assert rep["fails"] == 0, rep
+    assert rep["total"] == N_ACC * INIT
+    return final
+
+
+def check_latency(e, st, n):
+    arr = e.serve_arrivals()
+    lat = e.serve_latency()
+    assert np.all(np.diff(arr.astype(np.int64)) >= 0), "arrivals are not ordered"
+    assert st["commits"] == n
+    assert np.all(lat > 0)
+    return arr, lat
+
+
+def test_serve_mix_replays_and_reports_latency():
+    m = csmv.Memcd(n_reqs=1 << 15, seed=3, n_keys=20_000, n_sets=1 << 10, batch=4096)
+    b = serve_bank(1 << 13)
+    print(f"\n[case] serve mix: cache {m.n_reqs} reqs + bank {b.n_txns} txns, 20M req/s",
+          flush=True)
+    with cache_engine(m) as ec, bank_engine() as eb:
+        ec.fill(0)
+        ec.memcd_generate(m)
+        eb.fill(INIT)
+        eb.bank_generate(b)
+        out = csmv.serve(ec, eb, offered_rps=20e6, pct_cache=80, seed=5)
+        print("[serve]", {k: out[k] for k in ("kernel_ms", "admit_stalls", "peak_inflight")},
+              flush=True)
+        assert out["final_inflight"] == 0
+        check_cache(ec, m)
+        check_bank(eb, b)
+        _, lc = check_latency(ec, out["cache"], m.n_reqs)
+        _, lb = check_latency(eb, out["bank"], b.n_txns)
+        khz = out["wallclock_khz"]
+        print(f"[latency] cache p50 {np.percentile(lc, 50) / khz * 1e3:.1f} us p99 "
+              f"{np.percentile(lc, 99) / khz * 1e3:.1f} us | bank p50 "
+              f"{np.percentile(lb, 50) / khz * 1e3:.1f} us p99 "
+              f"{np.percentile(lb, 99) / khz * 1e3:.1f} us", flush=True)
+
+
+def test_serve_budget_caps_inflight_bytes():
+    m = csmv.Memcd(n_reqs=1 << 14, seed=4, n_keys=10_000, n_sets=1 << 9, pct_get=80,
+                   batch=2048)
+    b = serve_bank(1 << 12, seed=9)
+    budget = 4096
+    print(f"\n[case] serve with a {budget}-byte in-flight budget", flush=True)
+    with cache_engine(m) as ec, bank_engine() as eb:
+        ec.fill(0)
+        ec.memcd_generate(m)
+        eb.fill(INIT)
+        eb.bank_generate(b)
+        out = csmv.serve(ec, eb, offered_rps=50e6, pct_cache=80, seed=6, pool_budget=budget)
+        print("[serve]", {k: out[k] for k in ("kernel_ms", "admit_stalls", "peak_inflight")},
+              flush=True)
+        assert 0 < out["peak_inflight"] <= budget
+        assert out["admit_stalls"] > 0
+        assert out["final_inflight"] == 0
+        check_cache(ec, m)
+        check_bank(eb, b)
+        check_latency(ec, out["cache"], m.n_reqs)
+        check_latency(eb, out["bank"], b.n_txns)
+
+
+def test_each_result_same_alone_and_in_mix():
+    warm = csmv.Memcd(n_reqs=1 << 14, seed=21, n_keys=8000, n_sets=1 << 9, pct_get=0,
+                      batch=4096)
+    gets = csmv.Memcd(n_reqs=1 << 15, seed=22, n_keys=8000, n_sets=1 << 9, pct_get=100,
+                      batch=4096)
+    b = serve_bank(1 << 13, seed=23)
+    print("\n[case] alone vs mix: GET-only cache stream on a warmed cache + bank stream",
+          flush=True)
+    with cache_engine(warm) as ec, bank_engine() as eb:
+        ec.fill(0)
+        ec.memcd_generate(warm)
+        ec.memcd_run()
+        state = ec.read_state()
+        ec.memcd_generate(gets)
+        runs = {}
+        arr = None
+        for mode in ("alone", "mix"):
+            ec.load(state)
+            eb.fill(INIT)
+            eb.bank_generate(b)
+            if mode == "alone":
+                csmv.serve(ec, None, offered_rps=20e6, pct_cache=80, seed=8)
+                log, fc = check_cache(ec, gets, init_state=state)
+                arr = ec.serve_arrivals()
+                csmv.serve(None, eb, offered_rps=20e6, pct_cache=80, seed=8)
+                fb = check_bank(eb, b)
+            else:
+                csmv.serve(ec, eb, offered_rps=20e6, pct_cache=80, seed=8)
+                log, fc = check_cache(ec, gets, init_state=state)
+                fb = check_bank(eb, b)
+                # the cache class arrives at the same ticks alone and in the mix
+                assert np.array_equal(arr, ec.serve_arrivals())
+            runs[mode] = (log, fc, fb)
+    (la, fca, fba), (lm, fcm, fbm) = runs["alone"], runs["mix"]
+    mask = np.uint32(~2 & 0xFFFFFFFF)
+    assert np.array_equal(la["res"] & mask, lm["res"] & mask)
+    assert np.array_equal(la["vsum"], lm["vsum"])
+    assert (la["res"] & 1).sum() > gets.n_reqs // 4, "too few hits to be a meaningful check"
+    assert np.array_equal(fca, fcm)
+    assert np.array_equal(fba, fbm)
+    # and both equal the serial execution in arrival order
+    spec = memcd_spec(gets)
+    okey, oop = orc.memcd_gen(spec)
+    fo, _, ro, vo = orc.memcd_serial(spec, okey, oop, init=state)
+    assert np.array_equal(ro & mask, lm["res"] & mask) and np.array_equal(vo, lm["vsum"])
+    assert np.array_equal(fo, fcm)

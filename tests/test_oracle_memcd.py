@@ -75,3 +75,29 @@ def test_keys_sets_and_stream():
     L = orc._memcd_lib()
     sets = np.array([L.orc_memcd_set_of(int(k), 16) for k in key[:2000]])
     assert sets.min() >= 0 and sets.max() < 16 and len(np.unique(sets)) == 16
+
+
+def test_serial_from_initial_cache_and_get_only_order_independence():
+    """A GET-only stream over a warmed cache: results (hit, way, value checksum) and the
+    final cache do not depend on the order (timestamps end at the max clock); the
+    replay from the warm state accepts the serial log, the replay from empty rejects it."""
+    warm = orc.memcd_spec(4000, seed=11, n_keys=3000, n_sets=256, n_ways=4, pct_get=0,
+                          batch=500)
+    wkey, wop = orc.memcd_gen(warm)
+    state, _, _, _ = orc.memcd_serial(warm, wkey, wop)
+    gets = orc.memcd_spec(3000, seed=12, n_keys=3000, n_sets=256, n_ways=4, pct_get=100,
+                          batch=500)
+    key, op = orc.memcd_gen(gets)
+    assert (op == 0).all()
+    f1, ts1, res1, vs1 = orc.memcd_serial(gets, key, op, init=state)
+    rng = np.random.default_rng(3)
+    order = rng.permutation(gets.n_reqs).astype(np.uint64)
+    f2, _, res2, vs2 = orc.memcd_serial(gets, key, op, order=order, init=state)
+    assert np.array_equal(res1 & ~np.uint32(2), res2 & ~np.uint32(2))
+    assert np.array_equal(vs1, vs2)
+    assert np.array_equal(f1, f2)
+    assert (res1 & 1).sum() > 0
+    rep = orc.memcd_replay(gets, key, op, ts1, res1, vs1, f1, init=state)
+    assert rep["fails"] == 0, rep
+    bad = orc.memcd_replay(gets, key, op, ts1, res1, vs1, f1)
+    assert bad["fails"] > 0
