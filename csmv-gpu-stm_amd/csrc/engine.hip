@@ -328,7 +328,7 @@ __global__ void __launch_bounds__(512) bank_kernel(Ctx c, ServerLayout L)
 {
     extern __shared__ uint32_t lds[];
     if (blockIdx.x == 0) {
-        server_workgroup<R, W>(c, L, lds);
+        server_workgroup<R, W, true>(c, L, lds);
         return;
     }
     const uint32_t cw = (blockIdx.x - 1) * c.wg_waves + threadIdx.x / kWave;
@@ -365,7 +365,7 @@ __global__ void __launch_bounds__(512) memcd_kernel(Ctx c, ServerLayout L)
 {
     extern __shared__ uint32_t lds[];
     if (blockIdx.x == 0) {
-        server_workgroup<R, W>(c, L, lds);
+        server_workgroup<R, W, false>(c, L, lds);
         return;
     }
     const uint32_t cw = (blockIdx.x - 1) * c.wg_waves + threadIdx.x / kWave;
@@ -386,8 +386,8 @@ __global__ void __launch_bounds__(512) serve_kernel(Ctx cm, ServerLayout Lm, Ctx
     extern __shared__ uint32_t lds[];
     const uint32_t n_srv = roles.has_cache + roles.has_bank;
     if (blockIdx.x < n_srv) {
-        if (roles.has_cache && blockIdx.x == 0) server_workgroup<RM, WM>(cm, Lm, lds);
-        else server_workgroup<RB, WB>(cb, Lb, lds);
+        if (roles.has_cache && blockIdx.x == 0) server_workgroup<RM, WM, false>(cm, Lm, lds);
+        else server_workgroup<RB, WB, true>(cb, Lb, lds);
         return;
     }
     const uint32_t b = blockIdx.x - n_srv;
@@ -397,7 +397,9 @@ __global__ void __launch_bounds__(512) serve_kernel(Ctx cm, ServerLayout Lm, Ctx
         if (cw < cm.n_client_waves) memcd_client<RM, WM>(cm, cw);
     } else {
         const uint32_t cw = (b - roles.cache_wgs) * cb.wg_waves + wave;
-        if (cw < cb.n_client_waves) bank_serve_client<RB, WB>(cb, cw);
+        if (cw >= cb.n_client_waves) return;
+        if (cw >= cb.n_client_waves - cb.n_serial_waves) bank_serve_serial_client<RB, WB>(cb, cw);
+        else bank_serve_client<RB, WB>(cb, cw);
     }
 }
 
@@ -564,14 +566,16 @@ ServerLayout server_layout(const csmv_engine *e, bool delegation, uint32_t nmb =
     uint32_t w = L.mb_off + L.nmb * L.mb_rows * kWave;
     // every item a request inserts (W writes, plus R reads when delegated) fits at a
     // load factor of at most 3/4
-    L.hsz = delegation ? pow2_at_least((uint64_t)kWave * (R + W))
-                       : pow2_at_least(2ull * kWave * W);
+    L.hsz = (delegation && !kWideWire<R>) ? pow2_at_least((uint64_t)kWave * (R + W))
+                                          : pow2_at_least(2ull * kWave * W);
     L.hash_off = w;
     w += M * 5 * L.hsz;
     L.sup_off = w;
     w += kWave * W;
     L.jmp_off = w;
-    if (delegation) w += 3 * kWave * R + kWave * W;
+    // chain tables: next, weight, base per chain node; deltas (and, wide, new values)
+    // per write slot
+    if (delegation) w += 3 * kWave * kChainReads<R, W> + 2 * kWave * W;
     L.atr_off = w;
     if (vmode != kValStamp) w += W * e->cfg.atr_entries;
     L.bucket_off = w;
@@ -1566,11 +1570,11 @@ csmv_status poisson_arrivals(csmv_engine *e, uint64_t seed, double rate, uint64_
 
 // Server layout of a partition: fewer mailboxes (down to 4) when the wire is wide, so
 // that the stamp buckets keep at least 4096 entries.
-template <int R, int W> csmv_status fit_layout(csmv_engine *e, ServerLayout &L)
+template <int R, int W> csmv_status fit_layout(csmv_engine *e, ServerLayout &L, bool delegation)
 {
     uint32_t nmb = 0;
     for (;;) {
-        L = server_layout<R, W>(e, false, nmb);
+        L = server_layout<R, W>(e, delegation, nmb);
         const bool fits = (uint64_t)L.words * 4 <= e->lds_max && (L.exact || L.nbuckets >= 4096);
         if (fits) return CSMV_OK;
         if (L.nmb <= 4)
@@ -1613,7 +1617,7 @@ csmv_status csmv_serve_run(csmv_engine *cache, csmv_engine *bank, const csmv_ser
     if (cache) {
         if ((st = poisson_arrivals(cache, splitmix64(sp->seed * 2 + 1), rate_c, cache->mspec.n_reqs)))
             return st;
-        if ((st = fit_layout<kMcR, kMcW>(cache, Lm))) return st;
+        if ((st = fit_layout<kMcR, kMcW>(cache, Lm, false))) return st;
         base_ctx(cache, cm);
         cm.n_serial_waves = 0;
         cm.n_txns = cache->mspec.n_reqs;
@@ -1634,9 +1638,14 @@ csmv_status csmv_serve_run(csmv_engine *cache, csmv_engine *bank, const csmv_ser
     if (bank) {
         if ((st = poisson_arrivals(bank, splitmix64(sp->seed * 2 + 2), rate_b, bank->spec.n_txns)))
             return st;
-        if ((st = fit_layout<kServeBankR, kBankW>(bank, Lb))) return st;
+        const bool serial = bank->routed && bank->n_hot > 0;
+        if (serial && bank->cfg.n_serial_waves == 0)
+            return fail(bank, CSMV_E_ARG, "a routed bank partition needs n_serial_waves > 0");
+        if ((st = fit_layout<kServeBankR, kBankW>(bank, Lb, serial))) return st;
         base_ctx(bank, cb);
-        cb.n_serial_waves = 0;
+        cb.n_serial_waves = serial ? bank->cfg.n_serial_waves : 0;
+        cb.hot_list = serial ? bank->hot_list : nullptr;
+        cb.n_hot = serial ? bank->n_hot : 0;
         cb.op = bank->op;
         cb.addr = bank->addr;
         cb.n_txns = bank->spec.n_txns;
@@ -1709,7 +1718,9 @@ csmv_status csmv_serve_run(csmv_engine *cache, csmv_engine *bank, const csmv_ser
     o.wallclock_khz = p->wallclock_khz;
     if (out) *out = o;
     if ((st = device_error(cache ? cache : bank, cache ? derr_c : derr_b))) return st;
-    if (cache && bank) return device_error(bank, derr_b);
+    if (cache && bank && (st = device_error(bank, derr_b))) return st;
+    // the bank log of a completed run feeds the abort counters like csmv_bank_run's
+    if (bank) bank->bank_logged = true;
     return CSMV_OK;
 }
 

@@ -251,4 +251,96 @@ template <int R, int W> __device__ __forceinline__ void bank_serve_client(const 
                                           __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Serving-loop serialized lane for routed bank transactions: the wave hands them to the
+// commit server, which executes them in commit order on the newest committed values (the
+// transfer pair through its producer chains, the other reads from their producer or the
+// committed value) and logs them; the wave installs the returned versions in commit order.
+template <int R, int W> __device__ __forceinline__ void bank_serve_serial_client(const Ctx &c, uint32_t cw)
+{
+    const uint32_t lane = lane_id();
+    const uint64_t T = c.n_txns;
+    const uint64_t NH = c.n_hot;
+    const uint64_t t0 = serve_start(c);
+    bool has = false, adm = false, exhausted = false;
+    uint64_t k = 0;
+    uint32_t op = 0, ab = 0, held = 0;
+    uint64_t n_upd = 0, stalls = 0, t_reply = 0, t_round = 0, rounds = 0;
+    for (;;) {
+        if (ld_rlx(c.err)) break;
+        const uint64_t tr0 = wall_clock64();
+        if (!exhausted) {
+            const uint64_t need = ballot(!has);
+            if (need) {
+                const uint32_t cnt = (uint32_t)__builtin_popcountll(need);
+                uint64_t base = 0;
+                if (lane == 0) base = fetch_add(c.next_hot, (uint64_t)cnt);
+                base = bcast64(base);
+                if (base + cnt >= NH) exhausted = true;
+                if (!has) {
+                    const uint64_t mine = base + prefix_count(need);
+                    if (mine < NH) {
+                        has = true;
+                        adm = false;
+                        k = c.hot_list[mine];
+                        op = c.op[k];
+                        ab = 0;
+                    }
+                }
+            }
+        }
+        if (ballot(has) == 0) break;
+        serve_gate(c, t0, has, adm, k, W * kVersionBytes, held, stalls);
+        const bool act = has && adm;
+        if (ballot(act) == 0) {
+            __builtin_amdgcn_s_sleep(2);
+            continue;
+        }
+        Tx<R, W> tx;
+        tx.snap = act ? 0u : kNone;
+        tx.nw = 0;
+        tx.aborted = false;
+        tx.overflow = false;
+#pragma unroll
+        for (int w = 0; w < W; ++w) { tx.wa[w] = kNone; tx.wv[w] = 0; }
+        const uint32_t amt = (op >> 8) & 0xFFu;
+        tx.nr = act ? min((op >> 16) & 0xFFu, (uint32_t)R) : 0u;
+        if (act) {
+            tx_write(tx, c.addr[k], 0);
+            tx_write(tx, c.addr[T + k], 0);
+            tx.wv[0] = (int32_t)(uint32_t)k;
+            tx.wv[1] = (int32_t)amt;
+        }
+        bool fail = false;
+        const uint32_t ts = tx_commit<R, W, W, true>(c, cw, tx, act, &fail, &t_reply, true, nullptr,
+                                                     nullptr, -1, c.addr + k, T);
+        if (fail) break;
+        bool done = false;
+        if (act) {
+            if (ts) {
+                c.log_ab[k] = ab;
+                c.log_lat[k] = (uint32_t)(wall_clock64() - t0 - c.arr[k]);
+                done = true;
+                has = false;
+                ++n_upd;
+            } else {
+                ++ab;
+            }
+        }
+        serve_release(c, done, held);
+        if (has && ab > c.max_attempts) raise_error(c, kErrLivelock);
+        t_round += wall_clock64() - tr0;
+        ++rounds;
+    }
+    if (lane == 0) {
+        fetch_add(&c.prof[kProfClientReply], t_reply);
+        fetch_add(&c.prof[kProfClientRound], t_round);
+        fetch_add(&c.prof[kProfClientRounds], rounds);
+        if (stalls) fetch_add(&c.serve_ctr[0], stalls);
+    }
+    if (n_upd) fetch_add(&c.stats[kStatCommitUpd], n_upd);
+    fence_rel();
+    if (lane == 0) __hip_atomic_fetch_add(c.clients_done, 1u, __ATOMIC_RELEASE,
+                                          __HIP_MEMORY_SCOPE_AGENT);
+}
+
 }  // namespace csmv

@@ -74,20 +74,25 @@ struct ServerLayout {
 // transaction id and amount, prod packs each read's producer (the lower lane whose
 // write it must see: lane | slot << 6, 0xFF = none) and pval holds committed values
 // gathered at staging.
-// The serialized lane (server-executed requests) is compiled for narrow wires only: its
-// per-read staging rows and chain tables grow with R.
-template <int R> constexpr bool kDelegation = R <= 8;
+// A wide wire (read sets beyond 16) keeps its read set in the staged rows instead of
+// registers, validates with stamps only, and on the serialized lane runs the producer
+// chains over the transfer operands (the first W reads) alone: the other reads take the
+// value their producer wrote, or the committed value.
+template <int R> constexpr bool kWideWire = R > 16;
+// serialized-lane chain nodes per lane
+template <int R, int W> constexpr int kChainReads = kWideWire<R> ? W : R;
 
 template <int R, int W> struct Rows {
+    static constexpr int RO = kChainReads<R, W>;
     static constexpr uint32_t snap = 0;
     static constexpr uint32_t items = 1;              // R reads then W writes
     static constexpr uint32_t wv = 1 + R + W;
     static constexpr uint32_t conf_lo = 1 + R + 2 * W;
     static constexpr uint32_t conf_hi = 2 + R + 2 * W;
     static constexpr uint32_t pref = 3 + R + 2 * W;
-    static constexpr uint32_t prod = 4 + R + 2 * W;   // (R + 3) / 4 rows
-    static constexpr uint32_t pval = prod + (R + 3) / 4;
-    static constexpr uint32_t pstamp = pval + R;     // stamps of the writes at staging
+    static constexpr uint32_t prod = 4 + R + 2 * W;   // (R + 3) / 4 rows of producer codes
+    static constexpr uint32_t pval = prod + (R + 3) / 4;  // RO rows: operand values at staging
+    static constexpr uint32_t pstamp = pval + RO;    // stamps of the writes at staging
     static constexpr uint32_t count = pstamp + W;
 };
 
@@ -225,13 +230,13 @@ __device__ __forceinline__ void transfer_exec(const uint32_t *it, uint32_t amt, 
     }
 }
 
-template <int R, int W>
+template <int R, int W, bool D>
 __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, uint32_t *lds)
 {
     using RW = Rows<R, W>;
     // a wide wire is validated with stamps only, and its read set stays in the staged
     // rows (LDS) instead of registers
-    constexpr bool kWide = R > 16;
+    constexpr bool kWide = kWideWire<R>;
     const uint32_t lane = lane_id();
     const uint32_t E = c.atr_entries, emask = E - 1;
     const uint32_t vmode = kWide ? kValStamp : (c.variant & 0x3u);
@@ -245,10 +250,13 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
     V.bucket = lds + L.bucket_off;
     V.lval = lds + L.val_off;
     uint32_t *sup = lds + L.sup_off;
+    // serialized-lane chain tables (RO chain nodes per lane, W write slots per lane)
+    constexpr int RO = kChainReads<R, W>;
     uint32_t *jnext = lds + L.jmp_off;
-    int32_t *jw = (int32_t *)(jnext + kWave * R);
-    int32_t *jbase = jw + kWave * R;
-    int32_t *jdelta = jbase + kWave * R;
+    int32_t *jw = (int32_t *)(jnext + kWave * RO);
+    int32_t *jbase = jw + kWave * RO;
+    int32_t *jdelta = jbase + kWave * RO;
+    int32_t *jnv = jdelta + kWave * W;
     const uint32_t Q = c.qmask + 1;
     const uint64_t lt = (1ull << lane) - 1;
 
@@ -279,12 +287,12 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
         const uint64_t t1 = wall_clock64();
         const uint32_t cwr = mbwave[mbi];
         const uint32_t cw = cwr & ~kDelegateBit;
-        const bool delegate = kDelegation<R> && (cwr & kDelegateBit) != 0;
+        const bool delegate = D && (cwr & kDelegateBit) != 0;
         const uint32_t tg = mbtg[mbi];
         const uint32_t *box = lds + L.mb_off + mbi * L.mb_rows * kWave;
         const uint32_t snap = box[RW::snap * kWave + lane];
-        uint32_t rs[kWide ? 1 : R], wa[W], prod[(R + 3) / 4], pst[W];
-        int32_t wv[W], pval[R];
+        uint32_t rs[kWide ? 1 : R], wa[W], prod[kWide ? 1 : (R + 3) / 4], pst[W];
+        int32_t wv[W], pval[RO];
         if constexpr (!kWide) {
 #pragma unroll
             for (int r = 0; r < R; ++r) rs[r] = box[(RW::items + r) * kWave + lane];
@@ -301,11 +309,18 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
         const uint64_t conf = ((uint64_t)box[RW::conf_hi * kWave + lane] << 32) |
                               box[RW::conf_lo * kWave + lane];
         const bool pref = box[RW::pref * kWave + lane] != 0;
+        // producer code of read r: lane | write slot << 6 (0xFF: no lower writer)
+        auto PROD = [&](int r) -> uint32_t {
+            const uint32_t word = kWide ? box[(RW::prod + r / 4) * kWave + lane] : prod[kWide ? 0 : r / 4];
+            return (word >> (8 * (r % 4))) & 0xFFu;
+        };
         if (delegate) {
+            if constexpr (!kWide) {
 #pragma unroll
-            for (int q = 0; q < (R + 3) / 4; ++q) prod[q] = box[(RW::prod + q) * kWave + lane];
+                for (int q = 0; q < (R + 3) / 4; ++q) prod[q] = box[(RW::prod + q) * kWave + lane];
+            }
 #pragma unroll
-            for (int r = 0; r < R; ++r) pval[r] = (int32_t)box[(RW::pval + r) * kWave + lane];
+            for (int r = 0; r < RO; ++r) pval[r] = (int32_t)box[(RW::pval + r) * kWave + lane];
 #pragma unroll
             for (int w = 0; w < W; ++w) pst[w] = box[(RW::pstamp + w) * kWave + lane];
         }
@@ -332,18 +347,24 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
             // step per writer of a hot account.  The lanes exchange through LDS inside
             // this one wave, so wave-scope fences order the accesses; the HBM gathers of
             // committed values are issued first and consumed after the chain sums.
-            bool need_hbm_r[R], need_hbm_w[W];
-            int32_t base[R];
+            // Chain nodes are the first RO reads (every read of a narrow wire, the
+            // transfer operands of a wide one); a producer code names a write slot w of
+            // a lower lane, i.e. that lane's chain node w.
+            uint32_t ops[RO];
+#pragma unroll
+            for (int r = 0; r < RO; ++r) ops[r] = RS(r);
+            bool need_hbm_r[RO], need_hbm_w[W];
+            int32_t base[RO];
             uint32_t prev[W];
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const uint32_t pj = (prod[r / 4] >> (8 * (r % 4))) & 0xFFu;
-                const bool root_item = valid && RS(r) != kNone && pj == 0xFFu;
+            for (int r = 0; r < RO; ++r) {
+                const uint32_t pj = PROD(r);
+                const bool root_item = valid && ops[r] != kNone && pj == 0xFFu;
                 base[r] = 0;
                 need_hbm_r[r] = false;
                 if (root_item) {
-                    if (L.exact) base[r] = (int32_t)V.lval[RS(r)];
-                    else if (V.bucket[bucket_of(L, RS(r))] <= tg) base[r] = pval[r];
+                    if (L.exact) base[r] = (int32_t)V.lval[ops[r]];
+                    else if (V.bucket[bucket_of(L, ops[r])] <= tg) base[r] = pval[r];
                     else need_hbm_r[r] = true;
                 }
             }
@@ -358,9 +379,9 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
                 }
             }
 #pragma unroll
-            for (int r = 0; r < R; ++r)
+            for (int r = 0; r < RO; ++r)
                 if (need_hbm_r[r])
-                    base[r] = __hip_atomic_load(&c.vals[RS(r)], __ATOMIC_RELAXED,
+                    base[r] = __hip_atomic_load(&c.vals[ops[r]], __ATOMIC_RELAXED,
                                                 __HIP_MEMORY_SCOPE_WORKGROUP);
 #pragma unroll
             for (int w = 0; w < W; ++w)
@@ -370,30 +391,30 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
             tres = wall_clock64();
             jdelta[lane * W + 0] = (wa[1] == kNone) ? 0 : -(int32_t)amt;
             jdelta[lane * W + 1] = (int32_t)amt;
-            uint32_t nxt[R];
-            int32_t wgt[R];
+            uint32_t nxt[RO];
+            int32_t wgt[RO];
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const uint32_t node = lane * R + r;
-                const uint32_t pj = (prod[r / 4] >> (8 * (r % 4))) & 0xFFu;
-                nxt[r] = (pj == 0xFFu) ? node : ((pj & 63u) * R + (pj >> 6));
+            for (int r = 0; r < RO; ++r) {
+                const uint32_t node = lane * RO + r;
+                const uint32_t pj = PROD(r);
+                nxt[r] = (pj == 0xFFu) ? node : ((pj & 63u) * RO + (pj >> 6));
                 jnext[node] = nxt[r];
             }
             wave_fence();
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const uint32_t pj = (prod[r / 4] >> (8 * (r % 4))) & 0xFFu;
+            for (int r = 0; r < RO; ++r) {
+                const uint32_t pj = PROD(r);
                 wgt[r] = (pj == 0xFFu) ? 0 : jdelta[(pj & 63u) * W + (pj >> 6)];
-                jw[lane * R + r] = wgt[r];
+                jw[lane * RO + r] = wgt[r];
             }
             wave_fence();
             for (uint32_t round = 0; round < 6; ++round) {
                 bool more = false;
-                uint32_t nn[R];
-                int32_t ww[R];
+                uint32_t nn[RO];
+                int32_t ww[RO];
 #pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    const uint32_t node = lane * R + r;
+                for (int r = 0; r < RO; ++r) {
+                    const uint32_t node = lane * RO + r;
                     const uint32_t t = nxt[r];
                     const uint32_t tt = jnext[t];
                     ww[r] = wgt[r] + (t != node ? jw[t] : 0);
@@ -402,11 +423,11 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
                 }
                 wave_fence();
 #pragma unroll
-                for (int r = 0; r < R; ++r) {
+                for (int r = 0; r < RO; ++r) {
                     nxt[r] = nn[r];
                     wgt[r] = ww[r];
-                    jnext[lane * R + r] = nn[r];
-                    jw[lane * R + r] = ww[r];
+                    jnext[lane * RO + r] = nn[r];
+                    jw[lane * RO + r] = ww[r];
                 }
                 wave_fence();
                 if (ballot(more) == 0) break;
@@ -414,14 +435,14 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
             // every node now points at its chain's root; the root's base is the value
             // the chain starts from
 #pragma unroll
-            for (int r = 0; r < R; ++r) jbase[lane * R + r] = base[r];
+            for (int r = 0; r < RO; ++r) jbase[lane * RO + r] = base[r];
             wave_fence();
-            int32_t in[R], rd[R], nv[W];
+            int32_t in[RO], rd[RO], nv[W];
 #pragma unroll
-            for (int r = 0; r < R; ++r) in[r] = jbase[nxt[r]] + wgt[r];
+            for (int r = 0; r < RO; ++r) in[r] = jbase[nxt[r]] + wgt[r];
             const bool okx = valid;
             tchain = wall_clock64();
-            transfer_exec<R, W>(rs, amt, in, rd, wa, nv);
+            transfer_exec<RO, W>(ops, amt, in, rd, wa, nv);
             const uint64_t committed = ballot(okx);
             nc = (uint32_t)__builtin_popcountll(committed);
             if (okx) myts = last + 1 + (uint32_t)__builtin_popcountll(committed & lt);
@@ -432,26 +453,62 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
             // the request's timestamp range, since the clock moves over whole requests.
 #pragma unroll
             for (int w = 0; w < W; ++w) sup[lane * W + w] = 0;
+            if constexpr (kWide) {
+#pragma unroll
+                for (int w = 0; w < W; ++w) jnv[lane * W + w] = nv[w];
+            }
             wave_fence();
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const uint32_t pj = (prod[r / 4] >> (8 * (r % 4))) & 0xFFu;
+            for (int r = 0; r < RO; ++r) {
+                const uint32_t pj = PROD(r);
                 bool mine_w = false;
 #pragma unroll
-                for (int w = 0; w < W; ++w) mine_w |= wa[w] == RS(r);
+                for (int w = 0; w < W; ++w) mine_w |= wa[w] == ops[r];
                 if (okx && pj != 0xFFu && mine_w) sup[(pj & 63u) * W + (pj >> 6)] = 1u;
             }
             wave_fence();
             bool inst[W];
 #pragma unroll
             for (int w = 0; w < W; ++w) inst[w] = okx && wa[w] != kNone && sup[lane * W + w] == 0;
+            if constexpr (kWide) {
+                // the other reads see the write of their producer (a lower lane), this
+                // lane's own write, or the committed value before the request; their
+                // committed values are read before this request's values are recorded
+#pragma unroll 1
+                for (int c0 = RO; c0 < R; c0 += 16) {
+                    uint32_t x[16];
+                    int32_t v[16];
+#pragma unroll
+                    for (int j = 0; j < 16; ++j) x[j] = (c0 + j < R) ? RS(c0 + j) : kNone;
+#pragma unroll
+                    for (int j = 0; j < 16; ++j) {
+                        const bool own = x[j] != kNone && (x[j] == wa[0] || x[j] == wa[1]);
+                        const uint32_t pj = (c0 + j < R) ? PROD(c0 + j) : 0xFFu;
+                        v[j] = 0;
+                        if (okx && x[j] != kNone && !own && pj == 0xFFu)
+                            v[j] = L.exact ? (int32_t)V.lval[x[j]]
+                                           : __hip_atomic_load(&c.vals[x[j]], __ATOMIC_RELAXED,
+                                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+                    }
+#pragma unroll
+                    for (int j = 0; j < 16; ++j) {
+                        if (!okx || x[j] == kNone) continue;
+                        const uint32_t pj = PROD(c0 + j);
+                        int32_t val = v[j];
+                        if (pj != 0xFFu) val = jnv[(pj & 63u) * W + (pj >> 6)];
+#pragma unroll
+                        for (int w = 0; w < W; ++w) val = (wa[w] == x[j]) ? nv[w] : val;
+                        c.log_v[(uint64_t)(c0 + j) * c.n_txns + k] = val;
+                    }
+                }
+            }
             record_values<R, W>(c, L, V, wa, nv, inst);
             record_stamps<R, W>(c, L, V, wa, myts, okx);
             if (okx) {
                 c.log_ts[k] = myts;
 #pragma unroll
-                for (int r = 0; r < R; ++r)
-                    if (RS(r) != kNone) c.log_v[(uint64_t)r * c.n_txns + k] = rd[r];
+                for (int r = 0; r < RO; ++r)
+                    if (ops[r] != kNone) c.log_v[(uint64_t)r * c.n_txns + k] = rd[r];
             }
             uint64_t *rv = c.reply_v + ((size_t)cw * kWave + lane) * 2 * W;
 #pragma unroll
@@ -697,12 +754,12 @@ __device__ __forceinline__ uint32_t hash_find(const uint32_t *hkey, uint32_t hma
     return kNone;
 }
 
-template <int R, int W>
+template <int R, int W, bool D>
 __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, uint32_t *lds,
                                          uint32_t rid, uint32_t M)
 {
     using RW = Rows<R, W>;
-    constexpr bool kWide = R > 16;
+    constexpr bool kWide = kWideWire<R>;
     const bool stamp_mode = (c.variant & 0x3u) == kValStamp;
     const uint32_t lane = lane_id();
     const uint64_t lt = (1ull << lane) - 1;
@@ -768,7 +825,7 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
         fence_acq();
         const uint32_t cwr = (uint32_t)e;
         const uint32_t cw = cwr & ~kDelegateBit;
-        const bool delegate = kDelegation<R> && (cwr & kDelegateBit) != 0;
+        const bool delegate = D && (cwr & kDelegateBit) != 0;
         const uint32_t *src = c.mbox + (size_t)cw * in_rows * kWave;
         uint32_t *dst = lds + L.mb_off + mbi * L.mb_rows * kWave;
         // commits already visible in the HBM tables when the request is staged
@@ -794,33 +851,78 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
             wg_fence();
             uint64_t conf = 0;
             uint32_t p = 0;
-            constexpr int kChunk = 16;
+            constexpr int kChunk = 8;
+            static_assert(R % kChunk == 0, "wide wires stage whole chunks of reads");
+            constexpr int RO = Rows<R, W>::RO;
+            // the read rows, a chunk at a time
+#pragma unroll 1
+            for (int c0 = 0; c0 < R; c0 += kChunk) {
+                uint32_t x[kChunk], st[kChunk], plane[kChunk];
 #pragma unroll
-            for (int c0 = 0; c0 < R + W; c0 += kChunk) {
-                uint32_t x[kChunk], st[kChunk];
+                for (int j = 0; j < kChunk; ++j) x[j] = ld_rlx(&src[(1 + c0 + j) * kWave + lane]);
+                // staging-time stamps (normal lane) or, delegated, the committed values
+                // of the transfer operands
 #pragma unroll
                 for (int j = 0; j < kChunk; ++j) {
-                    const int r = c0 + j;
-                    x[j] = r < R ? ld_rlx(&src[(1 + r) * kWave + lane]) : r < R + W ? wa[r - R] : kNone;
+                    const bool want = !L.exact && valid && x[j] != kNone && (!delegate || c0 + j < RO);
+                    st[j] = 0u;
+                    if (want)
+                        st[j] = delegate ? (uint32_t)__hip_atomic_load(&c.vals[x[j]], __ATOMIC_RELAXED,
+                                                                       __HIP_MEMORY_SCOPE_WORKGROUP)
+                                         : __hip_atomic_load(&c.stamps[x[j]], __ATOMIC_RELAXED,
+                                                             __HIP_MEMORY_SCOPE_WORKGROUP);
                 }
 #pragma unroll
-                for (int j = 0; j < kChunk; ++j)
-                    st[j] = (!L.exact && valid && x[j] != kNone)
-                                ? __hip_atomic_load(&c.stamps[x[j]], __ATOMIC_RELAXED,
-                                                    __HIP_MEMORY_SCOPE_WORKGROUP)
-                                : 0u;
-#pragma unroll
                 for (int j = 0; j < kChunk; ++j) {
-                    const int r = c0 + j;
-                    if (r < R + W) {
-                        p = st[j] > p ? st[j] : p;
-                        if (valid && x[j] != kNone) {
-                            const uint32_t h = hash_find(hkey, hmask, x[j]);
-                            if (h != kNone) conf |= wmask(h) & lt;
+                    plane[j] = 0xFFu;
+                    if (!delegate) p = st[j] > p ? st[j] : p;
+                    if (valid && x[j] != kNone) {
+                        const uint32_t h = hash_find(hkey, hmask, x[j]);
+                        if (h != kNone) {
+                            const uint64_t wm = wmask(h) & lt;
+                            conf |= wm;
+                            if (wm) plane[j] = 63u - (uint32_t)__builtin_clzll(wm);
                         }
-                        dst[(RW::items + r) * kWave + lane] = x[j];
                     }
+                    dst[(RW::items + c0 + j) * kWave + lane] = x[j];
+                    if (delegate && c0 + j < RO) dst[(RW::pval + c0 + j) * kWave + lane] = st[j];
                 }
+                if (delegate) {
+                    // producer code: the lower writer's lane and the write slot holding
+                    // the item (the shuffles run on every lane)
+                    uint32_t word[kChunk / 4] = {};
+#pragma unroll
+                    for (int j = 0; j < kChunk; ++j) {
+                        const uint32_t pl = plane[j];
+                        const int srcl = (int)(pl == 0xFFu ? lane : pl);
+                        uint32_t code = 0xFFu;
+#pragma unroll
+                        for (int w = 0; w < W; ++w) {
+                            const uint32_t pw = (uint32_t)__shfl((int)wa[w], srcl, 64);
+                            code = (pl != 0xFFu && pw == x[j]) ? (pl | ((uint32_t)w << 6)) : code;
+                        }
+                        word[j / 4] |= code << (8 * (j % 4));
+                    }
+#pragma unroll
+                    for (int q = 0; q < kChunk / 4; ++q)
+                        dst[(RW::prod + c0 / 4 + q) * kWave + lane] = word[q];
+                }
+            }
+            // the write rows: lower lanes writing the same items, staging-time stamps
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                const uint32_t x = wa[w];
+                const uint32_t st = (!L.exact && valid && x != kNone)
+                                        ? __hip_atomic_load(&c.stamps[x], __ATOMIC_RELAXED,
+                                                            __HIP_MEMORY_SCOPE_WORKGROUP)
+                                        : 0u;
+                if (!delegate) p = st > p ? st : p;
+                if (valid && x != kNone) {
+                    const uint32_t h = hash_find(hkey, hmask, x);
+                    if (h != kNone) conf |= wmask(h) & lt;
+                }
+                dst[(RW::items + R + w) * kWave + lane] = x;
+                if (delegate) dst[(RW::pstamp + w) * kWave + lane] = st;
             }
             wg_fence();
 #pragma unroll
@@ -987,7 +1089,8 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
     }
 }
 
-template <int R, int W>
+// D: the serialized lane (delegated requests) is compiled in.
+template <int R, int W, bool D>
 __device__ __forceinline__ void server_workgroup(const Ctx &c, const ServerLayout &L, uint32_t *lds)
 {
     const uint32_t wave = threadIdx.x / kWave;
@@ -1007,10 +1110,10 @@ __device__ __forceinline__ void server_workgroup(const Ctx &c, const ServerLayou
             for (uint32_t q = 1; q < 5; ++q) h[q * L.hsz + i] = 0;
         }
     __syncthreads();
-    if (wave == 0) committer<R, W>(c, L, lds);
+    if (wave == 0) committer<R, W, D>(c, L, lds);
     else if (wave == 1) {
         if ((c.variant & kServerWriteback) == 0) advancer(c, lds);
-    } else receiver<R, W>(c, L, lds, wave - 2, M);
+    } else receiver<R, W, D>(c, L, lds, wave - 2, M);
 }
 
 }  // namespace csmv

@@ -177,3 +177,36 @@ def test_each_result_same_alone_and_in_mix():
     fo, _, ro, vo = orc.memcd_serial(spec, okey, oop, init=state)
     assert np.array_equal(ro & mask, lm["res"] & mask) and np.array_equal(vo, lm["vsum"])
     assert np.array_equal(fo, fcm)
+
+
+def test_serve_serialized_lane_for_routed_bank_transactions():
+    """Bank transactions routed to the serialized lane: the commit server executes them
+    (transfer pair through producer chains, the other reads from their producer or the
+    committed value); the log replays and no routed transaction aborts."""
+    m = csmv.Memcd(n_reqs=1 << 14, seed=31, n_keys=20_000, n_sets=1 << 10, batch=4096)
+    b = serve_bank(1 << 13, seed=32)
+    print("\n[case] serve mix with the bank partition on the serialized lane", flush=True)
+    res = {}
+    for routed in (False, True):
+        with cache_engine(m) as ec, csmv.Engine(N_ACC, depth=8, client_waves=12, wg_waves=6,
+                                                variant="stamp", timeout_ms=8000,
+                                                n_serial_waves=4) as eb:
+            ec.fill(0)
+            ec.memcd_generate(m)
+            eb.fill(INIT)
+            eb.bank_generate(b)
+            if routed:
+                op, _ = eb.bank_read_stream()
+                eb.bank_route(np.ones(b.n_txns, dtype=np.uint8))
+            out = csmv.serve(ec, eb, offered_rps=20e6, pct_cache=80, seed=5)
+            check_cache(ec, m)
+            check_bank(eb, b)
+            _, lb = check_latency(eb, out["bank"], b.n_txns)
+            _, _, ab = eb.bank_read_log()
+            if routed:
+                assert ab.sum() == 0 and out["bank"]["aborts"] == 0
+            res[routed] = (out["kernel_ms"], out["bank"]["abort_rate"], np.percentile(lb, 99))
+            print(f"[serve] routed={routed} kernel {out['kernel_ms']:.1f} ms bank abort "
+                  f"{out['bank']['abort_rate']:.3f} p99 {np.percentile(lb, 99) / out['wallclock_khz']:.2f} ms",
+                  flush=True)
+    assert res[True][2] < res[False][2]
