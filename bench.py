@@ -1,13 +1,27 @@
 #!/usr/bin/env python3
-"""Benchmark of the gfx950 CSMV engine on the 'bank at scale' workload.
+"""Benchmark of the gfx950 CSMV engine.
 
-Workload: 1,048,576 int32 accounts (initial balance 1000), account choice Zipf
-theta=0.99, transfers read 4 and write 2 accounts, 90/10 transfer/audit mix,
-audits read 4 accounts, 2^24 transactions, version depth 8.  The stream is
-generated on the device before timing (resident in HBM).  One step = one pass of
-the persistent kernel over the whole stream; the pool is reloaded between steps
-outside the timed kernel.  value = committed transactions / kernel seconds
-(HIP events on the engine's stream).
+Headline ("bank at scale"): 1,048,576 int32 accounts (initial balance 1000), account
+choice Zipf theta=0.99, transfers read 4 and write 2 accounts, 90/10 transfer/audit mix,
+audits read 4 accounts, 2^24 transactions, version depth 8.  The stream is generated on
+the device before timing (resident in HBM).  One step = one pass of the persistent kernel
+over the whole stream; the pool is reloaded between steps outside the timed kernel.
+value = committed transactions / kernel seconds (HIP events on the engine's stream).
+
+Beside it, in the same JSON line:
+  variants   the reference's four commit-path variants on the normal (client-executed)
+             lane, same workload on a shorter stream
+  memcd      the memcached-style cache at its configured shape (2^17 sets x 8 ways,
+             10M-key Zipf 0.99 keyspace, 95/5 GET/SET, batches of 65,536): req/s,
+             abort rate, pick-up-to-commit latency
+  scheduler  the learned contention predictor on the 'bank plain' shape (4096 accounts,
+             2-account transfers) at Zipf 0.6 / 0.9 / 1.2: arrival-order CSMV vs the
+             learned routing vs routing every transfer
+  serving    the persistent serving loop: Poisson arrivals of cache requests (80 %) and
+             bank transactions of 2..64 reads over 1M Zipf 0.99 accounts (20 %) at fixed
+             offered loads, p50/p99 latency from arrival to commit, with and without the
+             predictor
+  roofline / atomics / cpu_baseline   for the headline kernel
 
 Prints ONE JSON line.  One GPU only: --gpus above 1 exits with a message.
 """
@@ -27,6 +41,10 @@ HBM_PEAK_GBS = 8000.0
 N_ACCOUNTS = 1 << 20
 INIT = 1000
 REF_VARIANTS = ("vanilla", "batchedAtrInsert", "clientWriteback", "collabValidation")
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def bytes_per_commit(stats, n_reads=4, n_writes=2, n_reads_ro=4):
@@ -52,10 +70,17 @@ def load_traffic(commits):
         return None
 
 
+def pct_us(lat_ticks, khz):
+    import numpy as np
+    if lat_ticks.size == 0:
+        return {"p50_us": None, "p99_us": None}
+    return {"p50_us": float(np.percentile(lat_ticks, 50)) / khz * 1e3,
+            "p99_us": float(np.percentile(lat_ticks, 99)) / khz * 1e3}
+
+
 def cpu_baseline(op, addr, n_txns, sample):
     """The CPU restatement of the reference protocol (oracle), on the host cores."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import numpy as np
     import orc
 
     threads = max(2, min(16, os.cpu_count() or 2))
@@ -76,6 +101,157 @@ def cpu_baseline(op, addr, n_txns, sample):
     }
 
 
+def memcd_section(csmv, args):
+    m = csmv.Memcd(n_reqs=args.memcd_reqs, seed=1)
+    log(f"memcd: {m.n_reqs} requests, {m.n_sets} sets x {m.n_ways} ways")
+    with csmv.Engine(m.n_items, depth=4, client_waves=args.memcd_waves, wg_waves=6,
+                     variant="stamp", timeout_ms=20000) as e:
+        e.fill(0)
+        e.memcd_generate(m)
+        st = e.memcd_run()
+        lg = e.memcd_read_log()
+        khz = e.wallclock_khz()
+    return {"req_per_s": st["txn_per_s"], "abort_rate": st["abort_rate"],
+            "kernel_ms": st["kernel_ms"], "requests": m.n_reqs, "client_waves": args.memcd_waves,
+            "latency_pickup_to_commit": pct_us(lg["lat_ticks"], khz),
+            "workload": "2^17 sets x 8 ways, 64-bit keys Zipf 0.99 over 10M, 32-byte values, "
+                        "95/5 GET/SET, LRU clock per batch of 65,536"}
+
+
+def scheduler_section(csmv, args):
+    """C4: train on abort logs at theta 0.6 / 0.9 / 1.2, calibrate, evaluate."""
+    import numpy as np
+    from csmv import sched
+
+    def eng(serial):
+        return csmv.Engine(4096, depth=4, atr_entries=4096, client_waves=64, wg_waves=8,
+                           variant="stamp", timeout_ms=20000, n_serial_waves=serial)
+
+    out = {"workload": "4096 accounts, 2-account transfers, 32-account audits, 90/10, "
+                       f"Zipf theta; {args.sched_txns} txns per evaluation stream; "
+                       "64 client waves (routed: 48 normal + 16 serialized-lane)"}
+    with eng(0) as plain, eng(16) as routed:
+        t0 = time.time()
+        rep = sched.train(plain, n_txns=1 << 17, steps=300)
+        cal = sched.calibrate(plain, routed)
+        out["training"] = {"samples": rep.samples, "positive_rate": rep.positive_rate,
+                           "loss_first": float(rep.loss[0]), "loss_last": float(rep.loss[-1]),
+                           "loss_constant_predictor": rep.constant_loss, "steps": 300,
+                           "tau": cal["tau"], "seconds": time.time() - t0}
+        log(f"scheduler: trained ({rep.samples} samples), tau={cal['tau']}")
+        rows = {}
+        for th in (0.6, 0.9, 1.2):
+            bank = sched.c1_bank(args.sched_txns, seed=77, theta=th)
+            sched.history(plain, sched.c1_bank(1 << 17, seed=76, theta=th))
+            sched.transfer_state(plain, routed)
+            sp = sched.run_logged(plain, bank)
+            routed.bank_generate(bank)
+            nh = sched.route(routed, cal["tau"])
+            routed.fill(INIT)
+            sl = routed.bank_run()
+            op, _ = routed.bank_read_stream()
+            routed.bank_route(((op & 0xFF) == 0).astype(np.uint8))
+            routed.fill(INIT)
+            sa = routed.bank_run()
+            rows[str(th)] = {
+                "plain": {"txn_per_s": sp["txn_per_s"], "abort_rate": sp["abort_rate"]},
+                "learned": {"txn_per_s": sl["txn_per_s"], "abort_rate": sl["abort_rate"],
+                            "routed": nh},
+                "all_transfers_routed": {"txn_per_s": sa["txn_per_s"],
+                                         "abort_rate": sa["abort_rate"]},
+                "learned_over_plain": sl["txn_per_s"] / sp["txn_per_s"],
+            }
+            log(f"scheduler theta={th}: plain {sp['txn_per_s']:.3g} learned "
+                f"{sl['txn_per_s']:.3g} all {sa['txn_per_s']:.3g}")
+        out["theta"] = rows
+        params = plain.pred_get_params()
+    return out, params, cal["tau"]
+
+
+def class_rate(arr, lat):
+    """Requests per second of one class over its own window: first arrival to last
+    commit (arrival + latency), in ticks."""
+    import numpy as np
+    end = (arr.astype(np.int64) + lat.astype(np.int64)).max()
+    return arr.size, int(end - int(arr.min()))
+
+
+def serving_section(csmv, args, params, tau):
+    """C5: offered-load sweep, with and without the predictor."""
+    import numpy as np
+
+    loads = [float(x) for x in args.serve_loads.split(",")]
+    m = csmv.Memcd(n_reqs=args.serve_cache_reqs, seed=1)
+    b = csmv.Bank(n_txns=args.serve_bank_txns, seed=7, n_reads=64, n_reads_min=2, pct_ro=0,
+                  zipf=True, theta=0.99)
+
+    def hist(seed):
+        return csmv.Bank(n_txns=1 << 13, seed=seed, n_reads=64, n_reads_min=2, pct_ro=0,
+                         zipf=True, theta=0.99)
+
+    res = {"workload": f"{m.n_reqs} cache requests (the first requests of the cache config "
+                       f"stream, cold cache; 80 %) + {b.n_txns} bank transactions (transfer "
+                       "over the first 2 of 2..64 Zipf 0.99 reads of 1M accounts; 20 %), "
+                       "Poisson arrivals; cache 48 client waves, bank 16 (8 serialized-lane "
+                       "when routed)",
+           "loads": {}}
+    with csmv.Engine(m.n_items, depth=4, client_waves=48, wg_waves=6, variant="stamp",
+                     timeout_ms=30000) as ec, \
+         csmv.Engine(N_ACCOUNTS, depth=8, client_waves=16, wg_waves=6, variant="stamp",
+                     timeout_ms=30000, n_serial_waves=8) as eb:
+        ec.memcd_generate(m)
+        # the predictor (trained on the bank-plain abort logs) is fine-tuned on this
+        # partition's own logs: history stream 1 gives the counters, stream 2 the labels
+        t0 = time.time()
+        eb.pred_set_params(params)
+        for i, seed in enumerate((101, 102)):
+            eb.fill(INIT)
+            eb.bank_generate(hist(seed))
+            csmv.serve(None, eb, offered_rps=loads[0], pct_cache=80, seed=3 + i)
+            if i == 0:
+                eb.abort_fold(32)
+            else:
+                eb.pred_clear_dataset()
+                n = eb.pred_collect()
+                loss = eb.pred_train(200, 1e-2)
+                eb.abort_fold(1)
+        res["predictor"] = {"tau": tau, "finetune_samples": n, "loss_first": float(loss[0]),
+                            "loss_last": float(loss[-1]), "seconds": time.time() - t0}
+        eb.bank_generate(b)
+        n_hot = eb.bank_route_learned(tau)
+        for mode in ("without_predictor", "with_predictor"):
+            for load in loads:
+                if mode == "without_predictor":
+                    eb.bank_route(None)
+                else:
+                    eb.bank_route_learned(tau)
+                ec.fill(0)
+                eb.fill(INIT)
+                out = csmv.serve(ec, eb, offered_rps=load, pct_cache=80, seed=1)
+                khz = out["wallclock_khz"]
+                lc, lb = ec.serve_latency(), eb.serve_latency()
+                ac, ab = ec.serve_arrivals(), eb.serve_arrivals()
+                nc_, wc = class_rate(ac, lc)
+                nb_, wb = class_rate(ab, lb)
+                _, wall = class_rate(np.concatenate([ac, ab]), np.concatenate([lc, lb]))
+                row = {"offered_rps": load, "kernel_ms": out["kernel_ms"],
+                       "offered_window_ms": float(max(ac[-1], ab[-1])) / khz,
+                       "req_per_s": (nc_ + nb_) / (wall / (khz * 1e3)),
+                       "cache_req_per_s": nc_ / (wc / (khz * 1e3)),
+                       "bank_txn_per_s": nb_ / (wb / (khz * 1e3)),
+                       "all": pct_us(np.concatenate([lc, lb]), khz), "cache": pct_us(lc, khz),
+                       "bank": pct_us(lb, khz),
+                       "cache_abort_rate": out["cache"]["abort_rate"],
+                       "bank_abort_rate": out["bank"]["abort_rate"],
+                       "bank_routed": n_hot if mode == "with_predictor" else 0,
+                       "admit_stalls": out["admit_stalls"]}
+                res["loads"].setdefault(mode, []).append(row)
+                log(f"serving {mode} {load:.3g} req/s: {row['req_per_s']:.3g} served, p99 "
+                    f"{row['all']['p99_us']:.1f} us (cache {row['cache']['p99_us']:.1f}, bank "
+                    f"{row['bank']['p99_us']:.1f}), routed {row['bank_routed']}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,9 +265,18 @@ def main():
                     help="none: every transaction on the normal (client) lane; transfers: "
                          "every transfer on the serialized lane")
     ap.add_argument("--txns", type=int, default=1 << 24)
-    ap.add_argument("--variant-txns", type=int, default=1 << 18,
+    ap.add_argument("--variant-txns", type=int, default=1 << 16,
                     help="stream length for the side-by-side reference variants")
+    ap.add_argument("--memcd-reqs", type=int, default=1 << 24)
+    ap.add_argument("--memcd-waves", type=int, default=64)
+    ap.add_argument("--sched-txns", type=int, default=1 << 18)
+    ap.add_argument("--serve-loads", default="5e6,50e6,100e6,200e6,500e6")
+    ap.add_argument("--serve-cache-reqs", type=int, default=1 << 18)
+    ap.add_argument("--serve-bank-txns", type=int, default=1 << 16)
     ap.add_argument("--no-variants", action="store_true")
+    ap.add_argument("--no-memcd", action="store_true")
+    ap.add_argument("--no-sched", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=1 << 22)
     args = ap.parse_args()
@@ -100,12 +285,12 @@ def main():
               f"(the CSMV commit path has one global clock per engine)", file=sys.stderr)
         sys.exit(2)
 
+    import numpy as np
+
     import csmv
 
     bank = csmv.Bank(n_txns=args.txns, seed=1, n_reads=4, n_reads_ro=4, pct_ro=10, zipf=True,
                      theta=0.99)
-
-    import numpy as np
 
     def make(variant, b, router, serial):
         e = csmv.Engine(N_ACCOUNTS, depth=8, atr_entries=4096, client_waves=args.waves,
@@ -117,6 +302,7 @@ def main():
             e.bank_route(((op & 0xFF) == 0).astype(np.uint8))
         return e
 
+    log(f"headline: {args.txns} txns, router {args.router}")
     e = make(args.variant, bank, args.router, args.serial_waves)
     for _ in range(args.warmup):
         e.fill(INIT)
@@ -137,6 +323,7 @@ def main():
     if not args.no_cpu:
         op, addr = e.bank_read_stream()
     e.close()
+    log(f"headline: {value:.4g} txn/s")
 
     alg_bytes = bytes_per_commit(runs[-1])
     achieved = alg_bytes / (runs[-1]["kernel_ms"] * 1e-3) / 1e9
@@ -151,6 +338,7 @@ def main():
         vbank = csmv.Bank(n_txns=args.variant_txns, seed=1, n_reads=4, n_reads_ro=4, pct_ro=10,
                           zipf=True, theta=0.99)
         for v in REF_VARIANTS + ("stamp",):
+            log(f"variant {v}")
             ev = make(v, vbank, "none", 0)
             ev.fill(INIT)
             r = ev.bank_run()
@@ -195,7 +383,20 @@ def main():
                     "value_over_plain_ceiling": value / plain_ceiling},
         "variants": variants,
     }
+    if not args.no_memcd:
+        out["memcd"] = memcd_section(csmv, args)
+    params, tau = None, 0.02
+    if not args.no_sched:
+        out["scheduler"], params, tau = scheduler_section(csmv, args)
+    if not args.no_serve:
+        if params is None:
+            with csmv.Engine(4096, depth=4, client_waves=64, wg_waves=8, variant="stamp") as pe:
+                from csmv import sched
+                sched.train(pe, n_txns=1 << 17, steps=300)
+                params = pe.pred_get_params()
+        out["serving"] = serving_section(csmv, args, params, tau)
     if not args.no_cpu:
+        log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(op, addr, args.txns, min(args.cpu_sample, args.txns))
     print(json.dumps(out))
 
