@@ -192,11 +192,11 @@ def serving_section(csmv, args, params, tau):
     res = {"workload": f"{m.n_reqs} cache requests (the first requests of the cache config "
                        f"stream, cold cache; 80 %) + {b.n_txns} bank transactions (transfer "
                        "over the first 2 of 2..64 Zipf 0.99 reads of 1M accounts; 20 %), "
-                       "Poisson arrivals; cache 48 client waves, bank 16 (8 serialized-lane "
-                       "when routed)",
+                       f"Poisson arrivals; cache {args.serve_cache_waves} client waves, bank 16 "
+                       "(8 serialized-lane when routed)",
            "loads": {}}
-    with csmv.Engine(m.n_items, depth=4, client_waves=48, wg_waves=6, variant="stamp",
-                     timeout_ms=30000) as ec, \
+    with csmv.Engine(m.n_items, depth=4, client_waves=args.serve_cache_waves, wg_waves=6,
+                     variant="stamp", timeout_ms=30000) as ec, \
          csmv.Engine(N_ACCOUNTS, depth=8, client_waves=16, wg_waves=6, variant="stamp",
                      timeout_ms=30000, n_serial_waves=8) as eb:
         ec.memcd_generate(m)
@@ -268,11 +268,12 @@ def main():
     ap.add_argument("--variant-txns", type=int, default=1 << 16,
                     help="stream length for the side-by-side reference variants")
     ap.add_argument("--memcd-reqs", type=int, default=1 << 24)
-    ap.add_argument("--memcd-waves", type=int, default=64)
+    ap.add_argument("--memcd-waves", type=int, default=8)
     ap.add_argument("--sched-txns", type=int, default=1 << 18)
     ap.add_argument("--serve-loads", default="5e6,50e6,100e6,200e6,500e6")
     ap.add_argument("--serve-cache-reqs", type=int, default=1 << 18)
     ap.add_argument("--serve-bank-txns", type=int, default=1 << 16)
+    ap.add_argument("--serve-cache-waves", type=int, default=16)
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--no-memcd", action="store_true")
     ap.add_argument("--no-sched", action="store_true")

@@ -564,18 +564,17 @@ ServerLayout server_layout(const csmv_engine *e, bool delegation, uint32_t nmb =
     L.tg_off = L.wave_off + L.nmb;
     L.mb_off = (L.tg_off + L.nmb + 63) & ~63u;
     uint32_t w = L.mb_off + L.nmb * L.mb_rows * kWave;
-    // every item a request inserts (W writes, plus R reads when delegated) fits at a
-    // load factor of at most 3/4
-    L.hsz = (delegation && !kWideWire<R>) ? pow2_at_least((uint64_t)kWave * (R + W))
-                                          : pow2_at_least(2ull * kWave * W);
+    // the writes of a request fit at a load factor of at most 1/2
+    L.hsz = pow2_at_least(2ull * kWave * W);
     L.hash_off = w;
-    w += M * 5 * L.hsz;
+    w += M * 3 * L.hsz;
+    // serialized lane: per-receiver chain scratch (next, weight per chain node; superseded
+    // flags per write slot) and the committer's exchange of base and new values
+    constexpr uint32_t RO = (uint32_t)kChainReads<R, W>;
     L.sup_off = w;
-    w += kWave * W;
+    if (delegation) w += M * (2 * kWave * RO + kWave * W);
     L.jmp_off = w;
-    // chain tables: next, weight, base per chain node; deltas (and, wide, new values)
-    // per write slot
-    if (delegation) w += 3 * kWave * kChainReads<R, W> + 2 * kWave * W;
+    if (delegation) w += kWave * RO + kWave * W;
     L.atr_off = w;
     if (vmode != kValStamp) w += W * e->cfg.atr_entries;
     L.bucket_off = w;

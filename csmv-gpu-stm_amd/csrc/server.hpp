@@ -61,12 +61,11 @@ struct ServerLayout {
     uint32_t nbuckets;     // power of two (STAMP mode)
     uint32_t exact;        // bucket i is item i (n_items <= nbuckets)
     uint32_t val_off;      // exact mode: value of every item [n_items]
-    uint32_t sup_off;      // serialized lane: superseded-write flags [64 * W]
-    uint32_t jmp_off;      // serialized lane: next [64 * R], weight [64 * R], base [64 * R],
-                           // write deltas [64 * W]
+    uint32_t sup_off;      // serialized lane: per-receiver chain scratch [M][2 * 64 * RO + 64 * W]
+    uint32_t jmp_off;      // serialized lane: committer's base values [64 * RO], new values [64 * W]
     uint32_t atr_off;      // record ring [W][E] (SCAN / COLLAB)
-    uint32_t hash_off;     // per-receiver hash tables [M][5][hsz]
-    uint32_t hsz;          // power of two >= 64 * (R + W)
+    uint32_t hash_off;     // per-receiver hash tables [M][3][hsz]
+    uint32_t hsz;          // power of two >= 2 * 64 * W (the writes of one request)
     uint32_t words;        // total
 };
 
@@ -93,7 +92,12 @@ template <int R, int W> struct Rows {
     static constexpr uint32_t prod = 4 + R + 2 * W;   // (R + 3) / 4 rows of producer codes
     static constexpr uint32_t pval = prod + (R + 3) / 4;  // RO rows: operand values at staging
     static constexpr uint32_t pstamp = pval + RO;    // stamps of the writes at staging
-    static constexpr uint32_t count = pstamp + W;
+    // producer chains resolved at staging: each chain node's root node and the sum of the
+    // deltas between them, and the bit mask of this lane's superseded write slots
+    static constexpr uint32_t root = pstamp + W;
+    static constexpr uint32_t wgt = root + RO;
+    static constexpr uint32_t supf = wgt + RO;
+    static constexpr uint32_t count = supf + 1;
 };
 
 __device__ __forceinline__ uint32_t lds_ld_acq(const uint32_t *p)
@@ -249,14 +253,10 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
     ValueTables V;
     V.bucket = lds + L.bucket_off;
     V.lval = lds + L.val_off;
-    uint32_t *sup = lds + L.sup_off;
-    // serialized-lane chain tables (RO chain nodes per lane, W write slots per lane)
+    // serialized lane: base values per chain node (RO per lane), new values per write slot
     constexpr int RO = kChainReads<R, W>;
-    uint32_t *jnext = lds + L.jmp_off;
-    int32_t *jw = (int32_t *)(jnext + kWave * RO);
-    int32_t *jbase = jw + kWave * RO;
-    int32_t *jdelta = jbase + kWave * RO;
-    int32_t *jnv = jdelta + kWave * W;
+    int32_t *jbase = (int32_t *)(lds + L.jmp_off);
+    int32_t *jnv = jbase + kWave * RO;
     const uint32_t Q = c.qmask + 1;
     const uint64_t lt = (1ull << lane) - 1;
 
@@ -339,27 +339,27 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
         if (delegate) {
             // ---- serialized lane: the server executes the transactions ----------------
             const uint32_t k = (uint32_t)wv[0], amt = (uint32_t)wv[1];
-            // A transfer changes account a by -amt and b by +amt (0 when a == b), so the
-            // value a read sees inside the request is the committed value before the
-            // request plus the deltas of the lower lanes on its producer chain (the
-            // highest lower writer of the item, then that writer's producer, ...).
-            // Chains are summed by pointer jumping: ceil(log2(chain)) rounds instead of one
-            // step per writer of a hot account.  The lanes exchange through LDS inside
-            // this one wave, so wave-scope fences order the accesses; the HBM gathers of
-            // committed values are issued first and consumed after the chain sums.
-            // Chain nodes are the first RO reads (every read of a narrow wire, the
-            // transfer operands of a wide one); a producer code names a write slot w of
-            // a lower lane, i.e. that lane's chain node w.
-            uint32_t ops[RO];
+            // Each chain node's root and the sum of the deltas from it were resolved at
+            // staging (stage_chains); the root's value before the request is the one thing
+            // that depends on the commit order: the committed value (exact LDS table, the
+            // value gathered at staging when no commit since then wrote the item's bucket,
+            // or the HBM table this wave writes).  Chain nodes are the first RO reads
+            // (every read of a narrow wire, the transfer operands of a wide one).
+            uint32_t ops[RO], root[RO];
+            int32_t wgt[RO];
 #pragma unroll
-            for (int r = 0; r < RO; ++r) ops[r] = RS(r);
+            for (int r = 0; r < RO; ++r) {
+                ops[r] = RS(r);
+                root[r] = box[(RW::root + r) * kWave + lane];
+                wgt[r] = (int32_t)box[(RW::wgt + r) * kWave + lane];
+            }
+            const uint32_t supf = box[RW::supf * kWave + lane];
             bool need_hbm_r[RO], need_hbm_w[W];
             int32_t base[RO];
             uint32_t prev[W];
 #pragma unroll
             for (int r = 0; r < RO; ++r) {
-                const uint32_t pj = PROD(r);
-                const bool root_item = valid && ops[r] != kNone && pj == 0xFFu;
+                const bool root_item = valid && ops[r] != kNone && root[r] == lane * RO + r;
                 base[r] = 0;
                 need_hbm_r[r] = false;
                 if (root_item) {
@@ -389,57 +389,12 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
                     prev[w] = __hip_atomic_load(&c.stamps[wa[w]], __ATOMIC_RELAXED,
                                                 __HIP_MEMORY_SCOPE_WORKGROUP);
             tres = wall_clock64();
-            jdelta[lane * W + 0] = (wa[1] == kNone) ? 0 : -(int32_t)amt;
-            jdelta[lane * W + 1] = (int32_t)amt;
-            uint32_t nxt[RO];
-            int32_t wgt[RO];
-#pragma unroll
-            for (int r = 0; r < RO; ++r) {
-                const uint32_t node = lane * RO + r;
-                const uint32_t pj = PROD(r);
-                nxt[r] = (pj == 0xFFu) ? node : ((pj & 63u) * RO + (pj >> 6));
-                jnext[node] = nxt[r];
-            }
-            wave_fence();
-#pragma unroll
-            for (int r = 0; r < RO; ++r) {
-                const uint32_t pj = PROD(r);
-                wgt[r] = (pj == 0xFFu) ? 0 : jdelta[(pj & 63u) * W + (pj >> 6)];
-                jw[lane * RO + r] = wgt[r];
-            }
-            wave_fence();
-            for (uint32_t round = 0; round < 6; ++round) {
-                bool more = false;
-                uint32_t nn[RO];
-                int32_t ww[RO];
-#pragma unroll
-                for (int r = 0; r < RO; ++r) {
-                    const uint32_t node = lane * RO + r;
-                    const uint32_t t = nxt[r];
-                    const uint32_t tt = jnext[t];
-                    ww[r] = wgt[r] + (t != node ? jw[t] : 0);
-                    nn[r] = tt;
-                    more |= tt != t;
-                }
-                wave_fence();
-#pragma unroll
-                for (int r = 0; r < RO; ++r) {
-                    nxt[r] = nn[r];
-                    wgt[r] = ww[r];
-                    jnext[lane * RO + r] = nn[r];
-                    jw[lane * RO + r] = ww[r];
-                }
-                wave_fence();
-                if (ballot(more) == 0) break;
-            }
-            // every node now points at its chain's root; the root's base is the value
-            // the chain starts from
 #pragma unroll
             for (int r = 0; r < RO; ++r) jbase[lane * RO + r] = base[r];
             wave_fence();
             int32_t in[RO], rd[RO], nv[W];
 #pragma unroll
-            for (int r = 0; r < RO; ++r) in[r] = jbase[nxt[r]] + wgt[r];
+            for (int r = 0; r < RO; ++r) in[r] = jbase[root[r]] + wgt[r];
             const bool okx = valid;
             tchain = wall_clock64();
             transfer_exec<RO, W>(ops, amt, in, rd, wa, nv);
@@ -447,30 +402,17 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
             nc = (uint32_t)__builtin_popcountll(committed);
             if (okx) myts = last + 1 + (uint32_t)__builtin_popcountll(committed & lt);
             last += nc;
-            // A write is superseded when a higher committed lane of the request read the
-            // item from it (bank transfers read every item they write): only the last
-            // write of an item in the request is installed.  No snapshot can fall inside
-            // the request's timestamp range, since the clock moves over whole requests.
+            // only the last write of an item in the request is installed (staged flags);
+            // no snapshot can fall inside the request's timestamp range, since the clock
+            // moves over whole requests
+            bool inst[W];
 #pragma unroll
-            for (int w = 0; w < W; ++w) sup[lane * W + w] = 0;
+            for (int w = 0; w < W; ++w)
+                inst[w] = okx && wa[w] != kNone && ((supf >> w) & 1u) == 0;
             if constexpr (kWide) {
 #pragma unroll
                 for (int w = 0; w < W; ++w) jnv[lane * W + w] = nv[w];
-            }
-            wave_fence();
-#pragma unroll
-            for (int r = 0; r < RO; ++r) {
-                const uint32_t pj = PROD(r);
-                bool mine_w = false;
-#pragma unroll
-                for (int w = 0; w < W; ++w) mine_w |= wa[w] == ops[r];
-                if (okx && pj != 0xFFu && mine_w) sup[(pj & 63u) * W + (pj >> 6)] = 1u;
-            }
-            wave_fence();
-            bool inst[W];
-#pragma unroll
-            for (int w = 0; w < W; ++w) inst[w] = okx && wa[w] != kNone && sup[lane * W + w] == 0;
-            if constexpr (kWide) {
+                wave_fence();
                 // the other reads see the write of their producer (a lower lane), this
                 // lane's own write, or the committed value before the request; their
                 // committed values are read before this request's values are recorded
@@ -754,6 +696,96 @@ __device__ __forceinline__ uint32_t hash_find(const uint32_t *hkey, uint32_t hma
     return kNone;
 }
 
+// Serialized lane, at staging: resolve the producer chains of a delegated request from its
+// staged rows.  A chain node is one of the first RO reads of a lane; its producer (staged
+// code: lane | write slot << 6) is the highest lower lane writing the item, i.e. that lane's
+// node for the slot.  A transfer changes its first item by -amt and its second by +amt (0
+// when both are one item), so the value a node sees is its chain root's value before the
+// request plus the deltas of the writers on the way.  Pointer jumping over this wave's LDS
+// scratch gives every node its root and that sum in ceil(log2(chain)) rounds; a write slot
+// is superseded when a higher lane reads the item from it and writes it too.  Only the
+// root's value depends on the commit order, so the committer is left with one gather.
+template <int R, int W>
+__device__ __forceinline__ void stage_chains(uint32_t *dst, uint32_t *scr, uint32_t lane)
+{
+    using RW = Rows<R, W>;
+    constexpr int RO = RW::RO;
+    uint32_t *snext = scr;
+    int32_t *sw = (int32_t *)(scr + kWave * RO);
+    uint32_t *ssup = scr + 2 * kWave * RO;
+    wave_fence();
+    const bool valid = dst[RW::snap * kWave + lane] != kNone;
+    const uint32_t amt = dst[(RW::wv + 1) * kWave + lane];
+    uint32_t ops[RO], wa[W], pj[RO];
+#pragma unroll
+    for (int r = 0; r < RO; ++r) {
+        ops[r] = dst[(RW::items + r) * kWave + lane];
+        pj[r] = (dst[(RW::prod + r / 4) * kWave + lane] >> (8 * (r % 4))) & 0xFFu;
+    }
+#pragma unroll
+    for (int w = 0; w < W; ++w) wa[w] = dst[(RW::items + R + w) * kWave + lane];
+    // the producer's delta for the slot: shuffled from the producer lane on every lane
+    const int32_t d0 = (wa[1] == kNone) ? 0 : -(int32_t)amt, d1 = (int32_t)amt;
+    uint32_t nxt[RO];
+    int32_t wgt[RO];
+#pragma unroll
+    for (int r = 0; r < RO; ++r) {
+        const uint32_t node = lane * RO + r;
+        const bool has_p = pj[r] != 0xFFu;
+        const int srcl = (int)(has_p ? (pj[r] & 63u) : lane);
+        const int32_t q0 = __shfl(d0, srcl, 64), q1 = __shfl(d1, srcl, 64);
+        nxt[r] = has_p ? (pj[r] & 63u) * RO + (pj[r] >> 6) : node;
+        wgt[r] = has_p ? ((pj[r] >> 6) == 0 ? q0 : q1) : 0;
+        snext[node] = nxt[r];
+        sw[node] = wgt[r];
+    }
+#pragma unroll
+    for (int w = 0; w < W; ++w) ssup[lane * W + w] = 0;
+    wave_fence();
+    for (uint32_t round = 0; round < 6; ++round) {
+        bool more = false;
+        uint32_t nn[RO];
+        int32_t ww[RO];
+#pragma unroll
+        for (int r = 0; r < RO; ++r) {
+            const uint32_t node = lane * RO + r;
+            const uint32_t t = nxt[r];
+            const uint32_t tt = snext[t];
+            ww[r] = wgt[r] + (t != node ? sw[t] : 0);
+            nn[r] = tt;
+            more |= tt != t;
+        }
+        wave_fence();
+#pragma unroll
+        for (int r = 0; r < RO; ++r) {
+            nxt[r] = nn[r];
+            wgt[r] = ww[r];
+            snext[lane * RO + r] = nn[r];
+            sw[lane * RO + r] = ww[r];
+        }
+        wave_fence();
+        if (ballot(more) == 0) break;
+    }
+#pragma unroll
+    for (int r = 0; r < RO; ++r) {
+        bool mine_w = false;
+#pragma unroll
+        for (int w = 0; w < W; ++w) mine_w |= wa[w] == ops[r];
+        if (valid && pj[r] != 0xFFu && mine_w) ssup[(pj[r] & 63u) * W + (pj[r] >> 6)] = 1u;
+    }
+    wave_fence();
+    uint32_t supf = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) supf |= ssup[lane * W + w] ? (1u << w) : 0u;
+#pragma unroll
+    for (int r = 0; r < RO; ++r) {
+        dst[(RW::root + r) * kWave + lane] = nxt[r];
+        dst[(RW::wgt + r) * kWave + lane] = (uint32_t)wgt[r];
+    }
+    dst[RW::supf * kWave + lane] = supf;
+    wave_fence();
+}
+
 template <int R, int W, bool D>
 __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, uint32_t *lds,
                                          uint32_t rid, uint32_t M)
@@ -766,19 +798,16 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
     uint32_t *mbst = lds + L.state_off;
     uint32_t *mbwave = lds + L.wave_off;
     uint32_t *mbtg = lds + L.tg_off;
-    // per-receiver hash: key, write-lane mask (lo, hi), read-lane mask (lo, hi)
-    uint32_t *hkey = lds + L.hash_off + rid * 5 * L.hsz;
+    // per-receiver hash: key, write-lane mask (lo, hi)
+    uint32_t *hkey = lds + L.hash_off + rid * 3 * L.hsz;
     uint32_t *hwlo = hkey + L.hsz, *hwhi = hkey + 2 * L.hsz;
-    uint32_t *hrlo = hkey + 3 * L.hsz, *hrhi = hkey + 4 * L.hsz;
     const uint32_t hmask = L.hsz - 1;
     const uint32_t mybit = 1u << (lane & 31);
+    uint32_t *chain_scr = lds + L.sup_off +
+                          rid * (2 * kWave * Rows<R, W>::RO + kWave * W);
     auto wmask = [&](uint32_t h) -> uint64_t {
         return ((uint64_t)__hip_atomic_load(&hwhi[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) << 32) |
                __hip_atomic_load(&hwlo[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    };
-    auto rmask = [&](uint32_t h) -> uint64_t {
-        return ((uint64_t)__hip_atomic_load(&hrhi[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) << 32) |
-               __hip_atomic_load(&hrlo[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     };
     constexpr uint32_t in_rows = mbox_rows<R, W>();
     uint64_t t_window = 0, t_ring = 0, t_busy = 0;
@@ -929,7 +958,7 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
             for (int w = 0; w < W; ++w)
                 if (slot_w[w] != kNone) {
                     hkey[slot_w[w]] = kNone;
-                    hwlo[slot_w[w]] = 0; hwhi[slot_w[w]] = 0; hrlo[slot_w[w]] = 0; hrhi[slot_w[w]] = 0;
+                    hwlo[slot_w[w]] = 0; hwhi[slot_w[w]] = 0;
                 }
             dst[RW::snap * kWave + lane] = snap;
 #pragma unroll
@@ -978,11 +1007,11 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
                 }
             }
 
-            // Conflicts with lower lanes of the request.  Insert the write set (and for a
-            // delegated request the read set) into an LDS hash of item -> lane masks, then
-            // look up: conf = lower lanes that write an item this lane reads or writes
-            // (and, delegated, lower lanes that read an item this lane writes).
-            uint32_t slot_w[W], slot_r[R];
+            // Conflicts with lower lanes of the request.  Insert the write set into an LDS
+            // hash of item -> lane masks, then look up: conf = lower lanes that write an
+            // item this lane reads or writes; for a delegated request, the highest such
+            // lane per read is its producer.
+            uint32_t slot_w[W];
     #pragma unroll
             for (int w = 0; w < W; ++w) {
                 slot_w[w] = kNone;
@@ -992,16 +1021,7 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
                     if (slot_w[w] != kNone) atomicOr(lane < 32 ? &hwlo[slot_w[w]] : &hwhi[slot_w[w]], mybit);
                 }
             }
-    #pragma unroll
-            for (int r = 0; r < R; ++r) {
-                slot_r[r] = kNone;
-                const uint32_t x = it[r];
-                if (delegate && valid && x != kNone) {
-                    slot_r[r] = hash_claim(hkey, hmask, x);
-                    if (slot_r[r] != kNone) atomicOr(lane < 32 ? &hrlo[slot_r[r]] : &hrhi[slot_r[r]], mybit);
-                }
-            }
-            wg_fence();
+                wg_fence();
             uint64_t conf = 0;
             uint32_t prod[(R + 3) / 4];
     #pragma unroll
@@ -1018,7 +1038,6 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
                     if (h != kNone) {
                         const uint64_t wm = wmask(h) & lt;
                         conf |= wm;
-                        if (delegate && r >= R) conf |= rmask(h) & lt;
                         if (delegate && r < R && wm) plane[r < R ? r : 0] = 63u - (uint32_t)__builtin_clzll(wm);
                     }
                 }
@@ -1045,15 +1064,9 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
             for (int w = 0; w < W; ++w)
                 if (slot_w[w] != kNone) {
                     hkey[slot_w[w]] = kNone;
-                    hwlo[slot_w[w]] = 0; hwhi[slot_w[w]] = 0; hrlo[slot_w[w]] = 0; hrhi[slot_w[w]] = 0;
+                    hwlo[slot_w[w]] = 0; hwhi[slot_w[w]] = 0;
                 }
-    #pragma unroll
-            for (int r = 0; r < R; ++r)
-                if (slot_r[r] != kNone) {
-                    hkey[slot_r[r]] = kNone;
-                    hwlo[slot_r[r]] = 0; hwhi[slot_r[r]] = 0; hrlo[slot_r[r]] = 0; hrhi[slot_r[r]] = 0;
-                }
-
+    
             dst[RW::snap * kWave + lane] = snap;
     #pragma unroll
             for (int r = 0; r < R + W; ++r) dst[(RW::items + r) * kWave + lane] = it[r];
@@ -1071,6 +1084,7 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
                 for (int w = 0; w < W; ++w) dst[(RW::pstamp + w) * kWave + lane] = pst[w];
             }
         }
+        if (delegate) stage_chains<R, W>(dst, chain_scr, lane);
         if (lane == 0) {
             mbwave[mbi] = cwr;
             mbtg[mbi] = tg;
@@ -1100,14 +1114,13 @@ __device__ __forceinline__ void server_workgroup(const Ctx &c, const ServerLayou
         if (L.exact)
             for (uint32_t i = threadIdx.x; i < c.n_items; i += blockDim.x)
                 lds[L.val_off + i] = (uint32_t)c.vals[i];
-        for (uint32_t i = threadIdx.x; i < kWave * W; i += blockDim.x) lds[L.sup_off + i] = 0;
     }
     const uint32_t M = c.wg_waves - 2;
     for (uint32_t r = 0; r < M; ++r)
         for (uint32_t i = threadIdx.x; i < L.hsz; i += blockDim.x) {
-            uint32_t *h = lds + L.hash_off + r * 5 * L.hsz;
+            uint32_t *h = lds + L.hash_off + r * 3 * L.hsz;
             h[i] = kNone;
-            for (uint32_t q = 1; q < 5; ++q) h[q * L.hsz + i] = 0;
+            for (uint32_t q = 1; q < 3; ++q) h[q * L.hsz + i] = 0;
         }
     __syncthreads();
     if (wave == 0) committer<R, W, D>(c, L, lds);
