@@ -433,6 +433,8 @@ __device__ __forceinline__ uint32_t tx_commit(const Ctx &c, uint32_t cw, Tx<R, W
         // new values, and for each write the stamp its install must follow (kNone: the
         // write was superseded inside the request and is not installed)
         const uint64_t *rv = c.reply_v + ((size_t)cw * kWave + lane) * 2 * W;
+        uint32_t prev[W];
+        bool pend[W];
 #pragma unroll
         for (int w = 0; w < W; ++w) {
             uint64_t x, y;
@@ -445,19 +447,51 @@ __device__ __forceinline__ uint32_t tx_commit(const Ctx &c, uint32_t cw, Tx<R, W
                 if (!sp.tick(c, kErrReplyTimeout)) { *fail = true; return 0; }
             }
             tx.wv[w] = (int32_t)(uint32_t)x;
-            const uint32_t prev = (uint32_t)y;
-            if (ts == 0 || (uint32_t)w >= tx.nw || prev == kNone) tx.wa[w] = kNone;
-            // installs of one item happen in commit order: wait for the predecessor
-            for (;;) {
-                bool wait = false;
-                if (tx.wa[w] != kNone) {
-                    const uint32_t ht = ld_rlx(&c.ht[tx.wa[w]]);
-                    const uint64_t head = ld_rlx(&c.slot[(size_t)(ht & 0xFFFFu) * c.n_items + tx.wa[w]]);
-                    wait = (uint32_t)(head >> 32) != prev;
+            prev[w] = (uint32_t)y;
+            pend[w] = ts != 0 && (uint32_t)w < tx.nw && prev[w] != kNone && tx.wa[w] != kNone;
+            if (!pend[w]) tx.wa[w] = kNone;
+        }
+        // installs of one item happen in commit order: each write is installed as soon as
+        // the item's head carries its predecessor's stamp (the write of the same item by
+        // the previous request that wrote it), independently of the wave's other writes
+        for (;;) {
+            bool more = false, any = false;
+            uint32_t nh[W];
+            bool go[W];
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                go[w] = false;
+                nh[w] = 0;
+                if (pend[w]) {
+                    const uint32_t item = tx.wa[w];
+                    const uint32_t ht = ld_rlx(&c.ht[item]);
+                    const uint64_t head = ld_rlx(&c.slot[(size_t)(ht & 0xFFFFu) * c.n_items + item]);
+                    if ((uint32_t)(head >> 32) == prev[w]) {
+                        uint32_t h = ht & 0xFFFFu, t = ht >> 16;
+                        h = (h + 1 == c.depth) ? 0 : h + 1;
+                        if (h == t) t = (t + 1 == c.depth) ? 0 : t + 1;
+                        st_rlx(&c.slot[(size_t)h * c.n_items + item],
+                               ((uint64_t)ts << 32) | (uint32_t)tx.wv[w]);
+                        nh[w] = h | (t << 16);
+                        go[w] = true;
+                        any = true;
+                    } else {
+                        more = true;
+                    }
                 }
-                if (ballot(wait) == 0) break;
-                if (!sp.tick(c, kErrReplyTimeout)) { *fail = true; return 0; }
             }
+            if (ballot(any)) {
+                fence_rel();
+#pragma unroll
+                for (int w = 0; w < W; ++w)
+                    if (go[w]) {
+                        st_rlx(&c.ht[tx.wa[w]], nh[w]);
+                        pend[w] = false;
+                        tx.wa[w] = kNone;
+                    }
+            }
+            if (ballot(more) == 0) break;
+            if (!sp.tick(c, kErrReplyTimeout)) { *fail = true; return 0; }
         }
     }
     *wait_ticks += wall_clock64() - sp.t0;

@@ -20,8 +20,10 @@ def test_device_bank_stream_prefix(name):
     f = FX[name]
     sp = f["spec"]
     print(f"\n[case] golden {name}", flush=True)
+    # the fixture spec lists what differs from the stream defaults (32-read audits, 10 %)
     b = csmv.Bank(n_txns=sp["n_txns"], seed=sp["seed"], n_reads=sp["n_reads"],
-                  n_reads_ro=sp["n_reads_ro"], pct_ro=sp["pct_ro"], zipf=sp.get("zipf", False),
+                  n_reads_ro=sp.get("n_reads_ro", 32), pct_ro=sp.get("pct_ro", 10),
+                  zipf=sp.get("zipf", False),
                   theta=sp.get("theta", 0.99), n_reads_min=sp.get("n_reads_min", 0))
     with csmv.Engine(sp["n_accounts"], depth=4, client_waves=8) as e:
         e.bank_generate(b)
