@@ -110,25 +110,47 @@ template <int R, int W> __device__ __forceinline__ void memcd_client(const Ctx &
             const uint32_t set = mc_set_of(key, c.n_sets);
             const uint32_t clock = (uint32_t)(k / c.mc_batch + 1);
             if (op == kMcGet) {
-                int hit = -1;
-                for (uint32_t w = 0; w < ways && !tx.aborted; ++w) {
-                    const uint32_t slot = set * ways + w;
-                    const int32_t st = tx_read(c, tx, kMcState * ns + slot);
-                    if (st == 1 && !tx.aborted) {
-                        const uint32_t klo = (uint32_t)tx_read_only(c, tx, kMcKlo * ns + slot);
-                        const uint32_t khi = (uint32_t)tx_read_only(c, tx, kMcKhi * ns + slot);
-                        if (klo == lo && khi == hi) { hit = (int)w; break; }
-                    }
+                // states and keys of every way at the snapshot in two batches of loads,
+                // then the hit slot's value and timestamp; the states are logged up to the
+                // hit way (the ones a sequential probe reads)
+                uint32_t ia[16], ib[8];
+                int32_t va[16], vb[8];
+#pragma unroll
+                for (int w = 0; w < 8; ++w) {
+                    const bool on = (uint32_t)w < ways;
+                    const uint32_t slot = set * ways + (uint32_t)w;
+                    ia[w] = on ? kMcState * ns + slot : kNone;
+                    ia[8 + w] = on ? kMcKlo * ns + slot : kNone;
+                    ib[w] = on ? kMcKhi * ns + slot : kNone;
                 }
+                bool ok = pool_read_chunk<16>(c, ia, tx.snap, va);
+                ok = pool_read_chunk<8>(c, ib, tx.snap, vb) && ok;
+                if (!ok) tx.aborted = true;
+                int hit = -1;
+#pragma unroll
+                for (int w = 7; w >= 0; --w)
+                    if ((uint32_t)w < ways && va[w] == 1 && (uint32_t)va[8 + w] == lo &&
+                        (uint32_t)vb[w] == hi)
+                        hit = w;
+#pragma unroll
+                for (int w = 0; w < 8; ++w)
+                    if ((uint32_t)w < ways && (hit < 0 || w <= hit)) log_read(tx, ia[w]);
                 if (hit >= 0 && !tx.aborted) {
                     const uint32_t slot = set * ways + (uint32_t)hit;
+                    uint32_t iv[9];
+                    int32_t vv[9];
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) iv[j] = (kMcV0 + j) * ns + slot;
+                    iv[8] = kMcTs * ns + slot;
+                    if (!pool_read_chunk<9>(c, iv, tx.snap, vv)) tx.aborted = true;
                     uint32_t acc = 0;
-                    for (uint32_t j = 0; j < 8; ++j)
-                        acc += (uint32_t)tx_read_only(c, tx, (kMcV0 + j) * ns + slot) * (2u * j + 1u);
-                    const uint32_t t = (uint32_t)tx_read(c, tx, kMcTs * ns + slot);
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) acc += (uint32_t)vv[j] * (2u * j + 1u);
+                    const uint32_t t = (uint32_t)vv[8];
+                    log_read(tx, iv[8]);
                     res = 1u | ((uint32_t)hit << 4);
                     vsum = acc;
-                    if (t < clock) {
+                    if (t < clock && !tx.aborted) {
                         res |= 2u;
                         tx_write(tx, kMcTs * ns + slot, (int32_t)clock);
                         iwa[0] = kMcTs * ns + slot;
@@ -137,17 +159,35 @@ template <int R, int W> __device__ __forceinline__ void memcd_client(const Ctx &
                     }
                 }
             } else {
+                // states, keys and timestamps of every way in two batches of loads (states
+                // and keys low; keys high and timestamps); states and timestamps are logged
                 int found = -1, invalid = -1, lru = 0;
                 uint32_t lru_ts = kNone;
-                for (uint32_t w = 0; w < ways && !tx.aborted; ++w) {
-                    const uint32_t slot = set * ways + w;
-                    const int32_t st = tx_read(c, tx, kMcState * ns + slot);
-                    const uint32_t klo = (uint32_t)tx_read_only(c, tx, kMcKlo * ns + slot);
-                    const uint32_t khi = (uint32_t)tx_read_only(c, tx, kMcKhi * ns + slot);
-                    const uint32_t ts = (uint32_t)tx_read(c, tx, kMcTs * ns + slot);
-                    if (st == 1 && klo == lo && khi == hi && found < 0) found = (int)w;
-                    if (st != 1 && invalid < 0) invalid = (int)w;
-                    if (ts < lru_ts) { lru_ts = ts; lru = (int)w; }
+                uint32_t ia[16], ib[16];
+                int32_t va[16], vb[16];
+#pragma unroll
+                for (int w = 0; w < 8; ++w) {
+                    const bool on = (uint32_t)w < ways;
+                    const uint32_t slot = set * ways + (uint32_t)w;
+                    ia[w] = on ? kMcState * ns + slot : kNone;
+                    ia[8 + w] = on ? kMcKlo * ns + slot : kNone;
+                    ib[w] = on ? kMcKhi * ns + slot : kNone;
+                    ib[8 + w] = on ? kMcTs * ns + slot : kNone;
+                }
+                bool ok = pool_read_chunk<16>(c, ia, tx.snap, va);
+                ok = pool_read_chunk<16>(c, ib, tx.snap, vb) && ok;
+                if (!ok) tx.aborted = true;
+#pragma unroll
+                for (int w = 0; w < 8; ++w) {
+                    if ((uint32_t)w >= ways) continue;
+                    const int32_t st = va[w];
+                    const uint32_t ts = (uint32_t)vb[8 + w];
+                    log_read(tx, ia[w]);
+                    log_read(tx, ib[8 + w]);
+                    if (st == 1 && (uint32_t)va[8 + w] == lo && (uint32_t)vb[w] == hi && found < 0)
+                        found = w;
+                    if (st != 1 && invalid < 0) invalid = w;
+                    if (ts < lru_ts) { lru_ts = ts; lru = w; }
                 }
                 const uint32_t w = found >= 0 ? (uint32_t)found
                                  : invalid >= 0 ? (uint32_t)invalid : (uint32_t)lru;
