@@ -171,19 +171,40 @@ template <int R, int W> __device__ __forceinline__ void bank_client(const Ctx &c
         for (int j = 0; j < R; ++j) reads[j] = 0;
         uint32_t sum = 0;
         if (ro) {
-            for (uint32_t j = 0; j < nr && !tx.aborted; ++j)
-                sum += (uint32_t)tx_read_only(c, tx, c.addr[(uint64_t)j * T + k]);
+            // audit: snapshot reads, 16 at a time (two rounds of independent loads each)
+            bool ok = true;
+#pragma unroll 1
+            for (uint32_t c0 = 0; c0 < nr; c0 += 16) {
+                uint32_t it[16];
+                int32_t v[16];
+#pragma unroll
+                for (int j = 0; j < 16; ++j)
+                    it[j] = c0 + j < nr ? c.addr[(uint64_t)(c0 + j) * T + k] : kNone;
+                ok = pool_read_chunk<16>(c, it, tx.snap, v) && ok;
+#pragma unroll
+                for (int j = 0; j < 16; ++j) sum += it[j] != kNone ? (uint32_t)v[j] : 0u;
+            }
+            if (!ok) tx.aborted = true;
         } else if (upd) {
-            // transfer body (exe.cu:184-201): read a, write a-amt, read b, write b+amt,
-            // then the extra reads of the wider transaction
+            // transfer body (exe.cu:184-201) over one batch of snapshot reads: read a,
+            // write a-amt, read b (own write if b == a), write b+amt, then the extra reads
+            // of the wider transaction (own writes first); every read is logged
+            uint32_t it[R];
+            int32_t v[R];
+#pragma unroll
+            for (int j = 0; j < R; ++j) it[j] = (uint32_t)j < nr ? c.addr[(uint64_t)j * T + k] : kNone;
+            if (!pool_read_chunk<R>(c, it, tx.snap, v)) tx.aborted = true;
 #pragma unroll
             for (int j = 0; j < R; ++j) {
-                if ((uint32_t)j < nr && !tx.aborted) {
-                    const uint32_t it = c.addr[(uint64_t)j * T + k];
-                    const int32_t v = tx_read(c, tx, it);
-                    reads[j] = v;
-                    if (j == 0) tx_write(tx, it, (int32_t)((uint32_t)v - amt));
-                    else if (j == 1) tx_write(tx, it, (int32_t)((uint32_t)v + amt));
+                if ((uint32_t)j < nr) {
+                    int32_t x = v[j];
+#pragma unroll
+                    for (int w = 0; w < W; ++w)
+                        x = ((uint32_t)w < tx.nw && tx.wa[w] == it[j]) ? tx.wv[w] : x;
+                    reads[j] = x;
+                    log_read(tx, it[j]);
+                    if (j == 0) tx_write(tx, it[j], (int32_t)((uint32_t)x - amt));
+                    else if (j == 1) tx_write(tx, it[j], (int32_t)((uint32_t)x + amt));
                 }
             }
         }
