@@ -97,7 +97,9 @@ template <int R, int W> struct Rows {
     static constexpr uint32_t root = pstamp + W;
     static constexpr uint32_t wgt = root + RO;
     static constexpr uint32_t supf = wgt + RO;
-    static constexpr uint32_t count = supf + 1;
+    // stamp-bucket index (exact mode: the item) of each chain node, then of each write
+    static constexpr uint32_t bkt = supf + 1;
+    static constexpr uint32_t count = bkt + RO + W;
 };
 
 __device__ __forceinline__ uint32_t lds_ld_acq(const uint32_t *p)
@@ -354,40 +356,54 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
                 wgt[r] = (int32_t)box[(RW::wgt + r) * kWave + lane];
             }
             const uint32_t supf = box[RW::supf * kWave + lane];
-            bool need_hbm_r[RO], need_hbm_w[W];
+            uint32_t bkr[RO], bkw[W];
+#pragma unroll
+            for (int r = 0; r < RO; ++r) bkr[r] = box[(RW::bkt + r) * kWave + lane];
+#pragma unroll
+            for (int w = 0; w < W; ++w) bkw[w] = box[(RW::bkt + RO + w) * kWave + lane];
+            bool is_root[RO];
+#pragma unroll
+            for (int r = 0; r < RO; ++r) is_root[r] = valid && ops[r] != kNone && root[r] == lane * RO + r;
             int32_t base[RO];
             uint32_t prev[W];
+            // One round of loads, issued for every node and write whether needed or not
+            // (a load of item 0 stands in for an empty one), then selects: the branch-free
+            // form waits once instead of once per item.
+            if (L.exact) {
+                int32_t lv[RO];
+                uint32_t ls[W];
 #pragma unroll
-            for (int r = 0; r < RO; ++r) {
-                const bool root_item = valid && ops[r] != kNone && root[r] == lane * RO + r;
-                base[r] = 0;
-                need_hbm_r[r] = false;
-                if (root_item) {
-                    if (L.exact) base[r] = (int32_t)V.lval[ops[r]];
-                    else if (V.bucket[bucket_of(L, ops[r])] <= tg) base[r] = pval[r];
-                    else need_hbm_r[r] = true;
+                for (int r = 0; r < RO; ++r) lv[r] = (int32_t)V.lval[ops[r] != kNone ? ops[r] : 0u];
+#pragma unroll
+                for (int w = 0; w < W; ++w) ls[w] = V.bucket[wa[w] != kNone ? wa[w] : 0u];
+#pragma unroll
+                for (int r = 0; r < RO; ++r) base[r] = is_root[r] ? lv[r] : 0;
+#pragma unroll
+                for (int w = 0; w < W; ++w) prev[w] = (valid && wa[w] != kNone) ? ls[w] : kNone;
+            } else {
+                // the committed value and stamp: the ones gathered at staging when no
+                // commit since then wrote the item's bucket, else the HBM tables (which
+                // hold this wave's own earlier stores); both are loaded at once
+                uint32_t bv[RO], bw[W], hs[W];
+                int32_t hv[RO];
+#pragma unroll
+                for (int r = 0; r < RO; ++r) {
+                    bv[r] = V.bucket[bkr[r]];
+                    hv[r] = __hip_atomic_load(&c.vals[ops[r] != kNone ? ops[r] : 0u], __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_WORKGROUP);
                 }
-            }
 #pragma unroll
-            for (int w = 0; w < W; ++w) {
-                prev[w] = kNone;
-                need_hbm_w[w] = false;
-                if (valid && wa[w] != kNone) {
-                    if (L.exact) prev[w] = V.bucket[wa[w]];
-                    else if (V.bucket[bucket_of(L, wa[w])] <= tg) prev[w] = pst[w];
-                    else need_hbm_w[w] = true;
+                for (int w = 0; w < W; ++w) {
+                    bw[w] = V.bucket[bkw[w]];
+                    hs[w] = __hip_atomic_load(&c.stamps[wa[w] != kNone ? wa[w] : 0u], __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_WORKGROUP);
                 }
+#pragma unroll
+                for (int r = 0; r < RO; ++r) base[r] = is_root[r] ? (bv[r] <= tg ? pval[r] : hv[r]) : 0;
+#pragma unroll
+                for (int w = 0; w < W; ++w)
+                    prev[w] = (valid && wa[w] != kNone) ? (bw[w] <= tg ? pst[w] : hs[w]) : kNone;
             }
-#pragma unroll
-            for (int r = 0; r < RO; ++r)
-                if (need_hbm_r[r])
-                    base[r] = __hip_atomic_load(&c.vals[ops[r]], __ATOMIC_RELAXED,
-                                                __HIP_MEMORY_SCOPE_WORKGROUP);
-#pragma unroll
-            for (int w = 0; w < W; ++w)
-                if (need_hbm_w[w])
-                    prev[w] = __hip_atomic_load(&c.stamps[wa[w]], __ATOMIC_RELAXED,
-                                                __HIP_MEMORY_SCOPE_WORKGROUP);
             tres = wall_clock64();
 #pragma unroll
             for (int r = 0; r < RO; ++r) jbase[lane * RO + r] = base[r];
@@ -445,7 +461,15 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
                 }
             }
             record_values<R, W>(c, L, V, wa, nv, inst);
-            record_stamps<R, W>(c, L, V, wa, myts, okx);
+#pragma unroll
+            for (int w = 0; w < W; ++w)
+                if (okx && wa[w] != kNone) {
+                    __hip_atomic_fetch_max(&V.bucket[bkw[w]], myts, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                    if (!L.exact)
+                        __hip_atomic_fetch_max(&c.stamps[wa[w]], myts, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
             if (okx) {
                 c.log_ts[k] = myts;
 #pragma unroll
@@ -914,7 +938,10 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
                         }
                     }
                     dst[(RW::items + c0 + j) * kWave + lane] = x[j];
-                    if (delegate && c0 + j < RO) dst[(RW::pval + c0 + j) * kWave + lane] = st[j];
+                    if (delegate && c0 + j < RO) {
+                        dst[(RW::pval + c0 + j) * kWave + lane] = st[j];
+                        dst[(RW::bkt + c0 + j) * kWave + lane] = x[j] != kNone ? bucket_of(L, x[j]) : 0u;
+                    }
                 }
                 if (delegate) {
                     // producer code: the lower writer's lane and the write slot holding
@@ -951,7 +978,10 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
                     if (h != kNone) conf |= wmask(h) & lt;
                 }
                 dst[(RW::items + R + w) * kWave + lane] = x;
-                if (delegate) dst[(RW::pstamp + w) * kWave + lane] = st;
+                if (delegate) {
+                    dst[(RW::pstamp + w) * kWave + lane] = st;
+                    dst[(RW::bkt + RO + w) * kWave + lane] = x != kNone ? bucket_of(L, x) : 0u;
+                }
             }
             wg_fence();
 #pragma unroll
@@ -1082,6 +1112,9 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
                 for (int r = 0; r < R; ++r) dst[(RW::pval + r) * kWave + lane] = (uint32_t)pval[r];
     #pragma unroll
                 for (int w = 0; w < W; ++w) dst[(RW::pstamp + w) * kWave + lane] = pst[w];
+    #pragma unroll
+                for (int r = 0; r < R + W; ++r)
+                    dst[(RW::bkt + r) * kWave + lane] = it[r] != kNone ? bucket_of(L, it[r]) : 0u;
             }
         }
         if (delegate) stage_chains<R, W>(dst, chain_scr, lane);
