@@ -573,7 +573,7 @@ uint32_t pow2_floor(uint64_t v)
 
 // Server LDS layout for a request wire of R reads and W writes.
 template <int R, int W>
-ServerLayout server_layout(const csmv_engine *e, bool delegation, uint32_t nmb = 0)
+ServerLayout server_layout_at(const csmv_engine *e, bool delegation, uint32_t nmb, uint32_t hshift)
 {
     ServerLayout L{};
     const uint32_t vmode = e->cfg.variant & 3u;
@@ -585,8 +585,8 @@ ServerLayout server_layout(const csmv_engine *e, bool delegation, uint32_t nmb =
     L.tg_off = L.wave_off + L.nmb;
     L.mb_off = (L.tg_off + L.nmb + 63) & ~63u;
     uint32_t w = L.mb_off + L.nmb * L.mb_rows * kWave;
-    // the writes of a request fit at a load factor of at most 1/2
-    L.hsz = pow2_at_least(2ull * kWave * W);
+    // the writes of a request fill at most 1/2 >> hshift of a receiver's hash
+    L.hsz = pow2_at_least(2ull * kWave * W) << hshift;
     L.hash_off = w;
     w += M * 3 * L.hsz;
     // serialized lane: per-receiver chain scratch (next, weight per chain node; superseded
@@ -619,6 +619,19 @@ ServerLayout server_layout(const csmv_engine *e, bool delegation, uint32_t nmb =
     }
     L.words = w;
     return L;
+}
+
+// A wave probes its receiver hash until its longest probe sequence ends, so the hash is
+// kept sparse (load at most 1/8, else 1/4, else 1/2) as long as the stamp buckets keep
+// 8192 entries and the layout fits.
+template <int R, int W>
+ServerLayout server_layout(const csmv_engine *e, bool delegation, uint32_t nmb = 0)
+{
+    for (uint32_t hshift = 2; hshift > 0; --hshift) {
+        const ServerLayout L = server_layout_at<R, W>(e, delegation, nmb, hshift);
+        if ((uint64_t)L.words * 4 <= e->lds_max && (L.exact || L.nbuckets >= 8192)) return L;
+    }
+    return server_layout_at<R, W>(e, delegation, nmb, 0);
 }
 
 }  // namespace

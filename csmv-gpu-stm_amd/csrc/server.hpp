@@ -1040,37 +1040,99 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
             // Conflicts with lower lanes of the request.  Insert the write set into an LDS
             // hash of item -> lane masks, then look up: conf = lower lanes that write an
             // item this lane reads or writes; for a delegated request, the highest such
-            // lane per read is its producer.
-            uint32_t slot_w[W];
+            // lane per read is its producer.  The hash is this wave's own, and one wave's
+            // LDS operations take effect in program order, so only the compiler needs
+            // holding back between the steps.  Every step issues its loads or swaps for
+            // all items at once (an idle item swaps kNone for kNone at slot 0, which
+            // changes nothing) and waits once.
+            uint32_t mx[R + W];
+    #pragma unroll
+            for (int r = 0; r < R + W; ++r) mx[r] = mix32(it[r]);
+            uint32_t slot_w[W], hp[W];
+            bool pend[W];
     #pragma unroll
             for (int w = 0; w < W; ++w) {
                 slot_w[w] = kNone;
-                const uint32_t x = it[R + w];
-                if (valid && x != kNone) {
-                    slot_w[w] = hash_claim(hkey, hmask, x);
-                    if (slot_w[w] != kNone) atomicOr(lane < 32 ? &hwlo[slot_w[w]] : &hwhi[slot_w[w]], mybit);
-                }
+                hp[w] = mx[R + w] & hmask;
+                pend[w] = valid && it[R + w] != kNone;
             }
-                wg_fence();
+            for (uint32_t step = 0; step <= hmask; ++step) {
+                uint32_t old[W];
+    #pragma unroll
+                for (int w = 0; w < W; ++w) {
+                    old[w] = kNone;
+                    __hip_atomic_compare_exchange_strong(&hkey[pend[w] ? hp[w] : 0u], &old[w],
+                                                         pend[w] ? it[R + w] : kNone, __ATOMIC_RELAXED,
+                                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+                bool more = false;
+    #pragma unroll
+                for (int w = 0; w < W; ++w) {
+                    if (pend[w]) {
+                        if (old[w] == kNone || old[w] == it[R + w]) {
+                            slot_w[w] = hp[w];
+                            pend[w] = false;
+                        } else {
+                            hp[w] = (hp[w] + 1) & hmask;
+                        }
+                    }
+                    more |= pend[w];
+                }
+                if (ballot(more) == 0) break;
+            }
+    #pragma unroll
+            for (int w = 0; w < W; ++w)
+                if (slot_w[w] != kNone) atomicOr(lane < 32 ? &hwlo[slot_w[w]] : &hwhi[slot_w[w]], mybit);
+            asm volatile("" ::: "memory");
+            // the reads' slots (a write's slot is the one it claimed)
+            uint32_t fs[R], hs[R];
+            bool fp[R];
+    #pragma unroll
+            for (int r = 0; r < R; ++r) {
+                fs[r] = kNone;
+                hs[r] = mx[r] & hmask;
+                fp[r] = valid && it[r] != kNone;
+            }
+            for (uint32_t step = 0; step <= hmask; ++step) {
+                uint32_t k[R];
+    #pragma unroll
+                for (int r = 0; r < R; ++r)
+                    k[r] = __hip_atomic_load(&hkey[fp[r] ? hs[r] : 0u], __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_WORKGROUP);
+                bool more = false;
+    #pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    if (fp[r]) {
+                        if (k[r] == it[r]) { fs[r] = hs[r]; fp[r] = false; }
+                        else if (k[r] == kNone) fp[r] = false;
+                        else hs[r] = (hs[r] + 1) & hmask;
+                    }
+                    more |= fp[r];
+                }
+                if (ballot(more) == 0) break;
+            }
             uint64_t conf = 0;
             uint32_t prod[(R + 3) / 4];
     #pragma unroll
             for (int q = 0; q < (R + 3) / 4; ++q) prod[q] = 0xFFFFFFFFu;
             // producer of read r: the highest lower lane writing the item (0xFF: none)
             uint32_t plane[R];
+            {
+                uint64_t wm[R + W];
     #pragma unroll
-            for (int r = 0; r < R; ++r) plane[r] = 0xFFu;
-    #pragma unroll
-            for (int r = 0; r < R + W; ++r) {
-                const uint32_t x = it[r];
-                if (valid && x != kNone) {
-                    const uint32_t h = hash_find(hkey, hmask, x);
-                    if (h != kNone) {
-                        const uint64_t wm = wmask(h) & lt;
-                        conf |= wm;
-                        if (delegate && r < R && wm) plane[r < R ? r : 0] = 63u - (uint32_t)__builtin_clzll(wm);
-                    }
+                for (int r = 0; r < R + W; ++r) {
+                    const uint32_t h = r < R ? fs[r < R ? r : 0] : slot_w[r < R ? 0 : r - R];
+                    const uint32_t at = h != kNone ? h : 0u;
+                    const uint64_t m = ((uint64_t)__hip_atomic_load(&hwhi[at], __ATOMIC_RELAXED,
+                                                                    __HIP_MEMORY_SCOPE_WORKGROUP) << 32) |
+                                       __hip_atomic_load(&hwlo[at], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    wm[r] = h != kNone ? (m & lt) : 0ull;
                 }
+    #pragma unroll
+                for (int r = 0; r < R + W; ++r) conf |= wm[r];
+    #pragma unroll
+                for (int r = 0; r < R; ++r)
+                    plane[r] = (delegate && wm[r]) ? 63u - (uint32_t)__builtin_clzll(wm[r]) : 0xFFu;
             }
             if (delegate) {
                 // which write slot of the producer holds the item; the shuffles run on
@@ -1089,7 +1151,7 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
                     prod[r / 4] = (prod[r / 4] & ~(0xFFu << sh)) | (code << sh);
                 }
             }
-            wg_fence();
+            asm volatile("" ::: "memory");
     #pragma unroll
             for (int w = 0; w < W; ++w)
                 if (slot_w[w] != kNone) {
@@ -1114,7 +1176,8 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
                 for (int w = 0; w < W; ++w) dst[(RW::pstamp + w) * kWave + lane] = pst[w];
     #pragma unroll
                 for (int r = 0; r < R + W; ++r)
-                    dst[(RW::bkt + r) * kWave + lane] = it[r] != kNone ? bucket_of(L, it[r]) : 0u;
+                    dst[(RW::bkt + r) * kWave + lane] =
+                        it[r] == kNone ? 0u : L.exact ? it[r] : (mx[r] & (L.nbuckets - 1));
             }
         }
         if (delegate) stage_chains<R, W>(dst, chain_scr, lane);
