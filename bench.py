@@ -259,8 +259,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--variant", default="stamp")
     ap.add_argument("--waves", type=int, default=int(os.environ.get("CSMV_BENCH_WAVES", "64")))
-    ap.add_argument("--serial-waves", type=int, default=16)
+    ap.add_argument("--serial-waves", type=int, default=12)
     ap.add_argument("--wg-waves", type=int, default=6)
+    ap.add_argument("--stage-ahead", type=int, default=5,
+                    help="requests the receivers stage ahead of the committer")
     ap.add_argument("--router", default="transfers", choices=("none", "transfers"),
                     help="none: every transaction on the normal (client) lane; transfers: "
                          "every transfer on the serialized lane")
@@ -296,7 +298,8 @@ def main():
     def make(variant, b, router, serial):
         e = csmv.Engine(N_ACCOUNTS, depth=8, atr_entries=4096, client_waves=args.waves,
                         wg_waves=args.wg_waves, variant=variant, timeout_ms=8000,
-                        n_serial_waves=serial if router != "none" else 0)
+                        n_serial_waves=serial if router != "none" else 0,
+                        stage_ahead=args.stage_ahead)
         e.bank_generate(b)
         if router == "transfers":
             op, _ = e.bank_read_stream()
@@ -379,6 +382,7 @@ def main():
             "client_waves": args.waves,
             "serial_lane_waves": args.serial_waves if args.router != "none" else 0,
             "server_wg_waves": args.wg_waves,
+            "stage_ahead": args.stage_ahead,
             "client_lanes": args.waves * 64,
             "record_ring": 4096,
         },
