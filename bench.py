@@ -263,9 +263,9 @@ def main():
     ap.add_argument("--wg-waves", type=int, default=6)
     ap.add_argument("--stage-ahead", type=int, default=5,
                     help="requests the receivers stage ahead of the committer")
-    ap.add_argument("--router", default="transfers", choices=("none", "transfers"),
+    ap.add_argument("--router", default="transfers", choices=("none", "transfers", "all"),
                     help="none: every transaction on the normal (client) lane; transfers: "
-                         "every transfer on the serialized lane")
+                         "every transfer on the serialized lane; all: audits too")
     ap.add_argument("--txns", type=int, default=1 << 24)
     ap.add_argument("--variant-txns", type=int, default=1 << 16,
                     help="stream length for the side-by-side reference variants")
@@ -276,6 +276,9 @@ def main():
     ap.add_argument("--serve-cache-reqs", type=int, default=1 << 18)
     ap.add_argument("--serve-bank-txns", type=int, default=1 << 16)
     ap.add_argument("--serve-cache-waves", type=int, default=16)
+    ap.add_argument("--serial-waves-all", type=int, default=20,
+                    help="serialized-lane waves of the run with every transaction routed")
+    ap.add_argument("--no-route-all", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--no-memcd", action="store_true")
     ap.add_argument("--no-sched", action="store_true")
@@ -304,6 +307,8 @@ def main():
         if router == "transfers":
             op, _ = e.bank_read_stream()
             e.bank_route(((op & 0xFF) == 0).astype(np.uint8))
+        elif router == "all":
+            e.bank_route(np.ones(b.n_txns, dtype=np.uint8))
         return e
 
     log(f"headline: {args.txns} txns, router {args.router}")
@@ -337,6 +342,21 @@ def main():
     # the per-lane protocol spends at least two same-address atomics per committed
     # update (ready bit + commit counter); its ceiling is the measured rate / 2
     plain_ceiling = atomic_rate / 2.0
+    route_all = {}
+    if not args.no_route_all and args.router != "all":
+        # the same stream with audits on the serialized lane too: no transaction aborts
+        ea = csmv.Engine(N_ACCOUNTS, depth=8, atr_entries=4096, client_waves=args.serial_waves_all + 1,
+                         wg_waves=args.wg_waves, variant=args.variant, timeout_ms=8000,
+                         n_serial_waves=args.serial_waves_all, stage_ahead=args.stage_ahead)
+        ea.bank_generate(bank)
+        ea.bank_route(np.ones(bank.n_txns, dtype=np.uint8))
+        ea.fill(INIT)
+        ra = ea.bank_run()
+        ea.close()
+        route_all = {"txn_per_s": ra["txn_per_s"], "abort_rate": ra["abort_rate"],
+                     "kernel_ms": ra["kernel_ms"], "serial_lane_waves": args.serial_waves_all,
+                     "commits_ro": ra["commits_ro"]}
+        log(f"every transaction on the serialized lane: {ra['txn_per_s']:.4g} txn/s")
     variants = {}
     if not args.no_variants:
         vbank = csmv.Bank(n_txns=args.variant_txns, seed=1, n_reads=4, n_reads_ro=4, pct_ro=10,
@@ -394,6 +414,7 @@ def main():
         "wall_s_timed_steps": wall,
         "roofline": roofline,
         "atomics": atomics,
+        "every_transaction_serialized": route_all,
         "variants": variants,
     }
     if not args.no_memcd:

@@ -270,7 +270,7 @@ template <int R, int W> __device__ __forceinline__ void bank_serial_client(const
     bool has = false, exhausted = false;
     uint64_t k = 0;
     uint32_t op = 0, ab = 0;
-    uint64_t n_upd = 0, t_reply = 0, t_round = 0, rounds = 0;
+    uint64_t n_upd = 0, n_ro = 0, t_reply = 0, t_round = 0, rounds = 0;
     for (;;) {
         if (ld_rlx(c.err)) break;
         const uint64_t tr0 = wall_clock64();
@@ -305,17 +305,18 @@ template <int R, int W> __device__ __forceinline__ void bank_serial_client(const
 #pragma unroll
         for (int w = 0; w < W; ++w) { tx.wa[w] = kNone; tx.wv[w] = 0; }
         const uint32_t amt = (op >> 8) & 0xFFu, nr = (op >> 16) & 0xFFu;
+        const bool audit = (op & 0xFFu) == kKindAudit;
         if (has) {
 #pragma unroll
             for (int j = 0; j < R; ++j)
                 if ((uint32_t)j < nr) {
                     const uint32_t it = c.addr[(uint64_t)j * c.n_txns + k];
                     tx.rs[j] = it;
-                    if (j < 2) tx_write(tx, it, 0);
+                    if (j < 2 && !audit) tx_write(tx, it, 0);
                 }
             tx.nr = nr;
             tx.wv[0] = (int32_t)(uint32_t)k;
-            tx.wv[1] = (int32_t)amt;
+            tx.wv[1] = (int32_t)(audit ? kReadOnlyAmt : amt);
         }
         bool fail = false;
         const uint32_t ts = tx_commit<R, W>(c, cw, tx, has, &fail, &t_reply, true);
@@ -324,7 +325,8 @@ template <int R, int W> __device__ __forceinline__ void bank_serial_client(const
             if (ts) {
                 c.log_ab[k] = ab;
                 has = false;
-                ++n_upd;
+                if (ts & kSnapshotBit) ++n_ro;
+                else ++n_upd;
             } else {
                 ++ab;
             }
@@ -339,6 +341,7 @@ template <int R, int W> __device__ __forceinline__ void bank_serial_client(const
         fetch_add(&c.prof[kProfClientRounds], rounds);
     }
     if (n_upd) fetch_add(&c.stats[kStatCommitUpd], n_upd);
+    if (n_ro) fetch_add(&c.stats[kStatCommitRo], n_ro);
     fence_rel();
     if (lane == 0) __hip_atomic_fetch_add(c.clients_done, 1u, __ATOMIC_RELEASE,
                                           __HIP_MEMORY_SCOPE_AGENT);
@@ -874,14 +877,17 @@ csmv_status csmv_bank_route(csmv_engine *e, const uint8_t *hot, size_t n)
     if (!op || !lists) { delete[] op; delete[] lists; return fail(e, CSMV_E_CAPACITY, "route buffers"); }
     hipError_t r = hipMemcpy(op, e->op, T * 4, hipMemcpyDeviceToHost);
     if (r != hipSuccess) { delete[] op; delete[] lists; HIPCHK(e, r); }
-    // normal lane first (stream order), then the serialized lane (stream order);
-    // audits are read-only and always stay on the normal lane
+    // normal lane first (stream order), then the serialized lane (stream order); an audit
+    // goes to the serialized lane when its read set fits the lane's wire
+    auto serial = [&](uint64_t k) {
+        return hot[k] && ((op[k] & 0xFFu) == kKindTransfer || ((op[k] >> 16) & 0xFFu) <= (uint32_t)kBankR);
+    };
     uint64_t nn = 0;
     for (uint64_t k = 0; k < T; ++k)
-        if (!(hot[k] && (op[k] & 0xFFu) == kKindTransfer)) lists[nn++] = (uint32_t)k;
+        if (!serial(k)) lists[nn++] = (uint32_t)k;
     uint64_t nh = 0;
     for (uint64_t k = 0; k < T; ++k)
-        if (hot[k] && (op[k] & 0xFFu) == kKindTransfer) lists[nn + nh++] = (uint32_t)k;
+        if (serial(k)) lists[nn + nh++] = (uint32_t)k;
     delete[] op;
     e->routed = false;
     const csmv_status st = route_reserve(e, T);

@@ -151,6 +151,12 @@ __device__ __forceinline__ bool hits(uint32_t x, const uint32_t *rs, const uint3
     return h;
 }
 
+// A running sum held in lane 0 of a vector register (the other lanes add zero).
+struct LaneSum {
+    uint64_t v = 0;
+    __device__ void add(uint64_t x) { v += lane_id() == 0 ? x : 0ull; }
+};
+
 // ---- committer --------------------------------------------------------------------------
 // Committed-value tables: the committer knows the newest committed value of an item
 // when it fits the exact LDS table, when the recent-value cache still holds it, or when
@@ -207,11 +213,19 @@ __device__ __forceinline__ void record_stamps(const Ctx &c, const ServerLayout &
 }
 
 // The bank transfer body executed by the serialized lane on the values the items held
-// before this transaction (in[j]); read-own-write as bank/csmv_sh/API.cu:9-18.
+// before this transaction (in[j]); read-own-write as bank/csmv_sh/API.cu:9-18.  A
+// read-only transaction (upd false) reads in[j] and writes nothing.
 template <int R, int W>
 __device__ __forceinline__ void transfer_exec(const uint32_t *it, uint32_t amt, const int32_t *in,
-                                              int32_t *rd, const uint32_t *wa, int32_t *nv)
+                                              int32_t *rd, const uint32_t *wa, int32_t *nv, bool upd)
 {
+    if (!upd) {
+#pragma unroll
+        for (int j = 0; j < R; ++j) rd[j] = it[j] == kNone ? 0 : in[j];
+#pragma unroll
+        for (int w = 0; w < W; ++w) nv[w] = 0;
+        return;
+    }
     uint32_t wi[2] = {kNone, kNone};
     int32_t wvv[2] = {0, 0};
 #pragma unroll
@@ -263,8 +277,10 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
     const uint64_t lt = (1ull << lane) - 1;
 
     uint32_t last = 0;  // newest assigned commit timestamp
-    uint64_t n_rw = 0, n_pv = 0, n_rec = 0, n_unk = 0, t_idle = 0, t_busy = 0;
-    uint64_t t_read = 0, t_res = 0, t_chain = 0, t_fin = 0;
+    // statistics and timing sums, kept in lane 0 of vector registers: as wave-uniform
+    // values they would take 20 scalar registers for the whole loop and push the loop's
+    // other uniform values out to spill slots
+    LaneSum n_rw, n_pv, n_rec, n_unk, t_idle, t_busy, t_read, t_res, t_chain, t_fin;
     uint32_t n = 0;
     for (;; ++n) {
         const uint32_t mbi = n % L.nmb;
@@ -387,17 +403,17 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
                 uint32_t bv[RO], bw[W], hs[W];
                 int32_t hv[RO];
 #pragma unroll
-                for (int r = 0; r < RO; ++r) {
-                    bv[r] = V.bucket[bkr[r]];
+                for (int r = 0; r < RO; ++r) bv[r] = V.bucket[bkr[r]];
+#pragma unroll
+                for (int w = 0; w < W; ++w) bw[w] = V.bucket[bkw[w]];
+#pragma unroll
+                for (int r = 0; r < RO; ++r)
                     hv[r] = __hip_atomic_load(&c.vals[ops[r] != kNone ? ops[r] : 0u], __ATOMIC_RELAXED,
                                               __HIP_MEMORY_SCOPE_WORKGROUP);
-                }
 #pragma unroll
-                for (int w = 0; w < W; ++w) {
-                    bw[w] = V.bucket[bkw[w]];
+                for (int w = 0; w < W; ++w)
                     hs[w] = __hip_atomic_load(&c.stamps[wa[w] != kNone ? wa[w] : 0u], __ATOMIC_RELAXED,
                                               __HIP_MEMORY_SCOPE_WORKGROUP);
-                }
 #pragma unroll
                 for (int r = 0; r < RO; ++r) base[r] = is_root[r] ? (bv[r] <= tg ? pval[r] : hv[r]) : 0;
 #pragma unroll
@@ -413,10 +429,14 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
             for (int r = 0; r < RO; ++r) in[r] = jbase[root[r]] + wgt[r];
             const bool okx = valid;
             tchain = wall_clock64();
-            transfer_exec<RO, W>(ops, amt, in, rd, wa, nv);
-            const uint64_t committed = ballot(okx);
+            // read-only lanes take no timestamp: each reads the state after the commits
+            // of the lanes below it and replies with that snapshot
+            const bool ro = amt == kReadOnlyAmt;
+            transfer_exec<RO, W>(ops, amt, in, rd, wa, nv, !ro);
+            const uint64_t committed = ballot(okx && !ro);
             nc = (uint32_t)__builtin_popcountll(committed);
-            if (okx) myts = last + 1 + (uint32_t)__builtin_popcountll(committed & lt);
+            const uint32_t below = last + (uint32_t)__builtin_popcountll(committed & lt);
+            if (okx) myts = ro ? (kSnapshotBit | below) : below + 1;
             last += nc;
             // only the last write of an item in the request is installed (staged flags);
             // no snapshot can fall inside the request's timestamp range, since the clock
@@ -450,7 +470,7 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
                     }
 #pragma unroll
                     for (int j = 0; j < 16; ++j) {
-                        if (!okx || x[j] == kNone) continue;
+                        if (!okx || ro || x[j] == kNone) continue;
                         const uint32_t pj = PROD(c0 + j);
                         int32_t val = v[j];
                         if (pj != 0xFFu) val = jnv[(pj & 63u) * W + (pj >> 6)];
@@ -470,11 +490,18 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
                         __hip_atomic_fetch_max(&c.stamps[wa[w]], myts, __ATOMIC_RELAXED,
                                                __HIP_MEMORY_SCOPE_WORKGROUP);
                 }
-            if (okx) {
+            if (okx && !ro) {
                 c.log_ts[k] = myts;
 #pragma unroll
                 for (int r = 0; r < RO; ++r)
                     if (ops[r] != kNone) c.log_v[(uint64_t)r * c.n_txns + k] = rd[r];
+            } else if (okx) {
+                // an audit's log: its snapshot and the sum of its reads (exe.cu:213-223)
+                uint32_t sum = 0;
+#pragma unroll
+                for (int r = 0; r < RO; ++r) sum += (uint32_t)rd[r];
+                c.log_ts[k] = below;
+                c.log_v[k] = (int32_t)sum;
             }
             uint64_t *rv = c.reply_v + ((size_t)cw * kWave + lane) * 2 * W;
 #pragma unroll
@@ -573,8 +600,8 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
                 }
             }
             const bool ok = valid && !rec && !conflict;
-            n_rec += __builtin_popcountll(ballot(rec));
-            n_rw += __builtin_popcountll(ballot(valid && !rec && conflict));
+            n_rec.add(__builtin_popcountll(ballot(rec)));
+            n_rw.add(__builtin_popcountll(ballot(valid && !rec && conflict)));
 
             // Lane order inside the request: lane L commits only if no lower lane it
             // conflicts with commits.  Lanes without such conflicts are final at once;
@@ -588,7 +615,7 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
                 it &= it - 1;
                 if ((readlane64(conf, i) & committed) == 0) committed |= 1ull << i;
             }
-            n_pv += __builtin_popcountll(m & ~committed);
+            n_pv.add(__builtin_popcountll(m & ~committed));
             const bool mine = (committed >> lane) & 1ull;
             nc = (uint32_t)__builtin_popcountll(committed);
             auto record = [&](uint32_t ts) {
@@ -638,32 +665,32 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
         if (!server_wb && nc == 0 && lane == 0)
             st_rlx(&c.done[n & c.qmask], ((uint64_t)(n + 1) << 32) | last);
         const uint64_t t2 = wall_clock64();
-        t_idle += t1 - t0;
-        t_busy += t2 - t1;
-        t_read += tr - t1;
+        t_idle.add(t1 - t0);
+        t_busy.add(t2 - t1);
+        t_read.add(tr - t1);
         if (delegate) {
-            t_res += tres - tr;
-            t_chain += tchain - tres;
-            t_fin += t2 - tchain;
+            t_res.add(tres - tr);
+            t_chain.add(tchain - tres);
+            t_fin.add(t2 - tchain);
         } else {
-            t_res += t2 - tr;
+            t_res.add(t2 - tr);
         }
     }
     if (lane == 0) {
         lds[kLdsCfinal] = n;
         lds_st_rel(&lds[kLdsCdone], 1u);
-        fetch_add(&c.stats[kStatAbortRw], n_rw);
-        fetch_add(&c.stats[kStatAbortPrevalid], n_pv);
-        fetch_add(&c.stats[kStatAbortRecord], n_rec);
-        fetch_add(&c.stats[kStatAbortUnknown], n_unk);
+        fetch_add(&c.stats[kStatAbortRw], n_rw.v);
+        fetch_add(&c.stats[kStatAbortPrevalid], n_pv.v);
+        fetch_add(&c.stats[kStatAbortRecord], n_rec.v);
+        fetch_add(&c.stats[kStatAbortUnknown], n_unk.v);
         fetch_add(&c.stats[kStatBatches], (uint64_t)n);
         st_rlx(&c.stats[kStatFinalTs], (uint64_t)last);
-        fetch_add(&c.prof[kProfCommitIdle], t_idle);
-        fetch_add(&c.prof[kProfCommitBusy], t_busy);
-        fetch_add(&c.prof[kProfCommitRead], t_read);
-        fetch_add(&c.prof[kProfCommitResolve], t_res);
-        fetch_add(&c.prof[kProfCommitChain], t_chain);
-        fetch_add(&c.prof[kProfCommitFinish], t_fin);
+        fetch_add(&c.prof[kProfCommitIdle], t_idle.v);
+        fetch_add(&c.prof[kProfCommitBusy], t_busy.v);
+        fetch_add(&c.prof[kProfCommitRead], t_read.v);
+        fetch_add(&c.prof[kProfCommitResolve], t_res.v);
+        fetch_add(&c.prof[kProfCommitChain], t_chain.v);
+        fetch_add(&c.prof[kProfCommitFinish], t_fin.v);
     }
 }
 

@@ -30,6 +30,12 @@ constexpr uint32_t kDelegateBit = 0x80000000u;
 constexpr uint32_t kWave = 64;
 // bank stream: op[k] = kind | amount << 8 | reads << 16
 constexpr uint32_t kKindTransfer = 0, kKindAudit = 1;
+// Serialized lane: a read-only transaction is sent with this amount; the server reads its
+// items in commit order without assigning it a timestamp, and replies with its snapshot
+// (the timestamp of the last commit it sees) tagged by kSnapshotBit.  Commit timestamps
+// stay below 2^31.
+constexpr uint32_t kReadOnlyAmt = 0xFFFFFFFFu;
+constexpr uint32_t kSnapshotBit = 0x80000000u;
 
 // ---- wave64 primitives --------------------------------------------------------------
 __device__ __forceinline__ uint32_t lane_id()
@@ -500,8 +506,10 @@ __device__ __forceinline__ uint32_t tx_commit(const Ctx &c, uint32_t cw, Tx<R, W
         if (cm) {
             if (install_n >= 0) writeback<R, WI>(c, install_wa, install_wv, (uint32_t)install_n, ts);
             else writeback<R, W>(c, tx.wa, tx.wv, tx.nw, ts);
+            // a read-only lane of the serialized lane replies with its snapshot, which is
+            // the request's last timestamp when it is the highest committed lane
             const uint32_t last_lane = 63u - (uint32_t)__builtin_clzll(cm);
-            const uint32_t end_ts = readlane(ts, last_lane);
+            const uint32_t end_ts = readlane(ts & ~kSnapshotBit, last_lane);
             fence_rel();
             if (lane == 0) st_rlx(&c.done[n & c.qmask], ((n + 1) << 32) | end_ts);
         }

@@ -142,6 +142,21 @@ def test_serialized_lane_hot_zipf_hashed():
              wg_waves=6, check_stream=False)
 
 
+@pytest.mark.parametrize("n_items,pct_ro", [(4096, 30), (1 << 20, 30), (1 << 16, 100)])
+def test_serialized_lane_read_only(n_items, pct_ro):
+    """Every transaction, audits included, on the serialized lane: an audit is read in
+    commit order without a timestamp of its own and logged with the snapshot it saw (the
+    last commit below it); nothing aborts, and the commit timestamps stay dense."""
+    bank = csmv.Bank(n_txns=1 << 16, seed=53, n_reads=4, n_reads_ro=4, pct_ro=pct_ro,
+                     zipf=True)
+    st = run_bank(n_items, bank, variant="stamp", depth=8, waves=16, serial_waves=8,
+                  wg_waves=6, route=lambda op: np.ones(op.size, dtype=np.uint8))
+    assert st["aborts"] == 0
+    assert st["commits_ro"] > 0
+    if pct_ro == 100:
+        assert st["commits_update"] == 0
+
+
 def test_serialized_lane_small_cache_unknowns():
     """A tiny LDS budget forces a small recent-value cache: operands whose value is not at
     hand make the server refuse (the client resends); results must not change."""
