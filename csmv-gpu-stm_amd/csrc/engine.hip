@@ -110,17 +110,17 @@ __global__ void __launch_bounds__(256) pool_fill_kernel(uint32_t *ht, uint64_t *
     if (i >= n) return;
     ht[i] = 0;
     const int32_t v = values ? values[i] : fill;
-    slot[i] = (uint32_t)v;
+    slot[(size_t)i * depth] = (uint32_t)v;
     vals[i] = v;
-    for (uint32_t d = 1; d < depth; ++d) slot[(size_t)d * n + i] = 0;
+    for (uint32_t d = 1; d < depth; ++d) slot[(size_t)i * depth + d] = 0;
 }
 
 __global__ void __launch_bounds__(256) pool_head_kernel(const uint32_t *ht, const uint64_t *slot,
-                                                        uint32_t n, int32_t *out)
+                                                        uint32_t n, uint32_t depth, int32_t *out)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    out[i] = (int32_t)(uint32_t)slot[(size_t)(ht[i] & 0xFFFFu) * n + i];
+    out[i] = (int32_t)(uint32_t)slot[(size_t)i * depth + (ht[i] & 0xFFFFu)];
 }
 
 // ---- bank driver (client wavefronts) -----------------------------------------------------------
@@ -795,7 +795,7 @@ csmv_status csmv_read_state_i32(csmv_engine *e, int32_t *out, size_t n)
 {
     if (!e || !out || n != e->cfg.n_items) return fail(e, CSMV_E_ARG, "read_state: size mismatch");
     hipLaunchKernelGGL(pool_head_kernel, dim3((n + 255) / 256), dim3(256), 0, e->stream, e->ht,
-                       e->slot, (uint32_t)n, e->tmp);
+                       e->slot, (uint32_t)n, e->cfg.depth, e->tmp);
     HIPCHK(e, hipGetLastError());
     HIPCHK(e, hipMemcpyAsync(out, e->tmp, n * 4, hipMemcpyDeviceToHost, e->stream));
     HIPCHK(e, hipStreamSynchronize(e->stream));

@@ -118,7 +118,7 @@ __device__ __forceinline__ bool pool_read_chunk(const Ctx &c, const uint32_t (&i
     for (int j = 0; j < N; ++j) ht[j] = it[j] != kNone ? ld_rlx(&c.ht[it[j]]) : 0u;
 #pragma unroll
     for (int j = 0; j < N; ++j)
-        x[j] = it[j] != kNone ? ld_rlx(&c.slot[(size_t)(ht[j] & 0xFFFFu) * c.n_items + it[j]]) : 0ull;
+        x[j] = it[j] != kNone ? ld_rlx(&c.slot[(size_t)it[j] * c.depth + (ht[j] & 0xFFFFu)]) : 0ull;
     bool ok = true;
 #pragma unroll
     for (int j = 0; j < N; ++j) {

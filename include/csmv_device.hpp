@@ -13,8 +13,9 @@
 //
 // Memory layout (structure of arrays in HBM):
 //   ht[i]                  head | tail << 16 of item i's version ring
-//   slot[d * n_items + i]  version << 32 | value (one 64-bit word, read and written
-//                          atomically, so a reader never sees a torn pair)
+//   slot[i * depth + d]    version << 32 | value (one 64-bit word, read and written
+//                          atomically, so a reader never sees a torn pair); an item's
+//                          ring is contiguous, so a ring walk stays in one or two lines
 // Cross-workgroup traffic uses agent-scope atomics: gfx950's eight XCDs have private
 // L2 caches, and agent scope is what keeps them coherent.
 #pragma once
@@ -249,7 +250,7 @@ __device__ __forceinline__ bool pool_read(const Ctx &c, uint32_t item, uint32_t 
     const uint32_t h = ht & 0xFFFFu, t = ht >> 16;
     uint32_t j = h;
     for (uint32_t d = 0; d < c.depth; ++d) {
-        const uint64_t x = ld_rlx(&c.slot[(size_t)j * c.n_items + item]);
+        const uint64_t x = ld_rlx(&c.slot[(size_t)item * c.depth + j]);
         if ((uint32_t)(x >> 32) <= snap) {
             out = (int32_t)(uint32_t)x;
             return true;
@@ -364,7 +365,7 @@ __device__ __forceinline__ void writeback(const Ctx &c, const uint32_t *wa, cons
             uint32_t h = ht & 0xFFFFu, t = ht >> 16;
             h = (h + 1 == c.depth) ? 0 : h + 1;
             if (h == t) t = (t + 1 == c.depth) ? 0 : t + 1;
-            st_rlx(&c.slot[(size_t)h * c.n_items + item],
+            st_rlx(&c.slot[(size_t)item * c.depth + h],
                    ((uint64_t)ts << 32) | (uint32_t)wv[w]);
             nh[w] = h | (t << 16);
         }
@@ -471,12 +472,12 @@ __device__ __forceinline__ uint32_t tx_commit(const Ctx &c, uint32_t cw, Tx<R, W
                 if (pend[w]) {
                     const uint32_t item = tx.wa[w];
                     const uint32_t ht = ld_rlx(&c.ht[item]);
-                    const uint64_t head = ld_rlx(&c.slot[(size_t)(ht & 0xFFFFu) * c.n_items + item]);
+                    const uint64_t head = ld_rlx(&c.slot[(size_t)item * c.depth + (ht & 0xFFFFu)]);
                     if ((uint32_t)(head >> 32) == prev[w]) {
                         uint32_t h = ht & 0xFFFFu, t = ht >> 16;
                         h = (h + 1 == c.depth) ? 0 : h + 1;
                         if (h == t) t = (t + 1 == c.depth) ? 0 : t + 1;
-                        st_rlx(&c.slot[(size_t)h * c.n_items + item],
+                        st_rlx(&c.slot[(size_t)item * c.depth + h],
                                ((uint64_t)ts << 32) | (uint32_t)tx.wv[w]);
                         nh[w] = h | (t << 16);
                         go[w] = true;
