@@ -112,13 +112,17 @@ template <int N>
 __device__ __forceinline__ bool pool_read_chunk(const Ctx &c, const uint32_t (&it)[N], uint32_t snap,
                                                 int32_t (&out)[N])
 {
+    // every load of a round is issued (an empty entry reads item 0) and the round is pinned
+    // (pin_round): otherwise the compiler waits for each load before issuing the next
     uint32_t ht[N];
     uint64_t x[N];
 #pragma unroll
-    for (int j = 0; j < N; ++j) ht[j] = it[j] != kNone ? ld_rlx(&c.ht[it[j]]) : 0u;
+    for (int j = 0; j < N; ++j) ht[j] = ld_rlx(&c.ht[it[j] != kNone ? it[j] : 0u]);
+    pin_round(ht);
 #pragma unroll
     for (int j = 0; j < N; ++j)
-        x[j] = it[j] != kNone ? ld_rlx(&c.slot[(size_t)it[j] * c.depth + (ht[j] & 0xFFFFu)]) : 0ull;
+        x[j] = ld_rlx(&c.slot[(size_t)(it[j] != kNone ? it[j] : 0u) * c.depth + (ht[j] & 0xFFFFu)]);
+    pin_round(x);
     bool ok = true;
 #pragma unroll
     for (int j = 0; j < N; ++j) {

@@ -44,6 +44,32 @@ __device__ __forceinline__ uint32_t lane_id()
     return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
 }
 __device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+
+// An empty asm statement that takes every value of a round of loads: nothing computed from
+// them can be scheduled before the last one has arrived, so the compiler issues the round
+// as one batch and waits once, instead of interleaving each load with its use (which it
+// does under register pressure, one memory latency per load).
+#define CSMV_PIN(i) "+v"(a[i])
+template <typename V, int N>
+__device__ __forceinline__ void pin_round(V (&a)[N])
+{
+    if constexpr (N == 4) asm volatile("" : CSMV_PIN(0), CSMV_PIN(1), CSMV_PIN(2), CSMV_PIN(3));
+    else if constexpr (N == 8)
+        asm volatile("" : CSMV_PIN(0), CSMV_PIN(1), CSMV_PIN(2), CSMV_PIN(3), CSMV_PIN(4), CSMV_PIN(5),
+                     CSMV_PIN(6), CSMV_PIN(7));
+    else if constexpr (N == 9)
+        asm volatile("" : CSMV_PIN(0), CSMV_PIN(1), CSMV_PIN(2), CSMV_PIN(3), CSMV_PIN(4), CSMV_PIN(5),
+                     CSMV_PIN(6), CSMV_PIN(7), CSMV_PIN(8));
+    else if constexpr (N == 16)
+        asm volatile("" : CSMV_PIN(0), CSMV_PIN(1), CSMV_PIN(2), CSMV_PIN(3), CSMV_PIN(4), CSMV_PIN(5),
+                     CSMV_PIN(6), CSMV_PIN(7), CSMV_PIN(8), CSMV_PIN(9), CSMV_PIN(10), CSMV_PIN(11),
+                     CSMV_PIN(12), CSMV_PIN(13), CSMV_PIN(14), CSMV_PIN(15));
+    else {
+#pragma unroll
+        for (int i = 0; i < N; ++i) asm volatile("" : CSMV_PIN(i));
+    }
+}
+#undef CSMV_PIN
 // Number of set bits of m below this lane (the reference's popc(ballot & lanemask_lt)).
 __device__ __forceinline__ uint32_t prefix_count(uint64_t m)
 {
