@@ -524,13 +524,44 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
                     // was written since is checked against its exact stamp in HBM
                     conflict = pref;
                     uint32_t ex[R + W];
+                    if constexpr (kWide) {
 #pragma unroll
-                    for (int r = 0; r < R + W; ++r) {
-                        const uint32_t x = r < R ? RS(r) : wa[r - R];
-                        ex[r] = 0;
-                        if (x != kNone && V.bucket[bucket_of(L, x)] > tg)
-                            ex[r] = __hip_atomic_load(&c.stamps[x], __ATOMIC_RELAXED,
-                                                      __HIP_MEMORY_SCOPE_WORKGROUP);
+                        for (int r = 0; r < R + W; ++r) {
+                            const uint32_t x = r < R ? RS(r) : wa[r - R];
+                            ex[r] = 0;
+                            if (x != kNone && V.bucket[bucket_of(L, x)] > tg)
+                                ex[r] = __hip_atomic_load(&c.stamps[x], __ATOMIC_RELAXED,
+                                                          __HIP_MEMORY_SCOPE_WORKGROUP);
+                        }
+                    } else {
+                        // the flagged items' stamps in one batch of loads (an item not
+                        // flagged reads item 0 and is ignored), skipped when no lane has one
+                        uint32_t bx[R + W];
+                        bool fl[R + W], anyf = false;
+#pragma unroll
+                        for (int r = 0; r < R + W; ++r) {
+                            const uint32_t x = r < R ? RS(r) : wa[r - R];
+                            bx[r] = V.bucket[x != kNone ? bucket_of(L, x) : 0u];
+                        }
+#pragma unroll
+                        for (int r = 0; r < R + W; ++r) {
+                            const uint32_t x = r < R ? RS(r) : wa[r - R];
+                            fl[r] = x != kNone && bx[r] > tg;
+                            anyf |= fl[r];
+                        }
+#pragma unroll
+                        for (int r = 0; r < R + W; ++r) ex[r] = 0;
+                        if (ballot(anyf)) {
+#pragma unroll
+                            for (int r = 0; r < R + W; ++r) {
+                                const uint32_t x = r < R ? RS(r) : wa[r - R];
+                                ex[r] = __hip_atomic_load(&c.stamps[fl[r] ? x : 0u], __ATOMIC_RELAXED,
+                                                          __HIP_MEMORY_SCOPE_WORKGROUP);
+                            }
+                            pin_round(ex);
+#pragma unroll
+                            for (int r = 0; r < R + W; ++r) ex[r] = fl[r] ? ex[r] : 0u;
+                        }
                     }
 #pragma unroll
                     for (int r = 0; r < R + W; ++r) conflict |= ex[r] > snap;
@@ -1039,28 +1070,36 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
             for (int r = 0; r < R; ++r) pval[r] = 0;
     #pragma unroll
             for (int w = 0; w < W; ++w) pst[w] = 0;
-            if (!L.exact && valid) {
+            // HBM state as of T_g: every load is issued (an empty entry reads item 0) and
+            // each round is pinned, so the round costs one memory latency, not one per item
+            if (!L.exact) {
                 if (delegate) {
+                    uint32_t pv[R], ps[W];
     #pragma unroll
                     for (int r = 0; r < R; ++r)
-                        if (it[r] != kNone)
-                            pval[r] = __hip_atomic_load(&c.vals[it[r]], __ATOMIC_RELAXED,
-                                                        __HIP_MEMORY_SCOPE_WORKGROUP);
+                        pv[r] = (uint32_t)__hip_atomic_load(&c.vals[it[r] != kNone ? it[r] : 0u],
+                                                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     #pragma unroll
                     for (int w = 0; w < W; ++w)
-                        if (it[R + w] != kNone)
-                            pst[w] = __hip_atomic_load(&c.stamps[it[R + w]], __ATOMIC_RELAXED,
-                                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+                        ps[w] = __hip_atomic_load(&c.stamps[it[R + w] != kNone ? it[R + w] : 0u],
+                                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    pin_round(pv);
+                    pin_round(ps);
+    #pragma unroll
+                    for (int r = 0; r < R; ++r) pval[r] = (valid && it[r] != kNone) ? (int32_t)pv[r] : 0;
+    #pragma unroll
+                    for (int w = 0; w < W; ++w) pst[w] = (valid && it[R + w] != kNone) ? ps[w] : 0u;
                 } else if (stamp_mode) {
-                    uint32_t p = 0;
+                    uint32_t v[R + W];
     #pragma unroll
                     for (int r = 0; r < R + W; ++r)
-                        if (it[r] != kNone) {
-                            const uint32_t v = __hip_atomic_load(&c.stamps[it[r]], __ATOMIC_RELAXED,
-                                                                 __HIP_MEMORY_SCOPE_WORKGROUP);
-                            p = v > p ? v : p;
-                        }
-                    pref = p > snap;
+                        v[r] = __hip_atomic_load(&c.stamps[it[r] != kNone ? it[r] : 0u], __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+                    pin_round(v);
+                    uint32_t p = 0;
+    #pragma unroll
+                    for (int r = 0; r < R + W; ++r) p = (it[r] != kNone && v[r] > p) ? v[r] : p;
+                    pref = valid && p > snap;
                 }
             }
 

@@ -65,8 +65,13 @@ __device__ __forceinline__ void pin_round(V (&a)[N])
                      CSMV_PIN(6), CSMV_PIN(7), CSMV_PIN(8), CSMV_PIN(9), CSMV_PIN(10), CSMV_PIN(11),
                      CSMV_PIN(12), CSMV_PIN(13), CSMV_PIN(14), CSMV_PIN(15));
     else {
+        // other sizes in groups of eight (each group one batch)
 #pragma unroll
-        for (int i = 0; i < N; ++i) asm volatile("" : CSMV_PIN(i));
+        for (int i = 0; i + 7 < N; i += 8)
+            asm volatile("" : CSMV_PIN(i), CSMV_PIN(i + 1), CSMV_PIN(i + 2), CSMV_PIN(i + 3),
+                         CSMV_PIN(i + 4), CSMV_PIN(i + 5), CSMV_PIN(i + 6), CSMV_PIN(i + 7));
+#pragma unroll
+        for (int i = N & ~7; i < N; ++i) asm volatile("" : CSMV_PIN(i));
     }
 }
 #undef CSMV_PIN
