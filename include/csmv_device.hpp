@@ -386,13 +386,20 @@ template <int R, int W>
 __device__ __forceinline__ void writeback(const Ctx &c, const uint32_t *wa, const int32_t *wv,
                                           uint32_t nw, uint32_t ts)
 {
-    uint32_t nh[W];
+    // the head words of every write in one batch of loads (pin_round), then the stores
+    uint32_t nh[W], hw[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        const bool on = ts && (uint32_t)w < nw && wa[w] != kNone;
+        hw[w] = ld_rlx(&c.ht[on ? wa[w] : 0u]);
+    }
+    pin_round(hw);
 #pragma unroll
     for (int w = 0; w < W; ++w) {
         nh[w] = 0;
         if (ts && (uint32_t)w < nw && wa[w] != kNone) {
             const uint32_t item = wa[w];
-            const uint32_t ht = ld_rlx(&c.ht[item]);
+            const uint32_t ht = hw[w];
             uint32_t h = ht & 0xFFFFu, t = ht >> 16;
             h = (h + 1 == c.depth) ? 0 : h + 1;
             if (h == t) t = (t + 1 == c.depth) ? 0 : t + 1;
@@ -496,14 +503,24 @@ __device__ __forceinline__ uint32_t tx_commit(const Ctx &c, uint32_t cw, Tx<R, W
             bool more = false, any = false;
             uint32_t nh[W];
             bool go[W];
+            // head words, then head slots, of every pending write: two batches of loads
+            uint32_t hw[W];
+            uint64_t hs[W];
+#pragma unroll
+            for (int w = 0; w < W; ++w) hw[w] = ld_rlx(&c.ht[pend[w] ? tx.wa[w] : 0u]);
+            pin_round(hw);
+#pragma unroll
+            for (int w = 0; w < W; ++w)
+                hs[w] = ld_rlx(&c.slot[(size_t)(pend[w] ? tx.wa[w] : 0u) * c.depth + (hw[w] & 0xFFFFu)]);
+            pin_round(hs);
 #pragma unroll
             for (int w = 0; w < W; ++w) {
                 go[w] = false;
                 nh[w] = 0;
                 if (pend[w]) {
                     const uint32_t item = tx.wa[w];
-                    const uint32_t ht = ld_rlx(&c.ht[item]);
-                    const uint64_t head = ld_rlx(&c.slot[(size_t)item * c.depth + (ht & 0xFFFFu)]);
+                    const uint32_t ht = hw[w];
+                    const uint64_t head = hs[w];
                     if ((uint32_t)(head >> 32) == prev[w]) {
                         uint32_t h = ht & 0xFFFFu, t = ht >> 16;
                         h = (h + 1 == c.depth) ? 0 : h + 1;
