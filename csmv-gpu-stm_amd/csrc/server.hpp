@@ -1061,6 +1061,7 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
             for (int r = 0; r < R + W; ++r) it[r] = ld_rlx(&src[(1 + r) * kWave + lane]);
     #pragma unroll
             for (int w = 0; w < W; ++w) wv[w] = ld_rlx(&src[(1 + R + W + w) * kWave + lane]);
+            pin_round(it);
             const bool valid = snap != kNone;
 
             bool pref = false;
