@@ -458,16 +458,28 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
                     int32_t v[16];
 #pragma unroll
                     for (int j = 0; j < 16; ++j) x[j] = (c0 + j < R) ? RS(c0 + j) : kNone;
+                    // the committed values the reads need, as one pinned round of loads
+                    // (a read that does not need one loads item 0)
+                    uint32_t need = 0;
 #pragma unroll
                     for (int j = 0; j < 16; ++j) {
                         const bool own = x[j] != kNone && (x[j] == wa[0] || x[j] == wa[1]);
                         const uint32_t pj = (c0 + j < R) ? PROD(c0 + j) : 0xFFu;
-                        v[j] = 0;
-                        if (okx && x[j] != kNone && !own && pj == 0xFFu)
-                            v[j] = L.exact ? (int32_t)V.lval[x[j]]
-                                           : __hip_atomic_load(&c.vals[x[j]], __ATOMIC_RELAXED,
-                                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+                        if (okx && x[j] != kNone && !own && pj == 0xFFu) need |= 1u << j;
                     }
+                    if (L.exact) {
+#pragma unroll
+                        for (int j = 0; j < 16; ++j)
+                            v[j] = (int32_t)V.lval[((need >> j) & 1u) ? x[j] : 0u];
+                    } else {
+#pragma unroll
+                        for (int j = 0; j < 16; ++j)
+                            v[j] = __hip_atomic_load(&c.vals[((need >> j) & 1u) ? x[j] : 0u], __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_WORKGROUP);
+                        pin_round(v);
+                    }
+#pragma unroll
+                    for (int j = 0; j < 16; ++j) v[j] = ((need >> j) & 1u) ? v[j] : 0;
 #pragma unroll
                     for (int j = 0; j < 16; ++j) {
                         if (!okx || ro || x[j] == kNone) continue;
@@ -973,15 +985,25 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
                 for (int j = 0; j < kChunk; ++j) x[j] = ld_rlx(&src[(1 + c0 + j) * kWave + lane]);
                 // staging-time stamps (normal lane) or, delegated, the committed values
                 // of the transfer operands
+                // (one pinned round of loads; an item not wanted reads item 0)
+                const uint32_t *tab = delegate ? (const uint32_t *)c.vals : c.stamps;
+                if (!L.exact && (!delegate || c0 < RO)) {
+                    pin_round(x);
 #pragma unroll
-                for (int j = 0; j < kChunk; ++j) {
-                    const bool want = !L.exact && valid && x[j] != kNone && (!delegate || c0 + j < RO);
-                    st[j] = 0u;
-                    if (want)
-                        st[j] = delegate ? (uint32_t)__hip_atomic_load(&c.vals[x[j]], __ATOMIC_RELAXED,
-                                                                       __HIP_MEMORY_SCOPE_WORKGROUP)
-                                         : __hip_atomic_load(&c.stamps[x[j]], __ATOMIC_RELAXED,
-                                                             __HIP_MEMORY_SCOPE_WORKGROUP);
+                    for (int j = 0; j < kChunk; ++j) {
+                        const bool want = valid && x[j] != kNone && (!delegate || c0 + j < RO);
+                        st[j] = __hip_atomic_load(&tab[want ? x[j] : 0u], __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_WORKGROUP);
+                    }
+                    pin_round(st);
+#pragma unroll
+                    for (int j = 0; j < kChunk; ++j) {
+                        const bool want = valid && x[j] != kNone && (!delegate || c0 + j < RO);
+                        st[j] = want ? st[j] : 0u;
+                    }
+                } else {
+#pragma unroll
+                    for (int j = 0; j < kChunk; ++j) st[j] = 0u;
                 }
 #pragma unroll
                 for (int j = 0; j < kChunk; ++j) {
