@@ -177,9 +177,12 @@ template <int R, int W> __device__ __forceinline__ void bank_client(const Ctx &c
             for (uint32_t c0 = 0; c0 < nr; c0 += 16) {
                 uint32_t it[16];
                 int32_t v[16];
+                // (every row load issued, in bounds: a read past nr loads row 0)
 #pragma unroll
-                for (int j = 0; j < 16; ++j)
-                    it[j] = c0 + j < nr ? c.addr[(uint64_t)(c0 + j) * T + k] : kNone;
+                for (int j = 0; j < 16; ++j) it[j] = c.addr[(uint64_t)(c0 + j < nr ? c0 + j : 0u) * T + k];
+                pin_round(it);
+#pragma unroll
+                for (int j = 0; j < 16; ++j) it[j] = c0 + j < nr ? it[j] : kNone;
                 ok = pool_read_chunk<16>(c, it, tx.snap, v) && ok;
 #pragma unroll
                 for (int j = 0; j < 16; ++j) sum += it[j] != kNone ? (uint32_t)v[j] : 0u;
@@ -192,7 +195,10 @@ template <int R, int W> __device__ __forceinline__ void bank_client(const Ctx &c
             uint32_t it[R];
             int32_t v[R];
 #pragma unroll
-            for (int j = 0; j < R; ++j) it[j] = (uint32_t)j < nr ? c.addr[(uint64_t)j * T + k] : kNone;
+            for (int j = 0; j < R; ++j) it[j] = c.addr[(uint64_t)((uint32_t)j < nr ? j : 0) * T + k];
+            pin_round(it);
+#pragma unroll
+            for (int j = 0; j < R; ++j) it[j] = (uint32_t)j < nr ? it[j] : kNone;
             if (!pool_read_chunk<R>(c, it, tx.snap, v)) tx.aborted = true;
 #pragma unroll
             for (int j = 0; j < R; ++j) {

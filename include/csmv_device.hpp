@@ -480,19 +480,24 @@ __device__ __forceinline__ uint32_t tx_commit(const Ctx &c, uint32_t cw, Tx<R, W
         const uint64_t *rv = c.reply_v + ((size_t)cw * kWave + lane) * 2 * W;
         uint32_t prev[W];
         bool pend[W];
+        // all 2W words of the reply in one batch of loads per poll
+        uint64_t xy[2 * W];
+        for (;;) {
+#pragma unroll
+            for (int q = 0; q < 2 * W; ++q) xy[q] = ld_rlx(&rv[q]);
+            pin_round(xy);
+            bool stale = false;
+#pragma unroll
+            for (int w = 0; w < W; ++w)
+                stale |= ts != 0 && (uint32_t)w < tx.nw &&
+                         ((xy[w] >> 32) != n + 1 || (xy[W + w] >> 32) != n + 1);
+            if (ballot(stale) == 0) break;
+            if (!sp.tick(c, kErrReplyTimeout)) { *fail = true; return 0; }
+        }
 #pragma unroll
         for (int w = 0; w < W; ++w) {
-            uint64_t x, y;
-            for (;;) {
-                x = ld_rlx(&rv[w]);
-                y = ld_rlx(&rv[W + w]);
-                if (ballot(ts != 0 && (uint32_t)w < tx.nw &&
-                           ((x >> 32) != n + 1 || (y >> 32) != n + 1)) == 0)
-                    break;
-                if (!sp.tick(c, kErrReplyTimeout)) { *fail = true; return 0; }
-            }
-            tx.wv[w] = (int32_t)(uint32_t)x;
-            prev[w] = (uint32_t)y;
+            tx.wv[w] = (int32_t)(uint32_t)xy[w];
+            prev[w] = (uint32_t)xy[W + w];
             pend[w] = ts != 0 && (uint32_t)w < tx.nw && prev[w] != kNone && tx.wa[w] != kNone;
             if (!pend[w]) tx.wa[w] = kNone;
         }
