@@ -259,7 +259,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--variant", default="stamp")
     ap.add_argument("--waves", type=int, default=int(os.environ.get("CSMV_BENCH_WAVES", "64")))
-    ap.add_argument("--serial-waves", type=int, default=12)
+    ap.add_argument("--serial-waves", type=int, default=10)
     ap.add_argument("--wg-waves", type=int, default=6)
     ap.add_argument("--stage-ahead", type=int, default=5,
                     help="requests the receivers stage ahead of the committer")
@@ -276,7 +276,7 @@ def main():
     ap.add_argument("--serve-cache-reqs", type=int, default=1 << 18)
     ap.add_argument("--serve-bank-txns", type=int, default=1 << 16)
     ap.add_argument("--serve-cache-waves", type=int, default=16)
-    ap.add_argument("--serial-waves-all", type=int, default=20,
+    ap.add_argument("--serial-waves-all", type=int, default=12,
                     help="serialized-lane waves of the run with every transaction routed")
     ap.add_argument("--no-route-all", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
