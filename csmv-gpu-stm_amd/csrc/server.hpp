@@ -250,7 +250,10 @@ __device__ __forceinline__ void transfer_exec(const uint32_t *it, uint32_t amt, 
     }
 }
 
-template <int R, int W, bool D>
+// kStampCw: the committer is compiled for the STAMP preset (stamp validation, batched
+// insertion, client write-back) alone, so the other variants' code and the values it
+// keeps live drop out of the loop (fewer scalar registers spilled on the hot path).
+template <int R, int W, bool D, bool kStampCw = false>
 __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, uint32_t *lds)
 {
     using RW = Rows<R, W>;
@@ -259,9 +262,9 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
     constexpr bool kWide = kWideWire<R>;
     const uint32_t lane = lane_id();
     const uint32_t E = c.atr_entries, emask = E - 1;
-    const uint32_t vmode = kWide ? kValStamp : (c.variant & 0x3u);
-    const bool batched = (c.variant & kInsertBatch) != 0;
-    const bool server_wb = (c.variant & kServerWriteback) != 0;
+    const uint32_t vmode = (kWide || kStampCw) ? kValStamp : (c.variant & 0x3u);
+    const bool batched = kStampCw || (c.variant & kInsertBatch) != 0;
+    const bool server_wb = !kStampCw && (c.variant & kServerWriteback) != 0;
     uint32_t *mbst = lds + L.state_off;
     uint32_t *mbwave = lds + L.wave_off;
     uint32_t *mbtg = lds + L.tg_off;
@@ -1308,7 +1311,10 @@ __device__ __forceinline__ void server_workgroup(const Ctx &c, const ServerLayou
             for (uint32_t q = 1; q < 3; ++q) h[q * L.hsz + i] = 0;
         }
     __syncthreads();
-    if (wave == 0) committer<R, W, D>(c, L, lds);
+    if (wave == 0) {
+        if (c.variant == (kValStamp | kInsertBatch)) committer<R, W, D, true>(c, L, lds);
+        else committer<R, W, D>(c, L, lds);
+    }
     else if (wave == 1) {
         if ((c.variant & kServerWriteback) == 0) advancer(c, lds);
     } else receiver<R, W, D>(c, L, lds, wave - 2, M);
