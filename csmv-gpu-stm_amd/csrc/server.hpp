@@ -380,6 +380,10 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
             for (int r = 0; r < RO; ++r) bkr[r] = box[(RW::bkt + r) * kWave + lane];
 #pragma unroll
             for (int w = 0; w < W; ++w) bkw[w] = box[(RW::bkt + RO + w) * kWave + lane];
+            // one round for every staged row (the bucket rows would otherwise be sunk into
+            // the hashed branch below and cost a round of their own)
+            pin_round(bkr);
+            pin_round(bkw);
             bool is_root[RO];
 #pragma unroll
             for (int r = 0; r < RO; ++r) is_root[r] = valid && ops[r] != kNone && root[r] == lane * RO + r;
