@@ -158,32 +158,14 @@ struct LaneSum {
 };
 
 // ---- committer --------------------------------------------------------------------------
-// Committed-value tables: the committer knows the newest committed value of an item
-// when it fits the exact LDS table, when the recent-value cache still holds it, or when
-// the value gathered at staging is still current (no write to its bucket since T_g).
+// Committed-value tables.  Exact mode (every item fits LDS): stamp and value of every
+// item in LDS.  Hashed mode: the exact tables are in HBM (c.stamps, c.vals, written only
+// by the committer) and LDS holds the stamp buckets; a value or stamp gathered at
+// staging (as of commit T_g) is current when no commit since then wrote its bucket, and
+// the HBM tables (which hold this wave's own earlier stores) give it otherwise.
 struct ValueTables {
     uint32_t *bucket, *lval;
 };
-
-// Newest committed value and stamp of item x as of the commits decided so far.  Exact
-// mode reads the LDS tables.  Hashed mode uses the values gathered at staging (pv, ps,
-// as of commit tg) when no commit since then wrote x's bucket, and otherwise the HBM
-// tables, which hold this wave's own recent stores.
-__device__ __forceinline__ void committed_of(const Ctx &c, const ServerLayout &L, const ValueTables &V,
-                                             uint32_t x, uint32_t tg, int32_t pv, uint32_t ps,
-                                             int32_t &val, uint32_t &ts)
-{
-    if (L.exact) {
-        val = (int32_t)V.lval[x];
-        ts = V.bucket[x];
-    } else if (V.bucket[bucket_of(L, x)] <= tg) {
-        val = pv;
-        ts = ps;
-    } else {
-        val = __hip_atomic_load(&c.vals[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        ts = __hip_atomic_load(&c.stamps[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    }
-}
 
 template <int R, int W>
 __device__ __forceinline__ void record_values(const Ctx &c, const ServerLayout &L, const ValueTables &V,

@@ -124,7 +124,7 @@ def test_serialized_lane_exact(frac):
 
 
 def test_serialized_lane_hot_zipf_hashed():
-    """1M accounts (hashed stamps, recent-value cache): transfers touching the 256
+    """1M accounts (hashed stamps, HBM value table): transfers touching the 256
     hottest accounts go to the serialized lane."""
     def route(op):
         return None
@@ -158,8 +158,8 @@ def test_serialized_lane_read_only(n_items, pct_ro):
 
 
 def test_serialized_lane_small_cache_unknowns():
-    """A tiny LDS budget forces a small recent-value cache: operands whose value is not at
-    hand make the server refuse (the client resends); results must not change."""
+    """A tiny LDS budget leaves few stamp buckets, so more operands are flagged and resolved
+    from the HBM value table: results must not change."""
     n = 1 << 16
     bank = csmv.Bank(n_txns=1 << 15, seed=47, n_reads=4, n_reads_ro=4, zipf=True)
     print("\n[case] lds_kib=64 serialized lane", flush=True)
