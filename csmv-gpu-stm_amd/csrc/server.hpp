@@ -174,8 +174,10 @@ __device__ __forceinline__ void record_values(const Ctx &c, const ServerLayout &
 #pragma unroll
     for (int w = 0; w < W; ++w)
         if (on[w] && wa[w] != kNone) {
+            // (the HBM store is an atomic one so that the two branches cannot be merged
+            // into one flat store, whose completion later LDS waits would also wait for)
             if (L.exact) V.lval[wa[w]] = (uint32_t)v[w];
-            else c.vals[wa[w]] = v[w];
+            else __hip_atomic_store(&c.vals[wa[w]], v[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         }
 }
 
@@ -201,27 +203,27 @@ template <int R, int W>
 __device__ __forceinline__ void transfer_exec(const uint32_t *it, uint32_t amt, const int32_t *in,
                                               int32_t *rd, const uint32_t *wa, int32_t *nv, bool upd)
 {
-    if (!upd) {
-#pragma unroll
-        for (int j = 0; j < R; ++j) rd[j] = it[j] == kNone ? 0 : in[j];
-#pragma unroll
-        for (int w = 0; w < W; ++w) nv[w] = 0;
-        return;
-    }
+    // written with selects only (no per-lane branches): a read-only lane never records
+    // an own write, so its reads are in[j] unchanged and it writes nothing (wa = kNone)
     uint32_t wi[2] = {kNone, kNone};
     int32_t wvv[2] = {0, 0};
 #pragma unroll
     for (int j = 0; j < R; ++j) {
-        if (it[j] == kNone) { rd[j] = 0; continue; }
         int32_t v = in[j];
         v = (wi[0] == it[j]) ? wvv[0] : v;
         v = (wi[1] == it[j]) ? wvv[1] : v;
-        rd[j] = v;
-        if (j == 0) { wi[0] = it[0]; wvv[0] = (int32_t)((uint32_t)v - amt); }
+        rd[j] = it[j] == kNone ? 0 : v;
+        const bool w_on = upd && it[j] != kNone;
+        if (j == 0) {
+            wi[0] = w_on ? it[0] : kNone;
+            wvv[0] = (int32_t)((uint32_t)v - amt);
+        }
         if (j == 1) {
             const int32_t nvb = (int32_t)((uint32_t)v + amt);
-            if (wi[0] == it[1]) wvv[0] = nvb;
-            else { wi[1] = it[1]; wvv[1] = nvb; }
+            const bool same = w_on && wi[0] == it[1];
+            wvv[0] = same ? nvb : wvv[0];
+            wi[1] = (w_on && !same) ? it[1] : kNone;
+            wvv[1] = nvb;
         }
     }
 #pragma unroll
@@ -491,18 +493,17 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
                         __hip_atomic_fetch_max(&c.stamps[wa[w]], myts, __ATOMIC_RELAXED,
                                                __HIP_MEMORY_SCOPE_WORKGROUP);
                 }
-            if (okx && !ro) {
-                c.log_ts[k] = myts;
-#pragma unroll
-                for (int r = 0; r < RO; ++r)
-                    if (ops[r] != kNone) c.log_v[(uint64_t)r * c.n_txns + k] = rd[r];
-            } else if (okx) {
-                // an audit's log: its snapshot and the sum of its reads (exe.cu:213-223)
+            if (okx) {
+                // a transfer logs its timestamp and every read; an audit its snapshot and
+                // the sum of its reads (exe.cu:213-223), in row 0
                 uint32_t sum = 0;
 #pragma unroll
                 for (int r = 0; r < RO; ++r) sum += (uint32_t)rd[r];
-                c.log_ts[k] = below;
-                c.log_v[k] = (int32_t)sum;
+                c.log_ts[k] = ro ? below : myts;
+                c.log_v[k] = ro ? (int32_t)sum : rd[0];
+#pragma unroll
+                for (int r = 1; r < RO; ++r)
+                    if (!ro && ops[r] != kNone) c.log_v[(uint64_t)r * c.n_txns + k] = rd[r];
             }
             uint64_t *rv = c.reply_v + ((size_t)cw * kWave + lane) * 2 * W;
 #pragma unroll
