@@ -109,23 +109,25 @@ template <int R, int W> __device__ __forceinline__ void memcd_client(const Ctx &
             const uint32_t lo = (uint32_t)key, hi = (uint32_t)(key >> 32);
             const uint32_t set = mc_set_of(key, c.n_sets);
             const uint32_t clock = (uint32_t)(k / c.mc_batch + 1);
-            if (op == kMcGet) {
-                // states and keys of every way at the snapshot in two batches of loads,
-                // then the hit slot's value and timestamp; the states are logged up to the
-                // hit way (the ones a sequential probe reads)
-                uint32_t ia[16], ib[8];
-                int32_t va[16], vb[8];
+            // states, keys and timestamps of every way at the snapshot, in two batches of
+            // loads shared by GET and SET lanes (one pass for a wave that holds both)
+            uint32_t ia[16], ib[16];
+            int32_t va[16], vb[16];
 #pragma unroll
-                for (int w = 0; w < 8; ++w) {
-                    const bool on = (uint32_t)w < ways;
-                    const uint32_t slot = set * ways + (uint32_t)w;
-                    ia[w] = on ? kMcState * ns + slot : kNone;
-                    ia[8 + w] = on ? kMcKlo * ns + slot : kNone;
-                    ib[w] = on ? kMcKhi * ns + slot : kNone;
-                }
-                bool ok = pool_read_chunk<16>(c, ia, tx.snap, va);
-                ok = pool_read_chunk<8>(c, ib, tx.snap, vb) && ok;
-                if (!ok) tx.aborted = true;
+            for (int w = 0; w < 8; ++w) {
+                const bool on = (uint32_t)w < ways;
+                const uint32_t slot = set * ways + (uint32_t)w;
+                ia[w] = on ? kMcState * ns + slot : kNone;
+                ia[8 + w] = on ? kMcKlo * ns + slot : kNone;
+                ib[w] = on ? kMcKhi * ns + slot : kNone;
+                ib[8 + w] = on ? kMcTs * ns + slot : kNone;
+            }
+            bool ok = pool_read_chunk<16>(c, ia, tx.snap, va);
+            ok = pool_read_chunk<16>(c, ib, tx.snap, vb) && ok;
+            if (!ok) tx.aborted = true;
+            if (op == kMcGet) {
+                // the hit way, then its value; the states are logged up to the hit way
+                // (the ones a sequential probe reads) and the hit slot's timestamp
                 int hit = -1;
 #pragma unroll
                 for (int w = 7; w >= 0; --w)
@@ -137,17 +139,18 @@ template <int R, int W> __device__ __forceinline__ void memcd_client(const Ctx &
                     if ((uint32_t)w < ways && (hit < 0 || w <= hit)) log_read(tx, ia[w]);
                 if (hit >= 0 && !tx.aborted) {
                     const uint32_t slot = set * ways + (uint32_t)hit;
-                    uint32_t iv[9];
-                    int32_t vv[9];
+                    uint32_t iv[8];
+                    int32_t vv[8];
 #pragma unroll
                     for (int j = 0; j < 8; ++j) iv[j] = (kMcV0 + j) * ns + slot;
-                    iv[8] = kMcTs * ns + slot;
-                    if (!pool_read_chunk<9>(c, iv, tx.snap, vv)) tx.aborted = true;
+                    if (!pool_read_chunk<8>(c, iv, tx.snap, vv)) tx.aborted = true;
                     uint32_t acc = 0;
 #pragma unroll
                     for (int j = 0; j < 8; ++j) acc += (uint32_t)vv[j] * (2u * j + 1u);
-                    const uint32_t t = (uint32_t)vv[8];
-                    log_read(tx, iv[8]);
+                    uint32_t t = 0;
+#pragma unroll
+                    for (int w = 0; w < 8; ++w) t = (w == hit) ? (uint32_t)vb[8 + w] : t;
+                    log_read(tx, kMcTs * ns + slot);
                     res = 1u | ((uint32_t)hit << 4);
                     vsum = acc;
                     if (t < clock && !tx.aborted) {
@@ -159,24 +162,10 @@ template <int R, int W> __device__ __forceinline__ void memcd_client(const Ctx &
                     }
                 }
             } else {
-                // states, keys and timestamps of every way in two batches of loads (states
-                // and keys low; keys high and timestamps); states and timestamps are logged
+                // SET: the way holding the key, else the first invalid way, else the least
+                // recently used; states and timestamps are logged
                 int found = -1, invalid = -1, lru = 0;
                 uint32_t lru_ts = kNone;
-                uint32_t ia[16], ib[16];
-                int32_t va[16], vb[16];
-#pragma unroll
-                for (int w = 0; w < 8; ++w) {
-                    const bool on = (uint32_t)w < ways;
-                    const uint32_t slot = set * ways + (uint32_t)w;
-                    ia[w] = on ? kMcState * ns + slot : kNone;
-                    ia[8 + w] = on ? kMcKlo * ns + slot : kNone;
-                    ib[w] = on ? kMcKhi * ns + slot : kNone;
-                    ib[8 + w] = on ? kMcTs * ns + slot : kNone;
-                }
-                bool ok = pool_read_chunk<16>(c, ia, tx.snap, va);
-                ok = pool_read_chunk<16>(c, ib, tx.snap, vb) && ok;
-                if (!ok) tx.aborted = true;
 #pragma unroll
                 for (int w = 0; w < 8; ++w) {
                     if ((uint32_t)w >= ways) continue;
