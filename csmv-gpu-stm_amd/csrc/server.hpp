@@ -1144,17 +1144,14 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
                                                          pend[w] ? it[R + w] : kNone, __ATOMIC_RELAXED,
                                                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                 }
+                // (selects only: per-lane branches cost more than the step itself)
                 bool more = false;
     #pragma unroll
                 for (int w = 0; w < W; ++w) {
-                    if (pend[w]) {
-                        if (old[w] == kNone || old[w] == it[R + w]) {
-                            slot_w[w] = hp[w];
-                            pend[w] = false;
-                        } else {
-                            hp[w] = (hp[w] + 1) & hmask;
-                        }
-                    }
+                    const bool got = pend[w] && (old[w] == kNone || old[w] == it[R + w]);
+                    slot_w[w] = got ? hp[w] : slot_w[w];
+                    hp[w] = (pend[w] && !got) ? ((hp[w] + 1) & hmask) : hp[w];
+                    pend[w] = pend[w] && !got;
                     more |= pend[w];
                 }
                 if (ballot(more) == 0) break;
@@ -1181,12 +1178,12 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
                 bool more = false;
     #pragma unroll
                 for (int r = 0; r < R; ++r) {
-                    if (fp[r]) {
-                        if (k[r] == it[r]) { fs[r] = hs[r]; fp[r] = false; }
-                        else if (k[r] == kNone) fp[r] = false;
-                        else hs[r] = (hs[r] + 1) & hmask;
-                    }
-                    more |= fp[r];
+                    const bool hit = fp[r] && k[r] == it[r];
+                    const bool next = fp[r] && k[r] != it[r] && k[r] != kNone;
+                    fs[r] = hit ? hs[r] : fs[r];
+                    hs[r] = next ? ((hs[r] + 1) & hmask) : hs[r];
+                    fp[r] = next;
+                    more |= next;
                 }
                 if (ballot(more) == 0) break;
             }
