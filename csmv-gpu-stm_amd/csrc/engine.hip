@@ -313,12 +313,16 @@ template <int R, int W> __device__ __forceinline__ void bank_serial_client(const
         const uint32_t amt = (op >> 8) & 0xFFu, nr = (op >> 16) & 0xFFu;
         const bool audit = (op & 0xFFu) == kKindAudit;
         if (has) {
+            // the operand rows in one batch of loads (a row past nr reloads row 0)
+            uint32_t ops[R];
+#pragma unroll
+            for (int j = 0; j < R; ++j) ops[j] = c.addr[(uint64_t)((uint32_t)j < nr ? j : 0) * c.n_txns + k];
+            pin_round(ops);
 #pragma unroll
             for (int j = 0; j < R; ++j)
                 if ((uint32_t)j < nr) {
-                    const uint32_t it = c.addr[(uint64_t)j * c.n_txns + k];
-                    tx.rs[j] = it;
-                    if (j < 2 && !audit) tx_write(tx, it, 0);
+                    tx.rs[j] = ops[j];
+                    if (j < 2 && !audit) tx_write(tx, ops[j], 0);
                 }
             tx.nr = nr;
             tx.wv[0] = (int32_t)(uint32_t)k;
