@@ -591,7 +591,8 @@ ServerLayout server_layout_at(const csmv_engine *e, bool delegation, uint32_t nm
     ServerLayout L{};
     const uint32_t vmode = e->cfg.variant & 3u;
     const uint32_t M = e->cfg.wg_waves - 2;
-    L.nmb = nmb ? nmb : std::min<uint32_t>(std::max<uint32_t>(M + 2, 4), 16);
+    // enough mailboxes for the receivers and for the staging window
+    L.nmb = nmb ? nmb : std::min<uint32_t>(std::max<uint32_t>({M + 2, e->cfg.stage_ahead + 2, 4u}), 16);
     L.mb_rows = delegation ? Rows<R, W>::count : Rows<R, W>::pref + 1;
     L.state_off = kLdsHdr;
     L.wave_off = L.state_off + L.nmb;
