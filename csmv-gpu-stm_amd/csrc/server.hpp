@@ -115,6 +115,15 @@ __device__ __forceinline__ void wg_fence() { __builtin_amdgcn_fence(__ATOMIC_ACQ
 // order, so this only constrains the compiler).
 __device__ __forceinline__ void wave_fence() { __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront"); }
 
+// The committer's phase timers (read / resolve / chain / finish) cost a clock read each on
+// the commit path, so they are compiled in only on request (-DCSMV_PHASE_TIMERS=1); the
+// busy and idle sums are always kept.
+#ifndef CSMV_PHASE_TIMERS
+#define CSMV_PHASE_TIMERS 0
+#endif
+constexpr bool kPhaseTimers = CSMV_PHASE_TIMERS != 0;
+__device__ __forceinline__ uint64_t phase_clock() { return kPhaseTimers ? wall_clock64() : 0ull; }
+
 __device__ __forceinline__ uint32_t wave_min(uint32_t v)
 {
 #pragma unroll
@@ -268,74 +277,155 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
     // values they would take 20 scalar registers for the whole loop and push the loop's
     // other uniform values out to spill slots
     LaneSum n_rw, n_pv, n_rec, n_unk, t_idle, t_busy, t_read, t_res, t_chain, t_fin;
-    uint32_t n = 0;
-    for (;; ++n) {
-        const uint32_t mbi = n % L.nmb;
+    uint32_t n = 0, mbi = 0, adv_seen = 0;
+    uint64_t t_prev = wall_clock64();   // end of the previous request
+    for (;; ++n, mbi = mbi + 1 == L.nmb ? 0u : mbi + 1) {
         uint32_t s = 0;
         bool bad = false;
-        const uint64_t t0 = wall_clock64();
+        const uint32_t *box = lds + L.mb_off + mbi * L.mb_rows * kWave;
+        // staged rows (a wide wire keeps its read rows in LDS and reads them on use)
+        constexpr int kDr = D && !kWide ? (R + 3) / 4 + RO + W + RO + W + 2 * RO + 1 : 0;
+        constexpr int kNr = kWide ? 3 + 2 * W + 3 : 3 + R + 2 * W + 3 + kDr;
+        uint32_t row[kNr];
         {
             Spin sp;
             for (;;) {
-                s = lds_ld_acq(&mbst[mbi]);
+                // The state word, then every staged row, as one batch of LDS reads: the DS
+                // instructions of a wave execute in order and the receiver writes the rows
+                // before it releases the state, so rows read after a state that shows
+                // request n are request n's (a batch read too early is read again).
+                s = __hip_atomic_load(&mbst[mbi], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                wave_fence();
+                int q = 0;
+                row[q++] = mbwave[mbi];
+                row[q++] = mbtg[mbi];
+                row[q++] = box[RW::snap * kWave + lane];
+                if constexpr (!kWide) {
+#pragma unroll
+                    for (int r = 0; r < R; ++r) row[q++] = box[(RW::items + r) * kWave + lane];
+                }
+#pragma unroll
+                for (int w = 0; w < W; ++w) row[q++] = box[(RW::items + R + w) * kWave + lane];
+#pragma unroll
+                for (int w = 0; w < W; ++w) row[q++] = box[(RW::wv + w) * kWave + lane];
+                row[q++] = box[RW::conf_hi * kWave + lane];
+                row[q++] = box[RW::conf_lo * kWave + lane];
+                row[q++] = box[RW::pref * kWave + lane];
+                if constexpr (kDr > 0) {
+#pragma unroll
+                    for (int j = 0; j < (R + 3) / 4; ++j) row[q++] = box[(RW::prod + j) * kWave + lane];
+#pragma unroll
+                    for (int r = 0; r < RO; ++r) row[q++] = box[(RW::pval + r) * kWave + lane];
+#pragma unroll
+                    for (int w = 0; w < W; ++w) row[q++] = box[(RW::pstamp + w) * kWave + lane];
+#pragma unroll
+                    for (int j = 0; j < RO + W; ++j) row[q++] = box[(RW::bkt + j) * kWave + lane];
+#pragma unroll
+                    for (int r = 0; r < RO; ++r) row[q++] = box[(RW::root + r) * kWave + lane];
+#pragma unroll
+                    for (int r = 0; r < RO; ++r) row[q++] = box[(RW::wgt + r) * kWave + lane];
+                    row[q++] = box[RW::supf * kWave + lane];
+                }
+                pin_round(row);
                 if ((s & ~kExitBit) == n + 1) break;
                 if (!sp.tick(c, kErrMailboxTimeout)) { bad = true; break; }
             }
         }
         if (bad || (s & kExitBit)) break;
-        if (!server_wb && n >= Q) {   // the completion slot n mod Q must be consumed
+        // the completion slot n mod Q must be consumed (the advancer's position is read
+        // again only when the last value seen does not cover it)
+        if (!server_wb && n >= Q && adv_seen < n - Q + 1) {
             Spin sp;
-            while (lds_ld_acq(&lds[kLdsAdv]) < n - Q + 1)
+            while ((adv_seen = lds_ld_acq(&lds[kLdsAdv])) < n - Q + 1)
                 if (!sp.tick(c, kErrDoneRingTimeout)) { bad = true; break; }
             if (bad) break;
         }
         const uint64_t t1 = wall_clock64();
-        const uint32_t cwr = mbwave[mbi];
+        int q = 0;
+        const uint32_t cwr = row[q++];
         const uint32_t cw = cwr & ~kDelegateBit;
         const bool delegate = D && (cwr & kDelegateBit) != 0;
-        const uint32_t tg = mbtg[mbi];
-        const uint32_t *box = lds + L.mb_off + mbi * L.mb_rows * kWave;
-        const uint32_t snap = box[RW::snap * kWave + lane];
-        uint32_t rs[kWide ? 1 : R], wa[W], prod[kWide ? 1 : (R + 3) / 4], pst[W];
-        int32_t wv[W], pval[RO];
+        const uint32_t tg = row[q++];
+        const uint32_t snap = row[q++];
+        uint32_t rs[kWide ? 1 : R], wa[W], prod[kWide ? 1 : (R + 3) / 4], pst[W], bkr[RO], bkw[W];
+        uint32_t root[RO], supf = 0;
+        int32_t wv[W], pval[RO], wgt[RO];
         if constexpr (!kWide) {
 #pragma unroll
-            for (int r = 0; r < R; ++r) rs[r] = box[(RW::items + r) * kWave + lane];
+            for (int r = 0; r < R; ++r) rs[r] = row[q++];
         }
         auto RS = [&](int r) -> uint32_t {
             if constexpr (kWide) return box[(RW::items + r) * kWave + lane];
             else return rs[r];
         };
 #pragma unroll
-        for (int w = 0; w < W; ++w) {
-            wa[w] = box[(RW::items + R + w) * kWave + lane];
-            wv[w] = (int32_t)box[(RW::wv + w) * kWave + lane];
+        for (int w = 0; w < W; ++w) wa[w] = row[q++];
+#pragma unroll
+        for (int w = 0; w < W; ++w) wv[w] = (int32_t)row[q++];
+        const uint32_t chi = row[q++];
+        const uint64_t conf = ((uint64_t)chi << 32) | row[q++];
+        const bool pref = row[q++] != 0;
+        if constexpr (kDr > 0) {
+#pragma unroll
+            for (int j = 0; j < (R + 3) / 4; ++j) prod[j] = row[q++];
+#pragma unroll
+            for (int r = 0; r < RO; ++r) pval[r] = (int32_t)row[q++];
+#pragma unroll
+            for (int w = 0; w < W; ++w) pst[w] = row[q++];
+#pragma unroll
+            for (int r = 0; r < RO; ++r) bkr[r] = row[q++];
+#pragma unroll
+            for (int w = 0; w < W; ++w) bkw[w] = row[q++];
+#pragma unroll
+            for (int r = 0; r < RO; ++r) root[r] = row[q++];
+#pragma unroll
+            for (int r = 0; r < RO; ++r) wgt[r] = (int32_t)row[q++];
+            supf = row[q++];
+        } else if (delegate) {
+            // wide wire: the delegate rows of the chain nodes and writes
+#pragma unroll
+            for (int r = 0; r < RO; ++r) {
+                pval[r] = (int32_t)box[(RW::pval + r) * kWave + lane];
+                bkr[r] = box[(RW::bkt + r) * kWave + lane];
+                root[r] = box[(RW::root + r) * kWave + lane];
+                wgt[r] = (int32_t)box[(RW::wgt + r) * kWave + lane];
+            }
+            supf = box[RW::supf * kWave + lane];
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                pst[w] = box[(RW::pstamp + w) * kWave + lane];
+                bkw[w] = box[(RW::bkt + RO + w) * kWave + lane];
+            }
         }
-        const uint64_t conf = ((uint64_t)box[RW::conf_hi * kWave + lane] << 32) |
-                              box[RW::conf_lo * kWave + lane];
-        const bool pref = box[RW::pref * kWave + lane] != 0;
         // producer code of read r: lane | write slot << 6 (0xFF: no lower writer)
         auto PROD = [&](int r) -> uint32_t {
             const uint32_t word = kWide ? box[(RW::prod + r / 4) * kWave + lane] : prod[kWide ? 0 : r / 4];
             return (word >> (8 * (r % 4))) & 0xFFu;
         };
-        if (delegate) {
-            if constexpr (!kWide) {
+        // hashed mode: the committed values of the chain nodes and the stamps of the
+        // writes from the HBM tables (which hold this wave's own earlier stores), issued
+        // here so that their round trip overlaps the staged-row reads below
+        int32_t hv[RO];
+        uint32_t hs[W];
+        if (delegate && !L.exact) {
 #pragma unroll
-                for (int q = 0; q < (R + 3) / 4; ++q) prod[q] = box[(RW::prod + q) * kWave + lane];
+            for (int r = 0; r < RO; ++r) {
+                const uint32_t x = RS(r);
+                hv[r] = __hip_atomic_load(&c.vals[x != kNone ? x : 0u], __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_WORKGROUP);
             }
 #pragma unroll
-            for (int r = 0; r < RO; ++r) pval[r] = (int32_t)box[(RW::pval + r) * kWave + lane];
-#pragma unroll
-            for (int w = 0; w < W; ++w) pst[w] = box[(RW::pstamp + w) * kWave + lane];
+            for (int w = 0; w < W; ++w)
+                hs[w] = __hip_atomic_load(&c.stamps[wa[w] != kNone ? wa[w] : 0u], __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_WORKGROUP);
         }
-        // mailbox consumed; every stamp <= last is visible in HBM once this release
-        // has waited for the previous batch's stores
+        // mailbox consumed; every stamp <= last is in the HBM tables for the waves of this
+        // workgroup that acquire this release
         if (lane == 0) {
             lds_st_rel(&lds[kLdsStampTs], last);
             lds_st_rel(&lds[kLdsCnext], n + 1);
         }
-        const uint64_t tr = wall_clock64();
+        const uint64_t tr = phase_clock();
         uint64_t tres = tr, tchain = tr;
         const bool valid = snap != kNone;
         uint32_t myts = 0;
@@ -350,24 +440,9 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
             // value gathered at staging when no commit since then wrote the item's bucket,
             // or the HBM table this wave writes).  Chain nodes are the first RO reads
             // (every read of a narrow wire, the transfer operands of a wide one).
-            uint32_t ops[RO], root[RO];
-            int32_t wgt[RO];
+            uint32_t ops[RO];
 #pragma unroll
-            for (int r = 0; r < RO; ++r) {
-                ops[r] = RS(r);
-                root[r] = box[(RW::root + r) * kWave + lane];
-                wgt[r] = (int32_t)box[(RW::wgt + r) * kWave + lane];
-            }
-            const uint32_t supf = box[RW::supf * kWave + lane];
-            uint32_t bkr[RO], bkw[W];
-#pragma unroll
-            for (int r = 0; r < RO; ++r) bkr[r] = box[(RW::bkt + r) * kWave + lane];
-#pragma unroll
-            for (int w = 0; w < W; ++w) bkw[w] = box[(RW::bkt + RO + w) * kWave + lane];
-            // one round for every staged row (the bucket rows would otherwise be sunk into
-            // the hashed branch below and cost a round of their own)
-            pin_round(bkr);
-            pin_round(bkw);
+            for (int r = 0; r < RO; ++r) ops[r] = RS(r);
             bool is_root[RO];
 #pragma unroll
             for (int r = 0; r < RO; ++r) is_root[r] = valid && ops[r] != kNone && root[r] == lane * RO + r;
@@ -389,29 +464,20 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
                 for (int w = 0; w < W; ++w) prev[w] = (valid && wa[w] != kNone) ? ls[w] : kNone;
             } else {
                 // the committed value and stamp: the ones gathered at staging when no
-                // commit since then wrote the item's bucket, else the HBM tables (which
-                // hold this wave's own earlier stores); both are loaded at once
-                uint32_t bv[RO], bw[W], hs[W];
-                int32_t hv[RO];
+                // commit since then wrote the item's bucket, else the HBM tables (loaded
+                // when the request was read)
+                uint32_t bv[RO], bw[W];
 #pragma unroll
                 for (int r = 0; r < RO; ++r) bv[r] = V.bucket[bkr[r]];
 #pragma unroll
                 for (int w = 0; w < W; ++w) bw[w] = V.bucket[bkw[w]];
-#pragma unroll
-                for (int r = 0; r < RO; ++r)
-                    hv[r] = __hip_atomic_load(&c.vals[ops[r] != kNone ? ops[r] : 0u], __ATOMIC_RELAXED,
-                                              __HIP_MEMORY_SCOPE_WORKGROUP);
-#pragma unroll
-                for (int w = 0; w < W; ++w)
-                    hs[w] = __hip_atomic_load(&c.stamps[wa[w] != kNone ? wa[w] : 0u], __ATOMIC_RELAXED,
-                                              __HIP_MEMORY_SCOPE_WORKGROUP);
 #pragma unroll
                 for (int r = 0; r < RO; ++r) base[r] = is_root[r] ? (bv[r] <= tg ? pval[r] : hv[r]) : 0;
 #pragma unroll
                 for (int w = 0; w < W; ++w)
                     prev[w] = (valid && wa[w] != kNone) ? (bw[w] <= tg ? pst[w] : hs[w]) : kNone;
             }
-            tres = wall_clock64();
+            tres = phase_clock();
 #pragma unroll
             for (int r = 0; r < RO; ++r) jbase[lane * RO + r] = base[r];
             wave_fence();
@@ -419,7 +485,7 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
 #pragma unroll
             for (int r = 0; r < RO; ++r) in[r] = jbase[root[r]] + wgt[r];
             const bool okx = valid;
-            tchain = wall_clock64();
+            tchain = phase_clock();
             // read-only lanes take no timestamp: each reads the state after the commits
             // of the lanes below it and replies with that snapshot
             const bool ro = amt == kReadOnlyAmt;
@@ -698,15 +764,18 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
         if (!server_wb && nc == 0 && lane == 0)
             st_rlx(&c.done[n & c.qmask], ((uint64_t)(n + 1) << 32) | last);
         const uint64_t t2 = wall_clock64();
-        t_idle.add(t1 - t0);
+        t_idle.add(t1 - t_prev);
         t_busy.add(t2 - t1);
-        t_read.add(tr - t1);
-        if (delegate) {
-            t_res.add(tres - tr);
-            t_chain.add(tchain - tres);
-            t_fin.add(t2 - tchain);
-        } else {
-            t_res.add(t2 - tr);
+        t_prev = t2;
+        if constexpr (kPhaseTimers) {
+            t_read.add(tr - t1);
+            if (delegate) {
+                t_res.add(tres - tr);
+                t_chain.add(tchain - tres);
+                t_fin.add(t2 - tchain);
+            } else {
+                t_res.add(t2 - tr);
+            }
         }
     }
     if (lane == 0) {
@@ -1295,13 +1364,17 @@ __device__ __forceinline__ void server_workgroup(const Ctx &c, const ServerLayou
             for (uint32_t q = 1; q < 3; ++q) h[q * L.hsz + i] = 0;
         }
     __syncthreads();
+    // Waves are spread over the CU's four SIMDs in turn, so wave 4 shares the committer's
+    // SIMD: it runs the advancer (mostly asleep in its polls) rather than a receiver,
+    // whose hash and chain work would take issue slots from the committer.
+    const uint32_t adv = c.wg_waves >= 5 ? 4u : 1u;
     if (wave == 0) {
         if (c.variant == (kValStamp | kInsertBatch)) committer<R, W, D, true>(c, L, lds);
         else committer<R, W, D>(c, L, lds);
     }
-    else if (wave == 1) {
+    else if (wave == adv) {
         if ((c.variant & kServerWriteback) == 0) advancer(c, lds);
-    } else receiver<R, W, D>(c, L, lds, wave - 2, M);
+    } else receiver<R, W, D>(c, L, lds, wave - 1 - (wave > adv ? 1u : 0u), M);
 }
 
 }  // namespace csmv

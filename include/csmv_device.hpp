@@ -229,13 +229,16 @@ enum : uint32_t {
 };
 
 // Bounded wait: every spin loop in the engine ends, either by its condition or by the
-// device timeout, which records an error code that makes all other roles leave too.
+// device timeout, which records an error code that makes all other roles leave too.  The
+// clock is first read on the first tick, so a wait whose condition already holds costs no
+// clock read.
 struct Spin {
     uint64_t t0;
     uint32_t it;
-    __device__ Spin() : t0(wall_clock64()), it(0) {}
+    __device__ Spin() : t0(0), it(0) {}
     __device__ bool tick(const Ctx &c, uint32_t code)
     {
+        if (it == 0) t0 = wall_clock64();
         __builtin_amdgcn_s_sleep(1);
         if ((++it & 31u) == 0) {
             if (ld_rlx(c.err)) return false;
@@ -467,33 +470,37 @@ __device__ __forceinline__ uint32_t tx_commit(const Ctx &c, uint32_t cw, Tx<R, W
     n = bcast64(n);
 
     uint64_t v;
+    const uint64_t tw0 = wall_clock64();
     Spin sp;
+    // serialized lane: the reply word and, for each write, the new value and the stamp
+    // its install must follow (kNone: the write was superseded inside the request and is
+    // not installed), all 2W + 1 words in one batch of loads per poll; every word carries
+    // the request's tag, so a word read ahead of its update is seen as stale
+    uint64_t xy[2 * W + 1];
     for (;;) {
-        v = ld_rlx(&c.reply[(size_t)cw * kWave + lane]);
-        if (ballot((v >> 32) != n + 1) == 0) break;
+        if (delegate) {
+            const uint64_t *rv = c.reply_v + ((size_t)cw * kWave + lane) * 2 * W;
+#pragma unroll
+            for (int q = 0; q < 2 * W; ++q) xy[q] = ld_rlx(&rv[q]);
+        }
+        xy[2 * W] = ld_rlx(&c.reply[(size_t)cw * kWave + lane]);
+        pin_round(xy);
+        v = xy[2 * W];
+        bool stale = (v >> 32) != n + 1;
+        if (delegate) {
+            const bool live = req && (uint32_t)v != 0u;
+#pragma unroll
+            for (int w = 0; w < W; ++w)
+                stale |= live && (uint32_t)w < tx.nw &&
+                         ((xy[w] >> 32) != n + 1 || (xy[W + w] >> 32) != n + 1);
+        }
+        if (ballot(stale) == 0) break;
         if (!sp.tick(c, kErrReplyTimeout)) { *fail = true; return 0; }
     }
     const uint32_t ts = req ? (uint32_t)v : 0u;
     if (delegate && ballot(ts != 0)) {
-        // new values, and for each write the stamp its install must follow (kNone: the
-        // write was superseded inside the request and is not installed)
-        const uint64_t *rv = c.reply_v + ((size_t)cw * kWave + lane) * 2 * W;
         uint32_t prev[W];
         bool pend[W];
-        // all 2W words of the reply in one batch of loads per poll
-        uint64_t xy[2 * W];
-        for (;;) {
-#pragma unroll
-            for (int q = 0; q < 2 * W; ++q) xy[q] = ld_rlx(&rv[q]);
-            pin_round(xy);
-            bool stale = false;
-#pragma unroll
-            for (int w = 0; w < W; ++w)
-                stale |= ts != 0 && (uint32_t)w < tx.nw &&
-                         ((xy[w] >> 32) != n + 1 || (xy[W + w] >> 32) != n + 1);
-            if (ballot(stale) == 0) break;
-            if (!sp.tick(c, kErrReplyTimeout)) { *fail = true; return 0; }
-        }
 #pragma unroll
         for (int w = 0; w < W; ++w) {
             tx.wv[w] = (int32_t)(uint32_t)xy[w];
@@ -554,12 +561,17 @@ __device__ __forceinline__ uint32_t tx_commit(const Ctx &c, uint32_t cw, Tx<R, W
             if (!sp.tick(c, kErrReplyTimeout)) { *fail = true; return 0; }
         }
     }
-    *wait_ticks += wall_clock64() - sp.t0;
+    *wait_ticks += wall_clock64() - tw0;
     if ((c.variant & kServerWriteback) == 0) {
         const uint64_t cm = ballot(ts != 0);
         if (cm) {
-            if (install_n >= 0) writeback<R, WI>(c, install_wa, install_wv, (uint32_t)install_n, ts);
-            else writeback<R, W>(c, tx.wa, tx.wv, tx.nw, ts);
+            // (the serialized lane has installed every write above)
+            if (delegate) {
+            } else if (install_n >= 0) {
+                writeback<R, WI>(c, install_wa, install_wv, (uint32_t)install_n, ts);
+            } else {
+                writeback<R, W>(c, tx.wa, tx.wv, tx.nw, ts);
+            }
             // a read-only lane of the serialized lane replies with its snapshot, which is
             // the request's last timestamp when it is the highest committed lane
             const uint32_t last_lane = 63u - (uint32_t)__builtin_clzll(cm);
