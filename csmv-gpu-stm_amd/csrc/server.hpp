@@ -70,9 +70,8 @@ struct ServerLayout {
 };
 
 // Row indices inside an LDS mailbox.  For a delegated request the wv rows carry the
-// transaction id and amount, prod packs each read's producer (the lower lane whose
-// write it must see: lane | slot << 6, 0xFF = none) and pval holds committed values
-// gathered at staging.
+// transaction id and amount, and prod packs each read's producer (the lower lane whose
+// write it must see: lane | slot << 6, 0xFF = none).
 // A wide wire (read sets beyond 16) keeps its read set in the staged rows instead of
 // registers, validates with stamps only, and on the serialized lane runs the producer
 // chains over the transfer operands (the first W reads) alone: the other reads take the
@@ -90,16 +89,14 @@ template <int R, int W> struct Rows {
     static constexpr uint32_t conf_hi = 2 + R + 2 * W;
     static constexpr uint32_t pref = 3 + R + 2 * W;
     static constexpr uint32_t prod = 4 + R + 2 * W;   // (R + 3) / 4 rows of producer codes
-    static constexpr uint32_t pval = prod + (R + 3) / 4;  // RO rows: operand values at staging
-    static constexpr uint32_t pstamp = pval + RO;    // stamps of the writes at staging
     // producer chains resolved at staging: each chain node's root node and the sum of the
     // deltas between them, and the bit mask of this lane's superseded write slots
-    static constexpr uint32_t root = pstamp + W;
+    static constexpr uint32_t root = prod + (R + 3) / 4;
     static constexpr uint32_t wgt = root + RO;
     static constexpr uint32_t supf = wgt + RO;
-    // stamp-bucket index (exact mode: the item) of each chain node, then of each write
+    // stamp-bucket index (exact mode: the item) of each write
     static constexpr uint32_t bkt = supf + 1;
-    static constexpr uint32_t count = bkt + RO + W;
+    static constexpr uint32_t count = bkt + W;
 };
 
 __device__ __forceinline__ uint32_t lds_ld_acq(const uint32_t *p)
@@ -279,54 +276,55 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
     LaneSum n_rw, n_pv, n_rec, n_unk, t_idle, t_busy, t_read, t_res, t_chain, t_fin;
     uint32_t n = 0, mbi = 0, adv_seen = 0;
     uint64_t t_prev = wall_clock64();   // end of the previous request
+    // staged rows (a wide wire keeps its read rows in LDS and reads them on use)
+    constexpr int kDr = D && !kWide ? (R + 3) / 4 + W + 2 * RO + 1 : 0;
+    constexpr int kNr = kWide ? 3 + 2 * W + 3 : 3 + R + 2 * W + 3 + kDr;
+    uint32_t row[kNr];
+    // The state word of mailbox m, then every staged row, as one batch of LDS reads: the
+    // DS instructions of a wave execute in order and the receiver writes the rows before
+    // it releases the state, so rows read after a state that shows the request are that
+    // request's (a batch read too early is read again).
+    auto read_rows = [&](uint32_t m) -> uint32_t {
+        const uint32_t *bx = lds + L.mb_off + m * L.mb_rows * kWave;
+        const uint32_t st = __hip_atomic_load(&mbst[m], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        wave_fence();
+        int q = 0;
+        row[q++] = mbwave[m];
+        row[q++] = mbtg[m];
+        row[q++] = bx[RW::snap * kWave + lane];
+        if constexpr (!kWide) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) row[q++] = bx[(RW::items + r) * kWave + lane];
+        }
+#pragma unroll
+        for (int w = 0; w < W; ++w) row[q++] = bx[(RW::items + R + w) * kWave + lane];
+#pragma unroll
+        for (int w = 0; w < W; ++w) row[q++] = bx[(RW::wv + w) * kWave + lane];
+        row[q++] = bx[RW::conf_hi * kWave + lane];
+        row[q++] = bx[RW::conf_lo * kWave + lane];
+        row[q++] = bx[RW::pref * kWave + lane];
+        if constexpr (kDr > 0) {
+#pragma unroll
+            for (int j = 0; j < (R + 3) / 4; ++j) row[q++] = bx[(RW::prod + j) * kWave + lane];
+#pragma unroll
+            for (int w = 0; w < W; ++w) row[q++] = bx[(RW::bkt + w) * kWave + lane];
+#pragma unroll
+            for (int r = 0; r < RO; ++r) row[q++] = bx[(RW::root + r) * kWave + lane];
+#pragma unroll
+            for (int r = 0; r < RO; ++r) row[q++] = bx[(RW::wgt + r) * kWave + lane];
+            row[q++] = bx[RW::supf * kWave + lane];
+        }
+        pin_round(row);
+        return st;
+    };
     for (;; ++n, mbi = mbi + 1 == L.nmb ? 0u : mbi + 1) {
         uint32_t s = 0;
         bool bad = false;
         const uint32_t *box = lds + L.mb_off + mbi * L.mb_rows * kWave;
-        // staged rows (a wide wire keeps its read rows in LDS and reads them on use)
-        constexpr int kDr = D && !kWide ? (R + 3) / 4 + RO + W + RO + W + 2 * RO + 1 : 0;
-        constexpr int kNr = kWide ? 3 + 2 * W + 3 : 3 + R + 2 * W + 3 + kDr;
-        uint32_t row[kNr];
         {
             Spin sp;
             for (;;) {
-                // The state word, then every staged row, as one batch of LDS reads: the DS
-                // instructions of a wave execute in order and the receiver writes the rows
-                // before it releases the state, so rows read after a state that shows
-                // request n are request n's (a batch read too early is read again).
-                s = __hip_atomic_load(&mbst[mbi], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                wave_fence();
-                int q = 0;
-                row[q++] = mbwave[mbi];
-                row[q++] = mbtg[mbi];
-                row[q++] = box[RW::snap * kWave + lane];
-                if constexpr (!kWide) {
-#pragma unroll
-                    for (int r = 0; r < R; ++r) row[q++] = box[(RW::items + r) * kWave + lane];
-                }
-#pragma unroll
-                for (int w = 0; w < W; ++w) row[q++] = box[(RW::items + R + w) * kWave + lane];
-#pragma unroll
-                for (int w = 0; w < W; ++w) row[q++] = box[(RW::wv + w) * kWave + lane];
-                row[q++] = box[RW::conf_hi * kWave + lane];
-                row[q++] = box[RW::conf_lo * kWave + lane];
-                row[q++] = box[RW::pref * kWave + lane];
-                if constexpr (kDr > 0) {
-#pragma unroll
-                    for (int j = 0; j < (R + 3) / 4; ++j) row[q++] = box[(RW::prod + j) * kWave + lane];
-#pragma unroll
-                    for (int r = 0; r < RO; ++r) row[q++] = box[(RW::pval + r) * kWave + lane];
-#pragma unroll
-                    for (int w = 0; w < W; ++w) row[q++] = box[(RW::pstamp + w) * kWave + lane];
-#pragma unroll
-                    for (int j = 0; j < RO + W; ++j) row[q++] = box[(RW::bkt + j) * kWave + lane];
-#pragma unroll
-                    for (int r = 0; r < RO; ++r) row[q++] = box[(RW::root + r) * kWave + lane];
-#pragma unroll
-                    for (int r = 0; r < RO; ++r) row[q++] = box[(RW::wgt + r) * kWave + lane];
-                    row[q++] = box[RW::supf * kWave + lane];
-                }
-                pin_round(row);
+                s = read_rows(mbi);
                 if ((s & ~kExitBit) == n + 1) break;
                 if (!sp.tick(c, kErrMailboxTimeout)) { bad = true; break; }
             }
@@ -347,9 +345,9 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
         const bool delegate = D && (cwr & kDelegateBit) != 0;
         const uint32_t tg = row[q++];
         const uint32_t snap = row[q++];
-        uint32_t rs[kWide ? 1 : R], wa[W], prod[kWide ? 1 : (R + 3) / 4], pst[W], bkr[RO], bkw[W];
+        uint32_t rs[kWide ? 1 : R], wa[W], prod[kWide ? 1 : (R + 3) / 4], bkw[W];
         uint32_t root[RO], supf = 0;
-        int32_t wv[W], pval[RO], wgt[RO];
+        int32_t wv[W], wgt[RO];
         if constexpr (!kWide) {
 #pragma unroll
             for (int r = 0; r < R; ++r) rs[r] = row[q++];
@@ -369,12 +367,6 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
 #pragma unroll
             for (int j = 0; j < (R + 3) / 4; ++j) prod[j] = row[q++];
 #pragma unroll
-            for (int r = 0; r < RO; ++r) pval[r] = (int32_t)row[q++];
-#pragma unroll
-            for (int w = 0; w < W; ++w) pst[w] = row[q++];
-#pragma unroll
-            for (int r = 0; r < RO; ++r) bkr[r] = row[q++];
-#pragma unroll
             for (int w = 0; w < W; ++w) bkw[w] = row[q++];
 #pragma unroll
             for (int r = 0; r < RO; ++r) root[r] = row[q++];
@@ -385,17 +377,12 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
             // wide wire: the delegate rows of the chain nodes and writes
 #pragma unroll
             for (int r = 0; r < RO; ++r) {
-                pval[r] = (int32_t)box[(RW::pval + r) * kWave + lane];
-                bkr[r] = box[(RW::bkt + r) * kWave + lane];
                 root[r] = box[(RW::root + r) * kWave + lane];
                 wgt[r] = (int32_t)box[(RW::wgt + r) * kWave + lane];
             }
             supf = box[RW::supf * kWave + lane];
 #pragma unroll
-            for (int w = 0; w < W; ++w) {
-                pst[w] = box[(RW::pstamp + w) * kWave + lane];
-                bkw[w] = box[(RW::bkt + RO + w) * kWave + lane];
-            }
+            for (int w = 0; w < W; ++w) bkw[w] = box[(RW::bkt + w) * kWave + lane];
         }
         // producer code of read r: lane | write slot << 6 (0xFF: no lower writer)
         auto PROD = [&](int r) -> uint32_t {
@@ -404,7 +391,7 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
         };
         // hashed mode: the committed values of the chain nodes and the stamps of the
         // writes from the HBM tables (which hold this wave's own earlier stores), issued
-        // here so that their round trip overlaps the staged-row reads below
+        // as soon as the request's rows are in registers
         int32_t hv[RO];
         uint32_t hs[W];
         if (delegate && !L.exact) {
@@ -463,19 +450,12 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
 #pragma unroll
                 for (int w = 0; w < W; ++w) prev[w] = (valid && wa[w] != kNone) ? ls[w] : kNone;
             } else {
-                // the committed value and stamp: the ones gathered at staging when no
-                // commit since then wrote the item's bucket, else the HBM tables (loaded
-                // when the request was read)
-                uint32_t bv[RO], bw[W];
+                // the committed value and stamp: the HBM tables hold those of every commit
+                // (this wave's own stores included), loaded when the request was read
 #pragma unroll
-                for (int r = 0; r < RO; ++r) bv[r] = V.bucket[bkr[r]];
+                for (int r = 0; r < RO; ++r) base[r] = is_root[r] ? hv[r] : 0;
 #pragma unroll
-                for (int w = 0; w < W; ++w) bw[w] = V.bucket[bkw[w]];
-#pragma unroll
-                for (int r = 0; r < RO; ++r) base[r] = is_root[r] ? (bv[r] <= tg ? pval[r] : hv[r]) : 0;
-#pragma unroll
-                for (int w = 0; w < W; ++w)
-                    prev[w] = (valid && wa[w] != kNone) ? (bw[w] <= tg ? pst[w] : hs[w]) : kNone;
+                for (int w = 0; w < W; ++w) prev[w] = (valid && wa[w] != kNone) ? hs[w] : kNone;
             }
             tres = phase_clock();
 #pragma unroll
@@ -1077,10 +1057,6 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
                         }
                     }
                     dst[(RW::items + c0 + j) * kWave + lane] = x[j];
-                    if (delegate && c0 + j < RO) {
-                        dst[(RW::pval + c0 + j) * kWave + lane] = st[j];
-                        dst[(RW::bkt + c0 + j) * kWave + lane] = x[j] != kNone ? bucket_of(L, x[j]) : 0u;
-                    }
                 }
                 if (delegate) {
                     // producer code: the lower writer's lane and the write slot holding
@@ -1117,10 +1093,7 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
                     if (h != kNone) conf |= wmask(h) & lt;
                 }
                 dst[(RW::items + R + w) * kWave + lane] = x;
-                if (delegate) {
-                    dst[(RW::pstamp + w) * kWave + lane] = st;
-                    dst[(RW::bkt + RO + w) * kWave + lane] = x != kNone ? bucket_of(L, x) : 0u;
-                }
+                if (delegate) dst[(RW::bkt + w) * kWave + lane] = x != kNone ? bucket_of(L, x) : 0u;
             }
             wg_fence();
 #pragma unroll
@@ -1146,32 +1119,10 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
             const bool valid = snap != kNone;
 
             bool pref = false;
-            int32_t pval[R];
-            uint32_t pst[W];
-    #pragma unroll
-            for (int r = 0; r < R; ++r) pval[r] = 0;
-    #pragma unroll
-            for (int w = 0; w < W; ++w) pst[w] = 0;
-            // HBM state as of T_g: every load is issued (an empty entry reads item 0) and
-            // each round is pinned, so the round costs one memory latency, not one per item
-            if (!L.exact) {
-                if (delegate) {
-                    uint32_t pv[R], ps[W];
-    #pragma unroll
-                    for (int r = 0; r < R; ++r)
-                        pv[r] = (uint32_t)__hip_atomic_load(&c.vals[it[r] != kNone ? it[r] : 0u],
-                                                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    #pragma unroll
-                    for (int w = 0; w < W; ++w)
-                        ps[w] = __hip_atomic_load(&c.stamps[it[R + w] != kNone ? it[R + w] : 0u],
-                                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                    pin_round(pv);
-                    pin_round(ps);
-    #pragma unroll
-                    for (int r = 0; r < R; ++r) pval[r] = (valid && it[r] != kNone) ? (int32_t)pv[r] : 0;
-    #pragma unroll
-                    for (int w = 0; w < W; ++w) pst[w] = (valid && it[R + w] != kNone) ? ps[w] : 0u;
-                } else if (stamp_mode) {
+            // normal lane: the exact stamps as of T_g (every load is issued, an empty entry
+            // reads item 0, and the round is pinned, so it costs one memory latency)
+            if (!L.exact && !delegate) {
+                if (stamp_mode) {
                     uint32_t v[R + W];
     #pragma unroll
                     for (int r = 0; r < R + W; ++r)
@@ -1316,13 +1267,9 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
     #pragma unroll
                 for (int q = 0; q < (R + 3) / 4; ++q) dst[(RW::prod + q) * kWave + lane] = prod[q];
     #pragma unroll
-                for (int r = 0; r < R; ++r) dst[(RW::pval + r) * kWave + lane] = (uint32_t)pval[r];
-    #pragma unroll
-                for (int w = 0; w < W; ++w) dst[(RW::pstamp + w) * kWave + lane] = pst[w];
-    #pragma unroll
-                for (int r = 0; r < R + W; ++r)
-                    dst[(RW::bkt + r) * kWave + lane] =
-                        it[r] == kNone ? 0u : L.exact ? it[r] : (mx[r] & (L.nbuckets - 1));
+                for (int w = 0; w < W; ++w)
+                    dst[(RW::bkt + w) * kWave + lane] =
+                        it[R + w] == kNone ? 0u : L.exact ? it[R + w] : (mx[R + w] & (L.nbuckets - 1));
             }
         }
         if (delegate) stage_chains<R, W>(dst, chain_scr, lane);
