@@ -278,7 +278,7 @@ def main():
     ap.add_argument("--serve-cache-reqs", type=int, default=1 << 18)
     ap.add_argument("--serve-bank-txns", type=int, default=1 << 16)
     ap.add_argument("--serve-cache-waves", type=int, default=16)
-    ap.add_argument("--serial-waves-all", type=int, default=12,
+    ap.add_argument("--serial-waves-all", type=int, default=16,
                     help="serialized-lane waves of the run with every transaction routed")
     ap.add_argument("--no-route-all", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
