@@ -105,7 +105,8 @@ def memcd_section(csmv, args):
     m = csmv.Memcd(n_reqs=args.memcd_reqs, seed=1)
     log(f"memcd: {m.n_reqs} requests, {m.n_sets} sets x {m.n_ways} ways")
     with csmv.Engine(m.n_items, depth=4, client_waves=args.memcd_waves, wg_waves=6,
-                     variant="stamp", timeout_ms=20000) as e:
+                     variant="stamp", timeout_ms=20000,
+                     client_wg_waves=args.client_wg_waves) as e:
         e.fill(0)
         e.memcd_generate(m)
         st = e.memcd_run()
@@ -125,7 +126,8 @@ def scheduler_section(csmv, args):
 
     def eng(serial):
         return csmv.Engine(4096, depth=4, atr_entries=4096, client_waves=64, wg_waves=8,
-                           variant="stamp", timeout_ms=20000, n_serial_waves=serial)
+                           variant="stamp", timeout_ms=20000, n_serial_waves=serial,
+                           client_wg_waves=args.client_wg_waves)
 
     out = {"workload": "4096 accounts, 2-account transfers, 32-account audits, 90/10, "
                        f"Zipf theta; {args.sched_txns} txns per evaluation stream; "
@@ -196,9 +198,11 @@ def serving_section(csmv, args, params, tau):
                        "(8 serialized-lane when routed)",
            "loads": {}}
     with csmv.Engine(m.n_items, depth=4, client_waves=args.serve_cache_waves, wg_waves=6,
-                     variant="stamp", timeout_ms=30000) as ec, \
+                     variant="stamp", timeout_ms=30000,
+                     client_wg_waves=args.client_wg_waves) as ec, \
          csmv.Engine(N_ACCOUNTS, depth=8, client_waves=16, wg_waves=6, variant="stamp",
-                     timeout_ms=30000, n_serial_waves=8) as eb:
+                     timeout_ms=30000, n_serial_waves=8,
+                     client_wg_waves=args.client_wg_waves) as eb:
         ec.memcd_generate(m)
         # the predictor (trained on the bank-plain abort logs) is fine-tuned on this
         # partition's own logs: history stream 1 gives the counters, stream 2 the labels
@@ -263,6 +267,9 @@ def main():
     ap.add_argument("--wg-waves", type=int, default=5,
                     help="waves per workgroup: the commit server is the committer, the clock "
                          "advancer and wg_waves - 2 receivers")
+    ap.add_argument("--client-wg-waves", type=int, default=1,
+                    help="client waves per client workgroup (one per CU spreads the clients' "
+                         "memory traffic over more CUs)")
     ap.add_argument("--stage-ahead", type=int, default=5,
                     help="requests the receivers stage ahead of the committer")
     ap.add_argument("--router", default="transfers", choices=("none", "transfers", "all"),
@@ -272,7 +279,7 @@ def main():
     ap.add_argument("--variant-txns", type=int, default=1 << 16,
                     help="stream length for the side-by-side reference variants")
     ap.add_argument("--memcd-reqs", type=int, default=1 << 24)
-    ap.add_argument("--memcd-waves", type=int, default=8)
+    ap.add_argument("--memcd-waves", type=int, default=6)
     ap.add_argument("--sched-txns", type=int, default=1 << 18)
     ap.add_argument("--serve-loads", default="5e6,50e6,100e6,200e6,500e6")
     ap.add_argument("--serve-cache-reqs", type=int, default=1 << 18)
@@ -304,7 +311,7 @@ def main():
         e = csmv.Engine(N_ACCOUNTS, depth=8, atr_entries=4096, client_waves=args.waves,
                         wg_waves=args.wg_waves, variant=variant, timeout_ms=8000,
                         n_serial_waves=serial if router != "none" else 0,
-                        stage_ahead=args.stage_ahead)
+                        stage_ahead=args.stage_ahead, client_wg_waves=args.client_wg_waves)
         e.bank_generate(b)
         if router == "transfers":
             op, _ = e.bank_read_stream()
@@ -349,7 +356,8 @@ def main():
         # the same stream with audits on the serialized lane too: no transaction aborts
         ea = csmv.Engine(N_ACCOUNTS, depth=8, atr_entries=4096, client_waves=args.serial_waves_all + 1,
                          wg_waves=args.wg_waves, variant=args.variant, timeout_ms=8000,
-                         n_serial_waves=args.serial_waves_all, stage_ahead=args.stage_ahead)
+                         n_serial_waves=args.serial_waves_all, stage_ahead=args.stage_ahead,
+                         client_wg_waves=args.client_wg_waves)
         ea.bank_generate(bank)
         ea.bank_route(np.ones(bank.n_txns, dtype=np.uint8))
         ea.fill(INIT)
@@ -404,6 +412,7 @@ def main():
             "client_waves": args.waves,
             "serial_lane_waves": args.serial_waves if args.router != "none" else 0,
             "server_wg_waves": args.wg_waves,
+            "client_wg_waves": args.client_wg_waves,
             "stage_ahead": args.stage_ahead,
             "client_lanes": args.waves * 64,
             "record_ring": 4096,
@@ -426,7 +435,8 @@ def main():
         out["scheduler"], params, tau = scheduler_section(csmv, args)
     if not args.no_serve:
         if params is None:
-            with csmv.Engine(4096, depth=4, client_waves=64, wg_waves=8, variant="stamp") as pe:
+            with csmv.Engine(4096, depth=4, client_waves=64, wg_waves=8, variant="stamp",
+                             client_wg_waves=args.client_wg_waves) as pe:
                 from csmv import sched
                 sched.train(pe, n_txns=1 << 17, steps=300)
                 params = pe.pred_get_params()

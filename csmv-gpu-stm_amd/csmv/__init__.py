@@ -62,7 +62,8 @@ class Config(C.Structure):
                 ("client_waves", C.c_uint32), ("wg_waves", C.c_uint32), ("variant", C.c_uint32),
                 ("timeout_ms", C.c_uint32), ("max_attempts", C.c_uint32),
                 ("stage_ahead", C.c_uint32), ("lds_kib", C.c_uint32),
-                ("n_serial_waves", C.c_uint32), ("begin_wait", C.c_uint32)]
+                ("n_serial_waves", C.c_uint32), ("begin_wait", C.c_uint32),
+                ("client_wg_waves", C.c_uint32)]
 
 
 class BankSpec(C.Structure):
@@ -237,12 +238,13 @@ class Engine:
     def __init__(self, n_items: int, depth: int = 4, atr_entries: int = 4096,
                  client_waves: int = 64, wg_waves: int = 8, variant="stamp",
                  timeout_ms: int = 4000, max_attempts: int = 0, stage_ahead: int = 0,
-                 lds_kib: int = 0, n_serial_waves: int = 0, begin_wait: bool = True):
+                 lds_kib: int = 0, n_serial_waves: int = 0, begin_wait: bool = True,
+                 client_wg_waves: int = 0):
         if isinstance(variant, str):
             variant = VARIANTS[variant]
         self.cfg = Config(n_items, depth, atr_entries, client_waves, wg_waves, variant,
                           timeout_ms, max_attempts, stage_ahead, lds_kib, n_serial_waves,
-                          1 if begin_wait else 0)
+                          1 if begin_wait else 0, client_wg_waves)
         self.n_items = n_items
         self._h = C.c_void_p()
         st = lib().csmv_create(C.byref(self.cfg), C.byref(self._h))

@@ -365,7 +365,9 @@ __global__ void __launch_bounds__(512) bank_kernel(Ctx c, ServerLayout L)
         server_workgroup<R, W, true>(c, L, lds);
         return;
     }
-    const uint32_t cw = (blockIdx.x - 1) * c.wg_waves + threadIdx.x / kWave;
+    const uint32_t wave = threadIdx.x / kWave;
+    if (wave >= c.client_wg_waves) return;
+    const uint32_t cw = (blockIdx.x - 1) * c.client_wg_waves + wave;
     if (cw >= c.n_client_waves) return;
     if (cw >= c.n_client_waves - c.n_serial_waves) bank_serial_client<R, W>(c, cw);
     else bank_client<R, W>(c, cw);
@@ -402,7 +404,9 @@ __global__ void __launch_bounds__(512) memcd_kernel(Ctx c, ServerLayout L)
         server_workgroup<R, W, false>(c, L, lds);
         return;
     }
-    const uint32_t cw = (blockIdx.x - 1) * c.wg_waves + threadIdx.x / kWave;
+    const uint32_t wave = threadIdx.x / kWave;
+    if (wave >= c.client_wg_waves) return;
+    const uint32_t cw = (blockIdx.x - 1) * c.client_wg_waves + wave;
     if (cw < c.n_client_waves) memcd_client<R, W>(c, cw);
 }
 
@@ -427,10 +431,12 @@ __global__ void __launch_bounds__(512) serve_kernel(Ctx cm, ServerLayout Lm, Ctx
     const uint32_t b = blockIdx.x - n_srv;
     const uint32_t wave = threadIdx.x / kWave;
     if (b < roles.cache_wgs) {
-        const uint32_t cw = b * cm.wg_waves + wave;
+        if (wave >= cm.client_wg_waves) return;
+        const uint32_t cw = b * cm.client_wg_waves + wave;
         if (cw < cm.n_client_waves) memcd_client<RM, WM>(cm, cw);
     } else {
-        const uint32_t cw = (b - roles.cache_wgs) * cb.wg_waves + wave;
+        if (wave >= cb.client_wg_waves) return;
+        const uint32_t cw = (b - roles.cache_wgs) * cb.client_wg_waves + wave;
         if (cw >= cb.n_client_waves) return;
         if (cw >= cb.n_client_waves - cb.n_serial_waves) bank_serve_serial_client<RB, WB>(cb, cw);
         else bank_serve_client<RB, WB>(cb, cw);
@@ -699,9 +705,11 @@ csmv_status csmv_create(const csmv_config *cfg_in, csmv_engine **out)
     if (cfg.max_attempts == 0) cfg.max_attempts = 1u << 20;
     if (cfg.wg_waves == 0) cfg.wg_waves = 8;
     if (cfg.stage_ahead == 0) cfg.stage_ahead = 3;
+    if (cfg.client_wg_waves == 0) cfg.client_wg_waves = cfg.wg_waves;
     if (cfg.n_serial_waves >= cfg.client_waves && cfg.n_serial_waves) return CSMV_E_ARG;
     if (cfg.n_items == 0 || cfg.depth == 0 || cfg.depth > 64 || cfg.client_waves == 0 ||
-        cfg.wg_waves < 3 || cfg.wg_waves > 8 || cfg.stage_ahead > 64 || cfg.atr_entries < 64 ||
+        cfg.wg_waves < 3 || cfg.wg_waves > 8 || cfg.client_wg_waves > cfg.wg_waves ||
+        cfg.stage_ahead > 64 || cfg.atr_entries < 64 ||
         (cfg.atr_entries & (cfg.atr_entries - 1)) || cfg.atr_entries > (1u << 16) ||
         (cfg.variant & ~0xFu) || (cfg.variant & 3u) == 3u)
         return CSMV_E_ARG;
@@ -952,6 +960,7 @@ static void base_ctx(csmv_engine *e, Ctx &c)
     c.qmask = e->Q - 1;
     c.n_client_waves = e->cfg.client_waves;
     c.wg_waves = e->cfg.wg_waves;
+    c.client_wg_waves = e->cfg.client_wg_waves;
     c.variant = e->cfg.variant;
     c.atr_entries = e->cfg.atr_entries;
     c.stage_ahead = e->cfg.stage_ahead;
@@ -1028,7 +1037,8 @@ static csmv_status launch_persistent(csmv_engine *e, K kern, const Ctx &c, const
         return fail(e, CSMV_E_CAPACITY, "server LDS layout of " + std::to_string(lds_bytes) +
                                             " bytes exceeds the budget (reduce atr_entries)");
     const uint32_t block = e->cfg.wg_waves * kWave;
-    const uint32_t client_wgs = (e->cfg.client_waves + e->cfg.wg_waves - 1) / e->cfg.wg_waves;
+    const uint32_t cpw = e->cfg.client_wg_waves;
+    const uint32_t client_wgs = (e->cfg.client_waves + cpw - 1) / cpw;
     const uint32_t grid = 1 + client_wgs;
     HIPCHK(e, hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)lds_bytes));
@@ -1720,8 +1730,12 @@ csmv_status csmv_serve_run(csmv_engine *cache, csmv_engine *bank, const csmv_ser
     ServeRoles roles{};
     roles.has_cache = cache ? 1u : 0u;
     roles.has_bank = bank ? 1u : 0u;
-    roles.cache_wgs = cache ? (cache->cfg.client_waves + wg - 1) / wg : 0u;
-    const uint32_t bank_wgs = bank ? (bank->cfg.client_waves + wg - 1) / wg : 0u;
+    roles.cache_wgs = cache ? (cache->cfg.client_waves + cache->cfg.client_wg_waves - 1) /
+                                  cache->cfg.client_wg_waves
+                            : 0u;
+    const uint32_t bank_wgs =
+        bank ? (bank->cfg.client_waves + bank->cfg.client_wg_waves - 1) / bank->cfg.client_wg_waves
+             : 0u;
     const uint32_t grid = roles.has_cache + roles.has_bank + roles.cache_wgs + bank_wgs;
     auto kern = serve_kernel<kMcR, kMcW, kServeBankR, kBankW>;
     HIPCHK(p, hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,

@@ -59,7 +59,7 @@ typedef struct {
     uint32_t depth;            /* versions per item (MaxVersions, API.cuh:18), 1..64 */
     uint32_t atr_entries;      /* record ring capacity (TXRecordSize, API.cuh:17); power of two */
     uint32_t client_waves;     /* persistent client wavefronts (64 lanes each) */
-    uint32_t wg_waves;         /* waves per workgroup, 4..16; the server workgroup uses all */
+    uint32_t wg_waves;         /* waves per workgroup, 3..8; the server workgroup uses all */
     uint32_t variant;          /* CSMV_VAL_* | CSMV_INSERT_* | CSMV_WB_* */
     uint32_t timeout_ms;       /* bound of every device-side wait (0 = 4000) */
     uint32_t max_attempts;     /* per-transaction attempt bound (0 = 1<<20) */
@@ -68,6 +68,9 @@ typedef struct {
     uint32_t n_serial_waves;   /* client waves of the serialized lane (routed transfers) */
     uint32_t begin_wait;       /* 1: a wave's snapshot covers its own last commit
                                   (TXBegin's prevTs wait, bank/csmv_sh/API.cu:283) */
+    uint32_t client_wg_waves;  /* client waves per client workgroup, 1..wg_waves (0 = wg_waves):
+                                  fewer spread the clients over more CUs, each with its own
+                                  memory pipeline; the other waves of the workgroup leave */
 } csmv_config;
 
 typedef struct {

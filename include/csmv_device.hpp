@@ -163,6 +163,7 @@ struct Ctx {
     uint64_t *prof;          // kProf* timing counters (wall_clock64 ticks)
     uint32_t qmask;
     uint32_t n_client_waves, wg_waves;
+    uint32_t client_wg_waves; // client waves per client workgroup
     uint32_t variant, atr_entries;
     uint32_t stage_ahead;    // receivers stage request n once request n - stage_ahead is taken
     uint32_t n_serial_waves; // client waves of the serialized lane (the last ones)

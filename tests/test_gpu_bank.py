@@ -16,12 +16,15 @@ INIT = 1000
 
 
 def run_bank(n_items, bank, variant="stamp", depth=4, atr=4096, waves=8, wg_waves=8,
-             check_stream=True, timeout_ms=4000, route=None, serial_waves=0, begin_wait=True):
+             check_stream=True, timeout_ms=4000, route=None, serial_waves=0, begin_wait=True,
+             client_wg_waves=0):
     print(f"\n[case] n_items={n_items} bank={bank} variant={variant} depth={depth} atr={atr} "
-          f"waves={waves} serial_waves={serial_waves} routed={route is not None}", flush=True)
+          f"waves={waves} serial_waves={serial_waves} routed={route is not None} "
+          f"client_wg_waves={client_wg_waves}", flush=True)
     with csmv.Engine(n_items, depth=depth, atr_entries=atr, client_waves=waves,
                      wg_waves=wg_waves, variant=variant, timeout_ms=timeout_ms,
-                     n_serial_waves=serial_waves, begin_wait=begin_wait) as e:
+                     n_serial_waves=serial_waves, begin_wait=begin_wait,
+                     client_wg_waves=client_wg_waves) as e:
         e.fill(INIT)
         e.bank_generate(bank)
         if route is not None:
@@ -177,3 +180,17 @@ def test_serialized_lane_small_cache_unknowns():
     print("[stats]", {k: st[k] for k in ("commits", "aborts_unknown", "stamp_exact")}, flush=True)
     assert rep["fails"] == 0, rep
     assert st["stamp_exact"] == 0
+
+
+@pytest.mark.parametrize("client_wg_waves,route_all", [(1, False), (2, True)])
+def test_clients_spread_over_workgroups(client_wg_waves, route_all):
+    """Client waves spread over more workgroups (one or two per workgroup, the other waves
+    of each leave at once), at the bank-at-scale shape with the transfers, or everything, on
+    the serialized lane: the log replays and every transaction commits."""
+    bank = csmv.Bank(n_txns=1 << 17, seed=59, n_reads=4, n_reads_ro=4, pct_ro=10, zipf=True)
+    route = ((lambda op: np.ones(op.size, dtype=np.uint8)) if route_all
+             else (lambda op: ((op & 0xFF) == 0).astype(np.uint8)))
+    st = run_bank(1 << 20, bank, variant="stamp", depth=8, waves=24, serial_waves=6,
+                  wg_waves=5, route=route, client_wg_waves=client_wg_waves)
+    if route_all:
+        assert st["aborts"] == 0

@@ -9,10 +9,12 @@ import orc
 pytestmark = pytest.mark.gpu
 
 
-def run_memcd(m: csmv.Memcd, waves=16, wg_waves=8, depth=4, check_stream=True):
-    print(f"\n[case] memcd {m} waves={waves} wg_waves={wg_waves} depth={depth}", flush=True)
+def run_memcd(m: csmv.Memcd, waves=16, wg_waves=8, depth=4, check_stream=True,
+              client_wg_waves=0):
+    print(f"\n[case] memcd {m} waves={waves} wg_waves={wg_waves} depth={depth} "
+          f"client_wg_waves={client_wg_waves}", flush=True)
     with csmv.Engine(m.n_items, depth=depth, client_waves=waves, wg_waves=wg_waves,
-                     variant="stamp", timeout_ms=8000) as e:
+                     variant="stamp", timeout_ms=8000, client_wg_waves=client_wg_waves) as e:
         e.fill(0)
         e.memcd_generate(m)
         st = e.memcd_run()
@@ -51,3 +53,10 @@ def test_memcd_config_shape():
     st, rep, log = run_memcd(csmv.Memcd(n_reqs=1 << 18, seed=1), waves=32, check_stream=False)
     assert rep["hits"] > 0
     assert (log["lat_ticks"] > 0).all()
+
+
+def test_memcd_one_client_wave_per_workgroup():
+    """The cache driver with each client wave in a workgroup of its own (the layout the
+    bench uses): results replay exactly."""
+    m = csmv.Memcd(n_reqs=1 << 16, seed=61, n_keys=50_000, n_sets=1 << 11, batch=8192)
+    run_memcd(m, waves=6, wg_waves=6, client_wg_waves=1)

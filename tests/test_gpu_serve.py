@@ -22,14 +22,14 @@ INIT = 1000
 N_ACC = 1 << 16
 
 
-def cache_engine(m: csmv.Memcd, waves=16):
+def cache_engine(m: csmv.Memcd, waves=16, client_wg_waves=0):
     return csmv.Engine(m.n_items, depth=4, client_waves=waves, wg_waves=6, variant="stamp",
-                       timeout_ms=8000)
+                       timeout_ms=8000, client_wg_waves=client_wg_waves)
 
 
-def bank_engine(waves=8):
+def bank_engine(waves=8, client_wg_waves=0):
     return csmv.Engine(N_ACC, depth=8, client_waves=waves, wg_waves=6, variant="stamp",
-                       timeout_ms=8000)
+                       timeout_ms=8000, client_wg_waves=client_wg_waves)
 
 
 def serve_bank(n, seed=7):
@@ -83,12 +83,15 @@ def check_latency(e, st, n):
     return arr, lat
 
 
-def test_serve_mix_replays_and_reports_latency():
+@pytest.mark.parametrize("cpw", [0, 1])
+def test_serve_mix_replays_and_reports_latency(cpw):
+    """The serving mix replays exactly, with full client workgroups (cpw 0) and with one
+    client wave per workgroup (cpw 1)."""
     m = csmv.Memcd(n_reqs=1 << 15, seed=3, n_keys=20_000, n_sets=1 << 10, batch=4096)
     b = serve_bank(1 << 13)
-    print(f"\n[case] serve mix: cache {m.n_reqs} reqs + bank {b.n_txns} txns, 20M req/s",
-          flush=True)
-    with cache_engine(m) as ec, bank_engine() as eb:
+    print(f"\n[case] serve mix: cache {m.n_reqs} reqs + bank {b.n_txns} txns, 20M req/s, "
+          f"client_wg_waves={cpw}", flush=True)
+    with cache_engine(m, client_wg_waves=cpw) as ec, bank_engine(client_wg_waves=cpw) as eb:
         ec.fill(0)
         ec.memcd_generate(m)
         eb.fill(INIT)
