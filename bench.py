@@ -202,7 +202,7 @@ def serving_section(csmv, args, params, tau):
                      client_wg_waves=args.client_wg_waves) as ec, \
          csmv.Engine(N_ACCOUNTS, depth=8, client_waves=16, wg_waves=6, variant="stamp",
                      timeout_ms=30000, n_serial_waves=8,
-                     client_wg_waves=args.client_wg_waves) as eb:
+                     client_wg_waves=args.serve_bank_client_wg_waves) as eb:
         ec.memcd_generate(m)
         # the predictor (trained on the bank-plain abort logs) is fine-tuned on this
         # partition's own logs: history stream 1 gives the counters, stream 2 the labels
@@ -270,6 +270,9 @@ def main():
     ap.add_argument("--client-wg-waves", type=int, default=1,
                     help="client waves per client workgroup (one per CU spreads the clients' "
                          "memory traffic over more CUs)")
+    ap.add_argument("--serve-bank-client-wg-waves", type=int, default=0,
+                    help="client waves per workgroup of the serving loop's bank partition "
+                         "(0: six)")
     ap.add_argument("--stage-ahead", type=int, default=5,
                     help="requests the receivers stage ahead of the committer")
     ap.add_argument("--router", default="transfers", choices=("none", "transfers", "all"),
@@ -430,7 +433,7 @@ def main():
     }
     if not args.no_memcd:
         out["memcd"] = memcd_section(csmv, args)
-    params, tau = None, 0.02
+    params, tau = None, 0.002
     if not args.no_sched:
         out["scheduler"], params, tau = scheduler_section(csmv, args)
     if not args.no_serve:

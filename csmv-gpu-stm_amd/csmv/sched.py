@@ -90,7 +90,7 @@ def route(e: Engine, tau: float = 0.5) -> int:
     return e.bank_route_learned(tau)
 
 
-TAUS = (0.02, 0.05, 0.1, 0.2, 0.3, 0.5)
+TAUS = (0.002, 0.005, 0.01, 0.02, 0.05, 0.1, 0.2, 0.3, 0.5)
 
 
 def calibrate(plain: Engine, routed: Engine, thetas: Iterable[float] = (0.6, 0.9, 1.2),
