@@ -107,7 +107,7 @@ __device__ __forceinline__ void serve_release(const Ctx &c, bool done, uint32_t 
 
 // Snapshot reads of up to N items (kNone entries are skipped): all head words, then all
 // head slots, as two rounds of independent loads; a head newer than the snapshot falls
-// back to the ring walk.
+// back to the ring walk below the head (pool_read_below).
 template <int N>
 __device__ __forceinline__ bool pool_read_chunk(const Ctx &c, const uint32_t (&it)[N], uint32_t snap,
                                                 int32_t (&out)[N])
@@ -127,7 +127,7 @@ __device__ __forceinline__ bool pool_read_chunk(const Ctx &c, const uint32_t (&i
 #pragma unroll
     for (int j = 0; j < N; ++j) {
         int32_t v = (int32_t)(uint32_t)x[j];
-        if (it[j] != kNone && (uint32_t)(x[j] >> 32) > snap) ok = pool_read(c, it[j], snap, v) && ok;
+        if (it[j] != kNone && (uint32_t)(x[j] >> 32) > snap) ok = pool_read_below(c, it[j], snap, v) && ok;
         out[j] = v;
     }
     return ok;

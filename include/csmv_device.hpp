@@ -296,6 +296,28 @@ __device__ __forceinline__ bool pool_read(const Ctx &c, uint32_t item, uint32_t 
     return false;
 }
 
+// The ring walk for an item whose head was seen newer than snap: the head now is that
+// version or a newer one, so the walk starts one slot below it (an item with a single
+// version has none at or below snap).
+__device__ __forceinline__ bool pool_read_below(const Ctx &c, uint32_t item, uint32_t snap,
+                                                int32_t &out)
+{
+    const uint32_t ht = ld_rlx(&c.ht[item]);
+    const uint32_t h = ht & 0xFFFFu, t = ht >> 16;
+    if (h == t) return false;
+    uint32_t j = (h == 0) ? c.depth - 1 : h - 1;
+    for (uint32_t d = 1; d < c.depth; ++d) {
+        const uint64_t x = ld_rlx(&c.slot[(size_t)item * c.depth + j]);
+        if ((uint32_t)(x >> 32) <= snap) {
+            out = (int32_t)(uint32_t)x;
+            return true;
+        }
+        if (j == t) break;
+        j = (j == 0) ? c.depth - 1 : j - 1;
+    }
+    return false;
+}
+
 // TXBegin (collabValidation/csmv_sh/API.cu:581-586): wave-uniform snapshot.
 // With wait_ts > 0 the snapshot is taken once the clock covers the wave's own last
 // commit (TXBegin's prevTs wait, bank/csmv_sh/API.cu:283); otherwise a wave re-reading
