@@ -133,6 +133,31 @@ __device__ __forceinline__ bool pool_read_chunk(const Ctx &c, const uint32_t (&i
     return ok;
 }
 
+// Snapshot reads of up to N items without the ring walk: all head words, then all head
+// slots, as two rounds of independent loads.  Returns the items (bit j) whose head is newer
+// than the snapshot; their values in out are the heads', not the snapshot's.
+template <int N>
+__device__ __forceinline__ uint32_t pool_read_heads(const Ctx &c, const uint32_t (&it)[N], uint32_t snap,
+                                                    int32_t (&out)[N])
+{
+    uint32_t ht[N];
+    uint64_t x[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) ht[j] = ld_rlx(&c.ht[it[j] != kNone ? it[j] : 0u]);
+    pin_round(ht);
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+        x[j] = ld_rlx(&c.slot[(size_t)(it[j] != kNone ? it[j] : 0u) * c.depth + (ht[j] & 0xFFFFu)]);
+    pin_round(x);
+    uint32_t newer = 0;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        out[j] = (int32_t)(uint32_t)x[j];
+        newer |= (it[j] != kNone && (uint32_t)(x[j] >> 32) > snap) ? 1u << j : 0u;
+    }
+    return newer;
+}
+
 // Serving-loop bank transaction: a transfer over the first two of nr accounts
 // (2 <= nr <= R) that also reads the other nr - 2 (bank/csmv_sh/exe.cu:184-201 widened
 // to the serving mix's read sets).  Same commit path and log as the bank driver, plus
