@@ -122,15 +122,17 @@ template <int R, int W> __device__ __forceinline__ void memcd_client(const Ctx &
                 ib[w] = on ? kMcKhi * ns + slot : kNone;
                 ib[8 + w] = on ? kMcTs * ns + slot : kNone;
             }
-            // No head newer than the snapshot is walked back.  Every write of a slot's key
-            // or value writes its state too, in the same commit; a request that read a
-            // state, key or value written since its snapshot is doomed at validation (it
-            // logs the state) or, as a read-only GET, must not commit at that snapshot, so
-            // it aborts here and retries at a newer one.  The one exception is a GET's
-            // timestamps of the ways other than its hit, which it neither uses nor logs.
-            const uint32_t na = pool_read_heads<16>(c, ia, tx.snap, va);
-            const uint32_t nb = pool_read_heads<16>(c, ib, tx.snap, vb);
-            if (na != 0 || (nb & 0xFFu) != 0) tx.aborted = true;
+            // A GET walks back the states and keys newer than its snapshot (a read-only GET
+            // commits at its snapshot) and, once its hit is known, that slot's timestamp; it
+            // never uses the other ways' timestamps.  A SET logs the state and timestamp of
+            // every way, and every write of a key writes the slot's state too, so a SET that
+            // read anything written since its snapshot is doomed at validation: it aborts at
+            // once instead of walking.
+            const bool get = op == kMcGet;
+            bool ok = true;
+            const uint32_t na = pool_read_some<16>(c, ia, tx.snap, va, get ? 0xFFFFu : 0u, ok);
+            const uint32_t nb = pool_read_some<16>(c, ib, tx.snap, vb, get ? 0x00FFu : 0u, ok);
+            if (!ok) tx.aborted = true;
             if (op == kMcGet) {
                 // the hit way, then its value; the states are logged up to the hit way
                 // (the ones a sequential probe reads) and the hit slot's timestamp
@@ -143,14 +145,20 @@ template <int R, int W> __device__ __forceinline__ void memcd_client(const Ctx &
 #pragma unroll
                 for (int w = 0; w < 8; ++w)
                     if ((uint32_t)w < ways && (hit < 0 || w <= hit)) log_read(tx, ia[w]);
-                if (hit >= 0 && ((nb >> (8 + hit)) & 1u) != 0) tx.aborted = true;
+                if (hit >= 0 && ((nb >> (8 + hit)) & 1u) != 0) {
+                    int32_t tw = 0;
+                    if (!pool_read_below(c, kMcTs * ns + set * ways + (uint32_t)hit, tx.snap, tw))
+                        tx.aborted = true;
+#pragma unroll
+                    for (int w = 0; w < 8; ++w) vb[8 + w] = (w == hit) ? tw : vb[8 + w];
+                }
                 if (hit >= 0 && !tx.aborted) {
                     const uint32_t slot = set * ways + (uint32_t)hit;
                     uint32_t iv[8];
                     int32_t vv[8];
 #pragma unroll
                     for (int j = 0; j < 8; ++j) iv[j] = (kMcV0 + j) * ns + slot;
-                    if (pool_read_heads<8>(c, iv, tx.snap, vv) != 0) tx.aborted = true;
+                    if (!pool_read_chunk<8>(c, iv, tx.snap, vv)) tx.aborted = true;
                     uint32_t acc = 0;
 #pragma unroll
                     for (int j = 0; j < 8; ++j) acc += (uint32_t)vv[j] * (2u * j + 1u);
@@ -170,9 +178,8 @@ template <int R, int W> __device__ __forceinline__ void memcd_client(const Ctx &
                 }
             } else {
                 // SET: the way holding the key, else the first invalid way, else the least
-                // recently used; states and timestamps are logged (so a newer timestamp
-                // dooms it as well)
-                if (nb != 0) tx.aborted = true;
+                // recently used; states and timestamps are logged
+                if (na != 0 || nb != 0) tx.aborted = true;
                 int found = -1, invalid = -1, lru = 0;
                 uint32_t lru_ts = kNone;
 #pragma unroll

@@ -133,12 +133,13 @@ __device__ __forceinline__ bool pool_read_chunk(const Ctx &c, const uint32_t (&i
     return ok;
 }
 
-// Snapshot reads of up to N items without the ring walk: all head words, then all head
-// slots, as two rounds of independent loads.  Returns the items (bit j) whose head is newer
-// than the snapshot; their values in out are the heads', not the snapshot's.
+// Snapshot reads of up to N items: all head words, then all head slots, as two rounds of
+// independent loads.  An item in walk (bit j) whose head is newer than the snapshot is
+// walked back below the head (ok turns false when no version is old enough); any other
+// such item is returned (bit j), its value in out being the head's, not the snapshot's.
 template <int N>
-__device__ __forceinline__ uint32_t pool_read_heads(const Ctx &c, const uint32_t (&it)[N], uint32_t snap,
-                                                    int32_t (&out)[N])
+__device__ __forceinline__ uint32_t pool_read_some(const Ctx &c, const uint32_t (&it)[N], uint32_t snap,
+                                                   int32_t (&out)[N], uint32_t walk, bool &ok)
 {
     uint32_t ht[N];
     uint64_t x[N];
@@ -152,8 +153,12 @@ __device__ __forceinline__ uint32_t pool_read_heads(const Ctx &c, const uint32_t
     uint32_t newer = 0;
 #pragma unroll
     for (int j = 0; j < N; ++j) {
-        out[j] = (int32_t)(uint32_t)x[j];
-        newer |= (it[j] != kNone && (uint32_t)(x[j] >> 32) > snap) ? 1u << j : 0u;
+        int32_t v = (int32_t)(uint32_t)x[j];
+        if (it[j] != kNone && (uint32_t)(x[j] >> 32) > snap) {
+            if ((walk >> j) & 1u) ok = pool_read_below(c, it[j], snap, v) && ok;
+            else newer |= 1u << j;
+        }
+        out[j] = v;
     }
     return newer;
 }
