@@ -194,3 +194,22 @@ def test_clients_spread_over_workgroups(client_wg_waves, route_all):
                   wg_waves=5, route=route, client_wg_waves=client_wg_waves)
     if route_all:
         assert st["aborts"] == 0
+
+
+def test_empty_stream_is_rejected():
+    """A stream of no transactions is an argument error, raised before any launch."""
+    print("\n[case] empty bank stream", flush=True)
+    with csmv.Engine(4096, depth=4, client_waves=8) as e:
+        with pytest.raises(csmv.CsmvError):
+            e.bank_generate(csmv.Bank(n_txns=0, seed=1))
+
+
+@pytest.mark.parametrize("n_txns,routed", [(1, False), (1, True), (1000, False), (1000, True),
+                                           (4097, True)])
+def test_ragged_streams(n_txns, routed):
+    """Streams that do not fill the last wave (one transaction, 1000, 4097), on the normal
+    lane and with every transfer on the serialized lane: the log replays exactly."""
+    bank = csmv.Bank(n_txns=n_txns, seed=67, n_reads=4, n_reads_ro=4, pct_ro=10, zipf=True)
+    route = (lambda op: ((op & 0xFF) == 0).astype(np.uint8)) if routed else None
+    run_bank(1 << 16, bank, variant="stamp", depth=8, waves=12, wg_waves=5,
+             serial_waves=4 if routed else 0, route=route, client_wg_waves=1)

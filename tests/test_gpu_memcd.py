@@ -60,3 +60,10 @@ def test_memcd_one_client_wave_per_workgroup():
     bench uses): results replay exactly."""
     m = csmv.Memcd(n_reqs=1 << 16, seed=61, n_keys=50_000, n_sets=1 << 11, batch=8192)
     run_memcd(m, waves=6, wg_waves=6, client_wg_waves=1)
+
+
+@pytest.mark.parametrize("n_reqs", [1, 777])
+def test_memcd_ragged_streams(n_reqs):
+    """Request streams that do not fill the last wave: results replay exactly."""
+    m = csmv.Memcd(n_reqs=n_reqs, seed=71, n_keys=5_000, n_sets=1 << 8, batch=256)
+    run_memcd(m, waves=6, wg_waves=6, client_wg_waves=1)
