@@ -166,9 +166,8 @@ struct LaneSum {
 // ---- committer --------------------------------------------------------------------------
 // Committed-value tables.  Exact mode (every item fits LDS): stamp and value of every
 // item in LDS.  Hashed mode: the exact tables are in HBM (c.stamps, c.vals, written only
-// by the committer) and LDS holds the stamp buckets; a value or stamp gathered at
-// staging (as of commit T_g) is current when no commit since then wrote its bucket, and
-// the HBM tables (which hold this wave's own earlier stores) give it otherwise.
+// by the committer, so they hold every commit's value and stamp for this wave's own later
+// loads) and LDS holds the stamp buckets that filter the normal lane's validation.
 struct ValueTables {
     uint32_t *bucket, *lval;
 };
@@ -423,8 +422,7 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
             const uint32_t k = (uint32_t)wv[0], amt = (uint32_t)wv[1];
             // Each chain node's root and the sum of the deltas from it were resolved at
             // staging (stage_chains); the root's value before the request is the one thing
-            // that depends on the commit order: the committed value (exact LDS table, the
-            // value gathered at staging when no commit since then wrote the item's bucket,
+            // that depends on the commit order: the committed value (the exact LDS table,
             // or the HBM table this wave writes).  Chain nodes are the first RO reads
             // (every read of a narrow wire, the transfer operands of a wide one).
             uint32_t ops[RO];
