@@ -1013,29 +1013,26 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
             uint32_t p = 0;
             constexpr int kChunk = 8;
             static_assert(R % kChunk == 0, "wide wires stage whole chunks of reads");
-            constexpr int RO = Rows<R, W>::RO;
             // the read rows, a chunk at a time
 #pragma unroll 1
             for (int c0 = 0; c0 < R; c0 += kChunk) {
                 uint32_t x[kChunk], st[kChunk], plane[kChunk];
 #pragma unroll
                 for (int j = 0; j < kChunk; ++j) x[j] = ld_rlx(&src[(1 + c0 + j) * kWave + lane]);
-                // staging-time stamps (normal lane) or, delegated, the committed values
-                // of the transfer operands
-                // (one pinned round of loads; an item not wanted reads item 0)
-                const uint32_t *tab = delegate ? (const uint32_t *)c.vals : c.stamps;
-                if (!L.exact && (!delegate || c0 < RO)) {
+                // staging-time stamps of the normal lane (one pinned round of loads; an item
+                // not wanted reads item 0); the serialized lane needs none
+                if (!L.exact && !delegate) {
                     pin_round(x);
 #pragma unroll
                     for (int j = 0; j < kChunk; ++j) {
-                        const bool want = valid && x[j] != kNone && (!delegate || c0 + j < RO);
-                        st[j] = __hip_atomic_load(&tab[want ? x[j] : 0u], __ATOMIC_RELAXED,
+                        const bool want = valid && x[j] != kNone;
+                        st[j] = __hip_atomic_load(&c.stamps[want ? x[j] : 0u], __ATOMIC_RELAXED,
                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
                     }
                     pin_round(st);
 #pragma unroll
                     for (int j = 0; j < kChunk; ++j) {
-                        const bool want = valid && x[j] != kNone && (!delegate || c0 + j < RO);
+                        const bool want = valid && x[j] != kNone;
                         st[j] = want ? st[j] : 0u;
                     }
                 } else {
@@ -1045,7 +1042,7 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
 #pragma unroll
                 for (int j = 0; j < kChunk; ++j) {
                     plane[j] = 0xFFu;
-                    if (!delegate) p = st[j] > p ? st[j] : p;
+                    p = st[j] > p ? st[j] : p;
                     if (valid && x[j] != kNone) {
                         const uint32_t h = hash_find(hkey, hmask, x[j]);
                         if (h != kNone) {
@@ -1081,11 +1078,11 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
 #pragma unroll
             for (int w = 0; w < W; ++w) {
                 const uint32_t x = wa[w];
-                const uint32_t st = (!L.exact && valid && x != kNone)
+                const uint32_t st = (!L.exact && !delegate && valid && x != kNone)
                                         ? __hip_atomic_load(&c.stamps[x], __ATOMIC_RELAXED,
                                                             __HIP_MEMORY_SCOPE_WORKGROUP)
                                         : 0u;
-                if (!delegate) p = st > p ? st : p;
+                p = st > p ? st : p;
                 if (valid && x != kNone) {
                     const uint32_t h = hash_find(hkey, hmask, x);
                     if (h != kNone) conf |= wmask(h) & lt;
