@@ -263,7 +263,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--variant", default="stamp")
     ap.add_argument("--waves", type=int, default=int(os.environ.get("CSMV_BENCH_WAVES", "64")))
-    ap.add_argument("--serial-waves", type=int, default=10)
+    ap.add_argument("--serial-waves", type=int, default=14)
     ap.add_argument("--wg-waves", type=int, default=5,
                     help="waves per workgroup: the commit server is the committer, the clock "
                          "advancer and wg_waves - 2 receivers")
