@@ -102,7 +102,7 @@ __global__ void __launch_bounds__(256) bank_generate_kernel(GenArgs g)
     }
 }
 
-__global__ void __launch_bounds__(256) pool_fill_kernel(uint32_t *ht, uint64_t *slot, uint64_t *tab,
+__global__ void __launch_bounds__(256) pool_fill_kernel(uint32_t *ht, uint64_t *slot, int32_t *vals,
                                                         uint32_t n, uint32_t depth,
                                                         const int32_t *values, int32_t fill)
 {
@@ -111,15 +111,8 @@ __global__ void __launch_bounds__(256) pool_fill_kernel(uint32_t *ht, uint64_t *
     ht[i] = 0;
     const int32_t v = values ? values[i] : fill;
     slot[(size_t)i * depth] = (uint32_t)v;
-    tab[i] = (uint32_t)v;
+    vals[i] = v;
     for (uint32_t d = 1; d < depth; ++d) slot[(size_t)i * depth + d] = 0;
-}
-
-// A run's clock starts at 0: every committed stamp is cleared, every value kept.
-__global__ void __launch_bounds__(256) tab_clear_stamps_kernel(uint64_t *tab, uint32_t n)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) tab[i] &= 0xFFFFFFFFull;
 }
 
 __global__ void __launch_bounds__(256) pool_head_kernel(const uint32_t *ht, const uint64_t *slot,
@@ -481,8 +474,8 @@ struct csmv_engine {
     // control + messaging
     uint8_t *ctl = nullptr;
     uint64_t *ring = nullptr, *reply = nullptr, *done = nullptr;
-    uint32_t *mbox = nullptr;
-    uint64_t *tab = nullptr;
+    uint32_t *mbox = nullptr, *stamps = nullptr;
+    int32_t *vals = nullptr;
     uint64_t *reply_v = nullptr;
     // routing of the resident stream (empty = everything on the normal lane)
     uint32_t *norm_list = nullptr, *hot_list = nullptr;
@@ -760,7 +753,8 @@ csmv_status csmv_create(const csmv_config *cfg_in, csmv_engine **out)
         (s = dalloc(e, &e->done, e->Q)) ||
         (s = dalloc(e, &e->reply, (size_t)cfg.client_waves * kWave)) ||
         (s = dalloc(e, &e->mbox, (size_t)cfg.client_waves * rows * kWave)) ||
-        (s = dalloc(e, &e->tab, cfg.n_items)) || (s = dalloc(e, &e->tmp, cfg.n_items)) ||
+        (s = dalloc(e, &e->stamps, cfg.n_items)) || (s = dalloc(e, &e->tmp, cfg.n_items)) ||
+        (s = dalloc(e, &e->vals, cfg.n_items)) ||
         (s = dalloc(e, &e->reply_v, (size_t)cfg.client_waves * kWave * 2 * kBankW))) {
         csmv_destroy(e);
         return s;
@@ -774,7 +768,7 @@ csmv_status csmv_destroy(csmv_engine *e)
     if (!e) return CSMV_E_ARG;
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     dfree(e->ht); dfree(e->slot); dfree(e->ctl); dfree(e->ring); dfree(e->done);
-    dfree(e->reply); dfree(e->mbox); dfree(e->tab); dfree(e->tmp);
+    dfree(e->reply); dfree(e->mbox); dfree(e->stamps); dfree(e->tmp); dfree(e->vals);
     dfree(e->reply_v); dfree(e->norm_list); dfree(e->hot_list);
     dfree(e->mc_key); dfree(e->mc_op); dfree(e->mc_cdf); dfree(e->mc_ts); dfree(e->mc_res);
     dfree(e->mc_vsum); dfree(e->mc_ab); dfree(e->mc_lat);
@@ -796,7 +790,7 @@ static csmv_status fill_pool(csmv_engine *e, const int32_t *dvals, int32_t v)
 {
     const uint32_t n = e->cfg.n_items;
     hipLaunchKernelGGL(pool_fill_kernel, dim3((n + 255) / 256), dim3(256), 0, e->stream, e->ht,
-                       e->slot, e->tab, n, e->cfg.depth, dvals, v);
+                       e->slot, e->vals, n, e->cfg.depth, dvals, v);
     HIPCHK(e, hipGetLastError());
     HIPCHK(e, hipStreamSynchronize(e->stream));
     e->pool_loaded = true;
@@ -959,7 +953,8 @@ static void base_ctx(csmv_engine *e, Ctx &c)
     c.reply = e->reply;
     c.done = e->done;
     c.mbox = e->mbox;
-    c.tab = e->tab;
+    c.stamps = e->stamps;
+    c.vals = e->vals;
     c.reply_v = e->reply_v;
     c.prof = (uint64_t *)(e->ctl + kOffProf);
     c.qmask = e->Q - 1;
@@ -985,11 +980,8 @@ static csmv_status reset_run_state(csmv_engine *e, const ServerLayout &L)
     HIPCHK(e, hipMemsetAsync(e->reply, 0, (size_t)e->cfg.client_waves * kWave * 8, e->stream));
     HIPCHK(e, hipMemsetAsync(e->reply_v, 0, (size_t)e->cfg.client_waves * kWave * 2 * kBankW * 8,
                              e->stream));
-    if ((e->cfg.variant & 3u) == kValStamp && !L.exact) {
-        hipLaunchKernelGGL(tab_clear_stamps_kernel, dim3((e->cfg.n_items + 255) / 256), dim3(256),
-                           0, e->stream, e->tab, e->cfg.n_items);
-        HIPCHK(e, hipGetLastError());
-    }
+    if ((e->cfg.variant & 3u) == kValStamp && !L.exact)
+        HIPCHK(e, hipMemsetAsync(e->stamps, 0, (size_t)e->cfg.n_items * 4, e->stream));
     return CSMV_OK;
 }
 
@@ -1030,8 +1022,6 @@ static csmv_status device_error(csmv_engine *e, uint32_t derr)
 {
     if (derr == kErrLivelock) return fail(e, CSMV_E_LIVELOCK, "transaction attempt limit");
     if (derr == kErrReadSetOverflow) return fail(e, CSMV_E_CAPACITY, "read/write set overflow");
-    if (derr == kErrWireShape)
-        return fail(e, CSMV_E_ARG, "a serialized-lane write is not one of the request's first reads");
     if (derr) return fail(e, CSMV_E_TIMEOUT, "device wait timed out, code " + std::to_string(derr));
     return CSMV_OK;
 }

@@ -165,29 +165,39 @@ struct LaneSum {
 
 // ---- committer --------------------------------------------------------------------------
 // Committed-value tables.  Exact mode (every item fits LDS): stamp and value of every
-// item in LDS.  Hashed mode: the exact records are in HBM (c.tab, written only
+// item in LDS.  Hashed mode: the exact tables are in HBM (c.stamps, c.vals, written only
 // by the committer, so they hold every commit's value and stamp for this wave's own later
 // loads) and LDS holds the stamp buckets that filter the normal lane's validation.
 struct ValueTables {
     uint32_t *bucket, *lval;
 };
 
-// Records the commits of the writes in on: the stamp bucket, and the item's value and stamp
-// (the exact LDS table, or the item's HBM record).
 template <int R, int W>
-__device__ __forceinline__ void record_commits(const Ctx &c, const ServerLayout &L, const ValueTables &V,
-                                               const uint32_t *wa, const int32_t *v, const bool *on,
-                                               uint32_t ts)
+__device__ __forceinline__ void record_values(const Ctx &c, const ServerLayout &L, const ValueTables &V,
+                                              const uint32_t *wa, const int32_t *v, const bool *on)
 {
 #pragma unroll
     for (int w = 0; w < W; ++w)
         if (on[w] && wa[w] != kNone) {
-            __hip_atomic_fetch_max(&V.bucket[bucket_of(L, wa[w])], ts, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_WORKGROUP);
             // (the HBM store is an atomic one so that the two branches cannot be merged
             // into one flat store, whose completion later LDS waits would also wait for)
             if (L.exact) V.lval[wa[w]] = (uint32_t)v[w];
-            else tab_st(c, wa[w], ts, v[w]);
+            else __hip_atomic_store(&c.vals[wa[w]], v[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+}
+
+template <int R, int W>
+__device__ __forceinline__ void record_stamps(const Ctx &c, const ServerLayout &L, const ValueTables &V,
+                                              const uint32_t *wa, uint32_t ts, bool on)
+{
+#pragma unroll
+    for (int w = 0; w < W; ++w)
+        if (on && wa[w] != kNone) {
+            __hip_atomic_fetch_max(&V.bucket[bucket_of(L, wa[w])], ts, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (!L.exact)
+                __hip_atomic_fetch_max(&c.stamps[wa[w]], ts, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
         }
 }
 
@@ -378,17 +388,22 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
             const uint32_t word = kWide ? box[(RW::prod + r / 4) * kWave + lane] : prod[kWide ? 0 : r / 4];
             return (word >> (8 * (r % 4))) & 0xFFu;
         };
-        // hashed mode: the HBM records (value and stamp, which hold this wave's own earlier
-        // stores) of the chain nodes, issued as soon as the request's rows are in
-        // registers.  A serialized-lane request writes its first reads (the transfer
-        // operands), so the records of the first W nodes give the writes' stamps too.
-        uint64_t hr[RO];
+        // hashed mode: the committed values of the chain nodes and the stamps of the
+        // writes from the HBM tables (which hold this wave's own earlier stores), issued
+        // as soon as the request's rows are in registers
+        int32_t hv[RO];
+        uint32_t hs[W];
         if (delegate && !L.exact) {
 #pragma unroll
             for (int r = 0; r < RO; ++r) {
                 const uint32_t x = RS(r);
-                hr[r] = tab_ld(c, x != kNone ? x : 0u);
+                hv[r] = __hip_atomic_load(&c.vals[x != kNone ? x : 0u], __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_WORKGROUP);
             }
+#pragma unroll
+            for (int w = 0; w < W; ++w)
+                hs[w] = __hip_atomic_load(&c.stamps[wa[w] != kNone ? wa[w] : 0u], __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_WORKGROUP);
         }
         // mailbox consumed; every stamp <= last is in the HBM tables for the waves of this
         // workgroup that acquire this release
@@ -413,12 +428,6 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
             uint32_t ops[RO];
 #pragma unroll
             for (int r = 0; r < RO; ++r) ops[r] = RS(r);
-            // the write set is the transfer operands (write w is read w), which the loads
-            // of the HBM records rely on
-            bool odd_wire = false;
-#pragma unroll
-            for (int w = 0; w < W; ++w) odd_wire |= valid && wa[w] != kNone && wa[w] != ops[w];
-            if (ballot(odd_wire)) raise_error(c, kErrWireShape);
             bool is_root[RO];
 #pragma unroll
             for (int r = 0; r < RO; ++r) is_root[r] = valid && ops[r] != kNone && root[r] == lane * RO + r;
@@ -439,13 +448,12 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
 #pragma unroll
                 for (int w = 0; w < W; ++w) prev[w] = (valid && wa[w] != kNone) ? ls[w] : kNone;
             } else {
-                // the committed value and stamp: the HBM records hold those of every commit
+                // the committed value and stamp: the HBM tables hold those of every commit
                 // (this wave's own stores included), loaded when the request was read
 #pragma unroll
-                for (int r = 0; r < RO; ++r) base[r] = is_root[r] ? (int32_t)(uint32_t)hr[r] : 0;
+                for (int r = 0; r < RO; ++r) base[r] = is_root[r] ? hv[r] : 0;
 #pragma unroll
-                for (int w = 0; w < W; ++w)
-                    prev[w] = (valid && wa[w] != kNone) ? (uint32_t)(hr[w] >> 32) : kNone;
+                for (int w = 0; w < W; ++w) prev[w] = (valid && wa[w] != kNone) ? hs[w] : kNone;
             }
             tres = phase_clock();
 #pragma unroll
@@ -501,7 +509,8 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
                     } else {
 #pragma unroll
                         for (int j = 0; j < 16; ++j)
-                            v[j] = tab_value_at(c, ((need >> j) & 1u) ? x[j] : 0u);
+                            v[j] = __hip_atomic_load(&c.vals[((need >> j) & 1u) ? x[j] : 0u], __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_WORKGROUP);
                         pin_round(v);
                     }
 #pragma unroll
@@ -518,19 +527,15 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
                     }
                 }
             }
-            // every write stamps its bucket; the last write of each item in the request (the
-            // installed one, with the request's highest stamp for that item) records the
-            // item's value and stamp
+            record_values<R, W>(c, L, V, wa, nv, inst);
 #pragma unroll
             for (int w = 0; w < W; ++w)
-                if (okx && wa[w] != kNone)
+                if (okx && wa[w] != kNone) {
                     __hip_atomic_fetch_max(&V.bucket[bkw[w]], myts, __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_WORKGROUP);
-#pragma unroll
-            for (int w = 0; w < W; ++w)
-                if (inst[w]) {
-                    if (L.exact) V.lval[wa[w]] = (uint32_t)nv[w];
-                    else tab_st(c, wa[w], myts, nv[w]);
+                    if (!L.exact)
+                        __hip_atomic_fetch_max(&c.stamps[wa[w]], myts, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP);
                 }
             if (okx) {
                 // a transfer logs its timestamp and every read; an audit its snapshot and
@@ -571,7 +576,8 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
                             const uint32_t x = r < R ? RS(r) : wa[r - R];
                             ex[r] = 0;
                             if (x != kNone && V.bucket[bucket_of(L, x)] > tg)
-                                ex[r] = tab_stamp_at(c, x);
+                                ex[r] = __hip_atomic_load(&c.stamps[x], __ATOMIC_RELAXED,
+                                                          __HIP_MEMORY_SCOPE_WORKGROUP);
                         }
                     } else {
                         // the flagged items' stamps in one batch of loads (an item not
@@ -595,7 +601,8 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
 #pragma unroll
                             for (int r = 0; r < R + W; ++r) {
                                 const uint32_t x = r < R ? RS(r) : wa[r - R];
-                                ex[r] = tab_stamp_at(c, fl[r] ? x : 0u);
+                                ex[r] = __hip_atomic_load(&c.stamps[fl[r] ? x : 0u], __ATOMIC_RELAXED,
+                                                          __HIP_MEMORY_SCOPE_WORKGROUP);
                             }
                             pin_round(ex);
 #pragma unroll
@@ -693,7 +700,8 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
                     bool on[W];
 #pragma unroll
                     for (int w = 0; w < W; ++w) on[w] = true;
-                    record_commits<R, W>(c, L, V, wa, wv, on, ts);
+                    record_stamps<R, W>(c, L, V, wa, ts, true);
+                    record_values<R, W>(c, L, V, wa, wv, on);
                 } else {
 #pragma unroll
                     for (int w = 0; w < W; ++w) atr[w * E + (ts & emask)] = wa[w];
@@ -1018,7 +1026,8 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
 #pragma unroll
                     for (int j = 0; j < kChunk; ++j) {
                         const bool want = valid && x[j] != kNone;
-                        st[j] = tab_stamp_at(c, want ? x[j] : 0u);
+                        st[j] = __hip_atomic_load(&c.stamps[want ? x[j] : 0u], __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_WORKGROUP);
                     }
                     pin_round(st);
 #pragma unroll
@@ -1070,7 +1079,8 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
             for (int w = 0; w < W; ++w) {
                 const uint32_t x = wa[w];
                 const uint32_t st = (!L.exact && !delegate && valid && x != kNone)
-                                        ? tab_stamp_at(c, x)
+                                        ? __hip_atomic_load(&c.stamps[x], __ATOMIC_RELAXED,
+                                                            __HIP_MEMORY_SCOPE_WORKGROUP)
                                         : 0u;
                 p = st > p ? st : p;
                 if (valid && x != kNone) {
@@ -1111,7 +1121,8 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
                     uint32_t v[R + W];
     #pragma unroll
                     for (int r = 0; r < R + W; ++r)
-                        v[r] = tab_stamp_at(c, it[r] != kNone ? it[r] : 0u);
+                        v[r] = __hip_atomic_load(&c.stamps[it[r] != kNone ? it[r] : 0u], __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_WORKGROUP);
                     pin_round(v);
                     uint32_t p = 0;
     #pragma unroll
@@ -1285,7 +1296,7 @@ __device__ __forceinline__ void server_workgroup(const Ctx &c, const ServerLayou
         for (uint32_t i = threadIdx.x; i < L.nbuckets; i += blockDim.x) lds[L.bucket_off + i] = 0;
         if (L.exact)
             for (uint32_t i = threadIdx.x; i < c.n_items; i += blockDim.x)
-                lds[L.val_off + i] = (uint32_t)c.tab[i];
+                lds[L.val_off + i] = (uint32_t)c.vals[i];
     }
     const uint32_t M = c.wg_waves - 2;
     for (uint32_t r = 0; r < M; ++r)

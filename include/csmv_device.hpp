@@ -137,7 +137,6 @@ enum : uint32_t {
     kErrDoneRingTimeout = 6,
     kErrLivelock = 7,
     kErrReadSetOverflow = 8,
-    kErrWireShape = 9,       // a serialized-lane write is not one of the first reads
 };
 
 // Engine view passed by value to every kernel.
@@ -158,8 +157,8 @@ struct Ctx {
     uint64_t *reply;         // [client_waves * 64] (n+1) << 32 | commit ts
     uint64_t *done;          // [Q] (n+1) << 32 | last ts of batch n (write-back finished)
     uint32_t *mbox;          // [client_waves][rows][64] request mailboxes
-    uint64_t *tab;           // [n_items] stamp << 32 | value of each item's newest commit
-                             // (server-private, STAMP mode; written by the committer only)
+    uint32_t *stamps;        // [n_items] commit stamps (server-private, STAMP mode)
+    int32_t *vals;           // [n_items] newest committed values (server-private)
     uint64_t *reply_v;       // [client_waves * 64 * W] (n+1) << 32 | new value (delegated)
     uint64_t *prof;          // kProf* timing counters (wall_clock64 ticks)
     uint32_t qmask;
@@ -263,29 +262,6 @@ __device__ __forceinline__ void raise_error(const Ctx &c, uint32_t code)
     uint32_t expected = 0;
     __hip_atomic_compare_exchange_strong(c.err, &expected, code, __ATOMIC_RELAXED,
                                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// The committed-value table: one 64-bit record per item, read whole by the committer (one
-// load gives an operand's value and, for a write, its predecessor's stamp) and by halves
-// elsewhere (little-endian: the value is the low word, the stamp the high one).
-__device__ __forceinline__ uint64_t tab_ld(const Ctx &c, uint32_t x)
-{
-    return __hip_atomic_load(&c.tab[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-__device__ __forceinline__ int32_t tab_value_at(const Ctx &c, uint32_t x)
-{
-    return __hip_atomic_load(reinterpret_cast<const int32_t *>(c.tab) + 2 * (size_t)x,
-                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-__device__ __forceinline__ uint32_t tab_stamp_at(const Ctx &c, uint32_t x)
-{
-    return __hip_atomic_load(reinterpret_cast<const uint32_t *>(c.tab) + 2 * (size_t)x + 1,
-                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-__device__ __forceinline__ void tab_st(const Ctx &c, uint32_t x, uint32_t stamp, int32_t v)
-{
-    __hip_atomic_store(&c.tab[x], ((uint64_t)stamp << 32) | (uint32_t)v, __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 // ---- per-lane transaction state ----------------------------------------------------------
