@@ -51,3 +51,44 @@ def test_bad_config_is_rejected():
     with pytest.raises(csmv.CsmvError) as ei:
         csmv.Engine(4096, atr_entries=1000)   # not a power of two
     assert ei.value.status in (2, 5)
+
+
+STRUCTS = {  # ctypes mirror -> C type in include/csmv.h
+    "Config": "csmv_config",
+    "BankSpec": "csmv_bank_spec",
+    "MemcdSpec": "csmv_memcd_spec",
+    "Stats": "csmv_stats",
+    "ServeSpec": "csmv_serve_spec",
+    "ServeStats": "csmv_serve_stats",
+}
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """Every ctypes mirror has the size and field offsets that the C compiler gives the
+    header's struct (a field added on one side only shows up here)."""
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("no C compiler")
+    lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "csmv.h"', "int main(void) {"]
+    for py, c in STRUCTS.items():
+        lines.append(f'printf("{py} size %zu\\n", sizeof({c}));')
+        for name, _ in getattr(csmv, py)._fields_:
+            lines.append(f'printf("{py} {name} %zu\\n", offsetof({c}, {name}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                   check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    seen = 0
+    for line in out:
+        if not line:
+            continue
+        py, name, val = line.split()
+        st = getattr(csmv, py)
+        got = ctypes.sizeof(st) if name == "size" else getattr(st, name).offset
+        assert got == int(val), (py, name, got, val)
+        seen += 1
+    assert seen == sum(1 + len(getattr(csmv, p)._fields_) for p in STRUCTS)
