@@ -63,7 +63,7 @@ class Config(C.Structure):
                 ("timeout_ms", C.c_uint32), ("max_attempts", C.c_uint32),
                 ("stage_ahead", C.c_uint32), ("lds_kib", C.c_uint32),
                 ("n_serial_waves", C.c_uint32), ("begin_wait", C.c_uint32),
-                ("client_wg_waves", C.c_uint32)]
+                ("client_wg_waves", C.c_uint32), ("escalate_after", C.c_uint32)]
 
 
 class BankSpec(C.Structure):
@@ -87,7 +87,8 @@ class Stats(C.Structure):
                 ("error", C.c_uint32), ("client_lanes", C.c_uint32),
                 ("prof_ns", C.c_double * 16), ("stamp_buckets", C.c_uint32),
                 ("stamp_exact", C.c_uint32), ("aborts_unknown", C.c_uint64),
-                ("lds_bytes", C.c_uint32), ("reserved", C.c_uint32)]
+                ("lds_bytes", C.c_uint32), ("reserved", C.c_uint32),
+                ("escalated", C.c_uint64)]
     PROF = ("commit_idle", "commit_busy", "recv_window", "recv_ring", "recv_busy",
             "client_reply", "client_round", "client_rounds", "commit_read", "commit_resolve",
             "commit_chain", "commit_finish")
@@ -239,12 +240,12 @@ class Engine:
                  client_waves: int = 64, wg_waves: int = 8, variant="stamp",
                  timeout_ms: int = 4000, max_attempts: int = 0, stage_ahead: int = 0,
                  lds_kib: int = 0, n_serial_waves: int = 0, begin_wait: bool = True,
-                 client_wg_waves: int = 0):
+                 client_wg_waves: int = 0, escalate_after: int = 0):
         if isinstance(variant, str):
             variant = VARIANTS[variant]
         self.cfg = Config(n_items, depth, atr_entries, client_waves, wg_waves, variant,
                           timeout_ms, max_attempts, stage_ahead, lds_kib, n_serial_waves,
-                          1 if begin_wait else 0, client_wg_waves)
+                          1 if begin_wait else 0, client_wg_waves, escalate_after)
         self.n_items = n_items
         self._h = C.c_void_p()
         st = lib().csmv_create(C.byref(self.cfg), C.byref(self._h))

@@ -124,10 +124,48 @@ __global__ void __launch_bounds__(256) pool_head_kernel(const uint32_t *ht, cons
 }
 
 // ---- bank driver (client wavefronts) -----------------------------------------------------------
+// A bank transaction as the serialized lane carries it (the commit server executes it in
+// commit order): the operands as the read set, the transfer pair as the write set, and
+// the stream index and amount (kReadOnlyAmt for an audit) in the write-value rows.
+template <int R, int W>
+__device__ __forceinline__ void delegated_tx(const Ctx &c, Tx<R, W> &tx, bool on, uint64_t k,
+                                             uint32_t op)
+{
+    tx.snap = on ? 0u : kNone;
+    tx.nr = 0;
+    tx.nw = 0;
+    tx.aborted = false;
+    tx.overflow = false;
+#pragma unroll
+    for (int r = 0; r < R; ++r) tx.rs[r] = kNone;
+#pragma unroll
+    for (int w = 0; w < W; ++w) { tx.wa[w] = kNone; tx.wv[w] = 0; }
+    const uint32_t amt = (op >> 8) & 0xFFu, nr = (op >> 16) & 0xFFu;
+    const bool audit = (op & 0xFFu) == kKindAudit;
+    if (on) {
+        // the operand rows in one batch of loads (a row past nr reloads row 0)
+        uint32_t ops[R];
+#pragma unroll
+        for (int j = 0; j < R; ++j) ops[j] = c.addr[(uint64_t)((uint32_t)j < nr ? j : 0) * c.n_txns + k];
+        pin_round(ops);
+#pragma unroll
+        for (int j = 0; j < R; ++j)
+            if ((uint32_t)j < nr) {
+                tx.rs[j] = ops[j];
+                if (j < 2 && !audit) tx_write(tx, ops[j], 0);
+            }
+        tx.nr = nr;
+        tx.wv[0] = (int32_t)(uint32_t)k;
+        tx.wv[1] = (int32_t)(audit ? kReadOnlyAmt : amt);
+    }
+}
+
 // client_kernel (bank/csmv_sh/exe.cu:89-284) on a resident stream: a lane holds one
 // transaction until it commits; lanes that finish are refilled with one fetch_add per
 // wave (ballot + prefix count).  Audits commit locally at their snapshot
-// (exe.cu:213-223); transfers go through tx_commit.
+// (exe.cu:213-223); transfers go through tx_commit.  With c.escalate set, a transaction
+// that has failed that many attempts is sent on the serialized lane instead (forward
+// progress: the reference retries in arrival order until it commits, exe.cu:142-269).
 template <int R, int W> __device__ __forceinline__ void bank_client(const Ctx &c, uint32_t cw)
 {
     const uint32_t lane = lane_id();
@@ -136,7 +174,7 @@ template <int R, int W> __device__ __forceinline__ void bank_client(const Ctx &c
     bool has = false, exhausted = false;
     uint64_t k = 0;
     uint32_t op = 0, ab = 0, my_last = 0;
-    uint64_t n_upd = 0, n_ro = 0, n_age = 0, t_reply = 0, t_round = 0, rounds = 0;
+    uint64_t n_upd = 0, n_ro = 0, n_age = 0, n_esc = 0, t_reply = 0, t_round = 0, rounds = 0;
     for (;;) {
         if (ld_rlx(c.err)) break;
         const uint64_t tr0 = wall_clock64();
@@ -160,6 +198,36 @@ template <int R, int W> __device__ __forceinline__ void bank_client(const Ctx &c
             }
         }
         if (ballot(has) == 0) break;
+
+        if (c.escalate) {
+            // this round's request carries the wave's escalated lanes (transfers, and audits
+            // whose reads fit the wire) on the serialized lane; the other lanes sit it out
+            const bool esc = has && ab >= c.escalate &&
+                             ((op & 0xFFu) == kKindTransfer || ((op >> 16) & 0xFFu) <= (uint32_t)R);
+            if (ballot(esc)) {
+                Tx<R, W> tx;
+                delegated_tx<R, W>(c, tx, esc, k, op);
+                bool fail = false;
+                const uint32_t ts = tx_commit<R, W>(c, cw, tx, esc, &fail, &t_reply, true);
+                if (fail) break;
+                const uint64_t cmu = ballot(ts != 0 && (ts & kSnapshotBit) == 0);
+                if (cmu) my_last = readlane(ts, 63u - (uint32_t)__builtin_clzll(cmu));
+                if (esc) {
+                    if (ts) {
+                        c.log_ab[k] = ab;
+                        has = false;
+                        ++n_esc;
+                        if (ts & kSnapshotBit) ++n_ro;
+                        else ++n_upd;
+                    } else {
+                        ++ab;
+                    }
+                }
+                t_round += wall_clock64() - tr0;
+                ++rounds;
+                continue;
+            }
+        }
 
         Tx<R, W> tx;
         if (!tx_begin(c, tx, c.begin_wait ? my_last : 0u)) break;
@@ -261,6 +329,7 @@ template <int R, int W> __device__ __forceinline__ void bank_client(const Ctx &c
     if (n_upd) fetch_add(&c.stats[kStatCommitUpd], n_upd);
     if (n_ro) fetch_add(&c.stats[kStatCommitRo], n_ro);
     if (n_age) fetch_add(&c.stats[kStatAbortAge], n_age);
+    if (n_esc) fetch_add(&c.stats[kStatEscalated], n_esc);
     fence_rel();
     if (lane == 0) __hip_atomic_fetch_add(c.clients_done, 1u, __ATOMIC_RELEASE,
                                           __HIP_MEMORY_SCOPE_AGENT);
@@ -301,33 +370,7 @@ template <int R, int W> __device__ __forceinline__ void bank_serial_client(const
         }
         if (ballot(has) == 0) break;
         Tx<R, W> tx;
-        tx.snap = has ? 0u : kNone;
-        tx.nr = 0;
-        tx.nw = 0;
-        tx.aborted = false;
-        tx.overflow = false;
-#pragma unroll
-        for (int r = 0; r < R; ++r) tx.rs[r] = kNone;
-#pragma unroll
-        for (int w = 0; w < W; ++w) { tx.wa[w] = kNone; tx.wv[w] = 0; }
-        const uint32_t amt = (op >> 8) & 0xFFu, nr = (op >> 16) & 0xFFu;
-        const bool audit = (op & 0xFFu) == kKindAudit;
-        if (has) {
-            // the operand rows in one batch of loads (a row past nr reloads row 0)
-            uint32_t ops[R];
-#pragma unroll
-            for (int j = 0; j < R; ++j) ops[j] = c.addr[(uint64_t)((uint32_t)j < nr ? j : 0) * c.n_txns + k];
-            pin_round(ops);
-#pragma unroll
-            for (int j = 0; j < R; ++j)
-                if ((uint32_t)j < nr) {
-                    tx.rs[j] = ops[j];
-                    if (j < 2 && !audit) tx_write(tx, ops[j], 0);
-                }
-            tx.nr = nr;
-            tx.wv[0] = (int32_t)(uint32_t)k;
-            tx.wv[1] = (int32_t)(audit ? kReadOnlyAmt : amt);
-        }
+        delegated_tx<R, W>(c, tx, has, k, op);
         bool fail = false;
         const uint32_t ts = tx_commit<R, W>(c, cw, tx, has, &fail, &t_reply, true);
         if (fail) break;
@@ -711,7 +754,8 @@ csmv_status csmv_create(const csmv_config *cfg_in, csmv_engine **out)
         cfg.wg_waves < 3 || cfg.wg_waves > 8 || cfg.client_wg_waves > cfg.wg_waves ||
         cfg.stage_ahead > 64 || cfg.atr_entries < 64 ||
         (cfg.atr_entries & (cfg.atr_entries - 1)) || cfg.atr_entries > (1u << 16) ||
-        (cfg.variant & ~0xFu) || (cfg.variant & 3u) == 3u)
+        (cfg.variant & ~0xFu) || (cfg.variant & 3u) == 3u ||
+        (cfg.escalate_after && cfg.variant != (kValStamp | kInsertBatch)))
         return CSMV_E_ARG;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return CSMV_E_NODEVICE;
@@ -968,6 +1012,7 @@ static void base_ctx(csmv_engine *e, Ctx &c)
     c.begin_wait = e->cfg.begin_wait;
     c.timeout_ticks = (uint64_t)e->cfg.timeout_ms * e->wallclock_khz;
     c.max_attempts = e->cfg.max_attempts;
+    c.escalate = e->cfg.escalate_after;
     c.next_hot = (uint64_t *)(e->ctl + kOffNextHot);
 }
 
@@ -1014,6 +1059,7 @@ static csmv_status collect_stats(csmv_engine *e, const ServerLayout &L, float ms
         out->stamp_exact = L.exact;
         out->aborts_unknown = st[kStatAbortUnknown];
         out->lds_bytes = (uint32_t)L.words * 4;
+        out->escalated = st[kStatEscalated];
     }
     return CSMV_OK;
 }
@@ -1701,7 +1747,9 @@ csmv_status csmv_serve_run(csmv_engine *cache, csmv_engine *bank, const csmv_ser
         const bool serial = bank->routed && bank->n_hot > 0;
         if (serial && bank->cfg.n_serial_waves == 0)
             return fail(bank, CSMV_E_ARG, "a routed bank partition needs n_serial_waves > 0");
-        if ((st = fit_layout<kServeBankR, kBankW>(bank, Lb, serial))) return st;
+        // escalated transactions travel on the serialized lane too
+        const bool delegation = serial || bank->cfg.escalate_after != 0;
+        if ((st = fit_layout<kServeBankR, kBankW>(bank, Lb, delegation))) return st;
         base_ctx(bank, cb);
         cb.n_serial_waves = serial ? bank->cfg.n_serial_waves : 0;
         cb.hot_list = serial ? bank->hot_list : nullptr;

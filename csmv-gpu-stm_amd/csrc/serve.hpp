@@ -163,6 +163,28 @@ __device__ __forceinline__ uint32_t pool_read_some(const Ctx &c, const uint32_t 
     return newer;
 }
 
+// A serving-loop bank transaction as the serialized lane carries it: the transfer pair as
+// the write set, the stream index and amount in the write-value rows; the read set stays
+// in the stream (tx_commit copies its rows into the mailbox).
+template <int R, int W>
+__device__ __forceinline__ void serve_delegated_tx(const Ctx &c, Tx<R, W> &tx, bool on, uint64_t k,
+                                                   uint32_t op)
+{
+    tx.snap = on ? 0u : kNone;
+    tx.nw = 0;
+    tx.aborted = false;
+    tx.overflow = false;
+#pragma unroll
+    for (int w = 0; w < W; ++w) { tx.wa[w] = kNone; tx.wv[w] = 0; }
+    tx.nr = on ? min((op >> 16) & 0xFFu, (uint32_t)R) : 0u;
+    if (on) {
+        tx_write(tx, c.addr[k], 0);
+        tx_write(tx, c.addr[c.n_txns + k], 0);
+        tx.wv[0] = (int32_t)(uint32_t)k;
+        tx.wv[1] = (int32_t)((op >> 8) & 0xFFu);
+    }
+}
+
 // Serving-loop bank transaction: a transfer over the first two of nr accounts
 // (2 <= nr <= R) that also reads the other nr - 2 (bank/csmv_sh/exe.cu:184-201 widened
 // to the serving mix's read sets).  Same commit path and log as the bank driver, plus
@@ -176,7 +198,7 @@ template <int R, int W> __device__ __forceinline__ void bank_serve_client(const 
     bool has = false, adm = false, exhausted = false;
     uint64_t k = 0;
     uint32_t op = 0, ab = 0, my_last = 0, held = 0;
-    uint64_t n_upd = 0, n_age = 0, stalls = 0, t_reply = 0, t_round = 0, rounds = 0;
+    uint64_t n_upd = 0, n_age = 0, n_esc = 0, stalls = 0, t_reply = 0, t_round = 0, rounds = 0;
     for (;;) {
         if (ld_rlx(c.err)) break;
         const uint64_t tr0 = wall_clock64();
@@ -206,6 +228,38 @@ template <int R, int W> __device__ __forceinline__ void bank_serve_client(const 
         if (ballot(act) == 0) {
             __builtin_amdgcn_s_sleep(2);
             continue;
+        }
+        if (c.escalate) {
+            // forward progress: transactions that have failed c.escalate attempts go to the
+            // serialized lane in this round's request; the other lanes sit it out
+            const bool esc = act && ab >= c.escalate;
+            if (ballot(esc)) {
+                Tx<R, W> tx;
+                serve_delegated_tx<R, W>(c, tx, esc, k, op);
+                bool fail = false;
+                const uint32_t ts = tx_commit<R, W, W, true>(c, cw, tx, esc, &fail, &t_reply, true,
+                                                             nullptr, nullptr, -1, c.addr + k, T);
+                if (fail) break;
+                const uint64_t cmu = ballot(ts != 0);
+                if (cmu) my_last = readlane(ts, 63u - (uint32_t)__builtin_clzll(cmu));
+                bool done = false;
+                if (esc) {
+                    if (ts) {
+                        c.log_ab[k] = ab;
+                        c.log_lat[k] = (uint32_t)(wall_clock64() - t0 - c.arr[k]);
+                        done = true;
+                        has = false;
+                        ++n_upd;
+                        ++n_esc;
+                    } else {
+                        ++ab;
+                    }
+                }
+                serve_release(c, done, held);
+                t_round += wall_clock64() - tr0;
+                ++rounds;
+                continue;
+            }
         }
 
         Tx<R, W> tx;
@@ -280,6 +334,7 @@ template <int R, int W> __device__ __forceinline__ void bank_serve_client(const 
     }
     if (n_upd) fetch_add(&c.stats[kStatCommitUpd], n_upd);
     if (n_age) fetch_add(&c.stats[kStatAbortAge], n_age);
+    if (n_esc) fetch_add(&c.stats[kStatEscalated], n_esc);
     fence_rel();
     if (lane == 0) __hip_atomic_fetch_add(c.clients_done, 1u, __ATOMIC_RELEASE,
                                           __HIP_MEMORY_SCOPE_AGENT);
@@ -330,20 +385,7 @@ template <int R, int W> __device__ __forceinline__ void bank_serve_serial_client
             continue;
         }
         Tx<R, W> tx;
-        tx.snap = act ? 0u : kNone;
-        tx.nw = 0;
-        tx.aborted = false;
-        tx.overflow = false;
-#pragma unroll
-        for (int w = 0; w < W; ++w) { tx.wa[w] = kNone; tx.wv[w] = 0; }
-        const uint32_t amt = (op >> 8) & 0xFFu;
-        tx.nr = act ? min((op >> 16) & 0xFFu, (uint32_t)R) : 0u;
-        if (act) {
-            tx_write(tx, c.addr[k], 0);
-            tx_write(tx, c.addr[T + k], 0);
-            tx.wv[0] = (int32_t)(uint32_t)k;
-            tx.wv[1] = (int32_t)amt;
-        }
+        serve_delegated_tx<R, W>(c, tx, act, k, op);
         bool fail = false;
         const uint32_t ts = tx_commit<R, W, W, true>(c, cw, tx, act, &fail, &t_reply, true, nullptr,
                                                      nullptr, -1, c.addr + k, T);

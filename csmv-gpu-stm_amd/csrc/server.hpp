@@ -946,8 +946,13 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
         const uint32_t mbi = n % L.nmb;
         const uint64_t t0 = wall_clock64();
         // stage request n once its mailbox is free and the committer is at most
-        // stage_ahead requests behind it (a short window keeps T_g fresh)
-        const uint32_t need = n + 1 > L.nmb ? n + 1 - L.nmb : 0;
+        // stage_ahead requests behind it (a short window keeps T_g fresh).  The committer
+        // publishes its progress as soon as a narrow wire's rows are in registers, but it
+        // keeps reading a wide wire's mailbox (read rows, producer codes) until the
+        // request is decided: a wide mailbox is free once the committer has taken the
+        // request after the one that used it.
+        constexpr uint32_t kLag = kWide ? 2u : 1u;
+        const uint32_t need = n + kLag > L.nmb ? n + kLag - L.nmb : 0;
         const uint32_t need2 = n > c.stage_ahead ? n - c.stage_ahead : 0;
         const uint32_t want = need > need2 ? need : need2;
         if (want) {

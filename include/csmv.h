@@ -71,6 +71,15 @@ typedef struct {
     uint32_t client_wg_waves;  /* client waves per client workgroup, 1..wg_waves (0 = wg_waves):
                                   fewer spread the clients over more CUs, each with its own
                                   memory pipeline; the other waves of the workgroup leave */
+    uint32_t escalate_after;   /* forward progress of the bank drivers' normal lane: a
+                                  transaction that has failed this many attempts is sent on
+                                  the serialized lane in its wave's next request (executed
+                                  by the commit server in commit order, so it cannot abort).
+                                  0 = never: pure arrival order with unchanged operands,
+                                  bounded by max_attempts (the reference retries in arrival
+                                  order, bank/csmv_sh/exe.cu:142-269).  Needs the STAMP
+                                  variant with client write-back; the cache driver has no
+                                  serialized lane and ignores it. */
 } csmv_config;
 
 typedef struct {
@@ -111,6 +120,7 @@ typedef struct {
     uint64_t aborts_unknown;     /* serialized lane: an operand's committed value was not at hand */
     uint32_t lds_bytes;          /* server workgroup LDS */
     uint32_t reserved;
+    uint64_t escalated;          /* normal-lane transactions escalated to the serialized lane */
 } csmv_stats;
 
 const char *csmv_status_str(csmv_status s);

@@ -170,6 +170,7 @@ struct Ctx {
     uint32_t begin_wait;     // a wave's next snapshot covers its own last commit
     uint64_t timeout_ticks;
     uint32_t max_attempts;
+    uint32_t escalate;       // failed attempts after which a bank transaction is delegated (0 = never)
     // resident bank stream and its commit log
     const uint32_t *op;
     const uint32_t *addr;
@@ -208,6 +209,7 @@ enum : uint32_t {
     kStatBatches,
     kStatFinalTs,
     kStatAbortUnknown,   // serialized lane: an operand's committed value was not at hand
+    kStatEscalated,      // normal-lane transactions sent on the serialized lane
     kStatCount
 };
 
@@ -586,7 +588,11 @@ __device__ __forceinline__ uint32_t tx_commit(const Ctx &c, uint32_t cw, Tx<R, W
     }
     *wait_ticks += wall_clock64() - tw0;
     if ((c.variant & kServerWriteback) == 0) {
-        const uint64_t cm = ballot(ts != 0);
+        // The completion word of a request is written by exactly one side: the committer
+        // when no lane committed an update (read-only lanes of the serialized lane reply
+        // with their snapshot, tagged by kSnapshotBit, and install nothing), the client
+        // otherwise, after its installs.
+        const uint64_t cm = ballot(ts != 0 && (ts & kSnapshotBit) == 0);
         if (cm) {
             // (the serialized lane has installed every write above)
             if (delegate) {
@@ -595,10 +601,9 @@ __device__ __forceinline__ uint32_t tx_commit(const Ctx &c, uint32_t cw, Tx<R, W
             } else {
                 writeback<R, W>(c, tx.wa, tx.wv, tx.nw, ts);
             }
-            // a read-only lane of the serialized lane replies with its snapshot, which is
-            // the request's last timestamp when it is the highest committed lane
+            // the highest committed update holds the request's last timestamp
             const uint32_t last_lane = 63u - (uint32_t)__builtin_clzll(cm);
-            const uint32_t end_ts = readlane(ts & ~kSnapshotBit, last_lane);
+            const uint32_t end_ts = readlane(ts, last_lane);
             fence_rel();
             if (lane == 0) st_rlx(&c.done[n & c.qmask], ((n + 1) << 32) | end_ts);
         }

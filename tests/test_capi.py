@@ -53,6 +53,16 @@ def test_bad_config_is_rejected():
     assert ei.value.status in (2, 5)
 
 
+def test_escalation_needs_the_stamp_preset():
+    """Escalated transactions are executed on the serialized lane, which reads the stamp
+    variant's committed-value tables: another variant with escalation is rejected before
+    any device call."""
+    for v in ("vanilla", "batchedAtrInsert", "clientWriteback", "collabValidation"):
+        with pytest.raises(csmv.CsmvError) as ei:
+            csmv.Engine(4096, variant=v, escalate_after=2)
+        assert ei.value.status == 2
+
+
 STRUCTS = {  # ctypes mirror -> C type in include/csmv.h
     "Config": "csmv_config",
     "BankSpec": "csmv_bank_spec",

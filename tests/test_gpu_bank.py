@@ -17,14 +17,16 @@ INIT = 1000
 
 def run_bank(n_items, bank, variant="stamp", depth=4, atr=4096, waves=8, wg_waves=8,
              check_stream=True, timeout_ms=4000, route=None, serial_waves=0, begin_wait=True,
-             client_wg_waves=0):
+             client_wg_waves=0, escalate_after=0, max_attempts=0):
     print(f"\n[case] n_items={n_items} bank={bank} variant={variant} depth={depth} atr={atr} "
           f"waves={waves} serial_waves={serial_waves} routed={route is not None} "
-          f"client_wg_waves={client_wg_waves}", flush=True)
+          f"client_wg_waves={client_wg_waves} escalate_after={escalate_after} "
+          f"max_attempts={max_attempts}", flush=True)
     with csmv.Engine(n_items, depth=depth, atr_entries=atr, client_waves=waves,
                      wg_waves=wg_waves, variant=variant, timeout_ms=timeout_ms,
                      n_serial_waves=serial_waves, begin_wait=begin_wait,
-                     client_wg_waves=client_wg_waves) as e:
+                     client_wg_waves=client_wg_waves, escalate_after=escalate_after,
+                     max_attempts=max_attempts) as e:
         e.fill(INIT)
         e.bank_generate(bank)
         if route is not None:
@@ -43,7 +45,7 @@ def run_bank(n_items, bank, variant="stamp", depth=4, atr=4096, waves=8, wg_wave
         assert np.array_equal(oaddr, addr), "operand stream differs from the oracle's"
     rep, _ = orc.bank_replay(spec, op, addr, INIT, ts, v, final)
     print("[stats]", {k: stats[k] for k in ("commits", "aborts", "abort_rate", "kernel_ms",
-                                             "txn_per_s", "batches")}, flush=True)
+                                             "txn_per_s", "batches", "escalated")}, flush=True)
     assert rep["fails"] == 0, rep
     assert rep["total"] == n_items * INIT
     assert stats["commits"] == bank.n_txns
@@ -65,6 +67,52 @@ def test_bank_zipf_small(variant):
     """Config 'bank at scale' shape at 2^16 accounts: Zipf 0.99, 4 reads / 2 writes."""
     run_bank(1 << 16, csmv.Bank(n_txns=1 << 15, seed=5, n_reads=4, n_reads_ro=4, zipf=True),
              variant=variant, depth=8, waves=16)
+
+
+BANK_AT_SCALE = dict(n_reads=4, n_reads_ro=4, pct_ro=10, zipf=True, theta=0.99)
+
+
+@pytest.mark.parametrize("variant", ["vanilla", "batchedAtrInsert", "clientWriteback",
+                                     "collabValidation"])
+def test_bank_at_scale_reference_variants(variant):
+    """The bank-at-scale configuration's four reference variants at its shape: 2^20
+    accounts, Zipf 0.99, transfers reading 4 / writing 2, 4-read audits, 90/10, depth 8,
+    record ring 4096, 64 client waves in arrival order, on a 2^14-transaction stream (long
+    enough that most attempts abort)."""
+    st = run_bank(1 << 20, csmv.Bank(n_txns=1 << 14, seed=13, **BANK_AT_SCALE),
+                  variant=variant, depth=8, atr=4096, waves=64, wg_waves=5, client_wg_waves=1,
+                  timeout_ms=20000)
+    assert st["aborts"] > 0
+    assert st["escalated"] == 0
+
+
+def test_bank_at_scale_stamp_escalation():
+    """The stamp variant's normal lane at the bank-at-scale shape with forward progress:
+    a transaction that has failed two attempts is executed by the commit server on the
+    serialized lane.  The log replays and the escalated transactions are counted."""
+    st = run_bank(1 << 20, csmv.Bank(n_txns=1 << 16, seed=14, **BANK_AT_SCALE),
+                  variant="stamp", depth=8, waves=64, wg_waves=5, client_wg_waves=1,
+                  escalate_after=2, timeout_ms=20000)
+    assert st["escalated"] > 0
+
+
+def test_escalation_instead_of_livelock():
+    """A hot stream with an attempt bound of one: without escalation the run stops with
+    the attempt-limit error; with escalation after one failed attempt every transaction
+    commits and no transaction fails more than once."""
+    bank = csmv.Bank(n_txns=1 << 14, seed=15, n_reads=4, n_reads_ro=4, pct_ro=10, zipf=True,
+                     theta=1.2)
+    print("\n[case] arrival order, max_attempts=1 (expect the attempt-limit error)", flush=True)
+    with csmv.Engine(4096, depth=4, client_waves=16, wg_waves=6, variant="stamp",
+                     max_attempts=1) as e:
+        e.fill(INIT)
+        e.bank_generate(bank)
+        with pytest.raises(csmv.CsmvError) as ei:
+            e.bank_run()
+        assert ei.value.status == 7
+    st = run_bank(4096, bank, variant="stamp", depth=4, waves=16, wg_waves=6,
+                  escalate_after=1, max_attempts=1)
+    assert st["escalated"] > 0
 
 
 def test_record_age_aborts():

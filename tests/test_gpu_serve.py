@@ -37,8 +37,8 @@ def serve_bank(n, seed=7):
                      theta=0.99)
 
 
-def bank_spec(b):
-    return orc.bank_spec(N_ACC, b.n_txns, seed=b.seed, n_reads=b.n_reads,
+def bank_spec(b, n_acc=N_ACC):
+    return orc.bank_spec(n_acc, b.n_txns, seed=b.seed, n_reads=b.n_reads,
                          n_reads_ro=b.n_reads_ro, pct_ro=b.pct_ro, zipf=b.zipf, theta=b.theta,
                          amount_max=b.amount_max, n_reads_min=b.n_reads_min)
 
@@ -59,18 +59,18 @@ def check_cache(e, m, init_state=None):
     return log, final
 
 
-def check_bank(e, b):
+def check_bank(e, b, n_acc=N_ACC):
     op, addr = e.bank_read_stream()
     ts, v, ab = e.bank_read_log()
     final = e.read_state()
-    spec = bank_spec(b)
+    spec = bank_spec(b, n_acc)
     oop, oaddr = orc.bank_gen(spec)
     assert np.array_equal(op, oop) and np.array_equal(addr, oaddr), "stream differs"
     nr = (op >> 16) & 0xFF
     assert nr.min() >= 2 and nr.max() <= 64 and nr.max() > 32
     rep, _ = orc.bank_replay(spec, op, addr, INIT, ts, v, final)
     assert rep["fails"] == 0, rep
-    assert rep["total"] == N_ACC * INIT
+    assert rep["total"] == n_acc * INIT
     return final
 
 
@@ -213,3 +213,70 @@ def test_serve_serialized_lane_for_routed_bank_transactions():
                   f"{out['bank']['abort_rate']:.3f} p99 {np.percentile(lb, 99) / out['wallclock_khz']:.2f} ms",
                   flush=True)
     assert res[True][2] < res[False][2]
+
+
+def test_serve_at_config_shape():
+    """The serving configuration's shape: the cache configuration (2^17 sets x 8 ways,
+    10M-key Zipf 0.99 keyspace, 95/5 GET/SET) beside a 2^20-account Zipf 0.99 bank
+    partition with read sets of 2..64, at 50M req/s offered (80/20), with an in-flight
+    version budget and bank transactions escalated to the serialized lane after two failed
+    attempts.  Both partitions replay exactly, every request commits once after its
+    arrival, the budget is never exceeded and admission engages."""
+    n_acc = 1 << 20
+    m = csmv.Memcd(n_reqs=1 << 17, seed=1)
+    b = csmv.Bank(n_txns=1 << 14, seed=7, n_reads=64, n_reads_min=2, pct_ro=0, zipf=True,
+                  theta=0.99)
+    budget = 16384
+    print(f"\n[case] serving at the config shape: cache {m.n_reqs} reqs ({m.n_sets} sets x "
+          f"{m.n_ways} ways), bank {b.n_txns} txns over {n_acc} accounts, 50M req/s offered, "
+          f"budget {budget}, escalate_after=2", flush=True)
+    with cache_engine(m) as ec, csmv.Engine(n_acc, depth=8, client_waves=8, wg_waves=6,
+                                            variant="stamp", timeout_ms=20000,
+                                            escalate_after=2) as eb:
+        ec.fill(0)
+        ec.memcd_generate(m)
+        eb.fill(INIT)
+        eb.bank_generate(b)
+        out = csmv.serve(ec, eb, offered_rps=50e6, pct_cache=80, seed=11, pool_budget=budget)
+        print("[serve]", {k: out[k] for k in ("kernel_ms", "admit_stalls", "peak_inflight")},
+              "bank escalated", out["bank"]["escalated"], flush=True)
+        assert out["final_inflight"] == 0
+        assert 0 < out["peak_inflight"] <= budget
+        assert out["admit_stalls"] > 0
+        check_cache(ec, m)
+        check_bank(eb, b, n_acc)
+        _, lc = check_latency(ec, out["cache"], m.n_reqs)
+        _, lb = check_latency(eb, out["bank"], b.n_txns)
+        khz = out["wallclock_khz"]
+        print(f"[latency] cache p50 {np.percentile(lc, 50) / khz * 1e3:.1f} us p99 "
+              f"{np.percentile(lc, 99) / khz * 1e3:.1f} us | bank p50 "
+              f"{np.percentile(lb, 50) / khz * 1e3:.1f} us p99 "
+              f"{np.percentile(lb, 99) / khz * 1e3:.1f} us", flush=True)
+
+
+def test_serve_bank_escalation_bounds_latency():
+    """Without routing, bank transactions of the serving mix that fail two attempts go to
+    the serialized lane: every transaction commits, the log replays, and the bank p99
+    latency stays far below that of pure arrival order on the same arrivals."""
+    m = csmv.Memcd(n_reqs=1 << 14, seed=33, n_keys=20_000, n_sets=1 << 10, batch=4096)
+    b = serve_bank(1 << 13, seed=34)
+    p99 = {}
+    for esc in (0, 2):
+        print(f"\n[case] serve mix, bank escalate_after={esc}", flush=True)
+        with cache_engine(m) as ec, csmv.Engine(N_ACC, depth=8, client_waves=8, wg_waves=6,
+                                                variant="stamp", timeout_ms=20000,
+                                                escalate_after=esc) as eb:
+            ec.fill(0)
+            ec.memcd_generate(m)
+            eb.fill(INIT)
+            eb.bank_generate(b)
+            out = csmv.serve(ec, eb, offered_rps=5e6, pct_cache=80, seed=12)
+            check_cache(ec, m)
+            check_bank(eb, b)
+            _, lb = check_latency(eb, out["bank"], b.n_txns)
+            p99[esc] = np.percentile(lb, 99) / out["wallclock_khz"] * 1e3
+            print(f"[serve] escalate_after={esc}: bank abort {out['bank']['abort_rate']:.3f} "
+                  f"escalated {out['bank']['escalated']} p99 {p99[esc]:.1f} us", flush=True)
+            if esc:
+                assert out["bank"]["escalated"] > 0
+    assert p99[2] < p99[0]
