@@ -290,6 +290,9 @@ def main():
     ap.add_argument("--serve-cache-waves", type=int, default=16)
     ap.add_argument("--serial-waves-all", type=int, default=16,
                     help="serialized-lane waves of the run with every transaction routed")
+    ap.add_argument("--escalate-after", type=int, default=2,
+                    help="failed attempts after which the stamp_escalate variant line sends a "
+                         "transaction on the serialized lane")
     ap.add_argument("--no-route-all", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--no-memcd", action="store_true")
@@ -310,11 +313,12 @@ def main():
     bank = csmv.Bank(n_txns=args.txns, seed=1, n_reads=4, n_reads_ro=4, pct_ro=10, zipf=True,
                      theta=0.99)
 
-    def make(variant, b, router, serial):
+    def make(variant, b, router, serial, escalate=0):
         e = csmv.Engine(N_ACCOUNTS, depth=8, atr_entries=4096, client_waves=args.waves,
                         wg_waves=args.wg_waves, variant=variant, timeout_ms=8000,
                         n_serial_waves=serial if router != "none" else 0,
-                        stage_ahead=args.stage_ahead, client_wg_waves=args.client_wg_waves)
+                        stage_ahead=args.stage_ahead, client_wg_waves=args.client_wg_waves,
+                        escalate_after=escalate)
         e.bank_generate(b)
         if router == "transfers":
             op, _ = e.bank_read_stream()
@@ -374,15 +378,21 @@ def main():
     if not args.no_variants:
         vbank = csmv.Bank(n_txns=args.variant_txns, seed=1, n_reads=4, n_reads_ro=4, pct_ro=10,
                           zipf=True, theta=0.99)
-        for v in REF_VARIANTS + ("stamp",):
+        for v in REF_VARIANTS + ("stamp", "stamp_escalate"):
             log(f"variant {v}")
-            ev = make(v, vbank, "none", 0)
+            esc = args.escalate_after if v == "stamp_escalate" else 0
+            ev = make("stamp" if esc else v, vbank, "none", 0, escalate=esc)
             ev.fill(INIT)
             r = ev.bank_run()
             ev.close()
             variants[v] = {"txn_per_s": r["txn_per_s"], "abort_rate": r["abort_rate"],
                            "kernel_ms": r["kernel_ms"], "txns": args.variant_txns,
                            "lane": "normal (client-executed CSMV)"}
+            if esc:
+                variants[v]["escalate_after"] = esc
+                variants[v]["escalated"] = r["escalated"]
+                variants[v]["lane"] = ("normal, transactions failing " + str(esc) +
+                                       " attempts executed on the serialized lane")
     # the target's measure: the headline over the plain per-lane protocol's ceiling (the
     # measured same-address atomic rate / 2 atomics per committed update)
     atomics = {"same_address_per_s": atomic_rate,

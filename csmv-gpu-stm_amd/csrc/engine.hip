@@ -2,7 +2,7 @@
 //
 // One persistent kernel replaces the reference's device-side launch of server and client
 // grids (parent_kernel, bank/csmv_sh/exe.cu:319-338; HIP has no dynamic parallelism):
-// workgroup 0 is the commit server (server.hpp), every other workgroup holds client
+// workgroup 0 is the commit server (csmv_server.hpp), every other workgroup holds client
 // wavefronts.  The grid never exceeds what the device keeps resident, so spinning
 // clients cannot starve the server, and every wait is bounded (Spin).
 #include <hip/hip_runtime.h>
@@ -16,7 +16,7 @@
 
 #include "csmv.h"
 #include "csmv_device.hpp"
-#include "server.hpp"
+#include "csmv_server.hpp"
 #include "memcd.hpp"
 #include "mlp.hpp"
 #include "serve.hpp"
@@ -634,15 +634,15 @@ uint32_t pow2_floor(uint64_t v)
 }
 
 // Server LDS layout for a request wire of R reads and W writes.
-template <int R, int W>
-ServerLayout server_layout_at(const csmv_engine *e, bool delegation, uint32_t nmb, uint32_t hshift)
+ServerLayout server_layout_at(const csmv_engine *e, uint32_t R, uint32_t W, bool delegation,
+                              uint32_t nmb, uint32_t hshift)
 {
     ServerLayout L{};
     const uint32_t vmode = e->cfg.variant & 3u;
     const uint32_t M = e->cfg.wg_waves - 2;
     // enough mailboxes for the receivers and for the staging window
     L.nmb = nmb ? nmb : std::min<uint32_t>(std::max<uint32_t>({M + 2, e->cfg.stage_ahead + 2, 4u}), 16);
-    L.mb_rows = delegation ? Rows<R, W>::count : Rows<R, W>::pref + 1;
+    L.mb_rows = mailbox_rows(R, W, delegation);
     L.state_off = kLdsHdr;
     L.wave_off = L.state_off + L.nmb;
     L.tg_off = L.wave_off + L.nmb;
@@ -654,7 +654,7 @@ ServerLayout server_layout_at(const csmv_engine *e, bool delegation, uint32_t nm
     w += M * 3 * L.hsz;
     // serialized lane: per-receiver chain scratch (next, weight per chain node; superseded
     // flags per write slot) and the committer's exchange of base and new values
-    constexpr uint32_t RO = (uint32_t)kChainReads<R, W>;
+    const uint32_t RO = chain_reads(R, W);
     L.sup_off = w;
     if (delegation) w += M * (2 * kWave * RO + kWave * W);
     L.jmp_off = w;
@@ -687,14 +687,20 @@ ServerLayout server_layout_at(const csmv_engine *e, bool delegation, uint32_t nm
 // A wave probes its receiver hash until its longest probe sequence ends, so the hash is
 // kept sparse (load at most 1/8, else 1/4, else 1/2) as long as the stamp buckets keep
 // 8192 entries and the layout fits.
+ServerLayout server_layout_rt(const csmv_engine *e, uint32_t R, uint32_t W, bool delegation,
+                              uint32_t nmb = 0)
+{
+    for (uint32_t hshift = 2; hshift > 0; --hshift) {
+        const ServerLayout L = server_layout_at(e, R, W, delegation, nmb, hshift);
+        if ((uint64_t)L.words * 4 <= e->lds_max && (L.exact || L.nbuckets >= 8192)) return L;
+    }
+    return server_layout_at(e, R, W, delegation, nmb, 0);
+}
+
 template <int R, int W>
 ServerLayout server_layout(const csmv_engine *e, bool delegation, uint32_t nmb = 0)
 {
-    for (uint32_t hshift = 2; hshift > 0; --hshift) {
-        const ServerLayout L = server_layout_at<R, W>(e, delegation, nmb, hshift);
-        if ((uint64_t)L.words * 4 <= e->lds_max && (L.exact || L.nbuckets >= 8192)) return L;
-    }
-    return server_layout_at<R, W>(e, delegation, nmb, 0);
+    return server_layout_rt(e, (uint32_t)R, (uint32_t)W, delegation, nmb);
 }
 
 }  // namespace

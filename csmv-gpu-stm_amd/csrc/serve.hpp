@@ -113,12 +113,17 @@ __device__ __forceinline__ bool pool_read_chunk(const Ctx &c, const uint32_t (&i
                                                 int32_t (&out)[N])
 {
     // every load of a round is issued (an empty entry reads item 0) and the round is pinned
-    // (pin_round): otherwise the compiler waits for each load before issuing the next
+    // (pin_round): otherwise the compiler waits for each load before issuing the next.
+    // An installer writes the slot, then (after a release fence) the head word; the acquire
+    // fence between the two rounds makes a head seen here order the slot it names, so a
+    // new head never pairs with the slot's evicted contents.  (On gfx950 it is one
+    // buffer_inv sc1 per round; the bank headline measured the same with and without it.)
     uint32_t ht[N];
     uint64_t x[N];
 #pragma unroll
     for (int j = 0; j < N; ++j) ht[j] = ld_rlx(&c.ht[it[j] != kNone ? it[j] : 0u]);
     pin_round(ht);
+    fence_acq();
 #pragma unroll
     for (int j = 0; j < N; ++j)
         x[j] = ld_rlx(&c.slot[(size_t)(it[j] != kNone ? it[j] : 0u) * c.depth + (ht[j] & 0xFFFFu)]);
@@ -146,6 +151,7 @@ __device__ __forceinline__ uint32_t pool_read_some(const Ctx &c, const uint32_t 
 #pragma unroll
     for (int j = 0; j < N; ++j) ht[j] = ld_rlx(&c.ht[it[j] != kNone ? it[j] : 0u]);
     pin_round(ht);
+    fence_acq();   // as in pool_read_chunk
 #pragma unroll
     for (int j = 0; j < N; ++j)
         x[j] = ld_rlx(&c.slot[(size_t)(it[j] != kNone ? it[j] : 0u) * c.depth + (ht[j] & 0xFFFFu)]);

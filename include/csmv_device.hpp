@@ -284,6 +284,7 @@ __device__ __forceinline__ bool pool_read(const Ctx &c, uint32_t item, uint32_t 
                                           int32_t &out)
 {
     const uint32_t ht = ld_rlx(&c.ht[item]);
+    fence_acq();   // the slots the head names are read after it (see pool_read_chunk)
     const uint32_t h = ht & 0xFFFFu, t = ht >> 16;
     uint32_t j = h;
     for (uint32_t d = 0; d < c.depth; ++d) {
@@ -305,6 +306,7 @@ __device__ __forceinline__ bool pool_read_below(const Ctx &c, uint32_t item, uin
                                                 int32_t &out)
 {
     const uint32_t ht = ld_rlx(&c.ht[item]);
+    fence_acq();
     const uint32_t h = ht & 0xFFFFu, t = ht >> 16;
     if (h == t) return false;
     uint32_t j = (h == 0) ? c.depth - 1 : h - 1;
@@ -497,6 +499,20 @@ __device__ __forceinline__ uint32_t tx_commit(const Ctx &c, uint32_t cw, Tx<R, W
     uint64_t v;
     const uint64_t tw0 = wall_clock64();
     Spin sp;
+    // While the request is in flight only lane 0 polls, and only its own reply word: the
+    // committer writes the 64 reply words of a request with one store instruction, after
+    // the request's other reply words, so once lane 0's word carries the request's tag
+    // the wave reads everything once (one memory request per poll instead of 64 lanes'
+    // worth of tagged words).
+    {
+        const uint64_t *r0 = c.reply + (size_t)cw * kWave;
+        uint64_t w0 = 0;
+        for (;;) {
+            if (lane == 0) w0 = ld_rlx(r0);
+            if ((bcast64(w0) >> 32) == n + 1) break;
+            if (!sp.tick(c, kErrReplyTimeout)) { *fail = true; return 0; }
+        }
+    }
     // serialized lane: the reply word and, for each write, the new value and the stamp
     // its install must follow (kNone: the write was superseded inside the request and is
     // not installed), all 2W + 1 words in one batch of loads per poll; every word carries

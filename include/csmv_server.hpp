@@ -1,4 +1,9 @@
-// server.hpp -- the commit-server workgroup of the gfx950 CSMV engine.
+// csmv_server.hpp -- the commit-server workgroup of the gfx950 CSMV engine.
+//
+// A caller's own persistent kernel runs it in workgroup 0 beside its client waves (see
+// csmv_kernel_prepare in csmv.h): server_workgroup<R, W, false>(ctx, layout, lds), with
+// the wire R / W of the clients' Tx<R, W>, the context and layout the engine prepared and
+// the workgroup's dynamic LDS.
 //
 // The reference runs one persistent server block: a leader warp scans a ready bitmask
 // and hands 32-message chunks to worker warps through shared memory
@@ -80,6 +85,14 @@ template <int R> constexpr bool kWideWire = R > 16;
 // serialized-lane chain nodes per lane
 template <int R, int W> constexpr int kChainReads = kWideWire<R> ? W : R;
 
+// Rows of 64 words in one LDS mailbox for a wire of R reads and W writes (host and
+// device): without the serialized lane the rows up to pref, with it the chain rows too.
+__host__ __device__ constexpr uint32_t chain_reads(uint32_t R, uint32_t W) { return R > 16 ? W : R; }
+__host__ __device__ constexpr uint32_t mailbox_rows(uint32_t R, uint32_t W, bool delegation)
+{
+    return delegation ? 5 + R + 3 * W + (R + 3) / 4 + 2 * chain_reads(R, W) : 4 + R + 2 * W;
+}
+
 template <int R, int W> struct Rows {
     static constexpr int RO = kChainReads<R, W>;
     static constexpr uint32_t snap = 0;
@@ -97,6 +110,8 @@ template <int R, int W> struct Rows {
     // stamp-bucket index (exact mode: the item) of each write
     static constexpr uint32_t bkt = supf + 1;
     static constexpr uint32_t count = bkt + W;
+    static_assert(count == mailbox_rows(R, W, true) && pref + 1 == mailbox_rows(R, W, false),
+                  "host layout rows follow the staged rows");
 };
 
 __device__ __forceinline__ uint32_t lds_ld_acq(const uint32_t *p)
