@@ -564,6 +564,11 @@ struct csmv_engine {
     uint32_t *part_cnt = nullptr, *scr_a = nullptr, *scr_b = nullptr;
     uint64_t *part_total = nullptr;
     uint64_t pred_cap = 0, route_cap = 0;
+    // request mailbox rows per client wave (the widest wire the engine serves)
+    uint32_t mbox_rows = 0;
+    // a caller-supplied kernel prepared by csmv_kernel_prepare and not yet finished
+    ServerLayout ext_layout{};
+    bool ext_prepared = false;
     std::string err;
 };
 
@@ -797,6 +802,7 @@ csmv_status csmv_create(const csmv_config *cfg_in, csmv_engine **out)
     const uint32_t rows = std::max({mbox_rows<kBankR, kBankW>(), mbox_rows<kMcR, kMcW>(),
                                     mbox_rows<kServeBankR, kBankW>()});
     static_assert(kMcW <= kBankW, "reply values are sized for the bank wire");
+    e->mbox_rows = rows;
     if ((s = dalloc(e, &e->ht, cfg.n_items)) ||
         (s = dalloc(e, &e->slot, (size_t)cfg.n_items * cfg.depth)) ||
         (s = dalloc(e, &e->ctl, kCtlBytes)) || (s = dalloc(e, &e->ring, e->Q)) ||
@@ -1267,6 +1273,73 @@ csmv_status csmv_bank_read_log(csmv_engine *e, uint32_t *ts, int32_t *v, uint32_
     if (v) HIPCHK(e, hipMemcpy(v, e->log_v, (size_t)e->spec.n_reads * T * 4, hipMemcpyDeviceToHost));
     if (aborts) HIPCHK(e, hipMemcpy(aborts, e->log_ab, T * 4, hipMemcpyDeviceToHost));
     return CSMV_OK;
+}
+
+csmv_status csmv_kernel_prepare(csmv_engine *e, const csmv_kernel_spec *sp, void *ctx,
+                                size_t ctx_bytes, void *layout, size_t layout_bytes,
+                                csmv_launch *out)
+{
+    if (!e) return CSMV_E_ARG;
+    if (!sp || !sp->kernel || !ctx || !layout || !out)
+        return fail(e, CSMV_E_ARG, "kernel_prepare: null argument");
+    if (ctx_bytes != sizeof(Ctx) || layout_bytes != sizeof(ServerLayout))
+        return fail(e, CSMV_E_ARG, "kernel_prepare: context / layout buffers must be sizeof(csmv::Ctx) = " +
+                                       std::to_string(sizeof(Ctx)) + " and sizeof(csmv::ServerLayout) = " +
+                                       std::to_string(sizeof(ServerLayout)) + " bytes");
+    const uint32_t R = sp->reads, W = sp->writes;
+    if (R == 0 || R > 64 || W == 0 || W > 32 || 1 + R + 2 * W > e->mbox_rows)
+        return fail(e, CSMV_E_ARG, "kernel_prepare: wire beyond the request mailboxes");
+    if (e->ext_prepared) return fail(e, CSMV_E_STATE, "kernel_prepare: a prepared kernel was not finished");
+    if (!e->pool_loaded) return fail(e, CSMV_E_STATE, "kernel_prepare needs a freshly loaded pool");
+    const ServerLayout L = server_layout_rt(e, R, W, false);
+    const size_t lds_bytes = (size_t)L.words * 4;
+    if (lds_bytes > e->lds_max)
+        return fail(e, CSMV_E_CAPACITY, "server LDS layout of " + std::to_string(lds_bytes) +
+                                            " bytes exceeds the budget");
+    const uint32_t block = e->cfg.wg_waves * kWave;
+    const uint32_t cpw = e->cfg.client_wg_waves;
+    const uint32_t grid = 1 + (e->cfg.client_waves + cpw - 1) / cpw;
+    HIPCHK(e, hipFuncSetAttribute(sp->kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds_bytes));
+    int per_cu = 0;
+    HIPCHK(e, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, sp->kernel, (int)block, lds_bytes));
+    if ((uint64_t)per_cu * e->n_cu < grid)
+        return fail(e, CSMV_E_CAPACITY, "grid of " + std::to_string(grid) +
+                                            " workgroups exceeds what the device keeps resident");
+    Ctx c{};
+    base_ctx(e, c);
+    c.n_serial_waves = 0;
+    c.escalate = 0;
+    csmv_status s = reset_run_state(e, L);
+    if (s) return s;
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    std::memcpy(ctx, &c, sizeof(Ctx));
+    std::memcpy(layout, &L, sizeof(ServerLayout));
+    HIPCHK(e, hipEventRecord(e->ev0, e->stream));
+    e->ext_layout = L;
+    e->ext_prepared = true;
+    out->grid = grid;
+    out->block = block;
+    out->lds_bytes = (uint32_t)lds_bytes;
+    out->reserved = 0;
+    out->stream = (void *)e->stream;
+    return CSMV_OK;
+}
+
+csmv_status csmv_kernel_finish(csmv_engine *e, csmv_stats *out)
+{
+    if (!e) return CSMV_E_ARG;
+    if (!e->ext_prepared) return fail(e, CSMV_E_STATE, "kernel_finish without kernel_prepare");
+    e->ext_prepared = false;
+    e->pool_loaded = false;
+    HIPCHK(e, hipEventRecord(e->ev1, e->stream));
+    HIPCHK(e, hipEventSynchronize(e->ev1));
+    float ms = 0.f;
+    HIPCHK(e, hipEventElapsedTime(&ms, e->ev0, e->ev1));
+    uint32_t derr = 0;
+    csmv_status s = collect_stats(e, e->ext_layout, ms, out, &derr);
+    if (s) return s;
+    return device_error(e, derr);
 }
 
 csmv_status csmv_atomic_rate(csmv_engine *e, uint32_t waves, uint32_t iters, double *rate)

@@ -196,6 +196,42 @@ csmv_status csmv_memcd_read_log(csmv_engine *e, uint32_t *ts, uint32_t *res, uin
 /* Rate of the device wall clock used for latencies (kHz). */
 csmv_status csmv_wallclock_khz(csmv_engine *e, uint32_t *khz);
 
+/* ---- caller-supplied kernels -------------------------------------------------------------
+ * The reference's drivers are the caller's own kernels: a client kernel calls TXBegin /
+ * TXRead / TXReadOnly / TXWrite / TXCommit (collabValidation/csmv_sh/API.cuh:135-151) beside
+ * a server kernel built with the application's hook (bank/csmv_sh/exe.cu:56-79, 89-338).
+ * Here the caller writes one persistent kernel
+ *     __global__ void k(csmv::Ctx c, csmv::ServerLayout L, ...);
+ * whose workgroup 0 runs csmv::server_workgroup<R, W, false>(c, L, lds) (csmv_server.hpp)
+ * while its other waves take csmv::client_wave(c), run transactions with csmv_device.hpp's
+ * tx_begin / tx_read / tx_read_only / tx_write / tx_commit on Tx<R, W>, and end with
+ * csmv::client_finish.  csmv_kernel_prepare checks the wire and the grid (every workgroup
+ * resident), clears the engine's run state, fills the context and layout the kernel takes
+ * (sizes must be sizeof(csmv::Ctx) and sizeof(csmv::ServerLayout)), sets the kernel's LDS
+ * attribute and records the start event on the engine's stream; the caller launches
+ *     k<<<launch.grid, launch.block, launch.lds_bytes, (hipStream_t)launch.stream>>>(c, L, ...)
+ * and calls csmv_kernel_finish, which waits for the kernel and returns its statistics.  The
+ * pool must be freshly loaded; the caller owns its stream, its log buffers and the kernel's
+ * other arguments.  Such kernels run the normal lane (no serialized lane, no escalation). */
+typedef struct {
+    const void *kernel;       /* the caller's __global__ function */
+    uint32_t reads;           /* R of the kernel's Tx<R, W> (1..64) */
+    uint32_t writes;          /* W (1..32, with 1 + R + 2 W <= 69) */
+} csmv_kernel_spec;
+
+typedef struct {
+    uint32_t grid;            /* 1 server workgroup + the client workgroups */
+    uint32_t block;           /* wg_waves * 64 threads */
+    uint32_t lds_bytes;       /* dynamic LDS of every workgroup */
+    uint32_t reserved;
+    void *stream;             /* the engine's hipStream_t: the launch goes on it */
+} csmv_launch;
+
+csmv_status csmv_kernel_prepare(csmv_engine *e, const csmv_kernel_spec *spec, void *ctx,
+                                size_t ctx_bytes, void *layout, size_t layout_bytes,
+                                csmv_launch *launch);
+csmv_status csmv_kernel_finish(csmv_engine *e, csmv_stats *stats);
+
 /* Same-address device-scope atomic rate (atomics per second), measured by a
  * microbenchmark kernel: the ceiling unit for the per-lane commit protocol. */
 csmv_status csmv_atomic_rate(csmv_engine *e, uint32_t waves, uint32_t iters,

@@ -266,6 +266,35 @@ __device__ __forceinline__ void raise_error(const Ctx &c, uint32_t code)
                                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// ---- client waves of a persistent launch ---------------------------------------------------
+// Workgroup 0 of a launch is the commit server; client wave cw is wave cw mod
+// client_wg_waves of workgroup 1 + cw / client_wg_waves.  Returns the calling wave's client
+// index, or kNone for a wave without a role (it leaves at once and never calls
+// client_finish).
+__device__ __forceinline__ uint32_t client_wave(const Ctx &c)
+{
+    if (blockIdx.x == 0) return kNone;
+    const uint32_t wave = threadIdx.x / kWave;
+    if (wave >= c.client_wg_waves) return kNone;
+    const uint32_t cw = (blockIdx.x - 1) * c.client_wg_waves + wave;
+    return cw < c.n_client_waves ? cw : kNone;
+}
+
+// A client wave's last act (all 64 lanes): each lane's commits (updates, read-only) and the
+// attempts it abandoned before sending (no version old enough, DataAge) go into the engine
+// statistics; then the wave's exit is published, which the commit server's receivers wait
+// for before they leave.  Every client wave of a launch must call it once.
+__device__ __forceinline__ void client_finish(const Ctx &c, uint64_t commits_upd, uint64_t commits_ro,
+                                              uint64_t aborts_dataage)
+{
+    if (commits_upd) fetch_add(&c.stats[kStatCommitUpd], commits_upd);
+    if (commits_ro) fetch_add(&c.stats[kStatCommitRo], commits_ro);
+    if (aborts_dataage) fetch_add(&c.stats[kStatAbortAge], aborts_dataage);
+    fence_rel();
+    if (lane_id() == 0) __hip_atomic_fetch_add(c.clients_done, 1u, __ATOMIC_RELEASE,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // ---- per-lane transaction state ----------------------------------------------------------
 template <int R, int W> struct Tx {
     uint32_t snap;

@@ -102,3 +102,13 @@ def test_struct_layouts_match_the_header(tmp_path):
         assert got == int(val), (py, name, got, val)
         seen += 1
     assert seen == sum(1 + len(getattr(csmv, p)._fields_) for p in STRUCTS)
+
+
+def test_external_driver_library_loads():
+    """The external driver (built against include/ only) loads, finds libcsmv.so through
+    its runpath and exports its entry point; the engine exports the prepare/finish pair
+    that driver calls."""
+    import c1drv
+    assert hasattr(c1drv.lib(), "c1drv_run")
+    for n in ("csmv_kernel_prepare", "csmv_kernel_finish"):
+        assert hasattr(csmv.lib(), n)
