@@ -106,6 +106,13 @@ class Stats(C.Structure):
         return d
 
 
+class BankBuffers(C.Structure):
+    """csmv_bank_buffers: device pointers of an allocation-free bank run."""
+    _fields_ = [("init", C.c_void_p), ("final_state", C.c_void_p), ("op", C.c_void_p),
+                ("addr", C.c_void_p), ("n_txns", C.c_uint64), ("log_ts", C.c_void_p),
+                ("log_v", C.c_void_p), ("log_ab", C.c_void_p)]
+
+
 class ServeSpec(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("offered_rps", C.c_double), ("pct_cache", C.c_uint32),
                 ("reserved", C.c_uint32), ("pool_budget", C.c_uint64)]
@@ -168,6 +175,8 @@ def lib() -> C.CDLL:
         "csmv_pred_infer": ([P, C.c_float, P, SZ], C.c_int),
         "csmv_bank_route_learned": ([P, C.c_float, C.POINTER(U64)], C.c_int),
         "csmv_bank_read_route": ([P, P, P, C.POINTER(U64), C.POINTER(U64)], C.c_int),
+        "csmv_bank_run_noalloc": ([C.POINTER(Config), C.POINTER(BankBuffers), P,
+                                   C.POINTER(Stats)], C.c_int),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(L, name)
@@ -235,17 +244,38 @@ class Memcd:
         return 12 * self.n_sets * self.n_ways
 
 
+def config(n_items: int, depth: int = 4, atr_entries: int = 4096, client_waves: int = 64,
+           wg_waves: int = 8, variant="stamp", timeout_ms: int = 4000, max_attempts: int = 0,
+           stage_ahead: int = 0, lds_kib: int = 0, n_serial_waves: int = 0,
+           begin_wait: bool = True, client_wg_waves: int = 0, escalate_after: int = 0) -> Config:
+    """A csmv_config (0 fields take the engine's defaults)."""
+    if isinstance(variant, str):
+        variant = VARIANTS[variant]
+    return Config(n_items, depth, atr_entries, client_waves, wg_waves, variant, timeout_ms,
+                  max_attempts, stage_ahead, lds_kib, n_serial_waves, 1 if begin_wait else 0,
+                  client_wg_waves, escalate_after)
+
+
+def bank_run_noalloc(cfg: Config, buffers: BankBuffers, stream: Optional[int] = None) -> dict:
+    """csmv_bank_run_noalloc: a bank run on caller-owned device buffers, with no device
+    allocation (the engine's fixed scratch array holds everything else)."""
+    st = Stats()
+    rc = lib().csmv_bank_run_noalloc(C.byref(cfg), C.byref(buffers),
+                                     C.c_void_p(stream) if stream else None, C.byref(st))
+    if rc:
+        raise CsmvError(rc, "csmv_bank_run_noalloc")
+    return st.as_dict()
+
+
 class Engine:
     def __init__(self, n_items: int, depth: int = 4, atr_entries: int = 4096,
                  client_waves: int = 64, wg_waves: int = 8, variant="stamp",
                  timeout_ms: int = 4000, max_attempts: int = 0, stage_ahead: int = 0,
                  lds_kib: int = 0, n_serial_waves: int = 0, begin_wait: bool = True,
                  client_wg_waves: int = 0, escalate_after: int = 0):
-        if isinstance(variant, str):
-            variant = VARIANTS[variant]
-        self.cfg = Config(n_items, depth, atr_entries, client_waves, wg_waves, variant,
+        self.cfg = config(n_items, depth, atr_entries, client_waves, wg_waves, variant,
                           timeout_ms, max_attempts, stage_ahead, lds_kib, n_serial_waves,
-                          1 if begin_wait else 0, client_wg_waves, escalate_after)
+                          begin_wait, client_wg_waves, escalate_after)
         self.n_items = n_items
         self._h = C.c_void_p()
         st = lib().csmv_create(C.byref(self.cfg), C.byref(self._h))

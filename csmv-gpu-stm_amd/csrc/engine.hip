@@ -8,9 +8,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 
@@ -611,6 +613,25 @@ template <typename T> void dfree(T *&p)
     p = nullptr;
 }
 
+// Applies the configuration defaults and checks the ranges (CSMV_E_ARG when out of range).
+csmv_status normalize_config(csmv_config &cfg)
+{
+    if (cfg.timeout_ms == 0) cfg.timeout_ms = 4000;
+    if (cfg.max_attempts == 0) cfg.max_attempts = 1u << 20;
+    if (cfg.wg_waves == 0) cfg.wg_waves = 8;
+    if (cfg.stage_ahead == 0) cfg.stage_ahead = 3;
+    if (cfg.client_wg_waves == 0) cfg.client_wg_waves = cfg.wg_waves;
+    if (cfg.n_serial_waves >= cfg.client_waves && cfg.n_serial_waves) return CSMV_E_ARG;
+    if (cfg.n_items == 0 || cfg.depth == 0 || cfg.depth > 64 || cfg.client_waves == 0 ||
+        cfg.wg_waves < 3 || cfg.wg_waves > 8 || cfg.client_wg_waves > cfg.wg_waves ||
+        cfg.stage_ahead > 64 || cfg.atr_entries < 64 ||
+        (cfg.atr_entries & (cfg.atr_entries - 1)) || cfg.atr_entries > (1u << 16) ||
+        (cfg.variant & ~0xFu) || (cfg.variant & 3u) == 3u ||
+        (cfg.escalate_after && cfg.variant != (kValStamp | kInsertBatch)))
+        return CSMV_E_ARG;
+    return CSMV_OK;
+}
+
 uint32_t pow2_at_least(uint64_t v)
 {
     uint32_t q = 1;
@@ -639,14 +660,14 @@ uint32_t pow2_floor(uint64_t v)
 }
 
 // Server LDS layout for a request wire of R reads and W writes.
-ServerLayout server_layout_at(const csmv_engine *e, uint32_t R, uint32_t W, bool delegation,
-                              uint32_t nmb, uint32_t hshift)
+ServerLayout server_layout_at(const csmv_config &cfg, uint32_t lds_max, uint32_t R, uint32_t W,
+                              bool delegation, uint32_t nmb, uint32_t hshift)
 {
     ServerLayout L{};
-    const uint32_t vmode = e->cfg.variant & 3u;
-    const uint32_t M = e->cfg.wg_waves - 2;
+    const uint32_t vmode = cfg.variant & 3u;
+    const uint32_t M = cfg.wg_waves - 2;
     // enough mailboxes for the receivers and for the staging window
-    L.nmb = nmb ? nmb : std::min<uint32_t>(std::max<uint32_t>({M + 2, e->cfg.stage_ahead + 2, 4u}), 16);
+    L.nmb = nmb ? nmb : std::min<uint32_t>(std::max<uint32_t>({M + 2, cfg.stage_ahead + 2, 4u}), 16);
     L.mb_rows = mailbox_rows(R, W, delegation);
     L.state_off = kLdsHdr;
     L.wave_off = L.state_off + L.nmb;
@@ -665,20 +686,20 @@ ServerLayout server_layout_at(const csmv_engine *e, uint32_t R, uint32_t W, bool
     L.jmp_off = w;
     if (delegation) w += kWave * RO + kWave * W;
     L.atr_off = w;
-    if (vmode != kValStamp) w += W * e->cfg.atr_entries;
+    if (vmode != kValStamp) w += W * cfg.atr_entries;
     L.bucket_off = w;
     L.val_off = w;
     L.nbuckets = 0;
     L.exact = 0;
     if (vmode == kValStamp) {
-        const uint64_t budget = e->lds_max / 4;
+        const uint64_t budget = lds_max / 4;
         const uint64_t room = budget > w ? budget - w : 0;
-        if (2ull * pow2_at_least(e->cfg.n_items) <= room) {
+        if (2ull * pow2_at_least(cfg.n_items) <= room) {
             // exact: stamp and value of every item
             L.exact = 1;
-            L.nbuckets = pow2_at_least(e->cfg.n_items);
+            L.nbuckets = pow2_at_least(cfg.n_items);
             L.val_off = L.bucket_off + L.nbuckets;
-            w += L.nbuckets + e->cfg.n_items;
+            w += L.nbuckets + cfg.n_items;
         } else {
             // hashed stamps; exact stamps and values stay in HBM
             L.nbuckets = pow2_floor(room);
@@ -692,14 +713,20 @@ ServerLayout server_layout_at(const csmv_engine *e, uint32_t R, uint32_t W, bool
 // A wave probes its receiver hash until its longest probe sequence ends, so the hash is
 // kept sparse (load at most 1/8, else 1/4, else 1/2) as long as the stamp buckets keep
 // 8192 entries and the layout fits.
+ServerLayout server_layout_cfg(const csmv_config &cfg, uint32_t lds_max, uint32_t R, uint32_t W,
+                               bool delegation, uint32_t nmb = 0)
+{
+    for (uint32_t hshift = 2; hshift > 0; --hshift) {
+        const ServerLayout L = server_layout_at(cfg, lds_max, R, W, delegation, nmb, hshift);
+        if ((uint64_t)L.words * 4 <= lds_max && (L.exact || L.nbuckets >= 8192)) return L;
+    }
+    return server_layout_at(cfg, lds_max, R, W, delegation, nmb, 0);
+}
+
 ServerLayout server_layout_rt(const csmv_engine *e, uint32_t R, uint32_t W, bool delegation,
                               uint32_t nmb = 0)
 {
-    for (uint32_t hshift = 2; hshift > 0; --hshift) {
-        const ServerLayout L = server_layout_at(e, R, W, delegation, nmb, hshift);
-        if ((uint64_t)L.words * 4 <= e->lds_max && (L.exact || L.nbuckets >= 8192)) return L;
-    }
-    return server_layout_at(e, R, W, delegation, nmb, 0);
+    return server_layout_cfg(e->cfg, e->lds_max, R, W, delegation, nmb);
 }
 
 template <int R, int W>
@@ -755,19 +782,7 @@ csmv_status csmv_create(const csmv_config *cfg_in, csmv_engine **out)
     if (!cfg_in || !out) return CSMV_E_ARG;
     *out = nullptr;
     csmv_config cfg = *cfg_in;
-    if (cfg.timeout_ms == 0) cfg.timeout_ms = 4000;
-    if (cfg.max_attempts == 0) cfg.max_attempts = 1u << 20;
-    if (cfg.wg_waves == 0) cfg.wg_waves = 8;
-    if (cfg.stage_ahead == 0) cfg.stage_ahead = 3;
-    if (cfg.client_wg_waves == 0) cfg.client_wg_waves = cfg.wg_waves;
-    if (cfg.n_serial_waves >= cfg.client_waves && cfg.n_serial_waves) return CSMV_E_ARG;
-    if (cfg.n_items == 0 || cfg.depth == 0 || cfg.depth > 64 || cfg.client_waves == 0 ||
-        cfg.wg_waves < 3 || cfg.wg_waves > 8 || cfg.client_wg_waves > cfg.wg_waves ||
-        cfg.stage_ahead > 64 || cfg.atr_entries < 64 ||
-        (cfg.atr_entries & (cfg.atr_entries - 1)) || cfg.atr_entries > (1u << 16) ||
-        (cfg.variant & ~0xFu) || (cfg.variant & 3u) == 3u ||
-        (cfg.escalate_after && cfg.variant != (kValStamp | kInsertBatch)))
-        return CSMV_E_ARG;
+    if (normalize_config(cfg)) return CSMV_E_ARG;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return CSMV_E_NODEVICE;
     csmv_engine *e = new (std::nothrow) csmv_engine();
@@ -993,18 +1008,72 @@ csmv_status csmv_bank_read_stream(csmv_engine *e, uint32_t *op, uint32_t *addr)
 
 }  // extern "C"
 
+// The configuration fields of a kernel context.
+static void ctx_config(Ctx &c, const csmv_config &cfg, uint32_t khz, uint32_t Q)
+{
+    c.n_items = cfg.n_items;
+    c.depth = cfg.depth;
+    c.qmask = Q - 1;
+    c.n_client_waves = cfg.client_waves;
+    c.wg_waves = cfg.wg_waves;
+    c.client_wg_waves = cfg.client_wg_waves;
+    c.variant = cfg.variant;
+    c.atr_entries = cfg.atr_entries;
+    c.stage_ahead = cfg.stage_ahead;
+    c.n_serial_waves = cfg.n_serial_waves;
+    c.begin_wait = cfg.begin_wait;
+    c.timeout_ticks = (uint64_t)cfg.timeout_ms * khz;
+    c.max_attempts = cfg.max_attempts;
+    c.escalate = cfg.escalate_after;
+}
+
+// The control-word pointers of a kernel context over a control block.
+static void ctx_control(Ctx &c, uint8_t *ctl)
+{
+    c.gc = (uint32_t *)(ctl + kOffGc);
+    c.err = (uint32_t *)(ctl + kOffErr);
+    c.clients_done = (uint32_t *)(ctl + kOffDone);
+    c.next_txn = (uint64_t *)(ctl + kOffNext);
+    c.req_tail = (uint64_t *)(ctl + kOffTail);
+    c.stats = (uint64_t *)(ctl + kOffStats);
+    c.prof = (uint64_t *)(ctl + kOffProf);
+    c.next_hot = (uint64_t *)(ctl + kOffNextHot);
+}
+
+// Statistics of a launch from a host copy of its control block.
+static void decode_stats(const uint8_t *host_ctl, const ServerLayout &L, float ms,
+                         uint32_t client_waves, uint32_t khz, csmv_stats *out)
+{
+    const uint64_t *st = (const uint64_t *)(host_ctl + kOffStats);
+    std::memset(out, 0, sizeof(*out));
+    out->commits_update = st[kStatCommitUpd];
+    out->commits_ro = st[kStatCommitRo];
+    out->aborts_dataage = st[kStatAbortAge];
+    out->aborts_readwrite = st[kStatAbortRw];
+    out->aborts_prevalid = st[kStatAbortPrevalid];
+    out->aborts_recordage = st[kStatAbortRecord];
+    out->batches = st[kStatBatches];
+    out->final_ts = st[kStatFinalTs];
+    out->kernel_ms = ms;
+    std::memcpy(&out->error, host_ctl + kOffErr, 4);
+    out->client_lanes = client_waves * kWave;
+    const uint64_t *pf = (const uint64_t *)(host_ctl + kOffProf);
+    const double ns_per_tick = 1e6 / (double)khz;
+    for (int i = 0; i < kProfCount; ++i) out->prof_ns[i] = (double)pf[i] * ns_per_tick;
+    out->prof_ns[kProfClientRounds] = (double)pf[kProfClientRounds];
+    out->stamp_buckets = L.nbuckets;
+    out->stamp_exact = L.exact;
+    out->aborts_unknown = st[kStatAbortUnknown];
+    out->lds_bytes = (uint32_t)L.words * 4;
+    out->escalated = st[kStatEscalated];
+}
+
 static void base_ctx(csmv_engine *e, Ctx &c)
 {
+    ctx_config(c, e->cfg, e->wallclock_khz, e->Q);
+    ctx_control(c, e->ctl);
     c.ht = e->ht;
     c.slot = e->slot;
-    c.n_items = e->cfg.n_items;
-    c.depth = e->cfg.depth;
-    c.gc = (uint32_t *)(e->ctl + kOffGc);
-    c.err = (uint32_t *)(e->ctl + kOffErr);
-    c.clients_done = (uint32_t *)(e->ctl + kOffDone);
-    c.next_txn = (uint64_t *)(e->ctl + kOffNext);
-    c.req_tail = (uint64_t *)(e->ctl + kOffTail);
-    c.stats = (uint64_t *)(e->ctl + kOffStats);
     c.ring = e->ring;
     c.reply = e->reply;
     c.done = e->done;
@@ -1012,20 +1081,6 @@ static void base_ctx(csmv_engine *e, Ctx &c)
     c.stamps = e->stamps;
     c.vals = e->vals;
     c.reply_v = e->reply_v;
-    c.prof = (uint64_t *)(e->ctl + kOffProf);
-    c.qmask = e->Q - 1;
-    c.n_client_waves = e->cfg.client_waves;
-    c.wg_waves = e->cfg.wg_waves;
-    c.client_wg_waves = e->cfg.client_wg_waves;
-    c.variant = e->cfg.variant;
-    c.atr_entries = e->cfg.atr_entries;
-    c.stage_ahead = e->cfg.stage_ahead;
-    c.n_serial_waves = e->cfg.n_serial_waves;
-    c.begin_wait = e->cfg.begin_wait;
-    c.timeout_ticks = (uint64_t)e->cfg.timeout_ms * e->wallclock_khz;
-    c.max_attempts = e->cfg.max_attempts;
-    c.escalate = e->cfg.escalate_after;
-    c.next_hot = (uint64_t *)(e->ctl + kOffNextHot);
 }
 
 // Clears the control words and message rings of one engine before a launch.
@@ -1048,31 +1103,8 @@ static csmv_status collect_stats(csmv_engine *e, const ServerLayout &L, float ms
 {
     uint8_t host_ctl[kCtlBytes];
     HIPCHK(e, hipMemcpy(host_ctl, e->ctl, kCtlBytes, hipMemcpyDeviceToHost));
-    const uint64_t *st = (const uint64_t *)(host_ctl + kOffStats);
     std::memcpy(derr, host_ctl + kOffErr, 4);
-    if (out) {
-        std::memset(out, 0, sizeof(*out));
-        out->commits_update = st[kStatCommitUpd];
-        out->commits_ro = st[kStatCommitRo];
-        out->aborts_dataage = st[kStatAbortAge];
-        out->aborts_readwrite = st[kStatAbortRw];
-        out->aborts_prevalid = st[kStatAbortPrevalid];
-        out->aborts_recordage = st[kStatAbortRecord];
-        out->batches = st[kStatBatches];
-        out->final_ts = st[kStatFinalTs];
-        out->kernel_ms = ms;
-        out->error = *derr;
-        out->client_lanes = e->cfg.client_waves * kWave;
-        const uint64_t *pf = (const uint64_t *)(host_ctl + kOffProf);
-        const double ns_per_tick = 1e6 / (double)e->wallclock_khz;
-        for (int i = 0; i < kProfCount; ++i) out->prof_ns[i] = (double)pf[i] * ns_per_tick;
-        out->prof_ns[kProfClientRounds] = (double)pf[kProfClientRounds];
-        out->stamp_buckets = L.nbuckets;
-        out->stamp_exact = L.exact;
-        out->aborts_unknown = st[kStatAbortUnknown];
-        out->lds_bytes = (uint32_t)L.words * 4;
-        out->escalated = st[kStatEscalated];
-    }
+    if (out) decode_stats(host_ctl, L, ms, e->cfg.client_waves, e->wallclock_khz, out);
     return CSMV_OK;
 }
 
@@ -1935,3 +1967,171 @@ csmv_status csmv_serve_read_latency(csmv_engine *e, uint32_t *lat_ticks, size_t 
 }
 
 }  // extern "C"
+
+// =============================================================================================
+// bank run without allocation: every buffer beyond the caller's inputs and outputs lives in
+// one file-scope device array (its address per device from hipGetSymbolAddress)
+// =============================================================================================
+namespace {
+
+static_assert(CSMV_NOALLOC_SCRATCH_BYTES % 256 == 0, "scratch is a whole number of 256-byte blocks");
+// In host code `scratch` names a host-side stand-in; only the address hipGetSymbolAddress
+// returns for the current device may be handed to kernels or copies.
+static __device__ __align__(256) uint8_t scratch[CSMV_NOALLOC_SCRATCH_BYTES];
+
+struct ScratchDevice {
+    bool ready;
+    uint8_t *base;        // this device's copy of scratch
+    uint32_t n_cu, lds_max, khz;
+};
+constexpr int kMaxDevices = 64;
+ScratchDevice g_scratch_dev[kMaxDevices];
+std::mutex g_scratch_mu;
+
+// Once per device, with that device current: the array's address and the properties the
+// launch needs.
+csmv_status scratch_device(int dev, ScratchDevice **out)
+{
+    if (dev < 0 || dev >= kMaxDevices) return CSMV_E_CAPACITY;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    ScratchDevice &d = g_scratch_dev[dev];
+    if (!d.ready) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return CSMV_E_HIP;
+        if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return CSMV_E_NODEVICE;
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || p == nullptr)
+            return CSMV_E_HIP;
+        int khz = 0;
+        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0)
+            khz = 100000;
+        d.base = (uint8_t *)p;
+        d.n_cu = (uint32_t)prop.multiProcessorCount;
+        d.lds_max = (uint32_t)prop.sharedMemPerBlock;
+        d.khz = (uint32_t)khz;
+        d.ready = true;
+    }
+    *out = &d;
+    return CSMV_OK;
+}
+
+// Byte offsets of one run's buffers inside the array.  The words that must start at zero
+// (control block, rings, replies, HBM stamps) come first, so one clear covers [0, zero_end).
+struct ScratchMap {
+    size_t ctl, ring, done, reply, reply_v, stamps, zero_end, mbox, ht, slot, vals, total;
+};
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+ScratchMap scratch_map(const csmv_config &cfg, uint32_t Q)
+{
+    ScratchMap m{};
+    const size_t waves = cfg.client_waves, n = cfg.n_items;
+    size_t o = 0;
+    m.ctl = o; o = align256(o + kCtlBytes);
+    m.ring = o; o = align256(o + (size_t)Q * 8);
+    m.done = o; o = align256(o + (size_t)Q * 8);
+    m.reply = o; o = align256(o + waves * kWave * 8);
+    m.reply_v = o; o = align256(o + waves * kWave * 2 * kBankW * 8);
+    m.stamps = o; o = align256(o + n * 4);
+    m.zero_end = o;
+    m.mbox = o; o = align256(o + waves * mbox_rows<kBankR, kBankW>() * kWave * 4);
+    m.ht = o; o = align256(o + n * 4);
+    m.slot = o; o = align256(o + n * cfg.depth * 8);
+    m.vals = o; o = align256(o + n * 4);
+    m.total = o;
+    return m;
+}
+
+__global__ void __launch_bounds__(256) scratch_clear_kernel(uint64_t *p, size_t n_words)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_words;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
+}  // namespace
+
+extern "C" csmv_status csmv_bank_run_noalloc(const csmv_config *cfg_in, const csmv_bank_buffers *b,
+                                             void *stream, csmv_stats *out)
+{
+    if (!cfg_in || !b) return CSMV_E_ARG;
+    csmv_config cfg = *cfg_in;
+    if (normalize_config(cfg) || cfg.n_serial_waves) return CSMV_E_ARG;
+    if (!b->init || !b->final_state || !b->op || !b->addr || !b->log_ts || !b->log_v || !b->log_ab ||
+        b->n_txns == 0 || b->n_txns >= (1ull << 31))
+        return CSMV_E_ARG;
+    const uint32_t Q = pow2_at_least(std::max<uint64_t>(4096, 2ull * cfg.client_waves));
+    const ScratchMap m = scratch_map(cfg, Q);
+    if (m.total > CSMV_NOALLOC_SCRATCH_BYTES) return CSMV_E_CAPACITY;
+    int ndev = 0, dev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0 || hipGetDevice(&dev) != hipSuccess)
+        return CSMV_E_NODEVICE;
+    ScratchDevice *d = nullptr;
+    csmv_status s = scratch_device(dev, &d);
+    if (s) return s;
+    const uint32_t lds_max = cfg.lds_kib ? std::min<uint32_t>(d->lds_max, cfg.lds_kib * 1024u) : d->lds_max;
+    const ServerLayout L = server_layout_cfg(cfg, lds_max, kBankR, kBankW, true);
+    const size_t lds_bytes = (size_t)L.words * 4;
+    if (lds_bytes > lds_max) return CSMV_E_CAPACITY;
+    const uint32_t block = cfg.wg_waves * kWave;
+    const uint32_t grid = 1 + (cfg.client_waves + cfg.client_wg_waves - 1) / cfg.client_wg_waves;
+    auto kern = bank_kernel<kBankR, kBankW>;
+    if (hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds_bytes) != hipSuccess)
+        return CSMV_E_HIP;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, (int)block, lds_bytes) != hipSuccess)
+        return CSMV_E_HIP;
+    if ((uint64_t)per_cu * d->n_cu < grid) return CSMV_E_CAPACITY;
+
+    uint8_t *base = d->base;
+    Ctx c{};
+    ctx_config(c, cfg, d->khz, Q);
+    ctx_control(c, base + m.ctl);
+    c.ring = (uint64_t *)(base + m.ring);
+    c.done = (uint64_t *)(base + m.done);
+    c.reply = (uint64_t *)(base + m.reply);
+    c.reply_v = (uint64_t *)(base + m.reply_v);
+    c.stamps = (uint32_t *)(base + m.stamps);
+    c.mbox = (uint32_t *)(base + m.mbox);
+    c.ht = (uint32_t *)(base + m.ht);
+    c.slot = (uint64_t *)(base + m.slot);
+    c.vals = (int32_t *)(base + m.vals);
+    c.op = b->op;
+    c.addr = b->addr;
+    c.n_txns = b->n_txns;
+    c.n_norm = b->n_txns;
+    c.log_ts = b->log_ts;
+    c.log_v = b->log_v;
+    c.log_ab = b->log_ab;
+
+    hipStream_t st = (hipStream_t)stream;
+    const uint32_t n = cfg.n_items;
+    // the zero-initialised words by a fill kernel (no memset node if the caller captures),
+    // then the pool from the initial balances
+    hipLaunchKernelGGL(scratch_clear_kernel, dim3(256), dim3(256), 0, st, (uint64_t *)base,
+                       m.zero_end / 8);
+    hipLaunchKernelGGL(pool_fill_kernel, dim3((n + 255) / 256), dim3(256), 0, st, c.ht, c.slot,
+                       c.vals, n, cfg.depth, b->init, 0);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess) return CSMV_E_HIP;
+    const auto t0 = std::chrono::steady_clock::now();
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(block), lds_bytes, st, c, L);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess) return CSMV_E_HIP;
+    const auto t1 = std::chrono::steady_clock::now();
+    hipLaunchKernelGGL(pool_head_kernel, dim3((n + 255) / 256), dim3(256), 0, st, c.ht, c.slot, n,
+                       cfg.depth, b->final_state);
+    uint8_t host_ctl[kCtlBytes];
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(host_ctl, base + m.ctl, kCtlBytes, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return CSMV_E_HIP;
+    csmv_stats local;
+    decode_stats(host_ctl, L, (float)std::chrono::duration<double, std::milli>(t1 - t0).count(),
+                 cfg.client_waves, d->khz, &local);
+    if (out) *out = local;
+    if (local.error == kErrLivelock) return CSMV_E_LIVELOCK;
+    if (local.error == kErrReadSetOverflow) return CSMV_E_CAPACITY;
+    if (local.error) return CSMV_E_TIMEOUT;
+    return CSMV_OK;
+}

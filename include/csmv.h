@@ -232,6 +232,36 @@ csmv_status csmv_kernel_prepare(csmv_engine *e, const csmv_kernel_spec *spec, vo
                                 csmv_launch *launch);
 csmv_status csmv_kernel_finish(csmv_engine *e, csmv_stats *stats);
 
+/* ---- bank run without allocation -----------------------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the
+ * csmv_bank_run of a bank stream without an engine, on device buffers the caller owns.
+ * Everything else the run needs (version pool, request and completion rings, replies,
+ * mailboxes, control words, committed-value tables) lives in one fixed device array of
+ * CSMV_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends; the call makes no
+ * hipMalloc / hipFree.  A configuration whose layout does not fit the array is
+ * CSMV_E_CAPACITY (for example 16,384 accounts at depth 8 with 64 client waves fit).
+ * Every call on a GPU uses that GPU's one array, which keeps the last call's contents (each
+ * call clears what must start at zero): calls that overlap on one GPU, from several threads
+ * or streams, race and must not be made.  The run goes on the caller's stream (NULL: the
+ * null stream) and the call returns when it is done; kernel_ms is the host-timed run of the
+ * persistent kernel.  Normal lane only (n_serial_waves must be 0; escalate_after allowed);
+ * transfers read 2..4 accounts and audits up to 64, as for csmv_bank_run. */
+#define CSMV_NOALLOC_SCRATCH_BYTES (4u << 20)
+typedef struct {
+    const int32_t *init;       /* [n_items] initial balances (device) */
+    int32_t *final_state;      /* [n_items] head values after the run (device) */
+    const uint32_t *op;        /* [n_txns] kind | amount << 8 | reads << 16 (device) */
+    const uint32_t *addr;      /* [rmax][n_txns] operands (device) */
+    uint64_t n_txns;           /* 1 .. 2^31 - 1 */
+    uint32_t *log_ts;          /* [n_txns] commit ts; audit: snapshot (device) */
+    int32_t *log_v;            /* [reads][n_txns] transfer reads; log_v[k] = audit sum (device) */
+    uint32_t *log_ab;          /* [n_txns] failed attempts (device) */
+} csmv_bank_buffers;
+
+csmv_status csmv_bank_run_noalloc(const csmv_config *cfg, const csmv_bank_buffers *buf,
+                                  void *stream, csmv_stats *stats);
+
 /* Same-address device-scope atomic rate (atomics per second), measured by a
  * microbenchmark kernel: the ceiling unit for the per-lane commit protocol. */
 csmv_status csmv_atomic_rate(csmv_engine *e, uint32_t waves, uint32_t iters,

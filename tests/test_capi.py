@@ -70,6 +70,7 @@ STRUCTS = {  # ctypes mirror -> C type in include/csmv.h
     "Stats": "csmv_stats",
     "ServeSpec": "csmv_serve_spec",
     "ServeStats": "csmv_serve_stats",
+    "BankBuffers": "csmv_bank_buffers",
 }
 
 
@@ -112,3 +113,19 @@ def test_external_driver_library_loads():
     assert hasattr(c1drv.lib(), "c1drv_run")
     for n in ("csmv_kernel_prepare", "csmv_kernel_finish"):
         assert hasattr(csmv.lib(), n)
+
+
+def test_noalloc_rejects_before_touching_a_device():
+    """The allocation-free bank run checks its arguments and the scratch capacity on the
+    host: a null buffer is an argument error, a pool beyond the fixed array a capacity
+    error, with or without a device."""
+    bufs = csmv.BankBuffers(1, 1, 1, 1, 1 << 10, 1, 1, 1)
+    with pytest.raises(csmv.CsmvError) as ei:
+        csmv.bank_run_noalloc(csmv.config(4096), csmv.BankBuffers(0, 1, 1, 1, 1 << 10, 1, 1, 1))
+    assert ei.value.status == 2
+    with pytest.raises(csmv.CsmvError) as ei:
+        csmv.bank_run_noalloc(csmv.config(1 << 20, depth=8), bufs)
+    assert ei.value.status == 3
+    with pytest.raises(csmv.CsmvError) as ei:
+        csmv.bank_run_noalloc(csmv.config(4096, n_serial_waves=4, client_waves=8), bufs)
+    assert ei.value.status == 2
