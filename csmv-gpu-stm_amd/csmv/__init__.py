@@ -43,7 +43,7 @@ VARIANTS = {
 
 STATUS = {0: "ok", 1: "HIP runtime error", 2: "invalid argument", 3: "capacity exceeded",
           4: "device wait timed out", 5: "no gfx950 device", 6: "call out of order",
-          7: "transaction attempt limit exceeded"}
+          7: "transaction attempt limit exceeded", 8: "log file I/O failed or file malformed"}
 
 
 PRED_PARAMS = 4225      # W1[64][64] f-major, b1[64], w2[64], b2
@@ -113,6 +113,18 @@ class BankBuffers(C.Structure):
                 ("log_v", C.c_void_p), ("log_ab", C.c_void_p)]
 
 
+class LogHeader(C.Structure):
+    """csmv_log_header: the 64-byte header of a log file."""
+    _fields_ = [("magic", C.c_char * 8), ("version", C.c_uint32), ("kind", C.c_uint32),
+                ("n", C.c_uint64), ("rows", C.c_uint32), ("buckets", C.c_uint32),
+                ("seed", C.c_uint64), ("checksum", C.c_uint64), ("reserved", C.c_uint8 * 16)]
+
+
+LOG_BANK, LOG_CACHE, LOG_ABORTS = 1, 2, 3
+ABORT_CLASSES = ("commits_update", "commits_ro", "aborts_dataage", "aborts_readwrite",
+                 "aborts_prevalid", "aborts_recordage", "aborts_unknown", "escalated")
+
+
 class ServeSpec(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("offered_rps", C.c_double), ("pct_cache", C.c_uint32),
                 ("reserved", C.c_uint32), ("pool_budget", C.c_uint64)]
@@ -177,6 +189,12 @@ def lib() -> C.CDLL:
         "csmv_bank_read_route": ([P, P, P, C.POINTER(U64), C.POINTER(U64)], C.c_int),
         "csmv_bank_run_noalloc": ([C.POINTER(Config), C.POINTER(BankBuffers), P,
                                    C.POINTER(Stats)], C.c_int),
+        "csmv_log_write_bank": ([C.c_char_p, U64, U64, U32, P, P, P], C.c_int),
+        "csmv_log_read_bank": ([C.c_char_p, C.POINTER(LogHeader), P, P, P], C.c_int),
+        "csmv_log_write_cache": ([C.c_char_p, U64, U64, P, P, P, P], C.c_int),
+        "csmv_log_read_cache": ([C.c_char_p, C.POINTER(LogHeader), P, P, P, P], C.c_int),
+        "csmv_log_write_aborts": ([C.c_char_p, U64, C.POINTER(Stats), U64, P, P, U32], C.c_int),
+        "csmv_log_read_aborts": ([C.c_char_p, C.POINTER(LogHeader), P, P, P], C.c_int),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(L, name)
@@ -511,3 +529,79 @@ def serve(cache: Optional[Engine], bank: Optional[Engine], offered_rps: float,
             "kernel_ms": st.kernel_ms, "admit_stalls": st.admit_stalls,
             "peak_inflight": st.peak_inflight, "final_inflight": st.final_inflight,
             "wallclock_khz": st.wallclock_khz}
+
+
+# ---- log files (csmv.h: 64-byte header + little-endian arrays) --------------------------------
+def _chk_io(rc: int, what: str):
+    if rc:
+        raise CsmvError(rc, what)
+
+
+def _u32(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+def log_write_bank(path: str, ts, v, aborts, seed: int = 0):
+    ts, ab = _u32(ts), _u32(aborts)
+    v = np.ascontiguousarray(v, dtype=np.int32)
+    n = ts.size
+    assert v.size % n == 0 and ab.size == n
+    _chk_io(lib().csmv_log_write_bank(os.fsencode(path), seed, n, v.size // n, _ptr(ts), _ptr(v),
+                                      _ptr(ab)), "log_write_bank")
+
+
+def log_read_header(path: str, kind: int) -> LogHeader:
+    h = LogHeader()
+    fn = {LOG_BANK: lib().csmv_log_read_bank, LOG_CACHE: lib().csmv_log_read_cache,
+          LOG_ABORTS: lib().csmv_log_read_aborts}[kind]
+    nulls = [None] * (4 if kind == LOG_CACHE else 3)
+    _chk_io(fn(os.fsencode(path), C.byref(h), *nulls), "log_read_header")
+    return h
+
+
+def log_read_bank(path: str):
+    """Returns (header, ts, v, aborts) of a bank commit log file."""
+    h = log_read_header(path, LOG_BANK)
+    ts = np.empty(h.n, dtype=np.uint32)
+    v = np.empty(h.rows * h.n, dtype=np.int32)
+    ab = np.empty(h.n, dtype=np.uint32)
+    _chk_io(lib().csmv_log_read_bank(os.fsencode(path), None, _ptr(ts), _ptr(v), _ptr(ab)),
+            "log_read_bank")
+    return h, ts, v, ab
+
+
+def log_write_cache(path: str, log: dict, seed: int = 0):
+    a = [_u32(log[k]) for k in ("ts", "res", "vsum", "aborts")]
+    _chk_io(lib().csmv_log_write_cache(os.fsencode(path), seed, a[0].size, *[_ptr(x) for x in a]),
+            "log_write_cache")
+
+
+def log_read_cache(path: str):
+    h = log_read_header(path, LOG_CACHE)
+    out = {k: np.empty(h.n, dtype=np.uint32) for k in ("ts", "res", "vsum", "aborts")}
+    _chk_io(lib().csmv_log_read_cache(os.fsencode(path), None,
+                                      *[_ptr(out[k]) for k in ("ts", "res", "vsum", "aborts")]),
+            "log_read_cache")
+    return h, out
+
+
+def log_write_aborts(path: str, stats: dict, attempts, counters=None, seed: int = 0):
+    st = Stats()
+    for k in ABORT_CLASSES:
+        setattr(st, k, int(stats.get(k, 0)))
+    att = _u32(attempts)
+    cnt = None if counters is None else _u32(counters)
+    _chk_io(lib().csmv_log_write_aborts(os.fsencode(path), seed, C.byref(st), att.size, _ptr(att),
+                                        None if cnt is None else _ptr(cnt),
+                                        0 if cnt is None else cnt.size), "log_write_aborts")
+
+
+def log_read_aborts(path: str):
+    """Returns (header, classes dict, attempts, counters or None) of an abort log file."""
+    h = log_read_header(path, LOG_ABORTS)
+    cl = np.empty(8, dtype=np.uint64)
+    att = np.empty(h.n, dtype=np.uint32)
+    cnt = np.empty(h.buckets, dtype=np.uint32) if h.buckets else None
+    _chk_io(lib().csmv_log_read_aborts(os.fsencode(path), None, _ptr(cl), _ptr(att),
+                                       None if cnt is None else _ptr(cnt)), "log_read_aborts")
+    return h, dict(zip(ABORT_CLASSES, (int(x) for x in cl))), att, cnt

@@ -750,6 +750,7 @@ const char *csmv_status_str(csmv_status s)
     case CSMV_E_NODEVICE: return "no gfx950 device";
     case CSMV_E_STATE: return "call out of order";
     case CSMV_E_LIVELOCK: return "transaction attempt limit exceeded";
+    case CSMV_E_IO: return "log file I/O failed or file malformed";
     }
     return "unknown status";
 }

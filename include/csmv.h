@@ -25,7 +25,8 @@ typedef enum {
     CSMV_E_TIMEOUT = 4,    /* a device wait exceeded the spin timeout */
     CSMV_E_NODEVICE = 5,   /* no gfx950 device visible */
     CSMV_E_STATE = 6,      /* call out of order (e.g. run before generate) */
-    CSMV_E_LIVELOCK = 7    /* a transaction exceeded the attempt limit */
+    CSMV_E_LIVELOCK = 7,   /* a transaction exceeded the attempt limit */
+    CSMV_E_IO = 8          /* a log file could not be written or read, or is malformed */
 } csmv_status;
 
 /* Commit-path variants.  They are feature flags of one code path; the named
@@ -354,6 +355,53 @@ csmv_status csmv_bank_route_learned(csmv_engine *e, float tau, uint64_t *n_hot);
 /* The current routing lists (either pointer may be NULL). */
 csmv_status csmv_bank_read_route(csmv_engine *e, uint32_t *norm, uint32_t *hot,
                                  uint64_t *n_norm, uint64_t *n_hot);
+
+/* ---- log files ------------------------------------------------------------------------------
+ * The reference's only output is the statistics printf (bank/csmv_sh/exe.cu:470-500); here a
+ * run's logs can be kept on disk.  Every file is a 64-byte header followed by little-endian
+ * arrays, in the order listed per kind:
+ *   magic "CSMVLOG" + NUL (8 bytes), u32 version (CSMV_LOG_VERSION), u32 kind, u64 n (entries),
+ *   u32 rows, u32 buckets, u64 seed (of the stream the log belongs to), u64 checksum
+ *   (FNV-1a 64 over the arrays), 16 bytes zero.
+ *   CSMV_LOG_BANK   commit log of a bank stream: ts[n] u32 (commit ts; audit: snapshot),
+ *                   v[rows][n] i32 (transfer reads; v[0][k] = audit sum), aborts[n] u32
+ *   CSMV_LOG_CACHE  commit log of a cache stream: ts[n], res[n], vsum[n], aborts[n] (u32);
+ *                   rows = 0
+ *   CSMV_LOG_ABORTS abort log: classes[8] u64 (commits update / read-only, aborts DataAge /
+ *                   ReadWrite / pre-validation / RecordAge / unknown, escalated),
+ *                   attempts[n] u32 (failed attempts per transaction), counters[buckets] u32
+ *                   (per-bucket abort counters, CSMV_ABORT_BUCKETS or 0); rows = 0
+ * Writers take host arrays; readers called with NULL arrays return the header only (so the
+ * caller can size its buffers), and check magic, version, kind, sizes and checksum
+ * (CSMV_E_IO otherwise).  No device is needed. */
+#define CSMV_LOG_VERSION 1u
+enum { CSMV_LOG_BANK = 1, CSMV_LOG_CACHE = 2, CSMV_LOG_ABORTS = 3 };
+
+typedef struct {
+    char magic[8];
+    uint32_t version;
+    uint32_t kind;
+    uint64_t n;
+    uint32_t rows;
+    uint32_t buckets;
+    uint64_t seed;
+    uint64_t checksum;
+    uint8_t reserved[16];
+} csmv_log_header;
+
+csmv_status csmv_log_write_bank(const char *path, uint64_t seed, uint64_t n, uint32_t rows,
+                                const uint32_t *ts, const int32_t *v, const uint32_t *aborts);
+csmv_status csmv_log_read_bank(const char *path, csmv_log_header *hdr, uint32_t *ts, int32_t *v,
+                               uint32_t *aborts);
+csmv_status csmv_log_write_cache(const char *path, uint64_t seed, uint64_t n, const uint32_t *ts,
+                                 const uint32_t *res, const uint32_t *vsum, const uint32_t *aborts);
+csmv_status csmv_log_read_cache(const char *path, csmv_log_header *hdr, uint32_t *ts, uint32_t *res,
+                                uint32_t *vsum, uint32_t *aborts);
+csmv_status csmv_log_write_aborts(const char *path, uint64_t seed, const csmv_stats *stats,
+                                  uint64_t n, const uint32_t *attempts, const uint32_t *counters,
+                                  uint32_t buckets);
+csmv_status csmv_log_read_aborts(const char *path, csmv_log_header *hdr, uint64_t *classes,
+                                 uint32_t *attempts, uint32_t *counters);
 
 #ifdef __cplusplus
 }

@@ -28,7 +28,7 @@ def test_exports_every_declared_symbol():
 
 def test_status_strings():
     lib = csmv.lib()
-    for s in range(8):
+    for s in range(9):
         assert lib.csmv_status_str(s)
     assert lib.csmv_status_str(0) == b"ok"
 
@@ -71,6 +71,7 @@ STRUCTS = {  # ctypes mirror -> C type in include/csmv.h
     "ServeSpec": "csmv_serve_spec",
     "ServeStats": "csmv_serve_stats",
     "BankBuffers": "csmv_bank_buffers",
+    "LogHeader": "csmv_log_header",
 }
 
 

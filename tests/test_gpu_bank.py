@@ -261,3 +261,34 @@ def test_ragged_streams(n_txns, routed):
     route = (lambda op: ((op & 0xFF) == 0).astype(np.uint8)) if routed else None
     run_bank(1 << 16, bank, variant="stamp", depth=8, waves=12, wg_waves=5,
              serial_waves=4 if routed else 0, route=route, client_wg_waves=1)
+
+
+def test_run_logs_to_files_replay(tmp_path):
+    """A run's commit log and abort log written to files (csmv_log_write_bank /
+    csmv_log_write_aborts) and read back replay exactly and keep every count."""
+    bank = csmv.Bank(n_txns=1 << 15, seed=91, n_reads=4, n_reads_ro=4, zipf=True)
+    print("\n[case] logs through files", flush=True)
+    with csmv.Engine(1 << 16, depth=8, client_waves=16, wg_waves=6, variant="stamp",
+                     escalate_after=2) as e:
+        e.fill(INIT)
+        e.bank_generate(bank)
+        st = e.bank_run()
+        op, addr = e.bank_read_stream()
+        ts, v, ab = e.bank_read_log()
+        final = e.read_state()
+        e.abort_fold(32)
+        counters = e.abort_counters()
+    pc, pa = str(tmp_path / "commits.log"), str(tmp_path / "aborts.log")
+    csmv.log_write_bank(pc, ts, v, ab, seed=bank.seed)
+    csmv.log_write_aborts(pa, st, ab, counters, seed=bank.seed)
+    h, ts2, v2, ab2 = csmv.log_read_bank(pc)
+    _, classes, att, cnt = csmv.log_read_aborts(pa)
+    spec = orc.bank_spec(1 << 16, bank.n_txns, seed=h.seed, n_reads=h.rows, n_reads_ro=4,
+                         zipf=True)
+    rep, _ = orc.bank_replay(spec, op, addr, INIT, ts2, v2, final)
+    assert rep["fails"] == 0, rep
+    assert np.array_equal(ab2, ab) and np.array_equal(att, ab)
+    assert np.array_equal(cnt, counters)
+    assert classes["commits_update"] + classes["commits_ro"] == bank.n_txns
+    assert classes["escalated"] == st["escalated"]
+    assert int(att.sum()) == st["aborts"]
