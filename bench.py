@@ -78,27 +78,99 @@ def pct_us(lat_ticks, khz):
             "p99_us": float(np.percentile(lat_ticks, 99)) / khz * 1e3}
 
 
-def cpu_baseline(op, addr, n_txns, sample):
-    """The CPU restatement of the reference protocol (oracle), on the host cores."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(op, addr, n_txns, sample, sample_1t):
+    """The CPU restatement of the reference protocol (oracle) on the host cores: nproc - 1
+    pinned client threads + 1 commit-server thread on a bounded prefix of the same stream,
+    the same with one client thread, and the single-thread serial replay of the whole
+    stream in commit order."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import orc
 
-    threads = max(2, min(16, os.cpu_count() or 2))
+    cores = len(os.sched_getaffinity(0))
+    threads = max(2, cores)
     spec = orc.bank_spec(N_ACCOUNTS, n_txns, seed=1, n_reads=4, n_reads_ro=4, pct_ro=10,
                          zipf=True, theta=0.99)
-    _, _, _, st = orc.bank_cpu_csmv(spec, op, addr, INIT, n_run=sample, depth=8,
-                                    atr_entries=4096, n_clients=threads - 1, pin=False)
-    commits = st["commits_update"] + st["commits_ro"]
-    aborts = st["aborts_dataage"] + st["aborts_readwrite"] + st["aborts_recordage"]
+
+    def run(n_run, n_clients):
+        _, _, _, st = orc.bank_cpu_csmv(spec, op, addr, INIT, n_run=n_run, depth=8,
+                                        atr_entries=4096, n_clients=n_clients, pin=True)
+        commits = st["commits_update"] + st["commits_ro"]
+        aborts = st["aborts_dataage"] + st["aborts_readwrite"] + st["aborts_recordage"]
+        return {"txn_per_s": commits / st["seconds"], "seconds": st["seconds"],
+                "abort_rate": aborts / max(1, aborts + commits), "txns": n_run,
+                "client_threads": n_clients}
+
+    multi = run(sample, threads - 1)
+    single = run(sample_1t, 1)
+    t0 = time.perf_counter()
+    orc.bank_serial(spec, op, addr, INIT)
+    serial_s = time.perf_counter() - t0
     return {
-        "value": commits / st["seconds"],
+        "value": multi["txn_per_s"],
         "unit": "txn/s",
         "cores": threads,
         "kind": "port",
         "sample": f"first {sample} transactions of the same stream; {threads - 1} client "
-                  f"threads + 1 commit-server thread; abort rate "
-                  f"{aborts / max(1, aborts + commits):.3f}; {st['seconds']:.2f} s",
+                  f"threads + 1 commit-server thread pinned one per core; abort rate "
+                  f"{multi['abort_rate']:.3f}; {multi['seconds']:.2f} s",
+        "cpu_model": cpu_model(),
+        "nproc": cores,
+        "single_client": single,
+        "serial_replay": {"txns": n_txns, "seconds": serial_s, "txn_per_s": n_txns / serial_s,
+                          "threads": 1},
     }
+
+
+def cache_bytes(op, res):
+    """Algorithmic HBM bytes of committed cache requests (item-granular, 12 B per item read
+    or installed: head word 4 + (version, value) 8): every request reads the state, key
+    low/high and timestamp of the set's 8 ways (32 items); a GET hit reads the 8 value
+    words; a GET that moves the LRU timestamp installs 1 item, a SET 12; a request that
+    writes adds 12 B of request and reply."""
+    import numpy as np
+    get = op == 0
+    hit = (res & 1) != 0
+    wrote = (res & 2) != 0
+    items_read = 32 + 8 * (get & hit)
+    items_written = np.where(get, wrote.astype(np.int64), 12 * wrote.astype(np.int64))
+    return int((12 * items_read + 12 * items_written + 12 * wrote).sum())
+
+
+def roofline(nbytes, ms):
+    achieved = nbytes / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "algorithmic_bytes": nbytes}
+
+
+def bank_plain_section(csmv, args):
+    """The bank-plain configuration: 4096 uniform accounts of 1000, 2-account transfers and
+    32-account audits 90/10, 2^20 transactions, 64 client waves x 64 lanes + the server
+    workgroup, depth 4, on the normal lane in arrival order (vanilla = the reference's
+    protocol) and the stamp variant (with and without escalation)."""
+    bank = csmv.Bank(n_txns=args.plain_txns, seed=1, n_reads=2, n_reads_ro=32, pct_ro=10)
+    out = {"workload": f"4096 uniform accounts, 2-account transfers, 32-account audits, 90/10, "
+                       f"{args.plain_txns} txns, 64 client waves, depth 4"}
+    for v, esc in (("vanilla", 0), ("stamp", 0), ("stamp", args.escalate_after)):
+        name = v if not esc else "stamp_escalate"
+        log(f"bank plain {name}")
+        with csmv.Engine(4096, depth=4, atr_entries=4096, client_waves=64, wg_waves=6, variant=v,
+                         timeout_ms=20000, escalate_after=esc) as e:
+            e.fill(INIT)
+            e.bank_generate(bank)
+            r = e.bank_run()
+        out[name] = {"txn_per_s": r["txn_per_s"], "abort_rate": r["abort_rate"],
+                     "kernel_ms": r["kernel_ms"], "escalated": r["escalated"]}
+    return out
 
 
 def memcd_section(csmv, args):
@@ -111,10 +183,12 @@ def memcd_section(csmv, args):
         e.memcd_generate(m)
         st = e.memcd_run()
         lg = e.memcd_read_log()
+        _, mop = e.memcd_read_stream()
         khz = e.wallclock_khz()
     return {"req_per_s": st["txn_per_s"], "abort_rate": st["abort_rate"],
             "kernel_ms": st["kernel_ms"], "requests": m.n_reqs, "client_waves": args.memcd_waves,
             "latency_pickup_to_commit": pct_us(lg["lat_ticks"], khz),
+            "roofline": roofline(cache_bytes(mop, lg["res"]), st["kernel_ms"]),
             "workload": "2^17 sets x 8 ways, 64-bit keys Zipf 0.99 over 10M, 32-byte values, "
                         "95/5 GET/SET, LRU clock per batch of 65,536"}
 
@@ -196,13 +270,16 @@ def serving_section(csmv, args, params, tau):
                        "over the first 2 of 2..64 Zipf 0.99 reads of 1M accounts; 20 %), "
                        f"Poisson arrivals; cache {args.serve_cache_waves} client waves, bank 16 "
                        "(8 serialized-lane when routed)",
+           "pool_budget_bytes": args.serve_budget,
+           "bank_escalate_after": args.escalate_after,
            "loads": {}}
     with csmv.Engine(m.n_items, depth=4, client_waves=args.serve_cache_waves, wg_waves=6,
                      variant="stamp", timeout_ms=30000,
                      client_wg_waves=args.client_wg_waves) as ec, \
          csmv.Engine(N_ACCOUNTS, depth=8, client_waves=16, wg_waves=6, variant="stamp",
                      timeout_ms=30000, n_serial_waves=8,
-                     client_wg_waves=args.serve_bank_client_wg_waves) as eb:
+                     client_wg_waves=args.serve_bank_client_wg_waves,
+                     escalate_after=args.escalate_after) as eb:
         ec.memcd_generate(m)
         # the predictor (trained on the bank-plain abort logs) is fine-tuned on this
         # partition's own logs: history stream 1 gives the counters, stream 2 the labels
@@ -231,8 +308,15 @@ def serving_section(csmv, args, params, tau):
                     eb.bank_route_learned(tau)
                 ec.fill(0)
                 eb.fill(INIT)
-                out = csmv.serve(ec, eb, offered_rps=load, pct_cache=80, seed=1)
+                out = csmv.serve(ec, eb, offered_rps=load, pct_cache=80, seed=1,
+                                 pool_budget=args.serve_budget)
                 khz = out["wallclock_khz"]
+                # algorithmic bytes: cache requests as cache_bytes, bank transactions
+                # 12 R + 12 W + 12 (R = 2..64 reads, W = 2)
+                _, mop = ec.memcd_read_stream()
+                bop, _ = eb.bank_read_stream()
+                nbytes = cache_bytes(mop, ec.memcd_read_log()["res"]) + int(
+                    (12 * ((bop >> 16) & 0xFF).astype(np.int64) + 36).sum())
                 lc, lb = ec.serve_latency(), eb.serve_latency()
                 ac, ab = ec.serve_arrivals(), eb.serve_arrivals()
                 nc_, wc = class_rate(ac, lc)
@@ -248,7 +332,10 @@ def serving_section(csmv, args, params, tau):
                        "cache_abort_rate": out["cache"]["abort_rate"],
                        "bank_abort_rate": out["bank"]["abort_rate"],
                        "bank_routed": n_hot if mode == "with_predictor" else 0,
-                       "admit_stalls": out["admit_stalls"]}
+                       "bank_escalated": out["bank"]["escalated"],
+                       "admit_stalls": out["admit_stalls"],
+                       "peak_inflight_bytes": out["peak_inflight"],
+                       "roofline": roofline(nbytes, out["kernel_ms"])}
                 res["loads"].setdefault(mode, []).append(row)
                 log(f"serving {mode} {load:.3g} req/s: {row['req_per_s']:.3g} served, p99 "
                     f"{row['all']['p99_us']:.1f} us (cache {row['cache']['p99_us']:.1f}, bank "
@@ -288,6 +375,8 @@ def main():
     ap.add_argument("--serve-cache-reqs", type=int, default=1 << 18)
     ap.add_argument("--serve-bank-txns", type=int, default=1 << 16)
     ap.add_argument("--serve-cache-waves", type=int, default=16)
+    ap.add_argument("--serve-budget", type=int, default=32768,
+                    help="admission cap on in-flight version-pool bytes of the serving loop")
     ap.add_argument("--serial-waves-all", type=int, default=16,
                     help="serialized-lane waves of the run with every transaction routed")
     ap.add_argument("--escalate-after", type=int, default=2,
@@ -300,6 +389,10 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=1 << 22)
+    ap.add_argument("--cpu-sample-1t", type=int, default=1 << 20,
+                    help="transactions of the single-client CPU run")
+    ap.add_argument("--plain-txns", type=int, default=1 << 20)
+    ap.add_argument("--no-plain", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -436,6 +529,8 @@ def main():
         "every_transaction_serialized": route_all,
         "variants": variants,
     }
+    if not args.no_plain:
+        out["bank_plain"] = bank_plain_section(csmv, args)
     if not args.no_memcd:
         out["memcd"] = memcd_section(csmv, args)
     params, tau = None, 0.002
@@ -451,7 +546,8 @@ def main():
         out["serving"] = serving_section(csmv, args, params, tau)
     if not args.no_cpu:
         log("cpu baseline")
-        out["cpu_baseline"] = cpu_baseline(op, addr, args.txns, min(args.cpu_sample, args.txns))
+        out["cpu_baseline"] = cpu_baseline(op, addr, args.txns, min(args.cpu_sample, args.txns),
+                                           min(args.cpu_sample_1t, args.txns))
     print(json.dumps(out))
 
 

@@ -583,12 +583,23 @@ int orc_bank_cpu_csmv(const orc_bank_spec *s, const uint32_t *op,
     }
     c.log_ts = log_ts;
     c.log_v = log_v;
+    /* pinning: the server on the first CPU the process may run on, client i on the
+     * (i + 1)-th (wrapping when there are more threads than allowed CPUs) */
+    int cpus[CPU_SETSIZE], ncpu = 0;
+    {
+        cpu_set_t allowed;
+        CPU_ZERO(&allowed);
+        if (sched_getaffinity(0, sizeof(allowed), &allowed) == 0)
+            for (int k = 0; k < CPU_SETSIZE; ++k)
+                if (CPU_ISSET(k, &allowed)) cpus[ncpu++] = k;
+        if (ncpu == 0) cpus[ncpu++] = 0;
+    }
     double t0 = now_s();
-    args[0].c = &c; args[0].id = 0; args[0].core = cfg->pin ? 0 : -1;
+    args[0].c = &c; args[0].id = 0; args[0].core = cfg->pin ? cpus[0] : -1;
     pthread_create(&th[0], NULL, cpu_server, &args[0]);
     for (uint32_t i = 0; i < c.nc; ++i) {
         args[i + 1].c = &c; args[i + 1].id = i;
-        args[i + 1].core = cfg->pin ? (int)(i + 1) : -1;
+        args[i + 1].core = cfg->pin ? cpus[(i + 1) % (uint32_t)ncpu] : -1;
         pthread_create(&th[i + 1], NULL, cpu_client, &args[i + 1]);
     }
     for (uint32_t i = 0; i <= c.nc; ++i) pthread_join(th[i], NULL);
