@@ -210,10 +210,13 @@ def scheduler_section(csmv, args):
         t0 = time.time()
         rep = sched.train(plain, n_txns=1 << 17, steps=300)
         cal = sched.calibrate(plain, routed)
+        calk = sched.calibrate_heuristic(plain, routed)
         out["training"] = {"samples": rep.samples, "positive_rate": rep.positive_rate,
                            "loss_first": float(rep.loss[0]), "loss_last": float(rep.loss[-1]),
                            "loss_constant_predictor": rep.constant_loss, "steps": 300,
-                           "tau": cal["tau"], "seconds": time.time() - t0}
+                           "tau": cal["tau"], "tau_per_level": cal["per_level"].levels,
+                           "k_per_level": calk["per_level"].levels,
+                           "seconds": time.time() - t0}
         log(f"scheduler: trained ({rep.samples} samples), tau={cal['tau']}")
         rows = {}
         for th in (0.6, 0.9, 1.2):
@@ -222,10 +225,13 @@ def scheduler_section(csmv, args):
             sched.transfer_state(plain, routed)
             sp = sched.run_logged(plain, bank)
             routed.bank_generate(bank)
-            nh = sched.route(routed, cal["tau"])
+            nh = sched.route(routed, cal["per_level"])
             routed.fill(INIT)
             sl = routed.bank_run()
             op, _ = routed.bank_read_stream()
+            nk = sched.route_heuristic(routed, calk["per_level"])
+            routed.fill(INIT)
+            sk = routed.bank_run()
             routed.bank_route(((op & 0xFF) == 0).astype(np.uint8))
             routed.fill(INIT)
             sa = routed.bank_run()
@@ -233,6 +239,8 @@ def scheduler_section(csmv, args):
                 "plain": {"txn_per_s": sp["txn_per_s"], "abort_rate": sp["abort_rate"]},
                 "learned": {"txn_per_s": sl["txn_per_s"], "abort_rate": sl["abort_rate"],
                             "routed": nh},
+                "counter_rule": {"txn_per_s": sk["txn_per_s"], "abort_rate": sk["abort_rate"],
+                                 "routed": nk},
                 "all_transfers_routed": {"txn_per_s": sa["txn_per_s"],
                                          "abort_rate": sa["abort_rate"]},
                 "learned_over_plain": sl["txn_per_s"] / sp["txn_per_s"],
@@ -241,7 +249,76 @@ def scheduler_section(csmv, args):
                 f"{sl['txn_per_s']:.3g} all {sa['txn_per_s']:.3g}")
         out["theta"] = rows
         params = plain.pred_get_params()
-    return out, params, cal["tau"]
+    return out, params, cal["per_level"]
+
+
+def bounded_lane_section(csmv, args):
+    """Where the routing decision matters: a serialized lane of a bounded size (2 of 64
+    client waves), 2-account transfers and 32-account audits over 1M Zipf accounts.
+    Routing every transfer is capped by the lane's two waves; arrival order aborts on the
+    hot accounts; a router that sends only the contended transfers gets both.  Compared on
+    fresh streams at each skew: arrival order, every transfer routed, the counter rule
+    (route when an item's abort bucket exceeds k) and the learned predictor, both with
+    thresholds calibrated per contention level on the training skews; the final balances
+    of all four runs must be identical."""
+    import numpy as np
+    from csmv import sched
+
+    S = args.bounded_serial_waves
+
+    def mk(n, seed, th):
+        return csmv.Bank(n_txns=n, seed=seed, n_reads=2, n_reads_ro=32, pct_ro=10, zipf=True,
+                         theta=th)
+
+    def eng(serial):
+        return csmv.Engine(N_ACCOUNTS, depth=8, client_waves=64, wg_waves=6, variant="stamp",
+                           timeout_ms=30000, client_wg_waves=1, n_serial_waves=serial)
+
+    out = {"workload": f"1,048,576 Zipf accounts, 2-account transfers, 32-account audits, "
+                       f"90/10, {args.bounded_txns} txns per evaluation stream; 64 client "
+                       f"waves, {S} of them the serialized lane",
+           "serial_lane_waves": S}
+    with eng(0) as plain, eng(S) as routed:
+        t0 = time.time()
+        rep = sched.train(plain, n_txns=1 << 16, steps=300, make_bank=mk)
+        cal = sched.calibrate(plain, routed, n_txns=1 << 16, make_bank=mk)
+        calk = sched.calibrate_heuristic(plain, routed, n_txns=1 << 16, make_bank=mk)
+        out["training"] = {"samples": rep.samples, "positive_rate": rep.positive_rate,
+                           "loss_first": float(rep.loss[0]), "loss_last": float(rep.loss[-1]),
+                           "tau_per_level": cal["per_level"].levels,
+                           "k_per_level": calk["per_level"].levels,
+                           "seconds": time.time() - t0}
+        rows = {}
+        for th in (0.6, 0.9, 1.2):
+            bank = mk(args.bounded_txns, 77, th)
+            sched.history(plain, mk(1 << 16, 76, th))
+            sched.transfer_state(plain, routed)
+            sp = sched.run_logged(plain, bank)
+            finals = [plain.read_state()]
+            routed.bank_generate(bank)
+            op, _ = routed.bank_read_stream()
+            res = {"plain": {"txn_per_s": sp["txn_per_s"], "abort_rate": sp["abort_rate"]}}
+            for name, route in (
+                    ("all_transfers_routed",
+                     lambda: routed.bank_route(((op & 0xFF) == 0).astype(np.uint8)) or
+                     int(((op & 0xFF) == 0).sum())),
+                    ("counter_rule", lambda: sched.route_heuristic(routed, calk["per_level"])),
+                    ("learned", lambda: sched.route(routed, cal["per_level"]))):
+                nh = route()
+                routed.fill(INIT)
+                st = routed.bank_run()
+                finals.append(routed.read_state())
+                res[name] = {"txn_per_s": st["txn_per_s"], "abort_rate": st["abort_rate"],
+                             "routed": nh}
+            res["final_state_equal"] = all(np.array_equal(finals[0], f) for f in finals[1:])
+            res["learned_over_best_other"] = res["learned"]["txn_per_s"] / max(
+                res[k]["txn_per_s"] for k in ("plain", "all_transfers_routed", "counter_rule"))
+            rows[str(th)] = res
+            log(f"bounded lane theta={th}: plain {sp['txn_per_s']:.3g} all "
+                f"{res['all_transfers_routed']['txn_per_s']:.3g} rule "
+                f"{res['counter_rule']['txn_per_s']:.3g} learned {res['learned']['txn_per_s']:.3g}")
+        out["theta"] = rows
+    return out
 
 
 def class_rate(arr, lat):
@@ -296,16 +373,18 @@ def serving_section(csmv, args, params, tau):
                 n = eb.pred_collect()
                 loss = eb.pred_train(200, 1e-2)
                 eb.abort_fold(1)
-        res["predictor"] = {"tau": tau, "finetune_samples": n, "loss_first": float(loss[0]),
-                            "loss_last": float(loss[-1]), "seconds": time.time() - t0}
+        res["predictor"] = {"tau": getattr(tau, "levels", tau), "finetune_samples": n,
+                            "loss_first": float(loss[0]), "loss_last": float(loss[-1]),
+                            "seconds": time.time() - t0}
         eb.bank_generate(b)
-        n_hot = eb.bank_route_learned(tau)
+        from csmv import sched
+        n_hot = sched.route(eb, tau)
         for mode in ("without_predictor", "with_predictor"):
             for load in loads:
                 if mode == "without_predictor":
                     eb.bank_route(None)
                 else:
-                    eb.bank_route_learned(tau)
+                    sched.route(eb, tau)
                 ec.fill(0)
                 eb.fill(INIT)
                 out = csmv.serve(ec, eb, offered_rps=load, pct_cache=80, seed=1,
@@ -393,6 +472,9 @@ def main():
                     help="transactions of the single-client CPU run")
     ap.add_argument("--plain-txns", type=int, default=1 << 20)
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-bounded", action="store_true")
+    ap.add_argument("--bounded-txns", type=int, default=1 << 18)
+    ap.add_argument("--bounded-serial-waves", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -536,6 +618,8 @@ def main():
     params, tau = None, 0.002
     if not args.no_sched:
         out["scheduler"], params, tau = scheduler_section(csmv, args)
+    if not args.no_bounded:
+        out["scheduler_bounded_lane"] = bounded_lane_section(csmv, args)
     if not args.no_serve:
         if params is None:
             with csmv.Engine(4096, depth=4, client_waves=64, wg_waves=8, variant="stamp",
