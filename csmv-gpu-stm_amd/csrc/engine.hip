@@ -675,9 +675,14 @@ ServerLayout server_layout_at(const csmv_config &cfg, uint32_t lds_max, uint32_t
     L.mb_off = (L.tg_off + L.nmb + 63) & ~63u;
     uint32_t w = L.mb_off + L.nmb * L.mb_rows * kWave;
     // the writes of a request fill at most 1/2 >> hshift of a receiver's hash
-    L.hsz = pow2_at_least(2ull * kWave * W) << hshift;
+    // collaborative validation hashes every item of a request (lanes touching it: load at
+    // most 3/4, or 3/8 when the next shift fits), the other modes its writes (lanes
+    // writing it: load at most 1/2)
+    const bool collab = vmode == kValCollab;
+    L.hsz = pow2_at_least(kWave * (collab ? R + W : 2 * W)) << hshift;
+    L.harr = collab ? 5 : 3;
     L.hash_off = w;
-    w += M * 3 * L.hsz;
+    w += M * L.harr * L.hsz;
     // serialized lane: per-receiver chain scratch (next, weight per chain node; superseded
     // flags per write slot) and the committer's exchange of base and new values
     const uint32_t RO = chain_reads(R, W);
@@ -716,9 +721,10 @@ ServerLayout server_layout_at(const csmv_config &cfg, uint32_t lds_max, uint32_t
 ServerLayout server_layout_cfg(const csmv_config &cfg, uint32_t lds_max, uint32_t R, uint32_t W,
                                bool delegation, uint32_t nmb = 0)
 {
+    const bool stamp = (cfg.variant & 3u) == kValStamp;
     for (uint32_t hshift = 2; hshift > 0; --hshift) {
         const ServerLayout L = server_layout_at(cfg, lds_max, R, W, delegation, nmb, hshift);
-        if ((uint64_t)L.words * 4 <= lds_max && (L.exact || L.nbuckets >= 8192)) return L;
+        if ((uint64_t)L.words * 4 <= lds_max && (!stamp || L.exact || L.nbuckets >= 8192)) return L;
     }
     return server_layout_at(cfg, lds_max, R, W, delegation, nmb, 0);
 }

@@ -70,7 +70,10 @@ struct ServerLayout {
     uint32_t jmp_off;      // serialized lane: committer's base values [64 * RO], new values [64 * W]
     uint32_t atr_off;      // record ring [W][E] (SCAN / COLLAB)
     uint32_t hash_off;     // per-receiver hash tables [M][3][hsz]
-    uint32_t hsz;          // power of two >= 2 * 64 * W (the writes of one request)
+    uint32_t hsz;          // power of two >= 2 * 64 * W (the writes of one request); with
+                           // collaborative validation >= 2 * 64 * (R + W) (all its items)
+    uint32_t harr;         // arrays per receiver hash: key, writer lanes (lo, hi), and with
+                           // collaborative validation the lanes touching the item (lo, hi)
     uint32_t words;        // total
 };
 
@@ -142,6 +145,17 @@ __device__ __forceinline__ uint32_t wave_min(uint32_t v)
     for (int m = 32; m >= 1; m >>= 1) {
         uint32_t o = (uint32_t)__shfl_xor((int)v, m, 64);
         v = o < v ? o : v;
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint64_t wave_or64(uint64_t v)
+{
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, m, 64);
+        const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), m, 64);
+        v |= ((uint64_t)hi << 32) | lo;
     }
     return v;
 }
@@ -634,11 +648,19 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
                 rec = valid && (last - snap) > E;
                 const bool live = valid && !rec;
                 const uint64_t lm = ballot(live);
-                if (lm && vmode == kValScan) {
-                    // TXValidate: every lane walks back from the newest record to its
-                    // own snapshot (bank/csmv_sh/API.cu:104-122); lanes read the same
-                    // record at the same step, so the LDS read is a broadcast.
-                    const uint32_t lo = wave_min(live ? snap : kNone);
+                if (lm) {
+                    // TXValidate: every lane walks back from the newest record to its own
+                    // snapshot (bank/csmv_sh/API.cu:104-122); lanes read the same record at
+                    // the same step, so the LDS read is a broadcast.  With collaborative
+                    // validation the receiver has already scanned the records up to T_g
+                    // (pref), so the walk stops at T_g.
+                    uint32_t lo;
+                    if (vmode == kValCollab) {
+                        conflict = live && pref;
+                        lo = tg;
+                    } else {
+                        lo = wave_min(live ? snap : kNone);
+                    }
                     for (uint32_t t = last; t > lo; --t) {
                         const bool chk = live && !conflict && t > snap;
                         if (ballot(chk) == 0) break;
@@ -657,37 +679,6 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
                                 }
                             }
                         }
-                    }
-                } else if (lm) {
-                    // collaborative validation (collabValidation/csmv_sh/API.cu:212-264):
-                    // the 64 lanes split one client lane's window, lane l taking records
-                    // last - l, last - l - 64, ...; client lanes are served in lane order.
-                    uint64_t it = lm;
-                    while (it) {
-                        const uint32_t cl = (uint32_t)__builtin_ctzll(it);
-                        it &= it - 1;
-                        const uint32_t sc = readlane(snap, cl);
-                        uint32_t rsc[R], wac[W];
-#pragma unroll
-                        for (int r = 0; r < R; ++r) rsc[r] = readlane(RS(r), cl);
-#pragma unroll
-                        for (int w = 0; w < W; ++w) wac[w] = readlane(wa[w], cl);
-                        const uint32_t K = last - sc;
-                        bool cf = false;
-                        for (uint32_t base = 0; base < K; base += kWave) {
-                            const uint32_t d = base + lane;
-                            if (d < K) {
-                                const uint32_t e = (last - d) & emask;
-#pragma unroll
-                                for (int w = 0; w < W; ++w) {
-                                    const uint32_t x = atr[w * E + e];
-                                    if (x != kNone) cf |= hits<R, W>(x, rsc, wac);
-                                }
-                            }
-                            if (ballot(cf)) break;
-                        }
-                        const bool any = ballot(cf) != 0;
-                        if (lane == cl) conflict = any;
                     }
                 }
             }
@@ -842,6 +833,10 @@ __device__ __forceinline__ uint32_t hash_find(const uint32_t *hkey, uint32_t hma
     return kNone;
 }
 
+// Collaborative validation at staging: windows of at most this many records are walked by
+// every lane in step; longer ones are split across the lanes through the item hash.
+constexpr uint32_t kCollabWalk = 64;
+
 // Serialized lane, at staging: resolve the producer chains of a delegated request from its
 // staged rows.  A chain node is one of the first RO reads of a lane; its producer (staged
 // code: lane | write slot << 6) is the highest lower lane writing the item, i.e. that lane's
@@ -945,8 +940,13 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
     uint32_t *mbwave = lds + L.wave_off;
     uint32_t *mbtg = lds + L.tg_off;
     // per-receiver hash: key, write-lane mask (lo, hi)
-    uint32_t *hkey = lds + L.hash_off + rid * 3 * L.hsz;
+    uint32_t *hkey = lds + L.hash_off + rid * L.harr * L.hsz;
     uint32_t *hwlo = hkey + L.hsz, *hwhi = hkey + 2 * L.hsz;
+    // collaborative validation: lanes reading or writing each item
+    const bool collab = (c.variant & 0x3u) == kValCollab;
+    uint32_t *hrlo = hkey + 3 * L.hsz, *hrhi = hkey + 4 * L.hsz;
+    const uint32_t E = c.atr_entries, emask = E - 1;
+    const uint32_t *atr = lds + L.atr_off;
     const uint32_t hmask = L.hsz - 1;
     const uint32_t mybit = 1u << (lane & 31);
     uint32_t *chain_scr = lds + L.sup_off +
@@ -1263,11 +1263,118 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
                 }
             }
             asm volatile("" ::: "memory");
+            // Collaborative validation at staging (collabValidation/csmv_sh/API.cu:212-264
+            // splits one client's window over a warp, one client after another).  Here the
+            // records committed before the request was staged, ts in (snap, T_g], are
+            // scanned once for the whole request, the 64 lanes splitting the window: every
+            // item a record wrote is looked up in an LDS hash of the request's items (item ->
+            // lanes reading or writing it), and a lane conflicts when a record newer than
+            // its snapshot wrote one of its items.  The committer checks only (T_g, last].
+            uint32_t slot_r[R];
+    #pragma unroll
+            for (int r = 0; r < R; ++r) slot_r[r] = kNone;
+            // a record older than T_g - E belongs to lanes the committer fails with
+            // RecordAge (its ring no longer covers their window)
+            uint32_t clo = collab ? wave_min(valid ? snap : kNone) : kNone;
+            if (collab && clo != kNone && tg > E && clo < tg - E) clo = tg - E;
+            const uint32_t cwin = (clo != kNone && tg > clo) ? tg - clo : 0u;
+            if (collab && cwin <= kCollabWalk) {
+                // a short window: every lane walks it, reading the same record at each step
+                bool cf = false;
+                for (uint32_t t = tg; t > clo; --t) {
+                    const bool chk = valid && !cf && t > snap;
+                    if (ballot(chk) == 0) break;
+                    const uint32_t e = t & emask;
+                    if (chk) {
+    #pragma unroll
+                        for (int w = 0; w < W; ++w) {
+                            const uint32_t x = atr[w * E + e];
+                            bool h = false;
+    #pragma unroll
+                            for (int r = 0; r < R + W; ++r) h |= it[r] == x;
+                            cf |= x != kNone && h;
+                        }
+                    }
+                }
+                pref = valid && cf;
+            } else if (collab) {
+                // a slot for every read as well (the writes hold theirs)
+                uint32_t hq[R];
+                bool pq[R];
+    #pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    hq[r] = mx[r] & hmask;
+                    pq[r] = valid && it[r] != kNone;
+                }
+                for (uint32_t step = 0; step <= hmask; ++step) {
+                    uint32_t old[R];
+    #pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        old[r] = kNone;
+                        __hip_atomic_compare_exchange_strong(&hkey[pq[r] ? hq[r] : 0u], &old[r],
+                                                             pq[r] ? it[r] : kNone, __ATOMIC_RELAXED,
+                                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    }
+                    bool more = false;
+    #pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        const bool got = pq[r] && (old[r] == kNone || old[r] == it[r]);
+                        slot_r[r] = got ? hq[r] : slot_r[r];
+                        hq[r] = (pq[r] && !got) ? ((hq[r] + 1) & hmask) : hq[r];
+                        pq[r] = pq[r] && !got;
+                        more |= pq[r];
+                    }
+                    if (ballot(more) == 0) break;
+                }
+    #pragma unroll
+                for (int r = 0; r < R; ++r)
+                    if (slot_r[r] != kNone) atomicOr(lane < 32 ? &hrlo[slot_r[r]] : &hrhi[slot_r[r]], mybit);
+    #pragma unroll
+                for (int w = 0; w < W; ++w)
+                    if (slot_w[w] != kNone) atomicOr(lane < 32 ? &hrlo[slot_w[w]] : &hrhi[slot_w[w]], mybit);
+                // the lanes' snapshots, read by the lanes that scan the window
+                dst[RW::snap * kWave + lane] = snap;
+                asm volatile("" ::: "memory");
+                const uint32_t lo = clo;
+                uint64_t cm = 0;
+                for (uint32_t top = tg; top > lo && lo != kNone; top = (top - lo > kWave) ? top - kWave : lo) {
+                    const uint32_t t = top - lane;
+                    if (lane < top - lo) {
+                        const uint32_t e = t & emask;
+    #pragma unroll
+                        for (int w = 0; w < W; ++w) {
+                            const uint32_t x = atr[w * E + e];
+                            const uint32_t h = x != kNone ? hash_find(hkey, hmask, x) : kNone;
+                            if (h != kNone) {
+                                uint64_t m = ((uint64_t)__hip_atomic_load(&hrhi[h], __ATOMIC_RELAXED,
+                                                                          __HIP_MEMORY_SCOPE_WORKGROUP) << 32) |
+                                             __hip_atomic_load(&hrlo[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                                m &= ~cm;
+                                while (m) {
+                                    const uint32_t b = (uint32_t)__builtin_ctzll(m);
+                                    m &= m - 1;
+                                    if (dst[RW::snap * kWave + b] < t) cm |= 1ull << b;
+                                }
+                            }
+                        }
+                    }
+                }
+                cm = wave_or64(cm);
+                pref = valid && ((cm >> lane) & 1ull) != 0;
+                asm volatile("" ::: "memory");
+            }
     #pragma unroll
             for (int w = 0; w < W; ++w)
                 if (slot_w[w] != kNone) {
                     hkey[slot_w[w]] = kNone;
                     hwlo[slot_w[w]] = 0; hwhi[slot_w[w]] = 0;
+                    if (collab) { hrlo[slot_w[w]] = 0; hrhi[slot_w[w]] = 0; }
+                }
+    #pragma unroll
+            for (int r = 0; r < R; ++r)
+                if (slot_r[r] != kNone) {
+                    hkey[slot_r[r]] = kNone;
+                    hrlo[slot_r[r]] = 0; hrhi[slot_r[r]] = 0;
                 }
     
             dst[RW::snap * kWave + lane] = snap;
@@ -1321,9 +1428,9 @@ __device__ __forceinline__ void server_workgroup(const Ctx &c, const ServerLayou
     const uint32_t M = c.wg_waves - 2;
     for (uint32_t r = 0; r < M; ++r)
         for (uint32_t i = threadIdx.x; i < L.hsz; i += blockDim.x) {
-            uint32_t *h = lds + L.hash_off + r * 3 * L.hsz;
+            uint32_t *h = lds + L.hash_off + r * L.harr * L.hsz;
             h[i] = kNone;
-            for (uint32_t q = 1; q < 3; ++q) h[q * L.hsz + i] = 0;
+            for (uint32_t q = 1; q < L.harr; ++q) h[q * L.hsz + i] = 0;
         }
     __syncthreads();
     // Waves are spread over the CU's four SIMDs in turn, so wave 4 shares the committer's

@@ -115,10 +115,12 @@ def test_escalation_instead_of_livelock():
     assert st["escalated"] > 0
 
 
-def test_record_age_aborts():
+@pytest.mark.parametrize("variant", ["batchedAtrInsert", "collabValidation"])
+def test_record_age_aborts(variant):
     """A record ring much smaller than the in-flight window forces RecordAge aborts
-    (bank/csmv_sh/API.cu:93-100); the log must still replay."""
-    st = run_bank(4096, csmv.Bank(n_txns=1 << 14, seed=3), variant="batchedAtrInsert",
+    (bank/csmv_sh/API.cu:93-100), also while receivers scan records the committer is
+    overwriting (collaborative validation at staging); the log must still replay."""
+    st = run_bank(4096, csmv.Bank(n_txns=1 << 14, seed=3), variant=variant,
                   atr=64, waves=32)
     assert st["aborts_recordage"] > 0
 
