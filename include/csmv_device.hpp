@@ -159,7 +159,9 @@ struct Ctx {
     uint32_t *mbox;          // [client_waves][rows][64] request mailboxes
     uint32_t *stamps;        // [n_items] commit stamps (server-private, STAMP mode)
     int32_t *vals;           // [n_items] newest committed values (server-private)
-    uint64_t *reply_v;       // [client_waves * 64 * W] (n+1) << 32 | new value (delegated)
+    uint64_t *reply_v;       // [client_waves][2W][64] (n+1) << 32 | new value, then predecessor
+                             // stamp of each write (delegated); word-major per wave, so one
+                             // store or load of 64 lanes covers 512 contiguous bytes
     uint64_t *prof;          // kProf* timing counters (wall_clock64 ticks)
     uint32_t qmask;
     uint32_t n_client_waves, wg_waves;
@@ -549,9 +551,9 @@ __device__ __forceinline__ uint32_t tx_commit(const Ctx &c, uint32_t cw, Tx<R, W
     uint64_t xy[2 * W + 1];
     for (;;) {
         if (delegate) {
-            const uint64_t *rv = c.reply_v + ((size_t)cw * kWave + lane) * 2 * W;
+            const uint64_t *rv = c.reply_v + (size_t)cw * 2 * W * kWave + lane;
 #pragma unroll
-            for (int q = 0; q < 2 * W; ++q) xy[q] = ld_rlx(&rv[q]);
+            for (int q = 0; q < 2 * W; ++q) xy[q] = ld_rlx(&rv[(size_t)q * kWave]);
         }
         xy[2 * W] = ld_rlx(&c.reply[(size_t)cw * kWave + lane]);
         pin_round(xy);

@@ -312,7 +312,7 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
     // DS instructions of a wave execute in order and the receiver writes the rows before
     // it releases the state, so rows read after a state that shows the request are that
     // request's (a batch read too early is read again).
-    auto read_rows = [&](uint32_t m) -> uint32_t {
+    auto read_into = [&](uint32_t m, uint32_t (&row)[kNr]) -> uint32_t {
         const uint32_t *bx = lds + L.mb_off + m * L.mb_rows * kWave;
         const uint32_t st = __hip_atomic_load(&mbst[m], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         wave_fence();
@@ -345,14 +345,24 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
         pin_round(row);
         return st;
     };
+    // A narrow wire's next request is read ahead while the current one's HBM loads are in
+    // flight (its mailbox cannot be restaged before the committer publishes that it took
+    // it); a wide wire's mailbox is read on use, so it is read when its turn comes.
+    constexpr bool kAhead = !kWide;
+    uint32_t ahead[kNr];
+    uint32_t ahead_st = 0;
     for (;; ++n, mbi = mbi + 1 == L.nmb ? 0u : mbi + 1) {
         uint32_t s = 0;
         bool bad = false;
         const uint32_t *box = lds + L.mb_off + mbi * L.mb_rows * kWave;
-        {
+        if (kAhead && (ahead_st & ~kExitBit) == n + 1) {
+            s = ahead_st;
+#pragma unroll
+            for (int i = 0; i < kNr; ++i) row[i] = ahead[i];
+        } else {
             Spin sp;
             for (;;) {
-                s = read_rows(mbi);
+                s = read_into(mbi, row);
                 if ((s & ~kExitBit) == n + 1) break;
                 if (!sp.tick(c, kErrMailboxTimeout)) { bad = true; break; }
             }
@@ -440,6 +450,7 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
             lds_st_rel(&lds[kLdsStampTs], last);
             lds_st_rel(&lds[kLdsCnext], n + 1);
         }
+        if constexpr (kAhead) ahead_st = read_into(mbi + 1 == L.nmb ? 0u : mbi + 1, ahead);
         const uint64_t tr = phase_clock();
         uint64_t tres = tr, tchain = tr;
         const bool valid = snap != kNone;
@@ -578,11 +589,11 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
                 for (int r = 1; r < RO; ++r)
                     if (!ro && ops[r] != kNone) c.log_v[(uint64_t)r * c.n_txns + k] = rd[r];
             }
-            uint64_t *rv = c.reply_v + ((size_t)cw * kWave + lane) * 2 * W;
+            uint64_t *rv = c.reply_v + (size_t)cw * 2 * W * kWave + lane;
 #pragma unroll
             for (int w = 0; w < W; ++w) {
-                st_rlx(&rv[w], ((uint64_t)(n + 1) << 32) | (uint32_t)nv[w]);
-                st_rlx(&rv[W + w], ((uint64_t)(n + 1) << 32) | (inst[w] ? prev[w] : kNone));
+                st_rlx(&rv[(size_t)w * kWave], ((uint64_t)(n + 1) << 32) | (uint32_t)nv[w]);
+                st_rlx(&rv[(size_t)(W + w) * kWave], ((uint64_t)(n + 1) << 32) | (inst[w] ? prev[w] : kNone));
             }
         } else {
             bool rec = false, conflict = false;
