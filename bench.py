@@ -322,18 +322,33 @@ def bounded_lane_section(csmv, args):
 
 
 def class_rate(arr, lat):
-    """Requests per second of one class over its own window: first arrival to last
-    commit (arrival + latency), in ticks."""
+    """Requests and window of one class: first arrival to last commit (arrival +
+    latency), in ticks."""
     import numpy as np
     end = (arr.astype(np.int64) + lat.astype(np.int64)).max()
     return arr.size, int(end - int(arr.min()))
 
 
-def serving_section(csmv, args, params, tau):
-    """C5: offered-load sweep, with and without the predictor."""
+def class_rate99(arr, lat, khz):
+    """Requests per second of one class until 99 % of its requests have committed: 0.99 n
+    over first arrival to the 99th-percentile commit time (a straggler tail, such as SETs
+    queued behind the hottest key, does not set it)."""
     import numpy as np
+    end = arr.astype(np.int64) + lat.astype(np.int64)
+    t99 = float(np.percentile(end, 99)) - float(arr.min())
+    return 0.99 * arr.size / (t99 / (khz * 1e3))
+
+
+def serving_section(csmv, args, params, tau):
+    """C5: offered-load sweep, with and without the predictor; the cache class split over
+    args.serve_cache_parts commit servers by set."""
+    import dataclasses
+
+    import numpy as np
+    from csmv import sched
 
     loads = [float(x) for x in args.serve_loads.split(",")]
+    P = args.serve_cache_parts
     m = csmv.Memcd(n_reqs=args.serve_cache_reqs, seed=1)
     b = csmv.Bank(n_txns=args.serve_bank_txns, seed=7, n_reads=64, n_reads_min=2, pct_ro=0,
                   zipf=True, theta=0.99)
@@ -342,83 +357,104 @@ def serving_section(csmv, args, params, tau):
         return csmv.Bank(n_txns=1 << 13, seed=seed, n_reads=64, n_reads_min=2, pct_ro=0,
                          zipf=True, theta=0.99)
 
+    with csmv.Engine(m.n_items, depth=4, client_waves=1, wg_waves=6) as gen:
+        gen.memcd_generate(m)
+        key, op = gen.memcd_read_stream()
+    parts = csmv.memcd_partition(key, op, m.n_sets, P)
+    local = dataclasses.replace(m, n_sets=m.n_sets // P)
     res = {"workload": f"{m.n_reqs} cache requests (the first requests of the cache config "
-                       f"stream, cold cache; 80 %) + {b.n_txns} bank transactions (transfer "
-                       "over the first 2 of 2..64 Zipf 0.99 reads of 1M accounts; 20 %), "
-                       f"Poisson arrivals; cache {args.serve_cache_waves} client waves, bank 16 "
-                       "(8 serialized-lane when routed)",
+                       f"stream, cold cache; 80 %) split by set over {P} commit servers of "
+                       f"{args.serve_cache_waves} client waves each, + {b.n_txns} bank "
+                       "transactions (transfer over the first 2 of 2..64 Zipf 0.99 reads of 1M "
+                       "accounts; 20 %) on one commit server with 16 client waves (8 "
+                       "serialized-lane when routed), Poisson arrivals",
+           "cache_partitions": P,
            "pool_budget_bytes": args.serve_budget,
            "bank_escalate_after": args.escalate_after,
            "loads": {}}
-    with csmv.Engine(m.n_items, depth=4, client_waves=args.serve_cache_waves, wg_waves=6,
-                     variant="stamp", timeout_ms=30000,
-                     client_wg_waves=args.client_wg_waves) as ec, \
-         csmv.Engine(N_ACCOUNTS, depth=8, client_waves=16, wg_waves=6, variant="stamp",
-                     timeout_ms=30000, n_serial_waves=8,
-                     client_wg_waves=args.serve_bank_client_wg_waves,
-                     escalate_after=args.escalate_after) as eb:
-        ec.memcd_generate(m)
-        # the predictor (trained on the bank-plain abort logs) is fine-tuned on this
-        # partition's own logs: history stream 1 gives the counters, stream 2 the labels
-        t0 = time.time()
-        eb.pred_set_params(params)
-        for i, seed in enumerate((101, 102)):
-            eb.fill(INIT)
-            eb.bank_generate(hist(seed))
-            csmv.serve(None, eb, offered_rps=loads[0], pct_cache=80, seed=3 + i)
-            if i == 0:
-                eb.abort_fold(32)
-            else:
-                eb.pred_clear_dataset()
-                n = eb.pred_collect()
-                loss = eb.pred_train(200, 1e-2)
-                eb.abort_fold(1)
-        res["predictor"] = {"tau": getattr(tau, "levels", tau), "finetune_samples": n,
-                            "loss_first": float(loss[0]), "loss_last": float(loss[-1]),
-                            "seconds": time.time() - t0}
-        eb.bank_generate(b)
-        from csmv import sched
-        n_hot = sched.route(eb, tau)
-        for mode in ("without_predictor", "with_predictor"):
-            for load in loads:
-                if mode == "without_predictor":
-                    eb.bank_route(None)
-                else:
-                    sched.route(eb, tau)
-                ec.fill(0)
+    caches = [csmv.Engine(local.n_items, depth=4, client_waves=args.serve_cache_waves, wg_waves=6,
+                          variant="stamp", timeout_ms=30000, client_wg_waves=args.client_wg_waves)
+              for _ in range(P)]
+    try:
+        with csmv.Engine(N_ACCOUNTS, depth=8, client_waves=16, wg_waves=6, variant="stamp",
+                         timeout_ms=30000, n_serial_waves=8,
+                         client_wg_waves=args.serve_bank_client_wg_waves,
+                         escalate_after=args.escalate_after) as eb:
+            for ec, (k, o) in zip(caches, parts):
+                ec.memcd_load(local, k, o)
+            # the predictor (trained on the bank-plain abort logs) is fine-tuned on this
+            # partition's own logs: history stream 1 gives the counters, stream 2 the labels
+            t0 = time.time()
+            eb.pred_set_params(params)
+            for i, seed in enumerate((101, 102)):
                 eb.fill(INIT)
-                out = csmv.serve(ec, eb, offered_rps=load, pct_cache=80, seed=1,
-                                 pool_budget=args.serve_budget)
-                khz = out["wallclock_khz"]
-                # algorithmic bytes: cache requests as cache_bytes, bank transactions
-                # 12 R + 12 W + 12 (R = 2..64 reads, W = 2)
-                _, mop = ec.memcd_read_stream()
-                bop, _ = eb.bank_read_stream()
-                nbytes = cache_bytes(mop, ec.memcd_read_log()["res"]) + int(
-                    (12 * ((bop >> 16) & 0xFF).astype(np.int64) + 36).sum())
-                lc, lb = ec.serve_latency(), eb.serve_latency()
-                ac, ab = ec.serve_arrivals(), eb.serve_arrivals()
-                nc_, wc = class_rate(ac, lc)
-                nb_, wb = class_rate(ab, lb)
-                _, wall = class_rate(np.concatenate([ac, ab]), np.concatenate([lc, lb]))
-                row = {"offered_rps": load, "kernel_ms": out["kernel_ms"],
-                       "offered_window_ms": float(max(ac[-1], ab[-1])) / khz,
-                       "req_per_s": (nc_ + nb_) / (wall / (khz * 1e3)),
-                       "cache_req_per_s": nc_ / (wc / (khz * 1e3)),
-                       "bank_txn_per_s": nb_ / (wb / (khz * 1e3)),
-                       "all": pct_us(np.concatenate([lc, lb]), khz), "cache": pct_us(lc, khz),
-                       "bank": pct_us(lb, khz),
-                       "cache_abort_rate": out["cache"]["abort_rate"],
-                       "bank_abort_rate": out["bank"]["abort_rate"],
-                       "bank_routed": n_hot if mode == "with_predictor" else 0,
-                       "bank_escalated": out["bank"]["escalated"],
-                       "admit_stalls": out["admit_stalls"],
-                       "peak_inflight_bytes": out["peak_inflight"],
-                       "roofline": roofline(nbytes, out["kernel_ms"])}
-                res["loads"].setdefault(mode, []).append(row)
-                log(f"serving {mode} {load:.3g} req/s: {row['req_per_s']:.3g} served, p99 "
-                    f"{row['all']['p99_us']:.1f} us (cache {row['cache']['p99_us']:.1f}, bank "
-                    f"{row['bank']['p99_us']:.1f}), routed {row['bank_routed']}")
+                eb.bank_generate(hist(seed))
+                csmv.serve(None, eb, offered_rps=loads[0], pct_cache=80, seed=3 + i)
+                if i == 0:
+                    eb.abort_fold(32)
+                else:
+                    eb.pred_clear_dataset()
+                    n = eb.pred_collect()
+                    loss = eb.pred_train(200, 1e-2)
+                    eb.abort_fold(1)
+            res["predictor"] = {"tau": getattr(tau, "levels", tau), "finetune_samples": n,
+                                "loss_first": float(loss[0]), "loss_last": float(loss[-1]),
+                                "seconds": time.time() - t0}
+            eb.bank_generate(b)
+            n_hot = sched.route(eb, tau)
+            bop, _ = eb.bank_read_stream()
+            bank_bytes = int((12 * ((bop >> 16) & 0xFF).astype(np.int64) + 36).sum())
+            for mode in ("without_predictor", "with_predictor"):
+                for load in loads:
+                    if mode == "without_predictor":
+                        eb.bank_route(None)
+                    else:
+                        sched.route(eb, tau)
+                    for ec in caches:
+                        ec.fill(0)
+                    eb.fill(INIT)
+                    out = csmv.serve(caches, eb, offered_rps=load, pct_cache=80, seed=1,
+                                     pool_budget=args.serve_budget)
+                    khz = out["wallclock_khz"]
+                    lc = np.concatenate([ec.serve_latency() for ec in caches])
+                    ac = np.concatenate([ec.serve_arrivals() for ec in caches])
+                    lb, ab = eb.serve_latency(), eb.serve_arrivals()
+                    nc_, wc = class_rate(ac, lc)
+                    nb_, wb = class_rate(ab, lb)
+                    _, wall = class_rate(np.concatenate([ac, ab]), np.concatenate([lc, lb]))
+                    # algorithmic bytes: cache requests as cache_bytes, bank transactions
+                    # 12 R + 12 W + 12 (R = 2..64 reads, W = 2)
+                    nbytes = bank_bytes + sum(cache_bytes(ec.memcd_read_stream()[1],
+                                                          ec.memcd_read_log()["res"])
+                                              for ec in caches)
+                    row = {"offered_rps": load, "kernel_ms": out["kernel_ms"],
+                           "offered_window_ms": float(max(ac.max(), ab.max())) / khz,
+                           "req_per_s": (nc_ + nb_) / (wall / (khz * 1e3)),
+                           "cache_req_per_s": nc_ / (wc / (khz * 1e3)),
+                           "bank_txn_per_s": nb_ / (wb / (khz * 1e3)),
+                           "req_per_s_99": class_rate99(np.concatenate([ac, ab]),
+                                                        np.concatenate([lc, lb]), khz),
+                           "cache_req_per_s_99": class_rate99(ac, lc, khz),
+                           "bank_txn_per_s_99": class_rate99(ab, lb, khz),
+                           "cache_offered_rps": 0.8 * load, "bank_offered_rps": 0.2 * load,
+                           "all": pct_us(np.concatenate([lc, lb]), khz), "cache": pct_us(lc, khz),
+                           "bank": pct_us(lb, khz),
+                           "cache_abort_rate": out["cache"]["abort_rate"],
+                           "bank_abort_rate": out["bank"]["abort_rate"],
+                           "bank_routed": n_hot if mode == "with_predictor" else 0,
+                           "bank_escalated": out["bank"]["escalated"],
+                           "admit_stalls": out["admit_stalls"],
+                           "peak_inflight_bytes": out["peak_inflight"],
+                           "roofline": roofline(nbytes, out["kernel_ms"])}
+                    res["loads"].setdefault(mode, []).append(row)
+                    log(f"serving {mode} {load:.3g} req/s: {row['req_per_s']:.3g} served (cache "
+                        f"{row['cache_req_per_s']:.3g} / {row['cache_req_per_s_99']:.3g} to 99 %, "
+                        f"bank {row['bank_txn_per_s']:.3g}), p99 "
+                        f"{row['all']['p99_us']:.1f} us (cache {row['cache']['p99_us']:.1f}, bank "
+                        f"{row['bank']['p99_us']:.1f}), stalls {row['admit_stalls']}")
+    finally:
+        for ec in caches:
+            ec.close()
     return res
 
 
@@ -453,8 +489,11 @@ def main():
     ap.add_argument("--serve-loads", default="5e6,50e6,100e6,200e6,500e6")
     ap.add_argument("--serve-cache-reqs", type=int, default=1 << 18)
     ap.add_argument("--serve-bank-txns", type=int, default=1 << 16)
-    ap.add_argument("--serve-cache-waves", type=int, default=16)
-    ap.add_argument("--serve-budget", type=int, default=32768,
+    ap.add_argument("--serve-cache-waves", type=int, default=12,
+                    help="client waves of each cache partition of the serving loop")
+    ap.add_argument("--serve-cache-parts", type=int, default=4,
+                    help="commit servers the serving loop's cache class is split over (by set)")
+    ap.add_argument("--serve-budget", type=int, default=65536,
                     help="admission cap on in-flight version-pool bytes of the serving loop")
     ap.add_argument("--serial-waves-all", type=int, default=16,
                     help="serialized-lane waves of the run with every transaction routed")

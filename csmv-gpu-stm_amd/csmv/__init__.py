@@ -170,6 +170,8 @@ def lib() -> C.CDLL:
         "csmv_memcd_read_log": ([P, P, P, P, P, P], C.c_int),
         "csmv_wallclock_khz": ([P, C.POINTER(U32)], C.c_int),
         "csmv_serve_run": ([P, P, C.POINTER(ServeSpec), C.POINTER(ServeStats)], C.c_int),
+        "csmv_serve_run_multi": ([P, U32, P, C.POINTER(ServeSpec), C.POINTER(ServeStats)], C.c_int),
+        "csmv_memcd_load": ([P, C.POINTER(MemcdSpec), P, P], C.c_int),
         "csmv_serve_read_arrivals": ([P, P, SZ], C.c_int),
         "csmv_serve_read_latency": ([P, P, SZ], C.c_int),
         "csmv_pred_init": ([P, U64], C.c_int),
@@ -374,6 +376,16 @@ class Engine:
         self._chk(lib().csmv_memcd_generate(self._h, C.byref(m.spec())), "memcd_generate")
         self.memcd = m
 
+    def memcd_load(self, m: Memcd, key: np.ndarray, op: np.ndarray):
+        """A request stream from host arrays (one partition of a split stream); m gives the
+        engine's sets, ways and batch, and n_reqs = key.size."""
+        k = np.ascontiguousarray(key, dtype=np.uint64)
+        o = np.ascontiguousarray(op, dtype=np.uint32)
+        assert k.size == o.size
+        m = dataclasses.replace(m, n_reqs=int(k.size))
+        self._chk(lib().csmv_memcd_load(self._h, C.byref(m.spec()), _ptr(k), _ptr(o)), "memcd_load")
+        self.memcd = m
+
     def memcd_read_stream(self):
         m = self.memcd
         key = np.empty(m.n_reqs, dtype=np.uint64)
@@ -508,23 +520,52 @@ class Engine:
         return out
 
 
-def serve(cache: Optional[Engine], bank: Optional[Engine], offered_rps: float,
+def memcd_set_of(key: np.ndarray, n_sets: int) -> np.ndarray:
+    """The cache set of each key (the device's mc_set_of: a splitmix64 finaliser)."""
+    z = np.asarray(key, dtype=np.uint64) + np.uint64(0x9E3779B97F4A7C15)
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    z ^= z >> np.uint64(31)
+    return (z & np.uint64(n_sets - 1)).astype(np.uint32)
+
+
+def memcd_partition(key: np.ndarray, op: np.ndarray, n_sets: int, n_parts: int):
+    """Splits a cache request stream over n_parts partitions owning contiguous ranges of
+    n_sets / n_parts sets (partition = set // (n_sets / n_parts)).  Within a partition a
+    request's set is set mod (n_sets / n_parts), the set a cache of n_sets / n_parts sets
+    gives its key, so each partition is an ordinary cache stream (csmv_memcd_load)."""
+    assert n_parts >= 1 and n_sets % n_parts == 0 and (n_sets // n_parts) & (n_sets // n_parts - 1) == 0
+    per = n_sets // n_parts
+    part = memcd_set_of(key, n_sets) // np.uint32(per)
+    return [(np.ascontiguousarray(key[part == p]), np.ascontiguousarray(op[part == p]))
+            for p in range(n_parts)]
+
+
+def serve(cache, bank: Optional[Engine], offered_rps: float,
           pct_cache: int = 80, seed: int = 1, pool_budget: int = 0) -> dict:
-    """Runs the persistent serving loop over a cache partition and a bank partition
-    (either may be None: that class alone, at the arrival times it has in the mix)."""
+    """Runs the persistent serving loop over a cache partition (an Engine, a list of
+    Engines holding disjoint ranges of sets, or None) and a bank partition (or None): a
+    class that is None is not served, and the other keeps the arrival times it has in the
+    mix."""
     sp = ServeSpec(seed, float(offered_rps), pct_cache, 0, pool_budget)
     st = ServeStats()
-    h_c = cache._h if cache is not None else None
+    caches = [] if cache is None else (list(cache) if isinstance(cache, (list, tuple)) else [cache])
     h_b = bank._h if bank is not None else None
-    rc = lib().csmv_serve_run(h_c, h_b, C.byref(sp), C.byref(st))
-    for e, name in ((cache, "cache"), (bank, "bank")):
-        if e is not None:
-            e._served = name
+    if len(caches) <= 1:
+        h_c = caches[0]._h if caches else None
+        rc = lib().csmv_serve_run(h_c, h_b, C.byref(sp), C.byref(st))
+    else:
+        arr = (C.c_void_p * len(caches))(*[e._h.value for e in caches])
+        rc = lib().csmv_serve_run_multi(arr, len(caches), h_b, C.byref(sp), C.byref(st))
+    for e in caches:
+        e._served = "cache"
+    if bank is not None:
+        bank._served = "bank"
     if rc:
-        e = cache if cache is not None else bank
+        e = caches[0] if caches else bank
         msg = lib().csmv_last_error(e._h) or b""
         raise CsmvError(rc, f"serve: {msg.decode(errors='replace')}")
-    return {"cache": st.cache.as_dict() if cache is not None else None,
+    return {"cache": st.cache.as_dict() if caches else None,
             "bank": st.bank.as_dict() if bank is not None else None,
             "kernel_ms": st.kernel_ms, "admit_stalls": st.admit_stalls,
             "peak_inflight": st.peak_inflight, "final_inflight": st.final_inflight,

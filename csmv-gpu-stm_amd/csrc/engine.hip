@@ -15,6 +15,7 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "csmv.h"
 #include "csmv_device.hpp"
@@ -455,37 +456,43 @@ __global__ void __launch_bounds__(512) memcd_kernel(Ctx c, ServerLayout L)
     if (cw < c.n_client_waves) memcd_client<R, W>(c, cw);
 }
 
-// Serving loop: one persistent kernel with a commit server per partition (the cache
-// items and the bank accounts are disjoint, so each partition keeps its own clock and
-// commit order) and client workgroups of each class.
-struct ServeRoles {
-    uint32_t has_cache, has_bank, cache_wgs;
+// Serving loop: one persistent kernel with a commit server per partition and the client
+// workgroups of each partition.  Partitions hold disjoint data (cache partitions disjoint
+// ranges of sets, the bank partition the accounts), so each keeps its own clock and commit
+// order.  Partitions 0 .. n_cache - 1 are cache partitions; the bank partition, if any, is
+// the last.  Every partition's context travels in the kernel arguments.
+constexpr uint32_t kMaxServeParts = 8;
+struct ServeArgs {
+    Ctx c[kMaxServeParts];
+    ServerLayout L[kMaxServeParts];
+    uint32_t n_cache, has_bank;
+    uint32_t wg_end[kMaxServeParts];   // client workgroups of partitions 0..i (prefix sums)
 };
+static_assert(sizeof(ServeArgs) <= 4096, "the serving kernel's arguments fit the kernarg limit");
 
 template <int RM, int WM, int RB, int WB>
-__global__ void __launch_bounds__(512) serve_kernel(Ctx cm, ServerLayout Lm, Ctx cb,
-                                                    ServerLayout Lb, ServeRoles roles)
+__global__ void __launch_bounds__(512) serve_kernel(ServeArgs a)
 {
     extern __shared__ uint32_t lds[];
-    const uint32_t n_srv = roles.has_cache + roles.has_bank;
-    if (blockIdx.x < n_srv) {
-        if (roles.has_cache && blockIdx.x == 0) server_workgroup<RM, WM, false>(cm, Lm, lds);
-        else server_workgroup<RB, WB, true>(cb, Lb, lds);
+    const uint32_t n_part = a.n_cache + a.has_bank;
+    if (blockIdx.x < n_part) {
+        const uint32_t p = blockIdx.x;
+        if (p < a.n_cache) server_workgroup<RM, WM, false>(a.c[p], a.L[p], lds);
+        else server_workgroup<RB, WB, true>(a.c[p], a.L[p], lds);
         return;
     }
-    const uint32_t b = blockIdx.x - n_srv;
+    const uint32_t b = blockIdx.x - n_part;
+    uint32_t p = 0;
+    while (p + 1 < n_part && b >= a.wg_end[p]) ++p;
+    const uint32_t b0 = p ? a.wg_end[p - 1] : 0u;
+    const Ctx &c = a.c[p];
     const uint32_t wave = threadIdx.x / kWave;
-    if (b < roles.cache_wgs) {
-        if (wave >= cm.client_wg_waves) return;
-        const uint32_t cw = b * cm.client_wg_waves + wave;
-        if (cw < cm.n_client_waves) memcd_client<RM, WM>(cm, cw);
-    } else {
-        if (wave >= cb.client_wg_waves) return;
-        const uint32_t cw = (b - roles.cache_wgs) * cb.client_wg_waves + wave;
-        if (cw >= cb.n_client_waves) return;
-        if (cw >= cb.n_client_waves - cb.n_serial_waves) bank_serve_serial_client<RB, WB>(cb, cw);
-        else bank_serve_client<RB, WB>(cb, cw);
-    }
+    if (wave >= c.client_wg_waves) return;
+    const uint32_t cw = (b - b0) * c.client_wg_waves + wave;
+    if (cw >= c.n_client_waves) return;
+    if (p < a.n_cache) memcd_client<RM, WM>(c, cw);
+    else if (cw >= c.n_client_waves - c.n_serial_waves) bank_serve_serial_client<RB, WB>(c, cw);
+    else bank_serve_client<RB, WB>(c, cw);
 }
 
 // Same-address agent-scope atomic rate: lane 0 of every wave adds to one word.
@@ -1194,6 +1201,47 @@ csmv_status csmv_bank_run(csmv_engine *e, csmv_stats *out)
 }
 
 // ---- memcached-style cache ---------------------------------------------------------------
+}  // extern "C"
+
+// Device arrays of a cache request stream of T requests and its log.
+static csmv_status mc_reserve(csmv_engine *e, uint64_t T)
+{
+    if (T <= e->mc_cap) return CSMV_OK;
+    dfree(e->mc_key); dfree(e->mc_op); dfree(e->mc_ts); dfree(e->mc_res); dfree(e->mc_vsum);
+    dfree(e->mc_ab); dfree(e->mc_lat);
+    e->mc_cap = 0;
+    csmv_status st;
+    if ((st = dalloc(e, &e->mc_key, T)) || (st = dalloc(e, &e->mc_op, T)) ||
+        (st = dalloc(e, &e->mc_ts, T)) || (st = dalloc(e, &e->mc_res, T)) ||
+        (st = dalloc(e, &e->mc_vsum, T)) || (st = dalloc(e, &e->mc_ab, T)) ||
+        (st = dalloc(e, &e->mc_lat, T)))
+        return st;
+    e->mc_cap = T;
+    return CSMV_OK;
+}
+
+extern "C" {
+
+csmv_status csmv_memcd_load(csmv_engine *e, const csmv_memcd_spec *s, const uint64_t *key,
+                            const uint32_t *op)
+{
+    if (!e || !s || !key || !op) return CSMV_E_ARG;
+    if (s->n_reqs == 0 || s->n_reqs >= (1ull << 31) || s->n_sets == 0 ||
+        (s->n_sets & (s->n_sets - 1)) || s->n_ways == 0 || s->n_ways > 8 || s->batch == 0)
+        return fail(e, CSMV_E_ARG, "memcd spec out of range");
+    if ((uint64_t)kMcFields * s->n_sets * s->n_ways != e->cfg.n_items)
+        return fail(e, CSMV_E_ARG, "memcd: the engine needs n_items = 12 * n_sets * n_ways");
+    for (uint64_t k = 0; k < s->n_reqs; ++k)
+        if (op[k] != kMcGet && op[k] != kMcSet) return fail(e, CSMV_E_ARG, "memcd: request op not GET or SET");
+    csmv_status st = mc_reserve(e, s->n_reqs);
+    if (st) return st;
+    HIPCHK(e, hipMemcpy(e->mc_key, key, s->n_reqs * 8, hipMemcpyHostToDevice));
+    HIPCHK(e, hipMemcpy(e->mc_op, op, s->n_reqs * 4, hipMemcpyHostToDevice));
+    e->mspec = *s;
+    e->has_mc = true;
+    return CSMV_OK;
+}
+
 csmv_status csmv_memcd_generate(csmv_engine *e, const csmv_memcd_spec *s)
 {
     if (!e || !s) return CSMV_E_ARG;
@@ -1205,18 +1253,8 @@ csmv_status csmv_memcd_generate(csmv_engine *e, const csmv_memcd_spec *s)
     if ((uint64_t)kMcFields * s->n_sets * s->n_ways != e->cfg.n_items)
         return fail(e, CSMV_E_ARG, "memcd: the engine needs n_items = 12 * n_sets * n_ways");
     const uint64_t T = s->n_reqs;
-    if (T > e->mc_cap) {
-        dfree(e->mc_key); dfree(e->mc_op); dfree(e->mc_ts); dfree(e->mc_res); dfree(e->mc_vsum);
-        dfree(e->mc_ab); dfree(e->mc_lat);
-        e->mc_cap = 0;
-        csmv_status st;
-        if ((st = dalloc(e, &e->mc_key, T)) || (st = dalloc(e, &e->mc_op, T)) ||
-            (st = dalloc(e, &e->mc_ts, T)) || (st = dalloc(e, &e->mc_res, T)) ||
-            (st = dalloc(e, &e->mc_vsum, T)) || (st = dalloc(e, &e->mc_ab, T)) ||
-            (st = dalloc(e, &e->mc_lat, T)))
-            return st;
-        e->mc_cap = T;
-    }
+    csmv_status rs = mc_reserve(e, T);
+    if (rs) return rs;
     dfree(e->mc_cdf);
     if (s->dist == 1) {
         uint32_t *h = new (std::nothrow) uint32_t[s->n_keys];
@@ -1811,55 +1849,82 @@ template <int R, int W> csmv_status fit_layout(csmv_engine *e, ServerLayout &L, 
 
 extern "C" {
 
-csmv_status csmv_serve_run(csmv_engine *cache, csmv_engine *bank, const csmv_serve_spec *sp,
-                           csmv_serve_stats *out)
+csmv_status csmv_serve_run_multi(csmv_engine *const *caches, uint32_t n_cache, csmv_engine *bank,
+                                 const csmv_serve_spec *sp, csmv_serve_stats *out)
 {
+    if (n_cache && !caches) return CSMV_E_ARG;
+    csmv_engine *p = n_cache ? caches[0] : bank;   // holds the shared serving words
+    if (!p) return CSMV_E_ARG;
+    if (n_cache + (bank ? 1u : 0u) > kMaxServeParts)
+        return fail(p, CSMV_E_CAPACITY, "at most 7 cache partitions beside the bank partition");
     // each class arrives at its share of the offered load whether or not the other class
     // is served, so a partition served alone sees the arrivals it sees in the mix
     const double rate_c = sp ? sp->offered_rps * sp->pct_cache / 100.0 : 0.0;
     const double rate_b = sp ? sp->offered_rps * (100 - sp->pct_cache) / 100.0 : 0.0;
-    if (!sp || (!cache && !bank) || cache == bank || !(sp->offered_rps > 0.0) ||
-        sp->pct_cache > 100 || (cache && !(rate_c > 0.0)) || (bank && !(rate_b > 0.0)) ||
-        (sp->pool_budget && sp->pool_budget < 1024))
-        return fail(cache ? cache : bank, CSMV_E_ARG, "serve spec out of range");
-    csmv_engine *p = cache ? cache : bank;   // holds the shared serving words
-    for (csmv_engine *e : {cache, bank}) {
-        if (!e) continue;
+    if (!sp || !(sp->offered_rps > 0.0) || sp->pct_cache > 100 || (n_cache && !(rate_c > 0.0)) ||
+        (bank && !(rate_b > 0.0)) || (sp->pool_budget && sp->pool_budget < 1024))
+        return fail(p, CSMV_E_ARG, "serve spec out of range");
+    std::vector<csmv_engine *> parts;
+    for (uint32_t i = 0; i < n_cache; ++i) {
+        if (!caches[i]) return fail(p, CSMV_E_ARG, "null cache partition");
+        parts.push_back(caches[i]);
+    }
+    if (bank) parts.push_back(bank);
+    for (size_t i = 0; i < parts.size(); ++i)
+        for (size_t j = 0; j < i; ++j)
+            if (parts[i] == parts[j]) return fail(p, CSMV_E_ARG, "an engine serves one partition");
+    for (csmv_engine *e : parts) {
         if ((e->cfg.variant & 3u) != kValStamp || (e->cfg.variant & kServerWriteback))
             return fail(e, CSMV_E_ARG, "serving runs the STAMP variant with client write-back");
         if (!e->pool_loaded) return fail(e, CSMV_E_STATE, "serve needs a freshly loaded pool");
         if (e->device != p->device || e->cfg.wg_waves != p->cfg.wg_waves)
             return fail(e, CSMV_E_ARG, "partitions must share the device and wg_waves");
     }
-    if (cache && !cache->has_mc) return fail(cache, CSMV_E_STATE, "cache partition has no stream");
+    for (uint32_t i = 0; i < n_cache; ++i)
+        if (!caches[i]->has_mc) return fail(caches[i], CSMV_E_STATE, "cache partition has no stream");
     if (bank && (!bank->has_stream || bank->spec.pct_ro != 0))
         return fail(bank, CSMV_E_STATE, "bank partition needs a transfer-only stream");
-    // per-class Poisson processes: the thinning of one process of the offered rate
+    // Arrivals: one Poisson process per class; a cache request's partition depends on its
+    // key alone, so splitting the cache class by partition thins it into independent Poisson
+    // processes, each at the class rate times the partition's share of the requests.
+    uint64_t n_cache_reqs = 0;
+    for (uint32_t i = 0; i < n_cache; ++i) n_cache_reqs += caches[i]->mspec.n_reqs;
     csmv_status st;
-    ServerLayout Lm{}, Lb{};
-    Ctx cm{}, cb{};
-    if (cache) {
-        if ((st = poisson_arrivals(cache, splitmix64(sp->seed * 2 + 1), rate_c, cache->mspec.n_reqs)))
-            return st;
-        if ((st = fit_layout<kMcR, kMcW>(cache, Lm, false))) return st;
-        base_ctx(cache, cm);
-        cm.n_serial_waves = 0;
-        cm.n_txns = cache->mspec.n_reqs;
-        cm.req_key = cache->mc_key;
-        cm.req_op = cache->mc_op;
-        cm.n_sets = cache->mspec.n_sets;
-        cm.n_ways = cache->mspec.n_ways;
-        cm.n_slots = cache->mspec.n_sets * cache->mspec.n_ways;
-        cm.mc_batch = cache->mspec.batch;
-        cm.log_ts = cache->mc_ts;
-        cm.log_res = cache->mc_res;
-        cm.log_vsum = cache->mc_vsum;
-        cm.log_ab = cache->mc_ab;
-        cm.log_lat = cache->mc_lat;
-        cm.n_norm = cm.n_txns;
-        cm.arr = cache->arr;
+    ServeArgs args{};
+    args.n_cache = n_cache;
+    args.has_bank = bank ? 1u : 0u;
+    uint32_t words = 0, wgs = 0;
+    for (uint32_t i = 0; i < n_cache; ++i) {
+        csmv_engine *e = caches[i];
+        const double share = (double)e->mspec.n_reqs / (double)n_cache_reqs;
+        const uint64_t seed = i == 0 ? splitmix64(sp->seed * 2 + 1) : splitmix64(sp->seed * 2 + 1 + 1000003ull * i);
+        if ((st = poisson_arrivals(e, seed, rate_c * share, e->mspec.n_reqs))) return st;
+        ServerLayout &L = args.L[i];
+        if ((st = fit_layout<kMcR, kMcW>(e, L, false))) return st;
+        Ctx &c = args.c[i];
+        base_ctx(e, c);
+        c.n_serial_waves = 0;
+        c.escalate = 0;
+        c.n_txns = e->mspec.n_reqs;
+        c.req_key = e->mc_key;
+        c.req_op = e->mc_op;
+        c.n_sets = e->mspec.n_sets;
+        c.n_ways = e->mspec.n_ways;
+        c.n_slots = e->mspec.n_sets * e->mspec.n_ways;
+        c.mc_batch = e->mspec.batch;
+        c.log_ts = e->mc_ts;
+        c.log_res = e->mc_res;
+        c.log_vsum = e->mc_vsum;
+        c.log_ab = e->mc_ab;
+        c.log_lat = e->mc_lat;
+        c.n_norm = c.n_txns;
+        c.arr = e->arr;
+        words = std::max(words, L.words);
+        wgs += (e->cfg.client_waves + e->cfg.client_wg_waves - 1) / e->cfg.client_wg_waves;
+        args.wg_end[i] = wgs;
     }
     if (bank) {
+        const uint32_t bi = n_cache;
         if ((st = poisson_arrivals(bank, splitmix64(sp->seed * 2 + 2), rate_b, bank->spec.n_txns)))
             return st;
         const bool serial = bank->routed && bank->n_hot > 0;
@@ -1867,42 +1932,37 @@ csmv_status csmv_serve_run(csmv_engine *cache, csmv_engine *bank, const csmv_ser
             return fail(bank, CSMV_E_ARG, "a routed bank partition needs n_serial_waves > 0");
         // escalated transactions travel on the serialized lane too
         const bool delegation = serial || bank->cfg.escalate_after != 0;
-        if ((st = fit_layout<kServeBankR, kBankW>(bank, Lb, delegation))) return st;
-        base_ctx(bank, cb);
-        cb.n_serial_waves = serial ? bank->cfg.n_serial_waves : 0;
-        cb.hot_list = serial ? bank->hot_list : nullptr;
-        cb.n_hot = serial ? bank->n_hot : 0;
-        cb.op = bank->op;
-        cb.addr = bank->addr;
-        cb.n_txns = bank->spec.n_txns;
-        cb.log_ts = bank->log_ts;
-        cb.log_v = bank->log_v;
-        cb.log_ab = bank->log_ab;
-        cb.log_lat = bank->log_lat;
-        cb.norm_list = bank->routed ? bank->norm_list : nullptr;
-        cb.n_norm = bank->routed ? bank->n_norm : bank->spec.n_txns;
-        cb.arr = bank->arr;
+        ServerLayout &L = args.L[bi];
+        if ((st = fit_layout<kServeBankR, kBankW>(bank, L, delegation))) return st;
+        Ctx &c = args.c[bi];
+        base_ctx(bank, c);
+        c.n_serial_waves = serial ? bank->cfg.n_serial_waves : 0;
+        c.hot_list = serial ? bank->hot_list : nullptr;
+        c.n_hot = serial ? bank->n_hot : 0;
+        c.op = bank->op;
+        c.addr = bank->addr;
+        c.n_txns = bank->spec.n_txns;
+        c.log_ts = bank->log_ts;
+        c.log_v = bank->log_v;
+        c.log_ab = bank->log_ab;
+        c.log_lat = bank->log_lat;
+        c.norm_list = bank->routed ? bank->norm_list : nullptr;
+        c.n_norm = bank->routed ? bank->n_norm : bank->spec.n_txns;
+        c.arr = bank->arr;
+        words = std::max(words, L.words);
+        wgs += (bank->cfg.client_waves + bank->cfg.client_wg_waves - 1) / bank->cfg.client_wg_waves;
+        args.wg_end[bi] = wgs;
     }
-    for (Ctx *c : {&cm, &cb}) {
-        c->t_start = (uint64_t *)(p->ctl + kOffServeStart);
-        c->inflight = (uint64_t *)(p->ctl + kOffInflight);
-        c->serve_ctr = (uint64_t *)(p->ctl + kOffServeCtr);
-        c->budget = sp->pool_budget;
+    for (size_t i = 0; i < parts.size(); ++i) {
+        args.c[i].t_start = (uint64_t *)(p->ctl + kOffServeStart);
+        args.c[i].inflight = (uint64_t *)(p->ctl + kOffInflight);
+        args.c[i].serve_ctr = (uint64_t *)(p->ctl + kOffServeCtr);
+        args.c[i].budget = sp->pool_budget;
     }
-    const uint32_t words = std::max(cache ? Lm.words : 0u, bank ? Lb.words : 0u);
     const size_t lds_bytes = (size_t)words * 4;
     if (lds_bytes > p->lds_max) return fail(p, CSMV_E_CAPACITY, "serving LDS layout too large");
     const uint32_t wg = p->cfg.wg_waves;
-    ServeRoles roles{};
-    roles.has_cache = cache ? 1u : 0u;
-    roles.has_bank = bank ? 1u : 0u;
-    roles.cache_wgs = cache ? (cache->cfg.client_waves + cache->cfg.client_wg_waves - 1) /
-                                  cache->cfg.client_wg_waves
-                            : 0u;
-    const uint32_t bank_wgs =
-        bank ? (bank->cfg.client_waves + bank->cfg.client_wg_waves - 1) / bank->cfg.client_wg_waves
-             : 0u;
-    const uint32_t grid = roles.has_cache + roles.has_bank + roles.cache_wgs + bank_wgs;
+    const uint32_t grid = (uint32_t)parts.size() + wgs;
     auto kern = serve_kernel<kMcR, kMcW, kServeBankR, kBankW>;
     HIPCHK(p, hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)lds_bytes));
@@ -1911,31 +1971,45 @@ csmv_status csmv_serve_run(csmv_engine *cache, csmv_engine *bank, const csmv_ser
                                                            lds_bytes));
     if ((uint64_t)per_cu * p->n_cu < grid)
         return fail(p, CSMV_E_CAPACITY, "serving grid exceeds what the device keeps resident");
-    if (cache && (st = reset_run_state(cache, Lm))) return st;
-    if (bank && (st = reset_run_state(bank, Lb))) return st;
-    if (cache) HIPCHK(cache, hipStreamSynchronize(cache->stream));
-    if (bank) HIPCHK(bank, hipStreamSynchronize(bank->stream));
+    for (size_t i = 0; i < parts.size(); ++i)
+        if ((st = reset_run_state(parts[i], args.L[i]))) return st;
+    for (csmv_engine *e : parts) HIPCHK(e, hipStreamSynchronize(e->stream));
 
     HIPCHK(p, hipEventRecord(p->ev0, p->stream));
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(wg * kWave), lds_bytes, p->stream, cm, Lm, cb, Lb,
-                       roles);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(wg * kWave), lds_bytes, p->stream, args);
     HIPCHK(p, hipGetLastError());
     HIPCHK(p, hipEventRecord(p->ev1, p->stream));
     HIPCHK(p, hipEventSynchronize(p->ev1));
     float ms = 0.f;
     HIPCHK(p, hipEventElapsedTime(&ms, p->ev0, p->ev1));
-    uint32_t derr_c = 0, derr_b = 0;
     csmv_serve_stats o{};
-    if (cache) {
-        cache->pool_loaded = false;
-        cache->served = 1;
-        if ((st = collect_stats(cache, Lm, ms, &o.cache, &derr_c))) return st;
-    }
-    if (bank) {
-        bank->pool_loaded = false;
-        bank->served = 2;
-        bank->bank_logged = false;
-        if ((st = collect_stats(bank, Lb, ms, &o.bank, &derr_b))) return st;
+    std::vector<uint32_t> derr(parts.size(), 0);
+    for (size_t i = 0; i < parts.size(); ++i) {
+        csmv_engine *e = parts[i];
+        e->pool_loaded = false;
+        const bool is_bank = bank && i + 1 == parts.size();
+        e->served = is_bank ? 2 : 1;
+        if (is_bank) e->bank_logged = false;
+        csmv_stats one{};
+        if ((st = collect_stats(e, args.L[i], ms, &one, &derr[i]))) return st;
+        if (is_bank) {
+            o.bank = one;
+        } else if (i == 0) {
+            o.cache = one;
+        } else {
+            // the cache class's counters summed over its partitions
+            o.cache.commits_update += one.commits_update;
+            o.cache.commits_ro += one.commits_ro;
+            o.cache.aborts_dataage += one.aborts_dataage;
+            o.cache.aborts_readwrite += one.aborts_readwrite;
+            o.cache.aborts_prevalid += one.aborts_prevalid;
+            o.cache.aborts_recordage += one.aborts_recordage;
+            o.cache.aborts_unknown += one.aborts_unknown;
+            o.cache.batches += one.batches;
+            o.cache.client_lanes += one.client_lanes;
+            o.cache.error = o.cache.error ? o.cache.error : one.error;
+            for (int k = 0; k < 16; ++k) o.cache.prof_ns[k] += one.prof_ns[k];
+        }
     }
     uint64_t words3[3] = {0, 0, 0};
     uint64_t inflight = 0;
@@ -1947,11 +2021,18 @@ csmv_status csmv_serve_run(csmv_engine *cache, csmv_engine *bank, const csmv_ser
     o.final_inflight = inflight;
     o.wallclock_khz = p->wallclock_khz;
     if (out) *out = o;
-    if ((st = device_error(cache ? cache : bank, cache ? derr_c : derr_b))) return st;
-    if (cache && bank && (st = device_error(bank, derr_b))) return st;
+    for (size_t i = 0; i < parts.size(); ++i)
+        if ((st = device_error(parts[i], derr[i]))) return st;
     // the bank log of a completed run feeds the abort counters like csmv_bank_run's
     if (bank) bank->bank_logged = true;
     return CSMV_OK;
+}
+
+csmv_status csmv_serve_run(csmv_engine *cache, csmv_engine *bank, const csmv_serve_spec *sp,
+                           csmv_serve_stats *out)
+{
+    if ((!cache && !bank) || cache == bank) return fail(cache ? cache : bank, CSMV_E_ARG, "serve: two distinct partitions");
+    return csmv_serve_run_multi(cache ? &cache : nullptr, cache ? 1u : 0u, bank, sp, out);
 }
 
 csmv_status csmv_serve_read_arrivals(csmv_engine *e, uint32_t *arr_ticks, size_t n)

@@ -185,6 +185,12 @@ typedef struct {
 
 /* Request stream on the device (the host key generation loop, memcdSH.cu:586-595). */
 csmv_status csmv_memcd_generate(csmv_engine *e, const csmv_memcd_spec *spec);
+/* A request stream from host arrays (key[n_reqs], op[n_reqs]: 0 GET, 1 SET), e.g. one
+ * partition of a stream split by the caller: the engine uses spec's n_reqs, n_sets, n_ways
+ * and batch (request k runs with LRU clock k / batch + 1 and writes value words of (key, k));
+ * its seed, keyspace and distribution are only recorded. */
+csmv_status csmv_memcd_load(csmv_engine *e, const csmv_memcd_spec *spec, const uint64_t *key,
+                            const uint32_t *op);
 csmv_status csmv_memcd_read_stream(csmv_engine *e, uint64_t *key, uint32_t *op);
 /* Serves the whole stream with one persistent kernel (replaces the per-batch memcdTx
  * launches with a persistent server_kernel, memcdSH.cu:603-651). */
@@ -303,6 +309,13 @@ typedef struct {
 
 csmv_status csmv_serve_run(csmv_engine *cache, csmv_engine *bank, const csmv_serve_spec *spec,
                            csmv_serve_stats *stats);
+/* The same with the cache class split over n_cache engines (0..7 together with the bank
+ * partition), each its own commit server and clock: partition i holds a disjoint range of the
+ * cache's sets and the requests that map to it (csmv_memcd_load).  The class's arrivals are
+ * thinned by partition: each partition's Poisson rate is the class rate times its share of
+ * the class's requests.  stats->cache sums the partitions' counters. */
+csmv_status csmv_serve_run_multi(csmv_engine *const *caches, uint32_t n_cache, csmv_engine *bank,
+                                 const csmv_serve_spec *spec, csmv_serve_stats *stats);
 /* Per request of the stream an engine served: arrival tick, and latency from arrival to
  * commit in wall-clock ticks (queueing, admission and every attempt included). */
 csmv_status csmv_serve_read_arrivals(csmv_engine *e, uint32_t *arr_ticks, size_t n);

@@ -130,3 +130,21 @@ def test_noalloc_rejects_before_touching_a_device():
     with pytest.raises(csmv.CsmvError) as ei:
         csmv.bank_run_noalloc(csmv.config(4096, n_serial_waves=4, client_waves=8), bufs)
     assert ei.value.status == 2
+
+
+def test_cache_partition_split_matches_the_oracle_set_hash():
+    """The host split of a cache stream over partitions uses the device's set hash (pinned
+    here against the oracle's), and each partition's local set is the set a cache of
+    n_sets / n_parts sets gives the key."""
+    import orc
+    spec = orc.memcd_spec(20000, seed=3, n_keys=100_000, n_sets=1 << 10, zipf=True)
+    key, op = orc.memcd_gen(spec)
+    lib = orc._memcd_lib()
+    want = np.array([lib.orc_memcd_set_of(int(k), 1 << 10) for k in key[:2000]], dtype=np.uint32)
+    assert np.array_equal(csmv.memcd_set_of(key[:2000], 1 << 10), want)
+    parts = csmv.memcd_partition(key, op, 1 << 10, 4)
+    assert sum(k.size for k, _ in parts) == key.size
+    for p, (k, o) in enumerate(parts):
+        g = csmv.memcd_set_of(k, 1 << 10)
+        assert np.all(g // 256 == p)
+        assert np.array_equal(g % 256, csmv.memcd_set_of(k, 256))

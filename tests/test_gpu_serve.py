@@ -10,6 +10,8 @@ warmed cache with no SETs in flight, the final cache, and the final bank balance
 (transfers commute); what does depend on it (the LRU "wrote" bit of a GET, a transfer's
 read values) is checked by the replay of each run.
 """
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -280,3 +282,49 @@ def test_serve_bank_escalation_bounds_latency():
             if esc:
                 assert out["bank"]["escalated"] > 0
     assert p99[2] < p99[0]
+
+
+def test_serve_cache_partitions_at_config_shape():
+    """The cache configuration (2^17 sets x 8 ways, 10M-key Zipf 0.99, 95/5) split over four
+    commit servers, each owning a quarter of the sets, beside the 2^20-account bank partition
+    at 50M req/s offered with an in-flight budget.  Each partition is an ordinary cache of
+    2^15 sets and its log replays exactly, the bank replays, every request commits once
+    after its arrival, the budget holds and admission engages."""
+    n_parts, n_acc = 4, 1 << 20
+    full = csmv.Memcd(n_reqs=1 << 18, seed=1)
+    print(f"\n[case] serving, cache split over {n_parts} commit servers, 50M req/s", flush=True)
+    with csmv.Engine(full.n_items, depth=4, client_waves=1, wg_waves=6) as gen:
+        gen.memcd_generate(full)
+        key, op = gen.memcd_read_stream()
+    parts = csmv.memcd_partition(key, op, full.n_sets, n_parts)
+    local = dataclasses.replace(full, n_sets=full.n_sets // n_parts)
+    b = csmv.Bank(n_txns=1 << 14, seed=7, n_reads=64, n_reads_min=2, pct_ro=0, zipf=True,
+                  theta=0.99)
+    engines = [cache_engine(local, waves=8, client_wg_waves=1) for _ in range(n_parts)]
+    try:
+        for e, (k, o) in zip(engines, parts):
+            e.fill(0)
+            e.memcd_load(local, k, o)
+        with csmv.Engine(n_acc, depth=8, client_waves=8, wg_waves=6, variant="stamp",
+                         timeout_ms=20000, escalate_after=2) as eb:
+            eb.fill(INIT)
+            eb.bank_generate(b)
+            out = csmv.serve(engines, eb, offered_rps=50e6, pct_cache=80, seed=13,
+                             pool_budget=65536)
+            print("[serve]", {k: out[k] for k in ("kernel_ms", "admit_stalls", "peak_inflight")},
+                  flush=True)
+            assert out["final_inflight"] == 0
+            assert 0 < out["peak_inflight"] <= 65536
+            assert out["admit_stalls"] > 0
+            assert out["cache"]["commits"] == key.size
+            check_bank(eb, b, n_acc)
+            check_latency(eb, out["bank"], b.n_txns)
+        for e, (k, o) in zip(engines, parts):
+            m = dataclasses.replace(local, n_reqs=int(k.size))
+            check_cache(e, m)
+            arr = e.serve_arrivals()
+            lat = e.serve_latency()
+            assert np.all(np.diff(arr.astype(np.int64)) >= 0) and np.all(lat > 0)
+    finally:
+        for e in engines:
+            e.close()
