@@ -52,9 +52,12 @@ __device__ __forceinline__ uint64_t serve_start(const Ctx &c)
 }
 
 // Admission.  Lanes holding an arrived request ask for `need` bytes of in-flight
-// versions; the longest lane-order prefix of them that fits under the budget is admitted
-// with one compare-and-swap (so *inflight never exceeds the budget).  A refused round
-// counts as a stall; the lanes ask again next round.
+// versions.  One fetch_add claims the bytes of every such lane; the longest lane-order
+// prefix of them that fits under the budget above the counter's previous value is
+// admitted and the rest is returned at once.  The admitted bytes never exceed the budget
+// (the counter can pass it only by bytes about to be returned, which makes concurrent
+// waves refuse, never admit, too much).  A round that admits nothing counts as a stall;
+// the lanes ask again next round.
 __device__ __forceinline__ void serve_gate(const Ctx &c, uint64_t t0, bool has, bool &adm,
                                            uint64_t k, uint32_t need, uint32_t &held,
                                            uint64_t &stalls)
@@ -67,32 +70,22 @@ __device__ __forceinline__ void serve_gate(const Ctx &c, uint64_t t0, bool has, 
         return;
     }
     const uint32_t incl = wave_incl_scan(cand ? need : 0u);
-    bool ok = false;
-    uint64_t take = 0;
-    for (int attempt = 0; attempt < 8 && !ok; ++attempt) {
-        const uint64_t cur = bcast64(ld_rlx(c.inflight));
-        const uint64_t room = c.budget > cur ? c.budget - cur : 0;
-        const uint64_t fit = ballot(cand && incl <= room);
-        if (fit == 0) break;
-        // the admitted prefix ends at the highest fitting candidate
-        const uint32_t last = 63u - (uint32_t)__builtin_clzll(fit);
-        take = (uint64_t)readlane(incl, last);
-        bool won = false;
-        if (lane_id() == 0) {
-            uint64_t expected = cur;
-            won = __hip_atomic_compare_exchange_strong(c.inflight, &expected, cur + take,
-                                                       __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                                       __HIP_MEMORY_SCOPE_AGENT);
-            if (won) atomicMax((unsigned long long *)&c.serve_ctr[1],
-                               (unsigned long long)(cur + take));
-        }
-        ok = bcast(won ? 1u : 0u) != 0;
-        if (ok && cand && incl <= take) {
-            adm = true;
-            held = need;
-        }
+    const uint32_t want = readlane(incl, 63);
+    uint64_t old = 0;
+    if (lane_id() == 0) old = fetch_add(c.inflight, (uint64_t)want);
+    old = bcast64(old);
+    const uint64_t room = c.budget > old ? c.budget - old : 0;
+    const uint64_t fit = ballot(cand && incl <= room);
+    const uint32_t take = fit ? readlane(incl, 63u - (uint32_t)__builtin_clzll(fit)) : 0u;
+    if (lane_id() == 0) {
+        if (want > take) fetch_add(c.inflight, (uint64_t)0 - (uint64_t)(want - take));
+        if (take) atomicMax((unsigned long long *)&c.serve_ctr[1], (unsigned long long)(old + take));
     }
-    if (!ok) ++stalls;
+    if (cand && incl <= take) {
+        adm = true;
+        held = need;
+    }
+    if (take == 0) ++stalls;
 }
 
 // Returns the bytes of lanes that committed this round to the budget.
