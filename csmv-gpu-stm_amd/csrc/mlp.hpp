@@ -42,6 +42,40 @@ constexpr uint32_t kPartItems = 1024;      // items per workgroup of the routing
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
+// Float sums across lanes without shuffles.  Each step adds the partner lane's value to
+// the lane's own; the partners are lane ^ 1 and lane ^ 2 (DPP quad_perm), then the mirror
+// lane in the other half-row and in the other half of the row (DPP row_half_mirror,
+// row_mirror), lane ^ 16 (ds_swizzle SWAP16) and lane ^ 32 (v_permlane32_swap).  After each
+// step every lane of a group holds the same sum, so a mirror partner adds the same value
+// as the xor partner would: the results are bitwise those of a butterfly over lane ^ m.
+template <int Ctrl>
+__device__ __forceinline__ float dpp_add(float v)
+{
+    return v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), Ctrl, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float row_sum16(float v)
+{
+    v = dpp_add<dpp::kQuadSwap1>(v);
+    v = dpp_add<dpp::kQuadSwap2>(v);
+    v = dpp_add<dpp::kRowHalfMirror>(v);
+    return dpp_add<dpp::kRowMirror>(v);
+}
+__device__ __forceinline__ float add_lane_xor16(float v)
+{
+    return v + __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x401F));
+}
+__device__ __forceinline__ float add_lane_xor32(float v)
+{
+    // swapping v's upper half with a copy's lower half leaves [lo, lo] and [hi, hi]
+    const auto p = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    const uint32_t lane = lane_id();
+    return v + __uint_as_float(lane < 32 ? p[1] : p[0]);
+}
+__device__ __forceinline__ float wave_sum_f(float v)
+{
+    return add_lane_xor32(add_lane_xor16(row_sum16(v)));
+}
+
 // 32-bit integer finaliser (xor-shift-multiply, constants of the "lowbias32" hash)
 __host__ __device__ __forceinline__ uint32_t feat_mix(uint32_t x)
 {
@@ -177,12 +211,7 @@ __device__ __forceinline__ void mlp_tile(const bf16x4 (&a)[4][4], const uint16_t
     for (int bm = 0; bm < 4; ++bm)
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            float v = zp[bm][j];
-            v += __shfl_xor(v, 1);
-            v += __shfl_xor(v, 2);
-            v += __shfl_xor(v, 4);
-            v += __shfl_xor(v, 8);
-            z[bm][j] = v + b2;
+            z[bm][j] = row_sum16(zp[bm][j]) + b2;
         }
 }
 
@@ -326,21 +355,15 @@ __global__ void __launch_bounds__(64 * kMlpWaves) mlp_grad_kernel(const uint16_t
                 out[kParamW1 + (fm * 16 + 4 * q + j) * kHidden + bn * 16 + r] = dW[fm][bn][j];
 #pragma unroll
     for (int bn = 0; bn < 4; ++bn) {
-        float v1 = gb1[bn], v2 = gw2[bn];
-        v1 += __shfl_xor(v1, 16);
-        v1 += __shfl_xor(v1, 32);
-        v2 += __shfl_xor(v2, 16);
-        v2 += __shfl_xor(v2, 32);
+        const float v1 = add_lane_xor32(add_lane_xor16(gb1[bn]));
+        const float v2 = add_lane_xor32(add_lane_xor16(gw2[bn]));
         if (q == 0) {
             out[kParamB1 + bn * 16 + r] = v1;
             out[kParamW2 + bn * 16 + r] = v2;
         }
     }
-#pragma unroll
-    for (int m = 1; m < 64; m <<= 1) {
-        gb2 += __shfl_xor(gb2, m);
-        loss += __shfl_xor(loss, m);
-    }
+    gb2 = wave_sum_f(gb2);
+    loss = wave_sum_f(loss);
     if (lane == 0) {
         out[kParamB2] = gb2;
         out[kParamCount] = loss * inv_n;
@@ -448,7 +471,7 @@ __global__ void __launch_bounds__(256) part_count_kernel(const uint8_t *flag, ui
         const uint64_t k = b0 + i * 256 + threadIdx.x;
         c += (k < T && flag[k]) ? 1u : 0u;
     }
-    for (int m = 1; m < 64; m <<= 1) c += __shfl_xor(c, m);
+    c = wave_sum(c);
     if ((threadIdx.x & 63u) == 0) wsum[threadIdx.x / 64] = c;
     __syncthreads();
     if (threadIdx.x == 0) bcount[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
@@ -466,11 +489,7 @@ __global__ void __launch_bounds__(1024) part_scan_kernel(uint32_t *bcount, uint3
     for (uint32_t c0 = 0; c0 < nb; c0 += 1024) {
         const uint32_t i = c0 + threadIdx.x;
         const uint32_t v = i < nb ? bcount[i] : 0u;
-        uint32_t incl = v;
-        for (int m = 1; m < 64; m <<= 1) {
-            const uint32_t o = __shfl_up(incl, m);
-            if (lane >= (uint32_t)m) incl += o;
-        }
+        const uint32_t incl = wave_incl_scan(v);
         if (lane == 63) wsum[w] = incl;
         __syncthreads();
         uint32_t before = carry;

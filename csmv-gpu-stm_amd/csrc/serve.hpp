@@ -17,24 +17,6 @@ namespace csmv {
 // Version-pool bytes a request may install: (version, value) + head word per item.
 constexpr uint32_t kVersionBytes = 12;
 
-__device__ __forceinline__ uint32_t wave_sum(uint32_t v)
-{
-#pragma unroll
-    for (int m = 1; m < 64; m <<= 1) v += (uint32_t)__shfl_xor((int)v, m, 64);
-    return v;
-}
-
-__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
-{
-    const uint32_t lane = lane_id();
-#pragma unroll
-    for (int m = 1; m < 64; m <<= 1) {
-        const uint32_t o = (uint32_t)__shfl_up((int)v, m, 64);
-        if (lane >= (uint32_t)m) v += o;
-    }
-    return v;
-}
-
 // The serving start: the first client wave to run publishes its clock; arrival ticks
 // count from there.
 __device__ __forceinline__ uint64_t serve_start(const Ctx &c)

@@ -99,6 +99,87 @@ __device__ __forceinline__ uint64_t readlane64(uint64_t v, uint32_t l)
     return ((uint64_t)hi << 32) | lo;
 }
 
+// ---- lane exchange over DPP ------------------------------------------------------------
+// Wave scans and reductions move values between lanes with DPP row shifts and row
+// broadcasts, which the VALU executes as an operand modifier (hipcc folds the move into
+// the combining instruction, e.g. v_min_u32_dpp); a shuffle (ds_bpermute) goes through the
+// LDS pipeline instead.  GFX9 DPP controls: row_shr:k = 0x110 + k, row_mirror = 0x140,
+// row_half_mirror = 0x141, row_bcast:15 = 0x142, row_bcast:31 = 0x143, quad_perm = 0..0xFF.
+// Every lane of the wave must be active.
+namespace dpp {
+constexpr int kRowShr1 = 0x111, kRowShr2 = 0x112, kRowShr4 = 0x114, kRowShr8 = 0x118;
+constexpr int kRowMirror = 0x140, kRowHalfMirror = 0x141;
+constexpr int kRowBcast15 = 0x142, kRowBcast31 = 0x143;
+constexpr int kQuadSwap1 = 0xB1;   // quad_perm [1,0,3,2]: lane ^ 1
+constexpr int kQuadSwap2 = 0x4E;   // quad_perm [2,3,0,1]: lane ^ 2
+}  // namespace dpp
+
+// v of the lane the DPP control names; a lane whose source lies outside its row, or whose
+// row is masked off, gets `id`.
+template <int Ctrl, int RowMask = 0xF>
+__device__ __forceinline__ uint32_t dpp_mov(uint32_t id, uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)id, (int)v, Ctrl, RowMask, 0xF, false);
+}
+
+// Inclusive scan of op over the wave in lane order (id: op's identity).  Within each row
+// of 16 lanes, Hillis-Steele steps at distances 1, 2, 4, 8; then the total of row 0 goes
+// into row 1 and that of row 2 into row 3 (row_bcast:15 on rows 1 and 3), and the total of
+// lanes 0..31 into rows 2 and 3 (row_bcast:31).
+template <typename Op>
+__device__ __forceinline__ uint32_t wave_scan(uint32_t v, uint32_t id, Op op)
+{
+    v = op(v, dpp_mov<dpp::kRowShr1>(id, v));
+    v = op(v, dpp_mov<dpp::kRowShr2>(id, v));
+    v = op(v, dpp_mov<dpp::kRowShr4>(id, v));
+    v = op(v, dpp_mov<dpp::kRowShr8>(id, v));
+    v = op(v, dpp_mov<dpp::kRowBcast15, 0xA>(id, v));
+    v = op(v, dpp_mov<dpp::kRowBcast31, 0xC>(id, v));
+    return v;
+}
+
+struct OpMin {
+    __device__ uint32_t operator()(uint32_t a, uint32_t b) const { return a < b ? a : b; }
+};
+struct OpMax {
+    __device__ uint32_t operator()(uint32_t a, uint32_t b) const { return a > b ? a : b; }
+};
+struct OpOr {
+    __device__ uint32_t operator()(uint32_t a, uint32_t b) const { return a | b; }
+};
+struct OpAdd {
+    __device__ uint32_t operator()(uint32_t a, uint32_t b) const { return a + b; }
+};
+
+// v of lane src, for any src (a gather such as a producer lookup): ds_bpermute, which
+// crosses lanes through the LDS unit without touching LDS memory.  The source lane must be
+// active.
+__device__ __forceinline__ uint32_t lane_gather(uint32_t v, uint32_t src)
+{
+    return (uint32_t)__builtin_amdgcn_ds_bpermute((int)(src << 2), (int)v);
+}
+
+// Reductions: the scan's last lane, read into a scalar register (uniform across the wave).
+__device__ __forceinline__ uint32_t wave_min(uint32_t v)
+{
+    return readlane(wave_scan(v, 0xFFFFFFFFu, OpMin{}), 63);
+}
+__device__ __forceinline__ uint32_t wave_max(uint32_t v)
+{
+    return readlane(wave_scan(v, 0u, OpMax{}), 63);
+}
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v)
+{
+    return readlane(wave_scan(v, 0u, OpAdd{}), 63);
+}
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) { return wave_scan(v, 0u, OpAdd{}); }
+__device__ __forceinline__ uint64_t wave_or64(uint64_t v)
+{
+    const uint32_t lo = readlane(wave_scan((uint32_t)v, 0u, OpOr{}), 63);
+    const uint32_t hi = readlane(wave_scan((uint32_t)(v >> 32), 0u, OpOr{}), 63);
+    return ((uint64_t)hi << 32) | lo;
+}
+
 // ---- agent-scope atomics --------------------------------------------------------------
 template <typename T> __device__ __forceinline__ T ld_rlx(const T *p)
 {

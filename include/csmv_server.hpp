@@ -139,27 +139,6 @@ __device__ __forceinline__ void wave_fence() { __builtin_amdgcn_fence(__ATOMIC_A
 constexpr bool kPhaseTimers = CSMV_PHASE_TIMERS != 0;
 __device__ __forceinline__ uint64_t phase_clock() { return kPhaseTimers ? wall_clock64() : 0ull; }
 
-__device__ __forceinline__ uint32_t wave_min(uint32_t v)
-{
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) {
-        uint32_t o = (uint32_t)__shfl_xor((int)v, m, 64);
-        v = o < v ? o : v;
-    }
-    return v;
-}
-
-__device__ __forceinline__ uint64_t wave_or64(uint64_t v)
-{
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) {
-        const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, m, 64);
-        const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), m, 64);
-        v |= ((uint64_t)hi << 32) | lo;
-    }
-    return v;
-}
-
 __device__ __forceinline__ uint32_t mix32(uint32_t x)
 {
     x ^= x >> 16;
@@ -885,7 +864,8 @@ __device__ __forceinline__ void stage_chains(uint32_t *dst, uint32_t *scr, uint3
         const uint32_t node = lane * RO + r;
         const bool has_p = pj[r] != 0xFFu;
         const int srcl = (int)(has_p ? (pj[r] & 63u) : lane);
-        const int32_t q0 = __shfl(d0, srcl, 64), q1 = __shfl(d1, srcl, 64);
+        const int32_t q0 = (int32_t)lane_gather((uint32_t)d0, (uint32_t)srcl),
+                      q1 = (int32_t)lane_gather((uint32_t)d1, (uint32_t)srcl);
         nxt[r] = has_p ? (pj[r] & 63u) * RO + (pj[r] >> 6) : node;
         wgt[r] = has_p ? ((pj[r] >> 6) == 0 ? q0 : q1) : 0;
         snext[node] = nxt[r];
@@ -1095,7 +1075,7 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
                         uint32_t code = 0xFFu;
 #pragma unroll
                         for (int w = 0; w < W; ++w) {
-                            const uint32_t pw = (uint32_t)__shfl((int)wa[w], srcl, 64);
+                            const uint32_t pw = lane_gather(wa[w], (uint32_t)srcl);
                             code = (pl != 0xFFu && pw == x[j]) ? (pl | ((uint32_t)w << 6)) : code;
                         }
                         word[j / 4] |= code << (8 * (j % 4));
@@ -1266,7 +1246,7 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
                     uint32_t code = p;
     #pragma unroll
                     for (int w = 0; w < W; ++w) {
-                        const uint32_t pw = (uint32_t)__shfl((int)it[R + w], srcl, 64);
+                        const uint32_t pw = lane_gather(it[R + w], (uint32_t)srcl);
                         code = (p != 0xFFu && pw == it[r]) ? (p | ((uint32_t)w << 6)) : code;
                     }
                     const uint32_t sh = 8u * (r % 4);

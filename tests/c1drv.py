@@ -22,6 +22,8 @@ def lib():
         P = C.c_void_p
         L.c1drv_run.argtypes = [P, P, P, C.c_uint64, C.c_uint32, P, P, P, C.POINTER(csmv.Stats)]
         L.c1drv_run.restype = C.c_int
+        L.lanechk_run.argtypes = [P, P, P, C.c_uint32, P, P, P]
+        L.lanechk_run.restype = C.c_int
         _lib = L
     return _lib
 
@@ -41,3 +43,21 @@ def run(engine, op: np.ndarray, addr: np.ndarray, rmax: int):
     if rc:
         engine._chk(rc, "c1drv_run")
     return st.as_dict(), ts, v, ab
+
+
+def lane_check(v: np.ndarray, v64: np.ndarray, src: np.ndarray):
+    """Runs the lane-primitive kernel on waves of 64 words; returns (inclusive scans,
+    gathers v[src], reductions [min, max, sum, or64 lo, or64 hi] per wave)."""
+    v = np.ascontiguousarray(v, dtype=np.uint32)
+    v64 = np.ascontiguousarray(v64, dtype=np.uint64)
+    src = np.ascontiguousarray(src, dtype=np.uint32)
+    assert v.size % 64 == 0 and v.size == v64.size == src.size
+    nw = v.size // 64
+    scan = np.zeros_like(v)
+    gather = np.zeros_like(v)
+    red = np.zeros(5 * nw, dtype=np.uint32)
+    rc = lib().lanechk_run(v.ctypes.data, v64.ctypes.data, src.ctypes.data, nw, scan.ctypes.data,
+                           gather.ctypes.data, red.ctypes.data)
+    if rc:
+        raise RuntimeError(f"lanechk_run: HIP error {rc}")
+    return scan, gather, red.reshape(nw, 5)
