@@ -91,7 +91,7 @@ class Stats(C.Structure):
                 ("escalated", C.c_uint64)]
     PROF = ("commit_idle", "commit_busy", "recv_window", "recv_ring", "recv_busy",
             "client_reply", "client_round", "client_rounds", "commit_read", "commit_resolve",
-            "commit_chain", "commit_finish")
+            "commit_chain", "commit_finish", "commit_spin", "commit_adv", "commit_misses")
 
     def as_dict(self) -> dict:
         d = {f: getattr(self, f) for f, _ in self._fields_ if f != "prof_ns"}

@@ -105,7 +105,7 @@ __global__ void __launch_bounds__(256) bank_generate_kernel(GenArgs g)
     }
 }
 
-__global__ void __launch_bounds__(256) pool_fill_kernel(uint32_t *ht, uint64_t *slot, int32_t *vals,
+__global__ void __launch_bounds__(256) pool_fill_kernel(uint32_t *ht, uint64_t *slot, uint64_t *sv,
                                                         uint32_t n, uint32_t depth,
                                                         const int32_t *values, int32_t fill)
 {
@@ -114,8 +114,15 @@ __global__ void __launch_bounds__(256) pool_fill_kernel(uint32_t *ht, uint64_t *
     ht[i] = 0;
     const int32_t v = values ? values[i] : fill;
     slot[(size_t)i * depth] = (uint32_t)v;
-    vals[i] = v;
+    sv[i] = (uint32_t)v;   // stamp 0
     for (uint32_t d = 1; d < depth; ++d) slot[(size_t)i * depth + d] = 0;
+}
+
+// A run's commit stamps start at zero; the committed values carry over from the pool.
+__global__ void __launch_bounds__(256) stamp_clear_kernel(uint64_t *sv, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) sv[i] &= 0xFFFFFFFFull;
 }
 
 __global__ void __launch_bounds__(256) pool_head_kernel(const uint32_t *ht, const uint64_t *slot,
@@ -526,8 +533,8 @@ struct csmv_engine {
     // control + messaging
     uint8_t *ctl = nullptr;
     uint64_t *ring = nullptr, *reply = nullptr, *done = nullptr;
-    uint32_t *mbox = nullptr, *stamps = nullptr;
-    int32_t *vals = nullptr;
+    uint32_t *mbox = nullptr;
+    uint64_t *sv = nullptr;
     uint64_t *reply_v = nullptr;
     // routing of the resident stream (empty = everything on the normal lane)
     uint32_t *norm_list = nullptr, *hot_list = nullptr;
@@ -838,8 +845,7 @@ csmv_status csmv_create(const csmv_config *cfg_in, csmv_engine **out)
         (s = dalloc(e, &e->done, e->Q)) ||
         (s = dalloc(e, &e->reply, (size_t)cfg.client_waves * kWave)) ||
         (s = dalloc(e, &e->mbox, (size_t)cfg.client_waves * rows * kWave)) ||
-        (s = dalloc(e, &e->stamps, cfg.n_items)) || (s = dalloc(e, &e->tmp, cfg.n_items)) ||
-        (s = dalloc(e, &e->vals, cfg.n_items)) ||
+        (s = dalloc(e, &e->sv, cfg.n_items)) || (s = dalloc(e, &e->tmp, cfg.n_items)) ||
         (s = dalloc(e, &e->reply_v, (size_t)cfg.client_waves * kWave * 2 * kBankW))) {
         csmv_destroy(e);
         return s;
@@ -853,7 +859,7 @@ csmv_status csmv_destroy(csmv_engine *e)
     if (!e) return CSMV_E_ARG;
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     dfree(e->ht); dfree(e->slot); dfree(e->ctl); dfree(e->ring); dfree(e->done);
-    dfree(e->reply); dfree(e->mbox); dfree(e->stamps); dfree(e->tmp); dfree(e->vals);
+    dfree(e->reply); dfree(e->mbox); dfree(e->sv); dfree(e->tmp);
     dfree(e->reply_v); dfree(e->norm_list); dfree(e->hot_list);
     dfree(e->mc_key); dfree(e->mc_op); dfree(e->mc_cdf); dfree(e->mc_ts); dfree(e->mc_res);
     dfree(e->mc_vsum); dfree(e->mc_ab); dfree(e->mc_lat);
@@ -875,7 +881,7 @@ static csmv_status fill_pool(csmv_engine *e, const int32_t *dvals, int32_t v)
 {
     const uint32_t n = e->cfg.n_items;
     hipLaunchKernelGGL(pool_fill_kernel, dim3((n + 255) / 256), dim3(256), 0, e->stream, e->ht,
-                       e->slot, e->vals, n, e->cfg.depth, dvals, v);
+                       e->slot, e->sv, n, e->cfg.depth, dvals, v);
     HIPCHK(e, hipGetLastError());
     HIPCHK(e, hipStreamSynchronize(e->stream));
     e->pool_loaded = true;
@@ -1075,6 +1081,7 @@ static void decode_stats(const uint8_t *host_ctl, const ServerLayout &L, float m
     const double ns_per_tick = 1e6 / (double)khz;
     for (int i = 0; i < kProfCount; ++i) out->prof_ns[i] = (double)pf[i] * ns_per_tick;
     out->prof_ns[kProfClientRounds] = (double)pf[kProfClientRounds];
+    out->prof_ns[kProfCommitMiss] = (double)pf[kProfCommitMiss];
     out->stamp_buckets = L.nbuckets;
     out->stamp_exact = L.exact;
     out->aborts_unknown = st[kStatAbortUnknown];
@@ -1092,8 +1099,7 @@ static void base_ctx(csmv_engine *e, Ctx &c)
     c.reply = e->reply;
     c.done = e->done;
     c.mbox = e->mbox;
-    c.stamps = e->stamps;
-    c.vals = e->vals;
+    c.sv = e->sv;
     c.reply_v = e->reply_v;
 }
 
@@ -1106,8 +1112,11 @@ static csmv_status reset_run_state(csmv_engine *e, const ServerLayout &L)
     HIPCHK(e, hipMemsetAsync(e->reply, 0, (size_t)e->cfg.client_waves * kWave * 8, e->stream));
     HIPCHK(e, hipMemsetAsync(e->reply_v, 0, (size_t)e->cfg.client_waves * kWave * 2 * kBankW * 8,
                              e->stream));
-    if ((e->cfg.variant & 3u) == kValStamp && !L.exact)
-        HIPCHK(e, hipMemsetAsync(e->stamps, 0, (size_t)e->cfg.n_items * 4, e->stream));
+    if ((e->cfg.variant & 3u) == kValStamp && !L.exact) {
+        const uint32_t n = e->cfg.n_items;
+        hipLaunchKernelGGL(stamp_clear_kernel, dim3((n + 255) / 256), dim3(256), 0, e->stream, e->sv, n);
+        HIPCHK(e, hipGetLastError());
+    }
     return CSMV_OK;
 }
 
@@ -2104,9 +2113,10 @@ csmv_status scratch_device(int dev, ScratchDevice **out)
 }
 
 // Byte offsets of one run's buffers inside the array.  The words that must start at zero
-// (control block, rings, replies, HBM stamps) come first, so one clear covers [0, zero_end).
+// (control block, rings, replies) come first, so one clear covers [0, zero_end); the pool
+// fill sets the rings and the committed words (stamp 0).
 struct ScratchMap {
-    size_t ctl, ring, done, reply, reply_v, stamps, zero_end, mbox, ht, slot, vals, total;
+    size_t ctl, ring, done, reply, reply_v, zero_end, mbox, ht, slot, sv, total;
 };
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -2121,12 +2131,11 @@ ScratchMap scratch_map(const csmv_config &cfg, uint32_t Q)
     m.done = o; o = align256(o + (size_t)Q * 8);
     m.reply = o; o = align256(o + waves * kWave * 8);
     m.reply_v = o; o = align256(o + waves * kWave * 2 * kBankW * 8);
-    m.stamps = o; o = align256(o + n * 4);
     m.zero_end = o;
     m.mbox = o; o = align256(o + waves * mbox_rows<kBankR, kBankW>() * kWave * 4);
     m.ht = o; o = align256(o + n * 4);
     m.slot = o; o = align256(o + n * cfg.depth * 8);
-    m.vals = o; o = align256(o + n * 4);
+    m.sv = o; o = align256(o + n * 8);
     m.total = o;
     return m;
 }
@@ -2181,11 +2190,10 @@ extern "C" csmv_status csmv_bank_run_noalloc(const csmv_config *cfg_in, const cs
     c.done = (uint64_t *)(base + m.done);
     c.reply = (uint64_t *)(base + m.reply);
     c.reply_v = (uint64_t *)(base + m.reply_v);
-    c.stamps = (uint32_t *)(base + m.stamps);
     c.mbox = (uint32_t *)(base + m.mbox);
     c.ht = (uint32_t *)(base + m.ht);
     c.slot = (uint64_t *)(base + m.slot);
-    c.vals = (int32_t *)(base + m.vals);
+    c.sv = (uint64_t *)(base + m.sv);
     c.op = b->op;
     c.addr = b->addr;
     c.n_txns = b->n_txns;
@@ -2201,7 +2209,7 @@ extern "C" csmv_status csmv_bank_run_noalloc(const csmv_config *cfg_in, const cs
     hipLaunchKernelGGL(scratch_clear_kernel, dim3(256), dim3(256), 0, st, (uint64_t *)base,
                        m.zero_end / 8);
     hipLaunchKernelGGL(pool_fill_kernel, dim3((n + 255) / 256), dim3(256), 0, st, c.ht, c.slot,
-                       c.vals, n, cfg.depth, b->init, 0);
+                       c.sv, n, cfg.depth, b->init, 0);
     if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess) return CSMV_E_HIP;
     const auto t0 = std::chrono::steady_clock::now();
     hipLaunchKernelGGL(kern, dim3(grid), dim3(block), lds_bytes, st, c, L);

@@ -112,9 +112,11 @@ typedef struct {
      * [0] committer idle, [1] committer busy, [2] receivers waiting for the
      * committer, [3] receivers waiting for requests, [4] receivers staging,
      * [5] client waves waiting for replies, [6] client wave rounds, [7] rounds (count),
-     * committer stages: [8] mailbox read and progress publication, [9] committed
-     * values / validation, [10] serialized-lane chain sums, [11] timestamps, tables,
-     * replies and log; [12..15] unused */
+     * and, with phase timers compiled in (-DCSMV_PHASE_TIMERS=1), the committer's
+     * stages: [8] mailbox read and progress publication, [9] committed values /
+     * validation, [10] serialized-lane chain sums, [11] timestamps, tables, replies and
+     * log, [12] re-reading a mailbox the read-ahead found unstaged, [13] waiting for a
+     * completion slot, [14] requests the read-ahead missed (count); [15] unused */
     double prof_ns[16];
     uint32_t stamp_buckets;      /* LDS commit-stamp buckets (STAMP mode) */
     uint32_t stamp_exact;        /* 1 if every item has its own bucket */

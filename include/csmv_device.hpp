@@ -238,8 +238,8 @@ struct Ctx {
     uint64_t *reply;         // [client_waves * 64] (n+1) << 32 | commit ts
     uint64_t *done;          // [Q] (n+1) << 32 | last ts of batch n (write-back finished)
     uint32_t *mbox;          // [client_waves][rows][64] request mailboxes
-    uint32_t *stamps;        // [n_items] commit stamps (server-private, STAMP mode)
-    int32_t *vals;           // [n_items] newest committed values (server-private)
+    uint64_t *sv;            // [n_items] newest commit of each item: stamp << 32 | value
+                             // (server-private; the stamp is 0 before the run's first commit)
     uint64_t *reply_v;       // [client_waves][2W][64] (n+1) << 32 | new value, then predecessor
                              // stamp of each write (delegated); word-major per wave, so one
                              // store or load of 64 lanes covers 512 contiguous bytes
@@ -311,6 +311,9 @@ enum : uint32_t {
     kProfCommitResolve,    // committer: committed values / validation
     kProfCommitChain,      // committer: serialized-lane chain sums
     kProfCommitFinish,     // committer: timestamps, tables, replies, log
+    kProfCommitSpin,       // committer: re-reading a mailbox the read-ahead found unstaged
+    kProfCommitAdv,        // committer: waiting for the advancer to free a completion slot
+    kProfCommitMiss,       // committer: requests the read-ahead did not find staged (count)
     kProfCount
 };
 

@@ -131,13 +131,21 @@ __device__ __forceinline__ void wg_fence() { __builtin_amdgcn_fence(__ATOMIC_ACQ
 __device__ __forceinline__ void wave_fence() { __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront"); }
 
 // The committer's phase timers (read / resolve / chain / finish) cost a clock read each on
-// the commit path, so they are compiled in only on request (-DCSMV_PHASE_TIMERS=1); the
-// busy and idle sums are always kept.
+// the commit path, so they are compiled in only on request (-DCSMV_PHASE_TIMERS=1).  The
+// roles' busy and waiting sums (one clock read per role and step) are kept unless
+// -DCSMV_ROLE_TIMERS=0: without them the serialized-lane commit path runs about 2 % faster
+// (code generation, not the reads themselves: reading the clocks on one request in eight
+// gained nothing).
 #ifndef CSMV_PHASE_TIMERS
 #define CSMV_PHASE_TIMERS 0
 #endif
 constexpr bool kPhaseTimers = CSMV_PHASE_TIMERS != 0;
 __device__ __forceinline__ uint64_t phase_clock() { return kPhaseTimers ? wall_clock64() : 0ull; }
+#ifndef CSMV_ROLE_TIMERS
+#define CSMV_ROLE_TIMERS 1
+#endif
+constexpr bool kRoleTimers = CSMV_ROLE_TIMERS != 0 || kPhaseTimers;
+__device__ __forceinline__ uint64_t role_clock() { return kRoleTimers ? wall_clock64() : 0ull; }
 
 __device__ __forceinline__ uint32_t mix32(uint32_t x)
 {
@@ -173,40 +181,43 @@ struct LaneSum {
 
 // ---- committer --------------------------------------------------------------------------
 // Committed-value tables.  Exact mode (every item fits LDS): stamp and value of every
-// item in LDS.  Hashed mode: the exact tables are in HBM (c.stamps, c.vals, written only
-// by the committer, so they hold every commit's value and stamp for this wave's own later
-// loads) and LDS holds the stamp buckets that filter the normal lane's validation.
+// item in LDS.  Hashed mode: one 64-bit word per item in HBM, c.sv[x] = stamp << 32 |
+// value of x's newest commit (written only by the committer, so it holds every commit for
+// this wave's own later loads; receivers read the stamp half), and LDS holds the stamp
+// buckets that filter the normal lane's validation.  One word per item makes an operand's
+// value and stamp one load and an install one store: the committer's gathers and scatters
+// are line-granular, and its CU's memory pipeline sets the pace of the commit path.
 struct ValueTables {
     uint32_t *bucket, *lval;
 };
 
-template <int R, int W>
-__device__ __forceinline__ void record_values(const Ctx &c, const ServerLayout &L, const ValueTables &V,
-                                              const uint32_t *wa, const int32_t *v, const bool *on)
+__device__ __forceinline__ uint64_t sv_load(const uint64_t *sv, uint32_t x)
 {
-#pragma unroll
-    for (int w = 0; w < W; ++w)
-        if (on[w] && wa[w] != kNone) {
-            // (the HBM store is an atomic one so that the two branches cannot be merged
-            // into one flat store, whose completion later LDS waits would also wait for)
-            if (L.exact) V.lval[wa[w]] = (uint32_t)v[w];
-            else __hip_atomic_store(&c.vals[wa[w]], v[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        }
+    return __hip_atomic_load(&sv[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
-
-template <int R, int W>
-__device__ __forceinline__ void record_stamps(const Ctx &c, const ServerLayout &L, const ValueTables &V,
-                                              const uint32_t *wa, uint32_t ts, bool on)
+__device__ __forceinline__ uint32_t sv_stamp(uint64_t w) { return (uint32_t)(w >> 32); }
+// The stamp half alone: a 32-bit load of the word's upper half (little-endian), through a
+// type that may alias the word.  The committer stores whole words, so the half read is one
+// of the stamps it stored.
+typedef uint32_t __attribute__((may_alias)) sv_half_t;
+__device__ __forceinline__ uint32_t sv_stamp_load(const uint64_t *sv, uint32_t x)
 {
-#pragma unroll
-    for (int w = 0; w < W; ++w)
-        if (on && wa[w] != kNone) {
-            __hip_atomic_fetch_max(&V.bucket[bucket_of(L, wa[w])], ts, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_WORKGROUP);
-            if (!L.exact)
-                __hip_atomic_fetch_max(&c.stamps[wa[w]], ts, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_WORKGROUP);
-        }
+    return __hip_atomic_load((const sv_half_t *)&sv[x] + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ int32_t sv_value(uint64_t w) { return (int32_t)(uint32_t)w; }
+
+// Records a commit of value v at timestamp ts for item x: the exact LDS value (the exact
+// stamp is the bucket), or the HBM word.  At most one lane of a request records an item
+// (the last writer of a serialized-lane chain; a normal-lane lane commits only if no lower
+// committed lane writes what it reads or writes), so the store needs no atomic max.
+// (The HBM store is an atomic one so that the two branches cannot be merged into one
+// flat store, whose completion later LDS waits would also wait for.)
+__device__ __forceinline__ void record_value(uint64_t *sv, const ServerLayout &L, const ValueTables &V,
+                                             uint32_t x, uint32_t ts, int32_t v)
+{
+    if (L.exact) V.lval[x] = (uint32_t)v;
+    else __hip_atomic_store(&sv[x], ((uint64_t)ts << 32) | (uint32_t)v, __ATOMIC_RELAXED,
+                            __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 // The bank transfer body executed by the serialized lane on the values the items held
@@ -273,6 +284,12 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
     constexpr int RO = kChainReads<R, W>;
     int32_t *jbase = (int32_t *)(lds + L.jmp_off);
     int32_t *jnv = jbase + kWave * RO;
+    uint64_t *const sv = c.sv;
+    uint32_t *const log_ts = c.log_ts;
+    int32_t *const log_v = c.log_v;
+    uint64_t *const reply = c.reply;
+    uint64_t *const reply_v = c.reply_v;
+    const uint64_t n_txns = c.n_txns;
     const uint32_t Q = c.qmask + 1;
     const uint64_t lt = (1ull << lane) - 1;
 
@@ -280,9 +297,10 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
     // statistics and timing sums, kept in lane 0 of vector registers: as wave-uniform
     // values they would take 20 scalar registers for the whole loop and push the loop's
     // other uniform values out to spill slots
-    LaneSum n_rw, n_pv, n_rec, n_unk, t_idle, t_busy, t_read, t_res, t_chain, t_fin;
+    LaneSum n_rw, n_pv, n_rec, n_unk, t_idle, t_busy, t_read, t_res, t_chain, t_fin, t_spin, t_adv,
+        n_miss;
     uint32_t n = 0, mbi = 0, adv_seen = 0;
-    uint64_t t_prev = wall_clock64();   // end of the previous request
+    uint64_t t_prev = role_clock();   // end of the previous request
     // staged rows (a wide wire keeps its read rows in LDS and reads them on use)
     constexpr int kDr = D && !kWide ? (R + 3) / 4 + W + 2 * RO + 1 : 0;
     constexpr int kNr = kWide ? 3 + 2 * W + 3 : 3 + R + 2 * W + 3 + kDr;
@@ -339,23 +357,30 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
 #pragma unroll
             for (int i = 0; i < kNr; ++i) row[i] = ahead[i];
         } else {
+            const uint64_t ts0 = phase_clock();
             Spin sp;
             for (;;) {
                 s = read_into(mbi, row);
                 if ((s & ~kExitBit) == n + 1) break;
                 if (!sp.tick(c, kErrMailboxTimeout)) { bad = true; break; }
             }
+            if constexpr (kPhaseTimers) {
+                t_spin.add(phase_clock() - ts0);
+                n_miss.add(1);
+            }
         }
         if (bad || (s & kExitBit)) break;
         // the completion slot n mod Q must be consumed (the advancer's position is read
         // again only when the last value seen does not cover it)
         if (!server_wb && n >= Q && adv_seen < n - Q + 1) {
+            const uint64_t ta0 = phase_clock();
             Spin sp;
             while ((adv_seen = lds_ld_acq(&lds[kLdsAdv])) < n - Q + 1)
                 if (!sp.tick(c, kErrDoneRingTimeout)) { bad = true; break; }
             if (bad) break;
+            if constexpr (kPhaseTimers) t_adv.add(phase_clock() - ta0);
         }
-        const uint64_t t1 = wall_clock64();
+        const uint64_t t1 = role_clock();
         int q = 0;
         const uint32_t cwr = row[q++];
         const uint32_t cw = cwr & ~kDelegateBit;
@@ -409,19 +434,13 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
         // hashed mode: the committed values of the chain nodes and the stamps of the
         // writes from the HBM tables (which hold this wave's own earlier stores), issued
         // as soon as the request's rows are in registers
-        int32_t hv[RO];
-        uint32_t hs[W];
+        uint64_t hsv[RO];
         if (delegate && !L.exact) {
 #pragma unroll
             for (int r = 0; r < RO; ++r) {
                 const uint32_t x = RS(r);
-                hv[r] = __hip_atomic_load(&c.vals[x != kNone ? x : 0u], __ATOMIC_RELAXED,
-                                          __HIP_MEMORY_SCOPE_WORKGROUP);
+                hsv[r] = sv_load(sv, x != kNone ? x : 0u);
             }
-#pragma unroll
-            for (int w = 0; w < W; ++w)
-                hs[w] = __hip_atomic_load(&c.stamps[wa[w] != kNone ? wa[w] : 0u], __ATOMIC_RELAXED,
-                                          __HIP_MEMORY_SCOPE_WORKGROUP);
         }
         // mailbox consumed; every stamp <= last is in the HBM tables for the waves of this
         // workgroup that acquire this release
@@ -467,12 +486,34 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
 #pragma unroll
                 for (int w = 0; w < W; ++w) prev[w] = (valid && wa[w] != kNone) ? ls[w] : kNone;
             } else {
-                // the committed value and stamp: the HBM tables hold those of every commit
-                // (this wave's own stores included), loaded when the request was read
+                // the committed value and stamp: the HBM words hold those of every commit
+                // (this wave's own stores included), loaded when the request was read.  A
+                // write's stamp comes with the read of the same item (a transfer reads what
+                // it writes); a write no read covers loads its word here.
 #pragma unroll
-                for (int r = 0; r < RO; ++r) base[r] = is_root[r] ? hv[r] : 0;
+                for (int r = 0; r < RO; ++r) base[r] = is_root[r] ? sv_value(hsv[r]) : 0;
+                bool miss[W], any_miss = false;
 #pragma unroll
-                for (int w = 0; w < W; ++w) prev[w] = (valid && wa[w] != kNone) ? hs[w] : kNone;
+                for (int w = 0; w < W; ++w) {
+                    uint32_t st = 0;
+                    bool found = false;
+#pragma unroll
+                    for (int r = 0; r < RO; ++r) {
+                        const bool m = !found && ops[r] == wa[w];
+                        st = m ? sv_stamp(hsv[r]) : st;
+                        found |= m;
+                    }
+                    miss[w] = valid && wa[w] != kNone && !found;
+                    any_miss |= miss[w];
+                    prev[w] = (valid && wa[w] != kNone) ? st : kNone;
+                }
+                if (ballot(any_miss)) {
+                    uint32_t ex[W];
+#pragma unroll
+                    for (int w = 0; w < W; ++w) ex[w] = sv_stamp_load(sv, miss[w] ? wa[w] : 0u);
+#pragma unroll
+                    for (int w = 0; w < W; ++w) prev[w] = miss[w] ? ex[w] : prev[w];
+                }
             }
             tres = phase_clock();
 #pragma unroll
@@ -526,11 +567,12 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
                         for (int j = 0; j < 16; ++j)
                             v[j] = (int32_t)V.lval[((need >> j) & 1u) ? x[j] : 0u];
                     } else {
+                        uint64_t w64[16];
 #pragma unroll
-                        for (int j = 0; j < 16; ++j)
-                            v[j] = __hip_atomic_load(&c.vals[((need >> j) & 1u) ? x[j] : 0u], __ATOMIC_RELAXED,
-                                                     __HIP_MEMORY_SCOPE_WORKGROUP);
-                        pin_round(v);
+                        for (int j = 0; j < 16; ++j) w64[j] = sv_load(sv, ((need >> j) & 1u) ? x[j] : 0u);
+                        pin_round(w64);
+#pragma unroll
+                        for (int j = 0; j < 16; ++j) v[j] = sv_value(w64[j]);
                     }
 #pragma unroll
                     for (int j = 0; j < 16; ++j) v[j] = ((need >> j) & 1u) ? v[j] : 0;
@@ -542,19 +584,19 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
                         if (pj != 0xFFu) val = jnv[(pj & 63u) * W + (pj >> 6)];
 #pragma unroll
                         for (int w = 0; w < W; ++w) val = (wa[w] == x[j]) ? nv[w] : val;
-                        c.log_v[(uint64_t)(c0 + j) * c.n_txns + k] = val;
+                        log_v[(uint64_t)(c0 + j) * n_txns + k] = val;
                     }
                 }
             }
-            record_values<R, W>(c, L, V, wa, nv, inst);
+            // the last writer of each item records it and raises the item's bucket: its
+            // timestamp is the highest of the item's writers in the request (lower writers
+            // of a hot item would only queue on the same LDS word)
 #pragma unroll
             for (int w = 0; w < W; ++w)
-                if (okx && wa[w] != kNone) {
+                if (inst[w]) {
+                    record_value(sv, L, V, wa[w], myts, nv[w]);
                     __hip_atomic_fetch_max(&V.bucket[bkw[w]], myts, __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_WORKGROUP);
-                    if (!L.exact)
-                        __hip_atomic_fetch_max(&c.stamps[wa[w]], myts, __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_WORKGROUP);
                 }
             if (okx) {
                 // a transfer logs its timestamp and every read; an audit its snapshot and
@@ -562,13 +604,13 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
                 uint32_t sum = 0;
 #pragma unroll
                 for (int r = 0; r < RO; ++r) sum += (uint32_t)rd[r];
-                c.log_ts[k] = ro ? below : myts;
-                c.log_v[k] = ro ? (int32_t)sum : rd[0];
+                log_ts[k] = ro ? below : myts;
+                log_v[k] = ro ? (int32_t)sum : rd[0];
 #pragma unroll
                 for (int r = 1; r < RO; ++r)
-                    if (!ro && ops[r] != kNone) c.log_v[(uint64_t)r * c.n_txns + k] = rd[r];
+                    if (!ro && ops[r] != kNone) log_v[(uint64_t)r * n_txns + k] = rd[r];
             }
-            uint64_t *rv = c.reply_v + (size_t)cw * 2 * W * kWave + lane;
+            uint64_t *rv = reply_v + (size_t)cw * 2 * W * kWave + lane;
 #pragma unroll
             for (int w = 0; w < W; ++w) {
                 st_rlx(&rv[(size_t)w * kWave], ((uint64_t)(n + 1) << 32) | (uint32_t)nv[w]);
@@ -594,9 +636,7 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
                         for (int r = 0; r < R + W; ++r) {
                             const uint32_t x = r < R ? RS(r) : wa[r - R];
                             ex[r] = 0;
-                            if (x != kNone && V.bucket[bucket_of(L, x)] > tg)
-                                ex[r] = __hip_atomic_load(&c.stamps[x], __ATOMIC_RELAXED,
-                                                          __HIP_MEMORY_SCOPE_WORKGROUP);
+                            if (x != kNone && V.bucket[bucket_of(L, x)] > tg) ex[r] = sv_stamp_load(sv, x);
                         }
                     } else {
                         // the flagged items' stamps in one batch of loads (an item not
@@ -620,8 +660,7 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
 #pragma unroll
                             for (int r = 0; r < R + W; ++r) {
                                 const uint32_t x = r < R ? RS(r) : wa[r - R];
-                                ex[r] = __hip_atomic_load(&c.stamps[fl[r] ? x : 0u], __ATOMIC_RELAXED,
-                                                          __HIP_MEMORY_SCOPE_WORKGROUP);
+                                ex[r] = sv_stamp_load(sv, fl[r] ? x : 0u);
                             }
                             pin_round(ex);
 #pragma unroll
@@ -693,11 +732,13 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
             nc = (uint32_t)__builtin_popcountll(committed);
             auto record = [&](uint32_t ts) {
                 if (vmode == kValStamp) {
-                    bool on[W];
 #pragma unroll
-                    for (int w = 0; w < W; ++w) on[w] = true;
-                    record_stamps<R, W>(c, L, V, wa, ts, true);
-                    record_values<R, W>(c, L, V, wa, wv, on);
+                    for (int w = 0; w < W; ++w)
+                        if (wa[w] != kNone) {
+                            __hip_atomic_fetch_max(&V.bucket[bucket_of(L, wa[w])], ts, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+                            record_value(sv, L, V, wa[w], ts, wv[w]);
+                        }
                 } else {
 #pragma unroll
                     for (int w = 0; w < W; ++w) atr[w * E + (ts & emask)] = wa[w];
@@ -734,10 +775,10 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
                 if (lane == 0) st_rlx(c.gc, last);
             }
         }
-        st_rlx(&c.reply[(size_t)cw * kWave + lane], ((uint64_t)(n + 1) << 32) | myts);
+        st_rlx(&reply[(size_t)cw * kWave + lane], ((uint64_t)(n + 1) << 32) | myts);
         if (!server_wb && nc == 0 && lane == 0)
             st_rlx(&c.done[n & c.qmask], ((uint64_t)(n + 1) << 32) | last);
-        const uint64_t t2 = wall_clock64();
+        const uint64_t t2 = role_clock();
         t_idle.add(t1 - t_prev);
         t_busy.add(t2 - t1);
         t_prev = t2;
@@ -767,6 +808,11 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
         fetch_add(&c.prof[kProfCommitResolve], t_res.v);
         fetch_add(&c.prof[kProfCommitChain], t_chain.v);
         fetch_add(&c.prof[kProfCommitFinish], t_fin.v);
+        if constexpr (kPhaseTimers) {
+            fetch_add(&c.prof[kProfCommitSpin], t_spin.v);
+            fetch_add(&c.prof[kProfCommitAdv], t_adv.v);
+            fetch_add(&c.prof[kProfCommitMiss], n_miss.v);
+        }
     }
 }
 
@@ -927,6 +973,7 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
     const bool stamp_mode = (c.variant & 0x3u) == kValStamp;
     const uint32_t lane = lane_id();
     const uint64_t lt = (1ull << lane) - 1;
+    const uint64_t *const sv = c.sv;
     uint32_t *mbst = lds + L.state_off;
     uint32_t *mbwave = lds + L.wave_off;
     uint32_t *mbtg = lds + L.tg_off;
@@ -950,7 +997,7 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
     uint64_t t_window = 0, t_ring = 0, t_busy = 0;
     for (uint32_t n = rid;; n += M) {
         const uint32_t mbi = n % L.nmb;
-        const uint64_t t0 = wall_clock64();
+        const uint64_t t0 = role_clock();
         // stage request n once its mailbox is free and the committer is at most
         // stage_ahead requests behind it (a short window keeps T_g fresh).  The committer
         // publishes its progress as soon as a narrow wire's rows are in registers, but it
@@ -972,7 +1019,7 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
                 }
             if (bad) break;
         }
-        const uint64_t t1 = wall_clock64();
+        const uint64_t t1 = role_clock();
         uint64_t e;
         bool leave = false;
         {
@@ -992,7 +1039,7 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
             }
         }
         if (leave) break;
-        const uint64_t t2 = wall_clock64();
+        const uint64_t t2 = role_clock();
         fence_acq();
         const uint32_t cwr = (uint32_t)e;
         const uint32_t cw = cwr & ~kDelegateBit;
@@ -1037,8 +1084,7 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
 #pragma unroll
                     for (int j = 0; j < kChunk; ++j) {
                         const bool want = valid && x[j] != kNone;
-                        st[j] = __hip_atomic_load(&c.stamps[want ? x[j] : 0u], __ATOMIC_RELAXED,
-                                                  __HIP_MEMORY_SCOPE_WORKGROUP);
+                        st[j] = sv_stamp_load(sv, want ? x[j] : 0u);
                     }
                     pin_round(st);
 #pragma unroll
@@ -1089,10 +1135,7 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
 #pragma unroll
             for (int w = 0; w < W; ++w) {
                 const uint32_t x = wa[w];
-                const uint32_t st = (!L.exact && !delegate && valid && x != kNone)
-                                        ? __hip_atomic_load(&c.stamps[x], __ATOMIC_RELAXED,
-                                                            __HIP_MEMORY_SCOPE_WORKGROUP)
-                                        : 0u;
+                const uint32_t st = (!L.exact && !delegate && valid && x != kNone) ? sv_stamp_load(sv, x) : 0u;
                 p = st > p ? st : p;
                 if (valid && x != kNone) {
                     const uint32_t h = hash_find(hkey, hmask, x);
@@ -1132,8 +1175,7 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
                     uint32_t v[R + W];
     #pragma unroll
                     for (int r = 0; r < R + W; ++r)
-                        v[r] = __hip_atomic_load(&c.stamps[it[r] != kNone ? it[r] : 0u], __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+                        v[r] = sv_stamp_load(sv, it[r] != kNone ? it[r] : 0u);
                     pin_round(v);
                     uint32_t p = 0;
     #pragma unroll
@@ -1392,7 +1434,7 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
         }
         wg_fence();
         if (lane == 0) lds_st_rel(&mbst[mbi], n + 1);
-        const uint64_t t3 = wall_clock64();
+        const uint64_t t3 = role_clock();
         t_window += t1 - t0;
         t_ring += t2 - t1;
         t_busy += t3 - t2;
@@ -1414,7 +1456,7 @@ __device__ __forceinline__ void server_workgroup(const Ctx &c, const ServerLayou
         for (uint32_t i = threadIdx.x; i < L.nbuckets; i += blockDim.x) lds[L.bucket_off + i] = 0;
         if (L.exact)
             for (uint32_t i = threadIdx.x; i < c.n_items; i += blockDim.x)
-                lds[L.val_off + i] = (uint32_t)c.vals[i];
+                lds[L.val_off + i] = (uint32_t)c.sv[i];
     }
     const uint32_t M = c.wg_waves - 2;
     for (uint32_t r = 0; r < M; ++r)
@@ -1435,6 +1477,13 @@ __device__ __forceinline__ void server_workgroup(const Ctx &c, const ServerLayou
     else if (wave == adv) {
         if ((c.variant & kServerWriteback) == 0) advancer(c, lds);
     } else receiver<R, W, D>(c, L, lds, wave - 1 - (wave > adv ? 1u : 0u), M);
+    // exact mode: the committed values go back to the HBM words (stamp 0, as a run's
+    // stamps restart), so a later run without a new fill starts from this run's state, as
+    // in hashed mode, where the words are the table
+    if ((c.variant & 0x3u) == kValStamp && L.exact) {
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < c.n_items; i += blockDim.x) c.sv[i] = lds[L.val_off + i];
+    }
 }
 
 }  // namespace csmv
