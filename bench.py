@@ -466,7 +466,7 @@ def main():
     ap.add_argument("--variant", default="stamp")
     ap.add_argument("--waves", type=int, default=int(os.environ.get("CSMV_BENCH_WAVES", "64")))
     ap.add_argument("--serial-waves", type=int, default=14)
-    ap.add_argument("--wg-waves", type=int, default=5,
+    ap.add_argument("--wg-waves", type=int, default=6,
                     help="waves per workgroup: the commit server is the committer, the clock "
                          "advancer and wg_waves - 2 receivers")
     ap.add_argument("--client-wg-waves", type=int, default=1,
