@@ -1096,18 +1096,44 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
 #pragma unroll
                     for (int j = 0; j < kChunk; ++j) st[j] = 0u;
                 }
+                // the chunk's slots in the write hash, probed together: one LDS round per
+                // probe step for all its reads (a read per probe loop would wait the
+                // longest probe of the wave once per read)
+                uint32_t fs[kChunk], hs[kChunk];
+                bool fp[kChunk];
 #pragma unroll
                 for (int j = 0; j < kChunk; ++j) {
-                    plane[j] = 0xFFu;
-                    p = st[j] > p ? st[j] : p;
-                    if (valid && x[j] != kNone) {
-                        const uint32_t h = hash_find(hkey, hmask, x[j]);
-                        if (h != kNone) {
-                            const uint64_t wm = wmask(h) & lt;
-                            conf |= wm;
-                            if (wm) plane[j] = 63u - (uint32_t)__builtin_clzll(wm);
-                        }
+                    fs[j] = kNone;
+                    hs[j] = mix32(x[j]) & hmask;
+                    fp[j] = valid && x[j] != kNone;
+                }
+                for (uint32_t step = 0; step <= hmask; ++step) {
+                    uint32_t k[kChunk];
+#pragma unroll
+                    for (int j = 0; j < kChunk; ++j)
+                        k[j] = __hip_atomic_load(&hkey[fp[j] ? hs[j] : 0u], __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+                    bool more = false;
+#pragma unroll
+                    for (int j = 0; j < kChunk; ++j) {
+                        const bool hit = fp[j] && k[j] == x[j];
+                        const bool next = fp[j] && k[j] != x[j] && k[j] != kNone;
+                        fs[j] = hit ? hs[j] : fs[j];
+                        hs[j] = next ? ((hs[j] + 1) & hmask) : hs[j];
+                        fp[j] = next;
+                        more |= next;
                     }
+                    if (ballot(more) == 0) break;
+                }
+                uint64_t wm[kChunk];
+#pragma unroll
+                for (int j = 0; j < kChunk; ++j) wm[j] = wmask(fs[j] != kNone ? fs[j] : 0u);
+#pragma unroll
+                for (int j = 0; j < kChunk; ++j) {
+                    wm[j] = fs[j] != kNone ? (wm[j] & lt) : 0ull;
+                    conf |= wm[j];
+                    plane[j] = wm[j] ? 63u - (uint32_t)__builtin_clzll(wm[j]) : 0xFFu;
+                    p = st[j] > p ? st[j] : p;
                     dst[(RW::items + c0 + j) * kWave + lane] = x[j];
                 }
                 if (delegate) {
