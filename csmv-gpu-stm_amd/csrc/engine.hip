@@ -694,7 +694,8 @@ ServerLayout server_layout_at(const csmv_config &cfg, uint32_t lds_max, uint32_t
     // writing it: load at most 1/2)
     const bool collab = vmode == kValCollab;
     L.hsz = pow2_at_least(kWave * (collab ? R + W : 2 * W)) << hshift;
-    L.harr = collab ? 5 : 3;
+    // a wide wire's receivers also keep a bitmap of the request's written items
+    L.harr = (collab ? 5 : 3) + (R > 16 ? 1 : 0);
     L.hash_off = w;
     w += M * L.harr * L.hsz;
     // serialized lane: per-receiver chain scratch (next, weight per chain node; superseded

@@ -72,8 +72,9 @@ struct ServerLayout {
     uint32_t hash_off;     // per-receiver hash tables [M][3][hsz]
     uint32_t hsz;          // power of two >= 2 * 64 * W (the writes of one request); with
                            // collaborative validation >= 2 * 64 * (R + W) (all its items)
-    uint32_t harr;         // arrays per receiver hash: key, writer lanes (lo, hi), and with
-                           // collaborative validation the lanes touching the item (lo, hi)
+    uint32_t harr;         // arrays per receiver hash: key, writer lanes (lo, hi), with
+                           // collaborative validation the lanes touching the item (lo, hi),
+                           // and for a wide wire a bitmap of the written items (last array)
     uint32_t words;        // total
 };
 
@@ -203,6 +204,11 @@ typedef uint32_t __attribute__((may_alias)) sv_half_t;
 __device__ __forceinline__ uint32_t sv_stamp_load(const uint64_t *sv, uint32_t x)
 {
     return __hip_atomic_load((const sv_half_t *)&sv[x] + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+// The value half alone (the word's lower half).
+__device__ __forceinline__ int32_t sv_value_load(const uint64_t *sv, uint32_t x)
+{
+    return (int32_t)__hip_atomic_load((const sv_half_t *)&sv[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 __device__ __forceinline__ int32_t sv_value(uint64_t w) { return (int32_t)(uint32_t)w; }
 
@@ -546,16 +552,20 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
                 wave_fence();
                 // the other reads see the write of their producer (a lower lane), this
                 // lane's own write, or the committed value before the request; their
-                // committed values are read before this request's values are recorded
-#pragma unroll 1
-                for (int c0 = RO; c0 < R; c0 += 16) {
-                    uint32_t x[16];
-                    int32_t v[16];
+                // committed values are read before this request's values are recorded.
+                // Chunks of 16 reads, pipelined: a chunk's gathers are issued before the
+                // previous chunk's values are used (the loop is unrolled, so the compiler's
+                // wait counts cover exactly the loads they need).
+                constexpr int kNc = (R - RO + 15) / 16;
+                uint32_t cx[2][16], cneed[2];
+                int32_t cv[2][16];
+                auto prep = [&](int ci, uint32_t (&x)[16], uint32_t &need, int32_t (&v)[16]) {
+                    const int c0 = RO + 16 * ci;
 #pragma unroll
                     for (int j = 0; j < 16; ++j) x[j] = (c0 + j < R) ? RS(c0 + j) : kNone;
-                    // the committed values the reads need, as one pinned round of loads
-                    // (a read that does not need one loads item 0)
-                    uint32_t need = 0;
+                    // the committed values the reads need (a read that does not need one
+                    // loads item 0)
+                    need = 0;
 #pragma unroll
                     for (int j = 0; j < 16; ++j) {
                         const bool own = x[j] != kNone && (x[j] == wa[0] || x[j] == wa[1]);
@@ -567,23 +577,24 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
                         for (int j = 0; j < 16; ++j)
                             v[j] = (int32_t)V.lval[((need >> j) & 1u) ? x[j] : 0u];
                     } else {
-                        uint64_t w64[16];
 #pragma unroll
-                        for (int j = 0; j < 16; ++j) w64[j] = sv_load(sv, ((need >> j) & 1u) ? x[j] : 0u);
-                        pin_round(w64);
-#pragma unroll
-                        for (int j = 0; j < 16; ++j) v[j] = sv_value(w64[j]);
+                        for (int j = 0; j < 16; ++j) v[j] = sv_value_load(sv, ((need >> j) & 1u) ? x[j] : 0u);
                     }
+                };
+                prep(0, cx[0], cneed[0], cv[0]);
 #pragma unroll
-                    for (int j = 0; j < 16; ++j) v[j] = ((need >> j) & 1u) ? v[j] : 0;
+                for (int ci = 0; ci < kNc; ++ci) {
+                    const int cur = ci & 1;
+                    if (ci + 1 < kNc) prep(ci + 1, cx[cur ^ 1], cneed[cur ^ 1], cv[cur ^ 1]);
+                    const int c0 = RO + 16 * ci;
 #pragma unroll
                     for (int j = 0; j < 16; ++j) {
-                        if (!okx || ro || x[j] == kNone) continue;
+                        if (!okx || ro || cx[cur][j] == kNone) continue;
                         const uint32_t pj = PROD(c0 + j);
-                        int32_t val = v[j];
+                        int32_t val = ((cneed[cur] >> j) & 1u) ? cv[cur][j] : 0;
                         if (pj != 0xFFu) val = jnv[(pj & 63u) * W + (pj >> 6)];
 #pragma unroll
-                        for (int w = 0; w < W; ++w) val = (wa[w] == x[j]) ? nv[w] : val;
+                        for (int w = 0; w < W; ++w) val = (wa[w] == cx[cur][j]) ? nv[w] : val;
                         log_v[(uint64_t)(c0 + j) * n_txns + k] = val;
                     }
                 }
@@ -983,6 +994,12 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
     // collaborative validation: lanes reading or writing each item
     const bool collab = (c.variant & 0x3u) == kValCollab;
     uint32_t *hrlo = hkey + 3 * L.hsz, *hrhi = hkey + 4 * L.hsz;
+    // wide wires: a bitmap of the request's written items (hsz words; bit index from the
+    // upper bits of the item's hash), so that a read no write can match skips the probe
+    uint32_t *hbits = hkey + (L.harr - 1) * L.hsz;
+    const bool hb = kWide && L.harr > (collab ? 5u : 3u);
+    auto hbit = [&](uint32_t m) -> uint32_t { return 1u << ((m >> 16) & 31u); };
+    auto hword = [&](uint32_t m) -> uint32_t { return (m >> 21) & (L.hsz - 1); };
     const uint32_t E = c.atr_entries, emask = E - 1;
     const uint32_t *atr = lds + L.atr_off;
     const uint32_t hmask = L.hsz - 1;
@@ -1062,8 +1079,13 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
                 slot_w[w] = kNone;
                 if (valid && wa[w] != kNone) {
                     slot_w[w] = hash_claim(hkey, hmask, wa[w]);
-                    if (slot_w[w] != kNone)
+                    if (slot_w[w] != kNone) {
                         atomicOr(lane < 32 ? &hwlo[slot_w[w]] : &hwhi[slot_w[w]], mybit);
+                        if (hb) {
+                            const uint32_t m = mix32(wa[w]);
+                            atomicOr(&hbits[hword(m)], hbit(m));
+                        }
+                    }
                 }
             }
             wg_fence();
@@ -1071,12 +1093,21 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
             uint32_t p = 0;
             constexpr int kChunk = 8;
             static_assert(R % kChunk == 0, "wide wires stage whole chunks of reads");
-            // the read rows, a chunk at a time
+            // the read rows, a chunk at a time; the next chunk's rows are loaded while this
+            // chunk is looked up (for the serialized lane nothing else in the loop goes to
+            // memory, so waiting for them waits for nothing more)
+            uint32_t xn[kChunk];
+#pragma unroll
+            for (int j = 0; j < kChunk; ++j) xn[j] = ld_rlx(&src[(1 + j) * kWave + lane]);
 #pragma unroll 1
             for (int c0 = 0; c0 < R; c0 += kChunk) {
                 uint32_t x[kChunk], st[kChunk], plane[kChunk];
 #pragma unroll
-                for (int j = 0; j < kChunk; ++j) x[j] = ld_rlx(&src[(1 + c0 + j) * kWave + lane]);
+                for (int j = 0; j < kChunk; ++j) x[j] = xn[j];
+                if (c0 + kChunk < R) {
+#pragma unroll
+                    for (int j = 0; j < kChunk; ++j) xn[j] = ld_rlx(&src[(1 + c0 + kChunk + j) * kWave + lane]);
+                }
                 // staging-time stamps of the normal lane (one pinned round of loads; an item
                 // not wanted reads item 0); the serialized lane needs none
                 if (!L.exact && !delegate) {
@@ -1099,13 +1130,19 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
                 // the chunk's slots in the write hash, probed together: one LDS round per
                 // probe step for all its reads (a read per probe loop would wait the
                 // longest probe of the wave once per read)
-                uint32_t fs[kChunk], hs[kChunk];
+                uint32_t fs[kChunk], hs[kChunk], mj[kChunk], bw[kChunk];
                 bool fp[kChunk];
+#pragma unroll
+                for (int j = 0; j < kChunk; ++j) mj[j] = mix32(x[j]);
+                if (hb) {
+#pragma unroll
+                    for (int j = 0; j < kChunk; ++j) bw[j] = hbits[hword(mj[j])];
+                }
 #pragma unroll
                 for (int j = 0; j < kChunk; ++j) {
                     fs[j] = kNone;
-                    hs[j] = mix32(x[j]) & hmask;
-                    fp[j] = valid && x[j] != kNone;
+                    hs[j] = mj[j] & hmask;
+                    fp[j] = valid && x[j] != kNone && (!hb || (bw[j] & hbit(mj[j])) != 0);
                 }
                 for (uint32_t step = 0; step <= hmask; ++step) {
                     uint32_t k[kChunk];
@@ -1176,6 +1213,8 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
                 if (slot_w[w] != kNone) {
                     hkey[slot_w[w]] = kNone;
                     hwlo[slot_w[w]] = 0; hwhi[slot_w[w]] = 0;
+                    // every bit of the bitmap is one of this request's writes
+                    if (hb) hbits[hword(mix32(wa[w]))] = 0;
                 }
             dst[RW::snap * kWave + lane] = snap;
 #pragma unroll
