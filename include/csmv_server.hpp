@@ -449,10 +449,12 @@ __device__ __forceinline__ void committer(const Ctx &c, const ServerLayout &L, u
             }
         }
         // mailbox consumed; every stamp <= last is in the HBM tables for the waves of this
-        // workgroup that acquire this release
+        // workgroup that acquire either word (one release fence orders both stores, so the
+        // wave waits for its pending LDS operations once)
         if (lane == 0) {
-            lds_st_rel(&lds[kLdsStampTs], last);
-            lds_st_rel(&lds[kLdsCnext], n + 1);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            __hip_atomic_store(&lds[kLdsStampTs], last, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_store(&lds[kLdsCnext], n + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         }
         if constexpr (kAhead) ahead_st = read_into(mbi + 1 == L.nmb ? 0u : mbi + 1, ahead);
         const uint64_t tr = phase_clock();
