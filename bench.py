@@ -156,11 +156,12 @@ def bank_plain_section(csmv, args):
     """The bank-plain configuration: 4096 uniform accounts of 1000, 2-account transfers and
     32-account audits 90/10, 2^20 transactions, 64 client waves x 64 lanes + the server
     workgroup, depth 4, on the normal lane in arrival order (vanilla = the reference's
-    protocol) and the stamp variant (with and without escalation)."""
+    protocol with the serial scan, collabValidation = the same with the receivers' collaborative
+    scan) and the stamp variant (with and without escalation)."""
     bank = csmv.Bank(n_txns=args.plain_txns, seed=1, n_reads=2, n_reads_ro=32, pct_ro=10)
     out = {"workload": f"4096 uniform accounts, 2-account transfers, 32-account audits, 90/10, "
                        f"{args.plain_txns} txns, 64 client waves, depth 4"}
-    for v, esc in (("vanilla", 0), ("stamp", 0), ("stamp", args.escalate_after)):
+    for v, esc in (("vanilla", 0), ("collabValidation", 0), ("stamp", 0), ("stamp", args.escalate_after)):
         name = v if not esc else "stamp_escalate"
         log(f"bank plain {name}")
         with csmv.Engine(4096, depth=4, atr_entries=4096, client_waves=64, wg_waves=6, variant=v,
