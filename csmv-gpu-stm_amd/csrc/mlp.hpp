@@ -9,9 +9,11 @@
 //
 // Features of a transfer over items x_0..x_{nr-1} (x_0, x_1 written, nr <= 4):
 //   f[h1(x) mod 32]      += 1 for a written item, 1/2 for a read-only one
-//   f[32 + h2(x) mod 32] += floor(log2(1 + A[h0(x) mod 4096])) / 8
-// where A holds the per-bucket abort counters folded from the abort log.  Every feature
-// is a multiple of 1/8 below 32, so the bf16 features are exact.
+//   f[32 + h2(x) mod 31] += floor(log2(1 + A[h0(x) mod 4096])) / 8
+//   f[63]                 = max over the items of floor(log2(1 + A[h0(x) mod 4096])) / 8
+// where A holds the per-bucket abort counters folded from the abort log (the last feature
+// keeps the most-aborted item apart from the hashed sums).  Every feature is a multiple of
+// 1/8 below 32, so the bf16 features are exact.
 //
 // Network: z = w2 . relu(W1^T x + b1) + b2, p = sigmoid(z), trained with the mean binary
 // cross-entropy on "aborted at least once".  One wave owns a tile of 64 samples; the
@@ -150,12 +152,16 @@ __device__ __forceinline__ void txn_features(const uint32_t *op, const uint32_t 
     const uint32_t o = valid ? op[k] : 0xFFu;
     if ((o & 0xFFu) == kKindTransfer) {
         const uint32_t nr = min((o >> 16) & 0xFFu, (uint32_t)kFeatItems);
+        uint32_t lgmax = 0;
         for (uint32_t j = 0; j < nr; ++j) {
             const uint32_t x = addr[(uint64_t)j * T + k];
             feat_add(a, feat_mix(x ^ kSaltRw) & 31u, j < 2 ? 1.f : 0.5f, q);
             const uint32_t lg = log2p1(counters[feat_mix(x) & (kAbortBuckets - 1u)]);
-            feat_add(a, 32u + (feat_mix(x ^ kSaltAb) & 31u), (float)lg * 0.125f, q);
+            feat_add(a, 32u + feat_mix(x ^ kSaltAb) % 31u, (float)lg * 0.125f, q);
+            lgmax = lg > lgmax ? lg : lgmax;
         }
+        // the most-aborted item apart from the hashed sums (a hot item among cold ones)
+        feat_add(a, 63u, (float)lgmax * 0.125f, q);
     }
 #pragma unroll
     for (int kk = 0; kk < 4; ++kk)

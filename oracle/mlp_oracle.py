@@ -10,7 +10,8 @@ is the engine's own definition:
 
   abort counters   A[h0(x) mod 4096] += failed attempts of each transfer touching x
   features         f[h1(x) mod 32] += 1 (written item) or 1/2 (read-only item);
-                   f[32 + h2(x) mod 32] += floor(log2(1 + A[h0(x)])) / 8
+                   f[32 + h2(x) mod 31] += floor(log2(1 + A[h0(x)])) / 8;
+                   f[63] = max over the items of floor(log2(1 + A[h0(x)])) / 8
   network          z = w2 . relu(W1^T f + b1) + b2, p = sigmoid(z), W1 in bf16
   loss             mean binary cross-entropy of "aborted at least once"
   routing          stable partition of the stream: transfers with p > tau first
@@ -19,7 +20,8 @@ How each part is pinned (tests/test_oracle_pred.py): the gradient against
 central finite differences of the float64 loss; the forward pass against the
 closed form of a network with W1 = 0 and against a hand-evaluated two-unit
 case; the features against their closed-form sums (the read/write part sums to
-the item weights, the abort part to sum_j lg(A)/8); bf16 rounding against
+the item weights, the hashed abort part to sum_j lg(A)/8, the last feature to
+max_j lg(A)/8); bf16 rounding against
 hand-picked ties; routing against a brute-force definition; Adam's first step
 against its closed form (each parameter moves by lr against its gradient's sign).
 """
@@ -106,6 +108,7 @@ def features(op, addr, T, idx, counters):
     nr = np.where(tr, np.minimum((o >> 16) & 0xFF, FEAT_ITEMS), 0)
     lg_table = log2p1(np.asarray(counters, dtype=np.uint32))
     rows = np.arange(idx.size)
+    lgmax = np.zeros(idx.size, dtype=np.int64)
     for j in range(FEAT_ITEMS):
         m = nr > j
         if not m.any():
@@ -114,8 +117,10 @@ def features(op, addr, T, idx, counters):
         w = 1.0 if j < 2 else 0.5
         np.add.at(X, (rows[m], (feat_mix(x ^ SALT_RW) & np.uint32(31)).astype(np.int64)), w)
         lg = lg_table[(feat_mix(x) & np.uint32(ABORT_BUCKETS - 1)).astype(np.int64)]
-        np.add.at(X, (rows[m], 32 + (feat_mix(x ^ SALT_AB) & np.uint32(31)).astype(np.int64)),
+        np.add.at(X, (rows[m], 32 + (feat_mix(x ^ SALT_AB) % np.uint32(31)).astype(np.int64)),
                   lg * 0.125)
+        lgmax[m] = np.maximum(lgmax[m], lg)
+    X[:, 63] = np.where(tr, lgmax * 0.125, 0.0)
     return X
 
 

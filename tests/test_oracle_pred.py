@@ -60,12 +60,17 @@ def test_features_closed_form_sums():
             items = [int(addr[j * T + k]) for j in range(n_reads)]
             # read/write part: the item weights (1 for the two written, 1/2 for the others)
             assert X[k, :32].sum() == 2 + 0.5 * (n_reads - 2)
-            # abort part: sum of floor(log2(1 + A)) / 8 over the items
-            lg = 0
+            # abort part: the hashed terms sum floor(log2(1 + A)) / 8 over the items, the
+            # last feature is its largest term
+            lgs = []
             for x in items:
                 a = int(counters[int(M.feat_mix(np.uint32(x))) & 4095])
-                lg += (a + 1).bit_length() - 1
-            assert X[k, 32:].sum() == lg / 8
+                lgs.append((a + 1).bit_length() - 1)
+            assert X[k, 32:63].sum() == sum(lgs) / 8
+            assert X[k, 63] == max(lgs) / 8
+            for x in items:
+                assert X[k, 32 + int(M.feat_mix(np.uint32(x) ^ M.SALT_AB)) % 31] > 0 or \
+                    (int(counters[int(M.feat_mix(np.uint32(x))) & 4095]) == 0)
             for x in items:
                 assert X[k, int(M.feat_mix(np.uint32(x) ^ M.SALT_RW)) & 31] >= 0.5
         # exact in bf16
@@ -80,7 +85,8 @@ def test_features_zero_counters_have_no_abort_part():
     ones = np.full(M.ABORT_BUCKETS, 3, dtype=np.uint32)   # lg(4) = 2 -> 1/4 per item
     X = M.features(op, addr, spec.n_txns, np.arange(spec.n_txns), ones)
     tr = (op & 0xFF) == 0
-    assert np.all(X[tr, 32:].sum(axis=1) == 0.5)
+    assert np.all(X[tr, 32:63].sum(axis=1) == 0.5)
+    assert np.all(X[tr, 63] == 0.25) and not X[~tr].any()
 
 
 def test_abort_fold_brute_force():
