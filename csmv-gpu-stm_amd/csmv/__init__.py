@@ -40,6 +40,11 @@ VARIANTS = {
     "collabValidation": VAL_COLLAB | INSERT_BATCH | WB_CLIENT,
     "stamp": VAL_STAMP | INSERT_BATCH | WB_CLIENT,
 }
+# the pipelined serialized lane (bank driver, every transfer routed to it): clients stage
+# their requests' producer chains, the committer orders each item's request-wide delta with
+# one atomic and installs the versions itself
+SERIAL_PIPE = 0x10
+PRESETS = dict(VARIANTS, stamp_pipe=VARIANTS["stamp"] | SERIAL_PIPE)
 
 STATUS = {0: "ok", 1: "HIP runtime error", 2: "invalid argument", 3: "capacity exceeded",
           4: "device wait timed out", 5: "no gfx950 device", 6: "call out of order",
@@ -63,7 +68,8 @@ class Config(C.Structure):
                 ("timeout_ms", C.c_uint32), ("max_attempts", C.c_uint32),
                 ("stage_ahead", C.c_uint32), ("lds_kib", C.c_uint32),
                 ("n_serial_waves", C.c_uint32), ("begin_wait", C.c_uint32),
-                ("client_wg_waves", C.c_uint32), ("escalate_after", C.c_uint32)]
+                ("client_wg_waves", C.c_uint32), ("escalate_after", C.c_uint32),
+                ("commit_parts", C.c_uint32)]
 
 
 class BankSpec(C.Structure):
@@ -267,13 +273,14 @@ class Memcd:
 def config(n_items: int, depth: int = 4, atr_entries: int = 4096, client_waves: int = 64,
            wg_waves: int = 8, variant="stamp", timeout_ms: int = 4000, max_attempts: int = 0,
            stage_ahead: int = 0, lds_kib: int = 0, n_serial_waves: int = 0,
-           begin_wait: bool = True, client_wg_waves: int = 0, escalate_after: int = 0) -> Config:
+           begin_wait: bool = True, client_wg_waves: int = 0, escalate_after: int = 0,
+           commit_parts: int = 0) -> Config:
     """A csmv_config (0 fields take the engine's defaults)."""
     if isinstance(variant, str):
-        variant = VARIANTS[variant]
+        variant = PRESETS[variant]
     return Config(n_items, depth, atr_entries, client_waves, wg_waves, variant, timeout_ms,
                   max_attempts, stage_ahead, lds_kib, n_serial_waves, 1 if begin_wait else 0,
-                  client_wg_waves, escalate_after)
+                  client_wg_waves, escalate_after, commit_parts)
 
 
 def bank_run_noalloc(cfg: Config, buffers: BankBuffers, stream: Optional[int] = None) -> dict:
@@ -292,10 +299,10 @@ class Engine:
                  client_waves: int = 64, wg_waves: int = 8, variant="stamp",
                  timeout_ms: int = 4000, max_attempts: int = 0, stage_ahead: int = 0,
                  lds_kib: int = 0, n_serial_waves: int = 0, begin_wait: bool = True,
-                 client_wg_waves: int = 0, escalate_after: int = 0):
+                 client_wg_waves: int = 0, escalate_after: int = 0, commit_parts: int = 0):
         self.cfg = config(n_items, depth, atr_entries, client_waves, wg_waves, variant,
                           timeout_ms, max_attempts, stage_ahead, lds_kib, n_serial_waves,
-                          begin_wait, client_wg_waves, escalate_after)
+                          begin_wait, client_wg_waves, escalate_after, commit_parts)
         self.n_items = n_items
         self._h = C.c_void_p()
         st = lib().csmv_create(C.byref(self.cfg), C.byref(self._h))

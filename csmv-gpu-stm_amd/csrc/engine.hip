@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <new>
 #include <string>
@@ -123,6 +124,35 @@ __global__ void __launch_bounds__(256) stamp_clear_kernel(uint64_t *sv, uint32_t
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) sv[i] &= 0xFFFFFFFFull;
+}
+
+// Pipelined serialized lane: an item's committer word is its newest value << 32 | the
+// number of installs its version ring has taken (head h and tail t give it: t stays 0 until
+// the ring is full, after which t = h + 1).
+__global__ void __launch_bounds__(256) pipe_init_kernel(const uint32_t *ht, const uint64_t *slot,
+                                                        uint64_t *sv, uint32_t n, uint32_t depth)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t h = ht[i] & 0xFFFFu, t = ht[i] >> 16;
+    const uint32_t v = (uint32_t)slot[(size_t)i * depth + h];
+    sv[i] = ((uint64_t)v << 32) | (t == 0 ? h : h + depth);
+}
+
+// Pipelined serialized lane: an audit it ran logged the value of each read (each by the
+// commit partition of its item); row 0 gets their sum, the audit's logged result.
+__global__ void __launch_bounds__(256) pipe_audit_sum_kernel(const uint32_t *op, const uint32_t *hot,
+                                                             uint64_t n_hot, uint64_t T, int32_t *log_v)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_hot) return;
+    const uint32_t k = hot[i];
+    const uint32_t o = op[k];
+    if ((o & 0xFFu) != kKindAudit) return;
+    const uint32_t nr = (o >> 16) & 0xFFu;
+    uint32_t sum = 0;
+    for (uint32_t r = 0; r < nr; ++r) sum += (uint32_t)log_v[(uint64_t)r * T + k];
+    log_v[k] = (int32_t)sum;
 }
 
 __global__ void __launch_bounds__(256) pool_head_kernel(const uint32_t *ht, const uint64_t *slot,
@@ -348,10 +378,15 @@ template <int R, int W> __device__ __forceinline__ void bank_client(const Ctx &c
 // Serialized lane: the wave hands routed transfers to the commit server, which executes
 // them in commit order on the newest committed values (no snapshot, so no conflict
 // window); the wave installs the versions the server returns.
-template <int R, int W> __device__ __forceinline__ void bank_serial_client(const Ctx &c, uint32_t cw)
+template <int R, int W, bool kPipe = false>
+__device__ __forceinline__ void bank_serial_client(const Ctx &c, const ServerLayout &L, uint32_t *scr,
+                                                   uint32_t cw)
 {
     const uint32_t lane = lane_id();
     const uint64_t NH = c.n_hot;
+    // pipelined form: the wave stages its requests' chains in its LDS scratch and the
+    // server installs the versions
+    if constexpr (kPipe) pipe_client_init<R, W>(L, scr);
     bool has = false, exhausted = false;
     uint64_t k = 0;
     uint32_t op = 0, ab = 0;
@@ -382,7 +417,9 @@ template <int R, int W> __device__ __forceinline__ void bank_serial_client(const
         Tx<R, W> tx;
         delegated_tx<R, W>(c, tx, has, k, op);
         bool fail = false;
-        const uint32_t ts = tx_commit<R, W>(c, cw, tx, has, &fail, &t_reply, true);
+        uint32_t ts;
+        if constexpr (kPipe) ts = tx_commit_pipe<R, W>(c, L, scr, cw, tx, has, &fail, &t_reply);
+        else ts = tx_commit<R, W>(c, cw, tx, has, &fail, &t_reply, true);
         if (fail) break;
         if (has) {
             if (ts) {
@@ -422,7 +459,27 @@ __global__ void __launch_bounds__(512) bank_kernel(Ctx c, ServerLayout L)
     if (wave >= c.client_wg_waves) return;
     const uint32_t cw = (blockIdx.x - 1) * c.client_wg_waves + wave;
     if (cw >= c.n_client_waves) return;
-    if (cw >= c.n_client_waves - c.n_serial_waves) bank_serial_client<R, W>(c, cw);
+    if (cw >= c.n_client_waves - c.n_serial_waves) bank_serial_client<R, W>(c, L, lds, cw);
+    else bank_client<R, W>(c, cw);
+}
+
+// The pipelined serialized lane's kernel (variant bit kSerialPipe): its own server
+// workgroup of up to 12 waves (several committers), clients that stage their chains in
+// their workgroup's LDS, and the normal lane's audits.
+template <int R, int W>
+__global__ void __launch_bounds__(768) bank_pipe_kernel(Ctx c, ServerLayout L)
+{
+    extern __shared__ uint32_t lds[];
+    if (blockIdx.x < L.parts) {
+        pipe_server_workgroup<R, W>(c, L, lds);
+        return;
+    }
+    const uint32_t wave = threadIdx.x / kWave;
+    if (wave >= c.client_wg_waves) return;
+    const uint32_t cw = (blockIdx.x - L.parts) * c.client_wg_waves + wave;
+    if (cw >= c.n_client_waves) return;
+    if (cw >= c.n_client_waves - c.n_serial_waves)
+        bank_serial_client<R, W, true>(c, L, lds + wave * pipe_client_words(L.hsz, R, W), cw);
     else bank_client<R, W>(c, cw);
 }
 
@@ -536,6 +593,7 @@ struct csmv_engine {
     uint32_t *mbox = nullptr;
     uint64_t *sv = nullptr;
     uint64_t *reply_v = nullptr;
+    uint64_t *sink = nullptr;   // pipelined serialized lane: idle lanes' targets
     // routing of the resident stream (empty = everything on the normal lane)
     uint32_t *norm_list = nullptr, *hot_list = nullptr;
     uint64_t n_norm = 0, n_hot = 0;
@@ -594,7 +652,9 @@ constexpr size_t kCtlBytes = 4096;
 constexpr size_t kOffGc = 0, kOffErr = 128, kOffDone = 256, kOffNext = 384, kOffTail = 512,
                  kOffStats = 640, kOffProf = 1024, kOffNextHot = 1536,
                  // serving loop: start tick, in-flight version bytes, [stalls, peak]
-                 kOffServeStart = 2048, kOffInflight = 2176, kOffServeCtr = 2304;
+                 kOffServeStart = 2048, kOffInflight = 2176, kOffServeCtr = 2304,
+                 // pipelined serialized lane: each commit partition's published timestamp
+                 kOffPipeProg = 2432;
 
 csmv_status fail(csmv_engine *e, csmv_status s, const std::string &msg)
 {
@@ -635,12 +695,16 @@ csmv_status normalize_config(csmv_config &cfg)
     if (cfg.wg_waves == 0) cfg.wg_waves = 8;
     if (cfg.stage_ahead == 0) cfg.stage_ahead = 3;
     if (cfg.client_wg_waves == 0) cfg.client_wg_waves = cfg.wg_waves;
+    if (cfg.commit_parts == 0) cfg.commit_parts = 1;
+    if (cfg.commit_parts > 4 || (cfg.commit_parts > 1 && !(cfg.variant & kSerialPipe))) return CSMV_E_ARG;
     if (cfg.n_serial_waves >= cfg.client_waves && cfg.n_serial_waves) return CSMV_E_ARG;
     if (cfg.n_items == 0 || cfg.depth == 0 || cfg.depth > 64 || cfg.client_waves == 0 ||
-        cfg.wg_waves < 3 || cfg.wg_waves > 8 || cfg.client_wg_waves > cfg.wg_waves ||
+        cfg.wg_waves < 3 || cfg.wg_waves > ((cfg.variant & kSerialPipe) ? 12u : 8u) ||
+        cfg.client_wg_waves > cfg.wg_waves ||
         cfg.stage_ahead > 64 || cfg.atr_entries < 64 ||
         (cfg.atr_entries & (cfg.atr_entries - 1)) || cfg.atr_entries > (1u << 16) ||
-        (cfg.variant & ~0xFu) || (cfg.variant & 3u) == 3u ||
+        (cfg.variant & ~0x1Fu) || (cfg.variant & 3u) == 3u ||
+        ((cfg.variant & kSerialPipe) && (cfg.variant & 0xFu) != (kValStamp | kInsertBatch)) ||
         (cfg.escalate_after && cfg.variant != (kValStamp | kInsertBatch)))
         return CSMV_E_ARG;
     return CSMV_OK;
@@ -678,7 +742,32 @@ ServerLayout server_layout_at(const csmv_config &cfg, uint32_t lds_max, uint32_t
                               bool delegation, uint32_t nmb, uint32_t hshift)
 {
     ServerLayout L{};
+    L.parts = 1;
     const uint32_t vmode = cfg.variant & 3u;
+    if (delegation && (cfg.variant & kSerialPipe) && R <= 16) {
+        // pipelined serialized lane: the receivers copy the rows the clients staged (no
+        // receiver hash or chain scratch, no stamp buckets); each committer's exchange of
+        // words per chain node, and the install records the publisher takes.  The clients'
+        // staging scratch lives in their own workgroups' LDS, the same size.
+        L.nmb = nmb ? nmb : 16;
+        L.mb_rows = mailbox_rows(R, W, true);
+        L.state_off = kLdsHdr;
+        L.wave_off = L.state_off + L.nmb;
+        L.tg_off = L.wave_off + L.nmb;
+        L.mb_off = (L.tg_off + L.nmb + 63) & ~63u;
+        uint32_t w = L.mb_off + L.nmb * L.mb_rows * kWave;
+        L.hsz = pow2_at_least(kWave * 2 * W) << 1;
+        L.harr = 0;
+        L.hash_off = L.sup_off = w;
+        L.jmp_off = w;
+        w += pipe_committers(cfg.wg_waves) * 2 * kWave * R;
+        L.pipe_off = w;
+        w += pipe_words(W);
+        L.atr_off = L.bucket_off = L.val_off = w;
+        L.words = std::max<uint32_t>(w, cfg.client_wg_waves * pipe_client_words(L.hsz, R, W));
+        L.parts = cfg.commit_parts;
+        return L;
+    }
     const uint32_t M = cfg.wg_waves - 2;
     // enough mailboxes for the receivers and for the staging window
     L.nmb = nmb ? nmb : std::min<uint32_t>(std::max<uint32_t>({M + 2, cfg.stage_ahead + 2, 4u}), 16);
@@ -737,6 +826,8 @@ ServerLayout server_layout_cfg(const csmv_config &cfg, uint32_t lds_max, uint32_
                                bool delegation, uint32_t nmb = 0)
 {
     const bool stamp = (cfg.variant & 3u) == kValStamp;
+    if (delegation && (cfg.variant & kSerialPipe) && R <= 16)
+        return server_layout_at(cfg, lds_max, R, W, delegation, nmb, 0);
     for (uint32_t hshift = 2; hshift > 0; --hshift) {
         const ServerLayout L = server_layout_at(cfg, lds_max, R, W, delegation, nmb, hshift);
         if ((uint64_t)L.words * 4 <= lds_max && (!stamp || L.exact || L.nbuckets >= 8192)) return L;
@@ -847,6 +938,7 @@ csmv_status csmv_create(const csmv_config *cfg_in, csmv_engine **out)
         (s = dalloc(e, &e->reply, (size_t)cfg.client_waves * kWave)) ||
         (s = dalloc(e, &e->mbox, (size_t)cfg.client_waves * rows * kWave)) ||
         (s = dalloc(e, &e->sv, cfg.n_items)) || (s = dalloc(e, &e->tmp, cfg.n_items)) ||
+        (s = dalloc(e, &e->sink, kSinkWords)) ||
         (s = dalloc(e, &e->reply_v, (size_t)cfg.client_waves * kWave * 2 * kBankW))) {
         csmv_destroy(e);
         return s;
@@ -860,7 +952,7 @@ csmv_status csmv_destroy(csmv_engine *e)
     if (!e) return CSMV_E_ARG;
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     dfree(e->ht); dfree(e->slot); dfree(e->ctl); dfree(e->ring); dfree(e->done);
-    dfree(e->reply); dfree(e->mbox); dfree(e->sv); dfree(e->tmp);
+    dfree(e->reply); dfree(e->mbox); dfree(e->sv); dfree(e->tmp); dfree(e->sink);
     dfree(e->reply_v); dfree(e->norm_list); dfree(e->hot_list);
     dfree(e->mc_key); dfree(e->mc_op); dfree(e->mc_cdf); dfree(e->mc_ts); dfree(e->mc_res);
     dfree(e->mc_vsum); dfree(e->mc_ab); dfree(e->mc_lat);
@@ -1059,6 +1151,7 @@ static void ctx_control(Ctx &c, uint8_t *ctl)
     c.stats = (uint64_t *)(ctl + kOffStats);
     c.prof = (uint64_t *)(ctl + kOffProf);
     c.next_hot = (uint64_t *)(ctl + kOffNextHot);
+    c.pipe_prog = (uint32_t *)(ctl + kOffPipeProg);
 }
 
 // Statistics of a launch from a host copy of its control block.
@@ -1102,6 +1195,7 @@ static void base_ctx(csmv_engine *e, Ctx &c)
     c.mbox = e->mbox;
     c.sv = e->sv;
     c.reply_v = e->reply_v;
+    c.sink = e->sink;
 }
 
 // Clears the control words and message rings of one engine before a launch.
@@ -1113,8 +1207,12 @@ static csmv_status reset_run_state(csmv_engine *e, const ServerLayout &L)
     HIPCHK(e, hipMemsetAsync(e->reply, 0, (size_t)e->cfg.client_waves * kWave * 8, e->stream));
     HIPCHK(e, hipMemsetAsync(e->reply_v, 0, (size_t)e->cfg.client_waves * kWave * 2 * kBankW * 8,
                              e->stream));
-    if ((e->cfg.variant & 3u) == kValStamp && !L.exact) {
-        const uint32_t n = e->cfg.n_items;
+    const uint32_t n = e->cfg.n_items;
+    if (e->cfg.variant & kSerialPipe) {
+        hipLaunchKernelGGL(pipe_init_kernel, dim3((n + 255) / 256), dim3(256), 0, e->stream, e->ht,
+                           e->slot, e->sv, n, e->cfg.depth);
+        HIPCHK(e, hipGetLastError());
+    } else if ((e->cfg.variant & 3u) == kValStamp && !L.exact) {
         hipLaunchKernelGGL(stamp_clear_kernel, dim3((n + 255) / 256), dim3(256), 0, e->stream, e->sv, n);
         HIPCHK(e, hipGetLastError());
     }
@@ -1136,6 +1234,9 @@ static csmv_status device_error(csmv_engine *e, uint32_t derr)
 {
     if (derr == kErrLivelock) return fail(e, CSMV_E_LIVELOCK, "transaction attempt limit");
     if (derr == kErrReadSetOverflow) return fail(e, CSMV_E_CAPACITY, "read/write set overflow");
+    if (derr == kErrPipeNormal)
+        return fail(e, CSMV_E_ARG, "a normal-lane update reached the pipelined serialized lane "
+                                   "(route every transfer to it)");
     if (derr) return fail(e, CSMV_E_TIMEOUT, "device wait timed out, code " + std::to_string(derr));
     return CSMV_OK;
 }
@@ -1144,7 +1245,7 @@ static csmv_status device_error(csmv_engine *e, uint32_t derr)
 // prepared context and collects its statistics.
 template <typename K>
 static csmv_status launch_persistent(csmv_engine *e, K kern, const Ctx &c, const ServerLayout &L,
-                                     csmv_stats *out)
+                                     csmv_stats *out, const std::function<void()> &after = {})
 {
     const size_t lds_bytes = (size_t)L.words * 4;
     if (lds_bytes > e->lds_max)
@@ -1153,7 +1254,7 @@ static csmv_status launch_persistent(csmv_engine *e, K kern, const Ctx &c, const
     const uint32_t block = e->cfg.wg_waves * kWave;
     const uint32_t cpw = e->cfg.client_wg_waves;
     const uint32_t client_wgs = (e->cfg.client_waves + cpw - 1) / cpw;
-    const uint32_t grid = 1 + client_wgs;
+    const uint32_t grid = L.parts + client_wgs;
     HIPCHK(e, hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)lds_bytes));
     int per_cu = 0;
@@ -1169,6 +1270,11 @@ static csmv_status launch_persistent(csmv_engine *e, K kern, const Ctx &c, const
     HIPCHK(e, hipEventRecord(e->ev0, e->stream));
     hipLaunchKernelGGL(kern, dim3(grid), dim3(block), lds_bytes, e->stream, c, L);
     HIPCHK(e, hipGetLastError());
+    // work that completes the run (inside the timed region)
+    if (after) {
+        after();
+        HIPCHK(e, hipGetLastError());
+    }
     HIPCHK(e, hipEventRecord(e->ev1, e->stream));
     HIPCHK(e, hipEventSynchronize(e->ev1));
     float ms = 0.f;
@@ -1205,7 +1311,20 @@ csmv_status csmv_bank_run(csmv_engine *e, csmv_stats *out)
                     (e->cfg.variant & kServerWriteback)))
         return fail(e, CSMV_E_ARG, "the serialized lane needs n_serial_waves > 0 and the "
                                    "STAMP variant with client write-back");
-    const csmv_status st = launch_persistent(e, bank_kernel<kBankR, kBankW>, c, L, out);
+    if ((e->cfg.variant & kSerialPipe) && (c.n_hot == 0 || c.n_serial_waves == 0))
+        return fail(e, CSMV_E_ARG, "the pipelined serialized lane needs routed transfers and "
+                                   "n_serial_waves > 0");
+    csmv_status st;
+    if (e->cfg.variant & kSerialPipe) {
+        // the audits the serialized lane ran logged their reads; their sums go to row 0
+        const uint64_t nh = c.n_hot;
+        st = launch_persistent(e, bank_pipe_kernel<kBankR, kBankW>, c, L, out, [&]() {
+            hipLaunchKernelGGL(pipe_audit_sum_kernel, dim3((unsigned)((nh + 255) / 256)), dim3(256), 0,
+                               e->stream, e->op, e->hot_list, nh, e->spec.n_txns, e->log_v);
+        });
+    } else {
+        st = launch_persistent(e, bank_kernel<kBankR, kBankW>, c, L, out);
+    }
     e->bank_logged = st == CSMV_OK;
     return st;
 }
@@ -1308,7 +1427,7 @@ csmv_status csmv_memcd_run(csmv_engine *e, csmv_stats *out)
     if (!e) return CSMV_E_ARG;
     if (!e->has_mc) return fail(e, CSMV_E_STATE, "run before generate");
     if (!e->pool_loaded) return fail(e, CSMV_E_STATE, "run needs a freshly loaded pool");
-    if ((e->cfg.variant & 3u) != kValStamp || (e->cfg.variant & kServerWriteback))
+    if ((e->cfg.variant & 3u) != kValStamp || (e->cfg.variant & (kServerWriteback | kSerialPipe)))
         return fail(e, CSMV_E_ARG, "the cache driver runs the STAMP variant with client write-back");
     const ServerLayout L = server_layout<kMcR, kMcW>(e, false);
     Ctx c{};
@@ -1377,6 +1496,8 @@ csmv_status csmv_kernel_prepare(csmv_engine *e, const csmv_kernel_spec *sp, void
     if (R == 0 || R > 64 || W == 0 || W > 32 || 1 + R + 2 * W > e->mbox_rows)
         return fail(e, CSMV_E_ARG, "kernel_prepare: wire beyond the request mailboxes");
     if (e->ext_prepared) return fail(e, CSMV_E_STATE, "kernel_prepare: a prepared kernel was not finished");
+    if (e->cfg.variant & kSerialPipe)
+        return fail(e, CSMV_E_ARG, "kernel_prepare: caller kernels run the normal lane");
     if (!e->pool_loaded) return fail(e, CSMV_E_STATE, "kernel_prepare needs a freshly loaded pool");
     const ServerLayout L = server_layout_rt(e, R, W, false);
     const size_t lds_bytes = (size_t)L.words * 4;
@@ -1884,7 +2005,7 @@ csmv_status csmv_serve_run_multi(csmv_engine *const *caches, uint32_t n_cache, c
         for (size_t j = 0; j < i; ++j)
             if (parts[i] == parts[j]) return fail(p, CSMV_E_ARG, "an engine serves one partition");
     for (csmv_engine *e : parts) {
-        if ((e->cfg.variant & 3u) != kValStamp || (e->cfg.variant & kServerWriteback))
+        if ((e->cfg.variant & 3u) != kValStamp || (e->cfg.variant & (kServerWriteback | kSerialPipe)))
             return fail(e, CSMV_E_ARG, "serving runs the STAMP variant with client write-back");
         if (!e->pool_loaded) return fail(e, CSMV_E_STATE, "serve needs a freshly loaded pool");
         if (e->device != p->device || e->cfg.wg_waves != p->cfg.wg_waves)
@@ -2155,7 +2276,7 @@ extern "C" csmv_status csmv_bank_run_noalloc(const csmv_config *cfg_in, const cs
 {
     if (!cfg_in || !b) return CSMV_E_ARG;
     csmv_config cfg = *cfg_in;
-    if (normalize_config(cfg) || cfg.n_serial_waves) return CSMV_E_ARG;
+    if (normalize_config(cfg) || cfg.n_serial_waves || (cfg.variant & kSerialPipe)) return CSMV_E_ARG;
     if (!b->init || !b->final_state || !b->op || !b->addr || !b->log_ts || !b->log_v || !b->log_ab ||
         b->n_txns == 0 || b->n_txns >= (1ull << 31))
         return CSMV_E_ARG;

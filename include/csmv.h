@@ -46,7 +46,15 @@ enum {
     CSMV_INSERT_LANE = 0x0,   /* per-lane record insertion (bank/csmv_sh/API.cu:193-272) */
     CSMV_INSERT_BATCH = 0x4,  /* one batched insertion per wave (batchedAtrInsert/csmv_sh/API.cu:266-389) */
     CSMV_WB_CLIENT = 0x0,     /* clients install versions (batchedAtrInsert/csmv_sh/API.cu:231-263) */
-    CSMV_WB_SERVER = 0x8      /* the server installs versions (vanilla/csmv_sh/API.cu:269-272) */
+    CSMV_WB_SERVER = 0x8,     /* the server installs versions (vanilla/csmv_sh/API.cu:269-272) */
+    CSMV_SERIAL_PIPE = 0x10   /* pipelined serialized lane (bank driver, with CSMV_VARIANT_STAMP
+                                 only; no counterpart in the reference): the serialized-lane
+                                 clients stage their requests' producer chains, the commit
+                                 server applies each written account's request-wide delta with
+                                 one ordered atomic and installs the versions itself.  Every
+                                 transfer must be routed to the serialized lane (the normal lane
+                                 then carries audits only); no escalation, cache or serving
+                                 runs, caller kernels or allocation-free runs. */
 };
 
 #define CSMV_VARIANT_VANILLA          (CSMV_VAL_SCAN | CSMV_INSERT_LANE | CSMV_WB_SERVER)
@@ -54,6 +62,7 @@ enum {
 #define CSMV_VARIANT_CLIENT_WRITEBACK (CSMV_VAL_SCAN | CSMV_INSERT_BATCH | CSMV_WB_SERVER)
 #define CSMV_VARIANT_COLLAB           (CSMV_VAL_COLLAB | CSMV_INSERT_BATCH | CSMV_WB_CLIENT)
 #define CSMV_VARIANT_STAMP            (CSMV_VAL_STAMP | CSMV_INSERT_BATCH | CSMV_WB_CLIENT)
+#define CSMV_VARIANT_STAMP_PIPE       (CSMV_VARIANT_STAMP | CSMV_SERIAL_PIPE)
 
 typedef struct {
     uint32_t n_items;          /* data items (accounts); VertionedDataItem count */
@@ -61,7 +70,7 @@ typedef struct {
     uint32_t atr_entries;      /* record ring capacity (TXRecordSize, API.cuh:17); power of two */
     uint32_t client_waves;     /* persistent client wavefronts (64 lanes each) */
     uint32_t wg_waves;         /* waves per workgroup, 3..8; the server workgroup uses all */
-    uint32_t variant;          /* CSMV_VAL_* | CSMV_INSERT_* | CSMV_WB_* */
+    uint32_t variant;          /* CSMV_VAL_* | CSMV_INSERT_* | CSMV_WB_* (| CSMV_SERIAL_PIPE) */
     uint32_t timeout_ms;       /* bound of every device-side wait (0 = 4000) */
     uint32_t max_attempts;     /* per-transaction attempt bound (0 = 1<<20) */
     uint32_t stage_ahead;      /* requests staged ahead of the committer (0 = 3) */
@@ -81,6 +90,11 @@ typedef struct {
                                   order, bank/csmv_sh/exe.cu:142-269).  Needs the STAMP
                                   variant with client write-back; the cache driver has no
                                   serialized lane and ignores it. */
+    uint32_t commit_parts;     /* pipelined serialized lane (CSMV_SERIAL_PIPE): commit server
+                                  workgroups, 1..4 (0 = 1).  Items are split among them by a
+                                  hash; each applies, installs and publishes its own items'
+                                  writes for every request, in request order, and the clock is
+                                  the least of their published progress. */
 } csmv_config;
 
 typedef struct {

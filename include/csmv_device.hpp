@@ -207,6 +207,10 @@ __device__ __forceinline__ void fence_rel() { __builtin_amdgcn_fence(__ATOMIC_RE
 // ---- commit-path variant bits (csmv.h CSMV_VAL_* / CSMV_INSERT_* / CSMV_WB_*) -----------
 constexpr uint32_t kValScan = 0x0, kValCollab = 0x1, kValStamp = 0x2;
 constexpr uint32_t kInsertBatch = 0x4, kServerWriteback = 0x8;
+// Pipelined serialized lane (bank driver, STAMP preset): clients stage their requests'
+// producer chains, the committer applies each item's request-wide delta with one ordered
+// atomic and installs the versions itself.
+constexpr uint32_t kSerialPipe = 0x10;
 
 // ---- device error codes (ctx.err) ------------------------------------------------------
 enum : uint32_t {
@@ -218,6 +222,7 @@ enum : uint32_t {
     kErrDoneRingTimeout = 6,
     kErrLivelock = 7,
     kErrReadSetOverflow = 8,
+    kErrPipeNormal = 9,      // a normal-lane update reached the pipelined serialized lane's server
 };
 
 // Engine view passed by value to every kernel.
@@ -280,7 +285,12 @@ struct Ctx {
     uint64_t *inflight;    // admitted in-flight version bytes (shared by the partitions)
     uint64_t budget;       // admission cap on *inflight (0 = none)
     uint64_t *serve_ctr;   // [0] admission rounds refused, [1] peak in-flight bytes
+    // pipelined serialized lane: spare words that idle lanes' atomics and stores target,
+    // so every step issues the same memory instructions ([kSinkWords] u64)
+    uint64_t *sink;
+    uint32_t *pipe_prog;   // [commit partitions] timestamp each has published through
 };
+constexpr uint32_t kSinkWords = 32768;
 
 enum : uint32_t {
     kStatCommitUpd = 0,

@@ -50,6 +50,11 @@ enum : uint32_t {
     kLdsCfinal = 2,   // requests served (valid once kLdsCdone)
     kLdsAdv = 3,      // batches the advancer has moved the clock over
     kLdsStampTs = 4,  // every stamp <= this value is visible in the HBM stamp table
+    // pipelined serialized lane
+    kLdsPub = 7,      // requests whose head words and clock the publisher has published
+    kLdsEnd = 8,      // 1 + the number of requests, once the stream has ended (0 before)
+    kLdsTok = 10,     // 64-bit issue token: the request whose atomics go next | the commit
+                      // timestamps assigned before it << 32
     kLdsHdr = 16
 };
 constexpr uint32_t kExitBit = 0x80000000u;
@@ -76,7 +81,32 @@ struct ServerLayout {
                            // collaborative validation the lanes touching the item (lo, hi),
                            // and for a wide wire a bitmap of the written items (last array)
     uint32_t words;        // total
+    uint32_t pipe_off;     // pipelined serialized lane: install records [kPipeRing][2W][64], the
+                           // timestamps after each of those requests [kPipeRing] and their
+                           // ready words [kPipeRing]
+    uint32_t parts;        // pipelined serialized lane: commit server workgroups (item partitions)
 };
+
+// The commit partition of an item (pipelined serialized lane with `parts` server workgroups).
+__device__ __forceinline__ uint32_t mix32(uint32_t x);
+__device__ __forceinline__ uint32_t pipe_part(uint32_t x, uint32_t parts)
+{
+    return parts == 1 ? 0u : (mix32(x) >> 8) % parts;
+}
+
+// Pipelined serialized lane: install records of this many requests wait for the publisher.
+constexpr uint32_t kPipeRing = 16;
+__host__ __device__ constexpr uint32_t pipe_words(uint32_t W) { return kPipeRing * (2 * W * kWave + 2); }
+// Committer waves of a pipelined server workgroup of wg waves (one more publishes, the
+// rest receive).
+__host__ __device__ constexpr uint32_t pipe_committers(uint32_t wg) { return wg >= 5 ? (wg - 1) / 2 : 1; }
+// A client wave's LDS scratch for staging a pipelined request: the write hash (key, writer
+// lanes lo / hi, canonical node, request-wide delta), then the chain arrays (next, weight
+// per node; superseded flag per write slot).
+__host__ __device__ constexpr uint32_t pipe_client_words(uint32_t hsz, uint32_t R, uint32_t W)
+{
+    return 5 * hsz + 2 * kWave * R + kWave * W;
+}
 
 // Row indices inside an LDS mailbox.  For a delegated request the wv rows carry the
 // transaction id and amount, and prod packs each read's producer (the lower lane whose
@@ -1511,6 +1541,735 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
         fetch_add(&c.prof[kProfRecvRing], t_ring);
         fetch_add(&c.prof[kProfRecvBusy], t_busy);
     }
+}
+
+// ---- pipelined serialized lane --------------------------------------------------------------
+// The serialized lane above resolves each request's values with a round of loads of the
+// committed table, so the committer waits for that round once per request; and its clients
+// install the versions, each install of an item waiting for the previous one on another CU
+// (about 1.2 us per hand-off, measured; a Zipf-hot account is written by every request).
+// The pipelined form (variant bit kSerialPipe, bank transfers) takes both waits off the
+// commit path:
+//  - a bank transfer changes an item by a delta known before it runs (-amt, +amt), so an
+//    item's value after a request is its value before plus the sum of the request's deltas
+//    on it.  The client stages its request's producer chains; each written item has one
+//    canonical node (the lowest read of it with no lower writer), which carries the
+//    request-wide delta.  The committer adds that delta, and one install, to the item's
+//    word (value << 32 | installs) with one 64-bit atomic and gets the word before the
+//    request back.  A wave's atomics on one word are performed in the order it issues
+//    them, so request n+1's atomics go out before request n's have returned: the commit
+//    order of the values is kept by the L2, not by the committer waiting.
+//  - the committer installs the last write of each item itself (the install count gives
+//    the ring slot), publishes the head words once those slot stores are complete, and the
+//    clock once the head words are, so a reader's snapshot only ever covers whole requests
+//    whose versions are in place.  Nothing waits for completion: a store issued before a
+//    request's atomics is complete once those atomics have returned (one wave's memory
+//    operations complete in issue order), which the committer waits for anyway.
+// The reference has no counterpart: its server validates and records every client's
+// commit (bank/csmv_sh/API.cu:88-272) and its clients write back their own versions.
+
+// Client side (TXCommit for a wave of routed transfers, bank/csmv_sh/exe.cu:184-201 as the
+// body): stage the request's producer chains in the wave's LDS scratch, write the rows the
+// server stages (Rows<R, W>) to the wave's mailbox, send, and wait for the reply word
+// (commit timestamp, or kSnapshotBit | snapshot for a read-only lane).  The server installs
+// the versions, so the client has nothing more to do.  scr: pipe_client_words(L.hsz, R, W)
+// words of this wave's LDS, cleared by pipe_client_init.
+template <int R, int W>
+__device__ __forceinline__ void pipe_client_init(const ServerLayout &L, uint32_t *scr)
+{
+    const uint32_t lane = lane_id();
+    for (uint32_t i = lane; i < L.hsz; i += kWave) {
+        scr[i] = kNone;
+        scr[L.hsz + i] = 0;
+        scr[2 * L.hsz + i] = 0;
+        scr[3 * L.hsz + i] = kNone;
+        scr[4 * L.hsz + i] = 0;
+    }
+    wave_fence();
+}
+
+template <int R, int W>
+__device__ __forceinline__ uint32_t tx_commit_pipe(const Ctx &c, const ServerLayout &L, uint32_t *scr,
+                                                   uint32_t cw, const Tx<R, W> &tx, bool req,
+                                                   bool *fail, uint64_t *wait_ticks)
+{
+    static_assert(!kWideWire<R> && W == 2, "the pipelined lane carries bank transfers");
+    using RW = Rows<R, W>;
+    constexpr int RO = R;
+    const uint32_t lane = lane_id();
+    const uint64_t lt = (1ull << lane) - 1;
+    const uint32_t hsz = L.hsz, hmask = hsz - 1;
+    uint32_t *hkey = scr, *hwlo = scr + hsz, *hwhi = scr + 2 * hsz, *canon = scr + 3 * hsz;
+    int32_t *htot = (int32_t *)(scr + 4 * hsz);
+    uint32_t *snext = scr + 5 * hsz;
+    int32_t *sw = (int32_t *)(snext + kWave * RO);
+    uint32_t *ssup = snext + 2 * kWave * RO;
+    const uint32_t mybit = 1u << (lane & 31);
+
+    uint32_t ops[RO], wa[W];
+#pragma unroll
+    for (int r = 0; r < RO; ++r) ops[r] = (req && (uint32_t)r < tx.nr) ? tx.rs[r] : kNone;
+#pragma unroll
+    for (int w = 0; w < W; ++w) wa[w] = (req && (uint32_t)w < tx.nw) ? tx.wa[w] : kNone;
+    const uint32_t k = (uint32_t)tx.wv[0], amt = (uint32_t)tx.wv[1];
+
+    // 1. the write set into the hash (item -> writer lanes); every probe step swaps for all
+    //    slots at once (an idle slot swaps kNone for kNone at 0, which changes nothing)
+    uint32_t slot_w[W], hp[W];
+    bool pend[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        slot_w[w] = kNone;
+        hp[w] = mix32(wa[w]) & hmask;
+        pend[w] = wa[w] != kNone;
+    }
+    for (uint32_t step = 0; step <= hmask; ++step) {
+        uint32_t old[W];
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            old[w] = kNone;
+            __hip_atomic_compare_exchange_strong(&hkey[pend[w] ? hp[w] : 0u], &old[w],
+                                                 pend[w] ? wa[w] : kNone, __ATOMIC_RELAXED,
+                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+        bool more = false;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            const bool got = pend[w] && (old[w] == kNone || old[w] == wa[w]);
+            slot_w[w] = got ? hp[w] : slot_w[w];
+            hp[w] = (pend[w] && !got) ? ((hp[w] + 1) & hmask) : hp[w];
+            pend[w] = pend[w] && !got;
+            more |= pend[w];
+        }
+        if (ballot(more) == 0) break;
+    }
+#pragma unroll
+    for (int w = 0; w < W; ++w)
+        if (slot_w[w] != kNone) atomicOr(lane < 32 ? &hwlo[slot_w[w]] : &hwhi[slot_w[w]], mybit);
+    wave_fence();
+
+    // 2. each read's slot (kNone: no lane writes the item)
+    uint32_t fs[RO], hs[RO];
+    bool fp[RO];
+#pragma unroll
+    for (int r = 0; r < RO; ++r) {
+        fs[r] = kNone;
+        hs[r] = mix32(ops[r]) & hmask;
+        fp[r] = ops[r] != kNone;
+    }
+    for (uint32_t step = 0; step <= hmask; ++step) {
+        uint32_t kk[RO];
+#pragma unroll
+        for (int r = 0; r < RO; ++r)
+            kk[r] = __hip_atomic_load(&hkey[fp[r] ? hs[r] : 0u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        bool more = false;
+#pragma unroll
+        for (int r = 0; r < RO; ++r) {
+            const bool hit = fp[r] && kk[r] == ops[r];
+            const bool next = fp[r] && kk[r] != ops[r] && kk[r] != kNone;
+            fs[r] = hit ? hs[r] : fs[r];
+            hs[r] = next ? ((hs[r] + 1) & hmask) : hs[r];
+            fp[r] = next;
+            more |= next;
+        }
+        if (ballot(more) == 0) break;
+    }
+
+    // 3. each read's producer (the highest lower lane writing the item, and its write
+    //    slot: lane | slot << 6, 0xFF = none), and each written item's canonical node (the
+    //    lowest node of the item without a producer)
+    uint32_t pj[RO];
+#pragma unroll
+    for (int r = 0; r < RO; ++r) {
+        const uint32_t at = fs[r] != kNone ? fs[r] : 0u;
+        const uint64_t m = ((uint64_t)__hip_atomic_load(&hwhi[at], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) << 32) |
+                           __hip_atomic_load(&hwlo[at], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        const uint64_t lower = fs[r] != kNone ? (m & lt) : 0ull;
+        const uint32_t pl = lower ? 63u - (uint32_t)__builtin_clzll(lower) : 0xFFu;
+        const uint32_t srcl = pl == 0xFFu ? lane : pl;
+        uint32_t code = 0xFFu;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            const uint32_t pw = lane_gather(wa[w], srcl);
+            code = (pl != 0xFFu && pw == ops[r]) ? (pl | ((uint32_t)w << 6)) : code;
+        }
+        pj[r] = code;
+    }
+#pragma unroll
+    for (int r = 0; r < RO; ++r)
+        if (fs[r] != kNone && pj[r] == 0xFFu) atomicMin(&canon[fs[r]], lane * RO + (uint32_t)r);
+
+    // 4. producer chains by pointer jumping (as stage_chains): each node's chain root and
+    //    the sum of the writers' deltas between them; a write slot is superseded when a
+    //    higher lane reads the item from it and writes it too
+    const int32_t d0 = (wa[1] == kNone) ? 0 : -(int32_t)amt, d1 = (int32_t)amt;
+    uint32_t nxt[RO];
+    int32_t wgt[RO];
+#pragma unroll
+    for (int r = 0; r < RO; ++r) {
+        const uint32_t node = lane * RO + r;
+        const bool has_p = pj[r] != 0xFFu;
+        const uint32_t srcl = has_p ? (pj[r] & 63u) : lane;
+        const int32_t q0 = (int32_t)lane_gather((uint32_t)d0, srcl),
+                      q1 = (int32_t)lane_gather((uint32_t)d1, srcl);
+        nxt[r] = has_p ? (pj[r] & 63u) * RO + (pj[r] >> 6) : node;
+        wgt[r] = has_p ? ((pj[r] >> 6) == 0 ? q0 : q1) : 0;
+        snext[node] = nxt[r];
+        sw[node] = wgt[r];
+    }
+#pragma unroll
+    for (int w = 0; w < W; ++w) ssup[lane * W + w] = 0;
+    wave_fence();
+    for (uint32_t round = 0; round < 8; ++round) {
+        bool more = false;
+        uint32_t nn[RO];
+        int32_t ww[RO];
+#pragma unroll
+        for (int r = 0; r < RO; ++r) {
+            const uint32_t node = lane * RO + r;
+            const uint32_t t = nxt[r];
+            const uint32_t tt = snext[t];
+            ww[r] = wgt[r] + (t != node ? sw[t] : 0);
+            nn[r] = tt;
+            more |= tt != t;
+        }
+        wave_fence();
+#pragma unroll
+        for (int r = 0; r < RO; ++r) {
+            nxt[r] = nn[r];
+            wgt[r] = ww[r];
+            snext[lane * RO + r] = nn[r];
+            sw[lane * RO + r] = ww[r];
+        }
+        wave_fence();
+        if (ballot(more) == 0) break;
+    }
+#pragma unroll
+    for (int r = 0; r < RO; ++r) {
+        bool mine_w = false;
+#pragma unroll
+        for (int w = 0; w < W; ++w) mine_w |= wa[w] == ops[r] && ops[r] != kNone;
+        if (pj[r] != 0xFFu && mine_w) ssup[(pj[r] & 63u) * W + (pj[r] >> 6)] = 1u;
+    }
+    wave_fence();
+    uint32_t supf = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) supf |= ssup[lane * W + w] ? (1u << w) : 0u;
+
+    // 5. the item's request-wide delta, from its last (unsuperseded) write: node w reads
+    //    write slot w's item (a transfer reads what it writes), so that write's value is
+    //    the value before the request + wgt[w] + the slot's delta
+#pragma unroll
+    for (int w = 0; w < W; ++w)
+        if (wa[w] != kNone && ((supf >> w) & 1u) == 0)
+            htot[slot_w[w]] = wgt[w] + (w == 0 ? d0 : d1);
+    wave_fence();
+    uint32_t root_out[RO];
+    int32_t wgt_out[RO];
+#pragma unroll
+    for (int r = 0; r < RO; ++r) {
+        const uint32_t node = lane * RO + r;
+        const bool written = fs[r] != kNone;
+        const uint32_t at = written ? fs[r] : 0u;
+        const uint32_t cn = canon[at];
+        const int32_t tt = htot[at];
+        // every chain root of a written item is a node without a producer, and all of
+        // them see the value before the request: they all take the canonical node's
+        const uint32_t rt = written ? cn : nxt[r];
+        const bool is_canon = written && cn == node;
+        root_out[r] = is_canon ? (node | 0x80000000u) : rt;
+        wgt_out[r] = is_canon ? tt : wgt[r];
+    }
+
+    // 6. the staged rows into the wave's mailbox, then the hash slots back to empty
+    uint32_t *mb = c.mbox + (size_t)cw * RW::count * kWave;
+    st_rlx(&mb[RW::snap * kWave + lane], req ? 0u : kNone);
+#pragma unroll
+    for (int r = 0; r < R; ++r) st_rlx(&mb[(RW::items + r) * kWave + lane], ops[r]);
+#pragma unroll
+    for (int w = 0; w < W; ++w) st_rlx(&mb[(RW::items + R + w) * kWave + lane], wa[w]);
+    st_rlx(&mb[RW::wv * kWave + lane], k);
+    st_rlx(&mb[(RW::wv + 1) * kWave + lane], amt);
+#pragma unroll
+    for (int r = 0; r < RO; ++r) {
+        st_rlx(&mb[(RW::root + r) * kWave + lane], root_out[r]);
+        st_rlx(&mb[(RW::wgt + r) * kWave + lane], (uint32_t)wgt_out[r]);
+    }
+    st_rlx(&mb[RW::supf * kWave + lane], supf);
+    wave_fence();
+#pragma unroll
+    for (int w = 0; w < W; ++w)
+        if (slot_w[w] != kNone) {
+            hkey[slot_w[w]] = kNone;
+            hwlo[slot_w[w]] = 0;
+            hwhi[slot_w[w]] = 0;
+            canon[slot_w[w]] = kNone;
+            htot[slot_w[w]] = 0;
+        }
+    wave_fence();
+
+    // 7. send; wait for the reply words (lane 0's first, then every lane's tag)
+    fence_rel();
+    uint64_t n = 0;
+    if (lane == 0) {
+        n = fetch_add(c.req_tail, (uint64_t)1);
+        st_rel(&c.ring[n & c.qmask], ((n + 1) << 32) | cw | kDelegateBit);
+    }
+    n = bcast64(n);
+    const uint64_t tw0 = wall_clock64();
+    Spin sp;
+    // one reply word per lane from every commit partition (partition 0's in reply, the
+    // others' in the reply-value rows): the mailbox is reused only once every partition has
+    // taken the request
+    const uint64_t *rp = c.reply + (size_t)cw * kWave;
+    for (uint32_t pt = 0; pt < L.parts; ++pt) {
+        const uint64_t *rq = pt == 0 ? rp : c.reply_v + ((size_t)cw * 2 * W + pt) * kWave;
+        uint64_t w0 = 0;
+        for (;;) {
+            if (lane == 0) w0 = ld_rlx(rq);
+            if ((bcast64(w0) >> 32) == n + 1) break;
+            if (!sp.tick(c, kErrReplyTimeout)) { *fail = true; return 0; }
+        }
+    }
+    uint64_t v;
+    for (;;) {
+        v = ld_rlx(&rp[lane]);
+        bool stale = (v >> 32) != n + 1;
+        for (uint32_t pt = 1; pt < L.parts; ++pt)
+            stale |= (ld_rlx(&c.reply_v[((size_t)cw * 2 * W + pt) * kWave + lane]) >> 32) != n + 1;
+        if (ballot(stale) == 0) break;
+        if (!sp.tick(c, kErrReplyTimeout)) { *fail = true; return 0; }
+    }
+    *wait_ticks += wall_clock64() - tw0;
+    return req ? (uint32_t)v : 0u;
+}
+
+// Global-memory accesses through an address chosen by a select (an active lane's target or
+// its sink).  The address-space cast keeps them global instructions: a generic (flat) one
+// would also count against the LDS wait counter, so every later LDS wait would wait for it.
+typedef __attribute__((address_space(1))) uint64_t g_u64;
+typedef __attribute__((address_space(1))) uint32_t g_u32;
+__device__ __forceinline__ uint64_t g_fetch_add(uintptr_t a, uint64_t v)
+{
+    return __hip_atomic_fetch_add((g_u64 *)a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void g_st_rlx64(uintptr_t a, uint64_t v)
+{
+    __hip_atomic_store((g_u64 *)a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void g_st_rlx32(uintptr_t a, uint32_t v)
+{
+    __hip_atomic_store((g_u32 *)a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void g_st32(uintptr_t a, uint32_t v) { *(g_u32 *)a = v; }
+
+// Waits until an atomic's word has returned (an empty asm statement that reads it: the
+// compiler waits for the load before it, and one wave's memory operations complete in
+// issue order, so everything issued before the atomic is complete too).
+__device__ __forceinline__ void wait_word(uint64_t v) { asm volatile("" ::"v"(v)); }
+
+// The pipelined committers: K waves of a server workgroup, wave k taking requests
+// n = k, k + K, ...  With several server workgroups (commit partitions) each one runs every
+// request for the items hashed to it (pipe_part): a transfer changes each item by its own
+// delta, so an item's chain, value, installs and head words never need another partition.
+// The committers' only serial step is the issue of a request's atomics, passed on by an LDS
+// token: request n's atomics are issued once request n-1's are (the token's workgroup
+// release and acquire order the two waves' read-modify-writes on every item they share),
+// and the token carries the request's first timestamp.  The rest runs in parallel across
+// the waves: wait for the words, execute the transfers (read values, timestamps, installs
+// of the last write of each item, log, reply) and hand the head words to the publisher
+// once the stores are complete.  Idle lanes issue nothing (their instructions are masked).
+template <int R, int W>
+__device__ __forceinline__ void committer_pipe(const Ctx &c, const ServerLayout &L, uint32_t *lds,
+                                               uint32_t k, uint32_t K)
+{
+    static_assert(!kWideWire<R> && W == 2, "the pipelined lane carries bank transfers");
+    using RW = Rows<R, W>;
+    constexpr int RO = R;
+    const uint32_t lane = lane_id();
+    const uint64_t lt = (1ull << lane) - 1;
+    const uint32_t D = c.depth;
+    const bool dpow2 = (D & (D - 1)) == 0;
+    const uint32_t part = blockIdx.x, P = L.parts;
+    uint32_t *mbst = lds + L.state_off;
+    uint32_t *mbwave = lds + L.wave_off;
+    uint32_t *mbfree = lds + L.tg_off;                      // [nmb] 1 + request finished in it
+    int32_t *jbase = (int32_t *)(lds + L.jmp_off + k * 2 * kWave * RO);   // [64 * RO]
+    uint32_t *jcnt = (uint32_t *)jbase + kWave * RO;                      // [64 * RO]
+    uint32_t *irec = lds + L.pipe_off;                      // [kPipeRing][2W][64]
+    uint32_t *ilast = irec + kPipeRing * 2 * W * kWave;     // [kPipeRing]
+    uint32_t *iready = ilast + kPipeRing;                   // [kPipeRing]
+    uint64_t *const tab = c.sv;
+    // a word per committer for its completion probe
+    uint64_t *const probe_at = c.sink + (size_t)(part * 16 + k) * 16;
+    const uint64_t T = c.n_txns;
+    auto mod_d = [&](uint32_t j) -> uint32_t { return dpow2 ? (j & (D - 1)) : j % D; };
+    LaneSum t_idle, t_busy, t_tok, t_atom, t_fin, t_room, t_probe;
+    bool held = false;                          // a finished request not yet handed over
+    uint32_t held_n = 0, held_qi = 0, held_last = 0;
+    uint32_t n = k;
+    for (;; n += K) {
+        const uint32_t m = n % L.nmb;
+        const uint64_t t0 = role_clock();
+        // request n staged, or the end of the stream
+        bool end = false, bad = false;
+        {
+            Spin sp;
+            for (;;) {
+                const uint32_t st = lds_ld_acq(&mbst[m]);
+                if ((st & ~kExitBit) == n + 1) {
+                    if (st & kExitBit) {
+                        end = true;
+                        if (lane == 0) lds_st_rel(&lds[kLdsEnd], n + 1);
+                    }
+                    break;
+                }
+                const uint32_t e = lds_ld_acq(&lds[kLdsEnd]);
+                if (e && n + 1 >= e) { end = true; break; }
+                if (!sp.tick(c, kErrMailboxTimeout)) { bad = true; break; }
+            }
+        }
+        if (end || bad) break;
+        const uint32_t cwr = mbwave[m];
+        if ((cwr & kDelegateBit) == 0) {
+            raise_error(c, kErrPipeNormal);
+            break;
+        }
+        const uint64_t t1 = role_clock();
+        const uint32_t *box = lds + L.mb_off + m * L.mb_rows * kWave;
+        const bool valid = box[RW::snap * kWave + lane] != kNone;
+        const uint32_t ktx = box[RW::wv * kWave + lane];
+        const uint32_t amt = box[(RW::wv + 1) * kWave + lane];
+        const bool ro = amt == kReadOnlyAmt;
+        const uint32_t supf = box[RW::supf * kWave + lane];
+        uint32_t ops[RO], rt[RO], wa[W];
+        int32_t wg[RO];
+        bool mine[RO];
+#pragma unroll
+        for (int r = 0; r < RO; ++r) {
+            ops[r] = box[(RW::items + r) * kWave + lane];
+            rt[r] = box[(RW::root + r) * kWave + lane];
+            wg[r] = (int32_t)box[(RW::wgt + r) * kWave + lane];
+        }
+#pragma unroll
+        for (int w = 0; w < W; ++w) wa[w] = box[(RW::items + R + w) * kWave + lane];
+        // the mailbox is free again once its rows are in registers
+        if (lane == 0) lds_st_rel(&mbfree[m], n + 1);
+        // request n's atomics after request n-1's: each canonical node of this partition's
+        // items adds its item's request-wide delta (and one install when the item is
+        // written) and gets the word before the request.  The addresses and addends are
+        // ready before the token arrives, and the token carries the request's first
+        // timestamp, so the hand-off is one LDS load and one LDS store.
+        bool at[RO];
+        uint64_t addr[RO], add[RO];
+#pragma unroll
+        for (int r = 0; r < RO; ++r) {
+            const uint32_t node = lane * RO + r;
+            mine[r] = valid && ops[r] != kNone && pipe_part(ops[r], P) == part;
+            at[r] = mine[r] && (rt[r] & 0xFFFFu) == node;
+            addr[r] = (uint64_t)(uintptr_t)(tab + (at[r] ? ops[r] : 0u));
+            add[r] = ((uint64_t)(uint32_t)wg[r] << 32) | (rt[r] >> 31);
+        }
+        const uint64_t committed = ballot(valid && !ro);
+        const uint32_t nc = (uint32_t)__builtin_popcountll(committed);
+        pin_round(addr);
+        pin_round(add);
+        const uint64_t tp0 = phase_clock();
+        uint32_t base = 0;
+        {
+            Spin sp;
+            for (;;) {
+                const uint64_t tok = __hip_atomic_load((uint64_t *)&lds[kLdsTok], __ATOMIC_ACQUIRE,
+                                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+                if ((uint32_t)tok == n) {
+                    base = (uint32_t)(tok >> 32);
+                    break;
+                }
+                if (!sp.tick(c, kErrMailboxTimeout)) { bad = true; break; }
+            }
+        }
+        if (bad) break;
+        uint64_t ret[RO];
+#pragma unroll
+        for (int r = 0; r < RO; ++r)
+            if (at[r]) ret[r] = g_fetch_add((uintptr_t)addr[r], add[r]);
+        // a probe after them (one lane): its return proves every earlier memory operation
+        // of this wave complete (one wave's operations complete in issue order)
+        uint64_t probe = 0;
+        if (lane == 0) probe = g_fetch_add((uintptr_t)probe_at, 0ull);
+        if (lane == 0)
+            __hip_atomic_store((uint64_t *)&lds[kLdsTok], ((uint64_t)(base + nc) << 32) | (n + 1),
+                               __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        const uint64_t tp1 = phase_clock();
+        wait_word(bcast64(probe));
+        // the words before the request, to every node of their items
+#pragma unroll
+        for (int r = 0; r < RO; ++r)
+            if (at[r]) {
+                jbase[lane * RO + r] = (int32_t)(uint32_t)(ret[r] >> 32);
+                jcnt[lane * RO + r] = (uint32_t)ret[r];
+            }
+        wave_fence();
+        // this wave's previous request: its stores were issued before the probe
+        if (held && lane == 0) {
+            ilast[held_qi] = held_last;
+            lds_st_rel(&iready[held_qi], held_n + 1);
+        }
+        held = false;
+        const uint64_t tp2 = phase_clock();
+        int32_t in[RO];
+        uint32_t cnt[RO];
+#pragma unroll
+        for (int r = 0; r < RO; ++r) {
+            const uint32_t rr = rt[r] & (kWave * RO - 1);
+            const bool cn = rr == lane * RO + (uint32_t)r;
+            in[r] = jbase[rr] + (cn ? 0 : wg[r]);
+            cnt[r] = jcnt[rr];
+        }
+        wave_fence();
+        int32_t rd[RO], nv[W];
+        transfer_exec<RO, W>(ops, amt, in, rd, wa, nv, !ro);
+        const uint32_t below = base + (uint32_t)__builtin_popcountll(committed & lt);
+        const uint32_t myts = valid ? (ro ? (kSnapshotBit | below) : below + 1) : 0u;
+        // the install records wait for a free ring entry (the publisher is done with the
+        // request kPipeRing before)
+        const uint64_t tp3 = phase_clock();
+        const uint32_t qi = n % kPipeRing;
+        if (n >= kPipeRing) {
+            Spin sp;
+            while (lds_ld_acq(&lds[kLdsPub]) < n + 1 - kPipeRing)
+                if (!sp.tick(c, kErrSlotTimeout)) { bad = true; break; }
+        }
+        if (bad) break;
+        const uint64_t tp3b = phase_clock();
+        // the last write of each of this partition's items goes to the ring slot after the
+        // item's installs so far (j-th install: slot j mod depth; the tail moves once the
+        // ring is full); node w reads write slot w's item (a transfer reads what it writes)
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            const bool inst = valid && !ro && wa[w] != kNone && ((supf >> w) & 1u) == 0 && mine[w];
+            const uint32_t j = cnt[w] + 1;
+            const uint32_t pos = mod_d(j);
+            const uint32_t tail = j < D ? 0u : mod_d(j + 1);
+            if (inst) st_rlx(c.slot + (size_t)wa[w] * D + pos, ((uint64_t)myts << 32) | (uint32_t)nv[w]);
+            irec[(qi * 2 * W + w) * kWave + lane] = inst ? wa[w] : kNone;
+            irec[(qi * 2 * W + W + w) * kWave + lane] = pos | (tail << 16);
+        }
+        // the log: a transfer's timestamp and reads, an audit's snapshot and the values it
+        // read (summed into row 0 after the run: pipe_audit_sum_kernel), each value by the
+        // partition of its item; then this partition's reply word
+        if (part == 0 && valid) c.log_ts[ktx] = ro ? below : myts;
+#pragma unroll
+        for (int r = 0; r < RO; ++r)
+            if (mine[r]) c.log_v[(uint64_t)r * T + ktx] = ro ? in[r] : rd[r];
+        const uint32_t cw = cwr & ~kDelegateBit;
+        uint64_t *rp = part == 0 ? c.reply + (size_t)cw * kWave
+                                 : c.reply_v + ((size_t)cw * 2 * W + part) * kWave;
+        st_rlx(&rp[lane], ((uint64_t)(n + 1) << 32) | myts);
+        const uint64_t tp4 = phase_clock();
+        // the installs go to the publisher once the slot stores are complete: proven by the
+        // return of this wave's next probe (or, at the end, of a last one)
+        held = true;
+        held_n = n;
+        held_qi = qi;
+        held_last = base + nc;
+        const uint64_t t2 = role_clock();
+        t_idle.add(t1 - t0);
+        t_busy.add(t2 - t1);
+        if constexpr (kPhaseTimers) {
+            t_tok.add(tp1 - tp0);
+            t_atom.add(tp2 - tp1);
+            t_fin.add(tp3 - tp2);
+            t_room.add(tp3b - tp3);
+            t_probe.add(tp4 - tp3b);
+        }
+    }
+    if (held) {
+        asm volatile("" ::: "memory");
+        uint64_t probe = 0;
+        if (lane == 0) probe = g_fetch_add((uintptr_t)probe_at, 0ull);
+        wait_word(bcast64(probe));
+        if (lane == 0) {
+            ilast[held_qi] = held_last;
+            lds_st_rel(&iready[held_qi], held_n + 1);
+        }
+    }
+    if (lane == 0) {
+        fetch_add(&c.prof[kProfCommitIdle], t_idle.v);
+        fetch_add(&c.prof[kProfCommitBusy], t_busy.v);
+        if constexpr (kPhaseTimers) {
+            // phases of the pipelined committers: token wait and atomic issue, the atomics'
+            // return, execution, ring room, stores
+            fetch_add(&c.prof[kProfCommitRead], t_tok.v);
+            fetch_add(&c.prof[kProfCommitResolve], t_atom.v);
+            fetch_add(&c.prof[kProfCommitChain], t_fin.v);
+            fetch_add(&c.prof[kProfCommitSpin], t_room.v);
+            fetch_add(&c.prof[kProfCommitFinish], t_probe.v);
+        }
+    }
+}
+
+// The publisher (one wave of each pipelined server workgroup): takes the requests whose
+// installs are complete in request order, as many as are ready, publishes their head words
+// (in request order, so an item's head only moves forward), waits for those stores, and
+// publishes the partition's progress; the clock is the least progress over the
+// partitions, so a reader's snapshot only ever covers whole requests whose versions are in
+// place everywhere.
+template <int R, int W>
+__device__ __forceinline__ void publisher_pipe(const Ctx &c, const ServerLayout &L, uint32_t *lds)
+{
+    const uint32_t lane = lane_id();
+    const uint32_t part = blockIdx.x, P = L.parts;
+    uint32_t *irec = lds + L.pipe_off;
+    uint32_t *ilast = irec + kPipeRing * 2 * W * kWave;
+    uint32_t *iready = ilast + kPipeRing;
+    uint64_t *const probe_at = c.sink + (size_t)(part * 16 + 15) * 16;
+    uint32_t q = 0, gc = 0;
+    bool bad = false;
+    LaneSum p_batches, p_wait, p_busy;
+    for (;;) {
+        const uint64_t tq0 = phase_clock();
+        // the ready requests from q on (at most half the ring at a time)
+        uint32_t qe = q;
+        {
+            Spin sp;
+            for (;;) {
+                while (qe < q + kPipeRing / 2 && lds_ld_acq(&iready[qe % kPipeRing]) == qe + 1) ++qe;
+                if (qe > q) break;
+                const uint32_t e = lds_ld_acq(&lds[kLdsEnd]);
+                if (e && q + 1 >= e) break;
+                if (!sp.tick(c, kErrSlotTimeout)) { bad = true; break; }
+            }
+        }
+        if (bad || qe == q) break;
+        const uint64_t tq1 = phase_clock();
+        for (uint32_t x = q; x < qe; ++x) {
+            const uint32_t qi = x % kPipeRing;
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                const uint32_t item = irec[(qi * 2 * W + w) * kWave + lane];
+                const uint32_t nh = irec[(qi * 2 * W + W + w) * kWave + lane];
+                if (item != kNone) st_rlx(c.ht + item, nh);
+            }
+        }
+        gc = ilast[(qe - 1) % kPipeRing];
+        asm volatile("" ::: "memory");
+        uint64_t probe = 0;
+        if (lane == 0) probe = g_fetch_add((uintptr_t)probe_at, 0ull);
+        wait_word(bcast64(probe));
+        if (lane == 0) {
+            // this partition's versions through timestamp gc are visible: the clock moves to
+            // the least such timestamp over the partitions
+            st_rlx(&c.pipe_prog[part], gc);
+            uint32_t g = gc;
+            for (uint32_t o = 0; o < P; ++o) {
+                const uint32_t v = ld_rlx(&c.pipe_prog[o]);
+                g = v < g ? v : g;
+            }
+            // (every version and head word is an agent-scope store, complete by the probe)
+            __hip_atomic_fetch_max(c.gc, g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            lds_st_rel(&lds[kLdsPub], qe);
+        }
+        q = qe;
+        if constexpr (kPhaseTimers) {
+            p_batches.add(1);
+            p_wait.add(tq1 - tq0);
+            p_busy.add(phase_clock() - tq1);
+        }
+    }
+    if constexpr (kPhaseTimers)
+        if (lane == 0) {
+            // the publishers' batches, waiting and busy time (the receivers' counters: the
+            // pipelined receivers keep none)
+            fetch_add(&c.prof[kProfRecvWindow], p_batches.v);
+            fetch_add(&c.prof[kProfRecvRing], p_wait.v);
+            fetch_add(&c.prof[kProfRecvBusy], p_busy.v);
+        }
+    if (lane == 0) {
+        lds[kLdsCfinal] = q;
+        lds_st_rel(&lds[kLdsCdone], 1u);
+        if (part == 0) {
+            fetch_add(&c.stats[kStatBatches], (uint64_t)q);
+            st_rlx(&c.stats[kStatFinalTs], ((uint64_t *)&lds[kLdsTok])[0] >> 32);
+        }
+    }
+}
+
+// A receiver of the pipelined lane: polls the request ring for requests n = rid, rid + M,
+// ... and copies the rows the client staged into LDS mailbox n mod nmb once a committer has
+// finished that mailbox's previous request.
+template <int R, int W>
+__device__ __forceinline__ void receiver_pipe(const Ctx &c, const ServerLayout &L, uint32_t *lds,
+                                              uint32_t rid, uint32_t M)
+{
+    using RW = Rows<R, W>;
+    constexpr int kA = 1 + R + 2 * W, kB = 2 * RW::RO + 1;
+    const uint32_t lane = lane_id();
+    uint32_t *mbst = lds + L.state_off;
+    uint32_t *mbwave = lds + L.wave_off;
+    uint32_t *mbfree = lds + L.tg_off;
+    for (uint32_t n = rid;; n += M) {
+        const uint32_t mbi = n % L.nmb;
+        if (n >= L.nmb) {
+            Spin sp;
+            bool bad = false;
+            while (lds_ld_acq(&mbfree[mbi]) < n + 1 - L.nmb)
+                if (lds_ld_acq(&lds[kLdsCdone]) || !sp.tick(c, kErrSlotTimeout)) {
+                    bad = true;
+                    break;
+                }
+            if (bad) break;
+        }
+        uint64_t e;
+        bool leave = false;
+        {
+            Spin sp;
+            for (uint32_t polls = 0;; ++polls) {
+                e = ld_rlx(&c.ring[n & c.qmask]);
+                if ((e >> 32) == (uint64_t)n + 1) break;
+                if ((polls & 7u) == 7u && ld_acq(c.clients_done) == c.n_client_waves &&
+                    ld_acq(c.req_tail) <= n) {
+                    // every client has left and request n will never come
+                    if (ld_rlx(c.req_tail) == n && lane == 0) lds_st_rel(&mbst[mbi], kExitBit | (n + 1));
+                    leave = true;
+                    break;
+                }
+                if (!sp.tick(c, kErrRingTimeout)) { leave = true; break; }
+            }
+        }
+        if (leave) break;
+        fence_acq();
+        const uint32_t cwr = (uint32_t)e;
+        const uint32_t *ps = c.mbox + (size_t)(cwr & ~kDelegateBit) * RW::count * kWave;
+        uint32_t *dst = lds + L.mb_off + mbi * L.mb_rows * kWave;
+        uint32_t v[kA + kB];
+#pragma unroll
+        for (int i = 0; i < kA; ++i) v[i] = ld_rlx(&ps[i * kWave + lane]);
+#pragma unroll
+        for (int i = 0; i < kB; ++i) v[kA + i] = ld_rlx(&ps[(RW::root + i) * kWave + lane]);
+        pin_round(v);
+#pragma unroll
+        for (int i = 0; i < kA; ++i) dst[i * kWave + lane] = v[i];
+#pragma unroll
+        for (int i = 0; i < kB; ++i) dst[(RW::root + i) * kWave + lane] = v[kA + i];
+        if (lane == 0) mbwave[mbi] = cwr;
+        wg_fence();
+        if (lane == 0) lds_st_rel(&mbst[mbi], n + 1);
+    }
+}
+
+// The server workgroup of the pipelined lane: K committers (pipe_committers), one
+// publisher (it moves the clock: no advancer), the other waves receive.
+template <int R, int W>
+__device__ __forceinline__ void pipe_server_workgroup(const Ctx &c, const ServerLayout &L, uint32_t *lds)
+{
+    const uint32_t wave = threadIdx.x / kWave;
+    for (uint32_t i = threadIdx.x; i < L.mb_off; i += blockDim.x) lds[i] = 0;
+    __syncthreads();
+    const uint32_t K = pipe_committers(c.wg_waves);
+    if (wave < K) committer_pipe<R, W>(c, L, lds, wave, K);
+    else if (wave == K) publisher_pipe<R, W>(c, L, lds);
+    else receiver_pipe<R, W>(c, L, lds, wave - K - 1, c.wg_waves - K - 1);
 }
 
 // D: the serialized lane (delegated requests) is compiled in.
