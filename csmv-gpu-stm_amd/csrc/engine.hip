@@ -206,7 +206,8 @@ __device__ __forceinline__ void delegated_tx(const Ctx &c, Tx<R, W> &tx, bool on
 // (exe.cu:213-223); transfers go through tx_commit.  With c.escalate set, a transaction
 // that has failed that many attempts is sent on the serialized lane instead (forward
 // progress: the reference retries in arrival order until it commits, exe.cu:142-269).
-template <int R, int W> __device__ __forceinline__ void bank_client(const Ctx &c, uint32_t cw)
+template <int R, int W, bool kPipe = false>
+__device__ __forceinline__ void bank_client(const Ctx &c, uint32_t cw)
 {
     const uint32_t lane = lane_id();
     const uint64_t T = c.n_txns;
@@ -270,7 +271,10 @@ template <int R, int W> __device__ __forceinline__ void bank_client(const Ctx &c
         }
 
         Tx<R, W> tx;
-        if (!tx_begin(c, tx, c.begin_wait ? my_last : 0u)) break;
+        // the pipelined lane's commit partitions publish progress words instead of one clock
+        // (and this lane commits no updates to wait for)
+        if (!tx_begin(c, tx, (c.begin_wait && !kPipe) ? my_last : 0u)) break;
+        if constexpr (kPipe) tx.snap = pipe_snapshot(c);
         const uint32_t kind = op & 0xFFu, amt = (op >> 8) & 0xFFu, nr = (op >> 16) & 0xFFu;
         const bool ro = has && kind == kKindAudit;
         const bool upd = has && kind == kKindTransfer;
@@ -480,7 +484,7 @@ __global__ void __launch_bounds__(768) bank_pipe_kernel(Ctx c, ServerLayout L)
     if (cw >= c.n_client_waves) return;
     if (cw >= c.n_client_waves - c.n_serial_waves)
         bank_serial_client<R, W, true>(c, L, lds + wave * pipe_client_words(L.hsz, R, W), cw);
-    else bank_client<R, W>(c, cw);
+    else bank_client<R, W, true>(c, cw);
 }
 
 struct McGenArgs {
@@ -1307,6 +1311,7 @@ csmv_status csmv_bank_run(csmv_engine *e, csmv_stats *out)
     c.hot_list = e->routed ? e->hot_list : nullptr;
     c.n_norm = e->routed ? e->n_norm : e->spec.n_txns;
     c.n_hot = e->routed ? e->n_hot : 0;
+    c.pipe_parts = (e->cfg.variant & kSerialPipe) ? e->cfg.commit_parts : 0;
     if (c.n_hot && (c.n_serial_waves == 0 || (e->cfg.variant & 3u) != kValStamp ||
                     (e->cfg.variant & kServerWriteback)))
         return fail(e, CSMV_E_ARG, "the serialized lane needs n_serial_waves > 0 and the "

@@ -289,6 +289,7 @@ struct Ctx {
     // so every step issues the same memory instructions ([kSinkWords] u64)
     uint64_t *sink;
     uint32_t *pipe_prog;   // [commit partitions] timestamp each has published through
+    uint32_t pipe_parts;   // commit partitions (pipelined lane; 0 elsewhere)
 };
 constexpr uint32_t kSinkWords = 32768;
 

@@ -1868,6 +1868,19 @@ __device__ __forceinline__ void g_st32(uintptr_t a, uint32_t v) { *(g_u32 *)a = 
 // issue order, so everything issued before the atomic is complete too).
 __device__ __forceinline__ void wait_word(uint64_t v) { asm volatile("" ::"v"(v)); }
 
+// TXBegin on the pipelined lane: the snapshot is the least timestamp every commit partition
+// has published its versions through (each partition's progress word, stored once its
+// versions and head words are in place).
+__device__ __forceinline__ uint32_t pipe_snapshot(const Ctx &c)
+{
+    uint32_t g = kNone;
+    for (uint32_t o = 0; o < c.pipe_parts; ++o) {
+        const uint32_t v = ld_acq(&c.pipe_prog[o]);
+        g = v < g ? v : g;
+    }
+    return bcast(g);
+}
+
 // The pipelined committers: K waves of a server workgroup, wave k taking requests
 // n = k, k + K, ...  With several server workgroups (commit partitions) each one runs every
 // request for the items hashed to it (pipe_part): a transfer changes each item by its own
@@ -1907,6 +1920,7 @@ __device__ __forceinline__ void committer_pipe(const Ctx &c, const ServerLayout 
     LaneSum t_idle, t_busy, t_tok, t_atom, t_fin, t_room, t_probe;
     bool held = false;                          // a finished request not yet handed over
     uint32_t held_n = 0, held_qi = 0, held_last = 0;
+    uint64_t tail_probe = 0;                    // issued after its stores
     uint32_t n = k;
     for (;; n += K) {
         const uint32_t m = n % L.nmb;
@@ -1974,6 +1988,15 @@ __device__ __forceinline__ void committer_pipe(const Ctx &c, const ServerLayout 
         const uint32_t nc = (uint32_t)__builtin_popcountll(committed);
         pin_round(addr);
         pin_round(add);
+        // the previous request's stores are complete once the probe load issued after them
+        // has returned (one wave's memory operations complete in issue order): its versions
+        // go to the publisher
+        wait_word(bcast64(tail_probe));
+        if (held && lane == 0) {
+            ilast[held_qi] = held_last;
+            lds_st_rel(&iready[held_qi], held_n + 1);
+        }
+        held = false;
         const uint64_t tp0 = phase_clock();
         uint32_t base = 0;
         {
@@ -1993,15 +2016,10 @@ __device__ __forceinline__ void committer_pipe(const Ctx &c, const ServerLayout 
 #pragma unroll
         for (int r = 0; r < RO; ++r)
             if (at[r]) ret[r] = g_fetch_add((uintptr_t)addr[r], add[r]);
-        // a probe after them (one lane): its return proves every earlier memory operation
-        // of this wave complete (one wave's operations complete in issue order)
-        uint64_t probe = 0;
-        if (lane == 0) probe = g_fetch_add((uintptr_t)probe_at, 0ull);
         if (lane == 0)
             __hip_atomic_store((uint64_t *)&lds[kLdsTok], ((uint64_t)(base + nc) << 32) | (n + 1),
                                __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
         const uint64_t tp1 = phase_clock();
-        wait_word(bcast64(probe));
         // the words before the request, to every node of their items
 #pragma unroll
         for (int r = 0; r < RO; ++r)
@@ -2010,12 +2028,6 @@ __device__ __forceinline__ void committer_pipe(const Ctx &c, const ServerLayout 
                 jcnt[lane * RO + r] = (uint32_t)ret[r];
             }
         wave_fence();
-        // this wave's previous request: its stores were issued before the probe
-        if (held && lane == 0) {
-            ilast[held_qi] = held_last;
-            lds_st_rel(&iready[held_qi], held_n + 1);
-        }
-        held = false;
         const uint64_t tp2 = phase_clock();
         int32_t in[RO];
         uint32_t cnt[RO];
@@ -2068,7 +2080,11 @@ __device__ __forceinline__ void committer_pipe(const Ctx &c, const ServerLayout 
         st_rlx(&rp[lane], ((uint64_t)(n + 1) << 32) | myts);
         const uint64_t tp4 = phase_clock();
         // the installs go to the publisher once the slot stores are complete: proven by the
-        // return of this wave's next probe (or, at the end, of a last one)
+        // return of a probe issued after them, which the next request waits for
+        // (a load of one word by every lane: one memory request, and no partial update of
+        // the register the next request waits on)
+        asm volatile("" ::: "memory");
+        tail_probe = __hip_atomic_load(probe_at, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         held = true;
         held_n = n;
         held_qi = qi;
@@ -2085,10 +2101,7 @@ __device__ __forceinline__ void committer_pipe(const Ctx &c, const ServerLayout 
         }
     }
     if (held) {
-        asm volatile("" ::: "memory");
-        uint64_t probe = 0;
-        if (lane == 0) probe = g_fetch_add((uintptr_t)probe_at, 0ull);
-        wait_word(bcast64(probe));
+        wait_word(bcast64(tail_probe));
         if (lane == 0) {
             ilast[held_qi] = held_last;
             lds_st_rel(&iready[held_qi], held_n + 1);
@@ -2119,7 +2132,7 @@ template <int R, int W>
 __device__ __forceinline__ void publisher_pipe(const Ctx &c, const ServerLayout &L, uint32_t *lds)
 {
     const uint32_t lane = lane_id();
-    const uint32_t part = blockIdx.x, P = L.parts;
+    const uint32_t part = blockIdx.x;
     uint32_t *irec = lds + L.pipe_off;
     uint32_t *ilast = irec + kPipeRing * 2 * W * kWave;
     uint32_t *iready = ilast + kPipeRing;
@@ -2158,16 +2171,10 @@ __device__ __forceinline__ void publisher_pipe(const Ctx &c, const ServerLayout 
         if (lane == 0) probe = g_fetch_add((uintptr_t)probe_at, 0ull);
         wait_word(bcast64(probe));
         if (lane == 0) {
-            // this partition's versions through timestamp gc are visible: the clock moves to
-            // the least such timestamp over the partitions
+            // this partition's versions through timestamp gc are in place (every version and
+            // head word is an agent-scope store, complete by the probe); a snapshot is the
+            // least of the partitions' progress words (pipe_snapshot)
             st_rlx(&c.pipe_prog[part], gc);
-            uint32_t g = gc;
-            for (uint32_t o = 0; o < P; ++o) {
-                const uint32_t v = ld_rlx(&c.pipe_prog[o]);
-                g = v < g ? v : g;
-            }
-            // (every version and head word is an agent-scope store, complete by the probe)
-            __hip_atomic_fetch_max(c.gc, g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             lds_st_rel(&lds[kLdsPub], qe);
         }
         q = qe;
@@ -2189,8 +2196,10 @@ __device__ __forceinline__ void publisher_pipe(const Ctx &c, const ServerLayout 
         lds[kLdsCfinal] = q;
         lds_st_rel(&lds[kLdsCdone], 1u);
         if (part == 0) {
+            const uint32_t fin = (uint32_t)(((uint64_t *)&lds[kLdsTok])[0] >> 32);
             fetch_add(&c.stats[kStatBatches], (uint64_t)q);
-            st_rlx(&c.stats[kStatFinalTs], ((uint64_t *)&lds[kLdsTok])[0] >> 32);
+            st_rlx(&c.stats[kStatFinalTs], (uint64_t)fin);
+            st_rlx(c.gc, fin);
         }
     }
 }
