@@ -7,8 +7,15 @@ audits read 4 accounts, 2^24 transactions, version depth 8.  The stream is gener
 the device before timing (resident in HBM).  One step = one pass of the persistent kernel
 over the whole stream; the pool is reloaded between steps outside the timed kernel.
 value = committed transactions / kernel seconds (HIP events on the engine's stream).
+Commit path of the headline: the pipelined serialized lane (variant stamp_pipe) with every
+transaction routed to it, 4 commit server workgroups of 12 waves (5 committers, a
+publisher, 6 receivers each) over a hash split of the accounts, 80 client waves of which
+64 feed the lane.
 
 Beside it, in the same JSON line:
+  audits_on_normal_lane / one_committer_serialized_lane
+             the same stream with audits as snapshot reads on the normal lane, and on
+             the serialized lane of one committer wave (clients install their versions)
   variants   the reference's four commit-path variants on the normal (client-executed)
              lane, same workload on a shorter stream
   memcd      the memcached-style cache at its configured shape (2^17 sets x 8 ways,
@@ -56,18 +63,21 @@ def bytes_per_commit(stats, n_reads=4, n_writes=2, n_reads_ro=4):
     return stats["commits_update"] * upd + stats["commits_ro"] * ro
 
 
-def load_traffic(commits):
+def load_traffic(commits, kernel):
     """HBM bytes of one launch: the per-commit traffic of the committed counter profile
     (FETCH_SIZE and WRITE_SIZE passes, scaled by a copy kernel of known size) times the
-    launch's commits."""
+    launch's commits; None when that profile was taken of another kernel."""
     p = os.path.join(ROOT, "profiles", "bank_zipf_1M_traffic.json")
     if not os.path.exists(p):
         return None
     try:
         with open(p) as f:
-            return json.load(f)["hbm_bytes_per_commit"] * commits
-    except Exception:
+            t = json.load(f)
+    except (OSError, ValueError):
         return None
+    if t.get("kernel") != kernel:
+        return None
+    return t["hbm_bytes_per_commit"] * commits
 
 
 def pct_us(lat_ticks, khz):
@@ -464,12 +474,23 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--variant", default="stamp")
-    ap.add_argument("--waves", type=int, default=int(os.environ.get("CSMV_BENCH_WAVES", "64")))
-    ap.add_argument("--serial-waves", type=int, default=14)
+    ap.add_argument("--variant", default="stamp_pipe",
+                    help="the headline's commit path: stamp_pipe (the pipelined serialized "
+                         "lane) or stamp (one committer wave)")
+    ap.add_argument("--waves", type=int, default=int(os.environ.get("CSMV_BENCH_WAVES", "80")))
+    ap.add_argument("--serial-waves", type=int, default=64)
     ap.add_argument("--wg-waves", type=int, default=6,
-                    help="waves per workgroup: the commit server is the committer, the clock "
-                         "advancer and wg_waves - 2 receivers")
+                    help="waves per workgroup of the one-committer server: the committer, the "
+                         "clock advancer and wg_waves - 2 receivers")
+    ap.add_argument("--pipe-wg-waves", type=int, default=12,
+                    help="waves per workgroup of each pipelined commit server: (w - 1) / 2 "
+                         "committers, one publisher, the rest receivers")
+    ap.add_argument("--commit-parts", type=int, default=4,
+                    help="commit server workgroups of the pipelined lane (items split by hash)")
+    ap.add_argument("--one-waves", type=int, default=64,
+                    help="client waves of the one-committer line (and the reference variants)")
+    ap.add_argument("--one-serial-waves", type=int, default=14,
+                    help="serialized-lane waves of the one-committer line")
     ap.add_argument("--client-wg-waves", type=int, default=1,
                     help="client waves per client workgroup (one per CU spreads the clients' "
                          "memory traffic over more CUs)")
@@ -478,7 +499,7 @@ def main():
                          "(0: six)")
     ap.add_argument("--stage-ahead", type=int, default=5,
                     help="requests the receivers stage ahead of the committer")
-    ap.add_argument("--router", default="transfers", choices=("none", "transfers", "all"),
+    ap.add_argument("--router", default="all", choices=("none", "transfers", "all"),
                     help="none: every transaction on the normal (client) lane; transfers: "
                          "every transfer on the serialized lane; all: audits too")
     ap.add_argument("--txns", type=int, default=1 << 24)
@@ -496,12 +517,16 @@ def main():
                     help="commit servers the serving loop's cache class is split over (by set)")
     ap.add_argument("--serve-budget", type=int, default=65536,
                     help="admission cap on in-flight version-pool bytes of the serving loop")
-    ap.add_argument("--serial-waves-all", type=int, default=16,
+    ap.add_argument("--serial-waves-all", type=int, default=64,
                     help="serialized-lane waves of the run with every transaction routed")
+    ap.add_argument("--no-one", action="store_true",
+                    help="skip the one-committer serialized-lane line")
     ap.add_argument("--escalate-after", type=int, default=2,
                     help="failed attempts after which the stamp_escalate variant line sends a "
                          "transaction on the serialized lane")
-    ap.add_argument("--no-route-all", action="store_true")
+    ap.add_argument("--no-route-all", action="store_true",
+                    help="skip the line with the other router (transfers only when the "
+                         "headline routes every transaction, every transaction otherwise)")
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--no-memcd", action="store_true")
     ap.add_argument("--no-sched", action="store_true")
@@ -528,12 +553,14 @@ def main():
     bank = csmv.Bank(n_txns=args.txns, seed=1, n_reads=4, n_reads_ro=4, pct_ro=10, zipf=True,
                      theta=0.99)
 
-    def make(variant, b, router, serial, escalate=0):
-        e = csmv.Engine(N_ACCOUNTS, depth=8, atr_entries=4096, client_waves=args.waves,
-                        wg_waves=args.wg_waves, variant=variant, timeout_ms=8000,
-                        n_serial_waves=serial if router != "none" else 0,
+    def make(variant, b, router, serial, escalate=0, waves=None):
+        pipe = variant == "stamp_pipe"
+        e = csmv.Engine(N_ACCOUNTS, depth=8, atr_entries=4096,
+                        client_waves=waves or (args.waves if pipe else args.one_waves),
+                        wg_waves=args.pipe_wg_waves if pipe else args.wg_waves, variant=variant,
+                        timeout_ms=8000, n_serial_waves=serial if router != "none" else 0,
                         stage_ahead=args.stage_ahead, client_wg_waves=args.client_wg_waves,
-                        escalate_after=escalate)
+                        escalate_after=escalate, commit_parts=args.commit_parts if pipe else 0)
         e.bank_generate(b)
         if router == "transfers":
             op, _ = e.bank_read_stream()
@@ -542,6 +569,10 @@ def main():
             e.bank_route(np.ones(b.n_txns, dtype=np.uint8))
         return e
 
+    pipe_h = args.variant == "stamp_pipe"
+    waves_h = args.waves if pipe_h else args.one_waves
+    wg_h = args.pipe_wg_waves if pipe_h else args.wg_waves
+    committers = ((wg_h - 1) // 2 if wg_h >= 5 else 1) * args.commit_parts if pipe_h else 1
     log(f"headline: {args.txns} txns, router {args.router}")
     e = make(args.variant, bank, args.router, args.serial_waves)
     for _ in range(args.warmup):
@@ -567,21 +598,17 @@ def main():
 
     alg_bytes = bytes_per_commit(runs[-1])
     achieved = alg_bytes / (runs[-1]["kernel_ms"] * 1e-3) / 1e9
-    traffic = load_traffic(runs[-1]["commits"])
+    traffic = load_traffic(runs[-1]["commits"], "csmv::bank_pipe_kernel<4, 2>" if pipe_h
+                           else "csmv::bank_kernel<4, 2>")
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic}
     # the per-lane protocol spends at least two same-address atomics per committed
     # update (ready bit + commit counter); its ceiling is the measured rate / 2
     plain_ceiling = atomic_rate / 2.0
-    route_all = {}
+    route_all, route_transfers = {}, {}
     if not args.no_route_all and args.router != "all":
         # the same stream with audits on the serialized lane too: no transaction aborts
-        ea = csmv.Engine(N_ACCOUNTS, depth=8, atr_entries=4096, client_waves=args.serial_waves_all + 1,
-                         wg_waves=args.wg_waves, variant=args.variant, timeout_ms=8000,
-                         n_serial_waves=args.serial_waves_all, stage_ahead=args.stage_ahead,
-                         client_wg_waves=args.client_wg_waves)
-        ea.bank_generate(bank)
-        ea.bank_route(np.ones(bank.n_txns, dtype=np.uint8))
+        ea = make(args.variant, bank, "all", args.serial_waves_all)
         ea.fill(INIT)
         ra = ea.bank_run()
         ea.close()
@@ -589,6 +616,33 @@ def main():
                      "kernel_ms": ra["kernel_ms"], "serial_lane_waves": args.serial_waves_all,
                      "commits_ro": ra["commits_ro"]}
         log(f"every transaction on the serialized lane: {ra['txn_per_s']:.4g} txn/s")
+    elif not args.no_route_all:
+        # the same stream with only the transfers on the serialized lane: audits run on the
+        # normal lane as multi-version snapshot reads and abort when a version they need has
+        # left its account's 8-deep ring
+        et = make(args.variant, bank, "transfers", args.serial_waves)
+        et.fill(INIT)
+        rt = et.bank_run()
+        et.close()
+        route_transfers = {"txn_per_s": rt["txn_per_s"], "abort_rate": rt["abort_rate"],
+                           "kernel_ms": rt["kernel_ms"], "serial_lane_waves": args.serial_waves,
+                           "aborts_dataage": rt["aborts_dataage"], "commits_ro": rt["commits_ro"]}
+        log(f"transfers on the serialized lane, audits on the normal lane: {rt['txn_per_s']:.4g} txn/s")
+    one = {}
+    if not args.no_one and args.variant == "stamp_pipe":
+        # the same stream on the serialized lane with one committer wave, whose clients install
+        # their own versions (one client-side install hand-off per request on a hot account)
+        eo = make("stamp", bank, args.router, args.one_serial_waves)
+        eo.fill(INIT)
+        eo.bank_run()
+        eo.fill(INIT)
+        ro1 = eo.bank_run()
+        eo.close()
+        one = {"txn_per_s": ro1["txn_per_s"], "abort_rate": ro1["abort_rate"],
+               "kernel_ms": ro1["kernel_ms"], "client_waves": args.one_waves,
+               "serial_lane_waves": args.one_serial_waves, "server_wg_waves": args.wg_waves,
+               "committer_busy_frac": ro1["prof"]["commit_busy"] / (ro1["kernel_ms"] * 1e6)}
+        log(f"one committer wave: {ro1['txn_per_s']:.4g} txn/s")
     variants = {}
     if not args.no_variants:
         vbank = csmv.Bank(n_txns=args.variant_txns, seed=1, n_reads=4, n_reads_ro=4, pct_ro=10,
@@ -631,24 +685,37 @@ def main():
                         "/ write 2, audits read 4, 90/10, 2^24 txns, version depth 8",
             "txns_per_step": args.txns,
             "variant": args.variant,
+            "commit_path": {
+                ("stamp_pipe", "all"): "pipelined serialized lane for every transaction: clients "
+                    "stage their requests' producer chains; committer waves apply each account's "
+                    "request-wide delta with one ordered 64-bit atomic, whose return gives every "
+                    "read its value in the serial order, and install the versions",
+                ("stamp_pipe", "transfers"): "pipelined serialized lane for every transfer; "
+                    "audits on the normal lane as multi-version snapshot reads",
+            }.get((args.variant, args.router), args.variant),
             "router": args.router,
-            "client_waves": args.waves,
+            "client_waves": waves_h,
             "serial_lane_waves": args.serial_waves if args.router != "none" else 0,
-            "server_wg_waves": args.wg_waves,
+            "server_wg_waves": wg_h,
+            "commit_parts": args.commit_parts if pipe_h else 1,
+            "committer_waves": committers,
             "client_wg_waves": args.client_wg_waves,
             "stage_ahead": args.stage_ahead,
-            "client_lanes": args.waves * 64,
+            "client_lanes": waves_h * 64,
             "record_ring": 4096,
         },
         "aborts_by_class": {k: sum(r[k] for r in runs) for k in
                             ("aborts_dataage", "aborts_readwrite", "aborts_prevalid",
                              "aborts_recordage", "aborts_unknown")},
-        "committer_busy_frac": sum(r["prof"]["commit_busy"] for r in runs) / (kernel_s * 1e9),
+        # busy time of the committer waves over the kernel time, per committer wave
+        "committer_busy_frac": sum(r["prof"]["commit_busy"] for r in runs) / (kernel_s * 1e9) / committers,
         "abort_rate": aborts / (aborts + commits),
         "wall_s_timed_steps": wall,
         "roofline": roofline,
         "atomics": atomics,
         "every_transaction_serialized": route_all,
+        "audits_on_normal_lane": route_transfers,
+        "one_committer_serialized_lane": one,
         "variants": variants,
     }
     if not args.no_plain:
