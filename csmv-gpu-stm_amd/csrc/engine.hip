@@ -32,6 +32,12 @@ constexpr int kBankR = 4;
 constexpr int kBankW = 2;
 // serving loop: bank transactions read 2..64 accounts
 constexpr int kServeBankR = 64;
+// reply values per client wave: the serialized lane's rows (a new value and a previous
+// head per write and lane) or the pipelined lane's entry values of every partition
+constexpr uint32_t kReplyVWords = std::max<uint32_t>(2 * kBankW * kWave, kPipeParts * kPipeMaxE);
+// the pipelined lane's boxes (one per partition) fit a client wave's request mailbox
+static_assert(mbox_rows<kServeBankR, kBankW>() * kWave >= kPipeParts * kPipeBox,
+              "pipelined boxes exceed the request mailbox");
 
 // ---- stream generation ----------------------------------------------------------------------
 __host__ __device__ __forceinline__ uint64_t splitmix64(uint64_t x)
@@ -139,20 +145,17 @@ __global__ void __launch_bounds__(256) pipe_init_kernel(const uint32_t *ht, cons
     sv[i] = ((uint64_t)v << 32) | (t == 0 ? h : h + depth);
 }
 
-// Pipelined serialized lane: an audit it ran logged the value of each read (each by the
-// commit partition of its item); row 0 gets their sum, the audit's logged result.
-__global__ void __launch_bounds__(256) pipe_audit_sum_kernel(const uint32_t *op, const uint32_t *hot,
-                                                             uint64_t n_hot, uint64_t T, int32_t *log_v)
+// Pipelined serialized lane run with no normal-lane transaction (no snapshot reader during
+// the run): the commit servers install the versions but leave the head words, which the
+// install counts give at the end (j installs: head j mod depth, tail 0 until the ring is
+// full, then head + 1).
+__global__ void __launch_bounds__(256) pipe_heads_kernel(const uint64_t *sv, uint32_t *ht, uint32_t n,
+                                                         uint32_t depth)
 {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_hot) return;
-    const uint32_t k = hot[i];
-    const uint32_t o = op[k];
-    if ((o & 0xFFu) != kKindAudit) return;
-    const uint32_t nr = (o >> 16) & 0xFFu;
-    uint32_t sum = 0;
-    for (uint32_t r = 0; r < nr; ++r) sum += (uint32_t)log_v[(uint64_t)r * T + k];
-    log_v[k] = (int32_t)sum;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t j = (uint32_t)sv[i];
+    ht[i] = (j % depth) | ((j < depth ? 0u : (j + 1) % depth) << 16);
 }
 
 __global__ void __launch_bounds__(256) pool_head_kernel(const uint32_t *ht, const uint64_t *slot,
@@ -704,6 +707,7 @@ csmv_status normalize_config(csmv_config &cfg)
     if (cfg.n_serial_waves >= cfg.client_waves && cfg.n_serial_waves) return CSMV_E_ARG;
     if (cfg.n_items == 0 || cfg.depth == 0 || cfg.depth > 64 || cfg.client_waves == 0 ||
         cfg.wg_waves < 3 || cfg.wg_waves > ((cfg.variant & kSerialPipe) ? 12u : 8u) ||
+        ((cfg.variant & kSerialPipe) && cfg.client_waves > kPipeCwMask + 1) ||
         cfg.client_wg_waves > cfg.wg_waves ||
         cfg.stage_ahead > 64 || cfg.atr_entries < 64 ||
         (cfg.atr_entries & (cfg.atr_entries - 1)) || cfg.atr_entries > (1u << 16) ||
@@ -754,7 +758,7 @@ ServerLayout server_layout_at(const csmv_config &cfg, uint32_t lds_max, uint32_t
         // words per chain node, and the install records the publisher takes.  The clients'
         // staging scratch lives in their own workgroups' LDS, the same size.
         L.nmb = nmb ? nmb : 16;
-        L.mb_rows = mailbox_rows(R, W, true);
+        L.mb_rows = kPipeBox / kWave;
         L.state_off = kLdsHdr;
         L.wave_off = L.state_off + L.nmb;
         L.tg_off = L.wave_off + L.nmb;
@@ -764,9 +768,9 @@ ServerLayout server_layout_at(const csmv_config &cfg, uint32_t lds_max, uint32_t
         L.harr = 0;
         L.hash_off = L.sup_off = w;
         L.jmp_off = w;
-        w += pipe_committers(cfg.wg_waves) * 2 * kWave * R;
+        w += pipe_committers(cfg.wg_waves) * kPipeMaxE;
         L.pipe_off = w;
-        w += pipe_words(W);
+        w += pipe_words();
         L.atr_off = L.bucket_off = L.val_off = w;
         L.words = std::max<uint32_t>(w, cfg.client_wg_waves * pipe_client_words(L.hsz, R, W));
         L.parts = cfg.commit_parts;
@@ -943,7 +947,7 @@ csmv_status csmv_create(const csmv_config *cfg_in, csmv_engine **out)
         (s = dalloc(e, &e->mbox, (size_t)cfg.client_waves * rows * kWave)) ||
         (s = dalloc(e, &e->sv, cfg.n_items)) || (s = dalloc(e, &e->tmp, cfg.n_items)) ||
         (s = dalloc(e, &e->sink, kSinkWords)) ||
-        (s = dalloc(e, &e->reply_v, (size_t)cfg.client_waves * kWave * 2 * kBankW))) {
+        (s = dalloc(e, &e->reply_v, (size_t)cfg.client_waves * kReplyVWords))) {
         csmv_destroy(e);
         return s;
     }
@@ -1209,7 +1213,7 @@ static csmv_status reset_run_state(csmv_engine *e, const ServerLayout &L)
     HIPCHK(e, hipMemsetAsync(e->ring, 0, e->Q * 8, e->stream));
     HIPCHK(e, hipMemsetAsync(e->done, 0, e->Q * 8, e->stream));
     HIPCHK(e, hipMemsetAsync(e->reply, 0, (size_t)e->cfg.client_waves * kWave * 8, e->stream));
-    HIPCHK(e, hipMemsetAsync(e->reply_v, 0, (size_t)e->cfg.client_waves * kWave * 2 * kBankW * 8,
+    HIPCHK(e, hipMemsetAsync(e->reply_v, 0, (size_t)e->cfg.client_waves * kReplyVWords * 8,
                              e->stream));
     const uint32_t n = e->cfg.n_items;
     if (e->cfg.variant & kSerialPipe) {
@@ -1320,13 +1324,15 @@ csmv_status csmv_bank_run(csmv_engine *e, csmv_stats *out)
         return fail(e, CSMV_E_ARG, "the pipelined serialized lane needs routed transfers and "
                                    "n_serial_waves > 0");
     csmv_status st;
-    if (e->cfg.variant & kSerialPipe) {
-        // the audits the serialized lane ran logged their reads; their sums go to row 0
-        const uint64_t nh = c.n_hot;
+    if ((e->cfg.variant & kSerialPipe) && c.n_norm == 0) {
+        // no snapshot reader during the run: the head words follow from the install counts
+        const uint32_t n = e->cfg.n_items;
         st = launch_persistent(e, bank_pipe_kernel<kBankR, kBankW>, c, L, out, [&]() {
-            hipLaunchKernelGGL(pipe_audit_sum_kernel, dim3((unsigned)((nh + 255) / 256)), dim3(256), 0,
-                               e->stream, e->op, e->hot_list, nh, e->spec.n_txns, e->log_v);
+            hipLaunchKernelGGL(pipe_heads_kernel, dim3((n + 255) / 256), dim3(256), 0, e->stream,
+                               e->sv, e->ht, n, e->cfg.depth);
         });
+    } else if (e->cfg.variant & kSerialPipe) {
+        st = launch_persistent(e, bank_pipe_kernel<kBankR, kBankW>, c, L, out);
     } else {
         st = launch_persistent(e, bank_kernel<kBankR, kBankW>, c, L, out);
     }

@@ -202,6 +202,10 @@ template <typename T> __device__ __forceinline__ T fetch_add(T *p, T v)
     return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void fence_acq() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
+// Waits until every memory operation the wave has issued is complete: its agent-scope
+// stores are then at the device's point of coherence (the wait an agent-scope release
+// makes, without the release's write-back of the L2's dirty lines).
+__device__ __forceinline__ void wait_stores() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 __device__ __forceinline__ void fence_rel() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
 
 // ---- commit-path variant bits (csmv.h CSMV_VAL_* / CSMV_INSERT_* / CSMV_WB_*) -----------

@@ -94,18 +94,40 @@ __device__ __forceinline__ uint32_t pipe_part(uint32_t x, uint32_t parts)
     return parts == 1 ? 0u : (mix32(x) >> 8) % parts;
 }
 
-// Pipelined serialized lane: install records of this many requests wait for the publisher.
+// Pipelined serialized lane wire.  A request's chain nodes (4 reads x 64 lanes) are split
+// by the partition of their item into one list per commit partition: per client wave and
+// partition a box of kPipeBox words in HBM, word 0 the header (entries | update lanes <<
+// 16), then the entries' items, weights and meta words, kPipeMaxE of each.  A meta word
+// holds the entry's root entry (the entry whose atomic returns the value before the
+// request: itself, or the canonical node of a written item), the root flag (the entry
+// issues its item's atomic), the install flag (the canonical node of a written item: it
+// installs the item's last write) and that last writer's rank among the update lanes.
+// A partition replies with a value per entry (the value the node reads) and, once those
+// stores are complete, a header word per client wave (tag | the request's first timestamp).
+// The ring word of a request carries the client wave (bits 0-11) and, per partition, its
+// entries in 32-entry chunks (4 bits each from bit 12), so a receiver loads what the list
+// holds in one round.
+constexpr uint32_t kPipeParts = 4;                       // commit partitions at most
+constexpr uint32_t kPipeMaxE = 4 * kWave;                // entries of one request
+constexpr uint32_t kPipeItem = kWave, kPipeWgt = kPipeItem + kPipeMaxE, kPipeMeta = kPipeWgt + kPipeMaxE;
+constexpr uint32_t kPipeBox = kPipeMeta + kPipeMaxE;    // words per (client wave, partition)
+constexpr uint32_t kPipeRootBit = 1u << 8, kPipeInstBit = 1u << 9, kPipeRankShift = 10;
+constexpr uint32_t kPipeCwMask = 0xFFFu, kPipeChunkShift = 12, kPipeChunk = 32;
+// Install records of this many requests wait for the publisher: per request the item and
+// head word of each entry, the entry count, the timestamp after the request and the ready
+// word.
 constexpr uint32_t kPipeRing = 16;
-__host__ __device__ constexpr uint32_t pipe_words(uint32_t W) { return kPipeRing * (2 * W * kWave + 2); }
+__host__ __device__ constexpr uint32_t pipe_words() { return kPipeRing * (2 * kPipeMaxE + 3); }
 // Committer waves of a pipelined server workgroup of wg waves (one more publishes, the
 // rest receive).
 __host__ __device__ constexpr uint32_t pipe_committers(uint32_t wg) { return wg >= 5 ? (wg - 1) / 2 : 1; }
 // A client wave's LDS scratch for staging a pipelined request: the write hash (key, writer
 // lanes lo / hi, canonical node, request-wide delta), then the chain arrays (next, weight
-// per node; superseded flag per write slot).
+// per node; superseded flag per write slot), the last writers' ranks (by hash slot), each
+// node's entry, and the staged entry lists.
 __host__ __device__ constexpr uint32_t pipe_client_words(uint32_t hsz, uint32_t R, uint32_t W)
 {
-    return 5 * hsz + 2 * kWave * R + kWave * W;
+    return 6 * hsz + 3 * kWave * R + kWave * W + 3 * kPipeMaxE;
 }
 
 // Row indices inside an LDS mailbox.  For a delegated request the wv rows carry the
@@ -1569,11 +1591,12 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
 // commit (bank/csmv_sh/API.cu:88-272) and its clients write back their own versions.
 
 // Client side (TXCommit for a wave of routed transfers, bank/csmv_sh/exe.cu:184-201 as the
-// body): stage the request's producer chains in the wave's LDS scratch, write the rows the
-// server stages (Rows<R, W>) to the wave's mailbox, send, and wait for the reply word
-// (commit timestamp, or kSnapshotBit | snapshot for a read-only lane).  The server installs
-// the versions, so the client has nothing more to do.  scr: pipe_client_words(L.hsz, R, W)
-// words of this wave's LDS, cleared by pipe_client_init.
+// body): stage the request's producer chains in the wave's LDS scratch, write each
+// partition's entry list to the wave's boxes, send, and wait for the replies: the values
+// the nodes read, from which the wave runs its transactions and logs them (commit
+// timestamp, or kSnapshotBit | snapshot for a read-only lane, returned).  The server
+// installs the versions.  scr: pipe_client_words(L.hsz, R, W) words of this wave's LDS,
+// cleared by pipe_client_init.
 template <int R, int W>
 __device__ __forceinline__ void pipe_client_init(const ServerLayout &L, uint32_t *scr)
 {
@@ -1593,8 +1616,8 @@ __device__ __forceinline__ uint32_t tx_commit_pipe(const Ctx &c, const ServerLay
                                                    uint32_t cw, const Tx<R, W> &tx, bool req,
                                                    bool *fail, uint64_t *wait_ticks)
 {
-    static_assert(!kWideWire<R> && W == 2, "the pipelined lane carries bank transfers");
-    using RW = Rows<R, W>;
+    static_assert(!kWideWire<R> && W == 2 && R * kWave <= kPipeMaxE,
+                  "the pipelined lane carries bank transfers");
     constexpr int RO = R;
     const uint32_t lane = lane_id();
     const uint64_t lt = (1ull << lane) - 1;
@@ -1604,6 +1627,11 @@ __device__ __forceinline__ uint32_t tx_commit_pipe(const Ctx &c, const ServerLay
     uint32_t *snext = scr + 5 * hsz;
     int32_t *sw = (int32_t *)(snext + kWave * RO);
     uint32_t *ssup = snext + 2 * kWave * RO;
+    uint32_t *hrank = ssup + kWave * W;                 // [hsz] last writer's update rank
+    uint32_t *e2 = hrank + hsz;                         // [64 * RO] each node's entry
+    uint32_t *sitem = e2 + kWave * RO;                  // [kPipeMaxE] the lists, back to back
+    uint32_t *swgt = sitem + kPipeMaxE;
+    uint32_t *smeta = swgt + kPipeMaxE;
     const uint32_t mybit = 1u << (lane & 31);
 
     uint32_t ops[RO], wa[W];
@@ -1756,46 +1784,101 @@ __device__ __forceinline__ uint32_t tx_commit_pipe(const Ctx &c, const ServerLay
 #pragma unroll
     for (int w = 0; w < W; ++w) supf |= ssup[lane * W + w] ? (1u << w) : 0u;
 
-    // 5. the item's request-wide delta, from its last (unsuperseded) write: node w reads
-    //    write slot w's item (a transfer reads what it writes), so that write's value is
-    //    the value before the request + wgt[w] + the slot's delta
+    // 5. the item's request-wide delta and its last writer's rank among the request's
+    //    update lanes, from its last (unsuperseded) write: node w reads write slot w's item
+    //    (a transfer reads what it writes), so that write's value is the value before the
+    //    request + wgt[w] + the slot's delta
+    const bool ro = amt == kReadOnlyAmt;
+    const uint64_t umask = ballot(req && !ro);
+    const uint32_t rank = (uint32_t)__builtin_popcountll(umask & lt);
 #pragma unroll
     for (int w = 0; w < W; ++w)
-        if (wa[w] != kNone && ((supf >> w) & 1u) == 0)
+        if (wa[w] != kNone && ((supf >> w) & 1u) == 0) {
             htot[slot_w[w]] = wgt[w] + (w == 0 ? d0 : d1);
+            hrank[slot_w[w]] = rank;
+        }
     wave_fence();
-    uint32_t root_out[RO];
-    int32_t wgt_out[RO];
+
+    // 6. the entries: each node of an item of partition p becomes entry e of p's list (the
+    //    partition's nodes in node order), with its root entry, weight and flags; the lists
+    //    are staged back to back in LDS, then written to the boxes a row at a time
+    const uint32_t P = L.parts;
+    uint32_t pr[RO], e[RO];
+#pragma unroll
+    for (int r = 0; r < RO; ++r) {
+        pr[r] = ops[r] != kNone ? pipe_part(ops[r], P) : kNone;
+        e[r] = 0;
+    }
+    uint32_t cntq[kPipeParts], offq[kPipeParts];
+    uint32_t off = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < kPipeParts; ++q) {
+        cntq[q] = 0;
+        offq[q] = off;
+        if (q < P) {
+            uint32_t low = 0, tot = 0;
+#pragma unroll
+            for (int r = 0; r < RO; ++r) {
+                const uint64_t mq = ballot(pr[r] == q);
+                low += (uint32_t)__builtin_popcountll(mq & lt);
+                tot += (uint32_t)__builtin_popcountll(mq);
+            }
+            uint32_t own = 0;
+#pragma unroll
+            for (int r = 0; r < RO; ++r)
+                if (pr[r] == q) e[r] = low + own++;
+            cntq[q] = tot;
+            off += tot;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < RO; ++r) e2[lane * RO + r] = e[r];
+    wave_fence();
+    uint32_t *bx = c.mbox + (size_t)cw * kPipeParts * kPipeBox;
 #pragma unroll
     for (int r = 0; r < RO; ++r) {
         const uint32_t node = lane * RO + r;
         const bool written = fs[r] != kNone;
         const uint32_t at = written ? fs[r] : 0u;
         const uint32_t cn = canon[at];
-        const int32_t tt = htot[at];
         // every chain root of a written item is a node without a producer, and all of
         // them see the value before the request: they all take the canonical node's
         const uint32_t rt = written ? cn : nxt[r];
         const bool is_canon = written && cn == node;
-        root_out[r] = is_canon ? (node | 0x80000000u) : rt;
-        wgt_out[r] = is_canon ? tt : wgt[r];
+        const bool is_root = rt == node;
+        const uint32_t meta = e2[rt & (kWave * RO - 1)] | (is_root ? kPipeRootBit : 0u) |
+                              (is_canon ? kPipeInstBit | (hrank[at] << kPipeRankShift) : 0u);
+        const int32_t wout = is_canon ? htot[at] : wgt[r];
+        if (pr[r] != kNone) {
+            uint32_t o = offq[0];
+#pragma unroll
+            for (uint32_t q = 1; q < kPipeParts; ++q) o = pr[r] == q ? offq[q] : o;
+            sitem[o + e[r]] = ops[r];
+            swgt[o + e[r]] = (uint32_t)wout;
+            smeta[o + e[r]] = meta;
+        }
     }
-
-    // 6. the staged rows into the wave's mailbox, then the hash slots back to empty
-    uint32_t *mb = c.mbox + (size_t)cw * RW::count * kWave;
-    st_rlx(&mb[RW::snap * kWave + lane], req ? 0u : kNone);
+    wave_fence();
+    const uint32_t nc = (uint32_t)__builtin_popcountll(umask);
+    uint32_t chunks = 0;        // for the ring word
 #pragma unroll
-    for (int r = 0; r < R; ++r) st_rlx(&mb[(RW::items + r) * kWave + lane], ops[r]);
+    for (uint32_t q = 0; q < kPipeParts; ++q)
+        chunks |= ((cntq[q] + kPipeChunk - 1) / kPipeChunk) << (kPipeChunkShift + 4 * q);
 #pragma unroll
-    for (int w = 0; w < W; ++w) st_rlx(&mb[(RW::items + R + w) * kWave + lane], wa[w]);
-    st_rlx(&mb[RW::wv * kWave + lane], k);
-    st_rlx(&mb[(RW::wv + 1) * kWave + lane], amt);
+    for (uint32_t q = 0; q < kPipeParts; ++q) {
+        if (q >= P) break;
+        uint32_t *b = bx + q * kPipeBox;
 #pragma unroll
-    for (int r = 0; r < RO; ++r) {
-        st_rlx(&mb[(RW::root + r) * kWave + lane], root_out[r]);
-        st_rlx(&mb[(RW::wgt + r) * kWave + lane], (uint32_t)wgt_out[r]);
+        for (uint32_t g = 0; g < kPipeMaxE / kWave; ++g) {
+            const uint32_t x = g * kWave + lane;
+            if (x < cntq[q]) {
+                st_rlx(&b[kPipeItem + x], sitem[offq[q] + x]);
+                st_rlx(&b[kPipeWgt + x], swgt[offq[q] + x]);
+                st_rlx(&b[kPipeMeta + x], smeta[offq[q] + x]);
+            }
+        }
+        if (lane == 0) st_rlx(&b[0], cntq[q] | (nc << 16));
     }
-    st_rlx(&mb[RW::supf * kWave + lane], supf);
     wave_fence();
 #pragma unroll
     for (int w = 0; w < W; ++w)
@@ -1808,40 +1891,60 @@ __device__ __forceinline__ uint32_t tx_commit_pipe(const Ctx &c, const ServerLay
         }
     wave_fence();
 
-    // 7. send; wait for the reply words (lane 0's first, then every lane's tag)
-    fence_rel();
+    // 7. send; wait for every partition's header word (its tag, and the request's first
+    //    timestamp), then read the value of each of the lane's entries.  The
+    //    box words are agent-scope stores, written through to the device's point of
+    //    coherence, so their completion orders them before the ring word; a release fence
+    //    would also write back every dirty line of this XCD's L2 (commit servers' value
+    //    words among them) once per request.
+    wait_stores();
     uint64_t n = 0;
     if (lane == 0) {
         n = fetch_add(c.req_tail, (uint64_t)1);
-        st_rel(&c.ring[n & c.qmask], ((n + 1) << 32) | cw | kDelegateBit);
+        st_rlx(&c.ring[n & c.qmask], ((n + 1) << 32) | cw | chunks | kDelegateBit);
     }
     n = bcast64(n);
     const uint64_t tw0 = wall_clock64();
     Spin sp;
-    // one reply word per lane from every commit partition (partition 0's in reply, the
-    // others' in the reply-value rows): the mailbox is reused only once every partition has
-    // taken the request
-    const uint64_t *rp = c.reply + (size_t)cw * kWave;
-    for (uint32_t pt = 0; pt < L.parts; ++pt) {
-        const uint64_t *rq = pt == 0 ? rp : c.reply_v + ((size_t)cw * 2 * W + pt) * kWave;
+    const uint64_t *hdr = c.reply + (size_t)cw * kWave;
+    uint64_t h0 = 0;
+    for (uint32_t pt = 0; pt < P; ++pt) {
         uint64_t w0 = 0;
         for (;;) {
-            if (lane == 0) w0 = ld_rlx(rq);
-            if ((bcast64(w0) >> 32) == n + 1) break;
+            if (lane == 0) w0 = ld_rlx(&hdr[pt]);
+            w0 = bcast64(w0);
+            if ((w0 >> 32) == n + 1) break;
             if (!sp.tick(c, kErrReplyTimeout)) { *fail = true; return 0; }
         }
+        h0 = pt == 0 ? w0 : h0;
     }
-    uint64_t v;
-    for (;;) {
-        v = ld_rlx(&rp[lane]);
-        bool stale = (v >> 32) != n + 1;
-        for (uint32_t pt = 1; pt < L.parts; ++pt)
-            stale |= (ld_rlx(&c.reply_v[((size_t)cw * 2 * W + pt) * kWave + lane]) >> 32) != n + 1;
-        if (ballot(stale) == 0) break;
-        if (!sp.tick(c, kErrReplyTimeout)) { *fail = true; return 0; }
+    // (a partition stores its header once its values are complete)
+    const uint32_t *rv = (const uint32_t *)c.reply_v + (size_t)cw * kPipeParts * kPipeMaxE;
+    int32_t in[RO];
+#pragma unroll
+    for (int r = 0; r < RO; ++r) {
+        in[r] = 0;
+        if (pr[r] != kNone) in[r] = (int32_t)ld_rlx(&rv[pr[r] * kPipeMaxE + e[r]]);
     }
     *wait_ticks += wall_clock64() - tw0;
-    return req ? (uint32_t)v : 0u;
+
+    // 8. the transactions on the values before them, in commit order; the log: a transfer's
+    //    timestamp and every read, an audit's snapshot and the sum of its reads (row 0)
+    int32_t rd[RO], nv[W];
+    transfer_exec<RO, W>(ops, amt, in, rd, wa, nv, !ro);
+    const uint32_t below = (uint32_t)h0 + rank;
+    const uint32_t myts = ro ? (kSnapshotBit | below) : below + 1;
+    if (req) {
+        uint32_t sum = 0;
+#pragma unroll
+        for (int r = 0; r < RO; ++r) sum += (uint32_t)rd[r];
+        c.log_ts[k] = ro ? below : myts;
+        c.log_v[k] = ro ? (int32_t)sum : rd[0];
+#pragma unroll
+        for (int r = 1; r < RO; ++r)
+            if (!ro && ops[r] != kNone) c.log_v[(uint64_t)r * c.n_txns + k] = rd[r];
+    }
+    return req ? myts : 0u;
 }
 
 // Global-memory accesses through an address chosen by a select (an active lane's target or
@@ -1883,47 +1986,58 @@ __device__ __forceinline__ uint32_t pipe_snapshot(const Ctx &c)
 
 // The pipelined committers: K waves of a server workgroup, wave k taking requests
 // n = k, k + K, ...  With several server workgroups (commit partitions) each one runs every
-// request for the items hashed to it (pipe_part): a transfer changes each item by its own
+// request on the entries of its items (pipe_part): a transfer changes each item by its own
 // delta, so an item's chain, value, installs and head words never need another partition.
 // The committers' only serial step is the issue of a request's atomics, passed on by an LDS
 // token: request n's atomics are issued once request n-1's are (the token's workgroup
 // release and acquire order the two waves' read-modify-writes on every item they share),
 // and the token carries the request's first timestamp.  The rest runs in parallel across
-// the waves: wait for the words, execute the transfers (read values, timestamps, installs
-// of the last write of each item, log, reply) and hand the head words to the publisher
-// once the stores are complete.  Idle lanes issue nothing (their instructions are masked).
+// the waves: wait for the words, give each entry its value (its root's word before the
+// request plus the entry's weight), install the last write of each written item, reply
+// with the values, and hand the head words to the publisher once the stores are complete.
+// Lane i takes entries i, i + 64, ...; idle lanes issue nothing (their instructions are
+// masked).
 template <int R, int W>
 __device__ __forceinline__ void committer_pipe(const Ctx &c, const ServerLayout &L, uint32_t *lds,
                                                uint32_t k, uint32_t K)
 {
-    static_assert(!kWideWire<R> && W == 2, "the pipelined lane carries bank transfers");
-    using RW = Rows<R, W>;
-    constexpr int RO = R;
+    constexpr int G = kPipeMaxE / kWave;
     const uint32_t lane = lane_id();
-    const uint64_t lt = (1ull << lane) - 1;
     const uint32_t D = c.depth;
     const bool dpow2 = (D & (D - 1)) == 0;
-    const uint32_t part = blockIdx.x, P = L.parts;
+    const uint32_t part = blockIdx.x;
     uint32_t *mbst = lds + L.state_off;
     uint32_t *mbwave = lds + L.wave_off;
     uint32_t *mbfree = lds + L.tg_off;                      // [nmb] 1 + request finished in it
-    int32_t *jbase = (int32_t *)(lds + L.jmp_off + k * 2 * kWave * RO);   // [64 * RO]
-    uint32_t *jcnt = (uint32_t *)jbase + kWave * RO;                      // [64 * RO]
-    uint32_t *irec = lds + L.pipe_off;                      // [kPipeRing][2W][64]
-    uint32_t *ilast = irec + kPipeRing * 2 * W * kWave;     // [kPipeRing]
+    int32_t *jv = (int32_t *)(lds + L.jmp_off + k * kPipeMaxE);   // [kPipeMaxE] words before
+    uint32_t *irec = lds + L.pipe_off;                      // [kPipeRing][2][kPipeMaxE]
+    uint32_t *icnt = irec + kPipeRing * 2 * kPipeMaxE;      // [kPipeRing] entries
+    uint32_t *ilast = icnt + kPipeRing;                     // [kPipeRing]
     uint32_t *iready = ilast + kPipeRing;                   // [kPipeRing]
     uint64_t *const tab = c.sv;
     // a word per committer for its completion probe
     uint64_t *const probe_at = c.sink + (size_t)(part * 16 + k) * 16;
-    const uint64_t T = c.n_txns;
     auto mod_d = [&](uint32_t j) -> uint32_t { return dpow2 ? (j & (D - 1)) : j % D; };
     LaneSum t_idle, t_busy, t_tok, t_atom, t_fin, t_room, t_probe;
     bool held = false;                          // a finished request not yet handed over
-    uint32_t held_n = 0, held_qi = 0, held_last = 0;
+    uint32_t held_n = 0, held_qi = 0, held_last = 0, held_cw = 0, held_base = 0;
     uint64_t tail_probe = 0;                    // issued after its stores
+    // a finished request's stores (versions, reply values) are complete once the probe
+    // load issued after them has returned (one wave's memory operations complete in issue
+    // order): then its reply header goes out and its installs go to the publisher
+    auto hand_over = [&]() {
+        wait_word(bcast64(tail_probe));
+        if (lane == 0) {
+            st_rlx(&c.reply[(size_t)held_cw * kWave + part], ((uint64_t)(held_n + 1) << 32) | held_base);
+            ilast[held_qi] = held_last;
+            lds_st_rel(&iready[held_qi], held_n + 1);
+        }
+        held = false;
+    };
     uint32_t n = k;
     for (;; n += K) {
         const uint32_t m = n % L.nmb;
+        if (held) hand_over();
         const uint64_t t0 = role_clock();
         // request n staged, or the end of the stream
         bool end = false, bad = false;
@@ -1951,52 +2065,35 @@ __device__ __forceinline__ void committer_pipe(const Ctx &c, const ServerLayout 
         }
         const uint64_t t1 = role_clock();
         const uint32_t *box = lds + L.mb_off + m * L.mb_rows * kWave;
-        const bool valid = box[RW::snap * kWave + lane] != kNone;
-        const uint32_t ktx = box[RW::wv * kWave + lane];
-        const uint32_t amt = box[(RW::wv + 1) * kWave + lane];
-        const bool ro = amt == kReadOnlyAmt;
-        const uint32_t supf = box[RW::supf * kWave + lane];
-        uint32_t ops[RO], rt[RO], wa[W];
-        int32_t wg[RO];
-        bool mine[RO];
+        const uint32_t hdr = box[0];
+        const uint32_t cnt = hdr & 0x1FFu, nc = hdr >> 16;
+        uint32_t item[G], meta[G];
+        int32_t wgt[G];
+        bool act[G], at[G];
 #pragma unroll
-        for (int r = 0; r < RO; ++r) {
-            ops[r] = box[(RW::items + r) * kWave + lane];
-            rt[r] = box[(RW::root + r) * kWave + lane];
-            wg[r] = (int32_t)box[(RW::wgt + r) * kWave + lane];
+        for (int g = 0; g < G; ++g) {
+            const uint32_t e = g * kWave + lane;
+            act[g] = e < cnt;
+            item[g] = box[kPipeItem + e];
+            wgt[g] = (int32_t)box[kPipeWgt + e];
+            meta[g] = box[kPipeMeta + e];
         }
-#pragma unroll
-        for (int w = 0; w < W; ++w) wa[w] = box[(RW::items + R + w) * kWave + lane];
         // the mailbox is free again once its rows are in registers
         if (lane == 0) lds_st_rel(&mbfree[m], n + 1);
-        // request n's atomics after request n-1's: each canonical node of this partition's
-        // items adds its item's request-wide delta (and one install when the item is
-        // written) and gets the word before the request.  The addresses and addends are
-        // ready before the token arrives, and the token carries the request's first
-        // timestamp, so the hand-off is one LDS load and one LDS store.
-        bool at[RO];
-        uint64_t addr[RO], add[RO];
+        // request n's atomics after request n-1's: each root entry adds its item's
+        // request-wide delta (and one install when the item is written) and gets the word
+        // before the request.  The addresses and addends are ready before the token
+        // arrives, and the token carries the request's first timestamp, so the hand-off is
+        // one LDS load and one LDS store.
+        uint64_t addr[G], add[G];
 #pragma unroll
-        for (int r = 0; r < RO; ++r) {
-            const uint32_t node = lane * RO + r;
-            mine[r] = valid && ops[r] != kNone && pipe_part(ops[r], P) == part;
-            at[r] = mine[r] && (rt[r] & 0xFFFFu) == node;
-            addr[r] = (uint64_t)(uintptr_t)(tab + (at[r] ? ops[r] : 0u));
-            add[r] = ((uint64_t)(uint32_t)wg[r] << 32) | (rt[r] >> 31);
+        for (int g = 0; g < G; ++g) {
+            at[g] = act[g] && (meta[g] & kPipeRootBit) != 0;
+            addr[g] = (uint64_t)(uintptr_t)(tab + (at[g] ? item[g] : 0u));
+            add[g] = ((uint64_t)(uint32_t)wgt[g] << 32) | ((meta[g] & kPipeInstBit) ? 1u : 0u);
         }
-        const uint64_t committed = ballot(valid && !ro);
-        const uint32_t nc = (uint32_t)__builtin_popcountll(committed);
         pin_round(addr);
         pin_round(add);
-        // the previous request's stores are complete once the probe load issued after them
-        // has returned (one wave's memory operations complete in issue order): its versions
-        // go to the publisher
-        wait_word(bcast64(tail_probe));
-        if (held && lane == 0) {
-            ilast[held_qi] = held_last;
-            lds_st_rel(&iready[held_qi], held_n + 1);
-        }
-        held = false;
         const uint64_t tp0 = phase_clock();
         uint32_t base = 0;
         {
@@ -2012,37 +2109,27 @@ __device__ __forceinline__ void committer_pipe(const Ctx &c, const ServerLayout 
             }
         }
         if (bad) break;
-        uint64_t ret[RO];
+        uint64_t ret[G];
 #pragma unroll
-        for (int r = 0; r < RO; ++r)
-            if (at[r]) ret[r] = g_fetch_add((uintptr_t)addr[r], add[r]);
+        for (int g = 0; g < G; ++g) {
+            ret[g] = 0;
+            if (at[g]) ret[g] = g_fetch_add((uintptr_t)addr[g], add[g]);
+        }
         if (lane == 0)
             __hip_atomic_store((uint64_t *)&lds[kLdsTok], ((uint64_t)(base + nc) << 32) | (n + 1),
                                __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
         const uint64_t tp1 = phase_clock();
-        // the words before the request, to every node of their items
+        // the words before the request, to every entry of their items
 #pragma unroll
-        for (int r = 0; r < RO; ++r)
-            if (at[r]) {
-                jbase[lane * RO + r] = (int32_t)(uint32_t)(ret[r] >> 32);
-                jcnt[lane * RO + r] = (uint32_t)ret[r];
-            }
+        for (int g = 0; g < G; ++g)
+            if (at[g]) jv[g * kWave + lane] = (int32_t)(uint32_t)(ret[g] >> 32);
         wave_fence();
         const uint64_t tp2 = phase_clock();
-        int32_t in[RO];
-        uint32_t cnt[RO];
+        int32_t val[G];
 #pragma unroll
-        for (int r = 0; r < RO; ++r) {
-            const uint32_t rr = rt[r] & (kWave * RO - 1);
-            const bool cn = rr == lane * RO + (uint32_t)r;
-            in[r] = jbase[rr] + (cn ? 0 : wg[r]);
-            cnt[r] = jcnt[rr];
-        }
+        for (int g = 0; g < G; ++g)
+            val[g] = jv[act[g] ? (meta[g] & 0xFFu) : 0u] + (at[g] ? 0 : wgt[g]);
         wave_fence();
-        int32_t rd[RO], nv[W];
-        transfer_exec<RO, W>(ops, amt, in, rd, wa, nv, !ro);
-        const uint32_t below = base + (uint32_t)__builtin_popcountll(committed & lt);
-        const uint32_t myts = valid ? (ro ? (kSnapshotBit | below) : below + 1) : 0u;
         // the install records wait for a free ring entry (the publisher is done with the
         // request kPipeRing before)
         const uint64_t tp3 = phase_clock();
@@ -2054,41 +2141,45 @@ __device__ __forceinline__ void committer_pipe(const Ctx &c, const ServerLayout 
         }
         if (bad) break;
         const uint64_t tp3b = phase_clock();
-        // the last write of each of this partition's items goes to the ring slot after the
+        // an installing entry writes its item's last value (the word before + the
+        // request-wide delta) with its last writer's timestamp to the ring slot after the
         // item's installs so far (j-th install: slot j mod depth; the tail moves once the
-        // ring is full); node w reads write slot w's item (a transfer reads what it writes)
+        // ring is full)
 #pragma unroll
-        for (int w = 0; w < W; ++w) {
-            const bool inst = valid && !ro && wa[w] != kNone && ((supf >> w) & 1u) == 0 && mine[w];
-            const uint32_t j = cnt[w] + 1;
+        for (int g = 0; g < G; ++g) {
+            const uint32_t e = g * kWave + lane;
+            const bool inst = at[g] && (meta[g] & kPipeInstBit) != 0;
+            const uint32_t j = (uint32_t)ret[g] + 1;
             const uint32_t pos = mod_d(j);
             const uint32_t tail = j < D ? 0u : mod_d(j + 1);
-            if (inst) st_rlx(c.slot + (size_t)wa[w] * D + pos, ((uint64_t)myts << 32) | (uint32_t)nv[w]);
-            irec[(qi * 2 * W + w) * kWave + lane] = inst ? wa[w] : kNone;
-            irec[(qi * 2 * W + W + w) * kWave + lane] = pos | (tail << 16);
+            const uint32_t ts = base + ((meta[g] >> kPipeRankShift) & 63u) + 1;
+            if (inst)
+                st_rlx(c.slot + (size_t)item[g] * D + pos,
+                       ((uint64_t)ts << 32) | (uint32_t)(val[g] + wgt[g]));
+            if (act[g]) {
+                irec[qi * 2 * kPipeMaxE + e] = inst ? item[g] : kNone;
+                irec[(qi * 2 + 1) * kPipeMaxE + e] = pos | (tail << 16);
+            }
         }
-        // the log: a transfer's timestamp and reads, an audit's snapshot and the values it
-        // read (summed into row 0 after the run: pipe_audit_sum_kernel), each value by the
-        // partition of its item; then this partition's reply word
-        if (part == 0 && valid) c.log_ts[ktx] = ro ? below : myts;
-#pragma unroll
-        for (int r = 0; r < RO; ++r)
-            if (mine[r]) c.log_v[(uint64_t)r * T + ktx] = ro ? in[r] : rd[r];
+        if (lane == 0) icnt[qi] = cnt;
+        // the reply: each entry's value now; this partition's header word (tag and the
+        // request's first timestamp) once the values are complete (hand_over)
         const uint32_t cw = cwr & ~kDelegateBit;
-        uint64_t *rp = part == 0 ? c.reply + (size_t)cw * kWave
-                                 : c.reply_v + ((size_t)cw * 2 * W + part) * kWave;
-        st_rlx(&rp[lane], ((uint64_t)(n + 1) << 32) | myts);
+        uint32_t *rv = (uint32_t *)c.reply_v + ((size_t)cw * kPipeParts + part) * kPipeMaxE;
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+            if (act[g]) st_rlx(&rv[g * kWave + lane], (uint32_t)val[g]);
         const uint64_t tp4 = phase_clock();
-        // the installs go to the publisher once the slot stores are complete: proven by the
-        // return of a probe issued after them, which the next request waits for
-        // (a load of one word by every lane: one memory request, and no partial update of
-        // the register the next request waits on)
+        // the probe after the stores (a load of one word by every lane: one memory request,
+        // and no partial update of the register the hand-over waits on)
         asm volatile("" ::: "memory");
         tail_probe = __hip_atomic_load(probe_at, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         held = true;
         held_n = n;
         held_qi = qi;
         held_last = base + nc;
+        held_cw = cw;
+        held_base = base;
         const uint64_t t2 = role_clock();
         t_idle.add(t1 - t0);
         t_busy.add(t2 - t1);
@@ -2100,19 +2191,13 @@ __device__ __forceinline__ void committer_pipe(const Ctx &c, const ServerLayout 
             t_probe.add(tp4 - tp3b);
         }
     }
-    if (held) {
-        wait_word(bcast64(tail_probe));
-        if (lane == 0) {
-            ilast[held_qi] = held_last;
-            lds_st_rel(&iready[held_qi], held_n + 1);
-        }
-    }
+    if (held) hand_over();
     if (lane == 0) {
         fetch_add(&c.prof[kProfCommitIdle], t_idle.v);
         fetch_add(&c.prof[kProfCommitBusy], t_busy.v);
         if constexpr (kPhaseTimers) {
             // phases of the pipelined committers: token wait and atomic issue, the atomics'
-            // return, execution, ring room, stores
+            // return, the values, ring room, stores
             fetch_add(&c.prof[kProfCommitRead], t_tok.v);
             fetch_add(&c.prof[kProfCommitResolve], t_atom.v);
             fetch_add(&c.prof[kProfCommitChain], t_fin.v);
@@ -2124,19 +2209,23 @@ __device__ __forceinline__ void committer_pipe(const Ctx &c, const ServerLayout 
 
 // The publisher (one wave of each pipelined server workgroup): takes the requests whose
 // installs are complete in request order, as many as are ready, publishes their head words
-// (in request order, so an item's head only moves forward), waits for those stores, and
-// publishes the partition's progress; the clock is the least progress over the
+// (in request order, so an item's head only moves forward; only when normal-lane
+// transactions read snapshots during the run), waits for those stores, and publishes the
+// partition's progress; a snapshot is the least progress over the
 // partitions, so a reader's snapshot only ever covers whole requests whose versions are in
 // place everywhere.
 template <int R, int W>
 __device__ __forceinline__ void publisher_pipe(const Ctx &c, const ServerLayout &L, uint32_t *lds)
 {
+    constexpr int G = kPipeMaxE / kWave;
     const uint32_t lane = lane_id();
     const uint32_t part = blockIdx.x;
     uint32_t *irec = lds + L.pipe_off;
-    uint32_t *ilast = irec + kPipeRing * 2 * W * kWave;
+    uint32_t *icnt = irec + kPipeRing * 2 * kPipeMaxE;
+    uint32_t *ilast = icnt + kPipeRing;
     uint32_t *iready = ilast + kPipeRing;
     uint64_t *const probe_at = c.sink + (size_t)(part * 16 + 15) * 16;
+    const bool heads = c.n_norm != 0;
     uint32_t q = 0, gc = 0;
     bool bad = false;
     LaneSum p_batches, p_wait, p_busy;
@@ -2156,13 +2245,19 @@ __device__ __forceinline__ void publisher_pipe(const Ctx &c, const ServerLayout 
         }
         if (bad || qe == q) break;
         const uint64_t tq1 = phase_clock();
-        for (uint32_t x = q; x < qe; ++x) {
+        // (a run without normal-lane transactions has no snapshot reader: its head words are
+        // set from the install counts after the run)
+        for (uint32_t x = q; heads && x < qe; ++x) {
             const uint32_t qi = x % kPipeRing;
+            const uint32_t cn = icnt[qi];
 #pragma unroll
-            for (int w = 0; w < W; ++w) {
-                const uint32_t item = irec[(qi * 2 * W + w) * kWave + lane];
-                const uint32_t nh = irec[(qi * 2 * W + W + w) * kWave + lane];
-                if (item != kNone) st_rlx(c.ht + item, nh);
+            for (int g = 0; g < G; ++g) {
+                const uint32_t e = g * kWave + lane;
+                if (e < cn) {
+                    const uint32_t item = irec[qi * 2 * kPipeMaxE + e];
+                    const uint32_t nh = irec[(qi * 2 + 1) * kPipeMaxE + e];
+                    if (item != kNone) st_rlx(c.ht + item, nh);
+                }
             }
         }
         gc = ilast[(qe - 1) % kPipeRing];
@@ -2205,15 +2300,16 @@ __device__ __forceinline__ void publisher_pipe(const Ctx &c, const ServerLayout 
 }
 
 // A receiver of the pipelined lane: polls the request ring for requests n = rid, rid + M,
-// ... and copies the rows the client staged into LDS mailbox n mod nmb once a committer has
-// finished that mailbox's previous request.
+// ... and copies this partition's entry list from the client's box into LDS mailbox
+// n mod nmb once a committer has finished that mailbox's previous request: the header
+// and the list's 32-entry chunks (their number from the ring word) in one round of loads.
 template <int R, int W>
 __device__ __forceinline__ void receiver_pipe(const Ctx &c, const ServerLayout &L, uint32_t *lds,
                                               uint32_t rid, uint32_t M)
 {
-    using RW = Rows<R, W>;
-    constexpr int kA = 1 + R + 2 * W, kB = 2 * RW::RO + 1;
+    constexpr int G = kPipeMaxE / kWave;
     const uint32_t lane = lane_id();
+    const uint32_t part = blockIdx.x;
     uint32_t *mbst = lds + L.state_off;
     uint32_t *mbwave = lds + L.wave_off;
     uint32_t *mbfree = lds + L.tg_off;
@@ -2236,32 +2332,56 @@ __device__ __forceinline__ void receiver_pipe(const Ctx &c, const ServerLayout &
             for (uint32_t polls = 0;; ++polls) {
                 e = ld_rlx(&c.ring[n & c.qmask]);
                 if ((e >> 32) == (uint64_t)n + 1) break;
-                if ((polls & 7u) == 7u && ld_acq(c.clients_done) == c.n_client_waves &&
-                    ld_acq(c.req_tail) <= n) {
-                    // every client has left and request n will never come
-                    if (ld_rlx(c.req_tail) == n && lane == 0) lds_st_rel(&mbst[mbi], kExitBit | (n + 1));
-                    leave = true;
-                    break;
+                // (the acquire only once every client has left: an acquire invalidates this
+                // XCD's L2 lines, the committers' value words among them)
+                if ((polls & 7u) == 7u && ld_rlx(c.clients_done) == c.n_client_waves) {
+                    fence_acq();
+                    if (ld_acq(c.req_tail) <= n) {
+                        // every client has left and request n will never come
+                        if (ld_rlx(c.req_tail) == n && lane == 0)
+                            lds_st_rel(&mbst[mbi], kExitBit | (n + 1));
+                        leave = true;
+                        break;
+                    }
                 }
                 if (!sp.tick(c, kErrRingTimeout)) { leave = true; break; }
             }
         }
         if (leave) break;
-        fence_acq();
+        // the box loads take their address from the ring word, so they are issued after it
+        // has arrived; like it they are agent-scope loads, which read the device's point of
+        // coherence (no acquire, whose invalidation would take this XCD's L2 lines, the
+        // committers' value words among them)
         const uint32_t cwr = (uint32_t)e;
-        const uint32_t *ps = c.mbox + (size_t)(cwr & ~kDelegateBit) * RW::count * kWave;
+        const uint32_t C = ((cwr >> (kPipeChunkShift + 4 * part)) & 0xFu) * kPipeChunk;
+        const uint32_t *ps = c.mbox + ((size_t)(cwr & kPipeCwMask) * kPipeParts + part) * kPipeBox;
         uint32_t *dst = lds + L.mb_off + mbi * L.mb_rows * kWave;
-        uint32_t v[kA + kB];
+        uint32_t v[3 * G];
+        uint32_t h[1];
+        h[0] = ld_rlx(&ps[0]);
 #pragma unroll
-        for (int i = 0; i < kA; ++i) v[i] = ld_rlx(&ps[i * kWave + lane]);
+        for (int g = 0; g < G; ++g) {
+            const uint32_t x = g * kWave + lane;
 #pragma unroll
-        for (int i = 0; i < kB; ++i) v[kA + i] = ld_rlx(&ps[(RW::root + i) * kWave + lane]);
+            for (int f = 0; f < 3; ++f) {
+                v[f * G + g] = 0;
+                if (x < C) v[f * G + g] = ld_rlx(&ps[kPipeItem + f * kPipeMaxE + x]);
+            }
+        }
+        pin_round(h);
         pin_round(v);
+        const uint32_t cnt = h[0] & 0x1FFu;      // at most C
 #pragma unroll
-        for (int i = 0; i < kA; ++i) dst[i * kWave + lane] = v[i];
+        for (int g = 0; g < G; ++g) {
+            const uint32_t x = g * kWave + lane;
+            if (x < cnt)
 #pragma unroll
-        for (int i = 0; i < kB; ++i) dst[(RW::root + i) * kWave + lane] = v[kA + i];
-        if (lane == 0) mbwave[mbi] = cwr;
+                for (int f = 0; f < 3; ++f) dst[kPipeItem + f * kPipeMaxE + x] = v[f * G + g];
+        }
+        if (lane == 0) {
+            dst[0] = h[0];
+            mbwave[mbi] = (cwr & kPipeCwMask) | (cwr & kDelegateBit);
+        }
         wg_fence();
         if (lane == 0) lds_st_rel(&mbst[mbi], n + 1);
     }
@@ -2274,6 +2394,8 @@ __device__ __forceinline__ void pipe_server_workgroup(const Ctx &c, const Server
 {
     const uint32_t wave = threadIdx.x / kWave;
     for (uint32_t i = threadIdx.x; i < L.mb_off; i += blockDim.x) lds[i] = 0;
+    // the install ring too: its ready words must not match a request of this run
+    for (uint32_t i = threadIdx.x; i < pipe_words(); i += blockDim.x) lds[L.pipe_off + i] = 0;
     __syncthreads();
     const uint32_t K = pipe_committers(c.wg_waves);
     if (wave < K) committer_pipe<R, W>(c, L, lds, wave, K);
