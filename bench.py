@@ -8,8 +8,8 @@ the device before timing (resident in HBM).  One step = one pass of the persiste
 over the whole stream; the pool is reloaded between steps outside the timed kernel.
 value = committed transactions / kernel seconds (HIP events on the engine's stream).
 Commit path of the headline: the pipelined serialized lane (variant stamp_pipe) with every
-transaction routed to it, 4 commit server workgroups of 12 waves (5 committers, a
-publisher, 6 receivers each) over a hash split of the accounts, 80 client waves of which
+transaction routed to it, 4 commit server workgroups of 12 waves (8 committers, a
+publisher, 3 receivers each) over a hash split of the accounts, 80 client waves of which
 64 feed the lane.
 
 Beside it, in the same JSON line:
@@ -483,8 +483,8 @@ def main():
                     help="waves per workgroup of the one-committer server: the committer, the "
                          "clock advancer and wg_waves - 2 receivers")
     ap.add_argument("--pipe-wg-waves", type=int, default=12,
-                    help="waves per workgroup of each pipelined commit server: (w - 1) / 2 "
-                         "committers, one publisher, the rest receivers")
+                    help="waves per workgroup of each pipelined commit server: w / 4 "
+                         "receivers (1 below 8 waves), one publisher, the rest committers")
     ap.add_argument("--commit-parts", type=int, default=4,
                     help="commit server workgroups of the pipelined lane (items split by hash)")
     ap.add_argument("--one-waves", type=int, default=64,
@@ -572,7 +572,9 @@ def main():
     pipe_h = args.variant == "stamp_pipe"
     waves_h = args.waves if pipe_h else args.one_waves
     wg_h = args.pipe_wg_waves if pipe_h else args.wg_waves
-    committers = ((wg_h - 1) // 2 if wg_h >= 5 else 1) * args.commit_parts if pipe_h else 1
+    # pipelined server workgroup of w waves: w / 4 receivers (1 below 8 waves), a publisher,
+    # the rest committers
+    committers = (wg_h - 1 - (wg_h // 4 if wg_h >= 8 else 1)) * args.commit_parts if pipe_h else 1
     log(f"headline: {args.txns} txns, router {args.router}")
     e = make(args.variant, bank, args.router, args.serial_waves)
     for _ in range(args.warmup):

@@ -118,9 +118,11 @@ constexpr uint32_t kPipeCwMask = 0xFFFu, kPipeChunkShift = 12, kPipeChunk = 32;
 // word.
 constexpr uint32_t kPipeRing = 16;
 __host__ __device__ constexpr uint32_t pipe_words() { return kPipeRing * (2 * kPipeMaxE + 3); }
-// Committer waves of a pipelined server workgroup of wg waves (one more publishes, the
-// rest receive).
-__host__ __device__ constexpr uint32_t pipe_committers(uint32_t wg) { return wg >= 5 ? (wg - 1) / 2 : 1; }
+// Receiver and committer waves of a pipelined server workgroup of wg waves (one more
+// publishes): a receiver keeps two requests in flight, so a quarter of the waves receive
+// (12 waves: 3 receivers, 8 committers, the publisher).
+__host__ __device__ constexpr uint32_t pipe_receivers(uint32_t wg) { return wg >= 8 ? wg / 4 : 1; }
+__host__ __device__ constexpr uint32_t pipe_committers(uint32_t wg) { return wg - 1 - pipe_receivers(wg); }
 // A client wave's LDS scratch for staging a pipelined request: the write hash (key, writer
 // lanes lo / hi, canonical node, request-wide delta), then the chain arrays (next, weight
 // per node; superseded flag per write slot), the last writers' ranks (by hash slot), each
@@ -2037,9 +2039,9 @@ __device__ __forceinline__ void committer_pipe(const Ctx &c, const ServerLayout 
     uint32_t n = k;
     for (;; n += K) {
         const uint32_t m = n % L.nmb;
-        if (held) hand_over();
         const uint64_t t0 = role_clock();
-        // request n staged, or the end of the stream
+        // request n staged, or the end of the stream (the finished request is handed over
+        // while nothing is staged: its client may be waiting for it)
         bool end = false, bad = false;
         {
             Spin sp;
@@ -2052,6 +2054,7 @@ __device__ __forceinline__ void committer_pipe(const Ctx &c, const ServerLayout 
                     }
                     break;
                 }
+                if (held) hand_over();
                 const uint32_t e = lds_ld_acq(&lds[kLdsEnd]);
                 if (e && n + 1 >= e) { end = true; break; }
                 if (!sp.tick(c, kErrMailboxTimeout)) { bad = true; break; }
@@ -2094,6 +2097,8 @@ __device__ __forceinline__ void committer_pipe(const Ctx &c, const ServerLayout 
         }
         pin_round(addr);
         pin_round(add);
+        // otherwise the hand-over waits here, its probe's latency spent on the steps above
+        if (held) hand_over();
         const uint64_t tp0 = phase_clock();
         uint32_t base = 0;
         {
@@ -2299,10 +2304,12 @@ __device__ __forceinline__ void publisher_pipe(const Ctx &c, const ServerLayout 
     }
 }
 
-// A receiver of the pipelined lane: polls the request ring for requests n = rid, rid + M,
-// ... and copies this partition's entry list from the client's box into LDS mailbox
-// n mod nmb once a committer has finished that mailbox's previous request: the header
-// and the list's 32-entry chunks (their number from the ring word) in one round of loads.
+// A receiver of the pipelined lane: takes requests n = rid, rid + M, ... and copies this
+// partition's entry list from the client's box into LDS mailbox n mod nmb: the header and
+// the list's 32-entry chunks (their number from the ring word) in one round of loads.  The
+// receiver keeps two requests in flight: once request n's ring word is in, it issues n's
+// box loads and polls for request n + M, then (that poll in flight) stores n's list into
+// its mailbox when a committer has finished the mailbox's previous request.
 template <int R, int W>
 __device__ __forceinline__ void receiver_pipe(const Ctx &c, const ServerLayout &L, uint32_t *lds,
                                               uint32_t rid, uint32_t M)
@@ -2313,77 +2320,94 @@ __device__ __forceinline__ void receiver_pipe(const Ctx &c, const ServerLayout &
     uint32_t *mbst = lds + L.state_off;
     uint32_t *mbwave = lds + L.wave_off;
     uint32_t *mbfree = lds + L.tg_off;
-    for (uint32_t n = rid;; n += M) {
-        const uint32_t mbi = n % L.nmb;
-        if (n >= L.nmb) {
-            Spin sp;
+    uint32_t v[3 * G];          // the pending request's list, as loaded
+    uint32_t h[1] = {0};        // its header
+    bool pend = false;
+    uint32_t pn = 0, pcwr = 0;  // the pending request and its ring word
+    uint32_t n = rid;
+    Spin sp;
+    for (uint32_t polls = 0;; ++polls) {
+        const uint64_t e0 = ld_rlx(&c.ring[n & c.qmask]);
+        if (pend) {
+            // the pending request's list into its mailbox, once the mailbox is free
+            const uint32_t mbi = pn % L.nmb;
             bool bad = false;
-            while (lds_ld_acq(&mbfree[mbi]) < n + 1 - L.nmb)
-                if (lds_ld_acq(&lds[kLdsCdone]) || !sp.tick(c, kErrSlotTimeout)) {
-                    bad = true;
-                    break;
-                }
-            if (bad) break;
-        }
-        uint64_t e;
-        bool leave = false;
-        {
-            Spin sp;
-            for (uint32_t polls = 0;; ++polls) {
-                e = ld_rlx(&c.ring[n & c.qmask]);
-                if ((e >> 32) == (uint64_t)n + 1) break;
-                // (the acquire only once every client has left: an acquire invalidates this
-                // XCD's L2 lines, the committers' value words among them)
-                if ((polls & 7u) == 7u && ld_rlx(c.clients_done) == c.n_client_waves) {
-                    fence_acq();
-                    if (ld_acq(c.req_tail) <= n) {
-                        // every client has left and request n will never come
-                        if (ld_rlx(c.req_tail) == n && lane == 0)
-                            lds_st_rel(&mbst[mbi], kExitBit | (n + 1));
-                        leave = true;
+            if (pn >= L.nmb) {
+                Spin sf;
+                while (lds_ld_acq(&mbfree[mbi]) < pn + 1 - L.nmb)
+                    if (lds_ld_acq(&lds[kLdsCdone]) || !sf.tick(c, kErrSlotTimeout)) {
+                        bad = true;
                         break;
                     }
+            }
+            if (bad) break;
+            uint32_t *dst = lds + L.mb_off + mbi * L.mb_rows * kWave;
+            const uint32_t cnt = h[0] & 0x1FFu;
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                const uint32_t x = g * kWave + lane;
+                if (x < cnt)
+#pragma unroll
+                    for (int f = 0; f < 3; ++f) dst[kPipeItem + f * kPipeMaxE + x] = v[f * G + g];
+            }
+            if (lane == 0) {
+                dst[0] = h[0];
+                mbwave[mbi] = (pcwr & kPipeCwMask) | (pcwr & kDelegateBit);
+            }
+            wg_fence();
+            if (lane == 0) lds_st_rel(&mbst[mbi], pn + 1);
+            pend = false;
+        }
+        const uint64_t e = bcast64(e0);     // one word: the branch below is the wave's
+        if ((e >> 32) == (uint64_t)n + 1) {
+            // request n's box loads (their address from the ring word, so issued after it
+            // has arrived; agent-scope loads, which read the device's point of coherence:
+            // no acquire, whose invalidation would take this XCD's L2 lines, the
+            // committers' value words among them)
+            const uint32_t cwr = (uint32_t)e;
+            const uint32_t C = ((cwr >> (kPipeChunkShift + 4 * part)) & 0xFu) * kPipeChunk;
+            const uint32_t *ps = c.mbox + ((size_t)(cwr & kPipeCwMask) * kPipeParts + part) * kPipeBox;
+            h[0] = ld_rlx(&ps[0]);
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                const uint32_t x = g * kWave + lane;
+#pragma unroll
+                for (int f = 0; f < 3; ++f) {
+                    v[f * G + g] = 0;
+                    if (x < C) v[f * G + g] = ld_rlx(&ps[kPipeItem + f * kPipeMaxE + x]);
                 }
-                if (!sp.tick(c, kErrRingTimeout)) { leave = true; break; }
+            }
+            pend = true;
+            pn = n;
+            pcwr = cwr;
+            n += M;
+            polls = 0;
+            sp = Spin();
+            continue;
+        }
+        // (the acquire only once every client has left: an acquire invalidates this XCD's
+        // L2 lines, the committers' value words among them)
+        if ((polls & 7u) == 7u && ld_rlx(c.clients_done) == c.n_client_waves) {
+            fence_acq();
+            if (ld_acq(c.req_tail) <= n) {
+                // every client has left and request n will never come
+                if (ld_rlx(c.req_tail) == n && lane == 0) {
+                    const uint32_t mbi = n % L.nmb;
+                    bool bad = false;
+                    if (n >= L.nmb) {
+                        Spin sf;
+                        while (lds_ld_acq(&mbfree[mbi]) < n + 1 - L.nmb)
+                            if (lds_ld_acq(&lds[kLdsCdone]) || !sf.tick(c, kErrSlotTimeout)) {
+                                bad = true;
+                                break;
+                            }
+                    }
+                    if (!bad) lds_st_rel(&mbst[mbi], kExitBit | (n + 1));
+                }
+                break;
             }
         }
-        if (leave) break;
-        // the box loads take their address from the ring word, so they are issued after it
-        // has arrived; like it they are agent-scope loads, which read the device's point of
-        // coherence (no acquire, whose invalidation would take this XCD's L2 lines, the
-        // committers' value words among them)
-        const uint32_t cwr = (uint32_t)e;
-        const uint32_t C = ((cwr >> (kPipeChunkShift + 4 * part)) & 0xFu) * kPipeChunk;
-        const uint32_t *ps = c.mbox + ((size_t)(cwr & kPipeCwMask) * kPipeParts + part) * kPipeBox;
-        uint32_t *dst = lds + L.mb_off + mbi * L.mb_rows * kWave;
-        uint32_t v[3 * G];
-        uint32_t h[1];
-        h[0] = ld_rlx(&ps[0]);
-#pragma unroll
-        for (int g = 0; g < G; ++g) {
-            const uint32_t x = g * kWave + lane;
-#pragma unroll
-            for (int f = 0; f < 3; ++f) {
-                v[f * G + g] = 0;
-                if (x < C) v[f * G + g] = ld_rlx(&ps[kPipeItem + f * kPipeMaxE + x]);
-            }
-        }
-        pin_round(h);
-        pin_round(v);
-        const uint32_t cnt = h[0] & 0x1FFu;      // at most C
-#pragma unroll
-        for (int g = 0; g < G; ++g) {
-            const uint32_t x = g * kWave + lane;
-            if (x < cnt)
-#pragma unroll
-                for (int f = 0; f < 3; ++f) dst[kPipeItem + f * kPipeMaxE + x] = v[f * G + g];
-        }
-        if (lane == 0) {
-            dst[0] = h[0];
-            mbwave[mbi] = (cwr & kPipeCwMask) | (cwr & kDelegateBit);
-        }
-        wg_fence();
-        if (lane == 0) lds_st_rel(&mbst[mbi], n + 1);
+        if (!sp.tick(c, kErrRingTimeout)) break;
     }
 }
 
