@@ -49,12 +49,16 @@ enum {
     CSMV_WB_SERVER = 0x8,     /* the server installs versions (vanilla/csmv_sh/API.cu:269-272) */
     CSMV_SERIAL_PIPE = 0x10   /* pipelined serialized lane (bank driver, with CSMV_VARIANT_STAMP
                                  only; no counterpart in the reference): the serialized-lane
-                                 clients stage their requests' producer chains, the commit
-                                 server applies each written account's request-wide delta with
-                                 one ordered atomic and installs the versions itself.  Every
+                                 clients stage their requests' producer chains and split them
+                                 by commit partition; committer waves apply each account's
+                                 request-wide delta with one ordered 64-bit atomic, install the
+                                 versions and reply with the values every read sees, and the
+                                 clients execute and log their transactions on them.  Every
                                  transfer must be routed to the serialized lane (the normal lane
-                                 then carries audits only); no escalation, cache or serving
-                                 runs, caller kernels or allocation-free runs. */
+                                 then carries audits only; routing every transaction leaves
+                                 none to abort); at most 4096 client waves; no escalation,
+                                 cache or serving runs, caller kernels or allocation-free
+                                 runs. */
 };
 
 #define CSMV_VARIANT_VANILLA          (CSMV_VAL_SCAN | CSMV_INSERT_LANE | CSMV_WB_SERVER)
@@ -69,7 +73,8 @@ typedef struct {
     uint32_t depth;            /* versions per item (MaxVersions, API.cuh:18), 1..64 */
     uint32_t atr_entries;      /* record ring capacity (TXRecordSize, API.cuh:17); power of two */
     uint32_t client_waves;     /* persistent client wavefronts (64 lanes each) */
-    uint32_t wg_waves;         /* waves per workgroup, 3..8; the server workgroup uses all */
+    uint32_t wg_waves;         /* waves per workgroup, 3..8 (3..12 with CSMV_SERIAL_PIPE); the
+                                  server workgroup uses all */
     uint32_t variant;          /* CSMV_VAL_* | CSMV_INSERT_* | CSMV_WB_* (| CSMV_SERIAL_PIPE) */
     uint32_t timeout_ms;       /* bound of every device-side wait (0 = 4000) */
     uint32_t max_attempts;     /* per-transaction attempt bound (0 = 1<<20) */
