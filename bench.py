@@ -8,8 +8,8 @@ the device before timing (resident in HBM).  One step = one pass of the persiste
 over the whole stream; the pool is reloaded between steps outside the timed kernel.
 value = committed transactions / kernel seconds (HIP events on the engine's stream).
 Commit path of the headline: the pipelined serialized lane (variant stamp_pipe) with every
-transaction routed to it, 4 commit server workgroups of 12 waves (8 committers, a
-publisher, 3 receivers each) over a hash split of the accounts, 80 client waves of which
+transaction routed to it, 4 commit server workgroups of 16 waves (11 committers, a
+publisher, 4 receivers each) over a hash split of the accounts, 80 client waves of which
 64 feed the lane.
 
 Beside it, in the same JSON line:
@@ -482,7 +482,7 @@ def main():
     ap.add_argument("--wg-waves", type=int, default=6,
                     help="waves per workgroup of the one-committer server: the committer, the "
                          "clock advancer and wg_waves - 2 receivers")
-    ap.add_argument("--pipe-wg-waves", type=int, default=12,
+    ap.add_argument("--pipe-wg-waves", type=int, default=16,
                     help="waves per workgroup of each pipelined commit server: w / 4 "
                          "receivers (1 below 8 waves), one publisher, the rest committers")
     ap.add_argument("--commit-parts", type=int, default=4,

@@ -474,7 +474,7 @@ __global__ void __launch_bounds__(512) bank_kernel(Ctx c, ServerLayout L)
 // workgroup of up to 12 waves (several committers), clients that stage their chains in
 // their workgroup's LDS, and the normal lane's audits.
 template <int R, int W>
-__global__ void __launch_bounds__(768) bank_pipe_kernel(Ctx c, ServerLayout L)
+__global__ void __launch_bounds__(1024) bank_pipe_kernel(Ctx c, ServerLayout L)
 {
     extern __shared__ uint32_t lds[];
     if (blockIdx.x < L.parts) {
@@ -706,7 +706,7 @@ csmv_status normalize_config(csmv_config &cfg)
     if (cfg.commit_parts > 4 || (cfg.commit_parts > 1 && !(cfg.variant & kSerialPipe))) return CSMV_E_ARG;
     if (cfg.n_serial_waves >= cfg.client_waves && cfg.n_serial_waves) return CSMV_E_ARG;
     if (cfg.n_items == 0 || cfg.depth == 0 || cfg.depth > 64 || cfg.client_waves == 0 ||
-        cfg.wg_waves < 3 || cfg.wg_waves > ((cfg.variant & kSerialPipe) ? 12u : 8u) ||
+        cfg.wg_waves < 3 || cfg.wg_waves > ((cfg.variant & kSerialPipe) ? 16u : 8u) ||
         ((cfg.variant & kSerialPipe) && cfg.client_waves > kPipeCwMask + 1) ||
         cfg.client_wg_waves > cfg.wg_waves ||
         cfg.stage_ahead > 64 || cfg.atr_entries < 64 ||

@@ -119,8 +119,8 @@ constexpr uint32_t kPipeCwMask = 0xFFFu, kPipeChunkShift = 12, kPipeChunk = 32;
 constexpr uint32_t kPipeRing = 16;
 __host__ __device__ constexpr uint32_t pipe_words() { return kPipeRing * (2 * kPipeMaxE + 3); }
 // Receiver and committer waves of a pipelined server workgroup of wg waves (one more
-// publishes): a receiver keeps two requests in flight, so a quarter of the waves receive
-// (12 waves: 3 receivers, 8 committers, the publisher).
+// publishes): a receiver takes up to two requests per memory round trip, so a quarter of
+// the waves receive (16 waves: 4 receivers, 11 committers, the publisher).
 __host__ __device__ constexpr uint32_t pipe_receivers(uint32_t wg) { return wg >= 8 ? wg / 4 : 1; }
 __host__ __device__ constexpr uint32_t pipe_committers(uint32_t wg) { return wg - 1 - pipe_receivers(wg); }
 // A client wave's LDS scratch for staging a pipelined request: the write hash (key, writer
@@ -2266,14 +2266,18 @@ __device__ __forceinline__ void publisher_pipe(const Ctx &c, const ServerLayout 
             }
         }
         gc = ilast[(qe - 1) % kPipeRing];
-        asm volatile("" ::: "memory");
-        uint64_t probe = 0;
-        if (lane == 0) probe = g_fetch_add((uintptr_t)probe_at, 0ull);
-        wait_word(bcast64(probe));
+        if (heads) {
+            // the head words are complete once a probe issued after them has returned
+            asm volatile("" ::: "memory");
+            uint64_t probe = 0;
+            if (lane == 0) probe = g_fetch_add((uintptr_t)probe_at, 0ull);
+            wait_word(bcast64(probe));
+        }
         if (lane == 0) {
             // this partition's versions through timestamp gc are in place (every version and
-            // head word is an agent-scope store, complete by the probe); a snapshot is the
-            // least of the partitions' progress words (pipe_snapshot)
+            // head word is an agent-scope store, complete by the committers' probes and this
+            // wave's); a snapshot is the least of the partitions' progress words
+            // (pipe_snapshot)
             st_rlx(&c.pipe_prog[part], gc);
             lds_st_rel(&lds[kLdsPub], qe);
         }
@@ -2306,10 +2310,11 @@ __device__ __forceinline__ void publisher_pipe(const Ctx &c, const ServerLayout 
 
 // A receiver of the pipelined lane: takes requests n = rid, rid + M, ... and copies this
 // partition's entry list from the client's box into LDS mailbox n mod nmb: the header and
-// the list's 32-entry chunks (their number from the ring word) in one round of loads.  The
-// receiver keeps two requests in flight: once request n's ring word is in, it issues n's
-// box loads and polls for request n + M, then (that poll in flight) stores n's list into
-// its mailbox when a committer has finished the mailbox's previous request.
+// the list's 32-entry chunks (their number from the ring word) in one round of loads.  Each
+// step polls the ring words of its next two requests, issues the box loads of those that
+// have arrived, and stores the lists loaded in the previous step into their mailboxes (once
+// a committer has finished each mailbox's previous request) while those loads are in
+// flight: up to two requests per memory round trip.
 template <int R, int W>
 __device__ __forceinline__ void receiver_pipe(const Ctx &c, const ServerLayout &L, uint32_t *lds,
                                               uint32_t rid, uint32_t M)
@@ -2320,67 +2325,78 @@ __device__ __forceinline__ void receiver_pipe(const Ctx &c, const ServerLayout &
     uint32_t *mbst = lds + L.state_off;
     uint32_t *mbwave = lds + L.wave_off;
     uint32_t *mbfree = lds + L.tg_off;
-    uint32_t v[3 * G];          // the pending request's list, as loaded
-    uint32_t h[1] = {0};        // its header
-    bool pend = false;
-    uint32_t pn = 0, pcwr = 0;  // the pending request and its ring word
-    uint32_t n = rid;
+    // the lists in flight (at most two), as loaded
+    uint32_t va[3 * G], vb[3 * G];
+    uint32_t ha[1] = {0}, hb[1] = {0};
+    uint32_t np = 0;            // requests in flight: pn, pn + M
+    uint32_t pn = 0, pca = 0, pcb = 0;
+    LaneSum r_free, r_ring;     // phase timers: waiting for a mailbox, for the ring
+    // stores a loaded list into the mailbox of request q
+    auto stage = [&](uint32_t q, uint32_t cwr, const uint32_t *h, const uint32_t *v) -> bool {
+        const uint32_t mbi = q % L.nmb;
+        if (q >= L.nmb) {
+            const uint64_t tf = phase_clock();
+            Spin sf;
+            while (lds_ld_acq(&mbfree[mbi]) < q + 1 - L.nmb)
+                if (lds_ld_acq(&lds[kLdsCdone]) || !sf.tick(c, kErrSlotTimeout)) return false;
+            if constexpr (kPhaseTimers) r_free.add(phase_clock() - tf);
+        }
+        uint32_t *dst = lds + L.mb_off + mbi * L.mb_rows * kWave;
+        const uint32_t cnt = h[0] & 0x1FFu;
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const uint32_t x = g * kWave + lane;
+            if (x < cnt)
+#pragma unroll
+                for (int f = 0; f < 3; ++f) dst[kPipeItem + f * kPipeMaxE + x] = v[f * G + g];
+        }
+        if (lane == 0) {
+            dst[0] = h[0];
+            mbwave[mbi] = (cwr & kPipeCwMask) | (cwr & kDelegateBit);
+        }
+        wg_fence();
+        if (lane == 0) lds_st_rel(&mbst[mbi], q + 1);
+        return true;
+    };
+    // issues the box loads of a request whose ring word has arrived (their address from
+    // that word, so issued after it; agent-scope loads, which read the device's point of
+    // coherence: no acquire, whose invalidation would take this XCD's L2 lines, the
+    // committers' value words among them)
+    auto fetch = [&](uint32_t cwr, uint32_t *h, uint32_t *v) {
+        const uint32_t C = ((cwr >> (kPipeChunkShift + 4 * part)) & 0xFu) * kPipeChunk;
+        const uint32_t *ps = c.mbox + ((size_t)(cwr & kPipeCwMask) * kPipeParts + part) * kPipeBox;
+        h[0] = ld_rlx(&ps[0]);
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const uint32_t x = g * kWave + lane;
+#pragma unroll
+            for (int f = 0; f < 3; ++f) {
+                v[f * G + g] = 0;
+                if (x < C) v[f * G + g] = ld_rlx(&ps[kPipeItem + f * kPipeMaxE + x]);
+            }
+        }
+    };
+    uint32_t n = rid;           // the next request to fetch
     Spin sp;
     for (uint32_t polls = 0;; ++polls) {
+        const uint64_t tr = phase_clock();
         const uint64_t e0 = ld_rlx(&c.ring[n & c.qmask]);
-        if (pend) {
-            // the pending request's list into its mailbox, once the mailbox is free
-            const uint32_t mbi = pn % L.nmb;
-            bool bad = false;
-            if (pn >= L.nmb) {
-                Spin sf;
-                while (lds_ld_acq(&mbfree[mbi]) < pn + 1 - L.nmb)
-                    if (lds_ld_acq(&lds[kLdsCdone]) || !sf.tick(c, kErrSlotTimeout)) {
-                        bad = true;
-                        break;
-                    }
-            }
-            if (bad) break;
-            uint32_t *dst = lds + L.mb_off + mbi * L.mb_rows * kWave;
-            const uint32_t cnt = h[0] & 0x1FFu;
-#pragma unroll
-            for (int g = 0; g < G; ++g) {
-                const uint32_t x = g * kWave + lane;
-                if (x < cnt)
-#pragma unroll
-                    for (int f = 0; f < 3; ++f) dst[kPipeItem + f * kPipeMaxE + x] = v[f * G + g];
-            }
-            if (lane == 0) {
-                dst[0] = h[0];
-                mbwave[mbi] = (pcwr & kPipeCwMask) | (pcwr & kDelegateBit);
-            }
-            wg_fence();
-            if (lane == 0) lds_st_rel(&mbst[mbi], pn + 1);
-            pend = false;
-        }
-        const uint64_t e = bcast64(e0);     // one word: the branch below is the wave's
-        if ((e >> 32) == (uint64_t)n + 1) {
-            // request n's box loads (their address from the ring word, so issued after it
-            // has arrived; agent-scope loads, which read the device's point of coherence:
-            // no acquire, whose invalidation would take this XCD's L2 lines, the
-            // committers' value words among them)
-            const uint32_t cwr = (uint32_t)e;
-            const uint32_t C = ((cwr >> (kPipeChunkShift + 4 * part)) & 0xFu) * kPipeChunk;
-            const uint32_t *ps = c.mbox + ((size_t)(cwr & kPipeCwMask) * kPipeParts + part) * kPipeBox;
-            h[0] = ld_rlx(&ps[0]);
-#pragma unroll
-            for (int g = 0; g < G; ++g) {
-                const uint32_t x = g * kWave + lane;
-#pragma unroll
-                for (int f = 0; f < 3; ++f) {
-                    v[f * G + g] = 0;
-                    if (x < C) v[f * G + g] = ld_rlx(&ps[kPipeItem + f * kPipeMaxE + x]);
-                }
-            }
-            pend = true;
+        const uint64_t e1 = ld_rlx(&c.ring[(n + M) & c.qmask]);
+        if (np >= 1 && !stage(pn, pca, ha, va)) break;
+        if (np == 2 && !stage(pn + M, pcb, hb, vb)) break;
+        np = 0;
+        const uint64_t a = bcast64(e0), b = bcast64(e1);   // one word each: wave-wide branches
+        if ((a >> 32) == (uint64_t)n + 1) {
             pn = n;
-            pcwr = cwr;
-            n += M;
+            pca = (uint32_t)a;
+            fetch(pca, ha, va);
+            np = 1;
+            if ((b >> 32) == (uint64_t)n + M + 1) {
+                pcb = (uint32_t)b;
+                fetch(pcb, hb, vb);
+                np = 2;
+            }
+            n += np * M;
             polls = 0;
             sp = Spin();
             continue;
@@ -2393,22 +2409,30 @@ __device__ __forceinline__ void receiver_pipe(const Ctx &c, const ServerLayout &
                 // every client has left and request n will never come
                 if (ld_rlx(c.req_tail) == n && lane == 0) {
                     const uint32_t mbi = n % L.nmb;
-                    bool bad = false;
+                    bool ok = true;
                     if (n >= L.nmb) {
                         Spin sf;
                         while (lds_ld_acq(&mbfree[mbi]) < n + 1 - L.nmb)
                             if (lds_ld_acq(&lds[kLdsCdone]) || !sf.tick(c, kErrSlotTimeout)) {
-                                bad = true;
+                                ok = false;
                                 break;
                             }
                     }
-                    if (!bad) lds_st_rel(&mbst[mbi], kExitBit | (n + 1));
+                    if (ok) lds_st_rel(&mbst[mbi], kExitBit | (n + 1));
                 }
                 break;
             }
         }
+        if constexpr (kPhaseTimers) r_ring.add(phase_clock() - tr);
         if (!sp.tick(c, kErrRingTimeout)) break;
     }
+    if constexpr (kPhaseTimers)
+        if (lane == 0) {
+            // the pipelined receivers' waits for a free mailbox and for the ring (slots the
+            // pipelined committers leave unused)
+            fetch_add(&c.prof[kProfCommitMiss], r_free.v);
+            fetch_add(&c.prof[kProfCommitAdv], r_ring.v);
+        }
 }
 
 // The server workgroup of the pipelined lane: K committers (pipe_committers), one
