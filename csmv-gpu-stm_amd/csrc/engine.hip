@@ -478,7 +478,7 @@ __global__ void __launch_bounds__(1024) bank_pipe_kernel(Ctx c, ServerLayout L)
 {
     extern __shared__ uint32_t lds[];
     if (blockIdx.x < L.parts) {
-        pipe_server_workgroup<R, W>(c, L, lds);
+        pipe_server_workgroup<R, W>(c, L, lds, blockIdx.x);
         return;
     }
     const uint32_t wave = threadIdx.x / kWave;
@@ -537,22 +537,30 @@ struct ServeArgs {
     Ctx c[kMaxServeParts];
     ServerLayout L[kMaxServeParts];
     uint32_t n_cache, has_bank;
+    uint32_t bank_pipe;                // the bank partition runs the pipelined lane
     uint32_t wg_end[kMaxServeParts];   // client workgroups of partitions 0..i (prefix sums)
 };
 static_assert(sizeof(ServeArgs) <= 4096, "the serving kernel's arguments fit the kernarg limit");
 
-template <int RM, int WM, int RB, int WB>
+// kPipeBank: the bank partition runs the pipelined lane (a kernel of its own, so that the
+// other bank roles' registers do not add to it).
+template <int RM, int WM, int RB, int WB, bool kPipeBank>
 __global__ void __launch_bounds__(512) serve_kernel(ServeArgs a)
 {
     extern __shared__ uint32_t lds[];
     const uint32_t n_part = a.n_cache + a.has_bank;
-    if (blockIdx.x < n_part) {
+    // server workgroups: one per cache partition, then the bank partition's (one per
+    // commit partition of a pipelined bank lane)
+    const uint32_t n_srv = a.n_cache + (a.has_bank ? (kPipeBank ? a.L[a.n_cache].parts : 1u) : 0u);
+    if (blockIdx.x < n_srv) {
         const uint32_t p = blockIdx.x;
         if (p < a.n_cache) server_workgroup<RM, WM, false>(a.c[p], a.L[p], lds);
+        else if constexpr (kPipeBank)
+            pipe_server_workgroup<RB, WB>(a.c[a.n_cache], a.L[a.n_cache], lds, p - a.n_cache);
         else server_workgroup<RB, WB, true>(a.c[p], a.L[p], lds);
         return;
     }
-    const uint32_t b = blockIdx.x - n_part;
+    const uint32_t b = blockIdx.x - n_srv;
     uint32_t p = 0;
     while (p + 1 < n_part && b >= a.wg_end[p]) ++p;
     const uint32_t b0 = p ? a.wg_end[p - 1] : 0u;
@@ -562,6 +570,8 @@ __global__ void __launch_bounds__(512) serve_kernel(ServeArgs a)
     const uint32_t cw = (b - b0) * c.client_wg_waves + wave;
     if (cw >= c.n_client_waves) return;
     if (p < a.n_cache) memcd_client<RM, WM>(c, cw);
+    else if constexpr (kPipeBank)
+        bank_serve_pipe_client<RB, WB>(c, a.L[p], lds + wave * pipe_wide_words(a.L[p].hsz, RB), cw);
     else if (cw >= c.n_client_waves - c.n_serial_waves) bank_serve_serial_client<RB, WB>(c, cw);
     else bank_serve_client<RB, WB>(c, cw);
 }
@@ -752,11 +762,12 @@ ServerLayout server_layout_at(const csmv_config &cfg, uint32_t lds_max, uint32_t
     ServerLayout L{};
     L.parts = 1;
     const uint32_t vmode = cfg.variant & 3u;
-    if (delegation && (cfg.variant & kSerialPipe) && R <= 16) {
-        // pipelined serialized lane: the receivers copy the rows the clients staged (no
-        // receiver hash or chain scratch, no stamp buckets); each committer's exchange of
-        // words per chain node, and the install records the publisher takes.  The clients'
-        // staging scratch lives in their own workgroups' LDS, the same size.
+    if (delegation && (cfg.variant & kSerialPipe)) {
+        // pipelined serialized lane: the receivers copy the entry lists the clients staged
+        // (no receiver hash or chain scratch, no stamp buckets); each committer's words per
+        // entry, and the install records the publisher takes.  The clients' staging scratch
+        // lives in their own workgroups' LDS, the same size (a wide wire's clients also
+        // keep their read sets there).
         L.nmb = nmb ? nmb : 16;
         L.mb_rows = kPipeBox / kWave;
         L.state_off = kLdsHdr;
@@ -772,7 +783,8 @@ ServerLayout server_layout_at(const csmv_config &cfg, uint32_t lds_max, uint32_t
         L.pipe_off = w;
         w += pipe_words();
         L.atr_off = L.bucket_off = L.val_off = w;
-        L.words = std::max<uint32_t>(w, cfg.client_wg_waves * pipe_client_words(L.hsz, R, W));
+        L.words = std::max<uint32_t>(w, cfg.client_wg_waves * (R > 16 ? pipe_wide_words(L.hsz, R)
+                                                                      : pipe_client_words(L.hsz, R, W)));
         L.parts = cfg.commit_parts;
         return L;
     }
@@ -834,7 +846,7 @@ ServerLayout server_layout_cfg(const csmv_config &cfg, uint32_t lds_max, uint32_
                                bool delegation, uint32_t nmb = 0)
 {
     const bool stamp = (cfg.variant & 3u) == kValStamp;
-    if (delegation && (cfg.variant & kSerialPipe) && R <= 16)
+    if (delegation && (cfg.variant & kSerialPipe))
         return server_layout_at(cfg, lds_max, R, W, delegation, nmb, 0);
     for (uint32_t hshift = 2; hshift > 0; --hshift) {
         const ServerLayout L = server_layout_at(cfg, lds_max, R, W, delegation, nmb, hshift);
@@ -1316,6 +1328,7 @@ csmv_status csmv_bank_run(csmv_engine *e, csmv_stats *out)
     c.n_norm = e->routed ? e->n_norm : e->spec.n_txns;
     c.n_hot = e->routed ? e->n_hot : 0;
     c.pipe_parts = (e->cfg.variant & kSerialPipe) ? e->cfg.commit_parts : 0;
+    c.pipe_heads = c.n_norm != 0;
     if (c.n_hot && (c.n_serial_waves == 0 || (e->cfg.variant & 3u) != kValStamp ||
                     (e->cfg.variant & kServerWriteback)))
         return fail(e, CSMV_E_ARG, "the serialized lane needs n_serial_waves > 0 and the "
@@ -1979,7 +1992,8 @@ template <int R, int W> csmv_status fit_layout(csmv_engine *e, ServerLayout &L, 
     uint32_t nmb = 0;
     for (;;) {
         L = server_layout<R, W>(e, delegation, nmb);
-        const bool fits = (uint64_t)L.words * 4 <= e->lds_max && (L.exact || L.nbuckets >= 4096);
+        const bool pipe = delegation && (e->cfg.variant & kSerialPipe);
+        const bool fits = (uint64_t)L.words * 4 <= e->lds_max && (pipe || L.exact || L.nbuckets >= 4096);
         if (fits) return CSMV_OK;
         if (L.nmb <= 4)
             return fail(e, CSMV_E_CAPACITY, "serving server layout does not fit the LDS budget");
@@ -2016,8 +2030,10 @@ csmv_status csmv_serve_run_multi(csmv_engine *const *caches, uint32_t n_cache, c
         for (size_t j = 0; j < i; ++j)
             if (parts[i] == parts[j]) return fail(p, CSMV_E_ARG, "an engine serves one partition");
     for (csmv_engine *e : parts) {
-        if ((e->cfg.variant & 3u) != kValStamp || (e->cfg.variant & (kServerWriteback | kSerialPipe)))
-            return fail(e, CSMV_E_ARG, "serving runs the STAMP variant with client write-back");
+        if ((e->cfg.variant & 3u) != kValStamp || (e->cfg.variant & kServerWriteback) ||
+            ((e->cfg.variant & kSerialPipe) && e != bank))
+            return fail(e, CSMV_E_ARG, "serving runs the STAMP variant with client write-back "
+                                       "(the pipelined lane for the bank partition only)");
         if (!e->pool_loaded) return fail(e, CSMV_E_STATE, "serve needs a freshly loaded pool");
         if (e->device != p->device || e->cfg.wg_waves != p->cfg.wg_waves)
             return fail(e, CSMV_E_ARG, "partitions must share the device and wg_waves");
@@ -2069,15 +2085,20 @@ csmv_status csmv_serve_run_multi(csmv_engine *const *caches, uint32_t n_cache, c
         const uint32_t bi = n_cache;
         if ((st = poisson_arrivals(bank, splitmix64(sp->seed * 2 + 2), rate_b, bank->spec.n_txns)))
             return st;
-        const bool serial = bank->routed && bank->n_hot > 0;
+        // the pipelined lane takes every bank transaction (no routing, no normal lane)
+        const bool pipe = (bank->cfg.variant & kSerialPipe) != 0;
+        const bool serial = !pipe && bank->routed && bank->n_hot > 0;
         if (serial && bank->cfg.n_serial_waves == 0)
             return fail(bank, CSMV_E_ARG, "a routed bank partition needs n_serial_waves > 0");
         // escalated transactions travel on the serialized lane too
-        const bool delegation = serial || bank->cfg.escalate_after != 0;
+        const bool delegation = pipe || serial || bank->cfg.escalate_after != 0;
         ServerLayout &L = args.L[bi];
         if ((st = fit_layout<kServeBankR, kBankW>(bank, L, delegation))) return st;
         Ctx &c = args.c[bi];
         base_ctx(bank, c);
+        args.bank_pipe = pipe ? 1u : 0u;
+        c.pipe_parts = pipe ? L.parts : 0u;
+        c.pipe_heads = 0;
         c.n_serial_waves = serial ? bank->cfg.n_serial_waves : 0;
         c.hot_list = serial ? bank->hot_list : nullptr;
         c.n_hot = serial ? bank->n_hot : 0;
@@ -2088,8 +2109,8 @@ csmv_status csmv_serve_run_multi(csmv_engine *const *caches, uint32_t n_cache, c
         c.log_v = bank->log_v;
         c.log_ab = bank->log_ab;
         c.log_lat = bank->log_lat;
-        c.norm_list = bank->routed ? bank->norm_list : nullptr;
-        c.n_norm = bank->routed ? bank->n_norm : bank->spec.n_txns;
+        c.norm_list = (bank->routed && !pipe) ? bank->norm_list : nullptr;
+        c.n_norm = (bank->routed && !pipe) ? bank->n_norm : bank->spec.n_txns;
         c.arr = bank->arr;
         words = std::max(words, L.words);
         wgs += (bank->cfg.client_waves + bank->cfg.client_wg_waves - 1) / bank->cfg.client_wg_waves;
@@ -2104,8 +2125,11 @@ csmv_status csmv_serve_run_multi(csmv_engine *const *caches, uint32_t n_cache, c
     const size_t lds_bytes = (size_t)words * 4;
     if (lds_bytes > p->lds_max) return fail(p, CSMV_E_CAPACITY, "serving LDS layout too large");
     const uint32_t wg = p->cfg.wg_waves;
-    const uint32_t grid = (uint32_t)parts.size() + wgs;
-    auto kern = serve_kernel<kMcR, kMcW, kServeBankR, kBankW>;
+    // a pipelined bank partition has a server workgroup per commit partition
+    const uint32_t bank_srv = args.bank_pipe ? args.L[n_cache].parts : (bank ? 1u : 0u);
+    const uint32_t grid = n_cache + bank_srv + wgs;
+    auto kern = args.bank_pipe ? serve_kernel<kMcR, kMcW, kServeBankR, kBankW, true>
+                               : serve_kernel<kMcR, kMcW, kServeBankR, kBankW, false>;
     HIPCHK(p, hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)lds_bytes));
     int per_cu = 0;
@@ -2120,6 +2144,14 @@ csmv_status csmv_serve_run_multi(csmv_engine *const *caches, uint32_t n_cache, c
     HIPCHK(p, hipEventRecord(p->ev0, p->stream));
     hipLaunchKernelGGL(kern, dim3(grid), dim3(wg * kWave), lds_bytes, p->stream, args);
     HIPCHK(p, hipGetLastError());
+    if (args.bank_pipe) {
+        // no snapshot reader ran beside the pipelined bank lane: its head words follow from
+        // the install counts
+        const uint32_t n = bank->cfg.n_items;
+        hipLaunchKernelGGL(pipe_heads_kernel, dim3((n + 255) / 256), dim3(256), 0, p->stream, bank->sv,
+                           bank->ht, n, bank->cfg.depth);
+        HIPCHK(p, hipGetLastError());
+    }
     HIPCHK(p, hipEventRecord(p->ev1, p->stream));
     HIPCHK(p, hipEventSynchronize(p->ev1));
     float ms = 0.f;

@@ -11,6 +11,7 @@
 #pragma once
 
 #include "csmv_device.hpp"
+#include "csmv_server.hpp"
 
 namespace csmv {
 
@@ -385,6 +386,482 @@ template <int R, int W> __device__ __forceinline__ void bank_serve_serial_client
         }
         serve_release(c, done, held);
         if (has && ab > c.max_attempts) raise_error(c, kErrLivelock);
+        t_round += wall_clock64() - tr0;
+        ++rounds;
+    }
+    if (lane == 0) {
+        fetch_add(&c.prof[kProfClientReply], t_reply);
+        fetch_add(&c.prof[kProfClientRound], t_round);
+        fetch_add(&c.prof[kProfClientRounds], rounds);
+        if (stalls) fetch_add(&c.serve_ctr[0], stalls);
+    }
+    if (n_upd) fetch_add(&c.stats[kStatCommitUpd], n_upd);
+    fence_rel();
+    if (lane == 0) __hip_atomic_fetch_add(c.clients_done, 1u, __ATOMIC_RELEASE,
+                                          __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---- the serving bank class on the pipelined serialized lane --------------------------
+// A serving bank transaction reads 2..R accounts and transfers over the first two.  On the
+// pipelined lane (the bank engine's variant carries kSerialPipe) its nodes are the chain
+// nodes of the transfer pair (as in tx_commit_pipe) plus one leaf per other read: a leaf
+// takes the value its producer (the highest lower lane writing the account) wrote, or the
+// value before the request, which its root entry's atomic returns; a leaf reading the
+// transaction's own write takes that write on the client and sends nothing.  A request
+// carries the lanes, in lane order, whose nodes keep every partition's list within
+// kPipeMaxE; the other lanes wait for the next request.
+//
+// A client wave's LDS scratch (pipe_wide_words): the write hash (key, writer lanes lo / hi,
+// canonical node, request-wide delta, last writer's rank), per chain node (2 per lane) its
+// root, weight, own write delta, superseded flag and entry, the staged lists (a block of
+// kPipeMaxE per partition: items, weights, meta words), the read sets (R rows of 64) and
+// each read's (partition << 16 | entry).
+__host__ __device__ constexpr uint32_t pipe_wide_words(uint32_t hsz, uint32_t R)
+{
+    return 6 * hsz + 5 * 2 * kWave + 3 * kPipeParts * kPipeMaxE + 2 * R * kWave;
+}
+
+template <int R, int W>
+__device__ __forceinline__ uint32_t tx_commit_pipe_wide(const Ctx &c, const ServerLayout &L,
+                                                        uint32_t *scr, uint32_t cw, bool req,
+                                                        uint64_t k, uint32_t op, bool *fail,
+                                                        uint64_t *wait_ticks)
+{
+    static_assert(W == 2, "the pipelined lane carries transfers");
+    constexpr int CN = 2;   // chain nodes per lane: the transfer pair
+    const uint32_t lane = lane_id();
+    const uint64_t lt = (1ull << lane) - 1;
+    const uint64_t T = c.n_txns;
+    const uint32_t hsz = L.hsz, hmask = hsz - 1, P = L.parts;
+    uint32_t *hkey = scr, *hwlo = scr + hsz, *hwhi = scr + 2 * hsz, *canon = scr + 3 * hsz;
+    int32_t *htot = (int32_t *)(scr + 4 * hsz);
+    uint32_t *hrank = scr + 5 * hsz;
+    uint32_t *snext = scr + 6 * hsz;                    // [64 * CN] root node
+    int32_t *sw = (int32_t *)(snext + kWave * CN);      // [64 * CN] weight from the root
+    int32_t *sdel = sw + kWave * CN;                    // [64 * CN] the node's own write delta
+    uint32_t *ssup = (uint32_t *)(sdel + kWave * CN);   // [64 * CN] superseded flags
+    uint32_t *e2 = ssup + kWave * CN;                   // [64 * CN] entry of each chain node
+    uint32_t *sitem = e2 + kWave * CN;                  // [kPipeParts][kPipeMaxE]
+    uint32_t *swgt = sitem + kPipeParts * kPipeMaxE;
+    uint32_t *smeta = swgt + kPipeParts * kPipeMaxE;
+    uint32_t *rsx = smeta + kPipeParts * kPipeMaxE;     // [R][64] read sets
+    uint32_t *le = rsx + R * kWave;                     // [R][64] (partition << 16 | entry)
+    const uint32_t mybit = 1u << (lane & 31);
+
+    // 1. the read sets into LDS (16 rows a round; a row past a lane's reads loads its last
+    //    read again, and a lane without a transaction row 0 of transaction 0, so no load
+    //    leaves the stream; both keep kNone)
+    const uint32_t nr = req ? min((op >> 16) & 0xFFu, (uint32_t)R) : 0u;
+    const uint32_t amt = (op >> 8) & 0xFFu;
+    const uint64_t kk = req ? k : 0;
+#pragma unroll 1
+    for (uint32_t j0 = 0; j0 < (uint32_t)R; j0 += 16) {
+        if (ballot(j0 < nr) == 0) break;
+        uint32_t v[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) v[j] = c.addr[(uint64_t)min(j0 + j, nr ? nr - 1 : 0u) * T + kk];
+        pin_round(v);
+#pragma unroll
+        for (int j = 0; j < 16; ++j) rsx[(j0 + j) * kWave + lane] = j0 + j < nr ? v[j] : kNone;
+    }
+    wave_fence();
+    const uint32_t a0 = nr ? rsx[lane] : kNone, a1 = nr > 1 ? rsx[kWave + lane] : kNone;
+
+    // 2. the lanes this request carries: a lane-order prefix whose nodes keep every
+    //    partition's list within kPipeMaxE (a leaf on an own write is no node)
+    uint32_t cq[kPipeParts] = {0u, 0u, 0u, 0u};
+    for (uint32_t j = 0; j < nr; ++j) {
+        const uint32_t x = rsx[j * kWave + lane];
+        if (j >= CN && (x == a0 || x == a1)) continue;
+        const uint32_t q = pipe_part(x, P);
+#pragma unroll
+        for (uint32_t t = 0; t < kPipeParts; ++t) cq[t] += q == t ? 1u : 0u;
+    }
+    // (every lane takes part in the scans: a lane-prefix scan reads the lanes before it)
+    uint32_t pre[kPipeParts];
+#pragma unroll
+    for (uint32_t t = 0; t < kPipeParts; ++t) pre[t] = wave_incl_scan(cq[t]);
+    bool on = req;
+#pragma unroll
+    for (uint32_t t = 0; t < kPipeParts; ++t) on = on && pre[t] <= kPipeMaxE;
+
+    // 3. the write set into the hash (item -> writer lanes)
+    uint32_t wa[W];
+    wa[0] = on ? a0 : kNone;
+    wa[1] = (on && a1 != a0) ? a1 : kNone;
+    const int32_t d0 = (wa[1] == kNone) ? 0 : -(int32_t)amt, d1 = (int32_t)amt;
+    uint32_t slot_w[W], hp[W];
+    bool pend[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        slot_w[w] = kNone;
+        hp[w] = mix32(wa[w]) & hmask;
+        pend[w] = wa[w] != kNone;
+    }
+    for (uint32_t step = 0; step <= hmask; ++step) {
+        uint32_t old[W];
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            old[w] = kNone;
+            __hip_atomic_compare_exchange_strong(&hkey[pend[w] ? hp[w] : 0u], &old[w],
+                                                 pend[w] ? wa[w] : kNone, __ATOMIC_RELAXED,
+                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+        bool more = false;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            const bool got = pend[w] && (old[w] == kNone || old[w] == wa[w]);
+            slot_w[w] = got ? hp[w] : slot_w[w];
+            hp[w] = (pend[w] && !got) ? ((hp[w] + 1) & hmask) : hp[w];
+            pend[w] = pend[w] && !got;
+            more |= pend[w];
+        }
+        if (ballot(more) == 0) break;
+    }
+#pragma unroll
+    for (int w = 0; w < W; ++w)
+        if (slot_w[w] != kNone) atomicOr(lane < 32 ? &hwlo[slot_w[w]] : &hwhi[slot_w[w]], mybit);
+    wave_fence();
+
+    // looks an account up in the write hash: its slot, or kNone when no lane writes it
+    auto find = [&](uint32_t x) -> uint32_t {
+        uint32_t hs = mix32(x) & hmask, fs = kNone;
+        bool fp = x != kNone;
+        for (uint32_t step = 0; step <= hmask; ++step) {
+            const uint32_t kx = __hip_atomic_load(&hkey[fp ? hs : 0u], __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_WORKGROUP);
+            const bool hit = fp && kx == x;
+            const bool next = fp && kx != x && kx != kNone;
+            fs = hit ? hs : fs;
+            hs = next ? ((hs + 1) & hmask) : hs;
+            fp = next;
+            if (ballot(next) == 0) break;
+        }
+        return fs;
+    };
+    // the producer of a read of x (hash slot fs): the highest lower lane writing x and the
+    // write slot (lane | slot << 6), 0xFF = none
+    auto producer = [&](uint32_t x, uint32_t fs) -> uint32_t {
+        const uint32_t at = fs != kNone ? fs : 0u;
+        const uint64_t m = ((uint64_t)__hip_atomic_load(&hwhi[at], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) << 32) |
+                           __hip_atomic_load(&hwlo[at], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        const uint64_t lower = fs != kNone ? (m & lt) : 0ull;
+        const uint32_t pl = lower ? 63u - (uint32_t)__builtin_clzll(lower) : 0xFFu;
+        const uint32_t srcl = pl == 0xFFu ? lane : pl;
+        uint32_t code = 0xFFu;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            const uint32_t pw = lane_gather(wa[w], srcl);
+            code = (pl != 0xFFu && pw == x) ? (pl | ((uint32_t)w << 6)) : code;
+        }
+        return code;
+    };
+
+    // 4. the chain nodes (the transfer pair): producers, canonical nodes, chain sums by
+    //    pointer jumping, superseded writes, each written item's request-wide delta and
+    //    last writer's rank (as tx_commit_pipe, over two nodes a lane)
+    uint32_t ops[CN], fs[CN], pj[CN];
+    ops[0] = on ? a0 : kNone;
+    ops[1] = on ? a1 : kNone;
+#pragma unroll
+    for (int r = 0; r < CN; ++r) {
+        fs[r] = find(ops[r]);
+        pj[r] = producer(ops[r], fs[r]);
+    }
+#pragma unroll
+    for (int r = 0; r < CN; ++r)
+        if (fs[r] != kNone && pj[r] == 0xFFu) atomicMin(&canon[fs[r]], lane * CN + (uint32_t)r);
+    uint32_t nxt[CN];
+    int32_t wgt[CN];
+#pragma unroll
+    for (int r = 0; r < CN; ++r) {
+        const uint32_t node = lane * CN + r;
+        const bool has_p = pj[r] != 0xFFu;
+        const uint32_t srcl = has_p ? (pj[r] & 63u) : lane;
+        const int32_t q0 = (int32_t)lane_gather((uint32_t)d0, srcl),
+                      q1 = (int32_t)lane_gather((uint32_t)d1, srcl);
+        nxt[r] = has_p ? (pj[r] & 63u) * CN + (pj[r] >> 6) : node;
+        wgt[r] = has_p ? ((pj[r] >> 6) == 0 ? q0 : q1) : 0;
+        snext[node] = nxt[r];
+        sw[node] = wgt[r];
+        ssup[node] = 0;
+    }
+    wave_fence();
+    for (uint32_t round = 0; round < 8; ++round) {
+        bool more = false;
+        uint32_t nn[CN];
+        int32_t ww[CN];
+#pragma unroll
+        for (int r = 0; r < CN; ++r) {
+            const uint32_t node = lane * CN + r;
+            const uint32_t t = nxt[r];
+            const uint32_t tt = snext[t];
+            ww[r] = wgt[r] + (t != node ? sw[t] : 0);
+            nn[r] = tt;
+            more |= tt != t;
+        }
+        wave_fence();
+#pragma unroll
+        for (int r = 0; r < CN; ++r) {
+            nxt[r] = nn[r];
+            wgt[r] = ww[r];
+            snext[lane * CN + r] = nn[r];
+            sw[lane * CN + r] = ww[r];
+        }
+        wave_fence();
+        if (ballot(more) == 0) break;
+    }
+#pragma unroll
+    for (int r = 0; r < CN; ++r) {
+        bool mine_w = false;
+#pragma unroll
+        for (int w = 0; w < W; ++w) mine_w |= wa[w] == ops[r] && ops[r] != kNone;
+        if (pj[r] != 0xFFu && mine_w) ssup[(pj[r] & 63u) * CN + (pj[r] >> 6)] = 1u;
+    }
+    wave_fence();
+    uint32_t supf = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) supf |= ssup[lane * CN + w] ? (1u << w) : 0u;
+    const uint64_t umask = ballot(on);
+    const uint32_t rank = (uint32_t)__builtin_popcountll(umask & lt);
+#pragma unroll
+    for (int w = 0; w < W; ++w)
+        if (wa[w] != kNone && ((supf >> w) & 1u) == 0) {
+            htot[slot_w[w]] = wgt[w] + (w == 0 ? d0 : d1);
+            hrank[slot_w[w]] = rank;
+        }
+    wave_fence();
+
+    // 5. the entries, in rounds of one node a lane (the chain nodes first, then the
+    //    leaves): entry e of partition q's list is staged at q * kPipeMaxE + e
+    uint32_t cnt[kPipeParts] = {0u, 0u, 0u, 0u};
+    auto place = [&](bool node_on, uint32_t x) -> uint32_t {
+        const uint32_t q = node_on ? pipe_part(x, P) : kNone;
+        uint32_t e = 0;
+#pragma unroll
+        for (uint32_t t = 0; t < kPipeParts; ++t) {
+            const uint64_t mq = ballot(q == t);
+            if (q == t) e = cnt[t] + (uint32_t)__builtin_popcountll(mq & lt);
+            cnt[t] += (uint32_t)__builtin_popcountll(mq);
+        }
+        return node_on ? (q << 16) | e : kNone;
+    };
+    uint32_t ce[CN];
+#pragma unroll
+    for (int r = 0; r < CN; ++r) {
+        ce[r] = place(ops[r] != kNone, ops[r]);
+        e2[lane * CN + r] = ce[r] & 0xFFFFu;
+        le[r * kWave + lane] = ce[r];
+    }
+    wave_fence();
+#pragma unroll
+    for (int r = 0; r < CN; ++r) {
+        const uint32_t node = lane * CN + r;
+        const bool written = fs[r] != kNone;
+        const uint32_t at = written ? fs[r] : 0u;
+        const uint32_t cn = canon[at];
+        const uint32_t rt = written ? cn : nxt[r];
+        const bool is_canon = written && cn == node;
+        const bool is_root = rt == node;
+        snext[node] = rt;               // for the leaves: root and weight from it
+        sw[node] = wgt[r];
+        sdel[node] = r == 0 ? d0 : d1;
+        const uint32_t meta = e2[rt & (kWave * CN - 1)] | (is_root ? kPipeRootBit : 0u) |
+                              (is_canon ? kPipeInstBit | (hrank[at] << kPipeRankShift) : 0u);
+        const int32_t wout = is_canon ? htot[at] : wgt[r];
+        if (ce[r] != kNone) {
+            const uint32_t o = (ce[r] >> 16) * kPipeMaxE + (ce[r] & 0xFFFFu);
+            sitem[o] = ops[r];
+            swgt[o] = (uint32_t)wout;
+            smeta[o] = meta;
+        }
+    }
+    wave_fence();
+#pragma unroll 1
+    for (uint32_t j = CN; j < (uint32_t)R; ++j) {
+        const uint32_t x = j < nr ? rsx[j * kWave + lane] : kNone;
+        const bool leaf = on && x != kNone && x != a0 && x != a1;
+        if (ballot(leaf) == 0) {
+            if (ballot(j < nr) == 0) break;
+            le[j * kWave + lane] = kNone;
+            continue;
+        }
+        const uint32_t f = find(leaf ? x : kNone);
+        const uint32_t pc = producer(leaf ? x : kNone, f);
+        const uint32_t pe = place(leaf, x);
+        uint32_t root_e = pe & 0xFFFFu, meta_root = kPipeRootBit;
+        int32_t lw = 0;
+        if (pc != 0xFFu) {
+            // the producer's write: its node's value plus its delta
+            const uint32_t pn = (pc & 63u) * CN + (pc >> 6);
+            root_e = e2[snext[pn] & (kWave * CN - 1)];
+            lw = sw[pn] + sdel[pn];
+            meta_root = 0;
+        } else if (f != kNone) {
+            // written only by higher lanes: the value before the request
+            root_e = e2[canon[f] & (kWave * CN - 1)];
+            meta_root = 0;
+        }
+        le[j * kWave + lane] = leaf ? pe : kNone;
+        if (leaf) {
+            const uint32_t o = (pe >> 16) * kPipeMaxE + (pe & 0xFFFFu);
+            sitem[o] = x;
+            swgt[o] = (uint32_t)lw;
+            smeta[o] = root_e | meta_root;
+        }
+    }
+    wave_fence();
+
+    // 6. the lists into the boxes a row at a time, the headers, then the hash slots back to
+    //    empty; send once the box stores are complete (no release fence: it would write
+    //    back the XCD's dirty L2 lines)
+    uint32_t *bx = c.mbox + (size_t)cw * kPipeParts * kPipeBox;
+    const uint32_t nc = (uint32_t)__builtin_popcountll(umask);
+    uint32_t chunks = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < kPipeParts; ++q) {
+        chunks |= ((cnt[q] + kPipeChunk - 1) / kPipeChunk) << (kPipeChunkShift + 4 * q);
+        if (q >= P) continue;
+        uint32_t *b = bx + q * kPipeBox;
+#pragma unroll
+        for (uint32_t g = 0; g < kPipeMaxE / kWave; ++g) {
+            const uint32_t x = g * kWave + lane;
+            if (x < cnt[q]) {
+                st_rlx(&b[kPipeItem + x], sitem[q * kPipeMaxE + x]);
+                st_rlx(&b[kPipeWgt + x], swgt[q * kPipeMaxE + x]);
+                st_rlx(&b[kPipeMeta + x], smeta[q * kPipeMaxE + x]);
+            }
+        }
+        if (lane == 0) st_rlx(&b[0], cnt[q] | (nc << 16));
+    }
+    wave_fence();
+#pragma unroll
+    for (int w = 0; w < W; ++w)
+        if (slot_w[w] != kNone) {
+            hkey[slot_w[w]] = kNone;
+            hwlo[slot_w[w]] = 0;
+            hwhi[slot_w[w]] = 0;
+            canon[slot_w[w]] = kNone;
+            htot[slot_w[w]] = 0;
+        }
+    wave_fence();
+    if (nc == 0) return 0u;
+    wait_stores();
+    uint64_t n = 0;
+    if (lane == 0) {
+        n = fetch_add(c.req_tail, (uint64_t)1);
+        st_rlx(&c.ring[n & c.qmask], ((n + 1) << 32) | cw | chunks | kDelegateBit);
+    }
+    n = bcast64(n);
+
+    // 7. every partition's header word (tag and the request's first timestamp), then the
+    //    value of each of the lane's nodes
+    const uint64_t tw0 = wall_clock64();
+    Spin sp;
+    const uint64_t *hdr = c.reply + (size_t)cw * kWave;
+    uint64_t h0 = 0;
+    for (uint32_t pt = 0; pt < P; ++pt) {
+        uint64_t w0 = 0;
+        for (;;) {
+            if (lane == 0) w0 = ld_rlx(&hdr[pt]);
+            w0 = bcast64(w0);
+            if ((w0 >> 32) == n + 1) break;
+            if (!sp.tick(c, kErrReplyTimeout)) { *fail = true; return 0; }
+        }
+        h0 = pt == 0 ? w0 : h0;
+    }
+    *wait_ticks += wall_clock64() - tw0;
+    const uint32_t *rv = (const uint32_t *)c.reply_v + (size_t)cw * kPipeParts * kPipeMaxE;
+
+    // 8. the transaction in program order: read a, write a - amt, read b (own write if
+    //    b == a), write b + amt, then the other reads (own writes first); the log
+    const uint32_t ts = (uint32_t)h0 + rank + 1;
+    if (on) {
+        const int32_t x0 = (int32_t)ld_rlx(&rv[(ce[0] >> 16) * kPipeMaxE + (ce[0] & 0xFFFFu)]);
+        const int32_t w0v = (int32_t)((uint32_t)x0 - amt);
+        const int32_t x1 = a1 == a0 ? w0v : (int32_t)ld_rlx(&rv[(ce[1] >> 16) * kPipeMaxE + (ce[1] & 0xFFFFu)]);
+        const int32_t w1v = (int32_t)((uint32_t)x1 + amt);
+        const int32_t va0 = a1 == a0 ? w1v : w0v;   // a's value after the transfer
+        c.log_v[k] = x0;
+        c.log_v[T + k] = x1;
+        for (uint32_t j = CN; j < nr; ++j) {
+            const uint32_t x = rsx[j * kWave + lane];
+            const uint32_t p = le[j * kWave + lane];
+            int32_t v;
+            if (x == a1) v = w1v;
+            else if (x == a0) v = va0;
+            else v = (int32_t)ld_rlx(&rv[(p >> 16) * kPipeMaxE + (p & 0xFFFFu)]);
+            c.log_v[(uint64_t)j * T + k] = v;
+        }
+        c.log_ts[k] = ts;
+    }
+    return on ? ts : 0u;
+}
+
+// Serving-loop bank transactions on the pipelined lane: every bank wave takes
+// transactions in arrival order, admits them by version-pool bytes and sends the lanes a
+// request can carry; the commit servers execute them in commit order (no transaction
+// aborts) and the wave logs them with their latency from arrival to commit.
+template <int R, int W>
+__device__ __forceinline__ void bank_serve_pipe_client(const Ctx &c, const ServerLayout &L, uint32_t *scr,
+                                                       uint32_t cw)
+{
+    const uint32_t lane = lane_id();
+    const uint64_t NT = c.n_norm;
+    const uint64_t t0 = serve_start(c);
+    for (uint32_t i = lane; i < L.hsz; i += kWave) {
+        scr[i] = kNone;
+        scr[L.hsz + i] = 0;
+        scr[2 * L.hsz + i] = 0;
+        scr[3 * L.hsz + i] = kNone;
+        scr[4 * L.hsz + i] = 0;
+    }
+    wave_fence();
+    bool has = false, adm = false, exhausted = false;
+    uint64_t k = 0;
+    uint32_t op = 0, held = 0;
+    uint64_t n_upd = 0, stalls = 0, t_reply = 0, t_round = 0, rounds = 0;
+    for (;;) {
+        if (ld_rlx(c.err)) break;
+        const uint64_t tr0 = wall_clock64();
+        if (!exhausted) {
+            const uint64_t need = ballot(!has);
+            if (need) {
+                const uint32_t cnt = (uint32_t)__builtin_popcountll(need);
+                uint64_t base = 0;
+                if (lane == 0) base = fetch_add(c.next_txn, (uint64_t)cnt);
+                base = bcast64(base);
+                if (base + cnt >= NT) exhausted = true;
+                if (!has) {
+                    const uint64_t mine = base + prefix_count(need);
+                    if (mine < NT) {
+                        has = true;
+                        adm = false;
+                        k = c.norm_list ? c.norm_list[mine] : mine;
+                        op = c.op[k];
+                    }
+                }
+            }
+        }
+        if (ballot(has) == 0) break;
+        serve_gate(c, t0, has, adm, k, W * kVersionBytes, held, stalls);
+        const bool act = has && adm;
+        if (ballot(act) == 0) {
+            __builtin_amdgcn_s_sleep(2);
+            continue;
+        }
+        bool fail = false;
+        const uint32_t ts = tx_commit_pipe_wide<R, W>(c, L, scr, cw, act, k, op, &fail, &t_reply);
+        if (fail) break;
+        bool done = false;
+        if (act && ts) {
+            c.log_ab[k] = 0;
+            c.log_lat[k] = (uint32_t)(wall_clock64() - t0 - c.arr[k]);
+            done = true;
+            has = false;
+            ++n_upd;
+        }
+        serve_release(c, done, held);
         t_round += wall_clock64() - tr0;
         ++rounds;
     }

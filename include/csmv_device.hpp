@@ -294,6 +294,8 @@ struct Ctx {
     uint64_t *sink;
     uint32_t *pipe_prog;   // [commit partitions] timestamp each has published through
     uint32_t pipe_parts;   // commit partitions (pipelined lane; 0 elsewhere)
+    uint32_t pipe_heads;   // pipelined lane: 1 = snapshot readers run beside it, so the head
+                           // words move during the run (0: set from the install counts after)
 };
 constexpr uint32_t kSinkWords = 32768;
 

@@ -2001,13 +2001,12 @@ __device__ __forceinline__ uint32_t pipe_snapshot(const Ctx &c)
 // masked).
 template <int R, int W>
 __device__ __forceinline__ void committer_pipe(const Ctx &c, const ServerLayout &L, uint32_t *lds,
-                                               uint32_t k, uint32_t K)
+                                               uint32_t part, uint32_t k, uint32_t K)
 {
     constexpr int G = kPipeMaxE / kWave;
     const uint32_t lane = lane_id();
     const uint32_t D = c.depth;
     const bool dpow2 = (D & (D - 1)) == 0;
-    const uint32_t part = blockIdx.x;
     uint32_t *mbst = lds + L.state_off;
     uint32_t *mbwave = lds + L.wave_off;
     uint32_t *mbfree = lds + L.tg_off;                      // [nmb] 1 + request finished in it
@@ -2220,17 +2219,17 @@ __device__ __forceinline__ void committer_pipe(const Ctx &c, const ServerLayout 
 // partitions, so a reader's snapshot only ever covers whole requests whose versions are in
 // place everywhere.
 template <int R, int W>
-__device__ __forceinline__ void publisher_pipe(const Ctx &c, const ServerLayout &L, uint32_t *lds)
+__device__ __forceinline__ void publisher_pipe(const Ctx &c, const ServerLayout &L, uint32_t *lds,
+                                               uint32_t part)
 {
     constexpr int G = kPipeMaxE / kWave;
     const uint32_t lane = lane_id();
-    const uint32_t part = blockIdx.x;
     uint32_t *irec = lds + L.pipe_off;
     uint32_t *icnt = irec + kPipeRing * 2 * kPipeMaxE;
     uint32_t *ilast = icnt + kPipeRing;
     uint32_t *iready = ilast + kPipeRing;
     uint64_t *const probe_at = c.sink + (size_t)(part * 16 + 15) * 16;
-    const bool heads = c.n_norm != 0;
+    const bool heads = c.pipe_heads != 0;
     uint32_t q = 0, gc = 0;
     bool bad = false;
     LaneSum p_batches, p_wait, p_busy;
@@ -2317,11 +2316,10 @@ __device__ __forceinline__ void publisher_pipe(const Ctx &c, const ServerLayout 
 // flight: up to two requests per memory round trip.
 template <int R, int W>
 __device__ __forceinline__ void receiver_pipe(const Ctx &c, const ServerLayout &L, uint32_t *lds,
-                                              uint32_t rid, uint32_t M)
+                                              uint32_t part, uint32_t rid, uint32_t M)
 {
     constexpr int G = kPipeMaxE / kWave;
     const uint32_t lane = lane_id();
-    const uint32_t part = blockIdx.x;
     uint32_t *mbst = lds + L.state_off;
     uint32_t *mbwave = lds + L.wave_off;
     uint32_t *mbfree = lds + L.tg_off;
@@ -2435,10 +2433,12 @@ __device__ __forceinline__ void receiver_pipe(const Ctx &c, const ServerLayout &
         }
 }
 
-// The server workgroup of the pipelined lane: K committers (pipe_committers), one
-// publisher (it moves the clock: no advancer), the other waves receive.
+// The server workgroup of commit partition `part` of the pipelined lane: K committers
+// (pipe_committers), one publisher (it moves the clock: no advancer), the other waves
+// receive.
 template <int R, int W>
-__device__ __forceinline__ void pipe_server_workgroup(const Ctx &c, const ServerLayout &L, uint32_t *lds)
+__device__ __forceinline__ void pipe_server_workgroup(const Ctx &c, const ServerLayout &L, uint32_t *lds,
+                                                      uint32_t part)
 {
     const uint32_t wave = threadIdx.x / kWave;
     for (uint32_t i = threadIdx.x; i < L.mb_off; i += blockDim.x) lds[i] = 0;
@@ -2446,9 +2446,9 @@ __device__ __forceinline__ void pipe_server_workgroup(const Ctx &c, const Server
     for (uint32_t i = threadIdx.x; i < pipe_words(); i += blockDim.x) lds[L.pipe_off + i] = 0;
     __syncthreads();
     const uint32_t K = pipe_committers(c.wg_waves);
-    if (wave < K) committer_pipe<R, W>(c, L, lds, wave, K);
-    else if (wave == K) publisher_pipe<R, W>(c, L, lds);
-    else receiver_pipe<R, W>(c, L, lds, wave - K - 1, c.wg_waves - K - 1);
+    if (wave < K) committer_pipe<R, W>(c, L, lds, part, wave, K);
+    else if (wave == K) publisher_pipe<R, W>(c, L, lds, part);
+    else receiver_pipe<R, W>(c, L, lds, part, wave - K - 1, c.wg_waves - K - 1);
 }
 
 // D: the serialized lane (delegated requests) is compiled in.

@@ -217,6 +217,41 @@ def test_serve_serialized_lane_for_routed_bank_transactions():
     assert res[True][2] < res[False][2]
 
 
+@pytest.mark.parametrize("n_acc,parts,n_txns,cpw", [(1 << 16, 1, 1 << 13, 1), (1 << 16, 2, 1 << 13, 2),
+                                                 (1 << 20, 4, 1 << 14, 1), (1 << 16, 3, 777, 1)])
+def test_serve_pipelined_bank_lane(n_acc, parts, n_txns, cpw):
+    """The serving mix's bank partition on the pipelined serialized lane (its accounts split
+    over `parts` commit server workgroups; each request carries the lanes whose nodes fit
+    every partition's list): both partitions replay exactly, nothing aborts, every
+    transaction commits once after its arrival and the budget is respected."""
+    m = csmv.Memcd(n_reqs=1 << 14, seed=41, n_keys=20_000, n_sets=1 << 10, batch=4096)
+    b = serve_bank(n_txns, seed=43)
+    budget = 8192
+    print(f"\n[case] serve mix, pipelined bank lane: {n_acc} accounts, {parts} partitions, "
+          f"{n_txns} txns, client_wg_waves={cpw}", flush=True)
+    with cache_engine(m) as ec, csmv.Engine(n_acc, depth=8, client_waves=8, wg_waves=6,
+                                            variant="stamp_pipe", timeout_ms=8000,
+                                            n_serial_waves=4, commit_parts=parts,
+                                            client_wg_waves=cpw) as eb:
+        ec.fill(0)
+        ec.memcd_generate(m)
+        eb.fill(INIT)
+        eb.bank_generate(b)
+        out = csmv.serve(ec, eb, offered_rps=40e6, pct_cache=80, seed=13, pool_budget=budget)
+        print("[serve]", {k: out[k] for k in ("kernel_ms", "admit_stalls", "peak_inflight")},
+              flush=True)
+        assert out["final_inflight"] == 0
+        assert 0 < out["peak_inflight"] <= budget
+        check_cache(ec, m)
+        check_bank(eb, b, n_acc)
+        _, _, ab = eb.bank_read_log()
+        assert ab.sum() == 0 and out["bank"]["aborts"] == 0
+        _, lb = check_latency(eb, out["bank"], b.n_txns)
+        khz = out["wallclock_khz"]
+        print(f"[latency] bank p50 {np.percentile(lb, 50) / khz * 1e3:.1f} us p99 "
+              f"{np.percentile(lb, 99) / khz * 1e3:.1f} us", flush=True)
+
+
 def test_serve_at_config_shape():
     """The serving configuration's shape: the cache configuration (2^17 sets x 8 ways,
     10M-key Zipf 0.99 keyspace, 95/5 GET/SET) beside a 2^20-account Zipf 0.99 bank
