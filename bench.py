@@ -26,8 +26,8 @@ Beside it, in the same JSON line:
              learned routing vs routing every transfer
   serving    the persistent serving loop: Poisson arrivals of cache requests (80 %) and
              bank transactions of 2..64 reads over 1M Zipf 0.99 accounts (20 %) at fixed
-             offered loads, p50/p99 latency from arrival to commit, with and without the
-             predictor
+             offered loads, p50/p99 latency from arrival to commit: the bank class on one
+             commit server with and without the predictor, and on the pipelined lane
   roofline / atomics / cpu_baseline   for the headline kernel
 
 Prints ONE JSON line.  One GPU only: --gpus above 1 exits with a message.
@@ -377,8 +377,12 @@ def serving_section(csmv, args, params, tau):
                        f"stream, cold cache; 80 %) split by set over {P} commit servers of "
                        f"{args.serve_cache_waves} client waves each, + {b.n_txns} bank "
                        "transactions (transfer over the first 2 of 2..64 Zipf 0.99 reads of 1M "
-                       "accounts; 20 %) on one commit server with 16 client waves (8 "
-                       "serialized-lane when routed), Poisson arrivals",
+                       "accounts; 20 %), Poisson arrivals; modes without_predictor / "
+                       "with_predictor: the bank class on one commit server with 16 client "
+                       "waves (8 serialized-lane when routed, escalation after "
+                       f"{args.escalate_after} failed attempts); mode pipelined_bank_lane: "
+                       "the bank class on the pipelined serialized lane, 4 commit server "
+                       f"workgroups, {args.serve_pipe_waves} client waves",
            "cache_partitions": P,
            "pool_budget_bytes": args.serve_budget,
            "bank_escalate_after": args.escalate_after,
@@ -415,54 +419,69 @@ def serving_section(csmv, args, params, tau):
             n_hot = sched.route(eb, tau)
             bop, _ = eb.bank_read_stream()
             bank_bytes = int((12 * ((bop >> 16) & 0xFF).astype(np.int64) + 36).sum())
+            def measure(e_bank, mode, load, budget, routed):
+                """One serving run at one offered load: the row of the JSON line."""
+                for ec in caches:
+                    ec.fill(0)
+                e_bank.fill(INIT)
+                out = csmv.serve(caches, e_bank, offered_rps=load, pct_cache=80, seed=1,
+                                 pool_budget=budget)
+                khz = out["wallclock_khz"]
+                lc = np.concatenate([ec.serve_latency() for ec in caches])
+                ac = np.concatenate([ec.serve_arrivals() for ec in caches])
+                lb, ab = e_bank.serve_latency(), e_bank.serve_arrivals()
+                nc_, wc = class_rate(ac, lc)
+                nb_, wb = class_rate(ab, lb)
+                _, wall = class_rate(np.concatenate([ac, ab]), np.concatenate([lc, lb]))
+                # algorithmic bytes: cache requests as cache_bytes, bank transactions
+                # 12 R + 12 W + 12 (R = 2..64 reads, W = 2)
+                nbytes = bank_bytes + sum(cache_bytes(ec.memcd_read_stream()[1],
+                                                      ec.memcd_read_log()["res"])
+                                          for ec in caches)
+                row = {"offered_rps": load, "kernel_ms": out["kernel_ms"],
+                       "offered_window_ms": float(max(ac.max(), ab.max())) / khz,
+                       "req_per_s": (nc_ + nb_) / (wall / (khz * 1e3)),
+                       "cache_req_per_s": nc_ / (wc / (khz * 1e3)),
+                       "bank_txn_per_s": nb_ / (wb / (khz * 1e3)),
+                       "req_per_s_99": class_rate99(np.concatenate([ac, ab]),
+                                                    np.concatenate([lc, lb]), khz),
+                       "cache_req_per_s_99": class_rate99(ac, lc, khz),
+                       "bank_txn_per_s_99": class_rate99(ab, lb, khz),
+                       "cache_offered_rps": 0.8 * load, "bank_offered_rps": 0.2 * load,
+                       "all": pct_us(np.concatenate([lc, lb]), khz), "cache": pct_us(lc, khz),
+                       "bank": pct_us(lb, khz),
+                       "cache_abort_rate": out["cache"]["abort_rate"],
+                       "bank_abort_rate": out["bank"]["abort_rate"],
+                       "bank_routed": routed,
+                       "bank_escalated": out["bank"]["escalated"],
+                       "admit_stalls": out["admit_stalls"],
+                       "pool_budget_bytes": budget,
+                       "peak_inflight_bytes": out["peak_inflight"],
+                       "roofline": roofline(nbytes, out["kernel_ms"])}
+                res["loads"].setdefault(mode, []).append(row)
+                log(f"serving {mode} {load:.3g} req/s: {row['req_per_s']:.3g} served (cache "
+                    f"{row['cache_req_per_s']:.3g} / {row['cache_req_per_s_99']:.3g} to 99 %, "
+                    f"bank {row['bank_txn_per_s']:.3g}), p99 "
+                    f"{row['all']['p99_us']:.1f} us (cache {row['cache']['p99_us']:.1f}, bank "
+                    f"{row['bank']['p99_us']:.1f}), stalls {row['admit_stalls']}")
+
             for mode in ("without_predictor", "with_predictor"):
                 for load in loads:
                     if mode == "without_predictor":
                         eb.bank_route(None)
                     else:
                         sched.route(eb, tau)
-                    for ec in caches:
-                        ec.fill(0)
-                    eb.fill(INIT)
-                    out = csmv.serve(caches, eb, offered_rps=load, pct_cache=80, seed=1,
-                                     pool_budget=args.serve_budget)
-                    khz = out["wallclock_khz"]
-                    lc = np.concatenate([ec.serve_latency() for ec in caches])
-                    ac = np.concatenate([ec.serve_arrivals() for ec in caches])
-                    lb, ab = eb.serve_latency(), eb.serve_arrivals()
-                    nc_, wc = class_rate(ac, lc)
-                    nb_, wb = class_rate(ab, lb)
-                    _, wall = class_rate(np.concatenate([ac, ab]), np.concatenate([lc, lb]))
-                    # algorithmic bytes: cache requests as cache_bytes, bank transactions
-                    # 12 R + 12 W + 12 (R = 2..64 reads, W = 2)
-                    nbytes = bank_bytes + sum(cache_bytes(ec.memcd_read_stream()[1],
-                                                          ec.memcd_read_log()["res"])
-                                              for ec in caches)
-                    row = {"offered_rps": load, "kernel_ms": out["kernel_ms"],
-                           "offered_window_ms": float(max(ac.max(), ab.max())) / khz,
-                           "req_per_s": (nc_ + nb_) / (wall / (khz * 1e3)),
-                           "cache_req_per_s": nc_ / (wc / (khz * 1e3)),
-                           "bank_txn_per_s": nb_ / (wb / (khz * 1e3)),
-                           "req_per_s_99": class_rate99(np.concatenate([ac, ab]),
-                                                        np.concatenate([lc, lb]), khz),
-                           "cache_req_per_s_99": class_rate99(ac, lc, khz),
-                           "bank_txn_per_s_99": class_rate99(ab, lb, khz),
-                           "cache_offered_rps": 0.8 * load, "bank_offered_rps": 0.2 * load,
-                           "all": pct_us(np.concatenate([lc, lb]), khz), "cache": pct_us(lc, khz),
-                           "bank": pct_us(lb, khz),
-                           "cache_abort_rate": out["cache"]["abort_rate"],
-                           "bank_abort_rate": out["bank"]["abort_rate"],
-                           "bank_routed": n_hot if mode == "with_predictor" else 0,
-                           "bank_escalated": out["bank"]["escalated"],
-                           "admit_stalls": out["admit_stalls"],
-                           "peak_inflight_bytes": out["peak_inflight"],
-                           "roofline": roofline(nbytes, out["kernel_ms"])}
-                    res["loads"].setdefault(mode, []).append(row)
-                    log(f"serving {mode} {load:.3g} req/s: {row['req_per_s']:.3g} served (cache "
-                        f"{row['cache_req_per_s']:.3g} / {row['cache_req_per_s_99']:.3g} to 99 %, "
-                        f"bank {row['bank_txn_per_s']:.3g}), p99 "
-                        f"{row['all']['p99_us']:.1f} us (cache {row['cache']['p99_us']:.1f}, bank "
-                        f"{row['bank']['p99_us']:.1f}), stalls {row['admit_stalls']}")
+                    measure(eb, mode, load, args.serve_budget,
+                            n_hot if mode == "with_predictor" else 0)
+        # the bank class on the pipelined serialized lane: every bank transaction executed
+        # by the commit servers in commit order (nothing aborts, no routing), its accounts
+        # split over 4 commit server workgroups; admission engages at overload only
+        with csmv.Engine(N_ACCOUNTS, depth=8, client_waves=args.serve_pipe_waves, wg_waves=6,
+                         variant="stamp_pipe", timeout_ms=30000, commit_parts=4,
+                         client_wg_waves=1) as ep:
+            ep.bank_generate(b)
+            for load in loads:
+                measure(ep, "pipelined_bank_lane", load, args.serve_pipe_budget, 0)
     finally:
         for ec in caches:
             ec.close()
@@ -517,6 +536,10 @@ def main():
                     help="commit servers the serving loop's cache class is split over (by set)")
     ap.add_argument("--serve-budget", type=int, default=65536,
                     help="admission cap on in-flight version-pool bytes of the serving loop")
+    ap.add_argument("--serve-pipe-waves", type=int, default=48,
+                    help="client waves of the serving loop's bank class on the pipelined lane")
+    ap.add_argument("--serve-pipe-budget", type=int, default=196608,
+                    help="admission cap of the serving runs with the pipelined bank lane")
     ap.add_argument("--serial-waves-all", type=int, default=64,
                     help="serialized-lane waves of the run with every transaction routed")
     ap.add_argument("--no-one", action="store_true",
