@@ -56,9 +56,10 @@ enum {
                                  clients execute and log their transactions on them.  Every
                                  transfer must be routed to the serialized lane (the normal lane
                                  then carries audits only; routing every transaction leaves
-                                 none to abort); at most 4096 client waves; no escalation,
-                                 cache or serving runs, caller kernels or allocation-free
-                                 runs. */
+                                 none to abort); at most 4096 client waves.  A serving run's
+                                 bank partition with it runs every transaction on the lane
+                                 (read sets of 2..64).  No escalation, cache partitions,
+                                 caller kernels or allocation-free runs. */
 };
 
 #define CSMV_VARIANT_VANILLA          (CSMV_VAL_SCAN | CSMV_INSERT_LANE | CSMV_WB_SERVER)
@@ -73,7 +74,7 @@ typedef struct {
     uint32_t depth;            /* versions per item (MaxVersions, API.cuh:18), 1..64 */
     uint32_t atr_entries;      /* record ring capacity (TXRecordSize, API.cuh:17); power of two */
     uint32_t client_waves;     /* persistent client wavefronts (64 lanes each) */
-    uint32_t wg_waves;         /* waves per workgroup, 3..8 (3..12 with CSMV_SERIAL_PIPE); the
+    uint32_t wg_waves;         /* waves per workgroup, 3..8 (3..16 with CSMV_SERIAL_PIPE); the
                                   server workgroup uses all */
     uint32_t variant;          /* CSMV_VAL_* | CSMV_INSERT_* | CSMV_WB_* (| CSMV_SERIAL_PIPE) */
     uint32_t timeout_ms;       /* bound of every device-side wait (0 = 4000) */
