@@ -308,7 +308,8 @@ def _everything(op):
 @pytest.mark.parametrize("n_items,depth,route,parts,wg",
                          [(4096, 4, "transfers", 1, 6), (4096, 4, "transfers", 4, 8),
                           (1 << 20, 8, "transfers", 1, 8), (1 << 20, 8, "transfers", 4, 8),
-                          (1 << 20, 8, "all", 2, 12), (1 << 16, 8, "all", 3, 5)])
+                          (1 << 20, 8, "all", 2, 12), (1 << 16, 8, "all", 3, 5),
+                          (1 << 20, 8, "transfers", 4, 16)])
 def test_pipelined_lane_replays(n_items, depth, route, parts, wg):
     """The pipelined serialized lane: clients stage their requests' chains, committer waves
     apply each item's request-wide delta with one ordered atomic and install the versions
@@ -324,6 +325,17 @@ def test_pipelined_lane_replays(n_items, depth, route, parts, wg):
                   route=_transfers if route == "transfers" else _everything)
     if route == "all":
         assert st["aborts"] == 0
+
+
+def test_pipelined_lane_at_bench_shape():
+    """The benchmark's configuration on a 2^20-transaction stream of its workload: 1M Zipf
+    0.99 accounts, 4 reads / 2 writes, 4-read audits, depth 8, every transaction on the
+    pipelined lane, 4 commit server workgroups of 16 waves, 80 client waves of which 64 feed
+    the lane: the log replays exactly and nothing aborts."""
+    bank = csmv.Bank(n_txns=1 << 20, seed=1, n_reads=4, n_reads_ro=4, pct_ro=10, zipf=True)
+    st = run_bank(1 << 20, bank, variant="stamp_pipe", depth=8, waves=80, serial_waves=64,
+                  wg_waves=16, client_wg_waves=1, commit_parts=4, route=_everything)
+    assert st["aborts"] == 0
 
 
 @pytest.mark.parametrize("n_txns,parts", [(1, 1), (777, 2), (4097, 4)])
