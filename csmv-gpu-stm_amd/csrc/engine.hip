@@ -385,15 +385,11 @@ __device__ __forceinline__ void bank_client(const Ctx &c, uint32_t cw)
 // Serialized lane: the wave hands routed transfers to the commit server, which executes
 // them in commit order on the newest committed values (no snapshot, so no conflict
 // window); the wave installs the versions the server returns.
-template <int R, int W, bool kPipe = false>
-__device__ __forceinline__ void bank_serial_client(const Ctx &c, const ServerLayout &L, uint32_t *scr,
-                                                   uint32_t cw)
+template <int R, int W>
+__device__ __forceinline__ void bank_serial_client(const Ctx &c, uint32_t cw)
 {
     const uint32_t lane = lane_id();
     const uint64_t NH = c.n_hot;
-    // pipelined form: the wave stages its requests' chains in its LDS scratch and the
-    // server installs the versions
-    if constexpr (kPipe) pipe_client_init<R, W>(L, scr);
     bool has = false, exhausted = false;
     uint64_t k = 0;
     uint32_t op = 0, ab = 0;
@@ -424,9 +420,7 @@ __device__ __forceinline__ void bank_serial_client(const Ctx &c, const ServerLay
         Tx<R, W> tx;
         delegated_tx<R, W>(c, tx, has, k, op);
         bool fail = false;
-        uint32_t ts;
-        if constexpr (kPipe) ts = tx_commit_pipe<R, W>(c, L, scr, cw, tx, has, &fail, &t_reply);
-        else ts = tx_commit<R, W>(c, cw, tx, has, &fail, &t_reply, true);
+        const uint32_t ts = tx_commit<R, W>(c, cw, tx, has, &fail, &t_reply, true);
         if (fail) break;
         if (has) {
             if (ts) {
@@ -454,6 +448,88 @@ __device__ __forceinline__ void bank_serial_client(const Ctx &c, const ServerLay
                                           __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// The pipelined lane's client wave: two transactions a lane (slot 0 the older), a request
+// of up to 128 (tx_commit_pipe).  A transaction the request could not carry (its
+// partition's list was full) stays in its slot for the next request; nothing aborts.
+template <int R, int W>
+__device__ __forceinline__ void bank_pipe_client(const Ctx &c, const ServerLayout &L, uint32_t *scr,
+                                                 uint32_t cw)
+{
+    const uint32_t lane = lane_id();
+    const uint64_t NH = c.n_hot;
+    pipe_client_init<R, W>(L, scr);
+    bool has[2] = {false, false}, exhausted = false;
+    uint64_t k[2] = {0, 0};
+    uint32_t op[2] = {0, 0};
+    uint64_t n_upd = 0, n_ro = 0, t_reply = 0, t_round = 0, rounds = 0;
+    for (;;) {
+        if (ld_rlx(c.err)) break;
+        const uint64_t tr0 = wall_clock64();
+        if (!has[0] && has[1]) {
+            has[0] = true;
+            has[1] = false;
+            k[0] = k[1];
+            op[0] = op[1];
+        }
+        if (!exhausted) {
+            const uint32_t want = (has[0] ? 0u : 1u) + (has[1] ? 0u : 1u);
+            const uint32_t incl = wave_incl_scan(want);
+            const uint32_t total = readlane(incl, 63);
+            if (total) {
+                uint64_t base = 0;
+                if (lane == 0) base = fetch_add(c.next_hot, (uint64_t)total);
+                base = bcast64(base);
+                if (base + total >= NH) exhausted = true;
+                uint64_t mine = base + (incl - want);
+#pragma unroll
+                for (int s = 0; s < 2; ++s)
+                    if (!has[s]) {
+                        if (mine < NH) {
+                            has[s] = true;
+                            k[s] = c.hot_list[mine];
+                            op[s] = c.op[k[s]];
+                        }
+                        ++mine;
+                    }
+                if (!has[0] && has[1]) {
+                    has[0] = true;
+                    has[1] = false;
+                    k[0] = k[1];
+                    op[0] = op[1];
+                }
+            }
+        }
+        if (ballot(has[0] || has[1]) == 0) break;
+        Tx<R, W> tx[2];
+        delegated_tx<R, W>(c, tx[0], has[0], k[0], op[0]);
+        delegated_tx<R, W>(c, tx[1], has[1], k[1], op[1]);
+        bool fail = false;
+        uint32_t ts[2];
+        tx_commit_pipe<R, W>(c, L, scr, cw, tx, has, ts, &fail, &t_reply);
+        if (fail) break;
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+            if (has[s] && ts[s]) {
+                c.log_ab[k[s]] = 0;
+                has[s] = false;
+                if (ts[s] & kSnapshotBit) ++n_ro;
+                else ++n_upd;
+            }
+        t_round += wall_clock64() - tr0;
+        ++rounds;
+    }
+    if (lane == 0) {
+        fetch_add(&c.prof[kProfClientReply], t_reply);
+        fetch_add(&c.prof[kProfClientRound], t_round);
+        fetch_add(&c.prof[kProfClientRounds], rounds);
+    }
+    if (n_upd) fetch_add(&c.stats[kStatCommitUpd], n_upd);
+    if (n_ro) fetch_add(&c.stats[kStatCommitRo], n_ro);
+    fence_rel();
+    if (lane == 0) __hip_atomic_fetch_add(c.clients_done, 1u, __ATOMIC_RELEASE,
+                                          __HIP_MEMORY_SCOPE_AGENT);
+}
+
 template <int R, int W>
 __global__ void __launch_bounds__(512) bank_kernel(Ctx c, ServerLayout L)
 {
@@ -466,7 +542,7 @@ __global__ void __launch_bounds__(512) bank_kernel(Ctx c, ServerLayout L)
     if (wave >= c.client_wg_waves) return;
     const uint32_t cw = (blockIdx.x - 1) * c.client_wg_waves + wave;
     if (cw >= c.n_client_waves) return;
-    if (cw >= c.n_client_waves - c.n_serial_waves) bank_serial_client<R, W>(c, L, lds, cw);
+    if (cw >= c.n_client_waves - c.n_serial_waves) bank_serial_client<R, W>(c, cw);
     else bank_client<R, W>(c, cw);
 }
 
@@ -486,7 +562,7 @@ __global__ void __launch_bounds__(1024) bank_pipe_kernel(Ctx c, ServerLayout L)
     const uint32_t cw = (blockIdx.x - L.parts) * c.client_wg_waves + wave;
     if (cw >= c.n_client_waves) return;
     if (cw >= c.n_client_waves - c.n_serial_waves)
-        bank_serial_client<R, W, true>(c, L, lds + wave * pipe_client_words(L.hsz, R, W), cw);
+        bank_pipe_client<R, W>(c, L, lds + wave * pipe_client_words(L.hsz, R, W), cw);
     else bank_client<R, W, true>(c, cw);
 }
 

@@ -123,15 +123,6 @@ __host__ __device__ constexpr uint32_t pipe_words() { return kPipeRing * (2 * kP
 // the waves receive (16 waves: 4 receivers, 11 committers, the publisher).
 __host__ __device__ constexpr uint32_t pipe_receivers(uint32_t wg) { return wg >= 8 ? wg / 4 : 1; }
 __host__ __device__ constexpr uint32_t pipe_committers(uint32_t wg) { return wg - 1 - pipe_receivers(wg); }
-// A client wave's LDS scratch for staging a pipelined request: the write hash (key, writer
-// lanes lo / hi, canonical node, request-wide delta), then the chain arrays (next, weight
-// per node; superseded flag per write slot), the last writers' ranks (by hash slot), each
-// node's entry, and the staged entry lists.
-__host__ __device__ constexpr uint32_t pipe_client_words(uint32_t hsz, uint32_t R, uint32_t W)
-{
-    return 6 * hsz + 3 * kWave * R + kWave * W + 3 * kPipeMaxE;
-}
-
 // Row indices inside an LDS mailbox.  For a delegated request the wv rows carry the
 // transaction id and amount, and prod packs each read's producer (the lower lane whose
 // write it must see: lane | slot << 6, 0xFF = none).
@@ -1593,312 +1584,391 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
 // commit (bank/csmv_sh/API.cu:88-272) and its clients write back their own versions.
 
 // Client side (TXCommit for a wave of routed transfers, bank/csmv_sh/exe.cu:184-201 as the
-// body): stage the request's producer chains in the wave's LDS scratch, write each
-// partition's entry list to the wave's boxes, send, and wait for the replies: the values
-// the nodes read, from which the wave runs its transactions and logs them (commit
-// timestamp, or kSnapshotBit | snapshot for a read-only lane, returned).  The server
-// installs the versions.  scr: pipe_client_words(L.hsz, R, W) words of this wave's LDS,
-// cleared by pipe_client_init.
+// body), two transactions a lane: a request carries up to 128 transactions in the order of
+// virtual lanes v = s * 64 + lane (every lane's first transaction, then every lane's
+// second), so each token step, atomic round, reply and hand-off of the commit servers
+// serves 128.  The wave stages the request's producer chains in its LDS scratch (writer
+// masks of 128 bits, producer codes virtual lane | write slot << 7, nodes v * R + r), writes
+// each partition's entry list to the wave's boxes, sends, and waits for the replies: the
+// values the nodes read, from which it runs its transactions and logs them (commit
+// timestamp, or kSnapshotBit | snapshot for a read-only transaction, in ts_out; 0 for one
+// the request did not carry).  A request carries the virtual lanes, in order, whose nodes
+// keep every partition's list within kPipeMaxE; the others wait for the next request.
+// The server installs the versions.  Scratch (pipe_client_words, cleared by
+// pipe_client_init): the write hash (key, four writer-mask words, canonical node,
+// request-wide delta, last writer's rank) of L.hsz slots (at most half full), two buffers of
+// (next node, weight) per node, its entry and hash slot, per write slot its superseded flag,
+// and the
+// staged lists (a block of kPipeMaxE per partition).
+__host__ __device__ constexpr uint32_t pipe_client_words(uint32_t hsz, uint32_t R, uint32_t W)
+{
+    return 8 * hsz + 6 * 2 * kWave * R + 2 * kWave * W + 3 * kPipeParts * kPipeMaxE;
+}
+
 template <int R, int W>
 __device__ __forceinline__ void pipe_client_init(const ServerLayout &L, uint32_t *scr)
 {
-    const uint32_t lane = lane_id();
-    for (uint32_t i = lane; i < L.hsz; i += kWave) {
+    const uint32_t lane = lane_id(), hsz = L.hsz;
+    for (uint32_t i = lane; i < hsz; i += kWave) {
         scr[i] = kNone;
-        scr[L.hsz + i] = 0;
-        scr[2 * L.hsz + i] = 0;
-        scr[3 * L.hsz + i] = kNone;
-        scr[4 * L.hsz + i] = 0;
+#pragma unroll
+        for (uint32_t q = 1; q <= 4; ++q) scr[q * hsz + i] = 0;
+        scr[5 * hsz + i] = kNone;
+        scr[6 * hsz + i] = 0;
     }
     wave_fence();
 }
 
 template <int R, int W>
-__device__ __forceinline__ uint32_t tx_commit_pipe(const Ctx &c, const ServerLayout &L, uint32_t *scr,
-                                                   uint32_t cw, const Tx<R, W> &tx, bool req,
-                                                   bool *fail, uint64_t *wait_ticks)
+__device__ __forceinline__ void tx_commit_pipe(const Ctx &c, const ServerLayout &L, uint32_t *scr,
+                                                uint32_t cw, const Tx<R, W> (&tx)[2],
+                                                const bool (&req)[2], uint32_t (&ts_out)[2],
+                                                bool *fail, uint64_t *wait_ticks)
 {
     static_assert(!kWideWire<R> && W == 2 && R * kWave <= kPipeMaxE,
                   "the pipelined lane carries bank transfers");
-    constexpr int RO = R;
+    constexpr int RO = R, S = 2;
+    constexpr uint32_t kNoP = 0xFFFFu;            // no producer
     const uint32_t lane = lane_id();
     const uint64_t lt = (1ull << lane) - 1;
-    const uint32_t hsz = L.hsz, hmask = hsz - 1;
-    uint32_t *hkey = scr, *hwlo = scr + hsz, *hwhi = scr + 2 * hsz, *canon = scr + 3 * hsz;
-    int32_t *htot = (int32_t *)(scr + 4 * hsz);
-    uint32_t *snext = scr + 5 * hsz;
-    int32_t *sw = (int32_t *)(snext + kWave * RO);
-    uint32_t *ssup = snext + 2 * kWave * RO;
-    uint32_t *hrank = ssup + kWave * W;                 // [hsz] last writer's update rank
-    uint32_t *e2 = hrank + hsz;                         // [64 * RO] each node's entry
-    uint32_t *sitem = e2 + kWave * RO;                  // [kPipeMaxE] the lists, back to back
-    uint32_t *swgt = sitem + kPipeMaxE;
-    uint32_t *smeta = swgt + kPipeMaxE;
-    const uint32_t mybit = 1u << (lane & 31);
+    const uint32_t hsz = L.hsz, hmask = hsz - 1, P = L.parts;
+    uint32_t *hkey = scr, *hw = scr + hsz;        // hw[i * hsz + slot]: virtual lanes 32 i ..
+    uint32_t *canon = scr + 5 * hsz;
+    int32_t *htot = (int32_t *)(scr + 6 * hsz);
+    uint32_t *hrank = scr + 7 * hsz;
+    uint32_t *snext = scr + 8 * hsz;              // [2][2 * 64 * R] by node v * R + r
+    int32_t *sw = (int32_t *)(snext + S * kWave * RO);   // (interleaved with snext's buffers)
+    uint32_t *e2 = snext + 4 * S * kWave * RO;
+    uint32_t *sfs = e2 + S * kWave * RO;          // each node's write-hash slot (kNone: unwritten)
+    uint32_t *ssup = sfs + S * kWave * RO;        // [2 * 64 * W] by v * W + w
+    uint32_t *sitem = ssup + S * kWave * W;       // [kPipeParts][kPipeMaxE]
+    uint32_t *swgt = sitem + kPipeParts * kPipeMaxE;
+    uint32_t *smeta = swgt + kPipeParts * kPipeMaxE;
 
-    uint32_t ops[RO], wa[W];
+    uint32_t ops[S][RO], wa[S][W], kx[S], amt[S];
+    bool ro[S], on[S];
 #pragma unroll
-    for (int r = 0; r < RO; ++r) ops[r] = (req && (uint32_t)r < tx.nr) ? tx.rs[r] : kNone;
+    for (int s = 0; s < S; ++s) {
 #pragma unroll
-    for (int w = 0; w < W; ++w) wa[w] = (req && (uint32_t)w < tx.nw) ? tx.wa[w] : kNone;
-    const uint32_t k = (uint32_t)tx.wv[0], amt = (uint32_t)tx.wv[1];
-
-    // 1. the write set into the hash (item -> writer lanes); every probe step swaps for all
-    //    slots at once (an idle slot swaps kNone for kNone at 0, which changes nothing)
-    uint32_t slot_w[W], hp[W];
-    bool pend[W];
+        for (int r = 0; r < RO; ++r) ops[s][r] = (req[s] && (uint32_t)r < tx[s].nr) ? tx[s].rs[r] : kNone;
 #pragma unroll
-    for (int w = 0; w < W; ++w) {
-        slot_w[w] = kNone;
-        hp[w] = mix32(wa[w]) & hmask;
-        pend[w] = wa[w] != kNone;
+        for (int w = 0; w < W; ++w) wa[s][w] = (req[s] && (uint32_t)w < tx[s].nw) ? tx[s].wa[w] : kNone;
+        kx[s] = (uint32_t)tx[s].wv[0];
+        amt[s] = (uint32_t)tx[s].wv[1];
+        ro[s] = amt[s] == kReadOnlyAmt;
     }
-    for (uint32_t step = 0; step <= hmask; ++step) {
-        uint32_t old[W];
+
+    // 1. the virtual lanes this request carries: an order prefix whose nodes keep every
+    //    partition's list within kPipeMaxE (every lane takes part in the scans)
+    {
+        uint32_t cq[S][kPipeParts];
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+#pragma unroll
+            for (uint32_t t = 0; t < kPipeParts; ++t) {
+                uint32_t n = 0;
+#pragma unroll
+                for (int r = 0; r < RO; ++r)
+                    n += (ops[s][r] != kNone && pipe_part(ops[s][r], P) == t) ? 1u : 0u;
+                cq[s][t] = n;
+            }
+        bool ok0 = req[0], ok1 = req[1];
+#pragma unroll
+        for (uint32_t t = 0; t < kPipeParts; ++t) {
+            const uint32_t p0 = wave_incl_scan(cq[0][t]);
+            const uint32_t p1 = readlane(p0, 63) + wave_incl_scan(cq[1][t]);
+            ok0 = ok0 && p0 <= kPipeMaxE;
+            ok1 = ok1 && p1 <= kPipeMaxE;
+        }
+        on[0] = ok0;
+        on[1] = ok1;
+    }
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+#pragma unroll
+        for (int r = 0; r < RO; ++r) ops[s][r] = on[s] ? ops[s][r] : kNone;
+#pragma unroll
+        for (int w = 0; w < W; ++w) wa[s][w] = on[s] ? wa[s][w] : kNone;
+    }
+    const uint64_t u0 = ballot(on[0] && !ro[0]), u1 = ballot(on[1] && !ro[1]);
+    const uint32_t rank[S] = {(uint32_t)__builtin_popcountll(u0 & lt),
+                              (uint32_t)__builtin_popcountll(u0) + (uint32_t)__builtin_popcountll(u1 & lt)};
+    const uint32_t nc = (uint32_t)__builtin_popcountll(u0) + (uint32_t)__builtin_popcountll(u1);
+    int32_t d0[S], d1[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        d0[s] = (wa[s][1] == kNone) ? 0 : -(int32_t)amt[s];
+        d1[s] = (int32_t)amt[s];
+    }
+
+    // 2. the write sets into the hash (item -> writer virtual lanes)
+    uint32_t slot_w[S][W], hp[S][W];
+    bool pend[S][W];
+#pragma unroll
+    for (int s = 0; s < S; ++s)
 #pragma unroll
         for (int w = 0; w < W; ++w) {
-            old[w] = kNone;
-            __hip_atomic_compare_exchange_strong(&hkey[pend[w] ? hp[w] : 0u], &old[w],
-                                                 pend[w] ? wa[w] : kNone, __ATOMIC_RELAXED,
-                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            slot_w[s][w] = kNone;
+            hp[s][w] = mix32(wa[s][w]) & hmask;
+            pend[s][w] = wa[s][w] != kNone;
         }
+    for (uint32_t step = 0; step <= hmask; ++step) {
         bool more = false;
 #pragma unroll
-        for (int w = 0; w < W; ++w) {
-            const bool got = pend[w] && (old[w] == kNone || old[w] == wa[w]);
-            slot_w[w] = got ? hp[w] : slot_w[w];
-            hp[w] = (pend[w] && !got) ? ((hp[w] + 1) & hmask) : hp[w];
-            pend[w] = pend[w] && !got;
-            more |= pend[w];
-        }
+        for (int s = 0; s < S; ++s)
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                uint32_t old = kNone;
+                __hip_atomic_compare_exchange_strong(&hkey[pend[s][w] ? hp[s][w] : 0u], &old,
+                                                     pend[s][w] ? wa[s][w] : kNone, __ATOMIC_RELAXED,
+                                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                const bool got = pend[s][w] && (old == kNone || old == wa[s][w]);
+                slot_w[s][w] = got ? hp[s][w] : slot_w[s][w];
+                hp[s][w] = (pend[s][w] && !got) ? ((hp[s][w] + 1) & hmask) : hp[s][w];
+                pend[s][w] = pend[s][w] && !got;
+                more |= pend[s][w];
+            }
         if (ballot(more) == 0) break;
     }
 #pragma unroll
-    for (int w = 0; w < W; ++w)
-        if (slot_w[w] != kNone) atomicOr(lane < 32 ? &hwlo[slot_w[w]] : &hwhi[slot_w[w]], mybit);
+    for (int s = 0; s < S; ++s)
+#pragma unroll
+        for (int w = 0; w < W; ++w)
+            if (slot_w[s][w] != kNone) {
+                const uint32_t v = (uint32_t)s * kWave + lane;
+                atomicOr(&hw[(v >> 5) * hsz + slot_w[s][w]], 1u << (v & 31));
+            }
     wave_fence();
 
-    // 2. each read's slot (kNone: no lane writes the item)
-    uint32_t fs[RO], hs[RO];
-    bool fp[RO];
+    // 3. each read's slot, producer (the highest lower virtual lane writing the item, and
+    //    its write slot) and each written item's canonical node
+    uint32_t fs[S][RO], pj[S][RO];
 #pragma unroll
-    for (int r = 0; r < RO; ++r) {
-        fs[r] = kNone;
-        hs[r] = mix32(ops[r]) & hmask;
-        fp[r] = ops[r] != kNone;
-    }
-    for (uint32_t step = 0; step <= hmask; ++step) {
-        uint32_t kk[RO];
-#pragma unroll
-        for (int r = 0; r < RO; ++r)
-            kk[r] = __hip_atomic_load(&hkey[fp[r] ? hs[r] : 0u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        bool more = false;
+    for (int s = 0; s < S; ++s) {
+        uint32_t hs[RO];
+        bool fp[RO];
 #pragma unroll
         for (int r = 0; r < RO; ++r) {
-            const bool hit = fp[r] && kk[r] == ops[r];
-            const bool next = fp[r] && kk[r] != ops[r] && kk[r] != kNone;
-            fs[r] = hit ? hs[r] : fs[r];
-            hs[r] = next ? ((hs[r] + 1) & hmask) : hs[r];
-            fp[r] = next;
-            more |= next;
+            fs[s][r] = kNone;
+            hs[r] = mix32(ops[s][r]) & hmask;
+            fp[r] = ops[s][r] != kNone;
         }
-        if (ballot(more) == 0) break;
-    }
-
-    // 3. each read's producer (the highest lower lane writing the item, and its write
-    //    slot: lane | slot << 6, 0xFF = none), and each written item's canonical node (the
-    //    lowest node of the item without a producer)
-    uint32_t pj[RO];
+        for (uint32_t step = 0; step <= hmask; ++step) {
+            bool more = false;
 #pragma unroll
-    for (int r = 0; r < RO; ++r) {
-        const uint32_t at = fs[r] != kNone ? fs[r] : 0u;
-        const uint64_t m = ((uint64_t)__hip_atomic_load(&hwhi[at], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) << 32) |
-                           __hip_atomic_load(&hwlo[at], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        const uint64_t lower = fs[r] != kNone ? (m & lt) : 0ull;
-        const uint32_t pl = lower ? 63u - (uint32_t)__builtin_clzll(lower) : 0xFFu;
-        const uint32_t srcl = pl == 0xFFu ? lane : pl;
-        uint32_t code = 0xFFu;
-#pragma unroll
-        for (int w = 0; w < W; ++w) {
-            const uint32_t pw = lane_gather(wa[w], srcl);
-            code = (pl != 0xFFu && pw == ops[r]) ? (pl | ((uint32_t)w << 6)) : code;
+            for (int r = 0; r < RO; ++r) {
+                const uint32_t kk = __hip_atomic_load(&hkey[fp[r] ? hs[r] : 0u], __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_WORKGROUP);
+                const bool hit = fp[r] && kk == ops[s][r];
+                const bool next = fp[r] && kk != ops[s][r] && kk != kNone;
+                fs[s][r] = hit ? hs[r] : fs[s][r];
+                hs[r] = next ? ((hs[r] + 1) & hmask) : hs[r];
+                fp[r] = next;
+                more |= next;
+            }
+            if (ballot(more) == 0) break;
         }
-        pj[r] = code;
     }
 #pragma unroll
-    for (int r = 0; r < RO; ++r)
-        if (fs[r] != kNone && pj[r] == 0xFFu) atomicMin(&canon[fs[r]], lane * RO + (uint32_t)r);
+    for (int s = 0; s < S; ++s) {
+#pragma unroll
+        for (int r = 0; r < RO; ++r) {
+            const uint32_t at = fs[s][r] != kNone ? fs[s][r] : 0u;
+            uint32_t pv = kNoP;
+            // the virtual lanes below v: mask words 0 .. 2 s + 1, the last two as the lanes
+            // below the lane (words 0 and 1 whole for slot 1)
+#pragma unroll
+            for (int i = 2 * s + 1; i >= 0; --i) {
+                const uint32_t lo = (uint32_t)i * 32u;
+                const uint32_t below = i < 2 * s ? 0xFFFFFFFFu : (i & 1) ? (uint32_t)(lt >> 32) : (uint32_t)lt;
+                const uint32_t raw = __hip_atomic_load(&hw[(uint32_t)i * hsz + at], __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+                const uint32_t m = fs[s][r] != kNone ? (raw & below) : 0u;
+                pv = (pv == kNoP && m) ? lo + 31u - (uint32_t)__builtin_clz(m) : pv;
+            }
+            const uint32_t srcl = pv == kNoP ? lane : (pv & 63u);
+            uint32_t code = kNoP;
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                const uint32_t g0 = lane_gather(wa[0][w], srcl), g1 = lane_gather(wa[1][w], srcl);
+                const uint32_t pw = (pv != kNoP && (pv >> 6)) ? g1 : g0;
+                code = (pv != kNoP && pw == ops[s][r]) ? (pv | ((uint32_t)w << 7)) : code;
+            }
+            pj[s][r] = code;
+        }
+        // one slot's mask loads at a time (fewer registers in flight)
+        asm volatile("" ::: "memory");
+    }
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+#pragma unroll
+        for (int r = 0; r < RO; ++r)
+            if (fs[s][r] != kNone && pj[s][r] == kNoP)
+                atomicMin(&canon[fs[s][r]], ((uint32_t)s * kWave + lane) * RO + (uint32_t)r);
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+#pragma unroll
+        for (int r = 0; r < RO; ++r) sfs[((uint32_t)s * kWave + lane) * RO + (uint32_t)r] = fs[s][r];
 
-    // 4. producer chains by pointer jumping (as stage_chains): each node's chain root and
-    //    the sum of the writers' deltas between them; a write slot is superseded when a
-    //    higher lane reads the item from it and writes it too
-    const int32_t d0 = (wa[1] == kNone) ? 0 : -(int32_t)amt, d1 = (int32_t)amt;
-    uint32_t nxt[RO];
-    int32_t wgt[RO];
+    // 4. superseded write slots (a write slot is superseded when a higher virtual lane reads
+    //    the item from it and writes it too), then the producer chains by pointer jumping
+    //    over two buffers of (next node, weight): each node's chain root and the sum of the
+    //    writers' deltas between them
 #pragma unroll
-    for (int r = 0; r < RO; ++r) {
-        const uint32_t node = lane * RO + r;
-        const bool has_p = pj[r] != 0xFFu;
-        const uint32_t srcl = has_p ? (pj[r] & 63u) : lane;
-        const int32_t q0 = (int32_t)lane_gather((uint32_t)d0, srcl),
-                      q1 = (int32_t)lane_gather((uint32_t)d1, srcl);
-        nxt[r] = has_p ? (pj[r] & 63u) * RO + (pj[r] >> 6) : node;
-        wgt[r] = has_p ? ((pj[r] >> 6) == 0 ? q0 : q1) : 0;
-        snext[node] = nxt[r];
-        sw[node] = wgt[r];
-    }
+    for (int s = 0; s < S; ++s)
 #pragma unroll
-    for (int w = 0; w < W; ++w) ssup[lane * W + w] = 0;
+        for (int w = 0; w < W; ++w) ssup[((uint32_t)s * kWave + lane) * W + w] = 0;
+    wave_fence();
+    uint32_t *nA = snext, *nB = snext + 2 * S * kWave * RO;
+    int32_t *wA = sw, *wB = sw + 2 * S * kWave * RO;
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+#pragma unroll
+        for (int r = 0; r < RO; ++r) {
+            const uint32_t node = ((uint32_t)s * kWave + lane) * RO + (uint32_t)r;
+            const bool has_p = pj[s][r] != kNoP;
+            const uint32_t pv = pj[s][r] & 127u, pw = pj[s][r] >> 7;
+            const uint32_t srcl = has_p ? (pv & 63u) : lane;
+            const int32_t a0 = (int32_t)lane_gather((uint32_t)d0[0], srcl), a1 = (int32_t)lane_gather((uint32_t)d1[0], srcl);
+            const int32_t b0 = (int32_t)lane_gather((uint32_t)d0[1], srcl), b1 = (int32_t)lane_gather((uint32_t)d1[1], srcl);
+            const int32_t dd = (pv >> 6) ? (pw == 0 ? b0 : b1) : (pw == 0 ? a0 : a1);
+            nA[node] = has_p ? pv * RO + pw : node;
+            wA[node] = has_p ? dd : 0;
+            bool mine_w = false;
+#pragma unroll
+            for (int w = 0; w < W; ++w) mine_w |= wa[s][w] == ops[s][r] && ops[s][r] != kNone;
+            if (has_p && mine_w) ssup[pv * W + pw] = 1u;
+        }
     wave_fence();
     for (uint32_t round = 0; round < 8; ++round) {
         bool more = false;
-        uint32_t nn[RO];
-        int32_t ww[RO];
 #pragma unroll
-        for (int r = 0; r < RO; ++r) {
-            const uint32_t node = lane * RO + r;
-            const uint32_t t = nxt[r];
-            const uint32_t tt = snext[t];
-            ww[r] = wgt[r] + (t != node ? sw[t] : 0);
-            nn[r] = tt;
-            more |= tt != t;
-        }
-        wave_fence();
-#pragma unroll
-        for (int r = 0; r < RO; ++r) {
-            nxt[r] = nn[r];
-            wgt[r] = ww[r];
-            snext[lane * RO + r] = nn[r];
-            sw[lane * RO + r] = ww[r];
-        }
-        wave_fence();
-        if (ballot(more) == 0) break;
-    }
-#pragma unroll
-    for (int r = 0; r < RO; ++r) {
-        bool mine_w = false;
-#pragma unroll
-        for (int w = 0; w < W; ++w) mine_w |= wa[w] == ops[r] && ops[r] != kNone;
-        if (pj[r] != 0xFFu && mine_w) ssup[(pj[r] & 63u) * W + (pj[r] >> 6)] = 1u;
-    }
-    wave_fence();
-    uint32_t supf = 0;
-#pragma unroll
-    for (int w = 0; w < W; ++w) supf |= ssup[lane * W + w] ? (1u << w) : 0u;
-
-    // 5. the item's request-wide delta and its last writer's rank among the request's
-    //    update lanes, from its last (unsuperseded) write: node w reads write slot w's item
-    //    (a transfer reads what it writes), so that write's value is the value before the
-    //    request + wgt[w] + the slot's delta
-    const bool ro = amt == kReadOnlyAmt;
-    const uint64_t umask = ballot(req && !ro);
-    const uint32_t rank = (uint32_t)__builtin_popcountll(umask & lt);
-#pragma unroll
-    for (int w = 0; w < W; ++w)
-        if (wa[w] != kNone && ((supf >> w) & 1u) == 0) {
-            htot[slot_w[w]] = wgt[w] + (w == 0 ? d0 : d1);
-            hrank[slot_w[w]] = rank;
-        }
-    wave_fence();
-
-    // 6. the entries: each node of an item of partition p becomes entry e of p's list (the
-    //    partition's nodes in node order), with its root entry, weight and flags; the lists
-    //    are staged back to back in LDS, then written to the boxes a row at a time
-    const uint32_t P = L.parts;
-    uint32_t pr[RO], e[RO];
-#pragma unroll
-    for (int r = 0; r < RO; ++r) {
-        pr[r] = ops[r] != kNone ? pipe_part(ops[r], P) : kNone;
-        e[r] = 0;
-    }
-    uint32_t cntq[kPipeParts], offq[kPipeParts];
-    uint32_t off = 0;
-#pragma unroll
-    for (uint32_t q = 0; q < kPipeParts; ++q) {
-        cntq[q] = 0;
-        offq[q] = off;
-        if (q < P) {
-            uint32_t low = 0, tot = 0;
+        for (int s = 0; s < S; ++s)
 #pragma unroll
             for (int r = 0; r < RO; ++r) {
-                const uint64_t mq = ballot(pr[r] == q);
-                low += (uint32_t)__builtin_popcountll(mq & lt);
-                tot += (uint32_t)__builtin_popcountll(mq);
+                const uint32_t node = ((uint32_t)s * kWave + lane) * RO + (uint32_t)r;
+                const uint32_t t = nA[node];
+                const uint32_t tt = nA[t];
+                wB[node] = wA[node] + (t != node ? wA[t] : 0);
+                nB[node] = tt;
+                more |= tt != t;
             }
-            uint32_t own = 0;
-#pragma unroll
-            for (int r = 0; r < RO; ++r)
-                if (pr[r] == q) e[r] = low + own++;
-            cntq[q] = tot;
-            off += tot;
-        }
+        wave_fence();
+        uint32_t *const tn = nA;
+        int32_t *const tw = wA;
+        nA = nB;
+        wA = wB;
+        nB = tn;
+        wB = tw;
+        if (ballot(more) == 0) break;
     }
+    uint32_t supf[S];
 #pragma unroll
-    for (int r = 0; r < RO; ++r) e2[lane * RO + r] = e[r];
-    wave_fence();
-    uint32_t *bx = c.mbox + (size_t)cw * kPipeParts * kPipeBox;
+    for (int s = 0; s < S; ++s) {
+        supf[s] = 0;
 #pragma unroll
-    for (int r = 0; r < RO; ++r) {
-        const uint32_t node = lane * RO + r;
-        const bool written = fs[r] != kNone;
-        const uint32_t at = written ? fs[r] : 0u;
-        const uint32_t cn = canon[at];
-        // every chain root of a written item is a node without a producer, and all of
-        // them see the value before the request: they all take the canonical node's
-        const uint32_t rt = written ? cn : nxt[r];
-        const bool is_canon = written && cn == node;
-        const bool is_root = rt == node;
-        const uint32_t meta = e2[rt & (kWave * RO - 1)] | (is_root ? kPipeRootBit : 0u) |
-                              (is_canon ? kPipeInstBit | (hrank[at] << kPipeRankShift) : 0u);
-        const int32_t wout = is_canon ? htot[at] : wgt[r];
-        if (pr[r] != kNone) {
-            uint32_t o = offq[0];
-#pragma unroll
-            for (uint32_t q = 1; q < kPipeParts; ++q) o = pr[r] == q ? offq[q] : o;
-            sitem[o + e[r]] = ops[r];
-            swgt[o + e[r]] = (uint32_t)wout;
-            smeta[o + e[r]] = meta;
-        }
+        for (int w = 0; w < W; ++w) supf[s] |= ssup[((uint32_t)s * kWave + lane) * W + w] ? (1u << w) : 0u;
     }
-    wave_fence();
-    const uint32_t nc = (uint32_t)__builtin_popcountll(umask);
-    uint32_t chunks = 0;        // for the ring word
+
+    // 5. each written item's request-wide delta and last writer's rank
 #pragma unroll
-    for (uint32_t q = 0; q < kPipeParts; ++q)
-        chunks |= ((cntq[q] + kPipeChunk - 1) / kPipeChunk) << (kPipeChunkShift + 4 * q);
+    for (int s = 0; s < S; ++s)
+#pragma unroll
+        for (int w = 0; w < W; ++w)
+            if (wa[s][w] != kNone && ((supf[s] >> w) & 1u) == 0) {
+                htot[slot_w[s][w]] = wA[((uint32_t)s * kWave + lane) * RO + (uint32_t)w] + (w == 0 ? d0[s] : d1[s]);
+                hrank[slot_w[s][w]] = rank[s];
+            }
+    wave_fence();
+
+    // 6. the entries: node (v, r) of an item of partition q becomes entry e of q's list,
+    //    staged at q * kPipeMaxE + e; the node's place (q << 16 | e, kept in the next-node
+    //    array, which the chains no longer need) for reading its value back.  A node of an
+    //    item no one writes is its own root.
+    uint32_t cnt[kPipeParts] = {0u, 0u, 0u, 0u};
+    uint32_t pe[S][RO];
+    uint32_t *const splace = nB;
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+#pragma unroll
+        for (int r = 0; r < RO; ++r) {
+            const uint32_t q = ops[s][r] != kNone ? pipe_part(ops[s][r], P) : kNone;
+            uint32_t e = 0;
+#pragma unroll
+            for (uint32_t t = 0; t < kPipeParts; ++t) {
+                const uint64_t mq = ballot(q == t);
+                if (q == t) e = cnt[t] + (uint32_t)__builtin_popcountll(mq & lt);
+                cnt[t] += (uint32_t)__builtin_popcountll(mq);
+            }
+            pe[s][r] = q != kNone ? (q << 16) | e : kNone;
+            e2[((uint32_t)s * kWave + lane) * RO + (uint32_t)r] = e;
+            splace[((uint32_t)s * kWave + lane) * RO + (uint32_t)r] = pe[s][r];
+        }
+    wave_fence();
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+#pragma unroll
+        for (int r = 0; r < RO; ++r) {
+            const uint32_t node = ((uint32_t)s * kWave + lane) * RO + (uint32_t)r;
+            const uint32_t f = sfs[node];
+            const bool written = f != kNone;
+            const uint32_t at = written ? f : 0u;
+            const uint32_t cn = canon[at];
+            const uint32_t rt = written ? cn : node;
+            const bool is_canon = written && cn == node;
+            const bool is_root = rt == node;
+            const uint32_t meta = e2[rt & (S * kWave * RO - 1)] | (is_root ? kPipeRootBit : 0u) |
+                                  (is_canon ? kPipeInstBit | (hrank[at] << kPipeRankShift) : 0u);
+            const int32_t wout = is_canon ? htot[at] : wA[node];
+            if (pe[s][r] != kNone) {
+                const uint32_t o = (pe[s][r] >> 16) * kPipeMaxE + (pe[s][r] & 0xFFFFu);
+                sitem[o] = ops[s][r];
+                swgt[o] = (uint32_t)wout;
+                smeta[o] = meta;
+            }
+        }
+    wave_fence();
+
+    // 7. the lists into the boxes a row at a time, the headers, the hash slots back to
+    //    empty; send once the box stores are complete (no release fence: it would write
+    //    back the XCD's dirty L2 lines)
+    // (an opaque scalar base: the stores take it and immediate offsets, instead of per-lane
+    // addresses computed once outside the client's loop and kept in scratch)
+    uint64_t bxa = bcast64((uint64_t)(uintptr_t)(c.mbox + (size_t)cw * kPipeParts * kPipeBox));
+    asm volatile("" : "+s"(bxa));
+    uint32_t *bx = (uint32_t *)(uintptr_t)bxa;
+    uint32_t chunks = 0;
 #pragma unroll
     for (uint32_t q = 0; q < kPipeParts; ++q) {
-        if (q >= P) break;
+        chunks |= ((cnt[q] + kPipeChunk - 1) / kPipeChunk) << (kPipeChunkShift + 4 * q);
+        if (q >= P) continue;
         uint32_t *b = bx + q * kPipeBox;
 #pragma unroll
         for (uint32_t g = 0; g < kPipeMaxE / kWave; ++g) {
             const uint32_t x = g * kWave + lane;
-            if (x < cntq[q]) {
-                st_rlx(&b[kPipeItem + x], sitem[offq[q] + x]);
-                st_rlx(&b[kPipeWgt + x], swgt[offq[q] + x]);
-                st_rlx(&b[kPipeMeta + x], smeta[offq[q] + x]);
+            if (x < cnt[q]) {
+                st_rlx(&b[kPipeItem + x], sitem[q * kPipeMaxE + x]);
+                st_rlx(&b[kPipeWgt + x], swgt[q * kPipeMaxE + x]);
+                st_rlx(&b[kPipeMeta + x], smeta[q * kPipeMaxE + x]);
             }
         }
-        if (lane == 0) st_rlx(&b[0], cntq[q] | (nc << 16));
+        if (lane == 0) st_rlx(&b[0], cnt[q] | (nc << 16));
     }
     wave_fence();
 #pragma unroll
-    for (int w = 0; w < W; ++w)
-        if (slot_w[w] != kNone) {
-            hkey[slot_w[w]] = kNone;
-            hwlo[slot_w[w]] = 0;
-            hwhi[slot_w[w]] = 0;
-            canon[slot_w[w]] = kNone;
-            htot[slot_w[w]] = 0;
-        }
+    for (int s = 0; s < S; ++s)
+#pragma unroll
+        for (int w = 0; w < W; ++w)
+            if (slot_w[s][w] != kNone) {
+                hkey[slot_w[s][w]] = kNone;
+#pragma unroll
+                for (uint32_t i = 0; i < 4; ++i) hw[i * hsz + slot_w[s][w]] = 0;
+                canon[slot_w[s][w]] = kNone;
+                htot[slot_w[s][w]] = 0;
+            }
     wave_fence();
-
-    // 7. send; wait for every partition's header word (its tag, and the request's first
-    //    timestamp), then read the value of each of the lane's entries.  The
-    //    box words are agent-scope stores, written through to the device's point of
-    //    coherence, so their completion orders them before the ring word; a release fence
-    //    would also write back every dirty line of this XCD's L2 (commit servers' value
-    //    words among them) once per request.
+    ts_out[0] = ts_out[1] = 0;
+    if (ballot(on[0] || on[1]) == 0) return;
     wait_stores();
     uint64_t n = 0;
     if (lane == 0) {
@@ -1906,6 +1976,8 @@ __device__ __forceinline__ uint32_t tx_commit_pipe(const Ctx &c, const ServerLay
         st_rlx(&c.ring[n & c.qmask], ((n + 1) << 32) | cw | chunks | kDelegateBit);
     }
     n = bcast64(n);
+
+    // 8. every partition's header word, then the value of each of the lane's nodes
     const uint64_t tw0 = wall_clock64();
     Spin sp;
     const uint64_t *hdr = c.reply + (size_t)cw * kWave;
@@ -1916,37 +1988,41 @@ __device__ __forceinline__ uint32_t tx_commit_pipe(const Ctx &c, const ServerLay
             if (lane == 0) w0 = ld_rlx(&hdr[pt]);
             w0 = bcast64(w0);
             if ((w0 >> 32) == n + 1) break;
-            if (!sp.tick(c, kErrReplyTimeout)) { *fail = true; return 0; }
+            if (!sp.tick(c, kErrReplyTimeout)) { *fail = true; return; }
         }
         h0 = pt == 0 ? w0 : h0;
     }
-    // (a partition stores its header once its values are complete)
     const uint32_t *rv = (const uint32_t *)c.reply_v + (size_t)cw * kPipeParts * kPipeMaxE;
-    int32_t in[RO];
+    int32_t in[S][RO];
 #pragma unroll
-    for (int r = 0; r < RO; ++r) {
-        in[r] = 0;
-        if (pr[r] != kNone) in[r] = (int32_t)ld_rlx(&rv[pr[r] * kPipeMaxE + e[r]]);
-    }
+    for (int s = 0; s < S; ++s)
+#pragma unroll
+        for (int r = 0; r < RO; ++r) {
+            in[s][r] = 0;
+            const uint32_t pl = splace[((uint32_t)s * kWave + lane) * RO + (uint32_t)r];
+            if (pl != kNone) in[s][r] = (int32_t)ld_rlx(&rv[(pl >> 16) * kPipeMaxE + (pl & 0xFFFFu)]);
+        }
     *wait_ticks += wall_clock64() - tw0;
 
-    // 8. the transactions on the values before them, in commit order; the log: a transfer's
-    //    timestamp and every read, an audit's snapshot and the sum of its reads (row 0)
-    int32_t rd[RO], nv[W];
-    transfer_exec<RO, W>(ops, amt, in, rd, wa, nv, !ro);
-    const uint32_t below = (uint32_t)h0 + rank;
-    const uint32_t myts = ro ? (kSnapshotBit | below) : below + 1;
-    if (req) {
-        uint32_t sum = 0;
+    // 9. the transactions on the values before them, in commit order, and their log
 #pragma unroll
-        for (int r = 0; r < RO; ++r) sum += (uint32_t)rd[r];
-        c.log_ts[k] = ro ? below : myts;
-        c.log_v[k] = ro ? (int32_t)sum : rd[0];
+    for (int s = 0; s < S; ++s) {
+        int32_t rd[RO], nv[W];
+        transfer_exec<RO, W>(ops[s], amt[s], in[s], rd, wa[s], nv, !ro[s]);
+        const uint32_t below = (uint32_t)h0 + rank[s];
+        const uint32_t myts = ro[s] ? (kSnapshotBit | below) : below + 1;
+        if (on[s]) {
+            uint32_t sum = 0;
 #pragma unroll
-        for (int r = 1; r < RO; ++r)
-            if (!ro && ops[r] != kNone) c.log_v[(uint64_t)r * c.n_txns + k] = rd[r];
+            for (int r = 0; r < RO; ++r) sum += (uint32_t)rd[r];
+            c.log_ts[kx[s]] = ro[s] ? below : myts;
+            c.log_v[kx[s]] = ro[s] ? (int32_t)sum : rd[0];
+#pragma unroll
+            for (int r = 1; r < RO; ++r)
+                if (!ro[s] && ops[s][r] != kNone) c.log_v[(uint64_t)r * c.n_txns + kx[s]] = rd[r];
+            ts_out[s] = myts;
+        }
     }
-    return req ? myts : 0u;
 }
 
 // Global-memory accesses through an address chosen by a select (an active lane's target or
@@ -2156,7 +2232,7 @@ __device__ __forceinline__ void committer_pipe(const Ctx &c, const ServerLayout 
             const uint32_t j = (uint32_t)ret[g] + 1;
             const uint32_t pos = mod_d(j);
             const uint32_t tail = j < D ? 0u : mod_d(j + 1);
-            const uint32_t ts = base + ((meta[g] >> kPipeRankShift) & 63u) + 1;
+            const uint32_t ts = base + ((meta[g] >> kPipeRankShift) & 127u) + 1;
             if (inst)
                 st_rlx(c.slot + (size_t)item[g] * D + pos,
                        ((uint64_t)ts << 32) | (uint32_t)(val[g] + wgt[g]));

@@ -330,10 +330,11 @@ def test_pipelined_lane_replays(n_items, depth, route, parts, wg):
 def test_pipelined_lane_at_bench_shape():
     """The benchmark's configuration on a 2^20-transaction stream of its workload: 1M Zipf
     0.99 accounts, 4 reads / 2 writes, 4-read audits, depth 8, every transaction on the
-    pipelined lane, 4 commit server workgroups of 16 waves, 80 client waves of which 64 feed
-    the lane: the log replays exactly and nothing aborts."""
+    pipelined lane, 4 commit server workgroups of 16 waves, 160 client waves of which 128 feed
+    the lane (two transactions a lane, requests of up to 128): the log replays exactly and
+    nothing aborts."""
     bank = csmv.Bank(n_txns=1 << 20, seed=1, n_reads=4, n_reads_ro=4, pct_ro=10, zipf=True)
-    st = run_bank(1 << 20, bank, variant="stamp_pipe", depth=8, waves=80, serial_waves=64,
+    st = run_bank(1 << 20, bank, variant="stamp_pipe", depth=8, waves=160, serial_waves=128,
                   wg_waves=16, client_wg_waves=1, commit_parts=4, route=_everything)
     assert st["aborts"] == 0
 
