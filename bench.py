@@ -496,15 +496,15 @@ def main():
     ap.add_argument("--variant", default="stamp_pipe",
                     help="the headline's commit path: stamp_pipe (the pipelined serialized "
                          "lane) or stamp (one committer wave)")
-    ap.add_argument("--waves", type=int, default=int(os.environ.get("CSMV_BENCH_WAVES", "160")))
-    ap.add_argument("--serial-waves", type=int, default=128)
+    ap.add_argument("--waves", type=int, default=int(os.environ.get("CSMV_BENCH_WAVES", "200")))
+    ap.add_argument("--serial-waves", type=int, default=192)
     ap.add_argument("--wg-waves", type=int, default=6,
                     help="waves per workgroup of the one-committer server: the committer, the "
                          "clock advancer and wg_waves - 2 receivers")
     ap.add_argument("--pipe-wg-waves", type=int, default=16,
                     help="waves per workgroup of each pipelined commit server: w / 4 "
                          "receivers (1 below 8 waves), one publisher, the rest committers")
-    ap.add_argument("--commit-parts", type=int, default=4,
+    ap.add_argument("--commit-parts", type=int, default=8,
                     help="commit server workgroups of the pipelined lane (items split by hash)")
     ap.add_argument("--one-waves", type=int, default=64,
                     help="client waves of the one-committer line (and the reference variants)")
@@ -540,7 +540,7 @@ def main():
                     help="client waves of the serving loop's bank class on the pipelined lane")
     ap.add_argument("--serve-pipe-budget", type=int, default=196608,
                     help="admission cap of the serving runs with the pipelined bank lane")
-    ap.add_argument("--serial-waves-all", type=int, default=128,
+    ap.add_argument("--serial-waves-all", type=int, default=192,
                     help="serialized-lane waves of the run with every transaction routed")
     ap.add_argument("--no-one", action="store_true",
                     help="skip the one-committer serialized-lane line")

@@ -35,9 +35,8 @@ constexpr int kServeBankR = 64;
 // reply values per client wave: the serialized lane's rows (a new value and a previous
 // head per write and lane) or the pipelined lane's entry values of every partition
 constexpr uint32_t kReplyVWords = std::max<uint32_t>(2 * kBankW * kWave, kPipeParts * kPipeMaxE);
-// the pipelined lane's boxes (one per partition) fit a client wave's request mailbox
-static_assert(mbox_rows<kServeBankR, kBankW>() * kWave >= kPipeParts * kPipeBox,
-              "pipelined boxes exceed the request mailbox");
+// the pipelined lane's boxes of a client wave (one per partition), in mailbox rows
+constexpr uint32_t kPipeBoxRows = (kPipeParts * kPipeBox + kWave - 1) / kWave;
 
 // ---- stream generation ----------------------------------------------------------------------
 __host__ __device__ __forceinline__ uint64_t splitmix64(uint64_t x)
@@ -281,6 +280,14 @@ __device__ __forceinline__ void bank_client(const Ctx &c, uint32_t cw)
         const uint32_t kind = op & 0xFFu, amt = (op >> 8) & 0xFFu, nr = (op >> 16) & 0xFFu;
         const bool ro = has && kind == kKindAudit;
         const bool upd = has && kind == kKindTransfer;
+        if constexpr (kPipe) {
+            // the pipelined lane's servers take only staged entry lists: a transfer left on
+            // this lane is a routing error
+            if (ballot(upd)) {
+                raise_error(c, kErrPipeNormal);
+                break;
+            }
+        }
         int32_t reads[R];
 #pragma unroll
         for (int j = 0; j < R; ++j) reads[j] = 0;
@@ -448,31 +455,49 @@ __device__ __forceinline__ void bank_serial_client(const Ctx &c, uint32_t cw)
                                           __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// The pipelined lane's client wave: two transactions a lane (slot 0 the older), a request
-// of up to 128 (tx_commit_pipe).  A transaction the request could not carry (its
-// partition's list was full) stays in its slot for the next request; nothing aborts.
+// The pipelined lane's client wave: kPipeSlots transactions a lane (the lower slots the
+// older), a request of up to 64 kPipeSlots (tx_commit_pipe).  A transaction the request
+// could not carry (its partition's list was full) stays in its slot for the next request;
+// nothing aborts.
 template <int R, int W>
 __device__ __forceinline__ void bank_pipe_client(const Ctx &c, const ServerLayout &L, uint32_t *scr,
                                                  uint32_t cw)
 {
+    constexpr int S = kPipeSlots;
     const uint32_t lane = lane_id();
     const uint64_t NH = c.n_hot;
-    pipe_client_init<R, W>(L, scr);
-    bool has[2] = {false, false}, exhausted = false;
-    uint64_t k[2] = {0, 0};
-    uint32_t op[2] = {0, 0};
+    pipe_client_init<R, W, S>(scr);
+    bool has[S], exhausted = false;
+    uint64_t k[S];
+    uint32_t op[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        has[s] = false;
+        k[s] = 0;
+        op[s] = 0;
+    }
+    // the held transactions to the lowest slots, in order
+    auto compact = [&]() {
+#pragma unroll
+        for (int pass = 0; pass + 1 < S; ++pass)
+#pragma unroll
+            for (int s = 0; s + 1 < S; ++s)
+                if (!has[s] && has[s + 1]) {
+                    has[s] = true;
+                    has[s + 1] = false;
+                    k[s] = k[s + 1];
+                    op[s] = op[s + 1];
+                }
+    };
     uint64_t n_upd = 0, n_ro = 0, t_reply = 0, t_round = 0, rounds = 0;
     for (;;) {
         if (ld_rlx(c.err)) break;
         const uint64_t tr0 = wall_clock64();
-        if (!has[0] && has[1]) {
-            has[0] = true;
-            has[1] = false;
-            k[0] = k[1];
-            op[0] = op[1];
-        }
+        compact();
         if (!exhausted) {
-            const uint32_t want = (has[0] ? 0u : 1u) + (has[1] ? 0u : 1u);
+            uint32_t want = 0;
+#pragma unroll
+            for (int s = 0; s < S; ++s) want += has[s] ? 0u : 1u;
             const uint32_t incl = wave_incl_scan(want);
             const uint32_t total = readlane(incl, 63);
             if (total) {
@@ -482,7 +507,7 @@ __device__ __forceinline__ void bank_pipe_client(const Ctx &c, const ServerLayou
                 if (base + total >= NH) exhausted = true;
                 uint64_t mine = base + (incl - want);
 #pragma unroll
-                for (int s = 0; s < 2; ++s)
+                for (int s = 0; s < S; ++s)
                     if (!has[s]) {
                         if (mine < NH) {
                             has[s] = true;
@@ -491,24 +516,22 @@ __device__ __forceinline__ void bank_pipe_client(const Ctx &c, const ServerLayou
                         }
                         ++mine;
                     }
-                if (!has[0] && has[1]) {
-                    has[0] = true;
-                    has[1] = false;
-                    k[0] = k[1];
-                    op[0] = op[1];
-                }
+                compact();
             }
         }
-        if (ballot(has[0] || has[1]) == 0) break;
-        Tx<R, W> tx[2];
-        delegated_tx<R, W>(c, tx[0], has[0], k[0], op[0]);
-        delegated_tx<R, W>(c, tx[1], has[1], k[1], op[1]);
+        bool any = false;
+#pragma unroll
+        for (int s = 0; s < S; ++s) any = any || has[s];
+        if (ballot(any) == 0) break;
+        Tx<R, W> tx[S];
+#pragma unroll
+        for (int s = 0; s < S; ++s) delegated_tx<R, W>(c, tx[s], has[s], k[s], op[s]);
         bool fail = false;
-        uint32_t ts[2];
-        tx_commit_pipe<R, W>(c, L, scr, cw, tx, has, ts, &fail, &t_reply);
+        uint32_t ts[S];
+        tx_commit_pipe<R, W, S>(c, L, scr, cw, tx, has, ts, &fail, &t_reply);
         if (fail) break;
 #pragma unroll
-        for (int s = 0; s < 2; ++s)
+        for (int s = 0; s < S; ++s)
             if (has[s] && ts[s]) {
                 c.log_ab[k[s]] = 0;
                 has[s] = false;
@@ -562,7 +585,7 @@ __global__ void __launch_bounds__(1024) bank_pipe_kernel(Ctx c, ServerLayout L)
     const uint32_t cw = (blockIdx.x - L.parts) * c.client_wg_waves + wave;
     if (cw >= c.n_client_waves) return;
     if (cw >= c.n_client_waves - c.n_serial_waves)
-        bank_pipe_client<R, W>(c, L, lds + wave * pipe_client_words(L.hsz, R, W), cw);
+        bank_pipe_client<R, W>(c, L, lds + wave * pipe_client_words(R, W, kPipeSlots), cw);
     else bank_client<R, W, true>(c, cw);
 }
 
@@ -789,7 +812,7 @@ csmv_status normalize_config(csmv_config &cfg)
     if (cfg.stage_ahead == 0) cfg.stage_ahead = 3;
     if (cfg.client_wg_waves == 0) cfg.client_wg_waves = cfg.wg_waves;
     if (cfg.commit_parts == 0) cfg.commit_parts = 1;
-    if (cfg.commit_parts > 4 || (cfg.commit_parts > 1 && !(cfg.variant & kSerialPipe))) return CSMV_E_ARG;
+    if (cfg.commit_parts > kPipeParts || (cfg.commit_parts > 1 && !(cfg.variant & kSerialPipe))) return CSMV_E_ARG;
     if (cfg.n_serial_waves >= cfg.client_waves && cfg.n_serial_waves) return CSMV_E_ARG;
     if (cfg.n_items == 0 || cfg.depth == 0 || cfg.depth > 64 || cfg.client_waves == 0 ||
         cfg.wg_waves < 3 || cfg.wg_waves > ((cfg.variant & kSerialPipe) ? 16u : 8u) ||
@@ -860,7 +883,7 @@ ServerLayout server_layout_at(const csmv_config &cfg, uint32_t lds_max, uint32_t
         w += pipe_words();
         L.atr_off = L.bucket_off = L.val_off = w;
         L.words = std::max<uint32_t>(w, cfg.client_wg_waves * (R > 16 ? pipe_wide_words(L.hsz, R)
-                                                                      : pipe_client_words(L.hsz, R, W)));
+                                                                      : pipe_client_words(R, W, kPipeSlots)));
         L.parts = cfg.commit_parts;
         return L;
     }
@@ -1024,7 +1047,7 @@ csmv_status csmv_create(const csmv_config *cfg_in, csmv_engine **out)
     }
     // request mailboxes sized for the widest wire of the drivers
     const uint32_t rows = std::max({mbox_rows<kBankR, kBankW>(), mbox_rows<kMcR, kMcW>(),
-                                    mbox_rows<kServeBankR, kBankW>()});
+                                    mbox_rows<kServeBankR, kBankW>(), kPipeBoxRows});
     static_assert(kMcW <= kBankW, "reply values are sized for the bank wire");
     e->mbox_rows = rows;
     if ((s = dalloc(e, &e->ht, cfg.n_items)) ||

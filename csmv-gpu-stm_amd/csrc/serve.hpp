@@ -469,7 +469,9 @@ __device__ __forceinline__ uint32_t tx_commit_pipe_wide(const Ctx &c, const Serv
 
     // 2. the lanes this request carries: a lane-order prefix whose nodes keep every
     //    partition's list within kPipeMaxE (a leaf on an own write is no node)
-    uint32_t cq[kPipeParts] = {0u, 0u, 0u, 0u};
+    uint32_t cq[kPipeParts];
+#pragma unroll
+    for (uint32_t t = 0; t < kPipeParts; ++t) cq[t] = 0;
     for (uint32_t j = 0; j < nr; ++j) {
         const uint32_t x = rsx[j * kWave + lane];
         if (j >= CN && (x == a0 || x == a1)) continue;
@@ -478,12 +480,13 @@ __device__ __forceinline__ uint32_t tx_commit_pipe_wide(const Ctx &c, const Serv
         for (uint32_t t = 0; t < kPipeParts; ++t) cq[t] += q == t ? 1u : 0u;
     }
     // (every lane takes part in the scans: a lane-prefix scan reads the lanes before it)
-    uint32_t pre[kPipeParts];
-#pragma unroll
-    for (uint32_t t = 0; t < kPipeParts; ++t) pre[t] = wave_incl_scan(cq[t]);
     bool on = req;
 #pragma unroll
-    for (uint32_t t = 0; t < kPipeParts; ++t) on = on && pre[t] <= kPipeMaxE;
+    for (uint32_t t = 0; t < kPipeParts; ++t) {
+        if (t >= P) break;
+        const uint32_t pre = wave_incl_scan(cq[t]);
+        on = on && pre <= kPipeMaxE;
+    }
 
     // 3. the write set into the hash (item -> writer lanes)
     uint32_t wa[W];
@@ -634,12 +637,15 @@ __device__ __forceinline__ uint32_t tx_commit_pipe_wide(const Ctx &c, const Serv
 
     // 5. the entries, in rounds of one node a lane (the chain nodes first, then the
     //    leaves): entry e of partition q's list is staged at q * kPipeMaxE + e
-    uint32_t cnt[kPipeParts] = {0u, 0u, 0u, 0u};
+    uint32_t cnt[kPipeParts];
+#pragma unroll
+    for (uint32_t t = 0; t < kPipeParts; ++t) cnt[t] = 0;
     auto place = [&](bool node_on, uint32_t x) -> uint32_t {
         const uint32_t q = node_on ? pipe_part(x, P) : kNone;
         uint32_t e = 0;
 #pragma unroll
         for (uint32_t t = 0; t < kPipeParts; ++t) {
+            if (t >= P) break;
             const uint64_t mq = ballot(q == t);
             if (q == t) e = cnt[t] + (uint32_t)__builtin_popcountll(mq & lt);
             cnt[t] += (uint32_t)__builtin_popcountll(mq);
@@ -717,10 +723,10 @@ __device__ __forceinline__ uint32_t tx_commit_pipe_wide(const Ctx &c, const Serv
     //    back the XCD's dirty L2 lines)
     uint32_t *bx = c.mbox + (size_t)cw * kPipeParts * kPipeBox;
     const uint32_t nc = (uint32_t)__builtin_popcountll(umask);
-    uint32_t chunks = 0;
+    uint64_t chunks = 0;
 #pragma unroll
     for (uint32_t q = 0; q < kPipeParts; ++q) {
-        chunks |= ((cnt[q] + kPipeChunk - 1) / kPipeChunk) << (kPipeChunkShift + 4 * q);
+        chunks |= pipe_ring_chunks(cnt[q], q);
         if (q >= P) continue;
         uint32_t *b = bx + q * kPipeBox;
 #pragma unroll
@@ -750,7 +756,7 @@ __device__ __forceinline__ uint32_t tx_commit_pipe_wide(const Ctx &c, const Serv
     uint64_t n = 0;
     if (lane == 0) {
         n = fetch_add(c.req_tail, (uint64_t)1);
-        st_rlx(&c.ring[n & c.qmask], ((n + 1) << 32) | cw | chunks | kDelegateBit);
+        st_rlx(&c.ring[n & c.qmask], (pipe_ring_tag(n) << kPipeTagShift) | chunks | cw);
     }
     n = bcast64(n);
 

@@ -104,15 +104,27 @@ __device__ __forceinline__ uint32_t pipe_part(uint32_t x, uint32_t parts)
 // installs the item's last write) and that last writer's rank among the update lanes.
 // A partition replies with a value per entry (the value the node reads) and, once those
 // stores are complete, a header word per client wave (tag | the request's first timestamp).
-// The ring word of a request carries the client wave (bits 0-11) and, per partition, its
+// The ring word of a request carries the client wave (bits 0-11), per partition its
 // entries in 32-entry chunks (4 bits each from bit 12), so a receiver loads what the list
-// holds in one round.
-constexpr uint32_t kPipeParts = 4;                       // commit partitions at most
+// holds in one round, and its tag ((n + 1) mod 2^20, from bit 44; the ring holds fewer
+// words).
+constexpr uint32_t kPipeParts = 8;                       // commit partitions at most
 constexpr uint32_t kPipeMaxE = 4 * kWave;                // entries of one request
 constexpr uint32_t kPipeItem = kWave, kPipeWgt = kPipeItem + kPipeMaxE, kPipeMeta = kPipeWgt + kPipeMaxE;
 constexpr uint32_t kPipeBox = kPipeMeta + kPipeMaxE;    // words per (client wave, partition)
 constexpr uint32_t kPipeRootBit = 1u << 8, kPipeInstBit = 1u << 9, kPipeRankShift = 10;
-constexpr uint32_t kPipeCwMask = 0xFFFu, kPipeChunkShift = 12, kPipeChunk = 32;
+constexpr uint32_t kPipeCwMask = 0xFFFu, kPipeChunkShift = 12, kPipeChunk = 32, kPipeTagShift = 44;
+__host__ __device__ constexpr uint64_t pipe_ring_tag(uint64_t n) { return (n + 1) & 0xFFFFFull; }
+// entries of partition q in a ring word
+__host__ __device__ constexpr uint32_t pipe_ring_entries(uint64_t w, uint32_t q)
+{
+    return (uint32_t)((w >> (kPipeChunkShift + 4 * q)) & 0xFu) * kPipeChunk;
+}
+// the ring word's chunk field for partition q's cnt entries
+__host__ __device__ constexpr uint64_t pipe_ring_chunks(uint32_t cnt, uint32_t q)
+{
+    return (uint64_t)((cnt + kPipeChunk - 1) / kPipeChunk) << (kPipeChunkShift + 4 * q);
+}
 // Install records of this many requests wait for the publisher: per request the item and
 // head word of each entry, the entry count, the timestamp after the request and the ready
 // word.
@@ -1584,59 +1596,75 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
 // commit (bank/csmv_sh/API.cu:88-272) and its clients write back their own versions.
 
 // Client side (TXCommit for a wave of routed transfers, bank/csmv_sh/exe.cu:184-201 as the
-// body), two transactions a lane: a request carries up to 128 transactions in the order of
+// body), S transactions a lane: a request carries up to 64 S transactions in the order of
 // virtual lanes v = s * 64 + lane (every lane's first transaction, then every lane's
-// second), so each token step, atomic round, reply and hand-off of the commit servers
-// serves 128.  The wave stages the request's producer chains in its LDS scratch (writer
-// masks of 128 bits, producer codes virtual lane | write slot << 7, nodes v * R + r), writes
+// second, ...), so each token step, atomic round, reply and hand-off of the commit servers
+// serves 64 S.  The wave stages the request's producer chains in its LDS scratch (writer
+// masks of 64 S bits, producer codes virtual lane | write slot << 8, nodes v * R + r), writes
 // each partition's entry list to the wave's boxes, sends, and waits for the replies: the
 // values the nodes read, from which it runs its transactions and logs them (commit
 // timestamp, or kSnapshotBit | snapshot for a read-only transaction, in ts_out; 0 for one
 // the request did not carry).  A request carries the virtual lanes, in order, whose nodes
 // keep every partition's list within kPipeMaxE; the others wait for the next request.
 // The server installs the versions.  Scratch (pipe_client_words, cleared by
-// pipe_client_init): the write hash (key, four writer-mask words, canonical node,
-// request-wide delta, last writer's rank) of L.hsz slots (at most half full), two buffers of
-// (next node, weight) per node, its entry and hash slot, per write slot its superseded flag,
-// and the
-// staged lists (a block of kPipeMaxE per partition).
-__host__ __device__ constexpr uint32_t pipe_client_words(uint32_t hsz, uint32_t R, uint32_t W)
+// pipe_client_init): the write hash (key, 2 S writer-mask words, canonical node,
+// request-wide delta, last writer's rank) of pipe_client_hsz slots, two buffers of (next
+// node, weight) per node, its entry and hash slot, per write slot its superseded flag, and
+// the staged lists (a block of kPipeMaxE per partition).
+// Transactions a lane of the pipelined client, and the write-hash size its request needs
+// (the writes fill at most half of it).
+constexpr int kPipeSlots = 3;
+__host__ __device__ constexpr uint32_t pow2_ceil_u32(uint32_t v)
 {
-    return 8 * hsz + 6 * 2 * kWave * R + 2 * kWave * W + 3 * kPipeParts * kPipeMaxE;
+    uint32_t q = 1;
+    while (q < v) q <<= 1;
+    return q;
+}
+__host__ __device__ constexpr uint32_t pipe_client_hsz(uint32_t W, uint32_t S)
+{
+    return pow2_ceil_u32(2 * S * kWave * W);
+}
+__host__ __device__ constexpr uint32_t pipe_client_words(uint32_t R, uint32_t W, uint32_t S)
+{
+    return (4 + 2 * S) * pipe_client_hsz(W, S) + 6 * S * kWave * R + S * kWave * W +
+           3 * kPipeParts * kPipeMaxE;
 }
 
-template <int R, int W>
-__device__ __forceinline__ void pipe_client_init(const ServerLayout &L, uint32_t *scr)
+template <int R, int W, int S>
+__device__ __forceinline__ void pipe_client_init(uint32_t *scr)
 {
-    const uint32_t lane = lane_id(), hsz = L.hsz;
+    constexpr uint32_t hsz = pipe_client_hsz(W, S);
+    const uint32_t lane = lane_id();
     for (uint32_t i = lane; i < hsz; i += kWave) {
         scr[i] = kNone;
 #pragma unroll
-        for (uint32_t q = 1; q <= 4; ++q) scr[q * hsz + i] = 0;
-        scr[5 * hsz + i] = kNone;
-        scr[6 * hsz + i] = 0;
+        for (uint32_t q = 1; q <= 2 * S; ++q) scr[q * hsz + i] = 0;
+        scr[(1 + 2 * S) * hsz + i] = kNone;
+        scr[(2 + 2 * S) * hsz + i] = 0;
     }
     wave_fence();
 }
 
-template <int R, int W>
+template <int R, int W, int S>
 __device__ __forceinline__ void tx_commit_pipe(const Ctx &c, const ServerLayout &L, uint32_t *scr,
-                                                uint32_t cw, const Tx<R, W> (&tx)[2],
-                                                const bool (&req)[2], uint32_t (&ts_out)[2],
+                                                uint32_t cw, const Tx<R, W> (&tx)[S],
+                                                const bool (&req)[S], uint32_t (&ts_out)[S],
                                                 bool *fail, uint64_t *wait_ticks)
 {
-    static_assert(!kWideWire<R> && W == 2 && R * kWave <= kPipeMaxE,
+    static_assert(!kWideWire<R> && W == 2 && R * kWave <= kPipeMaxE && S >= 1 && S <= 4,
                   "the pipelined lane carries bank transfers");
-    constexpr int RO = R, S = 2;
+    constexpr int RO = R;
     constexpr uint32_t kNoP = 0xFFFFu;            // no producer
+    constexpr uint32_t NN = S * kWave * RO;       // nodes
+    constexpr uint32_t hsz = pipe_client_hsz(W, S), hmask = hsz - 1;
     const uint32_t lane = lane_id();
     const uint64_t lt = (1ull << lane) - 1;
-    const uint32_t hsz = L.hsz, hmask = hsz - 1, P = L.parts;
+    const uint32_t P = L.parts;
     uint32_t *hkey = scr, *hw = scr + hsz;        // hw[i * hsz + slot]: virtual lanes 32 i ..
-    uint32_t *canon = scr + 5 * hsz;
-    int32_t *htot = (int32_t *)(scr + 6 * hsz);
-    uint32_t *hrank = scr + 7 * hsz;
-    uint32_t *snext = scr + 8 * hsz;              // [2][2 * 64 * R] by node v * R + r
+    uint32_t *canon = scr + (1 + 2 * S) * hsz;
+    int32_t *htot = (int32_t *)(scr + (2 + 2 * S) * hsz);
+    uint32_t *hrank = scr + (3 + 2 * S) * hsz;
+    uint32_t *snext = scr + (4 + 2 * S) * hsz;    // [2][S * 64 * R] by node v * R + r
     int32_t *sw = (int32_t *)(snext + S * kWave * RO);   // (interleaved with snext's buffers)
     uint32_t *e2 = snext + 4 * S * kWave * RO;
     uint32_t *sfs = e2 + S * kWave * RO;          // each node's write-hash slot (kNone: unwritten)
@@ -1661,27 +1689,27 @@ __device__ __forceinline__ void tx_commit_pipe(const Ctx &c, const ServerLayout 
     // 1. the virtual lanes this request carries: an order prefix whose nodes keep every
     //    partition's list within kPipeMaxE (every lane takes part in the scans)
     {
-        uint32_t cq[S][kPipeParts];
+        uint32_t pq[S][RO];
 #pragma unroll
         for (int s = 0; s < S; ++s)
 #pragma unroll
-            for (uint32_t t = 0; t < kPipeParts; ++t) {
-                uint32_t n = 0;
+            for (int r = 0; r < RO; ++r) pq[s][r] = ops[s][r] != kNone ? pipe_part(ops[s][r], P) : kNone;
 #pragma unroll
-                for (int r = 0; r < RO; ++r)
-                    n += (ops[s][r] != kNone && pipe_part(ops[s][r], P) == t) ? 1u : 0u;
-                cq[s][t] = n;
-            }
-        bool ok0 = req[0], ok1 = req[1];
+        for (int s = 0; s < S; ++s) on[s] = req[s];
 #pragma unroll
         for (uint32_t t = 0; t < kPipeParts; ++t) {
-            const uint32_t p0 = wave_incl_scan(cq[0][t]);
-            const uint32_t p1 = readlane(p0, 63) + wave_incl_scan(cq[1][t]);
-            ok0 = ok0 && p0 <= kPipeMaxE;
-            ok1 = ok1 && p1 <= kPipeMaxE;
+            if (t >= P) break;
+            uint32_t below = 0;
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                uint32_t n = 0;
+#pragma unroll
+                for (int r = 0; r < RO; ++r) n += pq[s][r] == t ? 1u : 0u;
+                const uint32_t ps = below + wave_incl_scan(n);
+                on[s] = on[s] && ps <= kPipeMaxE;
+                below = readlane(ps, 63);
+            }
         }
-        on[0] = ok0;
-        on[1] = ok1;
     }
 #pragma unroll
     for (int s = 0; s < S; ++s) {
@@ -1690,10 +1718,13 @@ __device__ __forceinline__ void tx_commit_pipe(const Ctx &c, const ServerLayout 
 #pragma unroll
         for (int w = 0; w < W; ++w) wa[s][w] = on[s] ? wa[s][w] : kNone;
     }
-    const uint64_t u0 = ballot(on[0] && !ro[0]), u1 = ballot(on[1] && !ro[1]);
-    const uint32_t rank[S] = {(uint32_t)__builtin_popcountll(u0 & lt),
-                              (uint32_t)__builtin_popcountll(u0) + (uint32_t)__builtin_popcountll(u1 & lt)};
-    const uint32_t nc = (uint32_t)__builtin_popcountll(u0) + (uint32_t)__builtin_popcountll(u1);
+    uint32_t rank[S], nc = 0;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const uint64_t u = ballot(on[s] && !ro[s]);
+        rank[s] = nc + (uint32_t)__builtin_popcountll(u & lt);
+        nc += (uint32_t)__builtin_popcountll(u);
+    }
     int32_t d0[S], d1[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) {
@@ -1790,9 +1821,13 @@ __device__ __forceinline__ void tx_commit_pipe(const Ctx &c, const ServerLayout 
             uint32_t code = kNoP;
 #pragma unroll
             for (int w = 0; w < W; ++w) {
-                const uint32_t g0 = lane_gather(wa[0][w], srcl), g1 = lane_gather(wa[1][w], srcl);
-                const uint32_t pw = (pv != kNoP && (pv >> 6)) ? g1 : g0;
-                code = (pv != kNoP && pw == ops[s][r]) ? (pv | ((uint32_t)w << 7)) : code;
+                uint32_t pw = 0;
+#pragma unroll
+                for (int t = 0; t < S; ++t) {
+                    const uint32_t g = lane_gather(wa[t][w], srcl);
+                    pw = (pv >> 6) == (uint32_t)t ? g : pw;
+                }
+                code = (pv != kNoP && pw == ops[s][r]) ? (pv | ((uint32_t)w << 8)) : code;
             }
             pj[s][r] = code;
         }
@@ -1819,19 +1854,23 @@ __device__ __forceinline__ void tx_commit_pipe(const Ctx &c, const ServerLayout 
 #pragma unroll
         for (int w = 0; w < W; ++w) ssup[((uint32_t)s * kWave + lane) * W + w] = 0;
     wave_fence();
-    uint32_t *nA = snext, *nB = snext + 2 * S * kWave * RO;
-    int32_t *wA = sw, *wB = sw + 2 * S * kWave * RO;
+    uint32_t *nA = snext, *nB = snext + 2 * NN;
+    int32_t *wA = sw, *wB = sw + 2 * NN;
 #pragma unroll
     for (int s = 0; s < S; ++s)
 #pragma unroll
         for (int r = 0; r < RO; ++r) {
             const uint32_t node = ((uint32_t)s * kWave + lane) * RO + (uint32_t)r;
             const bool has_p = pj[s][r] != kNoP;
-            const uint32_t pv = pj[s][r] & 127u, pw = pj[s][r] >> 7;
+            const uint32_t pv = pj[s][r] & 0xFFu, pw = pj[s][r] >> 8;
             const uint32_t srcl = has_p ? (pv & 63u) : lane;
-            const int32_t a0 = (int32_t)lane_gather((uint32_t)d0[0], srcl), a1 = (int32_t)lane_gather((uint32_t)d1[0], srcl);
-            const int32_t b0 = (int32_t)lane_gather((uint32_t)d0[1], srcl), b1 = (int32_t)lane_gather((uint32_t)d1[1], srcl);
-            const int32_t dd = (pv >> 6) ? (pw == 0 ? b0 : b1) : (pw == 0 ? a0 : a1);
+            int32_t dd = 0;
+#pragma unroll
+            for (int t = 0; t < S; ++t) {
+                const int32_t g0 = (int32_t)lane_gather((uint32_t)d0[t], srcl);
+                const int32_t g1 = (int32_t)lane_gather((uint32_t)d1[t], srcl);
+                dd = (pv >> 6) == (uint32_t)t ? (pw == 0 ? g0 : g1) : dd;
+            }
             nA[node] = has_p ? pv * RO + pw : node;
             wA[node] = has_p ? dd : 0;
             bool mine_w = false;
@@ -1885,7 +1924,9 @@ __device__ __forceinline__ void tx_commit_pipe(const Ctx &c, const ServerLayout 
     //    staged at q * kPipeMaxE + e; the node's place (q << 16 | e, kept in the next-node
     //    array, which the chains no longer need) for reading its value back.  A node of an
     //    item no one writes is its own root.
-    uint32_t cnt[kPipeParts] = {0u, 0u, 0u, 0u};
+    uint32_t cnt[kPipeParts];
+#pragma unroll
+    for (uint32_t t = 0; t < kPipeParts; ++t) cnt[t] = 0;
     uint32_t pe[S][RO];
     uint32_t *const splace = nB;
 #pragma unroll
@@ -1896,6 +1937,7 @@ __device__ __forceinline__ void tx_commit_pipe(const Ctx &c, const ServerLayout 
             uint32_t e = 0;
 #pragma unroll
             for (uint32_t t = 0; t < kPipeParts; ++t) {
+                if (t >= P) break;
                 const uint64_t mq = ballot(q == t);
                 if (q == t) e = cnt[t] + (uint32_t)__builtin_popcountll(mq & lt);
                 cnt[t] += (uint32_t)__builtin_popcountll(mq);
@@ -1917,7 +1959,7 @@ __device__ __forceinline__ void tx_commit_pipe(const Ctx &c, const ServerLayout 
             const uint32_t rt = written ? cn : node;
             const bool is_canon = written && cn == node;
             const bool is_root = rt == node;
-            const uint32_t meta = e2[rt & (S * kWave * RO - 1)] | (is_root ? kPipeRootBit : 0u) |
+            const uint32_t meta = e2[rt < NN ? rt : 0u] | (is_root ? kPipeRootBit : 0u) |
                                   (is_canon ? kPipeInstBit | (hrank[at] << kPipeRankShift) : 0u);
             const int32_t wout = is_canon ? htot[at] : wA[node];
             if (pe[s][r] != kNone) {
@@ -1937,10 +1979,10 @@ __device__ __forceinline__ void tx_commit_pipe(const Ctx &c, const ServerLayout 
     uint64_t bxa = bcast64((uint64_t)(uintptr_t)(c.mbox + (size_t)cw * kPipeParts * kPipeBox));
     asm volatile("" : "+s"(bxa));
     uint32_t *bx = (uint32_t *)(uintptr_t)bxa;
-    uint32_t chunks = 0;
+    uint64_t chunks = 0;
 #pragma unroll
     for (uint32_t q = 0; q < kPipeParts; ++q) {
-        chunks |= ((cnt[q] + kPipeChunk - 1) / kPipeChunk) << (kPipeChunkShift + 4 * q);
+        chunks |= pipe_ring_chunks(cnt[q], q);
         if (q >= P) continue;
         uint32_t *b = bx + q * kPipeBox;
 #pragma unroll
@@ -1962,18 +2004,23 @@ __device__ __forceinline__ void tx_commit_pipe(const Ctx &c, const ServerLayout 
             if (slot_w[s][w] != kNone) {
                 hkey[slot_w[s][w]] = kNone;
 #pragma unroll
-                for (uint32_t i = 0; i < 4; ++i) hw[i * hsz + slot_w[s][w]] = 0;
+                for (uint32_t i = 0; i < 2 * S; ++i) hw[i * hsz + slot_w[s][w]] = 0;
                 canon[slot_w[s][w]] = kNone;
                 htot[slot_w[s][w]] = 0;
             }
     wave_fence();
-    ts_out[0] = ts_out[1] = 0;
-    if (ballot(on[0] || on[1]) == 0) return;
+    bool any = false;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        ts_out[s] = 0;
+        any = any || on[s];
+    }
+    if (ballot(any) == 0) return;
     wait_stores();
     uint64_t n = 0;
     if (lane == 0) {
         n = fetch_add(c.req_tail, (uint64_t)1);
-        st_rlx(&c.ring[n & c.qmask], ((n + 1) << 32) | cw | chunks | kDelegateBit);
+        st_rlx(&c.ring[n & c.qmask], (pipe_ring_tag(n) << kPipeTagShift) | chunks | cw);
     }
     n = bcast64(n);
 
@@ -2136,11 +2183,7 @@ __device__ __forceinline__ void committer_pipe(const Ctx &c, const ServerLayout 
             }
         }
         if (end || bad) break;
-        const uint32_t cwr = mbwave[m];
-        if ((cwr & kDelegateBit) == 0) {
-            raise_error(c, kErrPipeNormal);
-            break;
-        }
+        const uint32_t cw = mbwave[m];
         const uint64_t t1 = role_clock();
         const uint32_t *box = lds + L.mb_off + m * L.mb_rows * kWave;
         const uint32_t hdr = box[0];
@@ -2232,7 +2275,7 @@ __device__ __forceinline__ void committer_pipe(const Ctx &c, const ServerLayout 
             const uint32_t j = (uint32_t)ret[g] + 1;
             const uint32_t pos = mod_d(j);
             const uint32_t tail = j < D ? 0u : mod_d(j + 1);
-            const uint32_t ts = base + ((meta[g] >> kPipeRankShift) & 127u) + 1;
+            const uint32_t ts = base + ((meta[g] >> kPipeRankShift) & 255u) + 1;
             if (inst)
                 st_rlx(c.slot + (size_t)item[g] * D + pos,
                        ((uint64_t)ts << 32) | (uint32_t)(val[g] + wgt[g]));
@@ -2244,7 +2287,6 @@ __device__ __forceinline__ void committer_pipe(const Ctx &c, const ServerLayout 
         if (lane == 0) icnt[qi] = cnt;
         // the reply: each entry's value now; this partition's header word (tag and the
         // request's first timestamp) once the values are complete (hand_over)
-        const uint32_t cw = cwr & ~kDelegateBit;
         uint32_t *rv = (uint32_t *)c.reply_v + ((size_t)cw * kPipeParts + part) * kPipeMaxE;
 #pragma unroll
         for (int g = 0; g < G; ++g)
@@ -2403,10 +2445,11 @@ __device__ __forceinline__ void receiver_pipe(const Ctx &c, const ServerLayout &
     uint32_t va[3 * G], vb[3 * G];
     uint32_t ha[1] = {0}, hb[1] = {0};
     uint32_t np = 0;            // requests in flight: pn, pn + M
-    uint32_t pn = 0, pca = 0, pcb = 0;
+    uint32_t pn = 0;
+    uint64_t pca = 0, pcb = 0;
     LaneSum r_free, r_ring;     // phase timers: waiting for a mailbox, for the ring
     // stores a loaded list into the mailbox of request q
-    auto stage = [&](uint32_t q, uint32_t cwr, const uint32_t *h, const uint32_t *v) -> bool {
+    auto stage = [&](uint32_t q, uint64_t cwr, const uint32_t *h, const uint32_t *v) -> bool {
         const uint32_t mbi = q % L.nmb;
         if (q >= L.nmb) {
             const uint64_t tf = phase_clock();
@@ -2426,7 +2469,7 @@ __device__ __forceinline__ void receiver_pipe(const Ctx &c, const ServerLayout &
         }
         if (lane == 0) {
             dst[0] = h[0];
-            mbwave[mbi] = (cwr & kPipeCwMask) | (cwr & kDelegateBit);
+            mbwave[mbi] = (uint32_t)cwr & kPipeCwMask;
         }
         wg_fence();
         if (lane == 0) lds_st_rel(&mbst[mbi], q + 1);
@@ -2436,9 +2479,9 @@ __device__ __forceinline__ void receiver_pipe(const Ctx &c, const ServerLayout &
     // that word, so issued after it; agent-scope loads, which read the device's point of
     // coherence: no acquire, whose invalidation would take this XCD's L2 lines, the
     // committers' value words among them)
-    auto fetch = [&](uint32_t cwr, uint32_t *h, uint32_t *v) {
-        const uint32_t C = ((cwr >> (kPipeChunkShift + 4 * part)) & 0xFu) * kPipeChunk;
-        const uint32_t *ps = c.mbox + ((size_t)(cwr & kPipeCwMask) * kPipeParts + part) * kPipeBox;
+    auto fetch = [&](uint64_t cwr, uint32_t *h, uint32_t *v) {
+        const uint32_t C = pipe_ring_entries(cwr, part);
+        const uint32_t *ps = c.mbox + ((size_t)((uint32_t)cwr & kPipeCwMask) * kPipeParts + part) * kPipeBox;
         h[0] = ld_rlx(&ps[0]);
 #pragma unroll
         for (int g = 0; g < G; ++g) {
@@ -2460,13 +2503,13 @@ __device__ __forceinline__ void receiver_pipe(const Ctx &c, const ServerLayout &
         if (np == 2 && !stage(pn + M, pcb, hb, vb)) break;
         np = 0;
         const uint64_t a = bcast64(e0), b = bcast64(e1);   // one word each: wave-wide branches
-        if ((a >> 32) == (uint64_t)n + 1) {
+        if ((a >> kPipeTagShift) == pipe_ring_tag(n)) {
             pn = n;
-            pca = (uint32_t)a;
+            pca = a;
             fetch(pca, ha, va);
             np = 1;
-            if ((b >> 32) == (uint64_t)n + M + 1) {
-                pcb = (uint32_t)b;
+            if ((b >> kPipeTagShift) == pipe_ring_tag(n + M)) {
+                pcb = b;
                 fetch(pcb, hb, vb);
                 np = 2;
             }

@@ -309,7 +309,8 @@ def _everything(op):
                          [(4096, 4, "transfers", 1, 6), (4096, 4, "transfers", 4, 8),
                           (1 << 20, 8, "transfers", 1, 8), (1 << 20, 8, "transfers", 4, 8),
                           (1 << 20, 8, "all", 2, 12), (1 << 16, 8, "all", 3, 5),
-                          (1 << 20, 8, "transfers", 4, 16)])
+                          (1 << 20, 8, "transfers", 4, 16), (1 << 20, 8, "all", 8, 16),
+                          (1 << 16, 8, "transfers", 6, 8)])
 def test_pipelined_lane_replays(n_items, depth, route, parts, wg):
     """The pipelined serialized lane: clients stage their requests' chains, committer waves
     apply each item's request-wide delta with one ordered atomic and install the versions
@@ -330,16 +331,16 @@ def test_pipelined_lane_replays(n_items, depth, route, parts, wg):
 def test_pipelined_lane_at_bench_shape():
     """The benchmark's configuration on a 2^20-transaction stream of its workload: 1M Zipf
     0.99 accounts, 4 reads / 2 writes, 4-read audits, depth 8, every transaction on the
-    pipelined lane, 4 commit server workgroups of 16 waves, 160 client waves of which 128 feed
-    the lane (two transactions a lane, requests of up to 128): the log replays exactly and
+    pipelined lane, 8 commit server workgroups of 16 waves, 200 client waves of which 192 feed
+    the lane (three transactions a lane, requests of up to 192): the log replays exactly and
     nothing aborts."""
     bank = csmv.Bank(n_txns=1 << 20, seed=1, n_reads=4, n_reads_ro=4, pct_ro=10, zipf=True)
-    st = run_bank(1 << 20, bank, variant="stamp_pipe", depth=8, waves=160, serial_waves=128,
-                  wg_waves=16, client_wg_waves=1, commit_parts=4, route=_everything)
+    st = run_bank(1 << 20, bank, variant="stamp_pipe", depth=8, waves=200, serial_waves=192,
+                  wg_waves=16, client_wg_waves=1, commit_parts=8, route=_everything)
     assert st["aborts"] == 0
 
 
-@pytest.mark.parametrize("n_txns,parts", [(1, 1), (777, 2), (4097, 4)])
+@pytest.mark.parametrize("n_txns,parts", [(1, 1), (777, 2), (4097, 4), (4097, 8)])
 def test_pipelined_lane_ragged(n_txns, parts):
     """Streams that do not fill the last wave, and a single serialized-lane wave."""
     bank = csmv.Bank(n_txns=n_txns, seed=73, n_reads=4, n_reads_ro=4, pct_ro=10, zipf=True)

@@ -218,7 +218,8 @@ def test_serve_serialized_lane_for_routed_bank_transactions():
 
 
 @pytest.mark.parametrize("n_acc,parts,n_txns,cpw", [(1 << 16, 1, 1 << 13, 1), (1 << 16, 2, 1 << 13, 2),
-                                                 (1 << 20, 4, 1 << 14, 1), (1 << 16, 3, 777, 1)])
+                                                 (1 << 20, 4, 1 << 14, 1), (1 << 16, 3, 777, 1),
+                                                 (1 << 20, 8, 1 << 14, 1)])
 def test_serve_pipelined_bank_lane(n_acc, parts, n_txns, cpw):
     """The serving mix's bank partition on the pipelined serialized lane (its accounts split
     over `parts` commit server workgroups; each request carries the lanes whose nodes fit
