@@ -49,7 +49,8 @@ enum {
     CSMV_WB_SERVER = 0x8,     /* the server installs versions (vanilla/csmv_sh/API.cu:269-272) */
     CSMV_SERIAL_PIPE = 0x10   /* pipelined serialized lane (bank driver, with CSMV_VARIANT_STAMP
                                  only; no counterpart in the reference): the serialized-lane
-                                 clients stage their requests' producer chains and split them
+                                 clients (three transactions a lane, requests of up to 192)
+                                 stage their requests' producer chains and split them
                                  by commit partition; committer waves apply each account's
                                  request-wide delta with one ordered 64-bit atomic, install the
                                  versions and reply with the values every read sees, and the
@@ -97,7 +98,7 @@ typedef struct {
                                   variant with client write-back; the cache driver has no
                                   serialized lane and ignores it. */
     uint32_t commit_parts;     /* pipelined serialized lane (CSMV_SERIAL_PIPE): commit server
-                                  workgroups, 1..4 (0 = 1).  Items are split among them by a
+                                  workgroups, 1..8 (0 = 1).  Items are split among them by a
                                   hash; each applies, installs and publishes its own items'
                                   writes for every request, in request order, and the clock is
                                   the least of their published progress. */
