@@ -496,8 +496,8 @@ def main():
     ap.add_argument("--variant", default="stamp_pipe",
                     help="the headline's commit path: stamp_pipe (the pipelined serialized "
                          "lane) or stamp (one committer wave)")
-    ap.add_argument("--waves", type=int, default=int(os.environ.get("CSMV_BENCH_WAVES", "200")))
-    ap.add_argument("--serial-waves", type=int, default=192)
+    ap.add_argument("--waves", type=int, default=int(os.environ.get("CSMV_BENCH_WAVES", "248")))
+    ap.add_argument("--serial-waves", type=int, default=247)
     ap.add_argument("--wg-waves", type=int, default=6,
                     help="waves per workgroup of the one-committer server: the committer, the "
                          "clock advancer and wg_waves - 2 receivers")
@@ -540,8 +540,11 @@ def main():
                     help="client waves of the serving loop's bank class on the pipelined lane")
     ap.add_argument("--serve-pipe-budget", type=int, default=196608,
                     help="admission cap of the serving runs with the pipelined bank lane")
-    ap.add_argument("--serial-waves-all", type=int, default=192,
+    ap.add_argument("--serial-waves-all", type=int, default=247,
                     help="serialized-lane waves of the run with every transaction routed")
+    ap.add_argument("--serial-waves-transfers", type=int, default=216,
+                    help="serialized-lane waves of the run with only the transfers routed (the "
+                         "other client waves run the audits on the normal lane)")
     ap.add_argument("--no-one", action="store_true",
                     help="skip the one-committer serialized-lane line")
     ap.add_argument("--escalate-after", type=int, default=2,
@@ -645,12 +648,12 @@ def main():
         # the same stream with only the transfers on the serialized lane: audits run on the
         # normal lane as multi-version snapshot reads and abort when a version they need has
         # left its account's 8-deep ring
-        et = make(args.variant, bank, "transfers", args.serial_waves)
+        et = make(args.variant, bank, "transfers", args.serial_waves_transfers)
         et.fill(INIT)
         rt = et.bank_run()
         et.close()
         route_transfers = {"txn_per_s": rt["txn_per_s"], "abort_rate": rt["abort_rate"],
-                           "kernel_ms": rt["kernel_ms"], "serial_lane_waves": args.serial_waves,
+                           "kernel_ms": rt["kernel_ms"], "serial_lane_waves": args.serial_waves_transfers,
                            "aborts_dataage": rt["aborts_dataage"], "commits_ro": rt["commits_ro"]}
         log(f"transfers on the serialized lane, audits on the normal lane: {rt['txn_per_s']:.4g} txn/s")
     one = {}

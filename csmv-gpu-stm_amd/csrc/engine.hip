@@ -202,6 +202,50 @@ __device__ __forceinline__ void delegated_tx(const Ctx &c, Tx<R, W> &tx, bool on
     }
 }
 
+// delegated_tx for S transactions a lane, their operand rows in one batch of loads (an idle
+// slot loads stream index 0's rows, in bounds)
+template <int R, int W, int S>
+__device__ __forceinline__ void delegated_txs(const Ctx &c, Tx<R, W> (&tx)[S], const bool (&on)[S],
+                                              const uint64_t (&k)[S], const uint32_t (&op)[S])
+{
+    uint32_t ops[S][R];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const uint32_t nr = on[s] ? (op[s] >> 16) & 0xFFu : 0u;
+        const uint64_t ks = on[s] ? k[s] : 0;
+#pragma unroll
+        for (int j = 0; j < R; ++j) ops[s][j] = c.addr[(uint64_t)((uint32_t)j < nr ? j : 0) * c.n_txns + ks];
+    }
+#pragma unroll
+    for (int s = 0; s < S; ++s) pin_round(ops[s]);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        Tx<R, W> &t = tx[s];
+        t.snap = on[s] ? 0u : kNone;
+        t.nr = 0;
+        t.nw = 0;
+        t.aborted = false;
+        t.overflow = false;
+#pragma unroll
+        for (int r = 0; r < R; ++r) t.rs[r] = kNone;
+#pragma unroll
+        for (int w = 0; w < W; ++w) { t.wa[w] = kNone; t.wv[w] = 0; }
+        const uint32_t amt = (op[s] >> 8) & 0xFFu, nr = (op[s] >> 16) & 0xFFu;
+        const bool audit = (op[s] & 0xFFu) == kKindAudit;
+        if (on[s]) {
+#pragma unroll
+            for (int j = 0; j < R; ++j)
+                if ((uint32_t)j < nr) {
+                    t.rs[j] = ops[s][j];
+                    if (j < 2 && !audit) tx_write(t, ops[s][j], 0);
+                }
+            t.nr = nr;
+            t.wv[0] = (int32_t)(uint32_t)k[s];
+            t.wv[1] = (int32_t)(audit ? kReadOnlyAmt : amt);
+        }
+    }
+}
+
 // client_kernel (bank/csmv_sh/exe.cu:89-284) on a resident stream: a lane holds one
 // transaction until it commits; lanes that finish are refilled with one fetch_add per
 // wave (ballot + prefix count).  Audits commit locally at their snapshot
@@ -505,17 +549,30 @@ __device__ __forceinline__ void bank_pipe_client(const Ctx &c, const ServerLayou
                 if (lane == 0) base = fetch_add(c.next_hot, (uint64_t)total);
                 base = bcast64(base);
                 if (base + total >= NH) exhausted = true;
+                // every slot's loads in one batch per level (stream index, then kind, then
+                // operands): three round trips a round whatever the slot count
                 uint64_t mine = base + (incl - want);
+                bool take[S];
+                uint64_t at[S];
 #pragma unroll
-                for (int s = 0; s < S; ++s)
-                    if (!has[s]) {
-                        if (mine < NH) {
-                            has[s] = true;
-                            k[s] = c.hot_list[mine];
-                            op[s] = c.op[k[s]];
-                        }
-                        ++mine;
-                    }
+                for (int s = 0; s < S; ++s) {
+                    take[s] = !has[s] && mine < NH;
+                    at[s] = take[s] ? mine : 0;
+                    mine += has[s] ? 0 : 1;
+                }
+                uint64_t kn[S];
+#pragma unroll
+                for (int s = 0; s < S; ++s) kn[s] = c.hot_list[at[s]];
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    k[s] = take[s] ? kn[s] : k[s];
+                    has[s] = has[s] || take[s];
+                }
+                uint32_t on_[S];
+#pragma unroll
+                for (int s = 0; s < S; ++s) on_[s] = c.op[k[s]];
+#pragma unroll
+                for (int s = 0; s < S; ++s) op[s] = take[s] ? on_[s] : op[s];
                 compact();
             }
         }
@@ -524,8 +581,7 @@ __device__ __forceinline__ void bank_pipe_client(const Ctx &c, const ServerLayou
         for (int s = 0; s < S; ++s) any = any || has[s];
         if (ballot(any) == 0) break;
         Tx<R, W> tx[S];
-#pragma unroll
-        for (int s = 0; s < S; ++s) delegated_tx<R, W>(c, tx[s], has[s], k[s], op[s]);
+        delegated_txs<R, W, S>(c, tx, has, k, op);
         bool fail = false;
         uint32_t ts[S];
         tx_commit_pipe<R, W, S>(c, L, scr, cw, tx, has, ts, &fail, &t_reply);

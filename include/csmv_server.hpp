@@ -1613,7 +1613,7 @@ __device__ __forceinline__ void receiver(const Ctx &c, const ServerLayout &L, ui
 // the staged lists (a block of kPipeMaxE per partition).
 // Transactions a lane of the pipelined client, and the write-hash size its request needs
 // (the writes fill at most half of it).
-constexpr int kPipeSlots = 3;
+constexpr int kPipeSlots = 4;
 __host__ __device__ constexpr uint32_t pow2_ceil_u32(uint32_t v)
 {
     uint32_t q = 1;
@@ -1626,7 +1626,7 @@ __host__ __device__ constexpr uint32_t pipe_client_hsz(uint32_t W, uint32_t S)
 }
 __host__ __device__ constexpr uint32_t pipe_client_words(uint32_t R, uint32_t W, uint32_t S)
 {
-    return (4 + 2 * S) * pipe_client_hsz(W, S) + 6 * S * kWave * R + S * kWave * W +
+    return (4 + 2 * S) * pipe_client_hsz(W, S) + 8 * S * kWave * R + 3 * S * kWave * W + S * kWave +
            3 * kPipeParts * kPipeMaxE;
 }
 
@@ -1655,47 +1655,64 @@ __device__ __forceinline__ void tx_commit_pipe(const Ctx &c, const ServerLayout 
                   "the pipelined lane carries bank transfers");
     constexpr int RO = R;
     constexpr uint32_t kNoP = 0xFFFFu;            // no producer
-    constexpr uint32_t NN = S * kWave * RO;       // nodes
+    constexpr uint32_t NV = S * kWave, NN = NV * RO;   // virtual lanes, nodes
     constexpr uint32_t hsz = pipe_client_hsz(W, S), hmask = hsz - 1;
     const uint32_t lane = lane_id();
     const uint64_t lt = (1ull << lane) - 1;
     const uint32_t P = L.parts;
+    // the write hash
     uint32_t *hkey = scr, *hw = scr + hsz;        // hw[i * hsz + slot]: virtual lanes 32 i ..
     uint32_t *canon = scr + (1 + 2 * S) * hsz;
     int32_t *htot = (int32_t *)(scr + (2 + 2 * S) * hsz);
     uint32_t *hrank = scr + (3 + 2 * S) * hsz;
-    uint32_t *snext = scr + (4 + 2 * S) * hsz;    // [2][S * 64 * R] by node v * R + r
-    int32_t *sw = (int32_t *)(snext + S * kWave * RO);   // (interleaved with snext's buffers)
-    uint32_t *e2 = snext + 4 * S * kWave * RO;
-    uint32_t *sfs = e2 + S * kWave * RO;          // each node's write-hash slot (kNone: unwritten)
-    uint32_t *ssup = sfs + S * kWave * RO;        // [2 * 64 * W] by v * W + w
-    uint32_t *sitem = ssup + S * kWave * W;       // [kPipeParts][kPipeMaxE]
+    // per node (v * R + r): chain buffers A and B (next node, weight), the producer code
+    // (then the node's entry), hash slot, item and partition
+    uint32_t *snext = scr + (4 + 2 * S) * hsz;
+    int32_t *sw = (int32_t *)(snext + NN);
+    uint32_t *e2 = snext + 4 * NN;
+    uint32_t *sfs = e2 + NN;
+    uint32_t *sops = sfs + NN;
+    uint32_t *spq = sops + NN;
+    // per write slot (v * W + w): item, hash slot, superseded flag; per virtual lane: amount
+    uint32_t *swa = spq + NN;
+    uint32_t *sslot = swa + NV * W;
+    uint32_t *ssup = sslot + NV * W;
+    uint32_t *samt = ssup + NV * W;
+    uint32_t *sitem = samt + NV;                  // [kPipeParts][kPipeMaxE]
     uint32_t *swgt = sitem + kPipeParts * kPipeMaxE;
     uint32_t *smeta = swgt + kPipeParts * kPipeMaxE;
+    auto nd = [&](int s, int r) -> uint32_t { return ((uint32_t)s * kWave + lane) * RO + (uint32_t)r; };
+    auto vl = [&](int s) -> uint32_t { return (uint32_t)s * kWave + lane; };
+    // a virtual lane's deltas on its two write slots
+    auto delta = [&](uint32_t v, uint32_t w) -> int32_t {
+        const int32_t a = (int32_t)samt[v];
+        return w == 0 ? (swa[v * W + 1] == kNone ? 0 : -a) : a;
+    };
 
-    uint32_t ops[S][RO], wa[S][W], kx[S], amt[S];
-    bool ro[S], on[S];
-#pragma unroll
-    for (int s = 0; s < S; ++s) {
-#pragma unroll
-        for (int r = 0; r < RO; ++r) ops[s][r] = (req[s] && (uint32_t)r < tx[s].nr) ? tx[s].rs[r] : kNone;
-#pragma unroll
-        for (int w = 0; w < W; ++w) wa[s][w] = (req[s] && (uint32_t)w < tx[s].nw) ? tx[s].wa[w] : kNone;
-        kx[s] = (uint32_t)tx[s].wv[0];
-        amt[s] = (uint32_t)tx[s].wv[1];
-        ro[s] = amt[s] == kReadOnlyAmt;
-    }
-
-    // 1. the virtual lanes this request carries: an order prefix whose nodes keep every
-    //    partition's list within kPipeMaxE (every lane takes part in the scans)
+    // 1. the operands into LDS; the virtual lanes this request carries: an order prefix
+    //    whose nodes keep every partition's list within kPipeMaxE (every lane takes part in
+    //    the scans).  Each step below keeps one slot's values in registers at a time.
+    bool on[S], ro[S];
+    uint32_t kx[S];
     {
         uint32_t pq[S][RO];
 #pragma unroll
-        for (int s = 0; s < S; ++s)
+        for (int s = 0; s < S; ++s) {
+            on[s] = req[s];
+            ro[s] = (uint32_t)tx[s].wv[1] == kReadOnlyAmt;
+            kx[s] = (uint32_t)tx[s].wv[0];
+            samt[vl(s)] = (uint32_t)tx[s].wv[1];
 #pragma unroll
-            for (int r = 0; r < RO; ++r) pq[s][r] = ops[s][r] != kNone ? pipe_part(ops[s][r], P) : kNone;
+            for (int r = 0; r < RO; ++r) {
+                const uint32_t x = (req[s] && (uint32_t)r < tx[s].nr) ? tx[s].rs[r] : kNone;
+                pq[s][r] = x != kNone ? pipe_part(x, P) : kNone;
+                sops[nd(s, r)] = x;
+                spq[nd(s, r)] = pq[s][r];
+            }
 #pragma unroll
-        for (int s = 0; s < S; ++s) on[s] = req[s];
+            for (int w = 0; w < W; ++w)
+                swa[vl(s) * W + w] = (req[s] && (uint32_t)w < tx[s].nw) ? tx[s].wa[w] : kNone;
+        }
 #pragma unroll
         for (uint32_t t = 0; t < kPipeParts; ++t) {
             if (t >= P) break;
@@ -1712,12 +1729,16 @@ __device__ __forceinline__ void tx_commit_pipe(const Ctx &c, const ServerLayout 
         }
     }
 #pragma unroll
-    for (int s = 0; s < S; ++s) {
+    for (int s = 0; s < S; ++s)
+        if (!on[s]) {
 #pragma unroll
-        for (int r = 0; r < RO; ++r) ops[s][r] = on[s] ? ops[s][r] : kNone;
+            for (int r = 0; r < RO; ++r) {
+                sops[nd(s, r)] = kNone;
+                spq[nd(s, r)] = kNone;
+            }
 #pragma unroll
-        for (int w = 0; w < W; ++w) wa[s][w] = on[s] ? wa[s][w] : kNone;
-    }
+            for (int w = 0; w < W; ++w) swa[vl(s) * W + w] = kNone;
+        }
     uint32_t rank[S], nc = 0;
 #pragma unroll
     for (int s = 0; s < S; ++s) {
@@ -1725,64 +1746,63 @@ __device__ __forceinline__ void tx_commit_pipe(const Ctx &c, const ServerLayout 
         rank[s] = nc + (uint32_t)__builtin_popcountll(u & lt);
         nc += (uint32_t)__builtin_popcountll(u);
     }
-    int32_t d0[S], d1[S];
-#pragma unroll
-    for (int s = 0; s < S; ++s) {
-        d0[s] = (wa[s][1] == kNone) ? 0 : -(int32_t)amt[s];
-        d1[s] = (int32_t)amt[s];
-    }
+    wave_fence();
 
     // 2. the write sets into the hash (item -> writer virtual lanes)
-    uint32_t slot_w[S][W], hp[S][W];
-    bool pend[S][W];
-#pragma unroll
-    for (int s = 0; s < S; ++s)
-#pragma unroll
-        for (int w = 0; w < W; ++w) {
-            slot_w[s][w] = kNone;
-            hp[s][w] = mix32(wa[s][w]) & hmask;
-            pend[s][w] = wa[s][w] != kNone;
-        }
-    for (uint32_t step = 0; step <= hmask; ++step) {
-        bool more = false;
+    {
+        uint32_t a[S][W], slot_w[S][W], hp[S][W];
+        bool pend[S][W];
 #pragma unroll
         for (int s = 0; s < S; ++s)
 #pragma unroll
             for (int w = 0; w < W; ++w) {
-                uint32_t old = kNone;
-                __hip_atomic_compare_exchange_strong(&hkey[pend[s][w] ? hp[s][w] : 0u], &old,
-                                                     pend[s][w] ? wa[s][w] : kNone, __ATOMIC_RELAXED,
-                                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                const bool got = pend[s][w] && (old == kNone || old == wa[s][w]);
-                slot_w[s][w] = got ? hp[s][w] : slot_w[s][w];
-                hp[s][w] = (pend[s][w] && !got) ? ((hp[s][w] + 1) & hmask) : hp[s][w];
-                pend[s][w] = pend[s][w] && !got;
-                more |= pend[s][w];
+                a[s][w] = swa[vl(s) * W + w];
+                slot_w[s][w] = kNone;
+                hp[s][w] = mix32(a[s][w]) & hmask;
+                pend[s][w] = a[s][w] != kNone;
             }
-        if (ballot(more) == 0) break;
+        for (uint32_t step = 0; step <= hmask; ++step) {
+            bool more = false;
+#pragma unroll
+            for (int s = 0; s < S; ++s)
+#pragma unroll
+                for (int w = 0; w < W; ++w) {
+                    uint32_t old = kNone;
+                    __hip_atomic_compare_exchange_strong(&hkey[pend[s][w] ? hp[s][w] : 0u], &old,
+                                                         pend[s][w] ? a[s][w] : kNone, __ATOMIC_RELAXED,
+                                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    const bool got = pend[s][w] && (old == kNone || old == a[s][w]);
+                    slot_w[s][w] = got ? hp[s][w] : slot_w[s][w];
+                    hp[s][w] = (pend[s][w] && !got) ? ((hp[s][w] + 1) & hmask) : hp[s][w];
+                    pend[s][w] = pend[s][w] && !got;
+                    more |= pend[s][w];
+                }
+            if (ballot(more) == 0) break;
+        }
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                sslot[vl(s) * W + w] = slot_w[s][w];
+                if (slot_w[s][w] != kNone)
+                    atomicOr(&hw[(vl(s) >> 5) * hsz + slot_w[s][w]], 1u << (vl(s) & 31));
+            }
     }
-#pragma unroll
-    for (int s = 0; s < S; ++s)
-#pragma unroll
-        for (int w = 0; w < W; ++w)
-            if (slot_w[s][w] != kNone) {
-                const uint32_t v = (uint32_t)s * kWave + lane;
-                atomicOr(&hw[(v >> 5) * hsz + slot_w[s][w]], 1u << (v & 31));
-            }
     wave_fence();
 
-    // 3. each read's slot, producer (the highest lower virtual lane writing the item, and
-    //    its write slot) and each written item's canonical node
-    uint32_t fs[S][RO], pj[S][RO];
+    // 3. per slot: each read's hash slot and producer (the highest lower virtual lane
+    //    writing the item, and its write slot: v | w << 8), and each written item's
+    //    canonical node (the lowest node of the item without a producer)
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-        uint32_t hs[RO];
+        uint32_t x[RO], fs[RO], hs[RO];
         bool fp[RO];
 #pragma unroll
         for (int r = 0; r < RO; ++r) {
-            fs[s][r] = kNone;
-            hs[r] = mix32(ops[s][r]) & hmask;
-            fp[r] = ops[s][r] != kNone;
+            x[r] = sops[nd(s, r)];
+            fs[r] = kNone;
+            hs[r] = mix32(x[r]) & hmask;
+            fp[r] = x[r] != kNone;
         }
         for (uint32_t step = 0; step <= hmask; ++step) {
             bool more = false;
@@ -1790,92 +1810,68 @@ __device__ __forceinline__ void tx_commit_pipe(const Ctx &c, const ServerLayout 
             for (int r = 0; r < RO; ++r) {
                 const uint32_t kk = __hip_atomic_load(&hkey[fp[r] ? hs[r] : 0u], __ATOMIC_RELAXED,
                                                       __HIP_MEMORY_SCOPE_WORKGROUP);
-                const bool hit = fp[r] && kk == ops[s][r];
-                const bool next = fp[r] && kk != ops[s][r] && kk != kNone;
-                fs[s][r] = hit ? hs[r] : fs[s][r];
+                const bool hit = fp[r] && kk == x[r];
+                const bool next = fp[r] && kk != x[r] && kk != kNone;
+                fs[r] = hit ? hs[r] : fs[r];
                 hs[r] = next ? ((hs[r] + 1) & hmask) : hs[r];
                 fp[r] = next;
                 more |= next;
             }
             if (ballot(more) == 0) break;
         }
-    }
-#pragma unroll
-    for (int s = 0; s < S; ++s) {
 #pragma unroll
         for (int r = 0; r < RO; ++r) {
-            const uint32_t at = fs[s][r] != kNone ? fs[s][r] : 0u;
+            const uint32_t at = fs[r] != kNone ? fs[r] : 0u;
             uint32_t pv = kNoP;
             // the virtual lanes below v: mask words 0 .. 2 s + 1, the last two as the lanes
-            // below the lane (words 0 and 1 whole for slot 1)
+            // below the lane (the earlier words whole)
 #pragma unroll
             for (int i = 2 * s + 1; i >= 0; --i) {
                 const uint32_t lo = (uint32_t)i * 32u;
                 const uint32_t below = i < 2 * s ? 0xFFFFFFFFu : (i & 1) ? (uint32_t)(lt >> 32) : (uint32_t)lt;
                 const uint32_t raw = __hip_atomic_load(&hw[(uint32_t)i * hsz + at], __ATOMIC_RELAXED,
                                                        __HIP_MEMORY_SCOPE_WORKGROUP);
-                const uint32_t m = fs[s][r] != kNone ? (raw & below) : 0u;
+                const uint32_t m = fs[r] != kNone ? (raw & below) : 0u;
                 pv = (pv == kNoP && m) ? lo + 31u - (uint32_t)__builtin_clz(m) : pv;
             }
-            const uint32_t srcl = pv == kNoP ? lane : (pv & 63u);
             uint32_t code = kNoP;
+            if (pv != kNoP) {
 #pragma unroll
-            for (int w = 0; w < W; ++w) {
-                uint32_t pw = 0;
-#pragma unroll
-                for (int t = 0; t < S; ++t) {
-                    const uint32_t g = lane_gather(wa[t][w], srcl);
-                    pw = (pv >> 6) == (uint32_t)t ? g : pw;
-                }
-                code = (pv != kNoP && pw == ops[s][r]) ? (pv | ((uint32_t)w << 8)) : code;
+                for (int w = 0; w < W; ++w) code = swa[pv * W + w] == x[r] ? (pv | ((uint32_t)w << 8)) : code;
             }
-            pj[s][r] = code;
+            e2[nd(s, r)] = code;
+            sfs[nd(s, r)] = fs[r];
+            if (fs[r] != kNone && code == kNoP) atomicMin(&canon[fs[r]], nd(s, r));
         }
-        // one slot's mask loads at a time (fewer registers in flight)
+        // one slot's loads at a time (fewer registers in flight)
         asm volatile("" ::: "memory");
     }
 #pragma unroll
     for (int s = 0; s < S; ++s)
 #pragma unroll
-        for (int r = 0; r < RO; ++r)
-            if (fs[s][r] != kNone && pj[s][r] == kNoP)
-                atomicMin(&canon[fs[s][r]], ((uint32_t)s * kWave + lane) * RO + (uint32_t)r);
-#pragma unroll
-    for (int s = 0; s < S; ++s)
-#pragma unroll
-        for (int r = 0; r < RO; ++r) sfs[((uint32_t)s * kWave + lane) * RO + (uint32_t)r] = fs[s][r];
+        for (int w = 0; w < W; ++w) ssup[vl(s) * W + w] = 0;
+    wave_fence();
 
     // 4. superseded write slots (a write slot is superseded when a higher virtual lane reads
     //    the item from it and writes it too), then the producer chains by pointer jumping
     //    over two buffers of (next node, weight): each node's chain root and the sum of the
     //    writers' deltas between them
-#pragma unroll
-    for (int s = 0; s < S; ++s)
-#pragma unroll
-        for (int w = 0; w < W; ++w) ssup[((uint32_t)s * kWave + lane) * W + w] = 0;
-    wave_fence();
     uint32_t *nA = snext, *nB = snext + 2 * NN;
     int32_t *wA = sw, *wB = sw + 2 * NN;
 #pragma unroll
     for (int s = 0; s < S; ++s)
 #pragma unroll
         for (int r = 0; r < RO; ++r) {
-            const uint32_t node = ((uint32_t)s * kWave + lane) * RO + (uint32_t)r;
-            const bool has_p = pj[s][r] != kNoP;
-            const uint32_t pv = pj[s][r] & 0xFFu, pw = pj[s][r] >> 8;
-            const uint32_t srcl = has_p ? (pv & 63u) : lane;
-            int32_t dd = 0;
-#pragma unroll
-            for (int t = 0; t < S; ++t) {
-                const int32_t g0 = (int32_t)lane_gather((uint32_t)d0[t], srcl);
-                const int32_t g1 = (int32_t)lane_gather((uint32_t)d1[t], srcl);
-                dd = (pv >> 6) == (uint32_t)t ? (pw == 0 ? g0 : g1) : dd;
-            }
+            const uint32_t node = nd(s, r);
+            const uint32_t code = e2[node];
+            const bool has_p = code != kNoP;
+            const uint32_t pv = has_p ? (code & 0xFFu) : 0u, pw = code >> 8;
             nA[node] = has_p ? pv * RO + pw : node;
-            wA[node] = has_p ? dd : 0;
+            wA[node] = has_p ? delta(pv, pw & 1u) : 0;
+            const uint32_t x = sops[node];
             bool mine_w = false;
 #pragma unroll
-            for (int w = 0; w < W; ++w) mine_w |= wa[s][w] == ops[s][r] && ops[s][r] != kNone;
+            for (int w = 0; w < W; ++w) mine_w |= swa[vl(s) * W + w] == x && x != kNone;
             if (has_p && mine_w) ssup[pv * W + pw] = 1u;
         }
     wave_fence();
@@ -1885,7 +1881,7 @@ __device__ __forceinline__ void tx_commit_pipe(const Ctx &c, const ServerLayout 
         for (int s = 0; s < S; ++s)
 #pragma unroll
             for (int r = 0; r < RO; ++r) {
-                const uint32_t node = ((uint32_t)s * kWave + lane) * RO + (uint32_t)r;
+                const uint32_t node = nd(s, r);
                 const uint32_t t = nA[node];
                 const uint32_t tt = nA[t];
                 wB[node] = wA[node] + (t != node ? wA[t] : 0);
@@ -1901,39 +1897,35 @@ __device__ __forceinline__ void tx_commit_pipe(const Ctx &c, const ServerLayout 
         wB = tw;
         if (ballot(more) == 0) break;
     }
-    uint32_t supf[S];
-#pragma unroll
-    for (int s = 0; s < S; ++s) {
-        supf[s] = 0;
-#pragma unroll
-        for (int w = 0; w < W; ++w) supf[s] |= ssup[((uint32_t)s * kWave + lane) * W + w] ? (1u << w) : 0u;
-    }
 
-    // 5. each written item's request-wide delta and last writer's rank
+    // 5. each written item's request-wide delta and last writer's rank, from its last
+    //    (unsuperseded) write: node w reads write slot w's item, so that write's value is the
+    //    value before the request + the node's chain sum + the slot's delta
 #pragma unroll
     for (int s = 0; s < S; ++s)
 #pragma unroll
-        for (int w = 0; w < W; ++w)
-            if (wa[s][w] != kNone && ((supf[s] >> w) & 1u) == 0) {
-                htot[slot_w[s][w]] = wA[((uint32_t)s * kWave + lane) * RO + (uint32_t)w] + (w == 0 ? d0[s] : d1[s]);
-                hrank[slot_w[s][w]] = rank[s];
+        for (int w = 0; w < W; ++w) {
+            const uint32_t sl = sslot[vl(s) * W + w];
+            if (sl != kNone && ssup[vl(s) * W + w] == 0) {
+                htot[sl] = wA[nd(s, w)] + delta(vl(s), (uint32_t)w);
+                hrank[sl] = rank[s];
             }
+        }
     wave_fence();
 
     // 6. the entries: node (v, r) of an item of partition q becomes entry e of q's list,
-    //    staged at q * kPipeMaxE + e; the node's place (q << 16 | e, kept in the next-node
-    //    array, which the chains no longer need) for reading its value back.  A node of an
-    //    item no one writes is its own root.
+    //    staged at q * kPipeMaxE + e; the node's place (q << 16 | e, kept in chain buffer B,
+    //    which the chains no longer need) for reading its value back.  A node of an item no
+    //    one writes is its own root.
     uint32_t cnt[kPipeParts];
 #pragma unroll
     for (uint32_t t = 0; t < kPipeParts; ++t) cnt[t] = 0;
-    uint32_t pe[S][RO];
     uint32_t *const splace = nB;
 #pragma unroll
     for (int s = 0; s < S; ++s)
 #pragma unroll
         for (int r = 0; r < RO; ++r) {
-            const uint32_t q = ops[s][r] != kNone ? pipe_part(ops[s][r], P) : kNone;
+            const uint32_t q = spq[nd(s, r)];
             uint32_t e = 0;
 #pragma unroll
             for (uint32_t t = 0; t < kPipeParts; ++t) {
@@ -1942,16 +1934,16 @@ __device__ __forceinline__ void tx_commit_pipe(const Ctx &c, const ServerLayout 
                 if (q == t) e = cnt[t] + (uint32_t)__builtin_popcountll(mq & lt);
                 cnt[t] += (uint32_t)__builtin_popcountll(mq);
             }
-            pe[s][r] = q != kNone ? (q << 16) | e : kNone;
-            e2[((uint32_t)s * kWave + lane) * RO + (uint32_t)r] = e;
-            splace[((uint32_t)s * kWave + lane) * RO + (uint32_t)r] = pe[s][r];
+            e2[nd(s, r)] = e;
+            splace[nd(s, r)] = q != kNone ? (q << 16) | e : kNone;
         }
     wave_fence();
 #pragma unroll
     for (int s = 0; s < S; ++s)
 #pragma unroll
         for (int r = 0; r < RO; ++r) {
-            const uint32_t node = ((uint32_t)s * kWave + lane) * RO + (uint32_t)r;
+            const uint32_t node = nd(s, r);
+            const uint32_t pl = splace[node];
             const uint32_t f = sfs[node];
             const bool written = f != kNone;
             const uint32_t at = written ? f : 0u;
@@ -1962,9 +1954,9 @@ __device__ __forceinline__ void tx_commit_pipe(const Ctx &c, const ServerLayout 
             const uint32_t meta = e2[rt < NN ? rt : 0u] | (is_root ? kPipeRootBit : 0u) |
                                   (is_canon ? kPipeInstBit | (hrank[at] << kPipeRankShift) : 0u);
             const int32_t wout = is_canon ? htot[at] : wA[node];
-            if (pe[s][r] != kNone) {
-                const uint32_t o = (pe[s][r] >> 16) * kPipeMaxE + (pe[s][r] & 0xFFFFu);
-                sitem[o] = ops[s][r];
+            if (pl != kNone) {
+                const uint32_t o = (pl >> 16) * kPipeMaxE + (pl & 0xFFFFu);
+                sitem[o] = sops[node];
                 swgt[o] = (uint32_t)wout;
                 smeta[o] = meta;
             }
@@ -2000,14 +1992,16 @@ __device__ __forceinline__ void tx_commit_pipe(const Ctx &c, const ServerLayout 
 #pragma unroll
     for (int s = 0; s < S; ++s)
 #pragma unroll
-        for (int w = 0; w < W; ++w)
-            if (slot_w[s][w] != kNone) {
-                hkey[slot_w[s][w]] = kNone;
+        for (int w = 0; w < W; ++w) {
+            const uint32_t sl = sslot[vl(s) * W + w];
+            if (sl != kNone) {
+                hkey[sl] = kNone;
 #pragma unroll
-                for (uint32_t i = 0; i < 2 * S; ++i) hw[i * hsz + slot_w[s][w]] = 0;
-                canon[slot_w[s][w]] = kNone;
-                htot[slot_w[s][w]] = 0;
+                for (uint32_t i = 0; i < 2 * S; ++i) hw[i * hsz + sl] = 0;
+                canon[sl] = kNone;
+                htot[sl] = 0;
             }
+        }
     wave_fence();
     bool any = false;
 #pragma unroll
@@ -2046,7 +2040,7 @@ __device__ __forceinline__ void tx_commit_pipe(const Ctx &c, const ServerLayout 
 #pragma unroll
         for (int r = 0; r < RO; ++r) {
             in[s][r] = 0;
-            const uint32_t pl = splace[((uint32_t)s * kWave + lane) * RO + (uint32_t)r];
+            const uint32_t pl = splace[nd(s, r)];
             if (pl != kNone) in[s][r] = (int32_t)ld_rlx(&rv[(pl >> 16) * kPipeMaxE + (pl & 0xFFFFu)]);
         }
     *wait_ticks += wall_clock64() - tw0;
@@ -2054,8 +2048,13 @@ __device__ __forceinline__ void tx_commit_pipe(const Ctx &c, const ServerLayout 
     // 9. the transactions on the values before them, in commit order, and their log
 #pragma unroll
     for (int s = 0; s < S; ++s) {
+        uint32_t ops[RO], wa[W];
+#pragma unroll
+        for (int r = 0; r < RO; ++r) ops[r] = sops[nd(s, r)];
+#pragma unroll
+        for (int w = 0; w < W; ++w) wa[w] = swa[vl(s) * W + w];
         int32_t rd[RO], nv[W];
-        transfer_exec<RO, W>(ops[s], amt[s], in[s], rd, wa[s], nv, !ro[s]);
+        transfer_exec<RO, W>(ops, samt[vl(s)], in[s], rd, wa, nv, !ro[s]);
         const uint32_t below = (uint32_t)h0 + rank[s];
         const uint32_t myts = ro[s] ? (kSnapshotBit | below) : below + 1;
         if (on[s]) {
@@ -2066,7 +2065,7 @@ __device__ __forceinline__ void tx_commit_pipe(const Ctx &c, const ServerLayout 
             c.log_v[kx[s]] = ro[s] ? (int32_t)sum : rd[0];
 #pragma unroll
             for (int r = 1; r < RO; ++r)
-                if (!ro[s] && ops[s][r] != kNone) c.log_v[(uint64_t)r * c.n_txns + kx[s]] = rd[r];
+                if (!ro[s] && ops[r] != kNone) c.log_v[(uint64_t)r * c.n_txns + kx[s]] = rd[r];
             ts_out[s] = myts;
         }
     }

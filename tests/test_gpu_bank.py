@@ -331,11 +331,11 @@ def test_pipelined_lane_replays(n_items, depth, route, parts, wg):
 def test_pipelined_lane_at_bench_shape():
     """The benchmark's configuration on a 2^20-transaction stream of its workload: 1M Zipf
     0.99 accounts, 4 reads / 2 writes, 4-read audits, depth 8, every transaction on the
-    pipelined lane, 8 commit server workgroups of 16 waves, 200 client waves of which 192 feed
-    the lane (three transactions a lane, requests of up to 192): the log replays exactly and
+    pipelined lane, 8 commit server workgroups of 16 waves, 248 client waves of which 247 feed
+    the lane (four transactions a lane, requests of up to 256): the log replays exactly and
     nothing aborts."""
     bank = csmv.Bank(n_txns=1 << 20, seed=1, n_reads=4, n_reads_ro=4, pct_ro=10, zipf=True)
-    st = run_bank(1 << 20, bank, variant="stamp_pipe", depth=8, waves=200, serial_waves=192,
+    st = run_bank(1 << 20, bank, variant="stamp_pipe", depth=8, waves=248, serial_waves=247,
                   wg_waves=16, client_wg_waves=1, commit_parts=8, route=_everything)
     assert st["aborts"] == 0
 
