@@ -49,7 +49,7 @@ enum {
     CSMV_WB_SERVER = 0x8,     /* the server installs versions (vanilla/csmv_sh/API.cu:269-272) */
     CSMV_SERIAL_PIPE = 0x10   /* pipelined serialized lane (bank driver, with CSMV_VARIANT_STAMP
                                  only; no counterpart in the reference): the serialized-lane
-                                 clients (three transactions a lane, requests of up to 192)
+                                 clients (four transactions a lane, requests of up to 256)
                                  stage their requests' producer chains and split them
                                  by commit partition; committer waves apply each account's
                                  request-wide delta with one ordered 64-bit atomic, install the
