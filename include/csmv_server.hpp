@@ -2266,7 +2266,13 @@ __device__ __forceinline__ void committer_pipe(const Ctx &c, const ServerLayout 
         // an installing entry writes its item's last value (the word before + the
         // request-wide delta) with its last writer's timestamp to the ring slot after the
         // item's installs so far (j-th install: slot j mod depth; the tail moves once the
-        // ring is full)
+        // ring is full).  Installs j and j + depth of an item share a slot; they come from
+        // requests at least depth apart, made by different committer waves.  With depth >=
+        // kPipeRing, install j is complete before install j + depth is stored (the wait
+        // above), so a store does; with a shallower ring they may land in either order, and
+        // the slot takes the larger (timestamp << 32 | value) word, so a late older version
+        // never replaces a newer one.
+        const bool ordered = D >= kPipeRing;
 #pragma unroll
         for (int g = 0; g < G; ++g) {
             const uint32_t e = g * kWave + lane;
@@ -2275,9 +2281,12 @@ __device__ __forceinline__ void committer_pipe(const Ctx &c, const ServerLayout 
             const uint32_t pos = mod_d(j);
             const uint32_t tail = j < D ? 0u : mod_d(j + 1);
             const uint32_t ts = base + ((meta[g] >> kPipeRankShift) & 255u) + 1;
-            if (inst)
-                st_rlx(c.slot + (size_t)item[g] * D + pos,
-                       ((uint64_t)ts << 32) | (uint32_t)(val[g] + wgt[g]));
+            if (inst) {
+                uint64_t *const sl = c.slot + (size_t)item[g] * D + pos;
+                const uint64_t word = ((uint64_t)ts << 32) | (uint32_t)(val[g] + wgt[g]);
+                if (ordered) st_rlx(sl, word);
+                else __hip_atomic_fetch_max(sl, word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
             if (act[g]) {
                 irec[qi * 2 * kPipeMaxE + e] = inst ? item[g] : kNone;
                 irec[(qi * 2 + 1) * kPipeMaxE + e] = pos | (tail << 16);

@@ -310,21 +310,25 @@ def _everything(op):
                           (1 << 20, 8, "transfers", 1, 8), (1 << 20, 8, "transfers", 4, 8),
                           (1 << 20, 8, "all", 2, 12), (1 << 16, 8, "all", 3, 5),
                           (1 << 20, 8, "transfers", 4, 16), (1 << 20, 8, "all", 8, 16),
-                          (1 << 16, 8, "transfers", 6, 8)])
+                          (1 << 16, 8, "transfers", 6, 8), (4096, 1, "all", 8, 16),
+                          (1 << 20, 64, "transfers", 7, 16), (1 << 16, 2, "all", 5, 12)])
 def test_pipelined_lane_replays(n_items, depth, route, parts, wg):
     """The pipelined serialized lane: clients stage their requests' chains, committer waves
     apply each item's request-wide delta with one ordered atomic and install the versions
     themselves, split over commit partitions by item hash.  Every transfer (or every
     transaction) on it, at the bank-plain shape and the bank-at-scale shape (1M Zipf 0.99
-    accounts, 4 reads / 2 writes, depth 8): the log replays exactly and the audits on the
-    normal lane read consistent snapshots."""
+    accounts, 4 reads / 2 writes, depth 8), and at depth 1 (every install of an account
+    writes the same slot, so installs landing out of request order would leave an older
+    version there) and depth 64: the log replays exactly and the audits on the normal lane
+    read consistent snapshots."""
     zipf = n_items > 4096
     bank = csmv.Bank(n_txns=1 << 17, seed=71, n_reads=4 if zipf else 2,
                      n_reads_ro=4 if zipf else 32, pct_ro=10, zipf=zipf)
     st = run_bank(n_items, bank, variant="stamp_pipe", depth=depth, waves=24, serial_waves=16,
                   wg_waves=wg, client_wg_waves=1, commit_parts=parts,
                   route=_transfers if route == "transfers" else _everything)
-    if route == "all":
+    if route == "all" and bank.n_reads_ro <= 4:
+        # (audits of more reads than the lane's wire carries stay on the normal lane)
         assert st["aborts"] == 0
 
 
