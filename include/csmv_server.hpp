@@ -126,10 +126,10 @@ __host__ __device__ constexpr uint64_t pipe_ring_chunks(uint32_t cnt, uint32_t q
     return (uint64_t)((cnt + kPipeChunk - 1) / kPipeChunk) << (kPipeChunkShift + 4 * q);
 }
 // Install records of this many requests wait for the publisher: per request the item and
-// head word of each entry, the entry count, the timestamp after the request and the ready
-// word.
+// head word of each entry, the entry count, the timestamp after the request, the ready word
+// and the word that says its installs are issued.
 constexpr uint32_t kPipeRing = 16;
-__host__ __device__ constexpr uint32_t pipe_words() { return kPipeRing * (2 * kPipeMaxE + 3); }
+__host__ __device__ constexpr uint32_t pipe_words() { return kPipeRing * (2 * kPipeMaxE + 4); }
 // Receiver and committer waves of a pipelined server workgroup of wg waves (one more
 // publishes): a receiver takes up to two requests per memory round trip, so a quarter of
 // the waves receive (16 waves: 4 receivers, 11 committers, the publisher).
@@ -2137,6 +2137,7 @@ __device__ __forceinline__ void committer_pipe(const Ctx &c, const ServerLayout 
     uint32_t *icnt = irec + kPipeRing * 2 * kPipeMaxE;      // [kPipeRing] entries
     uint32_t *ilast = icnt + kPipeRing;                     // [kPipeRing]
     uint32_t *iready = ilast + kPipeRing;                   // [kPipeRing]
+    uint32_t *iissued = iready + kPipeRing;                 // [kPipeRing] 1 + request
     uint64_t *const tab = c.sv;
     // a word per committer for its completion probe
     uint64_t *const probe_at = c.sink + (size_t)(part * 16 + k) * 16;
@@ -2267,12 +2268,21 @@ __device__ __forceinline__ void committer_pipe(const Ctx &c, const ServerLayout 
         // request-wide delta) with its last writer's timestamp to the ring slot after the
         // item's installs so far (j-th install: slot j mod depth; the tail moves once the
         // ring is full).  Installs j and j + depth of an item share a slot; they come from
-        // requests at least depth apart, made by different committer waves.  With depth >=
-        // kPipeRing, install j is complete before install j + depth is stored (the wait
-        // above), so a store does; with a shallower ring they may land in either order, and
-        // the slot takes the larger (timestamp << 32 | value) word, so a late older version
-        // never replaces a newer one.
-        const bool ordered = D >= kPipeRing;
+        // requests at least depth apart, made by different committer waves of this
+        // workgroup.  With depth >= kPipeRing install j is complete before install j + depth
+        // is stored (the wait above); with a shallower ring the installs wait until those of
+        // request n - depth are issued: the memory operations this CU's waves issue in turn
+        // reach the L2 in that order (as the token relies on for the atomics), so install j
+        // lands first.  (Request n - depth + kPipeRing marks the same word only once the
+        // publisher is done with request n - depth.)
+        if (D < kPipeRing && n >= D) {
+            Spin sp;
+            while (__hip_atomic_load(&iissued[(n - D) % kPipeRing], __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_WORKGROUP) < n - D + 1)
+                if (!sp.tick(c, kErrSlotTimeout)) { bad = true; break; }
+        }
+        if (bad) break;
+        asm volatile("" ::: "memory");
 #pragma unroll
         for (int g = 0; g < G; ++g) {
             const uint32_t e = g * kWave + lane;
@@ -2281,18 +2291,20 @@ __device__ __forceinline__ void committer_pipe(const Ctx &c, const ServerLayout 
             const uint32_t pos = mod_d(j);
             const uint32_t tail = j < D ? 0u : mod_d(j + 1);
             const uint32_t ts = base + ((meta[g] >> kPipeRankShift) & 255u) + 1;
-            if (inst) {
-                uint64_t *const sl = c.slot + (size_t)item[g] * D + pos;
-                const uint64_t word = ((uint64_t)ts << 32) | (uint32_t)(val[g] + wgt[g]);
-                if (ordered) st_rlx(sl, word);
-                else __hip_atomic_fetch_max(sl, word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }
+            if (inst)
+                st_rlx(c.slot + (size_t)item[g] * D + pos,
+                       ((uint64_t)ts << 32) | (uint32_t)(val[g] + wgt[g]));
             if (act[g]) {
                 irec[qi * 2 * kPipeMaxE + e] = inst ? item[g] : kNone;
                 irec[(qi * 2 + 1) * kPipeMaxE + e] = pos | (tail << 16);
             }
         }
-        if (lane == 0) icnt[qi] = cnt;
+        // (the mark after the install stores' issue)
+        asm volatile("" ::: "memory");
+        if (lane == 0) {
+            icnt[qi] = cnt;
+            __hip_atomic_store(&iissued[qi], n + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
         // the reply: each entry's value now; this partition's header word (tag and the
         // request's first timestamp) once the values are complete (hand_over)
         uint32_t *rv = (uint32_t *)c.reply_v + ((size_t)cw * kPipeParts + part) * kPipeMaxE;

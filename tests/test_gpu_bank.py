@@ -311,7 +311,8 @@ def _everything(op):
                           (1 << 20, 8, "all", 2, 12), (1 << 16, 8, "all", 3, 5),
                           (1 << 20, 8, "transfers", 4, 16), (1 << 20, 8, "all", 8, 16),
                           (1 << 16, 8, "transfers", 6, 8), (4096, 1, "all", 8, 16),
-                          (1 << 20, 64, "transfers", 7, 16), (1 << 16, 2, "all", 5, 12)])
+                          (1 << 20, 64, "transfers", 7, 16), (1 << 16, 2, "all", 5, 12),
+                          (1 << 20, 1, "transfers", 8, 16)])
 def test_pipelined_lane_replays(n_items, depth, route, parts, wg):
     """The pipelined serialized lane: clients stage their requests' chains, committer waves
     apply each item's request-wide delta with one ordered atomic and install the versions
