@@ -714,8 +714,9 @@ def main():
             "txns_per_step": args.txns,
             "variant": args.variant,
             "commit_path": {
-                ("stamp_pipe", "all"): "pipelined serialized lane for every transaction: clients "
-                    "stage their requests' producer chains; committer waves apply each account's "
+                ("stamp_pipe", "all"): "pipelined serialized lane for every transaction: client "
+                    "waves hold four transactions a lane and stage each request's (up to 256 "
+                    "transactions) producer chains; committer waves apply each account's "
                     "request-wide delta with one ordered 64-bit atomic, whose return gives every "
                     "read its value in the serial order, and install the versions",
                 ("stamp_pipe", "transfers"): "pipelined serialized lane for every transfer; "
@@ -730,6 +731,7 @@ def main():
             "client_wg_waves": args.client_wg_waves,
             "stage_ahead": args.stage_ahead,
             "client_lanes": waves_h * 64,
+            "txns_per_lane": 4 if pipe_h else 1,
             "record_ring": 4096,
         },
         "aborts_by_class": {k: sum(r[k] for r in runs) for k in
