@@ -1491,6 +1491,13 @@ csmv_status csmv_bank_run(csmv_engine *e, csmv_stats *out)
     if ((e->cfg.variant & kSerialPipe) && (c.n_hot == 0 || c.n_serial_waves == 0))
         return fail(e, CSMV_E_ARG, "the pipelined serialized lane needs routed transfers and "
                                    "n_serial_waves > 0");
+    if ((e->cfg.variant & kSerialPipe) &&
+        (uint64_t)e->cfg.client_wg_waves * pipe_client_words(kBankR, kBankW, kPipeSlots) * 4 > e->lds_max)
+        return fail(e, CSMV_E_CAPACITY,
+                    "the pipelined lane's client waves stage their requests in " +
+                        std::to_string(pipe_client_words(kBankR, kBankW, kPipeSlots) * 4 / 1024) +
+                        " KiB of LDS each: client_wg_waves of " +
+                        std::to_string(e->cfg.client_wg_waves) + " do not fit (use 1)");
     csmv_status st;
     if ((e->cfg.variant & kSerialPipe) && c.n_norm == 0) {
         // no snapshot reader during the run: the head words follow from the install counts

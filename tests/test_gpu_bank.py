@@ -353,6 +353,22 @@ def test_pipelined_lane_ragged(n_txns, parts):
              client_wg_waves=1, route=_transfers, commit_parts=parts)
 
 
+def test_pipelined_lane_rejects_shared_client_workgroups():
+    """A lane-feeding client wave stages four transactions a lane in LDS: two of them do not
+    fit one workgroup, and the run is refused with a capacity error before any launch."""
+    bank = csmv.Bank(n_txns=1 << 12, seed=83, n_reads=4, n_reads_ro=4, pct_ro=10, zipf=True)
+    print("\n[case] pipelined lane with two client waves per workgroup", flush=True)
+    with csmv.Engine(1 << 16, depth=8, client_waves=8, wg_waves=8, variant="stamp_pipe",
+                     n_serial_waves=6, client_wg_waves=2) as e:
+        e.fill(INIT)
+        e.bank_generate(bank)
+        op, _ = e.bank_read_stream()
+        e.bank_route(_everything(op))
+        with pytest.raises(csmv.CsmvError) as ei:
+            e.bank_run()
+        assert ei.value.status == 3
+
+
 def test_pipelined_lane_needs_every_transfer_routed():
     """A transfer left on the normal lane reaches the pipelined server as a normal-lane
     update: the run ends with an argument error instead of committing it unvalidated."""
