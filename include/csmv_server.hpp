@@ -1627,7 +1627,7 @@ __host__ __device__ constexpr uint32_t pipe_client_hsz(uint32_t W, uint32_t S)
 __host__ __device__ constexpr uint32_t pipe_client_words(uint32_t R, uint32_t W, uint32_t S)
 {
     return (4 + 2 * S) * pipe_client_hsz(W, S) + 8 * S * kWave * R + 3 * S * kWave * W + S * kWave +
-           3 * kPipeParts * kPipeMaxE;
+           3 * kPipeParts * kPipeMaxE + kPipeParts;
 }
 
 template <int R, int W, int S>
@@ -1681,6 +1681,8 @@ __device__ __forceinline__ void tx_commit_pipe(const Ctx &c, const ServerLayout 
     uint32_t *sitem = samt + NV;                  // [kPipeParts][kPipeMaxE]
     uint32_t *swgt = sitem + kPipeParts * kPipeMaxE;
     uint32_t *smeta = swgt + kPipeParts * kPipeMaxE;
+    uint32_t *scnt = smeta + kPipeParts * kPipeMaxE;   // [kPipeParts] entries placed so far
+    static_assert(kPipeParts == 8, "entry placement splits on three partition bits");
     auto nd = [&](int s, int r) -> uint32_t { return ((uint32_t)s * kWave + lane) * RO + (uint32_t)r; };
     auto vl = [&](int s) -> uint32_t { return (uint32_t)s * kWave + lane; };
     // a virtual lane's deltas on its two write slots
@@ -1917,27 +1919,48 @@ __device__ __forceinline__ void tx_commit_pipe(const Ctx &c, const ServerLayout 
     //    staged at q * kPipeMaxE + e; the node's place (q << 16 | e, kept in chain buffer B,
     //    which the chains no longer need) for reading its value back.  A node of an item no
     //    one writes is its own root.
+    //    Root entries come first in each list, so the committer's atomics fill the fewest
+    //    instructions (the token is held while they issue).
+    //    A node's entry: the lanes before it with the same partition (three ballots on the
+    //    partition's bits) after the entries placed so far (scnt, which the group's last
+    //    lane moves on; one wave's LDS operations are performed in order).
+    uint32_t *const splace = nB;
+    if (lane < kPipeParts) scnt[lane] = 0;
+    wave_fence();
+#pragma unroll
+    for (int pass = 0; pass < 2; ++pass)
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+#pragma unroll
+            for (int r = 0; r < RO; ++r) {
+                const uint32_t node = nd(s, r);
+                const uint32_t q0 = spq[node];
+                const uint32_t f = sfs[node];
+                const bool root = (f != kNone ? canon[f] : node) == node;
+                const bool v = q0 != kNone && root == (pass == 0);
+                uint64_t m = ballot(v);
+#pragma unroll
+                for (uint32_t b = 0; b < 3; ++b) {
+                    const bool bit = ((q0 >> b) & 1u) != 0;
+                    const uint64_t mb = ballot(v && bit);
+                    m &= bit ? mb : ~mb;
+                }
+                if (v) {
+                    const uint32_t e = __hip_atomic_load(&scnt[q0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) +
+                                       (uint32_t)__builtin_popcountll(m & lt);
+                    if (((m >> lane) >> 1) == 0)
+                        __hip_atomic_store(&scnt[q0], e + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    e2[node] = e;
+                    splace[node] = (q0 << 16) | e;
+                } else if (pass == 0 && q0 == kNone) {
+                    e2[node] = 0;
+                    splace[node] = kNone;
+                }
+            }
+    wave_fence();
     uint32_t cnt[kPipeParts];
 #pragma unroll
-    for (uint32_t t = 0; t < kPipeParts; ++t) cnt[t] = 0;
-    uint32_t *const splace = nB;
-#pragma unroll
-    for (int s = 0; s < S; ++s)
-#pragma unroll
-        for (int r = 0; r < RO; ++r) {
-            const uint32_t q = spq[nd(s, r)];
-            uint32_t e = 0;
-#pragma unroll
-            for (uint32_t t = 0; t < kPipeParts; ++t) {
-                if (t >= P) break;
-                const uint64_t mq = ballot(q == t);
-                if (q == t) e = cnt[t] + (uint32_t)__builtin_popcountll(mq & lt);
-                cnt[t] += (uint32_t)__builtin_popcountll(mq);
-            }
-            e2[nd(s, r)] = e;
-            splace[nd(s, r)] = q != kNone ? (q << 16) | e : kNone;
-        }
-    wave_fence();
+    for (uint32_t t = 0; t < kPipeParts; ++t) cnt[t] = scnt[t];
 #pragma unroll
     for (int s = 0; s < S; ++s)
 #pragma unroll
